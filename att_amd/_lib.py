@@ -1,0 +1,73 @@
+"""ctypes binding of libattgnn.so (the C ABI declared in include/attgnn.h).
+
+The library is loaded once; a missing library raises immediately -- there is
+no fallback path.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libattgnn.so")
+
+# element types / modes / sources / layouts (attgnn.h)
+F32, F16, BF16, FP8E4M3, I32 = 0, 1, 2, 3, 4
+MODE_NATIVE, MODE_BF16X3, MODE_F32 = 0, 1, 2
+SRC_DEG_F16, SRC_DEG_F32, SRC_EDGE_F32, SRC_MAP = 0, 1, 2, 3
+LAYOUT_HMD, LAYOUT_MHD = 0, 1
+
+_c = ctypes
+_vp, _i64, _int = _c.c_void_p, _c.c_int64, _c.c_int
+
+_SIGNATURES = {
+    "attgnn_status_string": (_c.c_char_p, [_int]),
+    "attgnn_blocks_build": (_int, [_vp, _i64, _vp, _i64, _int, _vp, _int, _int, _int, _int, _int, _int, _int,
+                                   _c.POINTER(_vp)]),
+    "attgnn_blocks_sizes": (_int, [_vp, _c.POINTER(_i64), _c.POINTER(_i64), _c.POINTER(_i64),
+                                   _c.POINTER(_i64), _c.POINTER(_i64), _c.POINTER(_int)]),
+    "attgnn_blocks_export": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "attgnn_blocks_free": (None, [_vp]),
+    "attgnn_spmm_block": (_int, [_vp, _vp, _vp, _int, _vp, _vp, _i64, _vp, _int, _i64, _vp, _i64, _i64, _int,
+                                 _int, _int, _vp, _vp, _vp, _i64, _vp]),
+    "attgnn_spmm_workspace_elems": (_i64, [_i64, _int, _i64]),
+    "attgnn_spmm_edge_map": (_int, [_vp, _vp, _vp, _vp, _int, _i64, _vp, _vp, _i64, _vp, _int, _i64, _i64, _vp,
+                                    _i64, _i64, _int, _int, _int, _int, _vp, _i64, _vp]),
+    "attgnn_rowsum_edge_map": (_int, [_vp, _vp, _vp, _i64, _vp, _int, _i64, _i64, _vp, _int, _i64, _int, _int,
+                                      _vp]),
+    "attgnn_sddmm_block": (_int, [_vp, _vp, _vp, _vp, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _int, _i64, _i64,
+                                  _i64, _vp, _int, _i64, _int, _int, _int, _vp]),
+    "attgnn_node_scores": (_int, [_vp, _int, _i64, _i64, _vp, _int, _i64, _i64, _i64, _vp, _vp]),
+    "attgnn_edge_from_nodes": (_int, [_vp, _vp, _vp, _vp, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _int,
+                                      _i64, _i64, _vp, _int, _i64, _int, _int, _vp]),
+}
+
+# Every symbol include/attgnn.h declares (checked by the CPU test suite).
+EXPORTED = tuple(_SIGNATURES)
+
+_lib = None
+
+
+class AttGnnError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libattgnn.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise AttGnnError(
+                f"{LIB_PATH} is missing: build it with `python -m att_amd.build` "
+                "(there is no fallback implementation)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = lib().attgnn_status_string(rc).decode()
+        raise AttGnnError(f"{what} failed: {msg} (status {rc})")

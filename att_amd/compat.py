@@ -1,0 +1,247 @@
+"""Drop-in modules under the reference's extension names.
+
+``att_amd.compat.install()`` registers module objects named like the
+reference's pybind11 extensions, so reference-style callers run unchanged::
+
+    import att_amd.compat; att_amd.compat.install()
+    import Block_sr_bcrs, SpMM_sr_bcrs, ATT_Block_v3, ATT_SpMM_v3, ATT_SDDMM_v3
+
+Same positional arguments, same argument meaning, every op returns a list.
+Differences from the reference (all on the side of strictness):
+  * inputs are validated (device, dtype, contiguity, sizes) and errors raise
+    instead of corrupting memory;
+  * kernels run on torch's current stream, not the legacy default stream;
+  * FP16 ops accumulate in FP32 (the reference accumulates in FP16) and the
+    "tf32" ops use a split-BF16 (bf16x3) compensated product;
+  * split windows are reduced in a fixed order (bitwise reproducible).
+The window height of a structure is inferred from its array sizes; the block
+width (``wide``) is not recoverable from the arrays, so edge-map ops take it
+as a keyword (default 8, the value the reference's GAT datasets build with,
+end2end_ori/gat_final/mgat_csr/mdataset_fp16.py:62).
+"""
+import sys
+import types
+
+import torch
+
+from . import ops
+
+
+def _window_of(col, tiles):
+    c = col.numel()
+    if c == 0:
+        return 8
+    w = tiles.numel() // c
+    if w * c != tiles.numel() or w not in (8, 16):
+        raise ValueError(f"cannot infer the window height: {tiles.numel()} tile slots for {c} columns")
+    return w
+
+
+# ---------------------------------------------------------------- Block_sr_bcrs
+# gcn/Block_gcn/example.cpp:48-200, 203-348, 353-357
+
+def blockProcess_fp16_balance(row, column, degree, window, wide, partSize):
+    r = ops.blocks_build(row, column, "deg_f16", degree, torch.float16, window, wide, partSize, emit_empty=False)
+    return [r["rp"], r["col"], r["val"], r["window"], r["atomic"]]
+
+
+def blockProcess_tf32_balance(row, column, degree, window, wide, partSize):
+    r = ops.blocks_build(row, column, "deg_f32", degree, torch.float32, window, wide, partSize, emit_empty=True)
+    return [r["rp"], r["col"], r["val"], r["window"], r["atomic"]]
+
+
+# ---------------------------------------------------------------- SpMM_sr_bcrs
+# gcn/SpMM/src/benchmark.cpp:35-130, 490-496.  The reference kernels are fixed
+# to 8-row windows with wide = 8 (fp16) / 4 (tf32); the window is inferred here.
+
+def forward_fp16_gnn(row_offsets, col_indices, values, t_window_row, t_atomic, rhs, dimM1, dimN, mOri, wide=8):
+    if rhs.dtype != torch.float16 or values.dtype != torch.float16:
+        raise TypeError("forward_fp16_gnn expects fp16 values and rhs")
+    if rhs.shape[1] != dimN:
+        raise ValueError(f"dimN={dimN} but rhs has {rhs.shape[1]} columns")
+    window = _window_of(col_indices, values)
+    y = ops.spmm_block(row_offsets, col_indices, values, t_window_row, t_atomic, rhs.contiguous(), int(mOri),
+                       window, wide, mode="native")
+    return [y]
+
+
+def forward_tf32_gnn(row_offsets, col_indices, values, t_window_row, t_atomic, rhs, dimM1, dimN, mOri, wide=4):
+    if rhs.dtype != torch.float32 or values.dtype != torch.float32:
+        raise TypeError("forward_tf32_gnn expects fp32 values and rhs")
+    if rhs.shape[1] != dimN:
+        raise ValueError(f"dimN={dimN} but rhs has {rhs.shape[1]} columns")
+    window = _window_of(col_indices, values)
+    y = ops.spmm_block(row_offsets, col_indices, values, t_window_row, t_atomic, rhs.contiguous(), int(mOri),
+                       window, wide, mode="bf16x3")
+    return [y]
+
+
+# ---------------------------------------------------------------- ATT_Block(_v3)
+# Block/include/example.h:4-6 (binary-only); map layout output_tile.h:20,27-33
+
+def blockProcess_sddmm_balance_gnn(row, column, window, wide, partSize):
+    r = ops.blocks_build(row, column, "map", None, torch.int32, window, wide, partSize, emit_empty=False)
+    return [r["rp"], r["col"], r["val"], r["window"], r["atomic"]]
+
+
+def blockProcess_sddmm_balance_gnn_trans(row, column, window, wide, partSize):
+    r = ops.blocks_build(row, column, "map", None, torch.int32, window, wide, partSize, emit_empty=False,
+                         want_trans=True)
+    return [r["rp"], r["col"], r["val"], r["map_t"], r["window"], r["atomic"]]
+
+
+# ---------------------------------------------------------------- ATT_SpMM_v*
+# callers end2end_head/gat_final/mgat_csr/mgat_conv.py:42-59,79-86,99-106,126-132;
+# tests benchmark-v3/spmm-benchmark/test/spmm_{fp16,tf32}/*.py
+
+def _spmm_map(version, prec, layout):
+    mode = "native" if prec == "fp16" else "bf16x3"
+    want = torch.float16 if prec == "fp16" else torch.float32
+
+    def op(row_pointers, column_index, degrees, values, window, automic, rhs, wide=8):
+        if rhs.dtype != want:
+            raise TypeError(f"expected {want} rhs, got {rhs.dtype}")
+        w = _window_of(column_index, degrees)
+        x = rhs if rhs.dim() == 3 else rhs.unsqueeze(0)
+        vals = values.reshape(x.shape[0], -1)
+        y = ops.spmm_edge_map(row_pointers, column_index, degrees, vals.contiguous(), window, automic,
+                              x.contiguous(), layout, w, wide, mode=mode)
+        if version == 1 and rhs.dim() == 2:
+            y = y[0] if layout == "hmd" else y[:, 0]
+        return [y]
+
+    op.__name__ = f"spmm_{prec}_{'h_m' if layout == 'hmd' else 'm_h'}"
+    return op
+
+
+def _rowsum_map(version, prec, layout):
+    def op(row_pointers, degrees, values, window, automic, M, wide=8, win_rows=None):
+        if version == 4 and layout == "mhd" and values.dim() == 2:
+            vals = values.t().contiguous()       # v4 keeps per-edge values as [nnz, H]
+        else:
+            vals = values.reshape(-1, values.shape[-1]) if values.dim() >= 2 else values.reshape(1, -1)
+        # the window height needs the total column count rp[-1] (a device read)
+        # unless the caller names it
+        w = win_rows or (_window_from_rp(row_pointers, degrees) if window.numel() else 8)
+        s = ops.rowsum_edge_map(row_pointers, degrees, window, vals.contiguous(), int(M), layout, w, wide)
+        if version == 1:
+            return [s.reshape(int(M), 1)]
+        return [s.unsqueeze(-1)]
+
+    op.__name__ = f"spmm_{prec}_{'h_m' if layout == 'hmd' else 'm_h'}_rowsum"
+    return op
+
+
+def _window_from_rp(rp, tiles):
+    c = int(rp[-1].item()) if rp.numel() else 0
+    if c == 0:
+        return 8
+    w = tiles.numel() // c
+    if w * c != tiles.numel() or w not in (8, 16):
+        raise ValueError("cannot infer the window height from the structure")
+    return w
+
+
+# ---------------------------------------------------------------- ATT_SDDMM_v*
+# device headers benchmark-v3/sddmm-benchmark/include/{compute_utils,output_tile}.h;
+# tests benchmark-v3/sddmm-benchmark/test/sddmm_{fp16,tf32}/*.py
+
+def _prec_dtype(prec):
+    return torch.float16 if prec == "fp16" else torch.float32
+
+
+def _sddmm(version, prec):
+    def op(row_pointers, column_index, degrees, window, rhs, rhs1, max, nnz, wide=8):
+        w = _window_of(column_index, degrees)
+        layout = "mhd" if version == 4 else "hmd"
+        out = ops.sddmm_block(row_pointers, column_index, degrees, window, rhs.contiguous(), rhs1.contiguous(),
+                              int(nnz), w, wide, layout=layout, out_dtype=_prec_dtype(prec), mode="bf16x3")
+        return [out]
+
+    op.__name__ = f"sddmm_{prec}_h_nnz"
+    return op
+
+
+def _att(version, prec, v2):
+    def op(row_pointers, column_index, degrees, window, X, a0_or_X1, a1_or_a, max, nnz, wide=8):
+        w = _window_of(column_index, degrees)
+        layout = "mhd" if version == 4 else "hmd"
+        if v2:
+            X0, X1, a = X, a0_or_X1, a1_or_a
+            s0 = ops.node_scores(X0.contiguous(), a.contiguous(), layout)
+            s1 = ops.node_scores(X1.contiguous(), a.contiguous(), layout)
+        else:
+            s0 = ops.node_scores(X.contiguous(), a0_or_X1.contiguous(), layout)
+            s1 = ops.node_scores(X.contiguous(), a1_or_a.contiguous(), layout)
+        heads, n = s0.shape
+        out = ops.edge_from_nodes(row_pointers, column_index, degrees, window, int(nnz), heads, n, w, wide,
+                                  _prec_dtype(prec), s0=s0, s1=s1)
+        return [out]
+
+    op.__name__ = f"{'attv2' if v2 else 'att'}_{prec}_h_nnz"
+    return op
+
+
+def _ones(version, prec):
+    def op(row_pointers, column_index, degrees, window, rhs, max, nnz, wide=8):
+        w = _window_of(column_index, degrees)
+        g = rhs.contiguous()
+        if g.dim() == 3:          # [N, H, 1]
+            n, heads = g.shape[0], g.shape[1]
+            strides = (1, heads)
+        else:                     # [N, 1] / [N]
+            n, heads = g.shape[0], 1
+            strides = (0, 1)
+        out = ops.edge_from_nodes(row_pointers, column_index, degrees, window, int(nnz), heads, n, w, wide,
+                                  _prec_dtype(prec), g0=g, g0_strides=strides)
+        return [out]
+
+    op.__name__ = f"sddmm_{prec}_ones_h_nnz"
+    return op
+
+
+def _make_modules():
+    mods = {}
+
+    m = types.ModuleType("Block_sr_bcrs", "window-block preprocessing (gcn/Block_gcn/example.cpp)")
+    m.blockProcess_fp16_balance = blockProcess_fp16_balance
+    m.blockProcess_tf32_balance = blockProcess_tf32_balance
+    mods["Block_sr_bcrs"] = m
+
+    m = types.ModuleType("SpMM_sr_bcrs", "window-block SpMM (gcn/SpMM/src/benchmark.cpp)")
+    m.forward_fp16_gnn = forward_fp16_gnn
+    m.forward_tf32_gnn = forward_tf32_gnn
+    mods["SpMM_sr_bcrs"] = m
+
+    for name in ("ATT_Block", "ATT_Block_v3"):
+        m = types.ModuleType(name, "window-block preprocessing with edge-id maps")
+        m.blockProcess_sddmm_balance_gnn = blockProcess_sddmm_balance_gnn
+        m.blockProcess_sddmm_balance_gnn_trans = blockProcess_sddmm_balance_gnn_trans
+        mods[name] = m
+
+    for ver in (1, 3, 4):
+        m = types.ModuleType(f"ATT_SpMM_v{ver}", "edge-map window-block SpMM")
+        for prec in ("fp16", "tf32"):
+            for layout in ("hmd", "mhd"):
+                f = _spmm_map(ver, prec, layout)
+                setattr(m, f.__name__, f)
+                g = _rowsum_map(ver, prec, layout)
+                setattr(m, g.__name__, g)
+        mods[m.__name__] = m
+
+        m = types.ModuleType(f"ATT_SDDMM_v{ver}", "window-block SDDMM and attention terms")
+        for prec in ("fp16", "tf32"):
+            for f in (_sddmm(ver, prec), _att(ver, prec, False), _att(ver, prec, True), _ones(ver, prec)):
+                setattr(m, f.__name__, f)
+        mods[m.__name__] = m
+    return mods
+
+
+MODULES = _make_modules()
+
+
+def install():
+    """Register the reference-named modules in sys.modules (idempotent)."""
+    for name, mod in MODULES.items():
+        sys.modules.setdefault(name, mod)
+    return MODULES

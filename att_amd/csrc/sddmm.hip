@@ -1,0 +1,429 @@
+// Window-block SDDMM and the per-edge attention primitives for gfx950.
+//
+// Reference counterparts (the reference ships only device headers and Python
+// callers for these ops; the kernels themselves are absent):
+//   ATT_SDDMM_v{1,3}.sddmm_{fp16,tf32}_h_nnz
+//       benchmark-v3/sddmm-benchmark/include/compute_utils.h:142-280,427-530,
+//       output_tile.h:5-84 (scatter through the edge-id map)
+//   ATT_SDDMM_v{1,3}.att_{fp16,tf32}_h_nnz / attv2_*  (compute_utils.h:1144-1218,
+//       output_tile.h:89-202; callers end2end_*/gat*_final/mgat_csr/mgat_conv.py)
+//   ATT_SDDMM_v{1,3}.sddmm_{fp16,tf32}_ones_h_nnz     (mgat_conv.py:142-149)
+//   ATT_SpMM_v{1,3}.spmm_*_{h_m,m_h}_rowsum           (mgat_conv.py:126-132)
+//
+// SDDMM: one wave per (part, head).  For each k-block of 32 condensed columns
+// the window's rows (A operand) and the 32 gathered column rows (two B tiles)
+// are contracted over the feature dimension with MFMA 16x16x32, K = features,
+// so every operand is one contiguous 16-byte row load; the 16x32 result is
+// scattered to CSR edge order through the edge-id map.
+//
+// The attention projections a0.X[i] + a1.X[j] are separable, so they are
+// computed as two per-node GEMVs (N*H dot products instead of nnz*H) followed
+// by a per-edge scatter-add over the window tiles.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "attgnn.h"
+#include "mfma.hpp"
+
+using namespace attgnn;
+
+namespace {
+
+__device__ __forceinline__ float load_elt(const uint8_t *p, int type)
+{
+    switch (type) {
+    case ATTGNN_F32: return *reinterpret_cast<const float *>(p);
+    case ATTGNN_F16: return float(*reinterpret_cast<const _Float16 *>(p));
+    case ATTGNN_BF16: return __uint_as_float(uint32_t(*reinterpret_cast<const uint16_t *>(p)) << 16);
+    default: return 0.f;
+    }
+}
+
+__device__ __forceinline__ void store_elt(uint8_t *p, int type, float v)
+{
+    switch (type) {
+    case ATTGNN_F32: *reinterpret_cast<float *>(p) = v; break;
+    case ATTGNN_F16: *reinterpret_cast<_Float16 *>(p) = _Float16(v); break;
+    case ATTGNN_BF16: *reinterpret_cast<uint16_t *>(p) = uint16_t(bf16_bits_rne(v)); break;
+    default: break;
+    }
+}
+
+__host__ __device__ inline int type_bytes(int t) { return t == ATTGNN_F32 ? 4 : (t == ATTGNN_F16 || t == ATTGNN_BF16) ? 2 : 0; }
+
+struct SddmmArgs {
+    const int32_t *rp, *col, *map, *win;
+    const uint8_t *a, *b;     // element (h, i, d) at a[(h*hs + i*rs + d)*eb]
+    uint8_t *out;             // out[(h*nnz + e)*ob]
+    int64_t parts, nnz, n_rows, dim;
+    int64_t a_hs, a_rs, b_hs, b_rs;
+    int heads, window, wide_log2, in_type, out_type;
+};
+
+// VEC: rows are 16-byte aligned runs of 8 (16-bit) or 4 (fp32) elements and
+// dim is a multiple of 8, so each lane loads its 8 features with vector loads.
+// KIND: 0 = f16, 1 = bf16, 2 = fp32 via bf16x3, 3 = fp32 MFMA.
+template <int KIND, bool VEC>
+__global__ __launch_bounds__(256) void sddmm_block_kernel(SddmmArgs a)
+{
+    constexpr int EB = KIND >= 2 ? 4 : 2;
+    const int lane = threadIdx.x & 63;
+    const int64_t task = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    if (task >= a.parts * a.heads) return;
+    const int64_t part = task / a.heads;
+    const int head = int(task - part * a.heads);
+    const int c_begin = a.rp[2 * part];
+    const int plen = a.rp[2 * part + 2] - c_begin;
+    const int nreal = a.rp[2 * part + 1] - c_begin;
+    const int g = lane >> 4, n = lane & 15;
+    const int W = a.window, wide = 1 << a.wide_log2;
+    const int64_t row = int64_t(a.win[part]) * W + n;  // A-operand row of this lane
+    const bool row_ok = n < W && row < a.n_rows;
+    const uint8_t *arow = a.a + (int64_t(head) * a.a_hs + row * a.a_rs) * EB;
+
+    for (int kb = 0; kb < nreal; kb += 32) {
+        f32x4 acc[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
+        int cid[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int k = kb + 16 * t + n;
+            cid[t] = k < nreal ? a.col[c_begin + k] : -1;
+        }
+        for (int64_t d0 = 0; d0 < a.dim; d0 += 32) {
+            const int64_t d = d0 + 8 * g;
+            // 8 features d..d+7 of the A row and of the two B rows
+            float av[8], bv[2][8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) { av[i] = 0.f; bv[0][i] = 0.f; bv[1][i] = 0.f; }
+            uint32_t araw[4] = {0, 0, 0, 0}, braw[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+            if constexpr (VEC && EB == 2) {
+                if (d < a.dim) {
+                    if (row_ok) {
+                        const u32x4 v = *reinterpret_cast<const u32x4 *>(arow + d * 2);
+                        araw[0] = v[0]; araw[1] = v[1]; araw[2] = v[2]; araw[3] = v[3];
+                    }
+#pragma unroll
+                    for (int t = 0; t < 2; ++t)
+                        if (cid[t] >= 0) {
+                            const u32x4 v = *reinterpret_cast<const u32x4 *>(
+                                a.b + (int64_t(head) * a.b_hs + int64_t(cid[t]) * a.b_rs + d) * 2);
+                            braw[t][0] = v[0]; braw[t][1] = v[1]; braw[t][2] = v[2]; braw[t][3] = v[3];
+                        }
+                }
+            } else if constexpr (VEC && EB == 4) {
+                if (d < a.dim) {
+                    if (row_ok)
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) av[i] = reinterpret_cast<const float *>(arow)[d + i];
+#pragma unroll
+                    for (int t = 0; t < 2; ++t)
+                        if (cid[t] >= 0) {
+                            const float *br = reinterpret_cast<const float *>(
+                                a.b + (int64_t(head) * a.b_hs + int64_t(cid[t]) * a.b_rs) * 4);
+#pragma unroll
+                            for (int i = 0; i < 8; ++i) bv[t][i] = br[d + i];
+                        }
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    if (d + i < a.dim) {
+                        if (row_ok) av[i] = load_elt(arow + (d + i) * EB, a.in_type);
+#pragma unroll
+                        for (int t = 0; t < 2; ++t)
+                            if (cid[t] >= 0)
+                                bv[t][i] = load_elt(a.b + (int64_t(head) * a.b_hs + int64_t(cid[t]) * a.b_rs + d + i) * EB,
+                                                    a.in_type);
+                    }
+                }
+                if constexpr (EB == 2) {  // repack to 16-bit operands
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        auto enc = [](float x) -> uint32_t {
+                            if constexpr (KIND == 0) return __builtin_bit_cast(uint16_t, _Float16(x));
+                            else return bf16_bits_rne(x);
+                        };
+                        araw[q] = enc(av[2 * q]) | enc(av[2 * q + 1]) << 16;
+#pragma unroll
+                        for (int t = 0; t < 2; ++t) braw[t][q] = enc(bv[t][2 * q]) | enc(bv[t][2 * q + 1]) << 16;
+                    }
+                }
+            }
+            if constexpr (KIND == 0 || KIND == 1) {
+                const u32x4 A = {araw[0], araw[1], araw[2], araw[3]};
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    const u32x4 B = {braw[t][0], braw[t][1], braw[t][2], braw[t][3]};
+                    acc[t] = KIND == 0 ? mfma_f16(A, B, acc[t]) : mfma_bf16(A, B, acc[t]);
+                }
+            } else if constexpr (KIND == 3) {
+#pragma unroll
+                for (int t = 0; t < 2; ++t)
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) acc[t] = mfma_f32(av[i], bv[t][i], acc[t]);
+            } else {
+                uint32_t ah[8], al[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) bf16_split(av[i], ah[i], al[i]);
+                const u32x4 AH = {ah[0] | ah[1] << 16, ah[2] | ah[3] << 16, ah[4] | ah[5] << 16, ah[6] | ah[7] << 16};
+                const u32x4 AL = {al[0] | al[1] << 16, al[2] | al[3] << 16, al[4] | al[5] << 16, al[6] | al[7] << 16};
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    uint32_t bh[8], bl[8];
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) bf16_split(bv[t][i], bh[i], bl[i]);
+                    const u32x4 BH = {bh[0] | bh[1] << 16, bh[2] | bh[3] << 16, bh[4] | bh[5] << 16, bh[6] | bh[7] << 16};
+                    const u32x4 BL = {bl[0] | bl[1] << 16, bl[2] | bl[3] << 16, bl[4] | bl[5] << 16, bl[6] | bl[7] << 16};
+                    acc[t] = mfma_bf16(AL, BH, acc[t]);
+                    acc[t] = mfma_bf16(AH, BL, acc[t]);
+                    acc[t] = mfma_bf16(AH, BH, acc[t]);
+                }
+            }
+        }
+        // scatter C[4g+q][16t+n] to its edge
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int k = kb + 16 * t + n;
+            if (k >= nreal) continue;
+            const int64_t cc = int64_t(c_begin) + k;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = 4 * g + q;
+                if (r >= W) break;
+                const int e = a.map[(cc >> a.wide_log2) * W * wide + int64_t(r) * wide + (cc & (wide - 1))];
+                if (e >= 0) store_elt(a.out + (int64_t(head) * a.nnz + e) * type_bytes(a.out_type), a.out_type, acc[t][q]);
+            }
+        }
+    }
+}
+
+// s[h*n_rows + i] = sum_d X[h, i, d] * vec[h*dim + d]   (fp32 accumulate)
+// 16 lanes per (h, i).
+__global__ __launch_bounds__(256) void node_score_kernel(const uint8_t *x, int x_type, int64_t x_hs,
+                                                         int64_t x_rs, const uint8_t *vec, int v_type,
+                                                         int64_t heads, int64_t n_rows, int64_t dim,
+                                                         float *s)
+{
+    const int64_t idx = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 4;
+    const int l = threadIdx.x & 15;
+    const bool ok = idx < heads * n_rows;
+    float acc = 0.f;
+    int64_t h = 0, i = 0;
+    if (ok) {
+        h = idx / n_rows;
+        i = idx - h * n_rows;
+        const int xe = type_bytes(x_type), ve = type_bytes(v_type);
+        const uint8_t *xr = x + (h * x_hs + i * x_rs) * xe;
+        const uint8_t *vr = vec + h * dim * ve;
+        for (int64_t d = l; d < dim; d += 16) acc += load_elt(xr + d * xe, x_type) * load_elt(vr + d * ve, v_type);
+    }
+#pragma unroll
+    for (int off = 8; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 16);
+    if (ok && l == 0) s[h * n_rows + i] = acc;
+}
+
+struct EdgeArgs {
+    const int32_t *rp, *col, *map, *win;
+    const float *s0, *s1;       // node terms, element (h, i) at s[h*hs + i*rs]; s1 may be null
+    const uint8_t *g0;          // alternative typed source for s0 (ones op), or null
+    int g0_type;
+    int64_t s0_hs, s0_rs, s1_hs, s1_rs;
+    uint8_t *out;
+    int64_t parts, nnz, n_rows;
+    int heads, window, wide_log2, out_type;
+};
+
+// out[h*nnz + e] = s0[h, row(e)] + s1[h, col(e)] over every tile slot of a part.
+__global__ __launch_bounds__(256) void edge_from_nodes_kernel(EdgeArgs a)
+{
+    const int64_t part = blockIdx.x;
+    const int W = a.window, wide = 1 << a.wide_log2;
+    const int64_t c_begin = a.rp[2 * part];
+    const int64_t plen = a.rp[2 * part + 2] - c_begin;
+    const int64_t w = a.win[part];
+    const int ob = type_bytes(a.out_type);
+    for (int64_t s = threadIdx.x; s < plen * W; s += blockDim.x) {
+        const int64_t idx = c_begin * W + s;
+        const int e = a.map[idx];
+        if (e < 0) continue;
+        const int64_t within = idx & (int64_t(W) * wide - 1);   // W*wide is a power of two
+        const int r = int(within >> a.wide_log2);
+        const int64_t cc = (idx / (int64_t(W) * wide)) * wide + (within & (wide - 1));
+        const int64_t row = w * W + r;
+        const int64_t c = a.col[cc];
+        for (int h = 0; h < a.heads; ++h) {
+            float v = 0.f;
+            if (a.g0) v = row < a.n_rows ? load_elt(a.g0 + (h * a.s0_hs + row * a.s0_rs) * type_bytes(a.g0_type), a.g0_type) : 0.f;
+            else if (a.s0) v = row < a.n_rows ? a.s0[h * a.s0_hs + row * a.s0_rs] : 0.f;
+            if (a.s1) v += a.s1[h * a.s1_hs + c * a.s1_rs];
+            store_elt(a.out + (int64_t(h) * a.nnz + e) * ob, a.out_type, v);
+        }
+    }
+}
+
+// rowsum[h, r] = sum over the row's tile slots of val[h*nnz + map[slot]].
+// One workgroup per part; the first part of each window sums all of the
+// window's parts in a fixed order (deterministic).
+__global__ __launch_bounds__(256) void rowsum_map_kernel(const int32_t *rp, const int32_t *map,
+                                                         const int32_t *win, int64_t parts,
+                                                         const uint8_t *val, int val_type, int64_t nnz,
+                                                         int heads, int window, int wide_log2,
+                                                         float *out, int64_t out_hs, int64_t out_rs,
+                                                         int64_t m_rows)
+{
+    __shared__ float red[256];
+    const int64_t p = blockIdx.x;
+    const int w = win[p];
+    if (p > 0 && win[p - 1] == w) return;
+    int64_t last = p;
+    while (last + 1 < parts && win[last + 1] == w) ++last;
+    const int W = window, wide = 1 << wide_log2;
+    const int64_t c_lo = rp[2 * p], c_hi = rp[2 * last + 2];
+    const int lanes = 256 / W;                 // threads per row
+    const int r = threadIdx.x / lanes, l = threadIdx.x - r * lanes;
+    const int vb = type_bytes(val_type);
+    for (int h = 0; h < heads; ++h) {
+        float acc = 0.f;
+        for (int64_t cc = c_lo + l; cc < c_hi; cc += lanes) {
+            const int e = map[(cc >> wide_log2) * W * wide + int64_t(r) * wide + (cc & (wide - 1))];
+            if (e >= 0) acc += load_elt(val + (int64_t(h) * nnz + e) * vb, val_type);
+        }
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int off = lanes / 2; off >= 1; off >>= 1) {
+            if (l < off) red[threadIdx.x] += red[threadIdx.x + off];
+            __syncthreads();
+        }
+        const int64_t row = int64_t(w) * W + r;
+        if (l == 0 && row < m_rows) out[h * out_hs + row * out_rs] = red[threadIdx.x];
+        __syncthreads();
+    }
+}
+
+__global__ void fill_f32_kernel(float *p, int64_t n, float v)
+{
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+        p[i] = v;
+}
+
+int wl_of(int wide)
+{
+    switch (wide) { case 4: return 2; case 8: return 3; case 16: return 4; case 32: return 5; default: return -1; }
+}
+
+bool is_float_type(int t) { return t == ATTGNN_F32 || t == ATTGNN_F16 || t == ATTGNN_BF16; }
+
+}  // namespace
+
+extern "C" int attgnn_sddmm_block(const int32_t *rp, const int32_t *col, const int32_t *map,
+                                  const int32_t *window_of_part, int64_t parts, const void *A,
+                                  int64_t a_head_stride, int64_t a_row_stride, const void *B,
+                                  int64_t b_head_stride, int64_t b_row_stride, int in_type,
+                                  int64_t heads, int64_t n_rows, int64_t dim, void *out, int out_type,
+                                  int64_t nnz, int window, int wide, int mode, attgnn_stream_t stream)
+{
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (parts < 0 || heads <= 0 || n_rows < 0 || dim <= 0 || nnz < 0) return ATTGNN_E_ARG;
+    if (!(window == 8 || window == 16)) return ATTGNN_E_ARG;
+    const int wl = wl_of(wide);
+    if (wl < 0 || !is_float_type(in_type) || !is_float_type(out_type)) return ATTGNN_E_ARG;
+    if (parts == 0) return ATTGNN_OK;
+    if (!rp || !col || !map || !window_of_part || !A || !B || !out) return ATTGNN_E_ARG;
+    int kind;
+    if (in_type == ATTGNN_F16) kind = 0;
+    else if (in_type == ATTGNN_BF16) kind = 1;
+    else kind = mode == ATTGNN_MODE_F32 ? 3 : 2;
+    const int eb = type_bytes(in_type);
+    const bool vec = dim % 8 == 0 && a_row_stride % 8 == 0 && b_row_stride % 8 == 0 && a_head_stride % 8 == 0 &&
+                     b_head_stride % 8 == 0 && reinterpret_cast<uintptr_t>(A) % 16 == 0 &&
+                     reinterpret_cast<uintptr_t>(B) % 16 == 0 && (eb == 2 || eb == 4);
+    SddmmArgs a{rp, col, map, window_of_part, static_cast<const uint8_t *>(A), static_cast<const uint8_t *>(B),
+                static_cast<uint8_t *>(out), parts, nnz, n_rows, dim, a_head_stride, a_row_stride,
+                b_head_stride, b_row_stride, int(heads), window, wl, in_type, out_type};
+    const dim3 grid(unsigned((parts * heads + 3) / 4)), block(256);
+#define ATTGNN_SDDMM_CASE(K)                                                                         \
+    case K:                                                                                          \
+        if (vec) hipLaunchKernelGGL((sddmm_block_kernel<K, true>), grid, block, 0, s, a);            \
+        else hipLaunchKernelGGL((sddmm_block_kernel<K, false>), grid, block, 0, s, a);               \
+        break;
+    switch (kind) {
+        ATTGNN_SDDMM_CASE(0)
+        ATTGNN_SDDMM_CASE(1)
+        ATTGNN_SDDMM_CASE(2)
+        ATTGNN_SDDMM_CASE(3)
+    }
+#undef ATTGNN_SDDMM_CASE
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? ATTGNN_OK : int(e);
+}
+
+extern "C" int attgnn_node_scores(const void *X, int x_type, int64_t x_head_stride, int64_t x_row_stride,
+                                  const void *vec, int vec_type, int64_t heads, int64_t n_rows, int64_t dim,
+                                  float *scores, attgnn_stream_t stream)
+{
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (heads <= 0 || n_rows < 0 || dim <= 0 || !is_float_type(x_type) || !is_float_type(vec_type))
+        return ATTGNN_E_ARG;
+    if (n_rows == 0) return ATTGNN_OK;
+    if (!X || !vec || !scores) return ATTGNN_E_ARG;
+    const int64_t threads = heads * n_rows * 16;
+    hipLaunchKernelGGL(node_score_kernel, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, s,
+                       static_cast<const uint8_t *>(X), x_type, x_head_stride, x_row_stride,
+                       static_cast<const uint8_t *>(vec), vec_type, heads, n_rows, dim, scores);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? ATTGNN_OK : int(e);
+}
+
+extern "C" int attgnn_edge_from_nodes(const int32_t *rp, const int32_t *col, const int32_t *map,
+                                      const int32_t *window_of_part, int64_t parts, const float *s0,
+                                      int64_t s0_head_stride, int64_t s0_row_stride, const float *s1,
+                                      int64_t s1_head_stride, int64_t s1_row_stride, const void *g0,
+                                      int g0_type, int64_t heads, int64_t n_rows, void *out, int out_type,
+                                      int64_t nnz, int window, int wide, attgnn_stream_t stream)
+{
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (parts < 0 || heads <= 0 || nnz < 0 || n_rows < 0) return ATTGNN_E_ARG;
+    if (!(window == 8 || window == 16)) return ATTGNN_E_ARG;
+    const int wl = wl_of(wide);
+    if (wl < 0 || !is_float_type(out_type)) return ATTGNN_E_ARG;
+    if (g0 && !is_float_type(g0_type)) return ATTGNN_E_ARG;
+    if (parts == 0) return ATTGNN_OK;
+    if (!rp || !col || !map || !window_of_part || !out) return ATTGNN_E_ARG;
+    EdgeArgs a{rp, col, map, window_of_part, s0, s1, static_cast<const uint8_t *>(g0), g0_type,
+               s0_head_stride, s0_row_stride, s1_head_stride, s1_row_stride, static_cast<uint8_t *>(out),
+               parts, nnz, n_rows, int(heads), window, wl, out_type};
+    hipLaunchKernelGGL(edge_from_nodes_kernel, dim3(unsigned(parts)), dim3(256), 0, s, a);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? ATTGNN_OK : int(e);
+}
+
+extern "C" int attgnn_rowsum_edge_map(const int32_t *rp, const int32_t *map, const int32_t *window_of_part,
+                                      int64_t parts, const void *edge_val, int val_type, int64_t nnz,
+                                      int64_t heads, float *out, int out_layout, int64_t m_rows, int window,
+                                      int wide, attgnn_stream_t stream)
+{
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (parts < 0 || heads <= 0 || nnz < 0 || m_rows < 0) return ATTGNN_E_ARG;
+    if (!(window == 8 || window == 16)) return ATTGNN_E_ARG;
+    if (!(out_layout == ATTGNN_LAYOUT_HMD || out_layout == ATTGNN_LAYOUT_MHD)) return ATTGNN_E_ARG;
+    const int wl = wl_of(wide);
+    if (wl < 0 || !is_float_type(val_type)) return ATTGNN_E_ARG;
+    if (!out) return ATTGNN_E_ARG;
+    const int64_t n = heads * m_rows;
+    if (n > 0) {
+        hipLaunchKernelGGL(fill_f32_kernel, dim3(unsigned(std::min<int64_t>((n + 255) / 256, 65536))), dim3(256), 0, s,
+                           out, n, 0.f);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return int(e);
+    }
+    if (parts == 0) return ATTGNN_OK;
+    if (!rp || !map || !window_of_part || !edge_val) return ATTGNN_E_ARG;
+    const int64_t hs = out_layout == ATTGNN_LAYOUT_HMD ? m_rows : 1;
+    const int64_t rs = out_layout == ATTGNN_LAYOUT_HMD ? 1 : heads;
+    hipLaunchKernelGGL(rowsum_map_kernel, dim3(unsigned(parts)), dim3(256), 0, s, rp, map, window_of_part, parts,
+                       static_cast<const uint8_t *>(edge_val), val_type, nnz, int(heads), window, wl, out, hs, rs,
+                       m_rows);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? ATTGNN_OK : int(e);
+}

@@ -1,0 +1,431 @@
+// Window-block SpMM for gfx950: Y = A . X with A stored as condensed window
+// tiles (see attgnn.h, attgnn_spmm_block).
+//
+// Reference counterpart: spmm_forward_cuda_kernel_{fp16,tf32}_balance and their
+// gnn launchers (gcn/SpMM/src/spmmKernel.cu:158-265, 959-1075, 1693-1765),
+// which run one 8-row part per thread block, feed mma.sync from per-lane
+// 4-byte X gathers and combine split windows with float atomicAdd.
+//
+// This design:
+//  * one 64-lane wave per (part, feature slice); a slice is 16*T features,
+//    lane n of each 16-lane group owning features [n*T, n*T+T) of the slice;
+//  * per k-block of 32 condensed columns each lane gathers the T-element
+//    vectors of 8 X rows (one 16/8/4-byte load per row, coalesced across the
+//    16 lanes that share a row) and transposes them in registers into T MFMA
+//    B operands, one per N tile -- no LDS round trip;
+//  * the lane's A operand (8 tile values of window row n) is one vector load
+//    from the dense tile, shared by the T MFMAs;
+//  * FP32 accumulation in every mode; split windows write partial tiles that a
+//    second kernel sums in part order, so results are bitwise reproducible.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "attgnn.h"
+#include "mfma.hpp"
+
+using namespace attgnn;
+
+namespace {
+
+enum Kind { K_F16, K_BF16, K_FP8, K_BF16X3, K_F32 };
+
+template <int K> struct KindTraits;
+template <> struct KindTraits<K_F16> { static constexpr int EB = 2; };
+template <> struct KindTraits<K_BF16> { static constexpr int EB = 2; };
+template <> struct KindTraits<K_FP8> { static constexpr int EB = 1; };
+template <> struct KindTraits<K_BF16X3> { static constexpr int EB = 4; };
+template <> struct KindTraits<K_F32> { static constexpr int EB = 4; };
+
+template <int BYTES>
+__device__ __forceinline__ void load_bytes(const uint8_t *p, uint32_t *d)
+{
+    if constexpr (BYTES == 32) {
+        const u32x4 a = reinterpret_cast<const u32x4 *>(p)[0];
+        const u32x4 b = reinterpret_cast<const u32x4 *>(p)[1];
+        d[0] = a[0]; d[1] = a[1]; d[2] = a[2]; d[3] = a[3];
+        d[4] = b[0]; d[5] = b[1]; d[6] = b[2]; d[7] = b[3];
+    } else if constexpr (BYTES == 16) {
+        const u32x4 a = *reinterpret_cast<const u32x4 *>(p);
+        d[0] = a[0]; d[1] = a[1]; d[2] = a[2]; d[3] = a[3];
+    } else if constexpr (BYTES == 8) {
+        const u32x2 a = *reinterpret_cast<const u32x2 *>(p);
+        d[0] = a[0]; d[1] = a[1];
+    } else if constexpr (BYTES == 4) {
+        d[0] = *reinterpret_cast<const uint32_t *>(p);
+    } else if constexpr (BYTES == 2) {
+        d[0] = *reinterpret_cast<const uint16_t *>(p);
+    } else {
+        d[0] = *p;
+    }
+}
+
+struct SpmmArgs {
+    const int32_t *rp, *col, *win, *atomic;
+    const uint8_t *val;     // tiles, or per-edge values [heads][val_head_stride] (mapped)
+    const int32_t *map;     // mapped variant: CSR position per tile slot, -1 = none
+    const uint8_t *x;       // X rows: element (h, c, f) at x[h*x_head_stride + c*dim_n + f]
+    float *y, *partial;     // Y element (h, r, f) at y[h*y_head_stride + r*y_row_stride + f]
+    const float *row_scale, *col_scale;
+    int64_t parts, m_ori, dim_n;
+    int64_t x_head_stride, y_head_stride, y_row_stride, val_head_stride;
+    int slices, heads, window, wide_log2;
+};
+
+// One wave per (part, head, slice).  256-thread workgroups hold 4 tasks.
+// MAPPED: tile slot values are val[head][map[slot]] (edge-map SpMM).
+template <int KIND, int T, bool MAPPED>
+__global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
+{
+    constexpr int EB = KindTraits<KIND>::EB;
+    constexpr int AW = (8 * EB) / 4;                 // dwords of one A operand
+    constexpr int XB = T * EB;                       // bytes per X vector
+    constexpr int XW = XB >= 4 ? XB / 4 : 1;         // dwords per X vector
+
+    const int lane = threadIdx.x & 63;
+    const int64_t task = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    if (task >= a.parts * a.heads * a.slices) return;
+    const int64_t part = task / (int64_t(a.heads) * a.slices);
+    const int hs = int(task - part * a.heads * a.slices);
+    const int head = hs / a.slices, slice = hs - head * a.slices;
+    const uint8_t *xh = a.x + int64_t(head) * a.x_head_stride * EB;
+
+    const int c_begin = a.rp[2 * part];
+    const int nreal = a.rp[2 * part + 1] - c_begin;
+    const int plen = a.rp[2 * part + 2] - c_begin;
+    const int g = lane >> 4, n = lane & 15;
+    const int64_t f0 = int64_t(slice) * 16 * T + int64_t(n) * T;
+    const bool f_ok = f0 < a.dim_n;
+    const int W = a.window;
+    const int wide = 1 << a.wide_log2;
+    const bool a_row_ok = n < W;
+
+    f32x4 acc[T];
+#pragma unroll
+    for (int j = 0; j < T; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int kb = 0; kb < nreal; kb += 32) {
+        const int kg = kb + 8 * g;  // first k slot of this lane's group
+
+        // ---- A operand: tile values of window row n, condensed columns kg..kg+7
+        uint32_t av[AW];
+#pragma unroll
+        for (int w = 0; w < AW; ++w) av[w] = 0;
+        if (MAPPED) {
+            if (a_row_ok && kg < nreal) {
+                const uint8_t *vh = a.val + int64_t(head) * a.val_head_stride * EB;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int64_t cc = int64_t(c_begin) + kg + i;
+                    if (kg + i < plen) {
+                        const int m = a.map[(cc >> a.wide_log2) * W * wide + int64_t(n) * wide + (cc & (wide - 1))];
+                        if (m >= 0) {
+                            uint32_t e;
+                            load_bytes<EB>(vh + int64_t(m) * EB, &e);
+                            if constexpr (EB == 4) av[i] = e;
+                            else if constexpr (EB == 2) av[i >> 1] |= e << (16 * (i & 1));
+                            else av[i >> 2] |= e << (8 * (i & 3));
+                        }
+                    }
+                }
+            }
+        } else if (a_row_ok && kg < nreal) {
+            if (wide >= 8) {
+                const int64_t cc = int64_t(c_begin) + kg;
+                const int64_t off = (cc >> a.wide_log2) * W * wide + int64_t(n) * wide + (cc & (wide - 1));
+                load_bytes<8 * EB>(a.val + off * EB, av);
+            } else {  // wide == 4: two 4-column blocks
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    if (kg + 4 * h < plen) {
+                        const int64_t cc = int64_t(c_begin) + kg + 4 * h;
+                        const int64_t off = (cc >> a.wide_log2) * W * wide + int64_t(n) * wide;
+                        load_bytes<4 * EB>(a.val + off * EB, av + h * (AW / 2));
+                    }
+                }
+            }
+        }
+
+        // ---- B rows: X[col[k]] for the lane's 8 k slots
+        uint32_t xv[8][XW];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+#pragma unroll
+            for (int w = 0; w < XW; ++w) xv[i][w] = 0;
+            const int k = kg + i;
+            const int c = k < nreal ? a.col[c_begin + k] : -1;
+            if (c >= 0 && f_ok) load_bytes<XB>(xh + (int64_t(c) * a.dim_n + f0) * EB, xv[i]);
+        }
+
+        // ---- MFMA, one per N tile (feature offset j within the lane's T)
+        if constexpr (KIND == K_F16 || KIND == K_BF16) {
+            const u32x4 A = {av[0], av[1], av[2], av[3]};
+#pragma unroll
+            for (int j = 0; j < T; ++j) {
+                const u32x4 B = gather16<T>(xv, j);
+                acc[j] = KIND == K_F16 ? mfma_f16(A, B, acc[j]) : mfma_bf16(A, B, acc[j]);
+            }
+        } else if constexpr (KIND == K_FP8) {
+            const u32x2 A = {av[0], av[1]};
+#pragma unroll
+            for (int j = 0; j < T; ++j) acc[j] = mfma_fp8(A, gather8<T>(xv, j), acc[j]);
+        } else if constexpr (KIND == K_F32) {
+#pragma unroll
+            for (int j = 0; j < T; ++j)
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    acc[j] = mfma_f32(__uint_as_float(av[i]), __uint_as_float(xv[i][j]), acc[j]);
+        } else {  // K_BF16X3
+            uint32_t ah[8], al[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) bf16_split(__uint_as_float(av[i]), ah[i], al[i]);
+            const u32x4 AH = {ah[0] | ah[1] << 16, ah[2] | ah[3] << 16, ah[4] | ah[5] << 16, ah[6] | ah[7] << 16};
+            const u32x4 AL = {al[0] | al[1] << 16, al[2] | al[3] << 16, al[4] | al[5] << 16, al[6] | al[7] << 16};
+#pragma unroll
+            for (int j = 0; j < T; ++j) {
+                uint32_t bh[8], bl[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) bf16_split(__uint_as_float(xv[i][j]), bh[i], bl[i]);
+                const u32x4 BH = {bh[0] | bh[1] << 16, bh[2] | bh[3] << 16, bh[4] | bh[5] << 16, bh[6] | bh[7] << 16};
+                const u32x4 BL = {bl[0] | bl[1] << 16, bl[2] | bl[3] << 16, bl[4] | bl[5] << 16, bl[6] | bl[7] << 16};
+                acc[j] = mfma_bf16(AL, BH, acc[j]);
+                acc[j] = mfma_bf16(AH, BL, acc[j]);
+                acc[j] = mfma_bf16(AH, BH, acc[j]);
+            }
+        }
+    }
+
+    // ---- store rows 4g+q of the window, features f0..f0+T-1
+    if (!f_ok) return;
+    const int64_t wrow = int64_t(a.win[part]) * W;
+    const bool split = a.atomic[part] != 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int r = 4 * g + q;
+        if (r >= W) break;
+        const int64_t row = wrow + r;
+        float v[T];
+#pragma unroll
+        for (int j = 0; j < T; ++j) v[j] = acc[j][q];
+        if constexpr (KIND == K_FP8) {
+            const float rs = row < a.m_ori ? a.row_scale[row] : 0.f;
+#pragma unroll
+            for (int j = 0; j < T; ++j) v[j] *= rs * a.col_scale[f0 + j];
+        }
+        float *dst;
+        if (split) {
+            dst = a.partial + ((part * a.heads + head) * W + r) * a.dim_n + f0;
+        } else {
+            if (row >= a.m_ori) break;
+            dst = a.y + int64_t(head) * a.y_head_stride + row * a.y_row_stride + f0;
+        }
+        if constexpr (T % 4 == 0) {
+#pragma unroll
+            for (int j = 0; j < T; j += 4)
+                *reinterpret_cast<f32x4 *>(dst + j) = f32x4{v[j], v[j + 1], v[j + 2], v[j + 3]};
+        } else {
+#pragma unroll
+            for (int j = 0; j < T; ++j) dst[j] = v[j];
+        }
+    }
+}
+
+// Sum the partial tiles of every split window in part order.  One workgroup
+// per part; only the first part of a split window does work.
+__global__ __launch_bounds__(256) void spmm_combine_kernel(SpmmArgs a)
+{
+    const int64_t p = blockIdx.x;
+    const int head = blockIdx.y;
+    if (!a.atomic[p]) return;
+    const int w = a.win[p];
+    if (p > 0 && a.atomic[p - 1] && a.win[p - 1] == w) return;
+    int64_t last = p;
+    while (last + 1 < a.parts && a.atomic[last + 1] && a.win[last + 1] == w) ++last;
+    const int64_t rows = min(int64_t(a.window), a.m_ori - int64_t(w) * a.window);
+    const int64_t n = rows * a.dim_n;
+    for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
+        const int64_t r = e / a.dim_n, f = e - r * a.dim_n;
+        float s = 0.f;
+        for (int64_t q = p; q <= last; ++q) s += a.partial[((q * a.heads + head) * a.window + r) * a.dim_n + f];
+        a.y[int64_t(head) * a.y_head_stride + (int64_t(w) * a.window + r) * a.y_row_stride + f] = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void fill_zero_kernel(float4 *p, int64_t n4, float *tail, int64_t ntail)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t k = i; k < n4; k += stride) p[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int64_t k = i; k < ntail; k += stride) tail[k] = 0.f;
+}
+
+int kind_of(int mode, int tile_type, int x_type)
+{
+    if (mode == ATTGNN_MODE_NATIVE) {
+        if (tile_type == ATTGNN_F16 && x_type == ATTGNN_F16) return K_F16;
+        if (tile_type == ATTGNN_BF16 && x_type == ATTGNN_BF16) return K_BF16;
+        if (tile_type == ATTGNN_FP8E4M3 && x_type == ATTGNN_FP8E4M3) return K_FP8;
+        return -1;
+    }
+    if (tile_type != ATTGNN_F32 || x_type != ATTGNN_F32) return -1;
+    if (mode == ATTGNN_MODE_BF16X3) return K_BF16X3;
+    if (mode == ATTGNN_MODE_F32) return K_F32;
+    return -1;
+}
+
+// Features per lane: the widest power of two dividing dim_n that keeps a lane
+// vector <= 16 bytes, narrowed while half of it still covers dim_n.
+int pick_t(int64_t dim_n, int eb)
+{
+    int t = 1;
+    while (t < 8 && t * 2 * eb <= 16 && dim_n % (t * 2) == 0) t *= 2;
+    while (t > 1 && int64_t(16) * (t / 2) >= dim_n) t /= 2;
+    return t;
+}
+
+template <int KIND, bool MAPPED>
+hipError_t launch_main(int t, const SpmmArgs &a, hipStream_t s)
+{
+    const int64_t tasks = a.parts * a.heads * a.slices;
+    const dim3 grid(unsigned((tasks + 3) / 4)), block(256);
+    switch (t) {
+    case 1: hipLaunchKernelGGL((spmm_block_kernel<KIND, 1, MAPPED>), grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((spmm_block_kernel<KIND, 2, MAPPED>), grid, block, 0, s, a); break;
+    case 4: hipLaunchKernelGGL((spmm_block_kernel<KIND, 4, MAPPED>), grid, block, 0, s, a); break;
+    case 8:
+        if constexpr (KindTraits<KIND>::EB <= 2) {
+            hipLaunchKernelGGL((spmm_block_kernel<KIND, 8, MAPPED>), grid, block, 0, s, a);
+            break;
+        }
+        return hipErrorInvalidValue;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// Zero Y (rows no part stores stay zero), run the main kernel, then combine
+// split windows.  Y is one contiguous buffer of heads * m_ori * dim_n floats.
+hipError_t run_spmm(int kind, bool mapped, int t, const SpmmArgs &a, hipStream_t s)
+{
+    const int64_t ny = int64_t(a.heads) * a.m_ori * a.dim_n;
+    if (ny > 0) {
+        const int64_t n4 = reinterpret_cast<uintptr_t>(a.y) % 16 == 0 ? ny / 4 : 0;
+        const int64_t blocks = std::min<int64_t>((std::max<int64_t>(n4, ny - 4 * n4) + 255) / 256, 65536);
+        hipLaunchKernelGGL(fill_zero_kernel, dim3(unsigned(blocks)), dim3(256), 0, s,
+                           reinterpret_cast<float4 *>(a.y), n4, a.y + 4 * n4, ny - 4 * n4);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (a.parts == 0) return hipSuccess;
+    hipError_t e;
+    switch (kind) {
+    case K_F16: e = mapped ? launch_main<K_F16, true>(t, a, s) : launch_main<K_F16, false>(t, a, s); break;
+    case K_BF16: e = mapped ? launch_main<K_BF16, true>(t, a, s) : launch_main<K_BF16, false>(t, a, s); break;
+    case K_FP8: e = launch_main<K_FP8, false>(t, a, s); break;
+    case K_BF16X3: e = mapped ? launch_main<K_BF16X3, true>(t, a, s) : launch_main<K_BF16X3, false>(t, a, s); break;
+    default: e = mapped ? launch_main<K_F32, true>(t, a, s) : launch_main<K_F32, false>(t, a, s); break;
+    }
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(spmm_combine_kernel, dim3(unsigned(a.parts), unsigned(a.heads)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" int64_t attgnn_spmm_workspace_elems(int64_t parts, int window, int64_t dim_n)
+{
+    return parts * window * dim_n;
+}
+
+namespace {
+
+int wide_log2_of(int wide)
+{
+    switch (wide) { case 4: return 2; case 8: return 3; case 16: return 4; case 32: return 5; default: return -1; }
+}
+
+int elt_bytes(int kind) { return kind == K_FP8 ? 1 : (kind == K_F16 || kind == K_BF16) ? 2 : 4; }
+
+}  // namespace
+
+extern "C" int attgnn_spmm_block(const int32_t *rp, const int32_t *col, const void *val, int tile_type,
+                                 const int32_t *window_of_part, const int32_t *atomic, int64_t parts,
+                                 const void *X, int x_type, int64_t n_x, float *Y, int64_t m_ori,
+                                 int64_t dim_n, int window, int wide, int mode, const float *row_scale,
+                                 const float *col_scale, float *workspace, int64_t workspace_elems,
+                                 attgnn_stream_t stream)
+{
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (parts < 0 || m_ori < 0 || dim_n <= 0 || n_x < 0) return ATTGNN_E_ARG;
+    if (!(window == 8 || window == 16)) return ATTGNN_E_ARG;
+    const int wl = wide_log2_of(wide);
+    if (wl < 0) return ATTGNN_E_ARG;
+    const int kind = kind_of(mode, tile_type, x_type);
+    if (kind < 0) return ATTGNN_E_UNSUPPORTED;
+    if (kind == K_FP8 && (!row_scale || !col_scale)) return ATTGNN_E_ARG;
+    if (!Y) return ATTGNN_E_ARG;
+    if (parts > 0 && (!rp || !col || !val || !window_of_part || !atomic || !X || !workspace)) return ATTGNN_E_ARG;
+    if (parts > 0 && workspace_elems < attgnn_spmm_workspace_elems(parts, window, dim_n)) return ATTGNN_E_ARG;
+
+    const int eb = elt_bytes(kind);
+    const int t = pick_t(dim_n, eb);
+    if (reinterpret_cast<uintptr_t>(X) % (t * eb) != 0) return ATTGNN_E_ARG;
+    if (reinterpret_cast<uintptr_t>(Y) % 4 != 0) return ATTGNN_E_ARG;
+
+    SpmmArgs a{};
+    a.rp = rp; a.col = col; a.win = window_of_part; a.atomic = atomic;
+    a.val = static_cast<const uint8_t *>(val); a.map = nullptr;
+    a.x = static_cast<const uint8_t *>(X);
+    a.y = Y; a.partial = workspace;
+    a.row_scale = row_scale; a.col_scale = col_scale;
+    a.parts = parts; a.m_ori = m_ori; a.dim_n = dim_n;
+    a.x_head_stride = 0; a.y_head_stride = 0; a.y_row_stride = dim_n; a.val_head_stride = 0;
+    a.slices = int((dim_n + 16 * t - 1) / (16 * t));
+    a.heads = 1; a.window = window; a.wide_log2 = wl;
+    // T*16-byte vector stores need 16-byte aligned rows
+    if (t >= 4 && (reinterpret_cast<uintptr_t>(Y) % 16 != 0)) return ATTGNN_E_ARG;
+    const hipError_t e = run_spmm(kind, false, t, a, s);
+    return e == hipSuccess ? ATTGNN_OK : int(e);
+}
+
+extern "C" int attgnn_spmm_edge_map(const int32_t *rp, const int32_t *col, const int32_t *map,
+                                    const void *edge_val, int val_type, int64_t nnz,
+                                    const int32_t *window_of_part, const int32_t *atomic, int64_t parts,
+                                    const void *X, int x_type, int64_t heads, int64_t n_x, float *Y,
+                                    int64_t m_ori, int64_t dim_n, int out_layout, int window, int wide,
+                                    int mode, float *workspace, int64_t workspace_elems,
+                                    attgnn_stream_t stream)
+{
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (parts < 0 || m_ori < 0 || dim_n <= 0 || n_x < 0 || heads <= 0 || nnz < 0) return ATTGNN_E_ARG;
+    if (!(window == 8 || window == 16)) return ATTGNN_E_ARG;
+    if (!(out_layout == ATTGNN_LAYOUT_HMD || out_layout == ATTGNN_LAYOUT_MHD)) return ATTGNN_E_ARG;
+    const int wl = wide_log2_of(wide);
+    if (wl < 0) return ATTGNN_E_ARG;
+    const int kind = kind_of(mode, val_type, x_type);
+    if (kind < 0 || kind == K_FP8) return ATTGNN_E_UNSUPPORTED;
+    if (!Y) return ATTGNN_E_ARG;
+    if (parts > 0 && (!rp || !col || !map || !edge_val || !window_of_part || !atomic || !X || !workspace))
+        return ATTGNN_E_ARG;
+    if (parts > 0 && workspace_elems < heads * attgnn_spmm_workspace_elems(parts, window, dim_n)) return ATTGNN_E_ARG;
+    const int eb = elt_bytes(kind);
+    const int t = pick_t(dim_n, eb);
+    if (reinterpret_cast<uintptr_t>(X) % (t * eb) != 0) return ATTGNN_E_ARG;
+    if (reinterpret_cast<uintptr_t>(Y) % 16 != 0) return ATTGNN_E_ARG;
+
+    SpmmArgs a{};
+    a.rp = rp; a.col = col; a.win = window_of_part; a.atomic = atomic;
+    a.val = static_cast<const uint8_t *>(edge_val); a.map = map;
+    a.x = static_cast<const uint8_t *>(X);
+    a.y = Y; a.partial = workspace;
+    a.parts = parts; a.m_ori = m_ori; a.dim_n = dim_n;
+    a.x_head_stride = n_x * dim_n;
+    a.val_head_stride = nnz;
+    if (out_layout == ATTGNN_LAYOUT_HMD) { a.y_head_stride = m_ori * dim_n; a.y_row_stride = dim_n; }
+    else { a.y_head_stride = dim_n; a.y_row_stride = heads * dim_n; }
+    a.slices = int((dim_n + 16 * t - 1) / (16 * t));
+    a.heads = int(heads); a.window = window; a.wide_log2 = wl;
+    if (t >= 4 && ((a.y_row_stride * 4) % 16 != 0 || (a.y_head_stride * 4) % 16 != 0)) return ATTGNN_E_ARG;
+    const hipError_t e = run_spmm(kind, true, t, a, s);
+    return e == hipSuccess ? ATTGNN_OK : int(e);
+}

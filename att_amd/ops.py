@@ -1,0 +1,263 @@
+"""Torch-facing wrappers over the C ABI (libattgnn.so).
+
+Each wrapper validates device / dtype / contiguity / shape on the host,
+allocates outputs and scratch through torch's caching allocator, and launches
+on torch's current HIP stream.  Nothing here computes on the CPU except the
+window-block preprocessing, which is host code in the reference as well.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib
+
+_TYPE_OF = {
+    torch.float32: _lib.F32,
+    torch.float16: _lib.F16,
+    torch.bfloat16: _lib.BF16,
+    torch.float8_e4m3fn: _lib.FP8E4M3,
+    torch.int32: _lib.I32,
+}
+_DTYPE_OF = {v: k for k, v in _TYPE_OF.items()}
+_MODES = {"native": _lib.MODE_NATIVE, "bf16x3": _lib.MODE_BF16X3, "f32": _lib.MODE_F32}
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None and t.numel() > 0 else ctypes.c_void_p(0)
+
+
+def _stream(device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _need_gpu(name, *tensors):
+    dev = None
+    for t in tensors:
+        if t is None:
+            continue
+        if not isinstance(t, torch.Tensor):
+            raise TypeError(f"{name}: expected torch.Tensor, got {type(t)}")
+        if t.device.type != "cuda":
+            raise ValueError(f"{name}: tensors must be on the GPU (got {t.device})")
+        if not t.is_contiguous():
+            raise ValueError(f"{name}: tensors must be contiguous")
+        if dev is None:
+            dev = t.device
+        elif t.device != dev:
+            raise ValueError(f"{name}: tensors on different devices")
+    return dev
+
+
+def _need_i32(name, *tensors):
+    for t in tensors:
+        if t.dtype != torch.int32:
+            raise TypeError(f"{name}: index arrays must be int32 (got {t.dtype})")
+
+
+# --------------------------------------------------------------------------
+# preprocessing (host)
+# --------------------------------------------------------------------------
+
+def blocks_build(row, col, src, src_values, out_dtype, window, wide, part_size,
+                 emit_empty, want_trans=False, nthreads=0):
+    """Window-block structure of a CSR matrix (host, OpenMP).
+
+    ``src``: "deg_f16" | "deg_f32" | "edge_f32" | "map".  Returns a dict with
+    CPU int32 tensors ``rp``, ``col``, ``window``, ``atomic``, the tile values
+    ``val`` (``out_dtype``; int32 edge ids for "map"), ``map_t`` (want_trans)
+    and ``n_unique`` (sum over windows of distinct columns).
+    """
+    srcs = {"deg_f16": _lib.SRC_DEG_F16, "deg_f32": _lib.SRC_DEG_F32,
+            "edge_f32": _lib.SRC_EDGE_F32, "map": _lib.SRC_MAP}
+    if src not in srcs:
+        raise ValueError(f"unknown src {src!r}")
+    row = torch.as_tensor(row).to(torch.int32).contiguous().cpu()
+    col = torch.as_tensor(col).to(torch.int32).contiguous().cpu()
+    if row.dim() != 1 or col.dim() != 1:
+        raise ValueError("row and col must be 1-D")
+    keep = None
+    if src == "map":
+        sp = ctypes.c_void_p(0)
+        out_dtype = torch.int32
+    else:
+        want = {"deg_f16": torch.float16, "deg_f32": torch.float32, "edge_f32": torch.float32}[src]
+        keep = torch.as_tensor(src_values).to(want).contiguous().cpu()
+        expect = row.numel() - 1 if src.startswith("deg") else col.numel()
+        if keep.numel() < expect:
+            raise ValueError(f"{src}: need {expect} values, got {keep.numel()}")
+        sp = ctypes.c_void_p(keep.data_ptr())
+    handle = ctypes.c_void_p()
+    rc = lib().attgnn_blocks_build(_ptr(row), row.numel(), _ptr(col), col.numel(), srcs[src], sp,
+                                   _TYPE_OF[out_dtype], window, wide, part_size, int(bool(emit_empty)),
+                                   int(bool(want_trans)), nthreads, ctypes.byref(handle))
+    check(rc, "attgnn_blocks_build")
+    try:
+        n = [ctypes.c_int64() for _ in range(5)]
+        has_t = ctypes.c_int()
+        check(lib().attgnn_blocks_sizes(handle, *[ctypes.byref(x) for x in n], ctypes.byref(has_t)),
+              "attgnn_blocks_sizes")
+        n_rp, n_col, n_val, n_parts, n_unique = [x.value for x in n]
+        out = {
+            "rp": torch.empty(n_rp, dtype=torch.int32),
+            "col": torch.empty(n_col, dtype=torch.int32),
+            "val": torch.empty(n_val, dtype=out_dtype),
+            "window": torch.empty(n_parts, dtype=torch.int32),
+            "atomic": torch.empty(n_parts, dtype=torch.int32),
+            "map_t": torch.empty(n_val, dtype=torch.int32) if has_t.value else None,
+        }
+        check(lib().attgnn_blocks_export(handle, _ptr(out["rp"]), _ptr(out["col"]), _ptr(out["val"]),
+                                         _ptr(out["map_t"]), _ptr(out["window"]), _ptr(out["atomic"])),
+              "attgnn_blocks_export")
+        out["n_unique"] = n_unique
+    finally:
+        lib().attgnn_blocks_free(handle)
+    return out
+
+
+# --------------------------------------------------------------------------
+# SpMM
+# --------------------------------------------------------------------------
+
+def spmm_block(rp, col, val, window_of_part, atomic, X, m_ori, window, wide, mode="native",
+               row_scale=None, col_scale=None, out=None):
+    """Y[m_ori, D] (fp32) = A . X over a window-block structure (all on the GPU)."""
+    dev = _need_gpu("spmm_block", rp, col, val, window_of_part, atomic, X, row_scale, col_scale, out)
+    _need_i32("spmm_block", rp, col, window_of_part, atomic)
+    if X.dim() != 2:
+        raise ValueError("X must be 2-D [rows, dim]")
+    if rp.numel() != 2 * window_of_part.numel() + 1 or atomic.numel() != window_of_part.numel():
+        raise ValueError("rp / window / atomic sizes disagree")
+    parts = window_of_part.numel()
+    n_x, dim = X.shape
+    y = out if out is not None else torch.empty((m_ori, dim), dtype=torch.float32, device=dev)
+    if y.shape != (m_ori, dim) or y.dtype != torch.float32:
+        raise ValueError("out must be float32 [m_ori, dim]")
+    ws_n = lib().attgnn_spmm_workspace_elems(parts, window, dim) if parts else 0
+    ws = torch.empty(max(ws_n, 1), dtype=torch.float32, device=dev)
+    rc = lib().attgnn_spmm_block(_ptr(rp), _ptr(col), _ptr(val), _TYPE_OF[val.dtype], _ptr(window_of_part),
+                                 _ptr(atomic), parts, _ptr(X), _TYPE_OF[X.dtype], n_x, _ptr(y), m_ori, dim,
+                                 window, wide, _MODES[mode], _ptr(row_scale), _ptr(col_scale), _ptr(ws), ws_n,
+                                 _stream(dev))
+    check(rc, "attgnn_spmm_block")
+    return y
+
+
+def spmm_edge_map(rp, col, emap, edge_val, window_of_part, atomic, X, out_layout, window, wide,
+                  m_ori=None, mode="native"):
+    """Edge-map SpMM.  X [H, N, D]; edge_val [H, nnz] (or [nnz]); Y fp32 [H, M, D] or [M, H, D]."""
+    dev = _need_gpu("spmm_edge_map", rp, col, emap, edge_val, window_of_part, atomic, X)
+    _need_i32("spmm_edge_map", rp, col, emap, window_of_part, atomic)
+    if X.dim() == 2:
+        X = X.unsqueeze(0)
+    heads, n_x, dim = X.shape
+    ev = edge_val.reshape(heads, -1) if edge_val.dim() != 2 else edge_val
+    if ev.shape[0] != heads:
+        raise ValueError(f"edge values have {ev.shape[0]} heads, X has {heads}")
+    nnz = ev.shape[1]
+    m = n_x if m_ori is None else m_ori
+    layout = {"hmd": _lib.LAYOUT_HMD, "mhd": _lib.LAYOUT_MHD}[out_layout]
+    shape = (heads, m, dim) if layout == _lib.LAYOUT_HMD else (m, heads, dim)
+    y = torch.empty(shape, dtype=torch.float32, device=dev)
+    parts = window_of_part.numel()
+    ws_n = heads * lib().attgnn_spmm_workspace_elems(parts, window, dim) if parts else 0
+    ws = torch.empty(max(ws_n, 1), dtype=torch.float32, device=dev)
+    if ev.dtype != X.dtype:
+        ev = ev.to(X.dtype)
+    ev = ev.contiguous()
+    rc = lib().attgnn_spmm_edge_map(_ptr(rp), _ptr(col), _ptr(emap), _ptr(ev), _TYPE_OF[ev.dtype], nnz,
+                                    _ptr(window_of_part), _ptr(atomic), parts, _ptr(X), _TYPE_OF[X.dtype],
+                                    heads, n_x, _ptr(y), m, dim, layout, window, wide, _MODES[mode], _ptr(ws),
+                                    ws_n, _stream(dev))
+    check(rc, "attgnn_spmm_edge_map")
+    return y
+
+
+def rowsum_edge_map(rp, emap, window_of_part, edge_val, m_rows, out_layout, window, wide):
+    """Row sums of per-edge values; fp32 [H, M] ("hmd") or [M, H] ("mhd")."""
+    dev = _need_gpu("rowsum_edge_map", rp, emap, window_of_part, edge_val)
+    _need_i32("rowsum_edge_map", rp, emap, window_of_part)
+    ev = edge_val if edge_val.dim() == 2 else edge_val.reshape(1, -1)
+    heads, nnz = ev.shape
+    layout = {"hmd": _lib.LAYOUT_HMD, "mhd": _lib.LAYOUT_MHD}[out_layout]
+    shape = (heads, m_rows) if layout == _lib.LAYOUT_HMD else (m_rows, heads)
+    out = torch.empty(shape, dtype=torch.float32, device=dev)
+    rc = lib().attgnn_rowsum_edge_map(_ptr(rp), _ptr(emap), _ptr(window_of_part), window_of_part.numel(),
+                                      _ptr(ev), _TYPE_OF[ev.dtype], nnz, heads, _ptr(out), layout, m_rows,
+                                      window, wide, _stream(dev))
+    check(rc, "attgnn_rowsum_edge_map")
+    return out
+
+
+# --------------------------------------------------------------------------
+# SDDMM and per-edge attention terms
+# --------------------------------------------------------------------------
+
+def _hmd_strides(X, layout):
+    """(heads, rows, dim, head_stride, row_stride) of a 2-D or 3-D tensor."""
+    if X.dim() == 2:
+        return 1, X.shape[0], X.shape[1], 0, X.shape[1]
+    if layout == "hmd":
+        h, n, d = X.shape
+        return h, n, d, n * d, d
+    n, h, d = X.shape
+    return h, n, d, d, h * d
+
+
+def sddmm_block(rp, col, emap, window_of_part, A, B, nnz, window, wide, layout="hmd",
+                out_dtype=None, mode="bf16x3"):
+    """out[h, e=(i,j)] = <A_h[i], B_h[j]> in CSR edge order ([H, nnz])."""
+    dev = _need_gpu("sddmm_block", rp, col, emap, window_of_part, A, B)
+    _need_i32("sddmm_block", rp, col, emap, window_of_part)
+    if A.dtype != B.dtype:
+        raise TypeError("A and B must share a dtype")
+    ha, na, da, ahs, ars = _hmd_strides(A, layout)
+    hb, nb, db, bhs, brs = _hmd_strides(B, layout)
+    if ha != hb or da != db:
+        raise ValueError("A and B disagree in heads or dim")
+    out_dtype = out_dtype or A.dtype
+    out = torch.zeros((ha, nnz), dtype=out_dtype, device=dev)
+    rc = lib().attgnn_sddmm_block(_ptr(rp), _ptr(col), _ptr(emap), _ptr(window_of_part),
+                                  window_of_part.numel(), _ptr(A), ahs, ars, _ptr(B), bhs, brs,
+                                  _TYPE_OF[A.dtype], ha, na, da, _ptr(out), _TYPE_OF[out_dtype], nnz, window, wide,
+                                  _MODES[mode], _stream(dev))
+    check(rc, "attgnn_sddmm_block")
+    return out
+
+
+def node_scores(X, vec, layout="hmd"):
+    """s[h, i] = X_h[i] . vec_h  (fp32 [H, N])."""
+    dev = _need_gpu("node_scores", X, vec)
+    h, n, d, hs, rs = _hmd_strides(X, layout)
+    if vec.numel() != h * d:
+        raise ValueError(f"vector has {vec.numel()} elements, expected {h}*{d}")
+    s = torch.empty((h, n), dtype=torch.float32, device=dev)
+    rc = lib().attgnn_node_scores(_ptr(X), _TYPE_OF[X.dtype], hs, rs, _ptr(vec), _TYPE_OF[vec.dtype], h, n, d,
+                                  _ptr(s), _stream(dev))
+    check(rc, "attgnn_node_scores")
+    return s
+
+
+def edge_from_nodes(rp, col, emap, window_of_part, nnz, heads, n_rows, window, wide, out_dtype,
+                    s0=None, s1=None, g0=None, g0_strides=None):
+    """out[h, e=(i,j)] = S0[h, i] + S1[h, j] over the map's edges ([H, nnz])."""
+    dev = _need_gpu("edge_from_nodes", rp, col, emap, window_of_part, s0, s1, g0)
+    _need_i32("edge_from_nodes", rp, col, emap, window_of_part)
+    out = torch.zeros((heads, nnz), dtype=out_dtype, device=dev)
+    if g0 is not None:
+        hs0, rs0 = g0_strides
+        g0t = _TYPE_OF[g0.dtype]
+    else:
+        hs0, rs0 = (n_rows, 1)
+        g0t = 0
+    rc = lib().attgnn_edge_from_nodes(_ptr(rp), _ptr(col), _ptr(emap), _ptr(window_of_part),
+                                      window_of_part.numel(), _ptr(s0), hs0, rs0, _ptr(s1),
+                                      (s1.shape[1] if s1 is not None else 0), 1, _ptr(g0), g0t, heads, n_rows,
+                                      _ptr(out), _TYPE_OF[out_dtype], nnz, window, wide, _stream(dev))
+    check(rc, "attgnn_edge_from_nodes")
+    return out
+
+
+def dtype_code(dtype):
+    return _TYPE_OF[dtype]

@@ -1,0 +1,232 @@
+/*
+ * attgnn.h -- C ABI of the MI355X window-block sparse GNN library (libattgnn.so).
+ *
+ * Plain pointers and sizes only.  Device pointers are HIP device memory on the
+ * current device; every GPU entry point takes the hipStream_t it runs on
+ * (pass the caller's current stream) and returns 0 or a nonzero status code
+ * (an ATTGNN_E_* value, or a hipError_t for launch failures).  Outputs are
+ * allocated by the caller.  No entry point allocates device memory, keeps
+ * global state, or synchronises the stream.  Results are deterministic: a
+ * window split across workgroups is reduced in a fixed order.
+ *
+ * Each entry point names the reference interface it replaces (paths are
+ * relative to the reference repository root).
+ */
+#ifndef ATTGNN_H
+#define ATTGNN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t *attgnn_stream_t; /* == hipStream_t */
+
+/* ---- status codes -------------------------------------------------------- */
+enum {
+    ATTGNN_OK = 0,
+    ATTGNN_E_ARG = -1,      /* bad shape, size, pointer or enum value          */
+    ATTGNN_E_FORMAT = -2,   /* malformed CSR / block structure                 */
+    ATTGNN_E_NOMEM = -3,    /* host allocation failed (preprocessing)          */
+    ATTGNN_E_UNSUPPORTED = -4,
+};
+const char *attgnn_status_string(int status);
+
+/* ---- element types -------------------------------------------------------- */
+enum {
+    ATTGNN_F32 = 0,
+    ATTGNN_F16 = 1,
+    ATTGNN_BF16 = 2,
+    ATTGNN_FP8E4M3 = 3, /* OCP e4m3fn */
+    ATTGNN_I32 = 4,
+};
+
+/* ---- multi-head dense layouts ------------------------------------------- */
+enum {
+    ATTGNN_LAYOUT_HMD = 0, /* [heads][rows][dim]  ("h_m" in the reference)   */
+    ATTGNN_LAYOUT_MHD = 1, /* [rows][heads][dim]  ("m_h" in the reference)   */
+};
+
+/* ---- SpMM arithmetic modes ----------------------------------------------- */
+enum {
+    ATTGNN_MODE_NATIVE = 0, /* MFMA in the tile/X element type, FP32 accumulate:
+                               f16 x f16, bf16 x bf16, fp8 x fp8 (+ scales)     */
+    ATTGNN_MODE_BF16X3 = 1, /* fp32 tile/X split into bf16 hi+lo, 3 MFMA terms  */
+    ATTGNN_MODE_F32 = 2,    /* fp32 MFMA (16x16x4)                              */
+};
+
+/* ====================================================================== */
+/* Window-block preprocessing (host, OpenMP)                               */
+/* ====================================================================== */
+
+/* Source of the tile contents. */
+enum {
+    ATTGNN_SRC_DEG_F16 = 0, /* value = Half(float(d[row]) * float(d[col])), d fp16
+                               (gcn/Block_gcn/example.cpp:105)                  */
+    ATTGNN_SRC_DEG_F32 = 1, /* value = cvt(d[row] * d[col]) in fp32, d fp32
+                               (example.cpp:259 when out type is F32)           */
+    ATTGNN_SRC_EDGE_F32 = 2,/* value = cvt(v[m]) for CSR position m             */
+    ATTGNN_SRC_MAP = 3,     /* value = CSR position m (int32), -1 = padding     */
+};
+
+typedef struct attgnn_blocks attgnn_blocks; /* opaque host-side result */
+
+/*
+ * Build the window-block structure of a CSR matrix.
+ *
+ * Replaces Block_sr_bcrs.blockProcess_fp16_balance  (gcn/Block_gcn/example.cpp:48)
+ *          Block_sr_bcrs.blockProcess_tf32_balance  (gcn/Block_gcn/example.cpp:203)
+ *          ATT_Block[_v3].blockProcess_sddmm_balance_gnn[_trans]
+ *                                                   (Block/include/example.h:4-6)
+ *
+ * row[rows_plus1], col[nnz]: CSR (columns need not be sorted).  src: one of
+ * ATTGNN_SRC_*; src_ptr: d[rows] (fp16 bits or fp32) or v[nnz] (fp32) or NULL.
+ * out_type: tile element type (F16/BF16/F32 for values, I32 for MAP).
+ * window rows per window, wide columns per block, part_size blocks per part.
+ * emit_empty: 1 = an empty window emits the part (0,0) (tf32 variant),
+ *             0 = it emits nothing (fp16 / sddmm variants).
+ * want_trans: MAP only -- also build the transposed-edge map.
+ * nthreads: OpenMP threads (0 = default).
+ */
+int attgnn_blocks_build(const int32_t *row, int64_t rows_plus1, const int32_t *col, int64_t nnz,
+                        int src, const void *src_ptr, int out_type, int window, int wide,
+                        int part_size, int emit_empty, int want_trans, int nthreads,
+                        attgnn_blocks **out);
+
+/* Sizes of the arrays: rp[n_rp], col[n_col], val[n_val], map_t[n_val] (if
+ * built), window_of_part[n_parts], atomic[n_parts]; U = sum over windows of
+ * distinct columns (unpadded). */
+int attgnn_blocks_sizes(const attgnn_blocks *b, int64_t *n_rp, int64_t *n_col, int64_t *n_val,
+                        int64_t *n_parts, int64_t *n_unique, int *has_map_t);
+
+/* Copy the arrays out into caller buffers (any of them may be NULL). */
+int attgnn_blocks_export(const attgnn_blocks *b, int32_t *rp, int32_t *col, void *val,
+                         int32_t *map_t, int32_t *window_of_part, int32_t *atomic);
+
+void attgnn_blocks_free(attgnn_blocks *b);
+
+/* ====================================================================== */
+/* Window-block SpMM (device)                                              */
+/* ====================================================================== */
+
+/*
+ * Y[m_ori, dim_n] (fp32, row-major) = A . X, A given as window blocks.
+ *
+ * Replaces SpMM_sr_bcrs.forward_fp16_gnn  (gcn/SpMM/src/benchmark.cpp:35-44,
+ *                                          launcher spmmKernel.cu:1693-1727)
+ *          SpMM_sr_bcrs.forward_tf32_gnn  (benchmark.cpp:91-100,
+ *                                          launcher spmmKernel.cu:1731-1765)
+ *
+ * rp[2*parts+1], col[...], val[...] (tile_type), window_of_part[parts],
+ * atomic[parts] exactly as produced by attgnn_blocks_build.  X[n_x, dim_n] of
+ * x_type, row-major.  window: 8 or 16.  wide: 4, 8, 16 or 32.
+ * row_scale[m_ori] / col_scale[dim_n]: FP8 mode scales (NULL otherwise).
+ * workspace: fp32 scratch of at least attgnn_spmm_workspace_elems(...) elements
+ * (partial sums of split windows, reduced in part order); required when
+ * parts > 0.  Every row of Y is written (rows no part covers are zero).
+ * X must be aligned to its lane vector (<= 16 bytes) and Y to 16 bytes.
+ */
+int attgnn_spmm_block(const int32_t *rp, const int32_t *col, const void *val, int tile_type,
+                      const int32_t *window_of_part, const int32_t *atomic, int64_t parts,
+                      const void *X, int x_type, int64_t n_x, float *Y, int64_t m_ori,
+                      int64_t dim_n, int window, int wide, int mode, const float *row_scale,
+                      const float *col_scale, float *workspace, int64_t workspace_elems,
+                      attgnn_stream_t stream);
+
+int64_t attgnn_spmm_workspace_elems(int64_t parts, int window, int64_t dim_n);
+
+/*
+ * Edge-map SpMM: Y_h = A_h . X_h with A_h[i, j] = edge_val[h*nnz + map[slot(i,j)]],
+ * tiles given by the edge-id map of attgnn_blocks_build(ATTGNN_SRC_MAP) (or its
+ * transposed map, which aggregates with A^T on a symmetric pattern).
+ *
+ * Replaces ATT_SpMM_v{1,3}.spmm_{fp16,tf32}_h_m  (ATT_SpMM_v3.spmm_fp16_h_m, caller
+ *            end2end_head/gat_final/mgat_csr/mgat_conv.py:99-106; known answer
+ *            benchmark-v3/spmm-benchmark/test/spmm_fp16/test_spmm_h_m.py:45-66)
+ *          ATT_SpMM_v{1,3}.spmm_{fp16,tf32}_m_h  (mgat_conv.py:79-86;
+ *            test_spmm_m_h.py:49-84)
+ *
+ * edge_val[heads][nnz] (F16/BF16/F32, matching x_type), X[heads][n_x][dim_n],
+ * Y fp32 in out_layout (HMD: [heads][m_ori][dim_n], MHD: [m_ori][heads][dim_n]).
+ * mode: NATIVE for F16/BF16; BF16X3 or F32 for F32.  workspace: at least
+ * heads * attgnn_spmm_workspace_elems(parts, window, dim_n) floats.
+ */
+int attgnn_spmm_edge_map(const int32_t *rp, const int32_t *col, const int32_t *map,
+                         const void *edge_val, int val_type, int64_t nnz,
+                         const int32_t *window_of_part, const int32_t *atomic, int64_t parts,
+                         const void *X, int x_type, int64_t heads, int64_t n_x, float *Y,
+                         int64_t m_ori, int64_t dim_n, int out_layout, int window, int wide,
+                         int mode, float *workspace, int64_t workspace_elems,
+                         attgnn_stream_t stream);
+
+/*
+ * Row sums of per-edge values through the edge-id map:
+ * out(h, i) = sum_{slots of row i} edge_val[h*nnz + map[slot]] (fp32).
+ *
+ * Replaces ATT_SpMM_v{1,3}.spmm_{fp16,tf32}_{h_m,m_h}_rowsum
+ *            (callers end2end_head/gat_final/mgat_csr/mgat_conv.py:42-59,126-132;
+ *             known answer test_spmm_h_m_ones.py:45-70)
+ * out is [heads][m_rows] (HMD) or [m_rows][heads] (MHD); rows no window
+ * covers are zero.  Deterministic (fixed-order reduction per window).
+ */
+int attgnn_rowsum_edge_map(const int32_t *rp, const int32_t *map, const int32_t *window_of_part,
+                           int64_t parts, const void *edge_val, int val_type, int64_t nnz,
+                           int64_t heads, float *out, int out_layout, int64_t m_rows, int window,
+                           int wide, attgnn_stream_t stream);
+
+/* ====================================================================== */
+/* Window-block SDDMM and per-edge attention terms (device)                */
+/* ====================================================================== */
+
+/*
+ * out[h*nnz + e] = < A[h, i, :], B[h, j, :] > for every edge e = (i, j) in the
+ * edge-id map; A element (h,i,d) at A[h*a_head_stride + i*a_row_stride + d].
+ *
+ * Replaces ATT_SDDMM_v{1,3}.sddmm_{fp16,tf32}_h_nnz
+ *            (benchmark-v3/sddmm-benchmark/include/compute_utils.h:142-280,427-530,
+ *             output_tile.h:5-84; caller end2end_head/gat_final/mgat_csr/mgat_conv.py:109-117;
+ *             known answer test/sddmm_fp16/test_sddmm_fp16.py:50-69)
+ * in_type F16/BF16 (MFMA in that type) or F32 (mode BF16X3 or F32);
+ * out_type F16/BF16/F32.  Edges of the map never visited are left untouched.
+ */
+int attgnn_sddmm_block(const int32_t *rp, const int32_t *col, const int32_t *map,
+                       const int32_t *window_of_part, int64_t parts, const void *A,
+                       int64_t a_head_stride, int64_t a_row_stride, const void *B,
+                       int64_t b_head_stride, int64_t b_row_stride, int in_type,
+                       int64_t heads, int64_t n_rows, int64_t dim, void *out, int out_type,
+                       int64_t nnz, int window, int wide, int mode, attgnn_stream_t stream);
+
+/*
+ * scores[h*n_rows + i] = sum_d X[h*x_head_stride + i*x_row_stride + d] * vec[h*dim + d]
+ * (fp32 accumulate) -- the per-node half of the attention projections.
+ *
+ * Part of ATT_SDDMM_v{1,3}.att_{fp16,tf32}_h_nnz / attv2_{fp16,tf32}_h_nnz
+ *            (compute_utils.h:1144-1218; callers end2end_head/gat_final/mgat_csr/mgat_conv.py:21-30,
+ *             end2end_head/gatv2_final/mgat_csr/mgat_conv.py:22-31)
+ */
+int attgnn_node_scores(const void *X, int x_type, int64_t x_head_stride, int64_t x_row_stride,
+                       const void *vec, int vec_type, int64_t heads, int64_t n_rows, int64_t dim,
+                       float *scores, attgnn_stream_t stream);
+
+/*
+ * out[h*nnz + e] = S0(h, i) + S1(h, j) for every edge e = (i, j) of the map, with
+ * S0(h,i) = s0[h*s0_head_stride + i*s0_row_stride] (or the typed g0 at the same
+ * strides when g0 != NULL) and S1 likewise from s1 (NULL = 0).
+ *
+ * Replaces the per-edge half of ATT_SDDMM_v{1,3}.att_* / attv2_* (output_tile.h:89-202)
+ *          ATT_SDDMM_v{1,3}.sddmm_{fp16,tf32}_ones_h_nnz (g0 = g, s1 = NULL;
+ *            caller end2end_head/gat_final/mgat_csr/mgat_conv.py:142-149;
+ *            known answer test/sddmm_fp16/test_sddmm_fp16_ones.py:51-68)
+ */
+int attgnn_edge_from_nodes(const int32_t *rp, const int32_t *col, const int32_t *map,
+                           const int32_t *window_of_part, int64_t parts, const float *s0,
+                           int64_t s0_head_stride, int64_t s0_row_stride, const float *s1,
+                           int64_t s1_head_stride, int64_t s1_row_stride, const void *g0,
+                           int g0_type, int64_t heads, int64_t n_rows, void *out, int out_type,
+                           int64_t nnz, int window, int wide, attgnn_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ATTGNN_H */

@@ -1,0 +1,125 @@
+"""Window-block SpMM on the GPU against the FP64 oracle (all modes), the
+reference's known-answer test, torch.sparse, and run-to-run determinism."""
+import numpy as np
+import pytest
+import torch
+
+import att_amd.compat as compat
+from att_amd import graphs as G
+from att_amd import ops
+from oracle import dense as O
+from tests.gpu_util import TOY_COL, TOY_ROW, csr_rows, rand_csr
+
+pytestmark = pytest.mark.gpu
+
+
+def test_reference_balance_known_answer(cuda):
+    """gcn/SpMM/test/mgcn/test_balance.py:20-62 with partSize 1 (every window
+    split), small-integer X so FP16 products are exact; all columns compared."""
+    m = compat.MODULES
+    dd = torch.ones(32, dtype=torch.float16)
+    rp, col, val, win, at = m["Block_sr_bcrs"].blockProcess_fp16_balance(
+        torch.from_numpy(TOY_ROW), torch.from_numpy(TOY_COL), dd, 8, 8, 1)
+    for dimN in (16, 64, 66, 2):
+        rhs = torch.randint(1, 3, (32, dimN), generator=torch.Generator().manual_seed(dimN)).half()
+        y = m["SpMM_sr_bcrs"].forward_fp16_gnn(rp.to(cuda), col.to(cuda), val.to(cuda), win.to(cuda), at.to(cuda),
+                                               rhs.to(cuda), 32, dimN, 30)[0]
+        ref = O.spmm(TOY_ROW, TOY_COL, np.ones(38), rhs.double().numpy(), n_rows=30)
+        assert y.shape == (30, dimN)
+        assert np.array_equal(y.cpu().double().numpy(), ref)
+
+
+def _tile_values_f64(rp_csr, col_csr, d, dtype):
+    v = (d[csr_rows(rp_csr)].astype(np.float32) * d[col_csr].astype(np.float32))
+    return torch.from_numpy(v).to(dtype).double().numpy()
+
+
+@pytest.mark.parametrize("window,wide,ps", [(8, 8, 32), (8, 8, 1), (16, 32, 2), (16, 32, 1000), (8, 4, 2),
+                                            (16, 16, 1), (16, 8, 3), (8, 32, 2)])
+@pytest.mark.parametrize("dim", [16, 64, 48, 256, 66, 7])
+def test_spmm_native_vs_oracle(cuda, window, wide, ps, dim):
+    rng = np.random.default_rng(window * 1000 + wide * 10 + ps + dim)
+    n = 333
+    rp, col = rand_csr(rng, n, 24, hub_rows=3)
+    d = (rng.random(n) + 0.2).astype(np.float32)
+    for xdt, ydt in ((torch.float16, torch.float16), (torch.bfloat16, torch.bfloat16)):
+        src = "deg_f16" if xdt == torch.float16 else "deg_f32"
+        dv = d.astype(np.float16) if xdt == torch.float16 else d
+        b = ops.blocks_build(rp, col, src, dv, ydt, window, wide, ps, emit_empty=(window == 16))
+        X = torch.from_numpy(rng.uniform(-1, 1, (n, dim)).astype(np.float32)).to(xdt)
+        y = ops.spmm_block(b["rp"].to(cuda), b["col"].to(cuda), b["val"].to(cuda), b["window"].to(cuda),
+                           b["atomic"].to(cuda), X.to(cuda), n, window, wide)
+        # oracle with the same quantised tile values and X (error = FP32 accumulation only)
+        if xdt == torch.float16:
+            vals = torch.from_numpy(dv[csr_rows(rp)].astype(np.float32) * dv[col].astype(np.float32)).half().double().numpy()
+        else:
+            vals = _tile_values_f64(rp, col, d, torch.bfloat16)
+        covered = n // window * window
+        ref = O.spmm(rp, col, vals, X.double().numpy())
+        ref[covered:] = 0.0                     # trailing rows are dropped by the windowing
+        err = O.rel_fro(y.cpu().numpy(), ref)
+        assert err < 2e-6, (xdt, err)
+
+
+@pytest.mark.parametrize("mode,tol", [("bf16x3", 1e-5), ("f32", 2e-6)])
+@pytest.mark.parametrize("window,wide,ps", [(8, 4, 32), (16, 32, 2), (16, 8, 1)])
+def test_spmm_fp32_modes_vs_fp64(cuda, mode, tol, window, wide, ps):
+    rng = np.random.default_rng(7)
+    n = 400
+    rp, col = rand_csr(rng, n, 30, hub_rows=2)
+    vals = rng.standard_normal(col.size).astype(np.float32)
+    b = ops.blocks_build(rp, col, "edge_f32", vals, torch.float32, window, wide, ps, emit_empty=True)
+    for dim in (64, 128, 20):
+        X = rng.uniform(-1, 1, (n, dim)).astype(np.float32)
+        y = ops.spmm_block(b["rp"].to(cuda), b["col"].to(cuda), b["val"].to(cuda), b["window"].to(cuda),
+                           b["atomic"].to(cuda), torch.from_numpy(X).to(cuda), n, window, wide, mode=mode)
+        ref = O.spmm(rp, col, vals, X)
+        ref[n // window * window:] = 0.0
+        err = O.rel_fro(y.cpu().numpy(), ref)
+        assert err < tol, (mode, dim, err)
+
+
+def test_tf32_dropin_and_torch_sparse(cuda):
+    """SpMM_sr_bcrs.forward_tf32_gnn on the 16,384-node Chung-Lu graph (mean
+    degree 12) vs torch.sparse.mm in FP32 on the same device."""
+    g = G.chung_lu(16384, 12, 2.1, seed=0)
+    d = G.gcn_norm_degree(g)
+    rp, col, val, win, at = compat.MODULES["Block_sr_bcrs"].blockProcess_tf32_balance(
+        torch.from_numpy(g.rowptr32()), torch.from_numpy(g.indices), torch.from_numpy(d), 8, 4, 32)
+    X = torch.from_numpy(np.random.default_rng(1).uniform(-1, 1, (g.n, 64)).astype(np.float32)).to(cuda)
+    y = compat.MODULES["SpMM_sr_bcrs"].forward_tf32_gnn(rp.to(cuda), col.to(cuda), val.to(cuda), win.to(cuda),
+                                                         at.to(cuda), X, g.n, 64, g.n)[0]
+    A = torch.sparse_csr_tensor(torch.from_numpy(g.indptr), torch.from_numpy(g.indices.astype(np.int64)),
+                                torch.from_numpy(G.gcn_norm_values(g, d)), (g.n, g.n)).to(cuda)
+    ref = torch.sparse.mm(A, X)
+    err = (y - ref).norm() / ref.norm()
+    assert err.item() < 1e-5
+    ref64 = O.spmm(g.indptr, g.indices, G.gcn_norm_values(g, d), X.cpu().numpy())
+    assert O.rel_fro(y.cpu().numpy(), ref64) < 1e-5
+
+
+def test_determinism_run_to_run(cuda):
+    rng = np.random.default_rng(9)
+    n = 2000
+    rp, col = rand_csr(rng, n, 40, hub_rows=4)
+    vals = rng.standard_normal(col.size).astype(np.float32)
+    b = ops.blocks_build(rp, col, "edge_f32", vals, torch.bfloat16, 16, 32, 1, emit_empty=True)
+    args = [b[k].to(cuda) for k in ("rp", "col", "val", "window", "atomic")]
+    assert int(b["atomic"].sum()) > 0
+    X = torch.from_numpy(rng.uniform(-1, 1, (n, 128)).astype(np.float32)).to(torch.bfloat16).to(cuda)
+    y0 = ops.spmm_block(*args, X, n, 16, 32)
+    for _ in range(3):
+        y1 = ops.spmm_block(*args, X, n, 16, 32)
+        assert torch.equal(y0, y1)
+
+
+def test_empty_and_degenerate(cuda):
+    # graph with no edges: every row of Y is zero
+    rp = np.zeros(33, np.int32)
+    col = np.zeros(0, np.int32)
+    b = ops.blocks_build(rp, col, "deg_f32", np.ones(32, np.float32), torch.float32, 8, 4, 2, emit_empty=False)
+    assert b["window"].numel() == 0
+    X = torch.ones(32, 16, device=cuda)
+    y = ops.spmm_block(*[b[k].to(cuda) for k in ("rp", "col", "val", "window", "atomic")], X, 32, 8, 4,
+                       mode="f32")
+    assert torch.count_nonzero(y) == 0
