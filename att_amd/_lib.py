@@ -14,6 +14,7 @@ F32, F16, BF16, FP8E4M3, I32 = 0, 1, 2, 3, 4
 MODE_NATIVE, MODE_BF16X3, MODE_F32 = 0, 1, 2
 SRC_DEG_F16, SRC_DEG_F32, SRC_EDGE_F32, SRC_MAP = 0, 1, 2, 3
 LAYOUT_HMD, LAYOUT_MHD = 0, 1
+SPMM_ROWS_COVERED = 1
 
 _c = ctypes
 _vp, _i64, _int = _c.c_void_p, _c.c_int64, _c.c_int
@@ -26,11 +27,11 @@ _SIGNATURES = {
                                    _c.POINTER(_i64), _c.POINTER(_i64), _c.POINTER(_int)]),
     "attgnn_blocks_export": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "attgnn_blocks_free": (None, [_vp]),
-    "attgnn_spmm_block": (_int, [_vp, _vp, _vp, _int, _vp, _vp, _i64, _vp, _int, _i64, _vp, _i64, _i64, _int,
-                                 _int, _int, _vp, _vp, _vp, _i64, _vp]),
+    "attgnn_spmm_block": (_int, [_vp, _vp, _vp, _int, _vp, _vp, _i64, _vp, _int, _i64, _vp, _int, _i64, _i64,
+                                 _int, _int, _int, _vp, _vp, _vp, _i64, _int, _vp]),
     "attgnn_spmm_workspace_elems": (_i64, [_i64, _int, _i64]),
-    "attgnn_spmm_edge_map": (_int, [_vp, _vp, _vp, _vp, _int, _i64, _vp, _vp, _i64, _vp, _int, _i64, _i64, _vp,
-                                    _i64, _i64, _int, _int, _int, _int, _vp, _i64, _vp]),
+    "attgnn_spmm_edge_map": (_int, [_vp, _vp, _vp, _vp, _int, _i64, _vp, _vp, _i64, _vp, _int, _int, _i64, _i64,
+                                    _vp, _i64, _i64, _int, _int, _int, _int, _vp, _i64, _vp]),
     "attgnn_rowsum_edge_map": (_int, [_vp, _vp, _vp, _i64, _vp, _int, _i64, _i64, _vp, _int, _i64, _int, _int,
                                       _vp]),
     "attgnn_sddmm_block": (_int, [_vp, _vp, _vp, _vp, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _int, _i64, _i64,
@@ -38,6 +39,11 @@ _SIGNATURES = {
     "attgnn_node_scores": (_int, [_vp, _int, _i64, _i64, _vp, _int, _i64, _i64, _i64, _vp, _vp]),
     "attgnn_edge_from_nodes": (_int, [_vp, _vp, _vp, _vp, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _int,
                                       _i64, _i64, _vp, _int, _i64, _int, _int, _vp]),
+    "attgnn_edge_softmax_fwd": (_int, [_vp, _vp, _i64, _i64, _i64, _vp, _vp, _i64, _i64, _c.c_float, _c.c_float,
+                                       _c.c_uint64, _vp, _int, _vp, _vp]),
+    "attgnn_edge_softmax_bwd": (_int, [_vp, _vp, _i64, _i64, _i64, _vp, _vp, _i64, _i64, _vp, _vp, _int,
+                                       _c.c_float, _c.c_float, _c.c_uint64, _vp, _vp, _vp]),
+    "attgnn_gather_rowsum": (_int, [_vp, _vp, _i64, _i64, _i64, _vp, _vp, _vp]),
 }
 
 # Every symbol include/attgnn.h declares (checked by the CPU test suite).

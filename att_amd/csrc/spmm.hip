@@ -64,13 +64,49 @@ struct SpmmArgs {
     const int32_t *rp, *col, *win, *atomic;
     const uint8_t *val;     // tiles, or per-edge values [heads][val_head_stride] (mapped)
     const int32_t *map;     // mapped variant: CSR position per tile slot, -1 = none
-    const uint8_t *x;       // X rows: element (h, c, f) at x[h*x_head_stride + c*dim_n + f]
-    float *y, *partial;     // Y element (h, r, f) at y[h*y_head_stride + r*y_row_stride + f]
+    const uint8_t *x;       // X rows: element (h, c, f) at x[h*x_head_stride + c*x_row_stride + f]
+    void *y;                // Y element (h, r, f) at y[h*y_head_stride + r*y_row_stride + f]
+    float *partial;         // fp32 partial tiles of split windows
     const float *row_scale, *col_scale;
     int64_t parts, m_ori, dim_n;
-    int64_t x_head_stride, y_head_stride, y_row_stride, val_head_stride;
-    int slices, heads, window, wide_log2;
+    int64_t x_head_stride, x_row_stride, y_head_stride, y_row_stride, val_head_stride;
+    int slices, heads, window, wide_log2, y_type;
 };
+
+// Store n consecutive fp32 values as y_type (F32 / F16 / BF16).
+template <int N>
+__device__ __forceinline__ void store_row(void *base, int64_t idx, int y_type, const float (&v)[N])
+{
+    if (y_type == ATTGNN_F32) {
+        float *d = static_cast<float *>(base) + idx;
+        if constexpr (N % 4 == 0) {
+#pragma unroll
+            for (int j = 0; j < N; j += 4) *reinterpret_cast<f32x4 *>(d + j) = f32x4{v[j], v[j + 1], v[j + 2], v[j + 3]};
+        } else {
+#pragma unroll
+            for (int j = 0; j < N; ++j) d[j] = v[j];
+        }
+    } else {
+        uint16_t h[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+            h[j] = y_type == ATTGNN_F16 ? __builtin_bit_cast(uint16_t, _Float16(v[j])) : uint16_t(bf16_bits_rne(v[j]));
+        uint16_t *d = static_cast<uint16_t *>(base) + idx;
+        if constexpr (N % 8 == 0) {
+#pragma unroll
+            for (int j = 0; j < N; j += 8)
+                *reinterpret_cast<u32x4 *>(d + j) = u32x4{h[j] | uint32_t(h[j + 1]) << 16, h[j + 2] | uint32_t(h[j + 3]) << 16,
+                                                          h[j + 4] | uint32_t(h[j + 5]) << 16, h[j + 6] | uint32_t(h[j + 7]) << 16};
+        } else if constexpr (N % 4 == 0) {
+#pragma unroll
+            for (int j = 0; j < N; j += 4)
+                *reinterpret_cast<u32x2 *>(d + j) = u32x2{h[j] | uint32_t(h[j + 1]) << 16, h[j + 2] | uint32_t(h[j + 3]) << 16};
+        } else {
+#pragma unroll
+            for (int j = 0; j < N; ++j) d[j] = h[j];
+        }
+    }
+}
 
 // One wave per (part, head, slice).  256-thread workgroups hold 4 tasks.
 // MAPPED: tile slot values are val[head][map[slot]] (edge-map SpMM).
@@ -154,7 +190,7 @@ __global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
             for (int w = 0; w < XW; ++w) xv[i][w] = 0;
             const int k = kg + i;
             const int c = k < nreal ? a.col[c_begin + k] : -1;
-            if (c >= 0 && f_ok) load_bytes<XB>(xh + (int64_t(c) * a.dim_n + f0) * EB, xv[i]);
+            if (c >= 0 && f_ok) load_bytes<XB>(xh + (int64_t(c) * a.x_row_stride + f0) * EB, xv[i]);
         }
 
         // ---- MFMA, one per N tile (feature offset j within the lane's T)
@@ -212,20 +248,18 @@ __global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
 #pragma unroll
             for (int j = 0; j < T; ++j) v[j] *= rs * a.col_scale[f0 + j];
         }
-        float *dst;
         if (split) {
-            dst = a.partial + ((part * a.heads + head) * W + r) * a.dim_n + f0;
+            float *dst = a.partial + ((part * a.heads + head) * W + r) * a.dim_n + f0;
+            if constexpr (T % 4 == 0) {
+#pragma unroll
+                for (int j = 0; j < T; j += 4) *reinterpret_cast<f32x4 *>(dst + j) = f32x4{v[j], v[j + 1], v[j + 2], v[j + 3]};
+            } else {
+#pragma unroll
+                for (int j = 0; j < T; ++j) dst[j] = v[j];
+            }
         } else {
             if (row >= a.m_ori) break;
-            dst = a.y + int64_t(head) * a.y_head_stride + row * a.y_row_stride + f0;
-        }
-        if constexpr (T % 4 == 0) {
-#pragma unroll
-            for (int j = 0; j < T; j += 4)
-                *reinterpret_cast<f32x4 *>(dst + j) = f32x4{v[j], v[j + 1], v[j + 2], v[j + 3]};
-        } else {
-#pragma unroll
-            for (int j = 0; j < T; ++j) dst[j] = v[j];
+            store_row<T>(a.y, int64_t(head) * a.y_head_stride + row * a.y_row_stride + f0, a.y_type, v);
         }
     }
 }
@@ -245,18 +279,18 @@ __global__ __launch_bounds__(256) void spmm_combine_kernel(SpmmArgs a)
     const int64_t n = rows * a.dim_n;
     for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
         const int64_t r = e / a.dim_n, f = e - r * a.dim_n;
-        float s = 0.f;
-        for (int64_t q = p; q <= last; ++q) s += a.partial[((q * a.heads + head) * a.window + r) * a.dim_n + f];
-        a.y[int64_t(head) * a.y_head_stride + (int64_t(w) * a.window + r) * a.y_row_stride + f] = s;
+        float s[1] = {0.f};
+        for (int64_t q = p; q <= last; ++q) s[0] += a.partial[((q * a.heads + head) * a.window + r) * a.dim_n + f];
+        store_row<1>(a.y, int64_t(head) * a.y_head_stride + (int64_t(w) * a.window + r) * a.y_row_stride + f, a.y_type, s);
     }
 }
 
-__global__ __launch_bounds__(256) void fill_zero_kernel(float4 *p, int64_t n4, float *tail, int64_t ntail)
+__global__ __launch_bounds__(256) void fill_zero_kernel(u32x4 *p, int64_t n4, uint8_t *tail, int64_t ntail)
 {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const int64_t stride = int64_t(gridDim.x) * blockDim.x;
-    for (int64_t k = i; k < n4; k += stride) p[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int64_t k = i; k < ntail; k += stride) tail[k] = 0.f;
+    for (int64_t k = i; k < n4; k += stride) p[k] = u32x4{0u, 0u, 0u, 0u};
+    for (int64_t k = i; k < ntail; k += stride) tail[k] = 0;
 }
 
 int kind_of(int mode, int tile_type, int x_type)
@@ -303,16 +337,21 @@ hipError_t launch_main(int t, const SpmmArgs &a, hipStream_t s)
     return hipGetLastError();
 }
 
-// Zero Y (rows no part stores stay zero), run the main kernel, then combine
-// split windows.  Y is one contiguous buffer of heads * m_ori * dim_n floats.
-hipError_t run_spmm(int kind, bool mapped, int t, const SpmmArgs &a, hipStream_t s)
+int y_bytes(int y_type) { return y_type == ATTGNN_F32 ? 4 : 2; }
+
+// Zero Y unless the caller guarantees every row is stored, run the main
+// kernel, then combine split windows.  Y is one contiguous buffer of
+// heads * m_ori * dim_n elements of y_type.
+hipError_t run_spmm(int kind, bool mapped, int t, const SpmmArgs &a, bool covered, hipStream_t s)
 {
-    const int64_t ny = int64_t(a.heads) * a.m_ori * a.dim_n;
-    if (ny > 0) {
-        const int64_t n4 = reinterpret_cast<uintptr_t>(a.y) % 16 == 0 ? ny / 4 : 0;
-        const int64_t blocks = std::min<int64_t>((std::max<int64_t>(n4, ny - 4 * n4) + 255) / 256, 65536);
+    const int64_t nbytes = int64_t(a.heads) * a.m_ori * a.dim_n * y_bytes(a.y_type);
+    if (nbytes > 0 && !covered) {
+        uint8_t *yb = static_cast<uint8_t *>(a.y);
+        const int64_t n4 = reinterpret_cast<uintptr_t>(yb) % 16 == 0 ? nbytes / 16 : 0;
+        const int64_t tail = nbytes - 16 * n4;
+        const int64_t blocks = std::min<int64_t>((std::max<int64_t>(n4, tail) + 255) / 256, 65536);
         hipLaunchKernelGGL(fill_zero_kernel, dim3(unsigned(blocks)), dim3(256), 0, s,
-                           reinterpret_cast<float4 *>(a.y), n4, a.y + 4 * n4, ny - 4 * n4);
+                           reinterpret_cast<u32x4 *>(yb), n4, yb + 16 * n4, tail);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -350,10 +389,10 @@ int elt_bytes(int kind) { return kind == K_FP8 ? 1 : (kind == K_F16 || kind == K
 
 extern "C" int attgnn_spmm_block(const int32_t *rp, const int32_t *col, const void *val, int tile_type,
                                  const int32_t *window_of_part, const int32_t *atomic, int64_t parts,
-                                 const void *X, int x_type, int64_t n_x, float *Y, int64_t m_ori,
+                                 const void *X, int x_type, int64_t n_x, void *Y, int y_type, int64_t m_ori,
                                  int64_t dim_n, int window, int wide, int mode, const float *row_scale,
                                  const float *col_scale, float *workspace, int64_t workspace_elems,
-                                 attgnn_stream_t stream)
+                                 int flags, attgnn_stream_t stream)
 {
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     if (parts < 0 || m_ori < 0 || dim_n <= 0 || n_x < 0) return ATTGNN_E_ARG;
@@ -364,13 +403,14 @@ extern "C" int attgnn_spmm_block(const int32_t *rp, const int32_t *col, const vo
     if (kind < 0) return ATTGNN_E_UNSUPPORTED;
     if (kind == K_FP8 && (!row_scale || !col_scale)) return ATTGNN_E_ARG;
     if (!Y) return ATTGNN_E_ARG;
+    if (!(y_type == ATTGNN_F32 || y_type == ATTGNN_F16 || y_type == ATTGNN_BF16)) return ATTGNN_E_ARG;
     if (parts > 0 && (!rp || !col || !val || !window_of_part || !atomic || !X || !workspace)) return ATTGNN_E_ARG;
     if (parts > 0 && workspace_elems < attgnn_spmm_workspace_elems(parts, window, dim_n)) return ATTGNN_E_ARG;
 
     const int eb = elt_bytes(kind);
     const int t = pick_t(dim_n, eb);
     if (reinterpret_cast<uintptr_t>(X) % (t * eb) != 0) return ATTGNN_E_ARG;
-    if (reinterpret_cast<uintptr_t>(Y) % 4 != 0) return ATTGNN_E_ARG;
+    if (reinterpret_cast<uintptr_t>(Y) % 16 != 0) return ATTGNN_E_ARG;
 
     SpmmArgs a{};
     a.rp = rp; a.col = col; a.win = window_of_part; a.atomic = atomic;
@@ -379,27 +419,27 @@ extern "C" int attgnn_spmm_block(const int32_t *rp, const int32_t *col, const vo
     a.y = Y; a.partial = workspace;
     a.row_scale = row_scale; a.col_scale = col_scale;
     a.parts = parts; a.m_ori = m_ori; a.dim_n = dim_n;
-    a.x_head_stride = 0; a.y_head_stride = 0; a.y_row_stride = dim_n; a.val_head_stride = 0;
+    a.x_head_stride = 0; a.x_row_stride = dim_n; a.y_head_stride = 0; a.y_row_stride = dim_n;
+    a.val_head_stride = 0;
     a.slices = int((dim_n + 16 * t - 1) / (16 * t));
-    a.heads = 1; a.window = window; a.wide_log2 = wl;
-    // T*16-byte vector stores need 16-byte aligned rows
-    if (t >= 4 && (reinterpret_cast<uintptr_t>(Y) % 16 != 0)) return ATTGNN_E_ARG;
-    const hipError_t e = run_spmm(kind, false, t, a, s);
+    a.heads = 1; a.window = window; a.wide_log2 = wl; a.y_type = y_type;
+    const hipError_t e = run_spmm(kind, false, t, a, (flags & ATTGNN_SPMM_ROWS_COVERED) != 0, s);
     return e == hipSuccess ? ATTGNN_OK : int(e);
 }
 
 extern "C" int attgnn_spmm_edge_map(const int32_t *rp, const int32_t *col, const int32_t *map,
                                     const void *edge_val, int val_type, int64_t nnz,
                                     const int32_t *window_of_part, const int32_t *atomic, int64_t parts,
-                                    const void *X, int x_type, int64_t heads, int64_t n_x, float *Y,
-                                    int64_t m_ori, int64_t dim_n, int out_layout, int window, int wide,
-                                    int mode, float *workspace, int64_t workspace_elems,
+                                    const void *X, int x_type, int x_layout, int64_t heads, int64_t n_x,
+                                    float *Y, int64_t m_ori, int64_t dim_n, int out_layout, int window,
+                                    int wide, int mode, float *workspace, int64_t workspace_elems,
                                     attgnn_stream_t stream)
 {
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     if (parts < 0 || m_ori < 0 || dim_n <= 0 || n_x < 0 || heads <= 0 || nnz < 0) return ATTGNN_E_ARG;
     if (!(window == 8 || window == 16)) return ATTGNN_E_ARG;
     if (!(out_layout == ATTGNN_LAYOUT_HMD || out_layout == ATTGNN_LAYOUT_MHD)) return ATTGNN_E_ARG;
+    if (!(x_layout == ATTGNN_LAYOUT_HMD || x_layout == ATTGNN_LAYOUT_MHD)) return ATTGNN_E_ARG;
     const int wl = wide_log2_of(wide);
     if (wl < 0) return ATTGNN_E_ARG;
     const int kind = kind_of(mode, val_type, x_type);
@@ -419,13 +459,13 @@ extern "C" int attgnn_spmm_edge_map(const int32_t *rp, const int32_t *col, const
     a.x = static_cast<const uint8_t *>(X);
     a.y = Y; a.partial = workspace;
     a.parts = parts; a.m_ori = m_ori; a.dim_n = dim_n;
-    a.x_head_stride = n_x * dim_n;
+    if (x_layout == ATTGNN_LAYOUT_HMD) { a.x_head_stride = n_x * dim_n; a.x_row_stride = dim_n; }
+    else { a.x_head_stride = dim_n; a.x_row_stride = heads * dim_n; }
     a.val_head_stride = nnz;
     if (out_layout == ATTGNN_LAYOUT_HMD) { a.y_head_stride = m_ori * dim_n; a.y_row_stride = dim_n; }
     else { a.y_head_stride = dim_n; a.y_row_stride = heads * dim_n; }
     a.slices = int((dim_n + 16 * t - 1) / (16 * t));
-    a.heads = int(heads); a.window = window; a.wide_log2 = wl;
-    if (t >= 4 && ((a.y_row_stride * 4) % 16 != 0 || (a.y_head_stride * 4) % 16 != 0)) return ATTGNN_E_ARG;
-    const hipError_t e = run_spmm(kind, true, t, a, s);
+    a.heads = int(heads); a.window = window; a.wide_log2 = wl; a.y_type = ATTGNN_F32;
+    const hipError_t e = run_spmm(kind, true, t, a, false, s);
     return e == hipSuccess ? ATTGNN_OK : int(e);
 }

@@ -1,0 +1,284 @@
+"""GCN / GAT / GATv2 layers on the window-block kernels, with hand-written
+backward passes behind torch.autograd.Function.
+
+Reference layers (end2end_*/**/mgat_csr/mgat_conv.py):
+  MGCNFunction / GCNConv          end2end_ori/gcn_final/mgat_csr/mgat_conv.py:27-108
+  GATConv_multi                   end2end_head/gat_final/mgat_csr/mgat_conv.py:154-199
+  GATv2Conv_multi                 end2end_head/gatv2_final/mgat_csr/mgat_conv.py:151-198
+                                  (logits as end2end_ori/gatv2_final/mgat_csr/mgat_conv.py:174-196)
+Graph containers mirror the reference datasets' attribute names
+(end2end_ori/gcn_final/mgat_csr/mdataset_fp16.py:39-64,
+ end2end_head/gat_final/mgat_csr/mdataset_fp16.py:35-62).
+
+Differences from the reference, all deliberate:
+  * gradients are exact (the reference's attention-vector backward pairs a0
+    with the column sums and adds cross terms, mgat_conv.py:42-67);
+  * the edge softmax subtracts the row max (same result, no overflow) and is
+    one fused kernel; attention probabilities are recomputed in the backward
+    pass from the per-row log-sum-exp instead of being kept per edge;
+  * dropout on attention uses a counter-based hash so the backward pass
+    regenerates the mask;
+  * the SpMM backward uses the transposed structure, so non-symmetric graphs
+    are handled (the reference reuses A, valid only for symmetric A).
+"""
+import math
+
+import numpy as np
+import torch
+import torch.nn as nn
+import torch.nn.functional as F
+
+from . import graphs as G
+from . import ops
+
+
+def default_part_size(total_blocks, lo=4, hi=128):
+    """Blocks per part: enough parts to keep ~2k waves busy, long parts otherwise."""
+    return int(min(max(total_blocks // 8192, lo), hi))
+
+
+def _blocks_estimate(csr, wide):
+    """Upper bound of the block count (every entry its own condensed column)."""
+    return max(csr.nnz // wide, 1)
+
+
+def transpose_csr(csr):
+    """CSR of A^T plus, for each of its entries, the position of the entry in A."""
+    rows = csr.row_ids().astype(np.int64)
+    cols = csr.indices.astype(np.int64)
+    order = np.lexsort((rows, cols))          # by column, then row
+    t_indptr = np.zeros(csr.n + 1, np.int64)
+    t_indptr[1:] = np.cumsum(np.bincount(cols, minlength=csr.n))
+    return G.CSR(t_indptr, rows[order].astype(np.int32), csr.n), order.astype(np.int64)
+
+
+class GCNGraph:
+    """Device-resident window-block structure of A_hat = D^-1/2 (A+I) D^-1/2.
+
+    ``csr`` already holds the self-loops; values are d_i d_j with
+    d = 1/sqrt(row count) (mdataset_fp16.py:54-56), rounded once to
+    ``tile_dtype``.  Rows are padded to a multiple of ``window`` so every row
+    lies in a window that has a part (no zero-fill before the SpMM).
+    """
+
+    def __init__(self, csr, device, window=16, wide=32, part_size=None, tile_dtype=torch.float16,
+                 symmetric=None, values=None):
+        self.window, self.wide = window, wide
+        self.num_nodes_ori = csr.n
+        padded = csr.padded(window)
+        self.num_nodes = padded.n
+        self.num_edges = csr.nnz
+        if values is None:
+            d = G.gcn_norm_degree(padded)
+            d[~np.isfinite(d)] = 0.0
+            values = G.gcn_norm_values(padded, d)
+        self.part_size = part_size or default_part_size(_blocks_estimate(padded, wide))
+        self.tile_dtype = tile_dtype
+        self.fwd = self._build(padded, values, device)
+        if symmetric is None:
+            symmetric = bool(np.array_equal(np.sort(csr.transpose_perm()), np.arange(csr.nnz)))
+        self.symmetric = symmetric
+        if symmetric:
+            self.bwd = self.fwd
+        else:
+            t, order = transpose_csr(padded)
+            self.bwd = self._build(t, np.asarray(values)[order], device)
+        # reference attribute names
+        (self.row_pointers, self.column_index, self.degrees, self.t_window_rowTensor,
+         self.t_atomicTensor) = self.fwd["arrays"]
+
+    def _build(self, csr, values, device):
+        b = ops.blocks_build(csr.rowptr32(), csr.indices, "edge_f32", np.asarray(values, np.float32),
+                             self.tile_dtype, self.window, self.wide, self.part_size, emit_empty=True)
+        arrays = [b[k].to(device) for k in ("rp", "col", "val", "window", "atomic")]
+        self.n_unique = b["n_unique"]
+        return {"arrays": arrays, "parts": int(b["window"].numel()), "n_cols": int(b["col"].numel())}
+
+    def spmm(self, X, transpose=False, out_dtype=None):
+        s = self.bwd if transpose else self.fwd
+        rp, col, val, win, at = s["arrays"]
+        if X.shape[0] < self.num_nodes_ori:
+            raise ValueError("X has fewer rows than the graph")
+        return ops.spmm_block(rp, col, val, win, at, X.contiguous(), self.num_nodes_ori, self.window, self.wide,
+                              out_dtype=out_dtype or X.dtype, rows_covered=True)
+
+
+class MGCNFunction(torch.autograd.Function):
+    """Y = A_hat . (X W); backward dXW = A_hat^T dY, dX = dXW W^T, dW = X^T dXW
+    (end2end_ori/gcn_final/mgat_csr/mgat_conv.py:27-74)."""
+
+    @staticmethod
+    def forward(ctx, X, weights, inputInfo):
+        ctx.save_for_backward(X, weights)
+        ctx.inputInfo = inputInfo
+        X_prime = torch.mm(X, weights)
+        return inputInfo.spmm(X_prime)
+
+    @staticmethod
+    def backward(ctx, d_output):
+        X, weights = ctx.saved_tensors
+        g = ctx.inputInfo
+        d_input_prime = g.spmm(d_output.to(X.dtype).contiguous(), transpose=True)
+        d_input = torch.mm(d_input_prime, weights.t())
+        d_weights = torch.mm(X.t(), d_input_prime)
+        return d_input, d_weights, None
+
+
+class GCNConv(nn.Module):
+    """end2end_ori/gcn_final/mgat_csr/mgat_conv.py:82-108."""
+
+    def __init__(self, input_dim, output_dim, dtype=torch.float16):
+        super().__init__()
+        self.weights = nn.Parameter(torch.empty(input_dim, output_dim))
+        self.dtype = dtype
+        self.reset_parameters()
+
+    def reset_parameters(self):
+        stdv = 1.0 / math.sqrt(self.weights.size(1))
+        self.weights.data.uniform_(-stdv, stdv)
+
+    def forward(self, X, inputInfo):
+        return MGCNFunction.apply(X, self.weights.to(self.dtype), inputInfo)
+
+
+# ----------------------------------------------------------------------------
+# attention layers
+# ----------------------------------------------------------------------------
+
+class GATGraph:
+    """Device-resident structures for the attention layers: CSR rows (fused
+    edge softmax), window blocks with edge-id maps of A (aggregation, SDDMM)
+    and of A^T (aggregation backward), and the transpose permutation (column
+    sums).  Attribute names follow the reference dataset
+    (end2end_head/gat_final/mgat_csr/mdataset_fp16.py:53-65)."""
+
+    def __init__(self, csr, device, window=16, wide=32, part_size=None):
+        self.window, self.wide = window, wide
+        self.num_nodes_ori = csr.n
+        padded = csr.padded(window)
+        self.num_nodes = padded.n
+        self.num_edges = csr.nnz
+        self.part_size = part_size or default_part_size(_blocks_estimate(padded, wide))
+        self.rowptr = torch.from_numpy(csr.rowptr32()).to(device)
+        self.col = torch.from_numpy(csr.indices).to(device)
+        b = ops.blocks_build(padded.rowptr32(), padded.indices, "map", None, torch.int32, window, wide,
+                             self.part_size, emit_empty=True)
+        self.n_unique = b["n_unique"]
+        self.fwd = [b[k].to(device) for k in ("rp", "col", "val", "window", "atomic")]
+        t, order = transpose_csr(padded)
+        bt = ops.blocks_build(t.rowptr32(), t.indices, "map", None, torch.int32, window, wide, self.part_size,
+                              emit_empty=True)
+        mt = bt["val"].numpy()
+        mt = np.where(mt >= 0, order[np.maximum(mt, 0)], -1).astype(np.int32)
+        self.bwd = [bt["rp"].to(device), bt["col"].to(device), torch.from_numpy(mt).to(device),
+                    bt["window"].to(device), bt["atomic"].to(device)]
+        self.t_rowptr = torch.from_numpy(t.rowptr32()[: csr.n + 1].copy()).to(device)
+        self.t_perm = torch.from_numpy(order[: int(t.indptr[csr.n])]).to(device)
+        # reference names
+        self.row_pointers, self.column_index, self.degrees, self.window_t, self.automic = self.fwd
+        self.degrees_trans = self.bwd[2]
+
+
+class GATAttention(torch.autograd.Function):
+    """h[i,h,:] = sum_j softmax_j(LeakyReLU(a0_h.Xs[i,h] + a1_h.Xt[j,h])) Xt[j,h,:]
+    with dropout on the probabilities.  Xs/Xt [N, H, D]; a0/a1 [H, D].
+
+    GAT: Xs = Xt = X W, slope 0.2.  Reference GATv2: Xs = LReLU(X W0),
+    Xt = LReLU(X W1), a0 = a1 = a, slope 1 (no LeakyReLU on the logits).
+    """
+
+    @staticmethod
+    def forward(ctx, Xs, Xt, a0, a1, graph, slope, drop_p, seed):
+        g = graph
+        s0 = ops.node_scores(Xs, a0.to(Xs.dtype).contiguous(), "mhd")
+        s1 = ops.node_scores(Xt, a1.to(Xt.dtype).contiguous(), "mhd")
+        p, lse = ops.edge_softmax_fwd(g.rowptr, g.col, s0, s1, slope, drop_p, seed, p_dtype=Xt.dtype)
+        rp, col, emap, win, at = g.fwd
+        h = ops.spmm_edge_map(rp, col, emap, p, win, at, Xt, "mhd", g.window, g.wide, m_ori=g.num_nodes_ori,
+                              x_layout="mhd")
+        ctx.save_for_backward(Xs, Xt, a0, a1, s0, s1, lse, p)
+        ctx.graph, ctx.slope, ctx.drop_p, ctx.seed = g, slope, drop_p, seed
+        return h.to(Xt.dtype)
+
+    @staticmethod
+    def backward(ctx, dh):
+        Xs, Xt, a0, a1, s0, s1, lse, p = ctx.saved_tensors
+        g = ctx.graph
+        n, heads, dim = Xt.shape
+        dh = dh.to(Xt.dtype).contiguous()
+        # aggregation: dXt[j] = sum_i p[(i,j)] dh[i]  (A^T edge-map SpMM)
+        rp, col, emap_t, win, at = g.bwd
+        dXt = ops.spmm_edge_map(rp, col, emap_t, p, win, at, dh, "mhd", g.window, g.wide, m_ori=n,
+                                x_layout="mhd")
+        # dp[(i,j)] = <dh[i], Xt[j]>  (SDDMM over A)
+        frp, fcol, emap, fwin, _ = g.fwd
+        dp = ops.sddmm_block(frp, fcol, emap, fwin, dh, Xt, g.num_edges, g.window, g.wide, layout="mhd",
+                             out_dtype=torch.float32)
+        dz, ds0 = ops.edge_softmax_bwd(g.rowptr, g.col, s0, s1, lse, dp, ctx.slope, ctx.drop_p, ctx.seed)
+        ds1 = ops.gather_rowsum(g.t_rowptr, g.t_perm, dz, n)
+        Xs32, Xt32 = Xs.float(), Xt.float()
+        # s0 = Xs.a0, s1 = Xt.a1
+        da0 = (ds0.unsqueeze(1) @ Xs32.permute(1, 0, 2).contiguous()).squeeze(1)   # [H, D]
+        da1 = (ds1.unsqueeze(1) @ Xt32.permute(1, 0, 2).contiguous()).squeeze(1)
+        dXs = ds0.t().unsqueeze(-1) * a0.float().unsqueeze(0)
+        dXt = dXt + ds1.t().unsqueeze(-1) * a1.float().unsqueeze(0)
+        return (dXs.to(Xs.dtype), dXt.to(Xt.dtype), da0.to(a0.dtype).reshape(a0.shape),
+                da1.to(a1.dtype).reshape(a1.shape), None, None, None, None)
+
+
+class GATConv_multi(nn.Module):
+    """end2end_head/gat_final/mgat_csr/mgat_conv.py:154-199 (forward returns [N, H, D])."""
+
+    def __init__(self, input_dim, output_dim, dropout, alpha, head=1, dtype=torch.bfloat16):
+        super().__init__()
+        self.alpha, self.head, self.output_dim, self.dtype = alpha, head, output_dim, dtype
+        gain = nn.init.calculate_gain("relu")
+        self.fc = nn.Linear(input_dim, output_dim * head, bias=False)
+        nn.init.xavier_normal_(self.fc.weight.data, gain=gain)
+        self.a0 = nn.Parameter(torch.zeros(head, 1, output_dim))
+        nn.init.xavier_normal_(self.a0.data, gain=gain)
+        self.a1 = nn.Parameter(torch.zeros(head, 1, output_dim))
+        nn.init.xavier_normal_(self.a1.data, gain=gain)
+        self.dropout = dropout
+        self._step = 0
+
+    def forward(self, X, inputInfo, seed=None):
+        Xp = F.linear(X.to(self.dtype), self.fc.weight.to(self.dtype)).view(-1, self.head, self.output_dim)
+        drop = self.dropout if self.training else 0.0
+        if seed is None:
+            seed = self._step
+            self._step += 1
+        return GATAttention.apply(Xp, Xp, self.a0.view(self.head, -1), self.a1.view(self.head, -1), inputInfo,
+                                  self.alpha, drop, seed)
+
+
+class GATv2Conv_multi(nn.Module):
+    """end2end_head/gatv2_final/mgat_csr/mgat_conv.py:151-198 with the single-head
+    file's logits (end2end_ori/gatv2_final/mgat_csr/mgat_conv.py:174-196):
+    X0 = LReLU(X W0), X1 = LReLU(X W1), logit = a.X0[i] + a.X1[j], h = sum p X1."""
+
+    def __init__(self, input_dim, output_dim, dropout, alpha, head=1, dtype=torch.bfloat16):
+        super().__init__()
+        self.alpha, self.head, self.output_dim, self.dtype = alpha, head, output_dim, dtype
+        gain = nn.init.calculate_gain("relu")
+        self.fc0 = nn.Linear(input_dim, output_dim * head, bias=False)
+        nn.init.xavier_normal_(self.fc0.weight.data, gain=gain)
+        self.fc1 = nn.Linear(input_dim, output_dim * head, bias=False)
+        nn.init.xavier_normal_(self.fc1.weight.data, gain=gain)
+        self.a0 = nn.Parameter(torch.zeros(head, 1, output_dim))
+        nn.init.xavier_normal_(self.a0.data, gain=gain)
+        self.dropout = dropout
+        self._step = 0
+
+    def forward(self, X, inputInfo, seed=None):
+        Xd = X.to(self.dtype)
+        X0 = F.leaky_relu(F.linear(Xd, self.fc0.weight.to(self.dtype)), self.alpha).view(-1, self.head,
+                                                                                         self.output_dim)
+        X1 = F.leaky_relu(F.linear(Xd, self.fc1.weight.to(self.dtype)), self.alpha).view(-1, self.head,
+                                                                                         self.output_dim)
+        drop = self.dropout if self.training else 0.0
+        if seed is None:
+            seed = self._step
+            self._step += 1
+        a = self.a0.view(self.head, -1)
+        return GATAttention.apply(X0, X1, a, a, inputInfo, 1.0, drop, seed)

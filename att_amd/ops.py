@@ -24,6 +24,13 @@ _DTYPE_OF = {v: k for k, v in _TYPE_OF.items()}
 _MODES = {"native": _lib.MODE_NATIVE, "bf16x3": _lib.MODE_BF16X3, "f32": _lib.MODE_F32}
 
 
+def _mode(mode, dtype):
+    """Arithmetic mode: MFMA in the element type for 16/8-bit data, bf16x3 for fp32."""
+    if mode is None:
+        mode = "bf16x3" if dtype == torch.float32 else "native"
+    return _MODES[mode]
+
+
 def _ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None and t.numel() > 0 else ctypes.c_void_p(0)
 
@@ -120,9 +127,15 @@ def blocks_build(row, col, src, src_values, out_dtype, window, wide, part_size,
 # SpMM
 # --------------------------------------------------------------------------
 
-def spmm_block(rp, col, val, window_of_part, atomic, X, m_ori, window, wide, mode="native",
-               row_scale=None, col_scale=None, out=None):
-    """Y[m_ori, D] (fp32) = A . X over a window-block structure (all on the GPU)."""
+def spmm_block(rp, col, val, window_of_part, atomic, X, m_ori, window, wide, mode=None,
+               row_scale=None, col_scale=None, out=None, out_dtype=torch.float32, rows_covered=False,
+               workspace=None):
+    """Y[m_ori, D] = A . X over a window-block structure (all on the GPU).
+
+    FP32 accumulation; Y is written in ``out_dtype`` (fp32 / fp16 / bf16).
+    ``rows_covered``: the structure has a part for every window up to m_ori,
+    so Y needs no zero-fill.
+    """
     dev = _need_gpu("spmm_block", rp, col, val, window_of_part, atomic, X, row_scale, col_scale, out)
     _need_i32("spmm_block", rp, col, window_of_part, atomic)
     if X.dim() != 2:
@@ -131,27 +144,35 @@ def spmm_block(rp, col, val, window_of_part, atomic, X, m_ori, window, wide, mod
         raise ValueError("rp / window / atomic sizes disagree")
     parts = window_of_part.numel()
     n_x, dim = X.shape
-    y = out if out is not None else torch.empty((m_ori, dim), dtype=torch.float32, device=dev)
-    if y.shape != (m_ori, dim) or y.dtype != torch.float32:
-        raise ValueError("out must be float32 [m_ori, dim]")
+    y = out if out is not None else torch.empty((m_ori, dim), dtype=out_dtype, device=dev)
+    if y.shape != (m_ori, dim) or y.dtype not in (torch.float32, torch.float16, torch.bfloat16):
+        raise ValueError("out must be [m_ori, dim] of float32 / float16 / bfloat16")
     ws_n = lib().attgnn_spmm_workspace_elems(parts, window, dim) if parts else 0
-    ws = torch.empty(max(ws_n, 1), dtype=torch.float32, device=dev)
+    if workspace is not None and workspace.numel() >= ws_n and workspace.dtype == torch.float32:
+        ws = workspace
+    else:
+        ws = torch.empty(max(ws_n, 1), dtype=torch.float32, device=dev)
+    flags = _lib.SPMM_ROWS_COVERED if rows_covered else 0
     rc = lib().attgnn_spmm_block(_ptr(rp), _ptr(col), _ptr(val), _TYPE_OF[val.dtype], _ptr(window_of_part),
-                                 _ptr(atomic), parts, _ptr(X), _TYPE_OF[X.dtype], n_x, _ptr(y), m_ori, dim,
-                                 window, wide, _MODES[mode], _ptr(row_scale), _ptr(col_scale), _ptr(ws), ws_n,
-                                 _stream(dev))
+                                 _ptr(atomic), parts, _ptr(X), _TYPE_OF[X.dtype], n_x, _ptr(y), _TYPE_OF[y.dtype],
+                                 m_ori, dim, window, wide, _mode(mode, X.dtype), _ptr(row_scale), _ptr(col_scale), _ptr(ws),
+                                 ws_n, flags, _stream(dev))
     check(rc, "attgnn_spmm_block")
     return y
 
 
 def spmm_edge_map(rp, col, emap, edge_val, window_of_part, atomic, X, out_layout, window, wide,
-                  m_ori=None, mode="native"):
-    """Edge-map SpMM.  X [H, N, D]; edge_val [H, nnz] (or [nnz]); Y fp32 [H, M, D] or [M, H, D]."""
+                  m_ori=None, mode=None, x_layout="hmd"):
+    """Edge-map SpMM.  X [H, N, D] ("hmd") or [N, H, D] ("mhd"); edge_val [H, nnz]
+    (or [nnz]); Y fp32 [H, M, D] ("hmd") or [M, H, D] ("mhd")."""
     dev = _need_gpu("spmm_edge_map", rp, col, emap, edge_val, window_of_part, atomic, X)
     _need_i32("spmm_edge_map", rp, col, emap, window_of_part, atomic)
     if X.dim() == 2:
-        X = X.unsqueeze(0)
-    heads, n_x, dim = X.shape
+        X = X.unsqueeze(0 if x_layout == "hmd" else 1)
+    if x_layout == "hmd":
+        heads, n_x, dim = X.shape
+    else:
+        n_x, heads, dim = X.shape
     ev = edge_val.reshape(heads, -1) if edge_val.dim() != 2 else edge_val
     if ev.shape[0] != heads:
         raise ValueError(f"edge values have {ev.shape[0]} heads, X has {heads}")
@@ -166,9 +187,10 @@ def spmm_edge_map(rp, col, emap, edge_val, window_of_part, atomic, X, out_layout
     if ev.dtype != X.dtype:
         ev = ev.to(X.dtype)
     ev = ev.contiguous()
+    xl = {"hmd": _lib.LAYOUT_HMD, "mhd": _lib.LAYOUT_MHD}[x_layout]
     rc = lib().attgnn_spmm_edge_map(_ptr(rp), _ptr(col), _ptr(emap), _ptr(ev), _TYPE_OF[ev.dtype], nnz,
-                                    _ptr(window_of_part), _ptr(atomic), parts, _ptr(X), _TYPE_OF[X.dtype],
-                                    heads, n_x, _ptr(y), m, dim, layout, window, wide, _MODES[mode], _ptr(ws),
+                                    _ptr(window_of_part), _ptr(atomic), parts, _ptr(X), _TYPE_OF[X.dtype], xl,
+                                    heads, n_x, _ptr(y), m, dim, layout, window, wide, _mode(mode, X.dtype), _ptr(ws),
                                     ws_n, _stream(dev))
     check(rc, "attgnn_spmm_edge_map")
     return y
@@ -206,7 +228,7 @@ def _hmd_strides(X, layout):
 
 
 def sddmm_block(rp, col, emap, window_of_part, A, B, nnz, window, wide, layout="hmd",
-                out_dtype=None, mode="bf16x3"):
+                out_dtype=None, mode=None):
     """out[h, e=(i,j)] = <A_h[i], B_h[j]> in CSR edge order ([H, nnz])."""
     dev = _need_gpu("sddmm_block", rp, col, emap, window_of_part, A, B)
     _need_i32("sddmm_block", rp, col, emap, window_of_part)
@@ -221,7 +243,7 @@ def sddmm_block(rp, col, emap, window_of_part, A, B, nnz, window, wide, layout="
     rc = lib().attgnn_sddmm_block(_ptr(rp), _ptr(col), _ptr(emap), _ptr(window_of_part),
                                   window_of_part.numel(), _ptr(A), ahs, ars, _ptr(B), bhs, brs,
                                   _TYPE_OF[A.dtype], ha, na, da, _ptr(out), _TYPE_OF[out_dtype], nnz, window, wide,
-                                  _MODES[mode], _stream(dev))
+                                  _mode(mode, A.dtype), _stream(dev))
     check(rc, "attgnn_sddmm_block")
     return out
 
@@ -261,3 +283,49 @@ def edge_from_nodes(rp, col, emap, window_of_part, nnz, heads, n_rows, window, w
 
 def dtype_code(dtype):
     return _TYPE_OF[dtype]
+
+
+# --------------------------------------------------------------------------
+# fused edge softmax (CSR rows)
+# --------------------------------------------------------------------------
+
+def edge_softmax_fwd(rowptr, col, s0, s1, slope, drop_p=0.0, seed=0, p_dtype=torch.float32):
+    """p [H, nnz] and lse [H, N] of the fused LeakyReLU + softmax + dropout."""
+    dev = _need_gpu("edge_softmax_fwd", rowptr, col, s0, s1)
+    _need_i32("edge_softmax_fwd", rowptr, col)
+    heads, n = s0.shape
+    nnz = col.numel()
+    p = torch.empty((heads, nnz), dtype=p_dtype, device=dev)
+    lse = torch.empty((heads, n), dtype=torch.float32, device=dev)
+    rc = lib().attgnn_edge_softmax_fwd(_ptr(rowptr), _ptr(col), n, nnz, heads, _ptr(s0), _ptr(s1), n, 1,
+                                       float(slope), float(drop_p), int(seed) & (2**64 - 1), _ptr(p),
+                                       _TYPE_OF[p_dtype], _ptr(lse), _stream(dev))
+    check(rc, "attgnn_edge_softmax_fwd")
+    return p, lse
+
+
+def edge_softmax_bwd(rowptr, col, s0, s1, lse, dp, slope, drop_p=0.0, seed=0):
+    """dz [H, nnz] (fp32) and ds0 [H, N] (row sums of dz)."""
+    dev = _need_gpu("edge_softmax_bwd", rowptr, col, s0, s1, lse, dp)
+    _need_i32("edge_softmax_bwd", rowptr, col)
+    heads, n = s0.shape
+    nnz = col.numel()
+    dz = torch.empty((heads, nnz), dtype=torch.float32, device=dev)
+    ds0 = torch.empty((heads, n), dtype=torch.float32, device=dev)
+    rc = lib().attgnn_edge_softmax_bwd(_ptr(rowptr), _ptr(col), n, nnz, heads, _ptr(s0), _ptr(s1), n, 1,
+                                       _ptr(lse), _ptr(dp), _TYPE_OF[dp.dtype], float(slope), float(drop_p),
+                                       int(seed) & (2**64 - 1), _ptr(dz), _ptr(ds0), _stream(dev))
+    check(rc, "attgnn_edge_softmax_bwd")
+    return dz, ds0
+
+
+def gather_rowsum(rowptr, perm, vals, n):
+    """out[h, j] = sum_{e in row j} vals[h, perm[e]] (column sums via the transpose permutation)."""
+    dev = _need_gpu("gather_rowsum", rowptr, perm, vals)
+    heads, nnz = vals.shape
+    if perm.dtype != torch.int64:
+        raise TypeError("perm must be int64")
+    out = torch.empty((heads, n), dtype=torch.float32, device=dev)
+    rc = lib().attgnn_gather_rowsum(_ptr(rowptr), _ptr(perm), n, nnz, heads, _ptr(vals), _ptr(out), _stream(dev))
+    check(rc, "attgnn_gather_rowsum")
+    return out

@@ -111,7 +111,8 @@ void attgnn_blocks_free(attgnn_blocks *b);
 /* ====================================================================== */
 
 /*
- * Y[m_ori, dim_n] (fp32, row-major) = A . X, A given as window blocks.
+ * Y[m_ori, dim_n] (row-major, y_type F32/F16/BF16) = A . X, A given as window blocks;
+ * accumulation is FP32 in every mode, Y is rounded once at the store.
  *
  * Replaces SpMM_sr_bcrs.forward_fp16_gnn  (gcn/SpMM/src/benchmark.cpp:35-44,
  *                                          launcher spmmKernel.cu:1693-1727)
@@ -129,10 +130,17 @@ void attgnn_blocks_free(attgnn_blocks *b);
  */
 int attgnn_spmm_block(const int32_t *rp, const int32_t *col, const void *val, int tile_type,
                       const int32_t *window_of_part, const int32_t *atomic, int64_t parts,
-                      const void *X, int x_type, int64_t n_x, float *Y, int64_t m_ori,
+                      const void *X, int x_type, int64_t n_x, void *Y, int y_type, int64_t m_ori,
                       int64_t dim_n, int window, int wide, int mode, const float *row_scale,
                       const float *col_scale, float *workspace, int64_t workspace_elems,
-                      attgnn_stream_t stream);
+                      int flags, attgnn_stream_t stream);
+
+/* attgnn_spmm_block flags */
+enum {
+    ATTGNN_SPMM_ROWS_COVERED = 1, /* every row < m_ori lies in a window that has a
+                                     part (built with emit_empty=1 and m_ori <=
+                                     windows*window): Y is not pre-zeroed       */
+};
 
 int64_t attgnn_spmm_workspace_elems(int64_t parts, int window, int64_t dim_n);
 
@@ -147,17 +155,17 @@ int64_t attgnn_spmm_workspace_elems(int64_t parts, int window, int64_t dim_n);
  *          ATT_SpMM_v{1,3}.spmm_{fp16,tf32}_m_h  (mgat_conv.py:79-86;
  *            test_spmm_m_h.py:49-84)
  *
- * edge_val[heads][nnz] (F16/BF16/F32, matching x_type), X[heads][n_x][dim_n],
- * Y fp32 in out_layout (HMD: [heads][m_ori][dim_n], MHD: [m_ori][heads][dim_n]).
+ * edge_val[heads][nnz] (F16/BF16/F32, matching x_type), X in x_layout
+ * (HMD [heads][n_x][dim_n] or MHD [n_x][heads][dim_n]), Y fp32 in out_layout (HMD: [heads][m_ori][dim_n], MHD: [m_ori][heads][dim_n]).
  * mode: NATIVE for F16/BF16; BF16X3 or F32 for F32.  workspace: at least
  * heads * attgnn_spmm_workspace_elems(parts, window, dim_n) floats.
  */
 int attgnn_spmm_edge_map(const int32_t *rp, const int32_t *col, const int32_t *map,
                          const void *edge_val, int val_type, int64_t nnz,
                          const int32_t *window_of_part, const int32_t *atomic, int64_t parts,
-                         const void *X, int x_type, int64_t heads, int64_t n_x, float *Y,
-                         int64_t m_ori, int64_t dim_n, int out_layout, int window, int wide,
-                         int mode, float *workspace, int64_t workspace_elems,
+                         const void *X, int x_type, int x_layout, int64_t heads, int64_t n_x,
+                         float *Y, int64_t m_ori, int64_t dim_n, int out_layout, int window,
+                         int wide, int mode, float *workspace, int64_t workspace_elems,
                          attgnn_stream_t stream);
 
 /*
@@ -225,6 +233,53 @@ int attgnn_edge_from_nodes(const int32_t *rp, const int32_t *col, const int32_t 
                            int64_t s1_head_stride, int64_t s1_row_stride, const void *g0,
                            int g0_type, int64_t heads, int64_t n_rows, void *out, int out_type,
                            int64_t nnz, int window, int wide, attgnn_stream_t stream);
+
+/* ====================================================================== */
+/* Fused edge softmax over CSR rows (device)                               */
+/* ====================================================================== */
+
+/*
+ * Attention probabilities of one layer, fused:
+ *   z = s0[h,i] + s1[h,j];  v = LeakyReLU_slope(z);  p = exp(v - lse_i) with
+ *   lse_i = log sum_j exp(v) (max subtracted, FP32);  dropout with probability
+ *   drop_p from a counter-based hash of (seed, h, e), kept values scaled by
+ *   1/(1-drop_p).  Writes p[h*nnz + e] (p_type) and lse[h*n + i] (fp32).
+ * s0/s1 element (h, i) at s[h*s_head_stride + i*s_row_stride].
+ *
+ * Replaces the LeakyReLU / exp / rowsum / dropout / divide chain of
+ *   end2end_head/gat_final/mgat_csr/mgat_conv.py:180-197 (exp without max
+ *   subtraction there; the normalised result is the same) -- slope 0.2 for GAT,
+ *   slope 1 for the reference's GATv2 logits
+ *   (end2end_ori/gatv2_final/mgat_csr/mgat_conv.py:179-196).
+ */
+int attgnn_edge_softmax_fwd(const int32_t *rowptr, const int32_t *col, int64_t n, int64_t nnz,
+                            int64_t heads, const float *s0, const float *s1, int64_t s_head_stride,
+                            int64_t s_row_stride, float slope, float drop_p, uint64_t seed, void *p,
+                            int p_type, float *lse, attgnn_stream_t stream);
+
+/*
+ * Backward of attgnn_edge_softmax_fwd: p is recomputed from (s0, s1, lse) and
+ * the dropout mask regenerated from (seed, h, e); dp = dL/d(dropped p).
+ * Writes dz[h*nnz + e] = dL/dz and ds0[h*n + i] = sum_j dz[h, (i, j)].
+ *
+ * Replaces the backward of the same chain (mgat_conv.py:34-69, 91-151), with
+ * the exact gradient.
+ */
+int attgnn_edge_softmax_bwd(const int32_t *rowptr, const int32_t *col, int64_t n, int64_t nnz,
+                            int64_t heads, const float *s0, const float *s1, int64_t s_head_stride,
+                            int64_t s_row_stride, const float *lse, const void *dp, int dp_type,
+                            float slope, float drop_p, uint64_t seed, float *dz, float *ds0,
+                            attgnn_stream_t stream);
+
+/*
+ * out[h*n + j] = sum over CSR row j of vals[h*nnz + perm[e]] (perm[e] = -1 skips):
+ * with perm the CSR position of each entry's transpose this is a column sum.
+ *
+ * Replaces ATT_SpMM_v{1,3}.spmm_*_h_m_rowsum called with the transposed map
+ *   (end2end_head/gat_final/mgat_csr/mgat_conv.py:42-48).
+ */
+int attgnn_gather_rowsum(const int32_t *rowptr, const int64_t *perm, int64_t n, int64_t nnz,
+                         int64_t heads, const float *vals, float *out, attgnn_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -34,12 +34,12 @@ def test_spmm_rejects_bad_arguments_without_device():
     L = _lib.lib()
     nul = ctypes.c_void_p(0)
     # window 12 is not supported: rejected during host validation
-    rc = L.attgnn_spmm_block(nul, nul, nul, _lib.F16, nul, nul, 0, nul, _lib.F16, 0, nul, 0, 16, 12, 8,
-                             _lib.MODE_NATIVE, nul, nul, nul, 0, nul)
+    rc = L.attgnn_spmm_block(nul, nul, nul, _lib.F16, nul, nul, 0, nul, _lib.F16, 0, nul, _lib.F32, 0, 16, 12, 8,
+                             _lib.MODE_NATIVE, nul, nul, nul, 0, 0, nul)
     assert rc == -1
     # mismatched tile / X types
-    rc = L.attgnn_spmm_block(nul, nul, nul, _lib.F16, nul, nul, 0, nul, _lib.BF16, 0, nul, 0, 16, 8, 8,
-                             _lib.MODE_NATIVE, nul, nul, nul, 0, nul)
+    rc = L.attgnn_spmm_block(nul, nul, nul, _lib.F16, nul, nul, 0, nul, _lib.BF16, 0, nul, _lib.F32, 0, 16, 8, 8,
+                             _lib.MODE_NATIVE, nul, nul, nul, 0, 0, nul)
     assert rc == -4
     rc = L.attgnn_sddmm_block(nul, nul, nul, nul, 1, nul, 0, 0, nul, 0, 0, _lib.F16, 1, 1, 8, nul, _lib.F16, 1,
                               8, 7, 0, nul)
