@@ -1,0 +1,244 @@
+"""Benchmark: 2-layer GCN forward+backward on a 114.6M-nnz power-law graph.
+
+Workload (one GPU): Chung-Lu graph (gamma 2.1, seed 0) with N = 232,965 nodes
+and ~114.6M entries after symmetrisation and self-loops; A_hat = D^-1/2 (A+I)
+D^-1/2 stored as FP16 window tiles (16-row windows, 32-column blocks);
+X 232,965 x 602 FP16 ~ N(0,1); Glorot weights 602 -> 256 -> 41 (padded to 48);
+ReLU, dropout 0.5, log-softmax, NLL loss on random labels, Adam step.
+
+A step is one full training step.  Its hot path is four window-block SpMMs
+(feature widths 256 and 48, forward and backward); ``value`` is their
+algorithmic FLOPs (2 * nnz * width each) divided by the wall time of the whole
+step.  ``roofline`` covers the dominant kernel, the width-256 SpMM, timed with
+HIP events on the stream it runs on; its algorithmic bytes per launch are
+4(M+1) + nnz*(4+2) + U*256*2 + M*256*2 with U the number of distinct columns
+summed over 16-row windows.  ``cpu_baseline`` times torch.sparse.mm (CSR,
+FP32) on the host cores over a bounded row sample of the same A_hat.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+N_NODES = 232_965
+TARGET_NNZ = 114_600_000
+F_IN, F_HID, N_CLASS, F_OUT = 602, 256, 41, 48
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_graph(scale):
+    from att_amd import graphs as G
+    n = int(N_NODES * scale)
+    nnz = int(TARGET_NNZ * scale)
+    t = time.perf_counter()
+    g = G.chung_lu(n, (nnz - n) / n, 2.1, seed=0, target_nnz=nnz)
+    log(f"graph: N={g.n} nnz={g.nnz} max_degree={int(g.degrees().max())} ({time.perf_counter() - t:.1f}s)")
+    return g
+
+
+def cpu_baseline(g, vals, dim=256, budget_s=12.0):
+    """torch.sparse.mm (CSR FP32) on the host over a leading row sample."""
+    import torch
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    torch.set_num_threads(cores)
+    # sample rows so that one product is ~1/4 of the budget at ~5 GFLOP/s
+    target_nnz = int(budget_s / 4 * 5e9 / (2 * dim))
+    rows = int(np.searchsorted(g.indptr, min(target_nnz, g.nnz)))
+    rows = max(min(rows, g.n), 1)
+    e = int(g.indptr[rows])
+    A = torch.sparse_csr_tensor(torch.from_numpy(g.indptr[: rows + 1].copy()),
+                                torch.from_numpy(g.indices[:e].astype(np.int64)),
+                                torch.from_numpy(vals[:e].copy()), (rows, g.n))
+    X = torch.from_numpy(np.random.default_rng(5).uniform(-1, 1, (g.n, dim)).astype(np.float32))
+    torch.sparse.mm(A, X)  # warm-up
+    times = []
+    t_end = time.perf_counter() + budget_s
+    while time.perf_counter() < t_end and len(times) < 5:
+        t = time.perf_counter()
+        torch.sparse.mm(A, X)
+        times.append(time.perf_counter() - t)
+    tmed = float(np.median(times))
+    gflops = 2.0 * e * dim / tmed / 1e9
+    return {"value": round(gflops, 3), "unit": "GFLOP/s", "cores": cores, "kind": "port",
+            "sample": f"torch.sparse.mm CSR fp32 on the first {rows} rows ({e} nnz) of A_hat x X[{g.n},{dim}], "
+                      f"median of {len(times)}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--scale", type=float, default=1.0, help="graph size fraction (testing only)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--part-size", type=int, default=0)
+    ap.add_argument("--mode", choices=["train", "kernels"], default="train",
+                    help="kernels: only the width-256 SpMM and a 1 GiB copy (for counter runs)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import torch.nn as nn
+    import torch.nn.functional as F
+
+    from att_amd import graphs as G
+    from att_amd import ops
+    from att_amd.layers import GCNConv, GCNGraph
+
+    dev = torch.device("cuda", 0)
+    g = build_graph(args.scale)
+    t = time.perf_counter()
+    d = G.gcn_norm_degree(g)
+    vals = G.gcn_norm_values(g, d)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(g, vals)
+    if cpu:
+        log(f"cpu baseline: {cpu}")
+    graph = GCNGraph(g, dev, window=16, wide=32, part_size=args.part_size or None, tile_dtype=torch.float16,
+                     symmetric=True, values=np.concatenate([vals, np.zeros(0, np.float32)]))
+    log(f"window blocks: U={graph.n_unique} parts={graph.fwd['parts']} cols={graph.fwd['n_cols']} "
+        f"part_size={graph.part_size} ({time.perf_counter() - t:.1f}s)")
+
+    rng = torch.Generator().manual_seed(1)
+    if args.mode == "kernels":
+        # counter runs: 5 width-256 SpMMs and 5 copies of 1 GiB (the copy calibrates
+        # FETCH_SIZE / WRITE_SIZE against a kernel of known traffic)
+        Xs = torch.randn(g.n, F_HID, generator=rng).half().to(dev)
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            graph.spmm(Xs)
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("kernels done: 5 x spmm(width 256) and 5 x copy(1 GiB)")
+        return
+    X = torch.randn(g.n, F_IN, generator=rng).half().to(dev)
+    labels = torch.randint(0, N_CLASS, (g.n,), generator=torch.Generator().manual_seed(2)).to(dev)
+    torch.manual_seed(3)
+
+    class GCN2(nn.Module):
+        def __init__(self):
+            super().__init__()
+            self.conv1 = GCNConv(F_IN, F_HID)
+            self.conv2 = GCNConv(F_HID, F_OUT)
+            nn.init.xavier_uniform_(self.conv1.weights)
+            w2 = torch.empty(F_HID, N_CLASS)
+            nn.init.xavier_uniform_(w2)
+            with torch.no_grad():
+                self.conv2.weights.zero_()
+                self.conv2.weights[:, :N_CLASS] = w2
+
+        def forward(self, x):
+            h = F.relu(self.conv1(x, graph))
+            h = F.dropout(h, 0.5, self.training)
+            h = self.conv2(h, graph)
+            return F.log_softmax(h[:, :N_CLASS].float(), dim=1)
+
+    model = GCN2().to(dev)
+    opt = torch.optim.Adam(model.parameters(), lr=1e-2)
+
+    def step():
+        opt.zero_grad(set_to_none=True)
+        loss = F.nll_loss(model(X), labels)
+        loss.backward()
+        opt.step()
+        return loss
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        loss = step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    spmm_flops = 2.0 * g.nnz * (F_HID + F_OUT) * 2
+    value = spmm_flops / (ms * 1e-3) / 1e9
+    log(f"step {ms:.3f} ms, loss {loss.item():.4f}")
+
+    # ---- dominant kernel: width-256 fp16 SpMM, HIP events on its stream
+    Xs = torch.randn(g.n, F_HID, generator=rng).half().to(dev)
+    stream = torch.cuda.current_stream(dev)
+    for _ in range(3):
+        y = graph.spmm(Xs)
+    reps = 20
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(reps):
+        y = graph.spmm(Xs)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    t_spmm = ev0.elapsed_time(ev1) / reps  # ms
+    M = g.n
+    algo_bytes = 4 * (M + 1) + g.nnz * (4 + 2) + graph.n_unique * F_HID * 2 + M * F_HID * 2
+    achieved = algo_bytes / (t_spmm * 1e-3) / 1e9
+    # accuracy of that SpMM vs FP64 on a row sample (same fp16-rounded A_hat and X)
+    rows = np.arange(0, g.n, max(g.n // 2000, 1))
+    sub_ip = np.concatenate([[0], np.cumsum(np.diff(g.indptr)[rows])])
+    sub_ix = np.concatenate([g.indices[g.indptr[r]:g.indptr[r + 1]] for r in rows])
+    sub_v = np.concatenate([vals[g.indptr[r]:g.indptr[r + 1]] for r in rows]).astype(np.float16).astype(np.float64)
+    import scipy.sparse as sp
+    ref = sp.csr_matrix((sub_v, sub_ix, sub_ip), shape=(rows.size, g.n)) @ Xs.double().cpu().numpy()
+    got = y[torch.from_numpy(rows).to(dev)].double().cpu().numpy()
+    rel_err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+
+    # plain FP32 GPU SpMM (torch.sparse on the same A_hat) for comparison
+    A32 = torch.sparse_csr_tensor(torch.from_numpy(g.indptr), torch.from_numpy(g.indices.astype(np.int64)),
+                                  torch.from_numpy(vals), (g.n, g.n)).to(dev)
+    X32 = Xs.float()
+    torch.sparse.mm(A32, X32)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(3):
+        torch.sparse.mm(A32, X32)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    t_plain = ev0.elapsed_time(ev1) / 3
+    del A32, X32
+
+    out = {
+        "metric": "SpMM/SDDMM GFLOP/s & %HBM roofline; GNN req/s + p99 vs plain FP32 at stated err",
+        "value": round(value, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp16",
+        "data": "synthetic",
+        "config": {"workload": "2-layer GCN fwd+bwd (Adam), Chung-Lu gamma=2.1 seed 0",
+                   "nodes": g.n, "nnz": g.nnz, "features": [F_IN, F_HID, F_OUT], "window": 16, "wide": 32,
+                   "tiles": "fp16", "accumulate": "fp32", "scale": args.scale},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "window-block SpMM, width 256, fp16 (main + split-window combine)",
+                     "ms_per_launch": round(t_spmm, 4), "algorithmic_bytes": int(algo_bytes),
+                     "gflops": round(2.0 * g.nnz * F_HID / (t_spmm * 1e-3) / 1e9, 1)},
+        "cpu_baseline": cpu,
+        "accuracy": {"spmm256_rel_fro_vs_fp64": rel_err},
+        "plain_fp32_gpu": {"spmm256_ms": round(t_plain, 4),
+                           "gflops": round(2.0 * g.nnz * F_HID / (t_plain * 1e-3) / 1e9, 1),
+                           "speedup_window_block": round(t_plain / t_spmm, 3)},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
