@@ -70,7 +70,7 @@ struct SpmmArgs {
     const float *row_scale, *col_scale;
     int64_t parts, m_ori, dim_n;
     int64_t x_head_stride, x_row_stride, y_head_stride, y_row_stride, val_head_stride;
-    int slices, heads, window, wide_log2, y_type;
+    int slices, heads, window, wide_log2, y_type, pipe;
 };
 
 // Store n consecutive fp32 values as y_type (F32 / F16 / BF16).
@@ -108,92 +108,96 @@ __device__ __forceinline__ void store_row(void *base, int64_t idx, int y_type, c
     }
 }
 
-// One wave per (part, head, slice).  256-thread workgroups hold 4 tasks.
-// MAPPED: tile slot values are val[head][map[slot]] (edge-map SpMM).
-template <int KIND, int T, bool MAPPED>
-__global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
+template <int BYTES>
+__device__ __forceinline__ void load_bytes_nt(const uint8_t *p, uint32_t *d)
 {
-    constexpr int EB = KindTraits<KIND>::EB;
-    constexpr int AW = (8 * EB) / 4;                 // dwords of one A operand
-    constexpr int XB = T * EB;                       // bytes per X vector
-    constexpr int XW = XB >= 4 ? XB / 4 : 1;         // dwords per X vector
+    if constexpr (BYTES == 32) {
+        const u32x4 a = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p));
+        const u32x4 b = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p) + 1);
+        d[0] = a[0]; d[1] = a[1]; d[2] = a[2]; d[3] = a[3];
+        d[4] = b[0]; d[5] = b[1]; d[6] = b[2]; d[7] = b[3];
+    } else if constexpr (BYTES == 16) {
+        const u32x4 a = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p));
+        d[0] = a[0]; d[1] = a[1]; d[2] = a[2]; d[3] = a[3];
+    } else if constexpr (BYTES == 8) {
+        const u32x2 a = __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(p));
+        d[0] = a[0]; d[1] = a[1];
+    } else {
+        load_bytes<BYTES>(p, d);
+    }
+}
 
-    const int lane = threadIdx.x & 63;
-    const int64_t task = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
-    if (task >= a.parts * a.heads * a.slices) return;
-    const int64_t part = task / (int64_t(a.heads) * a.slices);
-    const int hs = int(task - part * a.heads * a.slices);
-    const int head = hs / a.slices, slice = hs - head * a.slices;
-    const uint8_t *xh = a.x + int64_t(head) * a.x_head_stride * EB;
+// Per-wave state and the three steps of one k-block (32 condensed columns):
+// column ids of the lane's 8 k slots, the A operand (tile row n), the X rows.
+template <int KIND, int T, bool MAPPED>
+struct KBlock {
+    static constexpr int EB = KindTraits<KIND>::EB;
+    static constexpr int AW = (8 * EB) / 4;
+    static constexpr int XB = T * EB;
+    static constexpr int XW = XB >= 4 ? XB / 4 : 1;
 
-    const int c_begin = a.rp[2 * part];
-    const int nreal = a.rp[2 * part + 1] - c_begin;
-    const int plen = a.rp[2 * part + 2] - c_begin;
-    const int g = lane >> 4, n = lane & 15;
-    const int64_t f0 = int64_t(slice) * 16 * T + int64_t(n) * T;
-    const bool f_ok = f0 < a.dim_n;
-    const int W = a.window;
-    const int wide = 1 << a.wide_log2;
-    const bool a_row_ok = n < W;
+    const SpmmArgs &a;
+    const uint8_t *xh, *vh;
+    int c_begin, nreal, plen, g, n, W, wide, head;
+    int64_t f0;
+    bool f_ok, a_row_ok;
 
-    f32x4 acc[T];
+    __device__ __forceinline__ void cols(int kb, int (&cid)[8]) const
+    {
+        const int kg = kb + 8 * g;
 #pragma unroll
-    for (int j = 0; j < T; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int i = 0; i < 8; ++i) cid[i] = kg + i < nreal ? __builtin_nontemporal_load(a.col + c_begin + kg + i) : -1;
+    }
 
-    for (int kb = 0; kb < nreal; kb += 32) {
-        const int kg = kb + 8 * g;  // first k slot of this lane's group
-
-        // ---- A operand: tile values of window row n, condensed columns kg..kg+7
-        uint32_t av[AW];
+    __device__ __forceinline__ void tile(int kb, uint32_t (&av)[AW]) const
+    {
+        const int kg = kb + 8 * g;
 #pragma unroll
         for (int w = 0; w < AW; ++w) av[w] = 0;
-        if (MAPPED) {
-            if (a_row_ok && kg < nreal) {
-                const uint8_t *vh = a.val + int64_t(head) * a.val_head_stride * EB;
+        if (!a_row_ok || kg >= nreal) return;
+        if constexpr (MAPPED) {
 #pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const int64_t cc = int64_t(c_begin) + kg + i;
-                    if (kg + i < plen) {
-                        const int m = a.map[(cc >> a.wide_log2) * W * wide + int64_t(n) * wide + (cc & (wide - 1))];
-                        if (m >= 0) {
-                            uint32_t e;
-                            load_bytes<EB>(vh + int64_t(m) * EB, &e);
-                            if constexpr (EB == 4) av[i] = e;
-                            else if constexpr (EB == 2) av[i >> 1] |= e << (16 * (i & 1));
-                            else av[i >> 2] |= e << (8 * (i & 3));
-                        }
+            for (int i = 0; i < 8; ++i) {
+                const int64_t cc = int64_t(c_begin) + kg + i;
+                if (kg + i < plen) {
+                    const int m = a.map[(cc >> a.wide_log2) * W * wide + int64_t(n) * wide + (cc & (wide - 1))];
+                    if (m >= 0) {
+                        uint32_t e;
+                        load_bytes<EB>(vh + int64_t(m) * EB, &e);
+                        if constexpr (EB == 4) av[i] = e;
+                        else if constexpr (EB == 2) av[i >> 1] |= e << (16 * (i & 1));
+                        else av[i >> 2] |= e << (8 * (i & 3));
                     }
                 }
             }
-        } else if (a_row_ok && kg < nreal) {
-            if (wide >= 8) {
-                const int64_t cc = int64_t(c_begin) + kg;
-                const int64_t off = (cc >> a.wide_log2) * W * wide + int64_t(n) * wide + (cc & (wide - 1));
-                load_bytes<8 * EB>(a.val + off * EB, av);
-            } else {  // wide == 4: two 4-column blocks
+        } else if (wide >= 8) {
+            const int64_t cc = int64_t(c_begin) + kg;
+            const int64_t off = (cc >> a.wide_log2) * W * wide + int64_t(n) * wide + (cc & (wide - 1));
+            load_bytes_nt<8 * EB>(a.val + off * EB, av);
+        } else {  // wide == 4: two 4-column blocks
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    if (kg + 4 * h < plen) {
-                        const int64_t cc = int64_t(c_begin) + kg + 4 * h;
-                        const int64_t off = (cc >> a.wide_log2) * W * wide + int64_t(n) * wide;
-                        load_bytes<4 * EB>(a.val + off * EB, av + h * (AW / 2));
-                    }
+            for (int h = 0; h < 2; ++h) {
+                if (kg + 4 * h < plen) {
+                    const int64_t cc = int64_t(c_begin) + kg + 4 * h;
+                    const int64_t off = (cc >> a.wide_log2) * W * wide + int64_t(n) * wide;
+                    load_bytes_nt<4 * EB>(a.val + off * EB, av + h * (AW / 2));
                 }
             }
         }
+    }
 
-        // ---- B rows: X[col[k]] for the lane's 8 k slots
-        uint32_t xv[8][XW];
+    __device__ __forceinline__ void rows(const int (&cid)[8], uint32_t (&xv)[8][XW]) const
+    {
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
 #pragma unroll
             for (int w = 0; w < XW; ++w) xv[i][w] = 0;
-            const int k = kg + i;
-            const int c = k < nreal ? a.col[c_begin + k] : -1;
-            if (c >= 0 && f_ok) load_bytes<XB>(xh + (int64_t(c) * a.x_row_stride + f0) * EB, xv[i]);
+            if (cid[i] >= 0 && f_ok) load_bytes<XB>(xh + (int64_t(cid[i]) * a.x_row_stride + f0) * EB, xv[i]);
         }
+    }
 
-        // ---- MFMA, one per N tile (feature offset j within the lane's T)
+    __device__ __forceinline__ static void mma(const uint32_t (&av)[AW], const uint32_t (&xv)[8][XW], f32x4 (&acc)[T])
+    {
         if constexpr (KIND == K_F16 || KIND == K_BF16) {
             const u32x4 A = {av[0], av[1], av[2], av[3]};
 #pragma unroll
@@ -230,9 +234,91 @@ __global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
             }
         }
     }
+};
+
+// One wave per (part, head, slice).  256-thread workgroups hold 4 tasks.
+// MAPPED: tile slot values are val[head][map[slot]] (edge-map SpMM).
+// PIPE 0: each k-block loads its column ids, then its rows, then multiplies.
+// PIPE 1: the next k-block's column ids load while this block multiplies.
+// PIPE 2: two k-blocks per iteration, the next pair's column ids prefetched.
+template <int KIND, int T, bool MAPPED, int PIPE>
+__global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
+{
+    using KB = KBlock<KIND, T, MAPPED>;
+    constexpr int EB = KB::EB, AW = KB::AW, XW = KB::XW;
+
+    const int lane = threadIdx.x & 63;
+    const int64_t task = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    if (task >= a.parts * a.heads * a.slices) return;
+    const int64_t part = task / (int64_t(a.heads) * a.slices);
+    const int hs = int(task - part * a.heads * a.slices);
+    const int head = hs / a.slices, slice = hs - head * a.slices;
+
+    KB k{a};
+    k.xh = a.x + int64_t(head) * a.x_head_stride * EB;
+    k.vh = a.val + int64_t(head) * a.val_head_stride * EB;
+    k.c_begin = a.rp[2 * part];
+    k.nreal = a.rp[2 * part + 1] - k.c_begin;
+    k.plen = a.rp[2 * part + 2] - k.c_begin;
+    k.g = lane >> 4;
+    k.n = lane & 15;
+    k.W = a.window;
+    k.wide = 1 << a.wide_log2;
+    k.head = head;
+    k.f0 = int64_t(slice) * 16 * T + int64_t(k.n) * T;
+    k.f_ok = k.f0 < a.dim_n;
+    k.a_row_ok = k.n < k.W;
+    const int g = k.g, W = k.W, nreal = k.nreal;
+    const int64_t f0 = k.f0;
+
+    f32x4 acc[T];
+#pragma unroll
+    for (int j = 0; j < T; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    if constexpr (PIPE == 0) {
+        for (int kb = 0; kb < nreal; kb += 32) {
+            int cid[8];
+            uint32_t av[AW], xv[8][XW];
+            k.cols(kb, cid);
+            k.tile(kb, av);
+            k.rows(cid, xv);
+            KB::mma(av, xv, acc);
+        }
+    } else if constexpr (PIPE == 1) {
+        int cid[8];
+        k.cols(0, cid);
+        for (int kb = 0; kb < nreal; kb += 32) {
+            uint32_t av[AW], xv[8][XW];
+            int nid[8];
+            k.tile(kb, av);
+            k.rows(cid, xv);
+            k.cols(kb + 32, nid);
+            KB::mma(av, xv, acc);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) cid[i] = nid[i];
+        }
+    } else {
+        int c0[8], c1[8];
+        k.cols(0, c0);
+        k.cols(32, c1);
+        for (int kb = 0; kb < nreal; kb += 64) {
+            uint32_t av0[AW], av1[AW], xv0[8][XW], xv1[8][XW];
+            int n0[8], n1[8];
+            k.tile(kb, av0);
+            k.tile(kb + 32, av1);
+            k.rows(c0, xv0);
+            k.rows(c1, xv1);
+            k.cols(kb + 64, n0);
+            k.cols(kb + 96, n1);
+            KB::mma(av0, xv0, acc);
+            KB::mma(av1, xv1, acc);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) { c0[i] = n0[i]; c1[i] = n1[i]; }
+        }
+    }
 
     // ---- store rows 4g+q of the window, features f0..f0+T-1
-    if (!f_ok) return;
+    if (!k.f_ok) return;
     const int64_t wrow = int64_t(a.win[part]) * W;
     const bool split = a.atomic[part] != 0;
 #pragma unroll
@@ -317,24 +403,34 @@ int pick_t(int64_t dim_n, int eb)
     return t;
 }
 
-template <int KIND, bool MAPPED>
-hipError_t launch_main(int t, const SpmmArgs &a, hipStream_t s)
+template <int KIND, bool MAPPED, int PIPE>
+hipError_t launch_pipe(int t, const SpmmArgs &a, hipStream_t s)
 {
     const int64_t tasks = a.parts * a.heads * a.slices;
     const dim3 grid(unsigned((tasks + 3) / 4)), block(256);
     switch (t) {
-    case 1: hipLaunchKernelGGL((spmm_block_kernel<KIND, 1, MAPPED>), grid, block, 0, s, a); break;
-    case 2: hipLaunchKernelGGL((spmm_block_kernel<KIND, 2, MAPPED>), grid, block, 0, s, a); break;
-    case 4: hipLaunchKernelGGL((spmm_block_kernel<KIND, 4, MAPPED>), grid, block, 0, s, a); break;
+    case 1: hipLaunchKernelGGL((spmm_block_kernel<KIND, 1, MAPPED, PIPE>), grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((spmm_block_kernel<KIND, 2, MAPPED, PIPE>), grid, block, 0, s, a); break;
+    case 4: hipLaunchKernelGGL((spmm_block_kernel<KIND, 4, MAPPED, PIPE>), grid, block, 0, s, a); break;
     case 8:
         if constexpr (KindTraits<KIND>::EB <= 2) {
-            hipLaunchKernelGGL((spmm_block_kernel<KIND, 8, MAPPED>), grid, block, 0, s, a);
+            hipLaunchKernelGGL((spmm_block_kernel<KIND, 8, MAPPED, PIPE>), grid, block, 0, s, a);
             break;
         }
         return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+template <int KIND, bool MAPPED>
+hipError_t launch_main(int t, const SpmmArgs &a, hipStream_t s)
+{
+    switch (a.pipe) {
+    case 0: return launch_pipe<KIND, MAPPED, 0>(t, a, s);
+    case 2: return launch_pipe<KIND, MAPPED, 2>(t, a, s);
+    default: return launch_pipe<KIND, MAPPED, 1>(t, a, s);
+    }
 }
 
 int y_bytes(int y_type) { return y_type == ATTGNN_F32 ? 4 : 2; }
@@ -423,6 +519,10 @@ extern "C" int attgnn_spmm_block(const int32_t *rp, const int32_t *col, const vo
     a.val_head_stride = 0;
     a.slices = int((dim_n + 16 * t - 1) / (16 * t));
     a.heads = 1; a.window = window; a.wide_log2 = wl; a.y_type = y_type;
+    {
+        const int sel = (flags >> 4) & 3;  // ATTGNN_SPMM_PIPE_*
+        a.pipe = sel == 3 ? 0 : sel == 2 ? 2 : 1;
+    }
     const hipError_t e = run_spmm(kind, false, t, a, (flags & ATTGNN_SPMM_ROWS_COVERED) != 0, s);
     return e == hipSuccess ? ATTGNN_OK : int(e);
 }
@@ -465,7 +565,7 @@ extern "C" int attgnn_spmm_edge_map(const int32_t *rp, const int32_t *col, const
     if (out_layout == ATTGNN_LAYOUT_HMD) { a.y_head_stride = m_ori * dim_n; a.y_row_stride = dim_n; }
     else { a.y_head_stride = dim_n; a.y_row_stride = heads * dim_n; }
     a.slices = int((dim_n + 16 * t - 1) / (16 * t));
-    a.heads = int(heads); a.window = window; a.wide_log2 = wl; a.y_type = ATTGNN_F32;
+    a.heads = int(heads); a.window = window; a.wide_log2 = wl; a.y_type = ATTGNN_F32; a.pipe = 1;
     const hipError_t e = run_spmm(kind, true, t, a, false, s);
     return e == hipSuccess ? ATTGNN_OK : int(e);
 }

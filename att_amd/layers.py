@@ -32,7 +32,7 @@ from . import graphs as G
 from . import ops
 
 
-def default_part_size(total_blocks, lo=4, hi=128):
+def default_part_size(total_blocks, lo=4, hi=512):
     """Blocks per part: enough parts to keep ~2k waves busy, long parts otherwise."""
     return int(min(max(total_blocks // 8192, lo), hi))
 

@@ -140,6 +140,10 @@ enum {
     ATTGNN_SPMM_ROWS_COVERED = 1, /* every row < m_ori lies in a window that has a
                                      part (built with emit_empty=1 and m_ori <=
                                      windows*window): Y is not pre-zeroed       */
+    /* k-loop schedule (bits 4-5); results are bitwise identical across them */
+    ATTGNN_SPMM_PIPE_DEFAULT = 0 << 4, /* column ids of the next block prefetched */
+    ATTGNN_SPMM_PIPE_NONE = 3 << 4,    /* no prefetch                            */
+    ATTGNN_SPMM_PIPE_PAIR = 2 << 4,    /* two blocks per iteration + prefetch    */
 };
 
 int64_t attgnn_spmm_workspace_elems(int64_t parts, int window, int64_t dim_n);

@@ -1,0 +1,37 @@
+"""CPU: the float64 dense attention reference used by the GPU layer tests is
+pinned by central finite differences on an N=512, degree ~8 Chung-Lu graph."""
+import numpy as np
+import torch
+
+from tests.test_gpu_layers import dense_adj, gat_reference, small_graph
+
+
+def test_reference_formula_pinned_by_finite_differences():
+    """The float64 reference gradient equals central finite differences (N=512,
+    degree ~8), so the GPU gradients compared with it above are exact gradients."""
+    g = small_graph(512, 8, seed=11)
+    A = dense_adj(g)
+    heads, dim = 2, 4
+    torch.manual_seed(0)
+    X = torch.randn(g.n, 6, dtype=torch.float64)
+    W = torch.randn(heads * dim, 6, dtype=torch.float64, requires_grad=True)
+    a0 = torch.randn(heads, dim, dtype=torch.float64, requires_grad=True)
+    a1 = torch.randn(heads, dim, dtype=torch.float64, requires_grad=True)
+    go = torch.randn(g.n, heads, dim, dtype=torch.float64)
+    f = lambda W_, a0_, a1_: (gat_reference(X, W_, None, a0_, a1_, A, heads, dim, 0.2) * go).sum()
+    f(W, a0, a1).backward()
+    eps = 1e-6
+    rng = np.random.default_rng(0)
+    for t, grad in ((W, W.grad), (a0, a0.grad), (a1, a1.grad)):
+        for _ in range(4):
+            idx = tuple(int(rng.integers(0, s)) for s in t.shape)
+            with torch.no_grad():
+                t[idx] += eps
+                fp = f(W, a0, a1).item()
+                t[idx] -= 2 * eps
+                fm = f(W, a0, a1).item()
+                t[idx] += eps
+            fd = (fp - fm) / (2 * eps)
+            assert abs(fd - grad[idx].item()) <= 1e-6 * max(1.0, abs(fd))
+
+
