@@ -44,6 +44,8 @@ _SIGNATURES = {
     "attgnn_edge_softmax_bwd": (_int, [_vp, _vp, _i64, _i64, _i64, _vp, _vp, _i64, _i64, _vp, _vp, _int,
                                        _c.c_float, _c.c_float, _c.c_uint64, _vp, _vp, _vp]),
     "attgnn_gather_rowsum": (_int, [_vp, _vp, _i64, _i64, _i64, _vp, _vp, _vp]),
+    "attgnn_pack_segments": (_int, [_vp, _i64, _vp]),
+    "attgnn_pack_segment_bytes": (_i64, []),
 }
 
 # Every symbol include/attgnn.h declares (checked by the CPU test suite).

@@ -1,0 +1,72 @@
+// Block-diagonal packing of many small graphs into one window-block structure
+// (device side of the serving loop).
+//
+// No reference counterpart: the reference serves nothing and runs one graph
+// per launch (end2end_ori/gcn_final/mgat_csr/test_mgat_csr.py:12-32).  Each
+// cached graph holds its own arrays (rows padded to the window height so
+// windows never straddle graphs); this kernel concatenates them into static,
+// capacity-sized batch buffers with the offsets applied:
+//   index arrays (column ids, window ids, CSR columns, edge ids): + offset,
+//     -1 entries kept (padding slots);
+//   running offsets (part offsets rp, CSR row pointers): + offset;
+//   raw bytes (tile values, features): copied.
+// The tail of every buffer beyond the batch is filled with "empty" entries
+// (rp / row pointers repeat the total, window ids point at the reserved last
+// window), so kernels launched with capacity sizes do no work there -- which
+// lets one HIP graph per capacity bucket serve every batch.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+
+#include "attgnn.h"
+
+namespace {
+
+// seg.kind: 0 = int32 + offset (keep -1), 1 = int32 + offset (all), 2 = bytes
+struct PackSeg {
+    const void *src;
+    void *dst;
+    int64_t count;   // elements (int32) or bytes
+    int32_t offset;
+    int32_t kind;
+};
+
+// One workgroup per table entry (the host splits long segments into chunks).
+__global__ __launch_bounds__(256) void pack_kernel(const PackSeg *segs, int64_t nsegs)
+{
+    const int64_t s = blockIdx.x;
+    if (s >= nsegs) return;
+    const PackSeg seg = segs[s];
+    if (seg.count <= 0) return;
+    if (seg.kind == 2) {
+        const uint8_t *src = static_cast<const uint8_t *>(seg.src);
+        uint8_t *dst = static_cast<uint8_t *>(seg.dst);
+        const bool vec = (reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) % 16 == 0;
+        const int64_t n16 = vec ? seg.count / 16 : 0;
+        for (int64_t i = threadIdx.x; i < n16; i += blockDim.x)
+            reinterpret_cast<uint4 *>(dst)[i] = reinterpret_cast<const uint4 *>(src)[i];
+        for (int64_t i = n16 * 16 + threadIdx.x; i < seg.count; i += blockDim.x) dst[i] = src[i];
+    } else {
+        const int32_t *src = static_cast<const int32_t *>(seg.src);
+        int32_t *dst = static_cast<int32_t *>(seg.dst);
+        for (int64_t i = threadIdx.x; i < seg.count; i += blockDim.x) {
+            const int32_t v = src ? src[i] : 0;
+            dst[i] = (seg.kind == 0 && v < 0) ? v : v + seg.offset;
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" int attgnn_pack_segments(const void *segs, int64_t nsegs, attgnn_stream_t stream)
+{
+    if (nsegs < 0 || nsegs > INT32_MAX) return ATTGNN_E_ARG;
+    if (nsegs == 0) return ATTGNN_OK;
+    if (!segs) return ATTGNN_E_ARG;
+    hipLaunchKernelGGL(pack_kernel, dim3(unsigned(nsegs)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                       static_cast<const PackSeg *>(segs), nsegs);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? ATTGNN_OK : int(e);
+}
+
+extern "C" int64_t attgnn_pack_segment_bytes(void) { return int64_t(sizeof(PackSeg)); }

@@ -1,0 +1,528 @@
+"""Continuous-batching GNN inference on one GPU.
+
+No counterpart in the reference, which times one graph per launch
+(end2end_ori/gcn_final/mgat_csr/test_mgat_csr.py:12-32).  Pieces:
+
+* ``CachedGraph``  a request graph's window-block structures (GCN tiles of
+  A_hat; GAT edge-id maps and CSR rows) and its features, resident in HBM.
+* ``GraphCache``   LRU cache of CachedGraph under an HBM byte budget; graphs
+  used by in-flight batches are never evicted; a graph that cannot be made to
+  fit waits in the queue (admission by HBM budget).
+* ``Bucket``       static, capacity-sized batch buffers for one model kind.  A
+  batch of cached graphs is packed block-diagonally (rows padded to the
+  16-row window, so windows never straddle graphs) by one device kernel that
+  reads a segment table from device memory; packing plus the model forward is
+  captured once in a HIP graph and replayed for every later batch.  Two
+  instances per capacity alternate so the host fills one while the other runs.
+* ``Server``       the loop: requests arrive on their own clock, are admitted
+  as cache space and batch slots free up, are packed into the next batch of
+  their kind while earlier batches are still on the GPU, and complete when
+  their batch's event fires.  p50 / p99 latency and throughput are reported.
+
+Models: 2-layer GCN 128 -> 128 -> 16 (ReLU) and 1-layer 4-head GAT
+128 -> 4 x 32, BF16 storage with FP32 accumulation.
+"""
+import collections
+import ctypes
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import graphs as G
+from . import ops
+
+WINDOW, WIDE = 16, 32
+F_IN, GCN_HID, GCN_OUT = 128, 128, 16
+GAT_HEADS, GAT_DIM = 4, 32
+SEG_DTYPE = np.dtype([("src", "<u8"), ("dst", "<u8"), ("count", "<i8"), ("offset", "<i4"), ("kind", "<i4")])
+KIND_IDX, KIND_ALL, KIND_BYTES = 0, 1, 2
+CHUNK_BYTES = 256 << 10      # a pack-table entry covers at most this much
+
+
+class Models:
+    """Random-init weights of the two served models (seeded)."""
+
+    def __init__(self, device, seed=0):
+        g = torch.Generator().manual_seed(seed)
+        glorot = lambda i, o: (torch.rand(i, o, generator=g) * 2 - 1) * np.sqrt(6.0 / (i + o))
+        self.W1 = glorot(F_IN, GCN_HID).to(device, torch.bfloat16)
+        self.W2 = glorot(GCN_HID, GCN_OUT).to(device, torch.bfloat16)
+        self.Wg = glorot(F_IN, GAT_HEADS * GAT_DIM).to(device, torch.bfloat16)
+        self.a0 = glorot(GAT_HEADS, GAT_DIM).to(device, torch.bfloat16).contiguous()
+        self.a1 = glorot(GAT_HEADS, GAT_DIM).to(device, torch.bfloat16).contiguous()
+        # FP32 copies for the plain per-graph baseline
+        self.f32 = {k: getattr(self, k).float() for k in ("W1", "W2", "Wg", "a0", "a1")}
+
+
+# ----------------------------------------------------------------------------
+# per-graph structures
+# ----------------------------------------------------------------------------
+
+def prepare_host(csr, features, part_size=512):
+    """Host-side preprocessing of one graph for both models (numpy / C++)."""
+    p = csr.padded(WINDOW)
+    d = G.gcn_norm_degree(p)
+    d[~np.isfinite(d)] = 0.0
+    vals = G.gcn_norm_values(p, d)
+    gb = ops.blocks_build(p.rowptr32(), p.indices, "edge_f32", vals, torch.bfloat16, WINDOW, WIDE, part_size,
+                          emit_empty=True)
+    mb = ops.blocks_build(p.rowptr32(), p.indices, "map", None, torch.int32, WINDOW, WIDE, part_size,
+                          emit_empty=True)
+    X = np.zeros((p.n, F_IN), np.float32)
+    X[: csr.n] = features
+    return {"n": csr.n, "n_pad": p.n, "nnz": csr.nnz,
+            "gcn": {k: gb[k] for k in ("rp", "col", "val", "window", "atomic")},
+            "gat": {"rowptr": torch.from_numpy(p.rowptr32()), "ccol": torch.from_numpy(p.indices),
+                    "rp": mb["rp"], "col": mb["col"], "map": mb["val"], "window": mb["window"],
+                    "atomic": mb["atomic"]},
+            "X": torch.from_numpy(X).to(torch.bfloat16)}
+
+
+class CachedGraph:
+    def __init__(self, key, host, device, stream):
+        self.key, self.n, self.n_pad, self.nnz = key, host["n"], host["n_pad"], host["nnz"]
+        with torch.cuda.stream(stream):
+            up = lambda t: t.pin_memory().to(device, non_blocking=True)
+            self.gcn = {k: up(v) for k, v in host["gcn"].items()}
+            self.gat = {k: up(v) for k, v in host["gat"].items()}
+            self.X = up(host["X"])
+        self.ready = torch.cuda.Event()
+        self.ready.record(stream)
+        self.parts_gcn = self.gcn["window"].numel()
+        self.cols_gcn = self.gcn["col"].numel()
+        self.parts_gat = self.gat["window"].numel()
+        self.cols_gat = self.gat["col"].numel()
+        self.nbytes = sum(t.numel() * t.element_size() for t in
+                          list(self.gcn.values()) + list(self.gat.values()) + [self.X])
+        self.pins = 0  # in-flight batches using this graph
+
+
+class GraphCache:
+    """LRU of CachedGraph with byte accounting under an HBM budget."""
+
+    def __init__(self, budget_bytes):
+        self.budget = budget_bytes
+        self.entries = collections.OrderedDict()
+        self.bytes = 0
+        self.hits = self.misses = self.evictions = 0
+
+    def get(self, key):
+        cg = self.entries.get(key)
+        if cg is not None:
+            self.entries.move_to_end(key)
+            self.hits += 1
+        return cg
+
+    def make_room(self, need, reserved):
+        """Evict unpinned LRU entries until `need` more bytes fit; False if impossible."""
+        if need + reserved > self.budget:
+            return False
+        for key in list(self.entries):
+            if self.bytes + need + reserved <= self.budget:
+                break
+            cg = self.entries[key]
+            if cg.pins == 0:
+                del self.entries[key]
+                self.bytes -= cg.nbytes
+                self.evictions += 1
+        return self.bytes + need + reserved <= self.budget
+
+    def put(self, cg):
+        self.entries[cg.key] = cg
+        self.bytes += cg.nbytes
+        self.misses += 1
+
+
+def estimate_bytes(csr):
+    """Device bytes of one cached graph (used for admission before building it)."""
+    n_pad = -(-csr.n // WINDOW) * WINDOW
+    cols = csr.nnz + 2 * n_pad  # distinct columns per window plus padding, upper estimate
+    return cols * WINDOW * (2 + 4) + cols * 8 + csr.nnz * 4 + n_pad * (F_IN * 2 + 4) + (n_pad // WINDOW) * 32
+
+
+# ----------------------------------------------------------------------------
+# batch buckets
+# ----------------------------------------------------------------------------
+
+class Bucket:
+    """Static batch buffers for one model kind and capacity, plus its HIP graph."""
+
+    def __init__(self, kind, rows_cap, device, models, use_graph=True):
+        self.kind, self.rows_cap, self.device, self.models = kind, rows_cap, device, models
+        self.parts_cap = rows_cap // WINDOW * 2
+        self.cols_cap = rows_cap * 36
+        self.nnz_cap = rows_cap * 34
+        self.max_graphs = min(rows_cap // 112, 512)
+        self.use_graph = use_graph
+        i32 = dict(dtype=torch.int32, device=device)
+        self.rp = torch.zeros(2 * self.parts_cap + 1, **i32)
+        self.col = torch.full((self.cols_cap,), -1, **i32)
+        self.win = torch.zeros(self.parts_cap, **i32)
+        self.atomic = torch.zeros(self.parts_cap, **i32)
+        self.X = torch.zeros(rows_cap, F_IN, dtype=torch.bfloat16, device=device)
+        if kind == "gcn":
+            self.val = torch.zeros(self.cols_cap * WINDOW, dtype=torch.bfloat16, device=device)
+            dmax = max(GCN_HID, GCN_OUT)
+        else:
+            self.map = torch.full((self.cols_cap * WINDOW,), -1, **i32)
+            self.rowptr = torch.zeros(rows_cap + 1, **i32)
+            self.ccol = torch.zeros(self.nnz_cap, **i32)
+            dmax = GAT_DIM
+        heads = GAT_HEADS if kind == "gat" else 1
+        self.ws = torch.empty(self.parts_cap * WINDOW * dmax * heads, dtype=torch.float32, device=device)
+        buf_bytes = sum(t.numel() * t.element_size() for t in self._buffers())
+        self.max_segs = self.max_graphs * 10 + 10 + buf_bytes // CHUNK_BYTES + 16
+        self.table_host = torch.zeros(self.max_segs * SEG_DTYPE.itemsize, dtype=torch.uint8).pin_memory()
+        self.table_dev = torch.zeros_like(self.table_host, device=device)
+        self.graph = None
+        self.out = None
+        self.done = None  # event of the last batch run on these buffers
+
+    def _buffers(self):
+        bufs = [self.rp, self.col, self.win, self.atomic, self.X]
+        bufs += [self.val] if self.kind == "gcn" else [self.map, self.rowptr, self.ccol]
+        return bufs
+
+    # -- capacity checks -------------------------------------------------------
+    def fits(self, graphs):
+        rows = sum(g.n_pad for g in graphs)
+        if self.kind == "gcn":
+            parts = sum(g.parts_gcn for g in graphs)
+            cols = sum(g.cols_gcn for g in graphs)
+            nnz = 0
+        else:
+            parts = sum(g.parts_gat for g in graphs)
+            cols = sum(g.cols_gat for g in graphs)
+            nnz = sum(g.nnz for g in graphs)
+        return (rows <= self.rows_cap - WINDOW and parts <= self.parts_cap and cols <= self.cols_cap
+                and nnz <= self.nnz_cap and len(graphs) <= self.max_graphs)
+
+    # -- segment table -----------------------------------------------------------
+    def build_table(self, graphs):
+        segs = []
+
+        def seg(src, dst_tensor, dst_off, count, offset, kind):
+            if count <= 0:
+                return
+            es = dst_tensor.element_size()
+            dst = dst_tensor.data_ptr() + dst_off * es
+            sp = src.data_ptr() if src is not None else 0
+            unit = 1 if kind == KIND_BYTES else 4                 # bytes per counted item
+            step = CHUNK_BYTES // unit
+            for c0 in range(0, count, step):
+                n = min(step, count - c0)
+                segs.append((sp + c0 * unit if sp else 0, dst + c0 * unit, n, offset, kind))
+
+        node = part = col = edge = 0
+        for g in graphs:
+            s = g.gcn if self.kind == "gcn" else g.gat
+            P, C = s["window"].numel(), s["col"].numel()
+            seg(s["rp"], self.rp, 2 * part, 2 * P, col, KIND_ALL)
+            seg(s["col"], self.col, col, C, node, KIND_IDX)
+            seg(s["window"], self.win, part, P, node // WINDOW, KIND_ALL)
+            seg(s["atomic"], self.atomic, part, P, 0, KIND_ALL)
+            seg(g.X, self.X, node * F_IN, g.n_pad * F_IN * 2, 0, KIND_BYTES)
+            if self.kind == "gcn":
+                seg(s["val"], self.val, col * WINDOW, C * WINDOW * 2, 0, KIND_BYTES)
+            else:
+                seg(s["map"], self.map, col * WINDOW, C * WINDOW, edge, KIND_IDX)
+                seg(s["rowptr"], self.rowptr, node, g.n_pad, edge, KIND_ALL)
+                seg(s["ccol"], self.ccol, edge, g.nnz, node, KIND_ALL)
+                edge += g.nnz
+            node += g.n_pad
+            part += P
+            col += C
+        # empty tails: parts with no columns in the reserved last window
+        seg(None, self.rp, 2 * part, 2 * (self.parts_cap - part) + 1, col, KIND_ALL)
+        seg(None, self.win, part, self.parts_cap - part, self.rows_cap // WINDOW - 1, KIND_ALL)
+        seg(None, self.atomic, part, self.parts_cap - part, 0, KIND_ALL)
+        if self.kind == "gat":
+            seg(None, self.rowptr, node, self.rows_cap + 1 - node, edge, KIND_ALL)
+        arr = np.zeros(self.max_segs, SEG_DTYPE)
+        if len(segs) > self.max_segs:
+            raise RuntimeError("segment table overflow")
+        arr[: len(segs)] = np.array(segs, dtype=SEG_DTYPE)
+        self.table_host.numpy()[:] = arr.view(np.uint8)
+        offsets, node = [], 0
+        for g in graphs:
+            offsets.append(node)
+            node += g.n_pad
+        return offsets
+
+    # -- packed forward --------------------------------------------------------
+    def _forward(self):
+        L = _lib.lib()
+        _lib.check(L.attgnn_pack_segments(ops._ptr(self.table_dev), self.max_segs, ops._stream(self.device)),
+                   "attgnn_pack_segments")
+        m = self.models
+        if self.kind == "gcn":
+            h = self.X @ m.W1
+            h = ops.spmm_block(self.rp, self.col, self.val, self.win, self.atomic, h, self.rows_cap, WINDOW, WIDE,
+                               out_dtype=torch.bfloat16, rows_covered=True, workspace=self.ws)
+            h = torch.relu(h) @ m.W2
+            return ops.spmm_block(self.rp, self.col, self.val, self.win, self.atomic, h.contiguous(), self.rows_cap,
+                                  WINDOW, WIDE, out_dtype=torch.float32, rows_covered=True, workspace=self.ws)
+        xp = (self.X @ m.Wg).view(self.rows_cap, GAT_HEADS, GAT_DIM)
+        s0 = ops.node_scores(xp, m.a0, "mhd")
+        s1 = ops.node_scores(xp, m.a1, "mhd")
+        p, _ = ops.edge_softmax_fwd(self.rowptr, self.ccol, s0, s1, 0.2, 0.0, 0, p_dtype=torch.bfloat16)
+        return ops.spmm_edge_map(self.rp, self.col, self.map, p, self.win, self.atomic, xp, "mhd", WINDOW, WIDE,
+                                 m_ori=self.rows_cap, x_layout="mhd")
+
+    def launch(self, graphs, stream):
+        """Pack `graphs` and run the model on `stream`; returns (row offsets, done event)."""
+        if self.done is not None:
+            self.done.synchronize()          # previous batch on these buffers has finished
+        offsets = self.build_table(graphs)
+        with torch.cuda.stream(stream):
+            for g in graphs:
+                stream.wait_event(g.ready)
+            self.table_dev.copy_(self.table_host, non_blocking=True)
+            if not self.use_graph:
+                self.out = self._forward()
+            elif self.graph is None:
+                self.out = self._forward()   # eager warm-up run, then capture
+                stream.synchronize()
+                self.graph = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(self.graph, stream=stream):
+                    self.out = self._forward()
+                self.graph.replay()
+            else:
+                self.graph.replay()
+            self.done = torch.cuda.Event()
+            self.done.record(stream)
+        return offsets, self.done
+
+
+# ----------------------------------------------------------------------------
+# the loop
+# ----------------------------------------------------------------------------
+
+class Request:
+    __slots__ = ("rid", "arrival", "kind", "key", "t_admit", "t_done", "result")
+
+    def __init__(self, rid, arrival, kind, key):
+        self.rid, self.arrival, self.kind, self.key = rid, arrival, kind, key
+        self.t_admit = self.t_done = None
+        self.result = None
+
+
+class Server:
+    def __init__(self, device, models, graph_source, hbm_budget=200 << 30,
+                 capacities=(1 << 14, 1 << 16, 1 << 18), use_graph=True, keep_results=False,
+                 max_in_flight=2):
+        self.device, self.models, self.source = device, models, graph_source
+        self.cache = GraphCache(hbm_budget)
+        self.stream = torch.cuda.Stream(device)        # compute (default priority)
+        self.copy_stream = torch.cuda.Stream(device)   # uploads of new graphs
+        self.buckets = {k: [[Bucket(k, c, device, models, use_graph) for _ in range(2)] for c in capacities]
+                        for k in ("gcn", "gat")}
+        self.turn = collections.defaultdict(int)
+        self.reserved = sum(b.X.numel() * 2 + b.col.numel() * 4 * (WINDOW + 1)
+                            for ks in self.buckets.values() for pair in ks for b in pair)
+        self.keep_results = keep_results
+        self.max_in_flight = max_in_flight
+        self.batches = 0
+
+    def _admit(self, req):
+        """Graph of the request resident on the device, or None if it must wait."""
+        cg = self.cache.get(req.key)
+        if cg is not None:
+            return cg
+        csr, feats = self.source(req.key)
+        if not self.cache.make_room(estimate_bytes(csr), self.reserved):
+            return None
+        cg = CachedGraph(req.key, prepare_host(csr, feats), self.device, self.copy_stream)
+        self.cache.put(cg)
+        return cg
+
+    def _pick_bucket(self, kind, graphs):
+        for i, pair in enumerate(self.buckets[kind]):
+            if pair[0].fits(graphs):
+                b = pair[self.turn[(kind, i)] % 2]
+                self.turn[(kind, i)] += 1
+                return b
+        return None
+
+    def run(self, requests, t0=None):
+        """Serve `requests` (sorted by arrival, seconds from t0); returns them with t_done set."""
+        pending = collections.deque(requests)
+        queue = {"gcn": collections.deque(), "gat": collections.deque()}
+        in_flight = collections.deque()
+        t0 = time.perf_counter() if t0 is None else t0
+        done = 0
+        while done < len(requests):
+            now = time.perf_counter() - t0
+            while pending and pending[0].arrival <= now:
+                r = pending.popleft()
+                queue[r.kind].append(r)
+            # retire finished batches (in order: one compute stream)
+            while in_flight and in_flight[0][0].query():
+                ev, batch, graphs, out, offs = in_flight.popleft()
+                t = time.perf_counter() - t0
+                for r, g, o in zip(batch, graphs, offs):
+                    r.t_done = t
+                    g.pins -= 1
+                    if self.keep_results:
+                        r.result = out[o:o + g.n].clone()
+                done += len(batch)
+            if len(in_flight) < self.max_in_flight and (queue["gcn"] or queue["gat"]):
+                kind = min((k for k in queue if queue[k]), key=lambda k: queue[k][0].arrival)
+                batch, graphs = [], []
+                big = self.buckets[kind][-1][0]
+                while queue[kind]:
+                    r = queue[kind][0]
+                    cg = self._admit(r)
+                    if cg is None:
+                        if not in_flight and not graphs:
+                            raise RuntimeError(f"request {r.rid}: graph does not fit the HBM budget")
+                        break
+                    if not big.fits(graphs + [cg]):
+                        if graphs:
+                            break
+                        # a graph larger than every bucket runs alone, unpacked
+                        queue[kind].popleft()
+                        r.t_admit = time.perf_counter() - t0
+                        self._run_alone(r, cg, t0)
+                        done += 1
+                        continue
+                    queue[kind].popleft()
+                    r.t_admit = time.perf_counter() - t0
+                    batch.append(r)
+                    graphs.append(cg)
+                    cg.pins += 1            # not evictable while in the batch
+                if batch:
+                    b = self._pick_bucket(kind, graphs)
+                    offs, ev = b.launch(graphs, self.stream)
+                    in_flight.append((ev, batch, graphs, b.out, offs))
+                    self.batches += 1
+                continue
+            if not in_flight and not any(queue.values()) and pending:
+                wait = pending[0].arrival - (time.perf_counter() - t0)
+                if wait > 0:
+                    time.sleep(min(wait, 0.001))
+            else:
+                time.sleep(2e-5)
+        return requests
+
+    def _run_alone(self, r, cg, t0):
+        """Unpacked path for a graph above the largest bucket."""
+        m = self.models
+        with torch.cuda.stream(self.stream):
+            self.stream.wait_event(cg.ready)
+            if r.kind == "gcn":
+                s = cg.gcn
+                h = cg.X @ m.W1
+                h = ops.spmm_block(s["rp"], s["col"], s["val"], s["window"], s["atomic"], h, cg.n_pad, WINDOW, WIDE,
+                                   out_dtype=torch.bfloat16, rows_covered=True)
+                h = (torch.relu(h) @ m.W2).contiguous()
+                out = ops.spmm_block(s["rp"], s["col"], s["val"], s["window"], s["atomic"], h, cg.n_pad, WINDOW,
+                                     WIDE, rows_covered=True)
+            else:
+                s = cg.gat
+                xp = (cg.X @ m.Wg).view(cg.n_pad, GAT_HEADS, GAT_DIM)
+                s0 = ops.node_scores(xp, m.a0, "mhd")
+                s1 = ops.node_scores(xp, m.a1, "mhd")
+                p, _ = ops.edge_softmax_fwd(s["rowptr"], s["ccol"], s0, s1, 0.2, 0.0, 0, p_dtype=torch.bfloat16)
+                out = ops.spmm_edge_map(s["rp"], s["col"], s["map"], p, s["window"], s["atomic"], xp, "mhd", WINDOW,
+                                        WIDE, m_ori=cg.n_pad, x_layout="mhd")
+        self.stream.synchronize()
+        r.t_done = time.perf_counter() - t0
+        if self.keep_results:
+            r.result = out[: cg.n].clone()
+
+
+# ----------------------------------------------------------------------------
+# plain FP32 per-graph baseline
+# ----------------------------------------------------------------------------
+
+class PlainServer:
+    """One request at a time, FIFO, plain FP32 PyTorch: torch.sparse CSR SpMM
+    for GCN, gather / scatter-softmax / index_add for GAT; structures and
+    features cached on the device as well."""
+
+    def __init__(self, device, models, graph_source):
+        self.device, self.models, self.source = device, models, graph_source
+        self.cache = {}
+
+    def _get(self, key):
+        if key not in self.cache:
+            from .benchmarks.plain import csr_tensor
+            csr, feats = self.source(key)
+            vals = G.gcn_norm_values(csr)
+            self.cache[key] = {
+                "A": csr_tensor(csr, vals, self.device),
+                "rows": torch.from_numpy(csr.row_ids().astype(np.int64)).to(self.device),
+                "cols": torch.from_numpy(csr.indices.astype(np.int64)).to(self.device),
+                "X": torch.from_numpy(np.asarray(feats, np.float32)).to(torch.bfloat16).float().to(self.device),
+                "n": csr.n}
+        return self.cache[key]
+
+    def forward(self, kind, c):
+        from .benchmarks.plain import gat_fp32
+        m = self.models.f32
+        if kind == "gcn":
+            h = torch.relu(torch.sparse.mm(c["A"], c["X"] @ m["W1"]))
+            return torch.sparse.mm(c["A"], h @ m["W2"])
+        return gat_fp32(c["X"], m["Wg"].t().contiguous(), m["a0"], m["a1"], c["rows"], c["cols"], c["n"], GAT_HEADS,
+                        GAT_DIM)
+
+    def run(self, requests, t0=None):
+        t0 = time.perf_counter() if t0 is None else t0
+        with torch.no_grad():
+            for r in requests:
+                c = self._get(r.key)
+                wait = r.arrival - (time.perf_counter() - t0)
+                if wait > 0:
+                    time.sleep(wait)
+                r.t_admit = time.perf_counter() - t0
+                out = self.forward(r.kind, c)
+                torch.cuda.synchronize(self.device)
+                r.t_done = time.perf_counter() - t0
+                r.result = out
+        return requests
+
+
+# ----------------------------------------------------------------------------
+# workload
+# ----------------------------------------------------------------------------
+
+class GraphPool:
+    """Seeded pool of request graphs: Chung-Lu (gamma 2.1) with N ~ lognormal
+    (median 2000, sigma 1.2) clipped to [100, 200000] and mean degree U[4, 32];
+    features N(0,1) rounded to BF16.  Graphs are generated on first use."""
+
+    def __init__(self, size, seed=0, n_max=200_000):
+        rng = np.random.Generator(np.random.PCG64(seed))
+        self.n = np.clip(np.exp(rng.normal(np.log(2000), 1.2, size)), 100, n_max).astype(np.int64)
+        self.deg = rng.uniform(4, 32, size)
+        self.seed = seed
+        self.memo = {}
+
+    def __call__(self, key):
+        if key not in self.memo:
+            csr = G.chung_lu(int(self.n[key]), float(self.deg[key]), 2.1, seed=self.seed * 100003 + key)
+            feats = np.random.Generator(np.random.PCG64(key + 7)).standard_normal((csr.n, F_IN)).astype(np.float32)
+            feats = torch.from_numpy(feats).to(torch.bfloat16).float().numpy()
+            self.memo[key] = (csr, feats)
+        return self.memo[key]
+
+
+def make_requests(count, rate, pool_size, seed=0):
+    """Poisson arrivals at `rate` req/s (rate None: all at t = 0), 50/50 GCN/GAT,
+    graphs drawn uniformly from the pool."""
+    rng = np.random.Generator(np.random.PCG64(seed + 1))
+    arr = np.zeros(count) if rate is None else np.cumsum(rng.exponential(1.0 / rate, count))
+    kinds = rng.random(count) < 0.5
+    keys = rng.integers(0, pool_size, count)
+    return [Request(i, float(arr[i]), "gcn" if kinds[i] else "gat", int(keys[i])) for i in range(count)]
+
+
+def latency_stats(requests):
+    lat = np.array([r.t_done - r.arrival for r in requests])
+    span = max(r.t_done for r in requests) - min(r.arrival for r in requests)
+    return {"requests": len(requests), "req_per_s": len(requests) / span if span > 0 else float("inf"),
+            "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "mean_ms": float(lat.mean() * 1e3)}

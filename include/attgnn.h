@@ -285,6 +285,26 @@ int attgnn_edge_softmax_bwd(const int32_t *rowptr, const int32_t *col, int64_t n
 int attgnn_gather_rowsum(const int32_t *rowptr, const int64_t *perm, int64_t n, int64_t nnz,
                          int64_t heads, const float *vals, float *out, attgnn_stream_t stream);
 
+/* ====================================================================== */
+/* Block-diagonal packing for batched inference (device)                   */
+/* ====================================================================== */
+
+/*
+ * Copy nsegs segments described by a device-resident table (attgnn_pack_segment_bytes()
+ * bytes per entry: const void *src; void *dst; int64_t count; int32_t offset;
+ * int32_t kind), one workgroup per entry (split long segments into chunks;
+ * entries with count 0 are skipped).  kind 0: int32 dst[i] = src[i] < 0 ?
+ * src[i] : src[i] + offset; kind 1: int32 dst[i] = (src ? src[i] : 0) + offset
+ * (src NULL = fill); kind 2: copy count bytes.
+ * One launch packs a whole batch of cached graphs into one block-diagonal
+ * window-block structure; the table lives in device memory so the launch can
+ * be captured in a HIP graph and replayed with new tables.
+ *
+ * No counterpart in the reference (it runs one graph per launch).
+ */
+int attgnn_pack_segments(const void *segs, int64_t nsegs, attgnn_stream_t stream);
+int64_t attgnn_pack_segment_bytes(void);
+
 #ifdef __cplusplus
 }
 #endif
