@@ -29,6 +29,7 @@ def main():
     ap.add_argument("--plain-requests", type=int, default=1024)
     ap.add_argument("--no-graph", action="store_true", help="eager launches instead of HIP graphs")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--part-size", type=int, default=S.SERVE_PART_SIZE, help="blocks per part of cached graphs")
     args = ap.parse_args()
     dev = torch.device("cuda", 0)
     models = S.Models(dev)
@@ -39,7 +40,7 @@ def main():
     print(f"pool of {args.pool} graphs generated ({time.perf_counter() - t:.1f}s); "
           f"nodes median {int(np.median(pool.n))} max {int(pool.n.max())}", file=sys.stderr)
 
-    server = S.Server(dev, models, pool, use_graph=not args.no_graph)
+    server = S.Server(dev, models, pool, use_graph=not args.no_graph, part_size=args.part_size)
     t = time.perf_counter()
     warm = [S.Request(i, 0.0, "gcn" if i % 2 else "gat", i) for i in range(args.pool)]
     server.run(warm)
@@ -81,7 +82,7 @@ def main():
                       "req_s_at_bound": best[1]["req_per_s"] if best else None,
                       "batches": server.batches, "cache_hits": server.cache.hits,
                       "cache_misses": server.cache.misses, "cold_warmup_s": cold_s,
-                      "hip_graphs": not args.no_graph}), flush=True)
+                      "hip_graphs": not args.no_graph, "part_size": args.part_size}), flush=True)
 
 
 if __name__ == "__main__":

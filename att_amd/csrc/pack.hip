@@ -43,9 +43,20 @@ __global__ __launch_bounds__(256) void pack_kernel(const PackSeg *segs, int64_t 
         uint8_t *dst = static_cast<uint8_t *>(seg.dst);
         const bool vec = (reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) % 16 == 0;
         const int64_t n16 = vec ? seg.count / 16 : 0;
-        for (int64_t i = threadIdx.x; i < n16; i += blockDim.x)
-            reinterpret_cast<uint4 *>(dst)[i] = reinterpret_cast<const uint4 *>(src)[i];
-        for (int64_t i = n16 * 16 + threadIdx.x; i < seg.count; i += blockDim.x) dst[i] = src[i];
+        const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
+        uint4 *d4 = reinterpret_cast<uint4 *>(dst);
+        // four independent 16-byte loads in flight per thread before the stores
+        constexpr int U = 4;
+        int64_t i = threadIdx.x;
+        for (; i + (U - 1) * blockDim.x < n16; i += U * blockDim.x) {
+            uint4 v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = s4[i + u * blockDim.x];
+#pragma unroll
+            for (int u = 0; u < U; ++u) d4[i + u * blockDim.x] = v[u];
+        }
+        for (; i < n16; i += blockDim.x) d4[i] = s4[i];
+        for (int64_t j = n16 * 16 + threadIdx.x; j < seg.count; j += blockDim.x) dst[j] = src[j];
     } else {
         const int32_t *src = static_cast<const int32_t *>(seg.src);
         int32_t *dst = static_cast<int32_t *>(seg.dst);

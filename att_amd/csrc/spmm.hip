@@ -156,19 +156,38 @@ struct KBlock {
         for (int w = 0; w < AW; ++w) av[w] = 0;
         if (!a_row_ok || kg >= nreal) return;
         if constexpr (MAPPED) {
+            // the 8 edge ids of row n (one or two vector loads), then the 8
+            // value gathers issued together
+            int m[8];
+            if (wide >= 8) {
+                const int64_t cc = int64_t(c_begin) + kg;
+                const int32_t *mp = a.map + (cc >> a.wide_log2) * W * wide + int64_t(n) * wide + (cc & (wide - 1));
+                const u32x4 m0 = *reinterpret_cast<const u32x4 *>(mp);
+                const u32x4 m1 = *reinterpret_cast<const u32x4 *>(mp + 4);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) { m[i] = int(m0[i]); m[i + 4] = int(m1[i]); }
+            } else {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int64_t cc = int64_t(c_begin) + kg + 4 * h;
+                    u32x4 mv = {~0u, ~0u, ~0u, ~0u};
+                    if (kg + 4 * h < plen)
+                        mv = *reinterpret_cast<const u32x4 *>(a.map + (cc >> a.wide_log2) * W * wide + int64_t(n) * wide);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) m[4 * h + i] = int(mv[i]);
+                }
+            }
+            uint32_t e[8];
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
-                const int64_t cc = int64_t(c_begin) + kg + i;
-                if (kg + i < plen) {
-                    const int m = a.map[(cc >> a.wide_log2) * W * wide + int64_t(n) * wide + (cc & (wide - 1))];
-                    if (m >= 0) {
-                        uint32_t e;
-                        load_bytes<EB>(vh + int64_t(m) * EB, &e);
-                        if constexpr (EB == 4) av[i] = e;
-                        else if constexpr (EB == 2) av[i >> 1] |= e << (16 * (i & 1));
-                        else av[i >> 2] |= e << (8 * (i & 3));
-                    }
-                }
+                e[i] = 0;
+                if (m[i] >= 0) load_bytes<EB>(vh + int64_t(m[i]) * EB, &e[i]);
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                if constexpr (EB == 4) av[i] = e[i];
+                else if constexpr (EB == 2) av[i >> 1] |= e[i] << (16 * (i & 1));
+                else av[i >> 2] |= e[i] << (8 * (i & 3));
             }
         } else if (wide >= 8) {
             const int64_t cc = int64_t(c_begin) + kg;
@@ -350,24 +369,27 @@ __global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
     }
 }
 
-// Sum the partial tiles of every split window in part order.  One workgroup
-// per part; only the first part of a split window does work.
+// Sum the partial tiles of every split window in part order.  Workgroups walk
+// the parts with a grid stride; only the first part of a split window does
+// work (the workgroup reduces rows x features of the window), so the launch
+// stays small however many parts there are.
 __global__ __launch_bounds__(256) void spmm_combine_kernel(SpmmArgs a)
 {
-    const int64_t p = blockIdx.x;
     const int head = blockIdx.y;
-    if (!a.atomic[p]) return;
-    const int w = a.win[p];
-    if (p > 0 && a.atomic[p - 1] && a.win[p - 1] == w) return;
-    int64_t last = p;
-    while (last + 1 < a.parts && a.atomic[last + 1] && a.win[last + 1] == w) ++last;
-    const int64_t rows = min(int64_t(a.window), a.m_ori - int64_t(w) * a.window);
-    const int64_t n = rows * a.dim_n;
-    for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
-        const int64_t r = e / a.dim_n, f = e - r * a.dim_n;
-        float s[1] = {0.f};
-        for (int64_t q = p; q <= last; ++q) s[0] += a.partial[((q * a.heads + head) * a.window + r) * a.dim_n + f];
-        store_row<1>(a.y, int64_t(head) * a.y_head_stride + (int64_t(w) * a.window + r) * a.y_row_stride + f, a.y_type, s);
+    for (int64_t p = blockIdx.x; p < a.parts; p += gridDim.x) {
+        if (!a.atomic[p]) continue;
+        const int w = a.win[p];
+        if (p > 0 && a.atomic[p - 1] && a.win[p - 1] == w) continue;
+        int64_t last = p;
+        while (last + 1 < a.parts && a.atomic[last + 1] && a.win[last + 1] == w) ++last;
+        const int64_t rows = min(int64_t(a.window), a.m_ori - int64_t(w) * a.window);
+        const int64_t n = rows * a.dim_n;
+        for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
+            const int64_t r = e / a.dim_n, f = e - r * a.dim_n;
+            float s[1] = {0.f};
+            for (int64_t q = p; q <= last; ++q) s[0] += a.partial[((q * a.heads + head) * a.window + r) * a.dim_n + f];
+            store_row<1>(a.y, int64_t(head) * a.y_head_stride + (int64_t(w) * a.window + r) * a.y_row_stride + f, a.y_type, s);
+        }
     }
 }
 
@@ -461,7 +483,8 @@ hipError_t run_spmm(int kind, bool mapped, int t, const SpmmArgs &a, bool covere
     default: e = mapped ? launch_main<K_F32, true>(t, a, s) : launch_main<K_F32, false>(t, a, s); break;
     }
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(spmm_combine_kernel, dim3(unsigned(a.parts), unsigned(a.heads)), dim3(256), 0, s, a);
+    const int64_t cblocks = std::min<int64_t>(a.parts, 2048);
+    hipLaunchKernelGGL(spmm_combine_kernel, dim3(unsigned(cblocks), unsigned(a.heads)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -38,7 +38,8 @@ F_IN, GCN_HID, GCN_OUT = 128, 128, 16
 GAT_HEADS, GAT_DIM = 4, 32
 SEG_DTYPE = np.dtype([("src", "<u8"), ("dst", "<u8"), ("count", "<i8"), ("offset", "<i4"), ("kind", "<i4")])
 KIND_IDX, KIND_ALL, KIND_BYTES = 0, 1, 2
-CHUNK_BYTES = 256 << 10      # a pack-table entry covers at most this much
+CHUNK_BYTES = 32 << 10       # a pack-table entry covers at most this much (one workgroup)
+SERVE_PART_SIZE = 8          # blocks per part: short per-wave chains for small graphs
 
 
 class Models:
@@ -60,7 +61,7 @@ class Models:
 # per-graph structures
 # ----------------------------------------------------------------------------
 
-def prepare_host(csr, features, part_size=512):
+def prepare_host(csr, features, part_size=SERVE_PART_SIZE):
     """Host-side preprocessing of one graph for both models (numpy / C++)."""
     p = csr.padded(WINDOW)
     d = G.gcn_norm_degree(p)
@@ -97,6 +98,17 @@ class CachedGraph:
         self.nbytes = sum(t.numel() * t.element_size() for t in
                           list(self.gcn.values()) + list(self.gat.values()) + [self.X])
         self.pins = 0  # in-flight batches using this graph
+        # (device pointer, positions) of every array the pack table copies, a
+        # position being one part (rp: 2 ints), column, row, or edge; rp and the
+        # CSR row pointers drop their final entry (the next graph's start)
+        P, Pg = self.parts_gcn, self.parts_gat
+        common = lambda s_, p_: {"rp": (s_["rp"].data_ptr(), p_), "col": (s_["col"].data_ptr(), s_["col"].numel()),
+                                 "window": (s_["window"].data_ptr(), p_), "atomic": (s_["atomic"].data_ptr(), p_),
+                                 "X": (self.X.data_ptr(), self.n_pad)}
+        self.info = {"gcn": {**common(self.gcn, P), "val": (self.gcn["val"].data_ptr(), self.cols_gcn)},
+                     "gat": {**common(self.gat, Pg), "map": (self.gat["map"].data_ptr(), self.cols_gat),
+                             "rowptr": (self.gat["rowptr"].data_ptr(), self.n_pad),
+                             "ccol": (self.gat["ccol"].data_ptr(), self.nnz)}}
 
 
 class GraphCache:
@@ -151,7 +163,7 @@ class Bucket:
 
     def __init__(self, kind, rows_cap, device, models, use_graph=True):
         self.kind, self.rows_cap, self.device, self.models = kind, rows_cap, device, models
-        self.parts_cap = rows_cap // WINDOW * 2
+        self.parts_cap = rows_cap // WINDOW * 8
         self.cols_cap = rows_cap * 36
         self.nnz_cap = rows_cap * 34
         self.max_graphs = min(rows_cap // 112, 512)
@@ -179,6 +191,8 @@ class Bucket:
         self.graph = None
         self.out = None
         self.done = None  # event of the last batch run on these buffers
+        self.used = 0     # table entries written by the previous batch
+        self.copy_bytes = 0
 
     def _buffers(self):
         bufs = [self.rp, self.col, self.win, self.atomic, self.X]
@@ -200,56 +214,88 @@ class Bucket:
                 and nnz <= self.nnz_cap and len(graphs) <= self.max_graphs)
 
     # -- segment table -----------------------------------------------------------
+    # Segment types: (source array, destination buffer, destination position
+    # (a running total), items per position, offset source, table kind).
+    # Running totals per graph: node (rows), part, col (condensed columns), edge.
+    def _types(self):
+        W = WINDOW
+        t = [("rp", self.rp, "part", 2, "col", KIND_ALL),
+             ("col", self.col, "col", 1, "node", KIND_IDX),
+             ("window", self.win, "part", 1, "win", KIND_ALL),
+             ("atomic", self.atomic, "part", 1, None, KIND_ALL),
+             ("X", self.X, "node", F_IN, None, KIND_BYTES)]
+        if self.kind == "gcn":
+            t.append(("val", self.val, "col", W, None, KIND_BYTES))
+        else:
+            t += [("map", self.map, "col", W, "edge", KIND_IDX),
+                  ("rowptr", self.rowptr, "node", 1, "edge", KIND_ALL),
+                  ("ccol", self.ccol, "edge", 1, "node", KIND_ALL)]
+        return t
+
     def build_table(self, graphs):
-        segs = []
-
-        def seg(src, dst_tensor, dst_off, count, offset, kind):
-            if count <= 0:
-                return
-            es = dst_tensor.element_size()
-            dst = dst_tensor.data_ptr() + dst_off * es
-            sp = src.data_ptr() if src is not None else 0
-            unit = 1 if kind == KIND_BYTES else 4                 # bytes per counted item
-            step = CHUNK_BYTES // unit
-            for c0 in range(0, count, step):
-                n = min(step, count - c0)
-                segs.append((sp + c0 * unit if sp else 0, dst + c0 * unit, n, offset, kind))
-
-        node = part = col = edge = 0
-        for g in graphs:
-            s = g.gcn if self.kind == "gcn" else g.gat
-            P, C = s["window"].numel(), s["col"].numel()
-            seg(s["rp"], self.rp, 2 * part, 2 * P, col, KIND_ALL)
-            seg(s["col"], self.col, col, C, node, KIND_IDX)
-            seg(s["window"], self.win, part, P, node // WINDOW, KIND_ALL)
-            seg(s["atomic"], self.atomic, part, P, 0, KIND_ALL)
-            seg(g.X, self.X, node * F_IN, g.n_pad * F_IN * 2, 0, KIND_BYTES)
-            if self.kind == "gcn":
-                seg(s["val"], self.val, col * WINDOW, C * WINDOW * 2, 0, KIND_BYTES)
-            else:
-                seg(s["map"], self.map, col * WINDOW, C * WINDOW, edge, KIND_IDX)
-                seg(s["rowptr"], self.rowptr, node, g.n_pad, edge, KIND_ALL)
-                seg(s["ccol"], self.ccol, edge, g.nnz, node, KIND_ALL)
-                edge += g.nnz
-            node += g.n_pad
-            part += P
-            col += C
-        # empty tails: parts with no columns in the reserved last window
-        seg(None, self.rp, 2 * part, 2 * (self.parts_cap - part) + 1, col, KIND_ALL)
-        seg(None, self.win, part, self.parts_cap - part, self.rows_cap // WINDOW - 1, KIND_ALL)
-        seg(None, self.atomic, part, self.parts_cap - part, 0, KIND_ALL)
+        """Write the pack table for `graphs` into the pinned host table (numpy,
+        vectorised over graphs and chunks); returns each graph's first row."""
+        k = len(graphs)
+        gi = graphs[0].info[self.kind]
+        names = list(gi.keys())
+        src = np.array([[g.info[self.kind][nm][0] for nm in names] for g in graphs], np.uint64)   # [k, T]
+        cnt = np.array([[g.info[self.kind][nm][1] for nm in names] for g in graphs], np.int64)    # [k, T]
+        sizes = np.array([[g.n_pad, g.info[self.kind]["window"][1], g.info[self.kind]["col"][1], g.nnz]
+                          for g in graphs], np.int64)
+        start = np.vstack([np.zeros((1, 4), np.int64), np.cumsum(sizes, axis=0)[:-1]])             # [k, 4]
+        pos_of = {"node": start[:, 0], "part": start[:, 1], "col": start[:, 2], "edge": start[:, 3]}
+        off_of = {"node": start[:, 0], "col": start[:, 2], "edge": start[:, 3], "win": start[:, 0] // WINDOW,
+                  None: np.zeros(k, np.int64)}
+        types = {t[0]: t for t in self._types()}
+        seg_src, seg_dst, seg_cnt, seg_off, seg_kind = [], [], [], [], []
+        for j, nm in enumerate(names):
+            _, buf, pos, per, offk, kind = types[nm]
+            es = buf.element_size()
+            unit = 1 if kind == KIND_BYTES else 4
+            seg_src.append(src[:, j])
+            seg_dst.append(np.uint64(buf.data_ptr()) + (pos_of[pos] * per * es).astype(np.uint64))
+            seg_cnt.append(cnt[:, j] * per * (es if kind == KIND_BYTES else 1))
+            seg_off.append(off_of[offk])
+            seg_kind.append(np.full(k, kind, np.int64))
+        tot = sizes.sum(axis=0)
+        node, part, col, edge = (int(x) for x in tot)
+        tails = [(self.rp, 2 * part, 2 * (self.parts_cap - part) + 1, col),
+                 (self.win, part, self.parts_cap - part, self.rows_cap // WINDOW - 1),
+                 (self.atomic, part, self.parts_cap - part, 0)]
         if self.kind == "gat":
-            seg(None, self.rowptr, node, self.rows_cap + 1 - node, edge, KIND_ALL)
-        arr = np.zeros(self.max_segs, SEG_DTYPE)
-        if len(segs) > self.max_segs:
+            tails.append((self.rowptr, node, self.rows_cap + 1 - node, edge))
+        for buf, at, n, val in tails:
+            seg_src.append(np.zeros(1, np.uint64))
+            seg_dst.append(np.array([buf.data_ptr() + at * 4], np.uint64))
+            seg_cnt.append(np.array([n], np.int64))
+            seg_off.append(np.array([val], np.int64))
+            seg_kind.append(np.array([KIND_ALL], np.int64))
+        s_src, s_dst, s_cnt = np.concatenate(seg_src), np.concatenate(seg_dst), np.concatenate(seg_cnt)
+        s_off, s_kind = np.concatenate(seg_off), np.concatenate(seg_kind)
+        keep = s_cnt > 0
+        s_src, s_dst, s_cnt, s_off, s_kind = s_src[keep], s_dst[keep], s_cnt[keep], s_off[keep], s_kind[keep]
+        # split into chunks of at most CHUNK_BYTES (counts are bytes or int32 items)
+        unit = np.where(s_kind == KIND_BYTES, 1, 4)
+        step = CHUNK_BYTES // unit
+        nch = (s_cnt + step - 1) // step
+        rep = np.repeat(np.arange(s_cnt.size), nch)
+        first = np.cumsum(nch) - nch
+        ci = np.arange(rep.size) - first[rep]
+        byte0 = (ci * step[rep] * unit[rep]).astype(np.uint64)
+        n_entries = rep.size
+        if n_entries > self.max_segs:
             raise RuntimeError("segment table overflow")
-        arr[: len(segs)] = np.array(segs, dtype=SEG_DTYPE)
-        self.table_host.numpy()[:] = arr.view(np.uint8)
-        offsets, node = [], 0
-        for g in graphs:
-            offsets.append(node)
-            node += g.n_pad
-        return offsets
+        arr = np.zeros(self.max_segs, SEG_DTYPE)
+        arr["src"][:n_entries] = np.where(s_src[rep] != 0, s_src[rep] + byte0, 0)
+        arr["dst"][:n_entries] = s_dst[rep] + byte0
+        arr["count"][:n_entries] = np.minimum(step[rep], s_cnt[rep] - ci * step[rep])
+        arr["offset"][:n_entries] = s_off[rep]
+        arr["kind"][:n_entries] = s_kind[rep]
+        used = max(n_entries, self.used)
+        self.table_host.numpy()[: used * SEG_DTYPE.itemsize] = arr[:used].view(np.uint8)
+        self.used = n_entries
+        self.copy_bytes = used * SEG_DTYPE.itemsize
+        return [int(x) for x in start[:, 0]]
 
     # -- packed forward --------------------------------------------------------
     def _forward(self):
@@ -271,15 +317,21 @@ class Bucket:
         return ops.spmm_edge_map(self.rp, self.col, self.map, p, self.win, self.atomic, xp, "mhd", WINDOW, WIDE,
                                  m_ori=self.rows_cap, x_layout="mhd")
 
-    def launch(self, graphs, stream):
+    def launch(self, graphs, stream, timing=None):
         """Pack `graphs` and run the model on `stream`; returns (row offsets, done event)."""
+        t = time.perf_counter()
         if self.done is not None:
             self.done.synchronize()          # previous batch on these buffers has finished
+        t1 = time.perf_counter()
         offsets = self.build_table(graphs)
+        if timing is not None:
+            timing["bucket_wait"] += t1 - t
+            timing["table"] += time.perf_counter() - t1
         with torch.cuda.stream(stream):
             for g in graphs:
                 stream.wait_event(g.ready)
-            self.table_dev.copy_(self.table_host, non_blocking=True)
+            nb = self.copy_bytes
+            self.table_dev[:nb].copy_(self.table_host[:nb], non_blocking=True)
             if not self.use_graph:
                 self.out = self._forward()
             elif self.graph is None:
@@ -311,9 +363,10 @@ class Request:
 
 class Server:
     def __init__(self, device, models, graph_source, hbm_budget=200 << 30,
-                 capacities=(1 << 14, 1 << 16, 1 << 18), use_graph=True, keep_results=False,
-                 max_in_flight=2):
+                 capacities=(1 << 12, 1 << 14, 1 << 16, 1 << 18), use_graph=True, keep_results=False,
+                 max_in_flight=2, part_size=SERVE_PART_SIZE):
         self.device, self.models, self.source = device, models, graph_source
+        self.part_size = part_size
         self.cache = GraphCache(hbm_budget)
         self.stream = torch.cuda.Stream(device)        # compute (default priority)
         self.copy_stream = torch.cuda.Stream(device)   # uploads of new graphs
@@ -325,6 +378,7 @@ class Server:
         self.keep_results = keep_results
         self.max_in_flight = max_in_flight
         self.batches = 0
+        self.timing = collections.Counter()   # host seconds per phase (observability)
 
     def _admit(self, req):
         """Graph of the request resident on the device, or None if it must wait."""
@@ -334,7 +388,7 @@ class Server:
         csr, feats = self.source(req.key)
         if not self.cache.make_room(estimate_bytes(csr), self.reserved):
             return None
-        cg = CachedGraph(req.key, prepare_host(csr, feats), self.device, self.copy_stream)
+        cg = CachedGraph(req.key, prepare_host(csr, feats, self.part_size), self.device, self.copy_stream)
         self.cache.put(cg)
         return cg
 
@@ -394,8 +448,11 @@ class Server:
                     graphs.append(cg)
                     cg.pins += 1            # not evictable while in the batch
                 if batch:
+                    t1 = time.perf_counter()
+                    self.timing["form"] += t1 - (now + t0)
                     b = self._pick_bucket(kind, graphs)
-                    offs, ev = b.launch(graphs, self.stream)
+                    offs, ev = b.launch(graphs, self.stream, self.timing)
+                    self.timing["launch"] += time.perf_counter() - t1
                     in_flight.append((ev, batch, graphs, b.out, offs))
                     self.batches += 1
                 continue
