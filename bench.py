@@ -210,6 +210,13 @@ def main():
     t_plain = ev0.elapsed_time(ev1) / 3
     del A32, X32
 
+    traffic = None
+    tfile = os.path.join(HERE, "profiles", "spmm256_traffic.json")
+    if args.scale == 1.0 and os.path.exists(tfile):
+        # HBM-side bytes per launch from the committed rocprofv3 FETCH_SIZE /
+        # WRITE_SIZE passes of this kernel, scaled by the copy calibration
+        t = json.load(open(tfile))
+        traffic = int((t["fetch_kib"] * t["fetch_factor"] + t["write_kib"] * t["write_factor"]) * 1024)
     out = {
         "metric": "SpMM/SDDMM GFLOP/s & %HBM roofline; GNN req/s + p99 vs plain FP32 at stated err",
         "value": round(value, 2),
@@ -227,7 +234,7 @@ def main():
                    "nodes": g.n, "nnz": g.nnz, "features": [F_IN, F_HID, F_OUT], "window": 16, "wide": 32,
                    "tiles": "fp16", "accumulate": "fp32", "scale": args.scale},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "window-block SpMM, width 256, fp16 (main + split-window combine)",
                      "ms_per_launch": round(t_spmm, 4), "algorithmic_bytes": int(algo_bytes),
                      "gflops": round(2.0 * g.nnz * F_HID / (t_spmm * 1e-3) / 1e9, 1)},
