@@ -71,6 +71,7 @@ struct SpmmArgs {
     int64_t parts, m_ori, dim_n;
     int64_t x_head_stride, x_row_stride, y_head_stride, y_row_stride, val_head_stride;
     int slices, heads, window, wide_log2, y_type, pipe;
+    int xcd_slices;         // 1: slice-per-XCD task mapping (slices divides 8)
 };
 
 // Store n consecutive fp32 values as y_type (F32 / F16 / BF16).
@@ -267,11 +268,28 @@ __global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
     constexpr int EB = KB::EB, AW = KB::AW, XW = KB::XW;
 
     const int lane = threadIdx.x & 63;
-    const int64_t task = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
-    if (task >= a.parts * a.heads * a.slices) return;
-    const int64_t part = task / (int64_t(a.heads) * a.slices);
-    const int hs = int(task - part * a.heads * a.slices);
-    const int head = hs / a.slices, slice = hs - head * a.slices;
+    int64_t part;
+    int head, slice;
+    if (a.xcd_slices) {
+        // Workgroups are dispatched round-robin over the 8 XCDs.  Give every
+        // XCD a single feature slice (slices divides 8) so each XCD's L2 caches
+        // one slice of the gathered rows: xcd = b % 8 -> slice xcd % S; the
+        // other bits of b and the wave pick the (part, head).
+        const int64_t b = blockIdx.x;
+        const int xcd = int(b & 7), S = a.slices, groups = 8 / S;
+        slice = xcd % S;
+        const int64_t ph = ((b >> 3) * groups + xcd / S) * 4 + (threadIdx.x >> 6);
+        if (ph >= a.parts * a.heads) return;
+        part = ph / a.heads;
+        head = int(ph - part * a.heads);
+    } else {
+        const int64_t task = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+        if (task >= a.parts * a.heads * a.slices) return;
+        part = task / (int64_t(a.heads) * a.slices);
+        const int hs = int(task - part * a.heads * a.slices);
+        head = hs / a.slices;
+        slice = hs - head * a.slices;
+    }
 
     KB k{a};
     k.xh = a.x + int64_t(head) * a.x_head_stride * EB;
@@ -429,7 +447,12 @@ template <int KIND, bool MAPPED, int PIPE>
 hipError_t launch_pipe(int t, const SpmmArgs &a, hipStream_t s)
 {
     const int64_t tasks = a.parts * a.heads * a.slices;
-    const dim3 grid(unsigned((tasks + 3) / 4)), block(256);
+    int64_t blocks = (tasks + 3) / 4;
+    if (a.xcd_slices) {
+        const int64_t per_round = int64_t(8 / a.slices) * 4;   // (part, head) pairs per 8 workgroups
+        blocks = (a.parts * a.heads + per_round - 1) / per_round * 8;
+    }
+    const dim3 grid{static_cast<unsigned>(blocks)}, block{256};
     switch (t) {
     case 1: hipLaunchKernelGGL((spmm_block_kernel<KIND, 1, MAPPED, PIPE>), grid, block, 0, s, a); break;
     case 2: hipLaunchKernelGGL((spmm_block_kernel<KIND, 2, MAPPED, PIPE>), grid, block, 0, s, a); break;
@@ -527,7 +550,11 @@ extern "C" int attgnn_spmm_block(const int32_t *rp, const int32_t *col, const vo
     if (parts > 0 && workspace_elems < attgnn_spmm_workspace_elems(parts, window, dim_n)) return ATTGNN_E_ARG;
 
     const int eb = elt_bytes(kind);
-    const int t = pick_t(dim_n, eb);
+    int t = pick_t(dim_n, eb);
+    // requested features per lane (flags bits 8-11): a power of two dividing
+    // dim_n, at most the automatic choice
+    const int want = (flags >> 8) & 15;
+    if (want && want <= t && (want & (want - 1)) == 0 && dim_n % want == 0) t = want;
     if (reinterpret_cast<uintptr_t>(X) % (t * eb) != 0) return ATTGNN_E_ARG;
     if (reinterpret_cast<uintptr_t>(Y) % 16 != 0) return ATTGNN_E_ARG;
 
@@ -546,6 +573,7 @@ extern "C" int attgnn_spmm_block(const int32_t *rp, const int32_t *col, const vo
         const int sel = (flags >> 4) & 3;  // ATTGNN_SPMM_PIPE_*
         a.pipe = sel == 3 ? 0 : sel == 2 ? 2 : 1;
     }
+    a.xcd_slices = !(flags & ATTGNN_SPMM_NO_XCD_SLICES) && a.slices > 1 && 8 % a.slices == 0;
     const hipError_t e = run_spmm(kind, false, t, a, (flags & ATTGNN_SPMM_ROWS_COVERED) != 0, s);
     return e == hipSuccess ? ATTGNN_OK : int(e);
 }
@@ -589,6 +617,7 @@ extern "C" int attgnn_spmm_edge_map(const int32_t *rp, const int32_t *col, const
     else { a.y_head_stride = dim_n; a.y_row_stride = heads * dim_n; }
     a.slices = int((dim_n + 16 * t - 1) / (16 * t));
     a.heads = int(heads); a.window = window; a.wide_log2 = wl; a.y_type = ATTGNN_F32; a.pipe = 1;
+    a.xcd_slices = a.slices > 1 && 8 % a.slices == 0;
     const hipError_t e = run_spmm(kind, true, t, a, false, s);
     return e == hipSuccess ? ATTGNN_OK : int(e);
 }

@@ -144,6 +144,12 @@ enum {
     ATTGNN_SPMM_PIPE_DEFAULT = 0 << 4, /* column ids of the next block prefetched */
     ATTGNN_SPMM_PIPE_NONE = 3 << 4,    /* no prefetch                            */
     ATTGNN_SPMM_PIPE_PAIR = 2 << 4,    /* two blocks per iteration + prefetch    */
+    ATTGNN_SPMM_NO_XCD_SLICES = 1 << 6, /* by default, when the feature-slice count
+                                          divides 8, every XCD works on a single
+                                          slice so its L2 caches one slice of the
+                                          gathered rows; this flag turns it off  */
+    /* bits 8-11: features per lane (1, 2, 4 or 8; 0 = automatic), i.e. slices of
+       16x that many features; only narrower than the automatic choice        */
 };
 
 int64_t attgnn_spmm_workspace_elems(int64_t parts, int window, int64_t dim_n);
