@@ -26,6 +26,7 @@ struct attgnn_blocks {
     std::vector<int32_t> rp, col, map_t, win, atomic;
     std::vector<uint8_t> val;  // n_val elements of out_type
     int out_type = ATTGNN_F32;
+    int window = 0, wide = 0;
     int64_t n_val = 0, n_unique = 0;
     bool has_map_t = false;
 };
@@ -109,6 +110,8 @@ extern "C" int attgnn_blocks_build(const int32_t *row, int64_t rows_plus1, const
     attgnn_blocks *b = new (std::nothrow) attgnn_blocks();
     if (!b) return ATTGNN_E_NOMEM;
     b->out_type = out_type;
+    b->window = window;
+    b->wide = wide;
 
     try {
         // Pass 1: per window, the ascending unique column list, kept in place
@@ -275,3 +278,83 @@ extern "C" int attgnn_blocks_export(const attgnn_blocks *b, int32_t *rp, int32_t
 }
 
 extern "C" void attgnn_blocks_free(attgnn_blocks *b) { delete b; }
+
+// ---------------------------------------------------------------------------
+// Compact tiles: per condensed column a 16-bit mask of the window rows holding
+// an entry, and the entries' values packed per 32-column block, within a block
+// per 8-column group, within a group row-major (row, then column).  A lane that
+// owns window row n and one 8-column group then finds its values contiguous at
+// block_base[b] + (entries of earlier groups) + (entries of rows < n in its
+// group).  An entry is a tile value whose bits are not all zero.
+// ---------------------------------------------------------------------------
+namespace {
+
+inline bool tile_nonzero(const uint8_t *val, int64_t i, size_t es)
+{
+    const uint8_t *p = val + i * es;
+    for (size_t k = 0; k < es; ++k)
+        if (p[k]) return true;
+    return false;
+}
+
+}  // namespace
+
+extern "C" int attgnn_blocks_compact_sizes(const attgnn_blocks *b, int64_t *n_blocks, int64_t *n_values)
+{
+    if (!b || b->wide != 32 || b->window > 16 || b->out_type == ATTGNN_I32) return ATTGNN_E_ARG;
+    const int64_t C = static_cast<int64_t>(b->col.size());
+    const size_t es = type_size(b->out_type);
+    int64_t nv = 0;
+#pragma omp parallel for reduction(+ : nv) schedule(static)
+    for (int64_t i = 0; i < b->n_val; ++i) nv += tile_nonzero(b->val.data(), i, es);
+    if (n_blocks) *n_blocks = C / 32;
+    if (n_values) *n_values = nv;
+    return ATTGNN_OK;
+}
+
+extern "C" int attgnn_blocks_compact_export(const attgnn_blocks *b, uint16_t *masks, int64_t *block_base,
+                                            void *values)
+{
+    if (!b || b->wide != 32 || b->window > 16 || b->out_type == ATTGNN_I32) return ATTGNN_E_ARG;
+    if (!masks || !block_base || !values) return ATTGNN_E_ARG;
+    const int64_t C = static_cast<int64_t>(b->col.size());
+    const int64_t nb = C / 32;
+    const int W = b->window;
+    const size_t es = type_size(b->out_type);
+    const uint8_t *val = b->val.data();
+    std::vector<int64_t> cnt(nb + 1, 0);
+#pragma omp parallel for schedule(static)
+    for (int64_t blk = 0; blk < nb; ++blk) {
+        int64_t n = 0;
+        for (int r = 0; r < W; ++r)
+            for (int c = 0; c < 32; ++c) {
+                const bool nz = tile_nonzero(val, (blk * W + r) * 32 + c, es);
+                n += nz;
+            }
+        cnt[blk + 1] = n;
+        for (int c = 0; c < 32; ++c) {
+            uint16_t m = 0;
+            for (int r = 0; r < W; ++r)
+                if (tile_nonzero(val, (blk * W + r) * 32 + c, es)) m |= uint16_t(1u << r);
+            masks[blk * 32 + c] = m;
+        }
+    }
+    for (int64_t blk = 0; blk < nb; ++blk) cnt[blk + 1] += cnt[blk];
+    uint8_t *out = static_cast<uint8_t *>(values);
+#pragma omp parallel for schedule(static)
+    for (int64_t blk = 0; blk < nb; ++blk) {
+        block_base[blk] = cnt[blk];
+        int64_t o = cnt[blk];
+        for (int g = 0; g < 4; ++g)
+            for (int r = 0; r < W; ++r)
+                for (int c = 8 * g; c < 8 * g + 8; ++c) {
+                    const int64_t i = (blk * W + r) * 32 + c;
+                    if (tile_nonzero(val, i, es)) {
+                        std::memcpy(out + o * es, val + i * es, es);
+                        ++o;
+                    }
+                }
+    }
+    block_base[nb] = cnt[nb];
+    return ATTGNN_OK;
+}

@@ -64,6 +64,8 @@ struct SpmmArgs {
     const int32_t *rp, *col, *win, *atomic;
     const uint8_t *val;     // tiles, or per-edge values [heads][val_head_stride] (mapped)
     const int32_t *map;     // mapped variant: CSR position per tile slot, -1 = none
+    const uint16_t *masks;  // compact variant: row mask per condensed column
+    const int64_t *bbase;   // compact variant: value offset per 32-column block
     const uint8_t *x;       // X rows: element (h, c, f) at x[h*x_head_stride + c*x_row_stride + f]
     void *y;                // Y element (h, r, f) at y[h*y_head_stride + r*y_row_stride + f]
     float *partial;         // fp32 partial tiles of split windows
@@ -130,7 +132,9 @@ __device__ __forceinline__ void load_bytes_nt(const uint8_t *p, uint32_t *d)
 
 // Per-wave state and the three steps of one k-block (32 condensed columns):
 // column ids of the lane's 8 k slots, the A operand (tile row n), the X rows.
-template <int KIND, int T, bool MAPPED>
+enum AMode { A_DENSE = 0, A_MAPPED = 1, A_COMPACT = 2 };
+
+template <int KIND, int T, int AMODE>
 struct KBlock {
     static constexpr int EB = KindTraits<KIND>::EB;
     static constexpr int AW = (8 * EB) / 4;
@@ -156,7 +160,9 @@ struct KBlock {
 #pragma unroll
         for (int w = 0; w < AW; ++w) av[w] = 0;
         if (!a_row_ok || kg >= nreal) return;
-        if constexpr (MAPPED) {
+        if constexpr (AMODE == A_COMPACT) {
+            return;  // built by tile_compact, which needs the whole wave
+        } else if constexpr (AMODE == A_MAPPED) {
             // the 8 edge ids of row n (one or two vector loads), then the 8
             // value gathers issued together
             int m[8];
@@ -204,6 +210,55 @@ struct KBlock {
                 }
             }
         }
+    }
+
+    // Compact tiles: every lane of the wave must call this (shuffles).
+    __device__ __forceinline__ void tile_compact(int kb, uint32_t (&av)[AW]) const
+    {
+        const int kg = kb + 8 * g;
+#pragma unroll
+        for (int w = 0; w < AW; ++w) av[w] = 0;
+        uint32_t m[8];
+        int tot = 0, below = 0;
+        if (kg < plen) {
+            const int64_t cc = int64_t(c_begin) + kg;
+            const u32x4 mv = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(a.masks + cc));
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                m[i] = (mv[i >> 1] >> (16 * (i & 1))) & 0xffffu;
+                tot += __builtin_popcount(m[i]);
+                below += __builtin_popcount(m[i] & ((1u << n) - 1u));
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) m[i] = 0;
+        }
+        // entries of the earlier 8-column groups of this block
+        const int t0 = __shfl(tot, 0), t1 = __shfl(tot, 16), t2 = __shfl(tot, 32);
+        const int prefix = (g > 0 ? t0 : 0) + (g > 1 ? t1 : 0) + (g > 2 ? t2 : 0);
+        if (kg >= nreal) return;
+        int64_t idx = a.bbase[(int64_t(c_begin) + kb) >> 5] + prefix + below;
+        uint32_t e[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            e[i] = 0;
+            if ((m[i] >> n) & 1u) {
+                load_bytes<EB>(a.val + idx * EB, &e[i]);
+                ++idx;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if constexpr (EB == 4) av[i] = e[i];
+            else if constexpr (EB == 2) av[i >> 1] |= e[i] << (16 * (i & 1));
+            else av[i >> 2] |= e[i] << (8 * (i & 3));
+        }
+    }
+
+    __device__ __forceinline__ void load_a(int kb, uint32_t (&av)[AW]) const
+    {
+        if constexpr (AMODE == A_COMPACT) tile_compact(kb, av);
+        else tile(kb, av);
     }
 
     __device__ __forceinline__ void rows(const int (&cid)[8], uint32_t (&xv)[8][XW]) const
@@ -261,10 +316,10 @@ struct KBlock {
 // PIPE 0: each k-block loads its column ids, then its rows, then multiplies.
 // PIPE 1: the next k-block's column ids load while this block multiplies.
 // PIPE 2: two k-blocks per iteration, the next pair's column ids prefetched.
-template <int KIND, int T, bool MAPPED, int PIPE>
+template <int KIND, int T, int AMODE, int PIPE>
 __global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
 {
-    using KB = KBlock<KIND, T, MAPPED>;
+    using KB = KBlock<KIND, T, AMODE>;
     constexpr int EB = KB::EB, AW = KB::AW, XW = KB::XW;
 
     const int lane = threadIdx.x & 63;
@@ -317,7 +372,7 @@ __global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
             int cid[8];
             uint32_t av[AW], xv[8][XW];
             k.cols(kb, cid);
-            k.tile(kb, av);
+            k.load_a(kb, av);
             k.rows(cid, xv);
             KB::mma(av, xv, acc);
         }
@@ -327,7 +382,7 @@ __global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
         for (int kb = 0; kb < nreal; kb += 32) {
             uint32_t av[AW], xv[8][XW];
             int nid[8];
-            k.tile(kb, av);
+            k.load_a(kb, av);
             k.rows(cid, xv);
             k.cols(kb + 32, nid);
             KB::mma(av, xv, acc);
@@ -341,8 +396,8 @@ __global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
         for (int kb = 0; kb < nreal; kb += 64) {
             uint32_t av0[AW], av1[AW], xv0[8][XW], xv1[8][XW];
             int n0[8], n1[8];
-            k.tile(kb, av0);
-            k.tile(kb + 32, av1);
+            k.load_a(kb, av0);
+            k.load_a(kb + 32, av1);
             k.rows(c0, xv0);
             k.rows(c1, xv1);
             k.cols(kb + 64, n0);
@@ -443,7 +498,7 @@ int pick_t(int64_t dim_n, int eb)
     return t;
 }
 
-template <int KIND, bool MAPPED, int PIPE>
+template <int KIND, int AMODE, int PIPE>
 hipError_t launch_pipe(int t, const SpmmArgs &a, hipStream_t s)
 {
     const int64_t tasks = a.parts * a.heads * a.slices;
@@ -454,12 +509,12 @@ hipError_t launch_pipe(int t, const SpmmArgs &a, hipStream_t s)
     }
     const dim3 grid{static_cast<unsigned>(blocks)}, block{256};
     switch (t) {
-    case 1: hipLaunchKernelGGL((spmm_block_kernel<KIND, 1, MAPPED, PIPE>), grid, block, 0, s, a); break;
-    case 2: hipLaunchKernelGGL((spmm_block_kernel<KIND, 2, MAPPED, PIPE>), grid, block, 0, s, a); break;
-    case 4: hipLaunchKernelGGL((spmm_block_kernel<KIND, 4, MAPPED, PIPE>), grid, block, 0, s, a); break;
+    case 1: hipLaunchKernelGGL((spmm_block_kernel<KIND, 1, AMODE, PIPE>), grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((spmm_block_kernel<KIND, 2, AMODE, PIPE>), grid, block, 0, s, a); break;
+    case 4: hipLaunchKernelGGL((spmm_block_kernel<KIND, 4, AMODE, PIPE>), grid, block, 0, s, a); break;
     case 8:
         if constexpr (KindTraits<KIND>::EB <= 2) {
-            hipLaunchKernelGGL((spmm_block_kernel<KIND, 8, MAPPED, PIPE>), grid, block, 0, s, a);
+            hipLaunchKernelGGL((spmm_block_kernel<KIND, 8, AMODE, PIPE>), grid, block, 0, s, a);
             break;
         }
         return hipErrorInvalidValue;
@@ -468,13 +523,13 @@ hipError_t launch_pipe(int t, const SpmmArgs &a, hipStream_t s)
     return hipGetLastError();
 }
 
-template <int KIND, bool MAPPED>
+template <int KIND, int AMODE>
 hipError_t launch_main(int t, const SpmmArgs &a, hipStream_t s)
 {
     switch (a.pipe) {
-    case 0: return launch_pipe<KIND, MAPPED, 0>(t, a, s);
-    case 2: return launch_pipe<KIND, MAPPED, 2>(t, a, s);
-    default: return launch_pipe<KIND, MAPPED, 1>(t, a, s);
+    case 0: return launch_pipe<KIND, AMODE, 0>(t, a, s);
+    case 2: return launch_pipe<KIND, AMODE, 2>(t, a, s);
+    default: return launch_pipe<KIND, AMODE, 1>(t, a, s);
     }
 }
 
@@ -483,7 +538,7 @@ int y_bytes(int y_type) { return y_type == ATTGNN_F32 ? 4 : 2; }
 // Zero Y unless the caller guarantees every row is stored, run the main
 // kernel, then combine split windows.  Y is one contiguous buffer of
 // heads * m_ori * dim_n elements of y_type.
-hipError_t run_spmm(int kind, bool mapped, int t, const SpmmArgs &a, bool covered, hipStream_t s)
+hipError_t run_spmm(int kind, int amode, int t, const SpmmArgs &a, bool covered, hipStream_t s)
 {
     const int64_t nbytes = int64_t(a.heads) * a.m_ori * a.dim_n * y_bytes(a.y_type);
     if (nbytes > 0 && !covered) {
@@ -498,13 +553,18 @@ hipError_t run_spmm(int kind, bool mapped, int t, const SpmmArgs &a, bool covere
     }
     if (a.parts == 0) return hipSuccess;
     hipError_t e;
+#define ATTGNN_BY_AMODE(K)                                                           \
+    (amode == A_MAPPED ? launch_main<K, A_MAPPED>(t, a, s)                       \
+                       : amode == A_COMPACT ? launch_main<K, A_COMPACT>(t, a, s) \
+                                            : launch_main<K, A_DENSE>(t, a, s))
     switch (kind) {
-    case K_F16: e = mapped ? launch_main<K_F16, true>(t, a, s) : launch_main<K_F16, false>(t, a, s); break;
-    case K_BF16: e = mapped ? launch_main<K_BF16, true>(t, a, s) : launch_main<K_BF16, false>(t, a, s); break;
-    case K_FP8: e = launch_main<K_FP8, false>(t, a, s); break;
-    case K_BF16X3: e = mapped ? launch_main<K_BF16X3, true>(t, a, s) : launch_main<K_BF16X3, false>(t, a, s); break;
-    default: e = mapped ? launch_main<K_F32, true>(t, a, s) : launch_main<K_F32, false>(t, a, s); break;
+    case K_F16: e = ATTGNN_BY_AMODE(K_F16); break;
+    case K_BF16: e = ATTGNN_BY_AMODE(K_BF16); break;
+    case K_FP8: e = amode == A_COMPACT ? launch_main<K_FP8, A_COMPACT>(t, a, s) : launch_main<K_FP8, A_DENSE>(t, a, s); break;
+    case K_BF16X3: e = ATTGNN_BY_AMODE(K_BF16X3); break;
+    default: e = ATTGNN_BY_AMODE(K_F32); break;
     }
+#undef ATTGNN_BY_AMODE
     if (e != hipSuccess) return e;
     const int64_t cblocks = std::min<int64_t>(a.parts, 2048);
     hipLaunchKernelGGL(spmm_combine_kernel, dim3(unsigned(cblocks), unsigned(a.heads)), dim3(256), 0, s, a);
@@ -574,7 +634,7 @@ extern "C" int attgnn_spmm_block(const int32_t *rp, const int32_t *col, const vo
         a.pipe = sel == 3 ? 0 : sel == 2 ? 2 : 1;
     }
     a.xcd_slices = !(flags & ATTGNN_SPMM_NO_XCD_SLICES) && a.slices > 1 && 8 % a.slices == 0;
-    const hipError_t e = run_spmm(kind, false, t, a, (flags & ATTGNN_SPMM_ROWS_COVERED) != 0, s);
+    const hipError_t e = run_spmm(kind, A_DENSE, t, a, (flags & ATTGNN_SPMM_ROWS_COVERED) != 0, s);
     return e == hipSuccess ? ATTGNN_OK : int(e);
 }
 
@@ -618,6 +678,54 @@ extern "C" int attgnn_spmm_edge_map(const int32_t *rp, const int32_t *col, const
     a.slices = int((dim_n + 16 * t - 1) / (16 * t));
     a.heads = int(heads); a.window = window; a.wide_log2 = wl; a.y_type = ATTGNN_F32; a.pipe = 1;
     a.xcd_slices = a.slices > 1 && 8 % a.slices == 0;
-    const hipError_t e = run_spmm(kind, true, t, a, false, s);
+    const hipError_t e = run_spmm(kind, A_MAPPED, t, a, false, s);
+    return e == hipSuccess ? ATTGNN_OK : int(e);
+}
+
+extern "C" int attgnn_spmm_compact(const int32_t *rp, const int32_t *col, const uint16_t *masks,
+                                   const int64_t *block_base, const void *values, int tile_type,
+                                   const int32_t *window_of_part, const int32_t *atomic, int64_t parts,
+                                   const void *X, int x_type, int64_t n_x, void *Y, int y_type, int64_t m_ori,
+                                   int64_t dim_n, int window, int mode, const float *row_scale,
+                                   const float *col_scale, float *workspace, int64_t workspace_elems, int flags,
+                                   attgnn_stream_t stream)
+{
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (parts < 0 || m_ori < 0 || dim_n <= 0 || n_x < 0) return ATTGNN_E_ARG;
+    if (!(window == 8 || window == 16)) return ATTGNN_E_ARG;
+    const int kind = kind_of(mode, tile_type, x_type);
+    if (kind < 0) return ATTGNN_E_UNSUPPORTED;
+    if (kind == K_FP8 && (!row_scale || !col_scale)) return ATTGNN_E_ARG;
+    if (!Y || !(y_type == ATTGNN_F32 || y_type == ATTGNN_F16 || y_type == ATTGNN_BF16)) return ATTGNN_E_ARG;
+    if (parts > 0 && (!rp || !col || !masks || !block_base || !values || !window_of_part || !atomic || !X ||
+                      !workspace))
+        return ATTGNN_E_ARG;
+    if (parts > 0 && workspace_elems < attgnn_spmm_workspace_elems(parts, window, dim_n)) return ATTGNN_E_ARG;
+    if (reinterpret_cast<uintptr_t>(masks) % 16 != 0) return ATTGNN_E_ARG;
+    const int eb = elt_bytes(kind);
+    int t = pick_t(dim_n, eb);
+    const int want = (flags >> 8) & 15;
+    if (want && want <= t && (want & (want - 1)) == 0 && dim_n % want == 0) t = want;
+    if (reinterpret_cast<uintptr_t>(X) % (t * eb) != 0) return ATTGNN_E_ARG;
+    if (reinterpret_cast<uintptr_t>(Y) % 16 != 0) return ATTGNN_E_ARG;
+
+    SpmmArgs a{};
+    a.rp = rp; a.col = col; a.win = window_of_part; a.atomic = atomic;
+    a.val = static_cast<const uint8_t *>(values); a.map = nullptr;
+    a.masks = masks; a.bbase = block_base;
+    a.x = static_cast<const uint8_t *>(X);
+    a.y = Y; a.partial = workspace;
+    a.row_scale = row_scale; a.col_scale = col_scale;
+    a.parts = parts; a.m_ori = m_ori; a.dim_n = dim_n;
+    a.x_head_stride = 0; a.x_row_stride = dim_n; a.y_head_stride = 0; a.y_row_stride = dim_n;
+    a.val_head_stride = 0;
+    a.slices = int((dim_n + 16 * t - 1) / (16 * t));
+    a.heads = 1; a.window = window; a.wide_log2 = 5; a.y_type = y_type;
+    {
+        const int sel = (flags >> 4) & 3;
+        a.pipe = sel == 3 ? 0 : sel == 2 ? 2 : 1;
+    }
+    a.xcd_slices = !(flags & ATTGNN_SPMM_NO_XCD_SLICES) && a.slices > 1 && 8 % a.slices == 0;
+    const hipError_t e = run_spmm(kind, A_COMPACT, t, a, (flags & ATTGNN_SPMM_ROWS_COVERED) != 0, s);
     return e == hipSuccess ? ATTGNN_OK : int(e);
 }

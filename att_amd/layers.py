@@ -62,8 +62,11 @@ class GCNGraph:
     """
 
     def __init__(self, csr, device, window=16, wide=32, part_size=None, tile_dtype=torch.float16,
-                 symmetric=None, values=None):
+                 symmetric=None, values=None, compact="auto"):
         self.window, self.wide = window, wide
+        # compact tiles win once a gathered row is >= 256 bytes; "auto" keeps
+        # both forms and picks per call
+        self.compact = compact if wide == 32 else False
         self.num_nodes_ori = csr.n
         padded = csr.padded(window)
         self.num_nodes = padded.n
@@ -89,16 +92,28 @@ class GCNGraph:
 
     def _build(self, csr, values, device):
         b = ops.blocks_build(csr.rowptr32(), csr.indices, "edge_f32", np.asarray(values, np.float32),
-                             self.tile_dtype, self.window, self.wide, self.part_size, emit_empty=True)
-        arrays = [b[k].to(device) for k in ("rp", "col", "val", "window", "atomic")]
+                             self.tile_dtype, self.window, self.wide, self.part_size, emit_empty=True,
+                             compact=self.compact)
+        keys = ("rp", "col", "val", "window", "atomic")
+        if self.compact is True:   # compact only: the dense tiles are not uploaded
+            b["val"] = torch.zeros(1, dtype=self.tile_dtype)
+        arrays = [b[k].to(device) for k in keys]
         self.n_unique = b["n_unique"]
-        return {"arrays": arrays, "parts": int(b["window"].numel()), "n_cols": int(b["col"].numel())}
+        out = {"arrays": arrays, "parts": int(b["window"].numel()), "n_cols": int(b["col"].numel())}
+        if self.compact:
+            out["compact"] = [b[k].to(device) for k in ("masks", "block_base", "values")]
+        return out
 
     def spmm(self, X, transpose=False, out_dtype=None):
         s = self.bwd if transpose else self.fwd
         rp, col, val, win, at = s["arrays"]
         if X.shape[0] < self.num_nodes_ori:
             raise ValueError("X has fewer rows than the graph")
+        use_compact = self.compact is True or (self.compact == "auto" and X.shape[1] * X.element_size() >= 256)
+        if use_compact:
+            masks, base, values = s["compact"]
+            return ops.spmm_compact(rp, col, masks, base, values, win, at, X.contiguous(), self.num_nodes_ori,
+                                    self.window, out_dtype=out_dtype or X.dtype, rows_covered=True)
         return ops.spmm_block(rp, col, val, win, at, X.contiguous(), self.num_nodes_ori, self.window, self.wide,
                               out_dtype=out_dtype or X.dtype, rows_covered=True)
 
@@ -124,6 +139,17 @@ class MGCNFunction(torch.autograd.Function):
         return d_input, d_weights, None
 
 
+def spmm_width(d, elt_bytes, line=128):
+    """Feature width to run the SpMM at: rows that neither divide nor fill whole
+    128-byte lines straddle lines on every other gather, so they are padded up
+    to the next multiple of a line (e.g. 48 fp16 features -> 64)."""
+    row = d * elt_bytes
+    if row % line == 0 or line % row == 0:
+        return d
+    per_line = line // elt_bytes
+    return -(-d // per_line) * per_line
+
+
 class GCNConv(nn.Module):
     """end2end_ori/gcn_final/mgat_csr/mgat_conv.py:82-108."""
 
@@ -138,7 +164,12 @@ class GCNConv(nn.Module):
         self.weights.data.uniform_(-stdv, stdv)
 
     def forward(self, X, inputInfo):
-        return MGCNFunction.apply(X, self.weights.to(self.dtype), inputInfo)
+        W = self.weights.to(self.dtype)
+        out = W.shape[1]
+        wide = spmm_width(out, W.element_size())
+        if wide != out:   # zero columns so gathered rows are whole cache lines
+            return MGCNFunction.apply(X, F.pad(W, (0, wide - out)), inputInfo)[:, :out]
+        return MGCNFunction.apply(X, W, inputInfo)
 
 
 # ----------------------------------------------------------------------------

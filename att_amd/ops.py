@@ -68,7 +68,7 @@ def _need_i32(name, *tensors):
 # --------------------------------------------------------------------------
 
 def blocks_build(row, col, src, src_values, out_dtype, window, wide, part_size,
-                 emit_empty, want_trans=False, nthreads=0):
+                 emit_empty, want_trans=False, nthreads=0, compact=False):
     """Window-block structure of a CSR matrix (host, OpenMP).
 
     ``src``: "deg_f16" | "deg_f32" | "edge_f32" | "map".  Returns a dict with
@@ -118,6 +118,15 @@ def blocks_build(row, col, src, src_values, out_dtype, window, wide, part_size,
                                          _ptr(out["map_t"]), _ptr(out["window"]), _ptr(out["atomic"])),
               "attgnn_blocks_export")
         out["n_unique"] = n_unique
+        if compact:
+            nb, nv = ctypes.c_int64(), ctypes.c_int64()
+            check(lib().attgnn_blocks_compact_sizes(handle, ctypes.byref(nb), ctypes.byref(nv)),
+                  "attgnn_blocks_compact_sizes")
+            out["masks"] = torch.empty(max(n_col, 8), dtype=torch.int16)
+            out["block_base"] = torch.empty(nb.value + 1, dtype=torch.int64)
+            out["values"] = torch.empty(max(nv.value, 1), dtype=out_dtype)
+            check(lib().attgnn_blocks_compact_export(handle, _ptr(out["masks"]), _ptr(out["block_base"]),
+                                                     _ptr(out["values"])), "attgnn_blocks_compact_export")
     finally:
         lib().attgnn_blocks_free(handle)
     return out
@@ -158,6 +167,28 @@ def spmm_block(rp, col, val, window_of_part, atomic, X, m_ori, window, wide, mod
                                  m_ori, dim, window, wide, _mode(mode, X.dtype), _ptr(row_scale), _ptr(col_scale), _ptr(ws),
                                  ws_n, flags, _stream(dev))
     check(rc, "attgnn_spmm_block")
+    return y
+
+
+def spmm_compact(rp, col, masks, block_base, values, window_of_part, atomic, X, m_ori, window, mode=None,
+                 row_scale=None, col_scale=None, out_dtype=torch.float32, rows_covered=False, workspace=None):
+    """attgnn_spmm_compact: spmm_block over compact tiles (wide = 32)."""
+    dev = _need_gpu("spmm_compact", rp, col, masks, block_base, values, window_of_part, atomic, X, row_scale,
+                    col_scale)
+    _need_i32("spmm_compact", rp, col, window_of_part, atomic)
+    parts = window_of_part.numel()
+    n_x, dim = X.shape
+    y = torch.empty((m_ori, dim), dtype=out_dtype, device=dev)
+    ws_n = lib().attgnn_spmm_workspace_elems(parts, window, dim) if parts else 0
+    ws = workspace if workspace is not None and workspace.numel() >= ws_n else \
+        torch.empty(max(ws_n, 1), dtype=torch.float32, device=dev)
+    flags = _lib.SPMM_ROWS_COVERED if rows_covered else 0
+    rc = lib().attgnn_spmm_compact(_ptr(rp), _ptr(col), _ptr(masks), _ptr(block_base), _ptr(values),
+                                   _TYPE_OF[values.dtype], _ptr(window_of_part), _ptr(atomic), parts, _ptr(X),
+                                   _TYPE_OF[X.dtype], n_x, _ptr(y), _TYPE_OF[y.dtype], m_ori, dim, window,
+                                   _mode(mode, X.dtype), _ptr(row_scale), _ptr(col_scale), _ptr(ws), ws_n, flags,
+                                   _stream(dev))
+    check(rc, "attgnn_spmm_compact")
     return y
 
 

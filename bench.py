@@ -2,7 +2,9 @@
 
 Workload (one GPU): Chung-Lu graph (gamma 2.1, seed 0) with N = 232,965 nodes
 and ~114.6M entries after symmetrisation and self-loops; A_hat = D^-1/2 (A+I)
-D^-1/2 stored as FP16 window tiles (16-row windows, 32-column blocks);
+D^-1/2 stored as FP16 window tiles (16-row windows, 32-column blocks; the
+width-256 products read the compact row-mask form, the width-48 products run
+at width 64 so gathered rows are whole 128-byte lines);
 X 232,965 x 602 FP16 ~ N(0,1); Glorot weights 602 -> 256 -> 41 (padded to 48);
 ReLU, dropout 0.5, log-softmax, NLL loss on random labels, Adam step.
 
@@ -235,7 +237,7 @@ def main():
                    "tiles": "fp16", "accumulate": "fp32", "scale": args.scale},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "window-block SpMM, width 256, fp16 (main + split-window combine)",
+                     "kernel": "window-block SpMM, width 256, fp16, compact tiles (main + split-window combine)",
                      "ms_per_launch": round(t_spmm, 4), "algorithmic_bytes": int(algo_bytes),
                      "gflops": round(2.0 * g.nnz * F_HID / (t_spmm * 1e-3) / 1e9, 1)},
         "cpu_baseline": cpu,

@@ -106,6 +106,20 @@ int attgnn_blocks_export(const attgnn_blocks *b, int32_t *rp, int32_t *col, void
 
 void attgnn_blocks_free(attgnn_blocks *b);
 
+/*
+ * Compact tiles of a value structure built with wide = 32 and window <= 16:
+ * masks[C] (uint16, bit r = window row r has an entry in that condensed
+ * column), block_base[C/32 + 1] (int64 offsets of each 32-column block's
+ * values) and values[n_values] (tile type) packed per block, per 8-column
+ * group, row-major within a group.  An entry is a tile value whose bits are
+ * not all zero.
+ *
+ * No counterpart in the reference, which stores dense window x wide tiles
+ * (gcn/Block_gcn/example.cpp:84-106); the reference format stays available.
+ */
+int attgnn_blocks_compact_sizes(const attgnn_blocks *b, int64_t *n_blocks, int64_t *n_values);
+int attgnn_blocks_compact_export(const attgnn_blocks *b, uint16_t *masks, int64_t *block_base, void *values);
+
 /* ====================================================================== */
 /* Window-block SpMM (device)                                              */
 /* ====================================================================== */
@@ -153,6 +167,23 @@ enum {
 };
 
 int64_t attgnn_spmm_workspace_elems(int64_t parts, int window, int64_t dim_n);
+
+/*
+ * attgnn_spmm_block over compact tiles (attgnn_blocks_compact_export: row masks,
+ * block offsets, packed values) of a structure built with wide = 32; same
+ * arguments otherwise, same results bit for bit (the multiply-adds are the
+ * same, absent entries contribute exact zeros in both).
+ *
+ * Replaces SpMM_sr_bcrs.forward_*_gnn (gcn/SpMM/src/benchmark.cpp:35-130) on
+ * a smaller tile format; no reference counterpart for the format itself.
+ */
+int attgnn_spmm_compact(const int32_t *rp, const int32_t *col, const uint16_t *masks,
+                        const int64_t *block_base, const void *values, int tile_type,
+                        const int32_t *window_of_part, const int32_t *atomic, int64_t parts,
+                        const void *X, int x_type, int64_t n_x, void *Y, int y_type, int64_t m_ori,
+                        int64_t dim_n, int window, int mode, const float *row_scale,
+                        const float *col_scale, float *workspace, int64_t workspace_elems, int flags,
+                        attgnn_stream_t stream);
 
 /*
  * Edge-map SpMM: Y_h = A_h . X_h with A_h[i, j] = edge_val[h*nnz + map[slot(i,j)]],

@@ -52,3 +52,25 @@ def test_modes_deterministic(cuda, band):
     for mode in BOUNDS:
         A = BlockMatrix(g, g.values, cuda, mode, part_size=1)
         assert torch.equal(A.matmul(Xt), A.matmul(Xt))
+
+
+def test_compact_tiles_bitwise_equal_dense(cuda):
+    """The compact-tile SpMM multiplies exactly the same numbers as the dense one."""
+    from att_amd import ops
+    rng = np.random.default_rng(3)
+    g = G.chung_lu(3000, 14, 2.1, seed=2)
+    v = rng.standard_normal(g.nnz).astype(np.float32)
+    for dtype, mode in ((torch.float16, None), (torch.bfloat16, None), (torch.float32, "bf16x3"),
+                        (torch.float32, "f32")):
+        for ps in (2, 512):
+            p = g.padded(16)
+            b = ops.blocks_build(p.rowptr32(), p.indices, "edge_f32", v, dtype, 16, 32, ps, emit_empty=True,
+                                 compact=True)
+            d = {k: b[k].to(cuda) for k in ("rp", "col", "val", "window", "atomic", "masks", "block_base", "values")}
+            for dim in (256, 48, 64, 20):
+                X = torch.from_numpy(rng.uniform(-1, 1, (g.n, dim)).astype(np.float32)).to(dtype).to(cuda)
+                y0 = ops.spmm_block(d["rp"], d["col"], d["val"], d["window"], d["atomic"], X, g.n, 16, 32, mode=mode,
+                                    rows_covered=True)
+                y1 = ops.spmm_compact(d["rp"], d["col"], d["masks"], d["block_base"], d["values"], d["window"],
+                                      d["atomic"], X, g.n, 16, mode=mode, rows_covered=True)
+                assert torch.equal(y0, y1), (dtype, mode, ps, dim)

@@ -104,3 +104,38 @@ def test_malformed_input_raises():
         ops.blocks_build([0, 1, 2], [0, 1], "deg_f32", np.ones(2), torch.float32, 0, 8, 2, emit_empty=True)
     with pytest.raises(AttGnnError):
         ops.blocks_build([0, 1, 3], [0, 1], "deg_f32", np.ones(2), torch.float32, 8, 8, 2, emit_empty=True)
+
+
+def _expand_compact(masks, base, values, C, W):
+    """Dense [C*W] tile array from compact tiles (test-side restatement of the format)."""
+    masks = masks.view(np.uint16) if masks.dtype != np.uint16 else masks
+    dense = np.zeros(C * W, values.dtype)
+    for b in range(C // 32):
+        idx = int(base[b])
+        for g in range(4):
+            for r in range(W):
+                for c in range(8 * g, 8 * g + 8):
+                    if (int(masks[b * 32 + c]) >> r) & 1:
+                        dense[(b * W + r) * 32 + c] = values[idx]
+                        idx += 1
+        assert idx == base[b + 1]
+    return dense
+
+
+@pytest.mark.parametrize("window,dtype", [(16, torch.float16), (16, torch.float32), (8, torch.bfloat16)])
+def test_compact_tiles_expand_to_dense(window, dtype):
+    rng = np.random.default_rng(21)
+    n = 300
+    rows = [np.unique(rng.integers(0, n, size=int(rng.integers(0, 25)))) for _ in range(n)]
+    rp = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
+    col = np.concatenate(rows).astype(np.int32)
+    v = rng.standard_normal(col.size).astype(np.float32)
+    r = ops.blocks_build(rp, col, "edge_f32", v, dtype, window, 32, 3, emit_empty=True, compact=True)
+    C = r["col"].numel()
+    dense = _expand_compact(r["masks"].numpy().view(np.uint16), r["block_base"].numpy(),
+                            r["values"].numpy() if dtype != torch.bfloat16 else r["values"].view(torch.int16).numpy(),
+                            C, window)
+    want = r["val"].numpy() if dtype != torch.bfloat16 else r["val"].view(torch.int16).numpy()
+    assert np.array_equal(dense, want)
+    # one packed value per CSR entry of the covered rows (values are nonzero a.s.)
+    assert int(r["block_base"][-1]) == rp[n // window * window]
