@@ -51,6 +51,23 @@ def main():
     X = torch.randn(g.n, 128, generator=torch.Generator().manual_seed(1)).to(dev)
     Xb = X.bfloat16()
     dim = 64
+    # SDDMM alone: out[h, e=(i,j)] = <A[i,h], B[j,h]> in BF16, fp32 out.
+    # Algorithmic bytes: 4(M+1) + nnz*(4+4) + M*D*H*2 + U*D*H*2 (U: distinct
+    # columns summed over 16-row windows), flops 2*nnz*D*H.
+    from att_amd import ops
+    rp, col, emap, win, _ = graph.fwd
+    for heads in [int(h) for h in args.heads.split(",")]:
+        A = torch.randn(g.n, heads, dim, device=dev).bfloat16()
+        B = torch.randn(g.n, heads, dim, device=dev).bfloat16()
+        f = lambda: ops.sddmm_block(rp, col, emap, win, A, B, g.nnz, graph.window, graph.wide, layout="mhd",
+                                    out_dtype=torch.float32)
+        ms = timeit(f, args.reps)
+        M = g.n
+        nbytes = 4 * (M + 1) + g.nnz * 8 + M * dim * heads * 2 + graph.n_unique * dim * heads * 2
+        print(json.dumps({"op": "sddmm", "heads": heads, "dim": dim, "nnz": g.nnz, "ms": round(ms, 4),
+                          "gflops": round(2.0 * g.nnz * dim * heads / ms / 1e6, 1),
+                          "algorithmic_gbs": round(nbytes / ms / 1e6, 1),
+                          "hbm_frac": round(nbytes / ms / 1e6 / 8000.0, 3)}), flush=True)
     for kind in ("gat", "gatv2"):
         for heads in [int(h) for h in args.heads.split(",")]:
             torch.manual_seed(heads)
