@@ -33,7 +33,9 @@ _SIGNATURES = {
     "attgnn_spmm_compact": (_int, [_vp, _vp, _vp, _vp, _vp, _int, _vp, _vp, _i64, _vp, _int, _i64, _vp, _int, _i64,
                                    _i64, _int, _int, _vp, _vp, _vp, _i64, _int, _vp]),
     "attgnn_blocks_compact_sizes": (_int, [_vp, _c.POINTER(_i64), _c.POINTER(_i64)]),
-    "attgnn_blocks_compact_export": (_int, [_vp, _vp, _vp, _vp]),
+    "attgnn_blocks_compact_export": (_int, [_vp, _vp, _vp, _vp, _vp]),
+    "attgnn_sddmm_edges": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _int, _i64,
+                                  _i64, _i64, _vp, _int, _i64, _int, _int, _vp]),
     "attgnn_spmm_edge_map": (_int, [_vp, _vp, _vp, _vp, _int, _i64, _vp, _vp, _i64, _vp, _int, _int, _i64, _i64,
                                     _vp, _i64, _i64, _int, _int, _int, _int, _vp, _i64, _vp]),
     "attgnn_rowsum_edge_map": (_int, [_vp, _vp, _vp, _i64, _vp, _int, _i64, _i64, _vp, _int, _i64, _int, _int,

@@ -25,9 +25,7 @@ def _emap(graph, transpose, values, X, n_out):
 
 
 def _sddmm(graph, A, B):
-    rp, col, emap, win, _ = graph.fwd
-    return ops.sddmm_block(rp, col, emap, win, A.contiguous(), B.contiguous(), graph.num_edges, graph.window,
-                           graph.wide, layout="mhd", out_dtype=torch.float32)
+    return graph.sddmm(A, B)
 
 
 class SpMM(torch.autograd.Function):

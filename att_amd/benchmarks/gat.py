@@ -59,15 +59,17 @@ def main():
     for heads in [int(h) for h in args.heads.split(",")]:
         A = torch.randn(g.n, heads, dim, device=dev).bfloat16()
         B = torch.randn(g.n, heads, dim, device=dev).bfloat16()
-        f = lambda: ops.sddmm_block(rp, col, emap, win, A, B, g.nnz, graph.window, graph.wide, layout="mhd",
-                                    out_dtype=torch.float32)
-        ms = timeit(f, args.reps)
-        M = g.n
-        nbytes = 4 * (M + 1) + g.nnz * 8 + M * dim * heads * 2 + graph.n_unique * dim * heads * 2
-        print(json.dumps({"op": "sddmm", "heads": heads, "dim": dim, "nnz": g.nnz, "ms": round(ms, 4),
-                          "gflops": round(2.0 * g.nnz * dim * heads / ms / 1e6, 1),
-                          "algorithmic_gbs": round(nbytes / ms / 1e6, 1),
-                          "hbm_frac": round(nbytes / ms / 1e6 / 8000.0, 3)}), flush=True)
+        variants = {"dense_map": lambda: ops.sddmm_block(rp, col, emap, win, A, B, g.nnz, graph.window, graph.wide,
+                                                         layout="mhd", out_dtype=torch.float32),
+                    "edge_list": lambda: graph.sddmm(A, B)}
+        for name, f in variants.items():
+            ms = timeit(f, args.reps)
+            M = g.n
+            nbytes = 4 * (M + 1) + g.nnz * 8 + M * dim * heads * 2 + graph.n_unique * dim * heads * 2
+            print(json.dumps({"op": "sddmm", "variant": name, "heads": heads, "dim": dim, "nnz": g.nnz,
+                              "ms": round(ms, 4), "gflops": round(2.0 * g.nnz * dim * heads / ms / 1e6, 1),
+                              "algorithmic_gbs": round(nbytes / ms / 1e6, 1),
+                              "hbm_frac": round(nbytes / ms / 1e6 / 8000.0, 3)}), flush=True)
     for kind in ("gat", "gatv2"):
         for heads in [int(h) for h in args.heads.split(",")]:
             torch.manual_seed(heads)

@@ -289,8 +289,11 @@ extern "C" void attgnn_blocks_free(attgnn_blocks *b) { delete b; }
 // ---------------------------------------------------------------------------
 namespace {
 
-inline bool tile_nonzero(const uint8_t *val, int64_t i, size_t es)
+// Entry test: a value tile entry is any nonzero bit pattern; an edge-id map
+// entry is an id >= 0 (id 0 is a real edge).
+inline bool tile_nonzero(const uint8_t *val, int64_t i, size_t es, bool is_map)
 {
+    if (is_map) return reinterpret_cast<const int32_t *>(val)[i] >= 0;
     const uint8_t *p = val + i * es;
     for (size_t k = 0; k < es; ++k)
         if (p[k]) return true;
@@ -301,26 +304,28 @@ inline bool tile_nonzero(const uint8_t *val, int64_t i, size_t es)
 
 extern "C" int attgnn_blocks_compact_sizes(const attgnn_blocks *b, int64_t *n_blocks, int64_t *n_values)
 {
-    if (!b || b->wide != 32 || b->window > 16 || b->out_type == ATTGNN_I32) return ATTGNN_E_ARG;
+    if (!b || b->wide != 32 || b->window > 16) return ATTGNN_E_ARG;
     const int64_t C = static_cast<int64_t>(b->col.size());
     const size_t es = type_size(b->out_type);
+    const bool is_map = b->out_type == ATTGNN_I32;
     int64_t nv = 0;
 #pragma omp parallel for reduction(+ : nv) schedule(static)
-    for (int64_t i = 0; i < b->n_val; ++i) nv += tile_nonzero(b->val.data(), i, es);
+    for (int64_t i = 0; i < b->n_val; ++i) nv += tile_nonzero(b->val.data(), i, es, is_map);
     if (n_blocks) *n_blocks = C / 32;
     if (n_values) *n_values = nv;
     return ATTGNN_OK;
 }
 
 extern "C" int attgnn_blocks_compact_export(const attgnn_blocks *b, uint16_t *masks, int64_t *block_base,
-                                            void *values)
+                                            void *values, uint16_t *positions)
 {
-    if (!b || b->wide != 32 || b->window > 16 || b->out_type == ATTGNN_I32) return ATTGNN_E_ARG;
+    if (!b || b->wide != 32 || b->window > 16) return ATTGNN_E_ARG;
     if (!masks || !block_base || !values) return ATTGNN_E_ARG;
     const int64_t C = static_cast<int64_t>(b->col.size());
     const int64_t nb = C / 32;
     const int W = b->window;
     const size_t es = type_size(b->out_type);
+    const bool is_map = b->out_type == ATTGNN_I32;
     const uint8_t *val = b->val.data();
     std::vector<int64_t> cnt(nb + 1, 0);
 #pragma omp parallel for schedule(static)
@@ -328,14 +333,14 @@ extern "C" int attgnn_blocks_compact_export(const attgnn_blocks *b, uint16_t *ma
         int64_t n = 0;
         for (int r = 0; r < W; ++r)
             for (int c = 0; c < 32; ++c) {
-                const bool nz = tile_nonzero(val, (blk * W + r) * 32 + c, es);
+                const bool nz = tile_nonzero(val, (blk * W + r) * 32 + c, es, is_map);
                 n += nz;
             }
         cnt[blk + 1] = n;
         for (int c = 0; c < 32; ++c) {
             uint16_t m = 0;
             for (int r = 0; r < W; ++r)
-                if (tile_nonzero(val, (blk * W + r) * 32 + c, es)) m |= uint16_t(1u << r);
+                if (tile_nonzero(val, (blk * W + r) * 32 + c, es, is_map)) m |= uint16_t(1u << r);
             masks[blk * 32 + c] = m;
         }
     }
@@ -349,8 +354,9 @@ extern "C" int attgnn_blocks_compact_export(const attgnn_blocks *b, uint16_t *ma
             for (int r = 0; r < W; ++r)
                 for (int c = 8 * g; c < 8 * g + 8; ++c) {
                     const int64_t i = (blk * W + r) * 32 + c;
-                    if (tile_nonzero(val, i, es)) {
+                    if (tile_nonzero(val, i, es, is_map)) {
                         std::memcpy(out + o * es, val + i * es, es);
+                        if (positions) positions[o] = uint16_t(r * 32 + c);
                         ++o;
                     }
                 }

@@ -61,6 +61,108 @@ struct SddmmArgs {
     int heads, window, wide_log2, in_type, out_type;
 };
 
+// The 16x32 tile C[r][16t+n] = <A[row r], B[col cid[t]]> over all features, for
+// one head: lane (g, n) holds C[4g+q][16t+n] in acc[t][q].
+template <int KIND, bool VEC>
+__device__ __forceinline__ void sddmm_tile(const SddmmArgs &a, int head, const uint8_t *arow, bool row_ok,
+                                           const int (&cid)[2], int g, f32x4 (&acc)[2])
+{
+    constexpr int EB = KIND >= 2 ? 4 : 2;
+    acc[0] = f32x4{0.f, 0.f, 0.f, 0.f};
+    acc[1] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int64_t d0 = 0; d0 < a.dim; d0 += 32) {
+        const int64_t d = d0 + 8 * g;
+        // 8 features d..d+7 of the A row and of the two B rows
+        float av[8], bv[2][8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) { av[i] = 0.f; bv[0][i] = 0.f; bv[1][i] = 0.f; }
+        uint32_t araw[4] = {0, 0, 0, 0}, braw[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+        if constexpr (VEC && EB == 2) {
+            if (d < a.dim) {
+                if (row_ok) {
+                    const u32x4 v = *reinterpret_cast<const u32x4 *>(arow + d * 2);
+                    araw[0] = v[0]; araw[1] = v[1]; araw[2] = v[2]; araw[3] = v[3];
+                }
+#pragma unroll
+                for (int t = 0; t < 2; ++t)
+                    if (cid[t] >= 0) {
+                        const u32x4 v = *reinterpret_cast<const u32x4 *>(
+                            a.b + (int64_t(head) * a.b_hs + int64_t(cid[t]) * a.b_rs + d) * 2);
+                        braw[t][0] = v[0]; braw[t][1] = v[1]; braw[t][2] = v[2]; braw[t][3] = v[3];
+                    }
+            }
+        } else if constexpr (VEC && EB == 4) {
+            if (d < a.dim) {
+                if (row_ok)
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) av[i] = reinterpret_cast<const float *>(arow)[d + i];
+#pragma unroll
+                for (int t = 0; t < 2; ++t)
+                    if (cid[t] >= 0) {
+                        const float *br = reinterpret_cast<const float *>(
+                            a.b + (int64_t(head) * a.b_hs + int64_t(cid[t]) * a.b_rs) * 4);
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) bv[t][i] = br[d + i];
+                    }
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                if (d + i < a.dim) {
+                    if (row_ok) av[i] = load_elt(arow + (d + i) * EB, a.in_type);
+#pragma unroll
+                    for (int t = 0; t < 2; ++t)
+                        if (cid[t] >= 0)
+                            bv[t][i] = load_elt(a.b + (int64_t(head) * a.b_hs + int64_t(cid[t]) * a.b_rs + d + i) * EB,
+                                                a.in_type);
+                }
+            }
+            if constexpr (EB == 2) {  // repack to 16-bit operands
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    auto enc = [](float x) -> uint32_t {
+                        if constexpr (KIND == 0) return __builtin_bit_cast(uint16_t, _Float16(x));
+                        else return bf16_bits_rne(x);
+                    };
+                    araw[q] = enc(av[2 * q]) | enc(av[2 * q + 1]) << 16;
+#pragma unroll
+                    for (int t = 0; t < 2; ++t) braw[t][q] = enc(bv[t][2 * q]) | enc(bv[t][2 * q + 1]) << 16;
+                }
+            }
+        }
+        if constexpr (KIND == 0 || KIND == 1) {
+            const u32x4 A = {araw[0], araw[1], araw[2], araw[3]};
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const u32x4 B = {braw[t][0], braw[t][1], braw[t][2], braw[t][3]};
+                acc[t] = KIND == 0 ? mfma_f16(A, B, acc[t]) : mfma_bf16(A, B, acc[t]);
+            }
+        } else if constexpr (KIND == 3) {
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int i = 0; i < 8; ++i) acc[t] = mfma_f32(av[i], bv[t][i], acc[t]);
+        } else {
+            uint32_t ah[8], al[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) bf16_split(av[i], ah[i], al[i]);
+            const u32x4 AH = {ah[0] | ah[1] << 16, ah[2] | ah[3] << 16, ah[4] | ah[5] << 16, ah[6] | ah[7] << 16};
+            const u32x4 AL = {al[0] | al[1] << 16, al[2] | al[3] << 16, al[4] | al[5] << 16, al[6] | al[7] << 16};
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                uint32_t bh[8], bl[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) bf16_split(bv[t][i], bh[i], bl[i]);
+                const u32x4 BH = {bh[0] | bh[1] << 16, bh[2] | bh[3] << 16, bh[4] | bh[5] << 16, bh[6] | bh[7] << 16};
+                const u32x4 BL = {bl[0] | bl[1] << 16, bl[2] | bl[3] << 16, bl[4] | bl[5] << 16, bl[6] | bl[7] << 16};
+                acc[t] = mfma_bf16(AL, BH, acc[t]);
+                acc[t] = mfma_bf16(AH, BL, acc[t]);
+                acc[t] = mfma_bf16(AH, BH, acc[t]);
+            }
+        }
+    }
+}
+
 // VEC: rows are 16-byte aligned runs of 8 (16-bit) or 4 (fp32) elements and
 // dim is a multiple of 8, so each lane loads its 8 features with vector loads.
 // KIND: 0 = f16, 1 = bf16, 2 = fp32 via bf16x3, 3 = fp32 MFMA.
@@ -83,104 +185,14 @@ __global__ __launch_bounds__(256) void sddmm_block_kernel(SddmmArgs a)
     const uint8_t *arow = a.a + (int64_t(head) * a.a_hs + row * a.a_rs) * EB;
 
     for (int kb = 0; kb < nreal; kb += 32) {
-        f32x4 acc[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
+        f32x4 acc[2];
         int cid[2];
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
             const int k = kb + 16 * t + n;
             cid[t] = k < nreal ? a.col[c_begin + k] : -1;
         }
-        for (int64_t d0 = 0; d0 < a.dim; d0 += 32) {
-            const int64_t d = d0 + 8 * g;
-            // 8 features d..d+7 of the A row and of the two B rows
-            float av[8], bv[2][8];
-#pragma unroll
-            for (int i = 0; i < 8; ++i) { av[i] = 0.f; bv[0][i] = 0.f; bv[1][i] = 0.f; }
-            uint32_t araw[4] = {0, 0, 0, 0}, braw[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
-            if constexpr (VEC && EB == 2) {
-                if (d < a.dim) {
-                    if (row_ok) {
-                        const u32x4 v = *reinterpret_cast<const u32x4 *>(arow + d * 2);
-                        araw[0] = v[0]; araw[1] = v[1]; araw[2] = v[2]; araw[3] = v[3];
-                    }
-#pragma unroll
-                    for (int t = 0; t < 2; ++t)
-                        if (cid[t] >= 0) {
-                            const u32x4 v = *reinterpret_cast<const u32x4 *>(
-                                a.b + (int64_t(head) * a.b_hs + int64_t(cid[t]) * a.b_rs + d) * 2);
-                            braw[t][0] = v[0]; braw[t][1] = v[1]; braw[t][2] = v[2]; braw[t][3] = v[3];
-                        }
-                }
-            } else if constexpr (VEC && EB == 4) {
-                if (d < a.dim) {
-                    if (row_ok)
-#pragma unroll
-                        for (int i = 0; i < 8; ++i) av[i] = reinterpret_cast<const float *>(arow)[d + i];
-#pragma unroll
-                    for (int t = 0; t < 2; ++t)
-                        if (cid[t] >= 0) {
-                            const float *br = reinterpret_cast<const float *>(
-                                a.b + (int64_t(head) * a.b_hs + int64_t(cid[t]) * a.b_rs) * 4);
-#pragma unroll
-                            for (int i = 0; i < 8; ++i) bv[t][i] = br[d + i];
-                        }
-                }
-            } else {
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    if (d + i < a.dim) {
-                        if (row_ok) av[i] = load_elt(arow + (d + i) * EB, a.in_type);
-#pragma unroll
-                        for (int t = 0; t < 2; ++t)
-                            if (cid[t] >= 0)
-                                bv[t][i] = load_elt(a.b + (int64_t(head) * a.b_hs + int64_t(cid[t]) * a.b_rs + d + i) * EB,
-                                                    a.in_type);
-                    }
-                }
-                if constexpr (EB == 2) {  // repack to 16-bit operands
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        auto enc = [](float x) -> uint32_t {
-                            if constexpr (KIND == 0) return __builtin_bit_cast(uint16_t, _Float16(x));
-                            else return bf16_bits_rne(x);
-                        };
-                        araw[q] = enc(av[2 * q]) | enc(av[2 * q + 1]) << 16;
-#pragma unroll
-                        for (int t = 0; t < 2; ++t) braw[t][q] = enc(bv[t][2 * q]) | enc(bv[t][2 * q + 1]) << 16;
-                    }
-                }
-            }
-            if constexpr (KIND == 0 || KIND == 1) {
-                const u32x4 A = {araw[0], araw[1], araw[2], araw[3]};
-#pragma unroll
-                for (int t = 0; t < 2; ++t) {
-                    const u32x4 B = {braw[t][0], braw[t][1], braw[t][2], braw[t][3]};
-                    acc[t] = KIND == 0 ? mfma_f16(A, B, acc[t]) : mfma_bf16(A, B, acc[t]);
-                }
-            } else if constexpr (KIND == 3) {
-#pragma unroll
-                for (int t = 0; t < 2; ++t)
-#pragma unroll
-                    for (int i = 0; i < 8; ++i) acc[t] = mfma_f32(av[i], bv[t][i], acc[t]);
-            } else {
-                uint32_t ah[8], al[8];
-#pragma unroll
-                for (int i = 0; i < 8; ++i) bf16_split(av[i], ah[i], al[i]);
-                const u32x4 AH = {ah[0] | ah[1] << 16, ah[2] | ah[3] << 16, ah[4] | ah[5] << 16, ah[6] | ah[7] << 16};
-                const u32x4 AL = {al[0] | al[1] << 16, al[2] | al[3] << 16, al[4] | al[5] << 16, al[6] | al[7] << 16};
-#pragma unroll
-                for (int t = 0; t < 2; ++t) {
-                    uint32_t bh[8], bl[8];
-#pragma unroll
-                    for (int i = 0; i < 8; ++i) bf16_split(bv[t][i], bh[i], bl[i]);
-                    const u32x4 BH = {bh[0] | bh[1] << 16, bh[2] | bh[3] << 16, bh[4] | bh[5] << 16, bh[6] | bh[7] << 16};
-                    const u32x4 BL = {bl[0] | bl[1] << 16, bl[2] | bl[3] << 16, bl[4] | bl[5] << 16, bl[6] | bl[7] << 16};
-                    acc[t] = mfma_bf16(AL, BH, acc[t]);
-                    acc[t] = mfma_bf16(AH, BL, acc[t]);
-                    acc[t] = mfma_bf16(AH, BH, acc[t]);
-                }
-            }
-        }
+        sddmm_tile<KIND, VEC>(a, head, arow, row_ok, cid, g, acc);
         // scatter C[4g+q][16t+n] to its edge
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
@@ -194,6 +206,71 @@ __global__ __launch_bounds__(256) void sddmm_block_kernel(SddmmArgs a)
                 const int e = a.map[(cc >> a.wide_log2) * W * wide + int64_t(r) * wide + (cc & (wide - 1))];
                 if (e >= 0) store_elt(a.out + (int64_t(head) * a.nnz + e) * type_bytes(a.out_type), a.out_type, acc[t][q]);
             }
+        }
+    }
+}
+
+// SDDMM over compact edge lists: one wave per part, every head in the same
+// wave.  Per 32-column block the wave computes the 16x32 tile with MFMA (all
+// rows of the window against the block's 32 gathered columns), stages it in a
+// wave-private 2 KB LDS tile and writes only the block's real edges, listed as
+// (packed position r*32+c, edge id) -- the dense edge-id map is never read.
+// Column ids and the edge list are loaded once per block for all heads.
+template <int KIND, bool VEC>
+__global__ __launch_bounds__(256) void sddmm_edges_kernel(SddmmArgs a, const int64_t *bbase, const int32_t *eids,
+                                                          const uint16_t *epos)
+{
+    constexpr int EB = KIND >= 2 ? 4 : 2;
+    __shared__ float tile[4][16 * 32];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t part = int64_t(blockIdx.x) * 4 + wave;
+    if (part >= a.parts) return;
+    const int c_begin = a.rp[2 * part];
+    const int nreal = a.rp[2 * part + 1] - c_begin;
+    const int g = lane >> 4, n = lane & 15;
+    const int W = a.window;
+    const int64_t row = int64_t(a.win[part]) * W + n;
+    const bool row_ok = n < W && row < a.n_rows;
+    float *S = tile[wave];
+    const int ob = type_bytes(a.out_type);
+
+    for (int kb = 0; kb < nreal; kb += 32) {
+        const int64_t blk = (int64_t(c_begin) + kb) >> 5;
+        const int64_t e0 = bbase[blk];
+        const int cnt = int(bbase[blk + 1] - e0);
+        if (cnt == 0) continue;
+        int cid[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int k = kb + 16 * t + n;
+            cid[t] = k < nreal ? a.col[c_begin + k] : -1;
+        }
+        // this lane's first edge of the block (most blocks have < 64 edges)
+        int my_pos = -1, my_e = -1;
+        if (lane < cnt) {
+            my_pos = epos[e0 + lane];
+            my_e = eids[e0 + lane];
+        }
+        for (int h = 0; h < a.heads; ++h) {
+            const uint8_t *arow = a.a + (int64_t(h) * a.a_hs + row * a.a_rs) * EB;
+            f32x4 acc[2];
+            sddmm_tile<KIND, VEC>(a, h, arow, row_ok, cid, g, acc);
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) S[(4 * g + q) * 32 + 16 * t + n] = acc[t][q];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (my_e >= 0) store_elt(a.out + (int64_t(h) * a.nnz + my_e) * ob, a.out_type, S[my_pos]);
+            for (int j = 64 + lane; j < cnt; j += 64) {
+                const int e = eids[e0 + j];
+                store_elt(a.out + (int64_t(h) * a.nnz + e) * ob, a.out_type, S[epos[e0 + j]]);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
     }
 }
@@ -424,6 +501,51 @@ extern "C" int attgnn_rowsum_edge_map(const int32_t *rp, const int32_t *map, con
     hipLaunchKernelGGL(rowsum_map_kernel, dim3(unsigned(parts)), dim3(256), 0, s, rp, map, window_of_part, parts,
                        static_cast<const uint8_t *>(edge_val), val_type, nnz, int(heads), window, wl, out, hs, rs,
                        m_rows);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? ATTGNN_OK : int(e);
+}
+
+extern "C" int attgnn_sddmm_edges(const int32_t *rp, const int32_t *col, const int64_t *block_base,
+                                  const int32_t *edge_ids, const uint16_t *positions,
+                                  const int32_t *window_of_part, int64_t parts, const void *A,
+                                  int64_t a_head_stride, int64_t a_row_stride, const void *B,
+                                  int64_t b_head_stride, int64_t b_row_stride, int in_type, int64_t heads,
+                                  int64_t n_rows, int64_t dim, void *out, int out_type, int64_t nnz, int window,
+                                  int mode, attgnn_stream_t stream)
+{
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (parts < 0 || heads <= 0 || n_rows < 0 || dim <= 0 || nnz < 0) return ATTGNN_E_ARG;
+    if (!(window == 8 || window == 16)) return ATTGNN_E_ARG;
+    if (!is_float_type(in_type) || !is_float_type(out_type)) return ATTGNN_E_ARG;
+    if (parts == 0) return ATTGNN_OK;
+    if (!rp || !col || !block_base || !edge_ids || !positions || !window_of_part || !A || !B || !out)
+        return ATTGNN_E_ARG;
+    int kind;
+    if (in_type == ATTGNN_F16) kind = 0;
+    else if (in_type == ATTGNN_BF16) kind = 1;
+    else kind = mode == ATTGNN_MODE_F32 ? 3 : 2;
+    const int eb = type_bytes(in_type);
+    const bool vec = dim % 8 == 0 && a_row_stride % 8 == 0 && b_row_stride % 8 == 0 && a_head_stride % 8 == 0 &&
+                     b_head_stride % 8 == 0 && reinterpret_cast<uintptr_t>(A) % 16 == 0 &&
+                     reinterpret_cast<uintptr_t>(B) % 16 == 0 && (eb == 2 || eb == 4);
+    SddmmArgs a{rp, col, nullptr, window_of_part, static_cast<const uint8_t *>(A), static_cast<const uint8_t *>(B),
+                static_cast<uint8_t *>(out), parts, nnz, n_rows, dim, a_head_stride, a_row_stride,
+                b_head_stride, b_row_stride, int(heads), window, 5, in_type, out_type};
+    const dim3 grid{static_cast<unsigned>((parts + 3) / 4)}, block{256};
+#define ATTGNN_EDGES_CASE(K)                                                                               \
+    case K:                                                                                                \
+        if (vec) hipLaunchKernelGGL((sddmm_edges_kernel<K, true>), grid, block, 0, s, a, block_base,       \
+                                    edge_ids, positions);                                                  \
+        else hipLaunchKernelGGL((sddmm_edges_kernel<K, false>), grid, block, 0, s, a, block_base, edge_ids, \
+                                positions);                                                                \
+        break;
+    switch (kind) {
+        ATTGNN_EDGES_CASE(0)
+        ATTGNN_EDGES_CASE(1)
+        ATTGNN_EDGES_CASE(2)
+        ATTGNN_EDGES_CASE(3)
+    }
+#undef ATTGNN_EDGES_CASE
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? ATTGNN_OK : int(e);
 }

@@ -193,9 +193,11 @@ class GATGraph:
         self.rowptr = torch.from_numpy(csr.rowptr32()).to(device)
         self.col = torch.from_numpy(csr.indices).to(device)
         b = ops.blocks_build(padded.rowptr32(), padded.indices, "map", None, torch.int32, window, wide,
-                             self.part_size, emit_empty=True)
+                             self.part_size, emit_empty=True, compact=(wide == 32))
         self.n_unique = b["n_unique"]
         self.fwd = [b[k].to(device) for k in ("rp", "col", "val", "window", "atomic")]
+        # compact edge list of A (block offsets, edge ids, packed positions) for the SDDMM
+        self.edges = [b[k].to(device) for k in ("block_base", "values", "positions")] if wide == 32 else None
         t, order = transpose_csr(padded)
         bt = ops.blocks_build(t.rowptr32(), t.indices, "map", None, torch.int32, window, wide, self.part_size,
                               emit_empty=True)
@@ -208,6 +210,16 @@ class GATGraph:
         # reference names
         self.row_pointers, self.column_index, self.degrees, self.window_t, self.automic = self.fwd
         self.degrees_trans = self.bwd[2]
+
+    def sddmm(self, A, B, out_dtype=torch.float32):
+        """out[h, e=(i,j)] = <A[i,h], B[j,h]> over A's edges ([N, H, D] inputs, [H, nnz] out)."""
+        rp, col, emap, win, _ = self.fwd
+        if self.edges is not None:
+            base, eids, pos = self.edges
+            return ops.sddmm_edges(rp, col, base, eids, pos, win, A.contiguous(), B.contiguous(), self.num_edges,
+                                   self.window, layout="mhd", out_dtype=out_dtype, covered=True)
+        return ops.sddmm_block(rp, col, emap, win, A.contiguous(), B.contiguous(), self.num_edges, self.window,
+                               self.wide, layout="mhd", out_dtype=out_dtype)
 
 
 class GATAttention(torch.autograd.Function):
@@ -242,9 +254,7 @@ class GATAttention(torch.autograd.Function):
         dXt = ops.spmm_edge_map(rp, col, emap_t, p, win, at, dh, "mhd", g.window, g.wide, m_ori=n,
                                 x_layout="mhd")
         # dp[(i,j)] = <dh[i], Xt[j]>  (SDDMM over A)
-        frp, fcol, emap, fwin, _ = g.fwd
-        dp = ops.sddmm_block(frp, fcol, emap, fwin, dh, Xt, g.num_edges, g.window, g.wide, layout="mhd",
-                             out_dtype=torch.float32)
+        dp = g.sddmm(dh, Xt)
         dz, ds0 = ops.edge_softmax_bwd(g.rowptr, g.col, s0, s1, lse, dp, ctx.slope, ctx.drop_p, ctx.seed)
         ds1 = ops.gather_rowsum(g.t_rowptr, g.t_perm, dz, n)
         Xs32, Xt32 = Xs.float(), Xt.float()

@@ -125,8 +125,10 @@ def blocks_build(row, col, src, src_values, out_dtype, window, wide, part_size,
             out["masks"] = torch.empty(max(n_col, 8), dtype=torch.int16)
             out["block_base"] = torch.empty(nb.value + 1, dtype=torch.int64)
             out["values"] = torch.empty(max(nv.value, 1), dtype=out_dtype)
+            out["positions"] = torch.empty(max(nv.value, 1), dtype=torch.int16)
             check(lib().attgnn_blocks_compact_export(handle, _ptr(out["masks"]), _ptr(out["block_base"]),
-                                                     _ptr(out["values"])), "attgnn_blocks_compact_export")
+                                                     _ptr(out["values"]), _ptr(out["positions"])),
+                  "attgnn_blocks_compact_export")
     finally:
         lib().attgnn_blocks_free(handle)
     return out
@@ -276,6 +278,28 @@ def sddmm_block(rp, col, emap, window_of_part, A, B, nnz, window, wide, layout="
                                   _TYPE_OF[A.dtype], ha, na, da, _ptr(out), _TYPE_OF[out_dtype], nnz, window, wide,
                                   _mode(mode, A.dtype), _stream(dev))
     check(rc, "attgnn_sddmm_block")
+    return out
+
+
+def sddmm_edges(rp, col, block_base, edge_ids, positions, window_of_part, A, B, nnz, window, layout="hmd",
+                out_dtype=None, mode=None, covered=False):
+    """sddmm_block over a compact edge list (wide = 32); [H, nnz].  ``covered``:
+    every edge lies in a window of the structure, so the output needs no zero-fill."""
+    dev = _need_gpu("sddmm_edges", rp, col, block_base, edge_ids, positions, window_of_part, A, B)
+    _need_i32("sddmm_edges", rp, col, edge_ids, window_of_part)
+    if A.dtype != B.dtype:
+        raise TypeError("A and B must share a dtype")
+    ha, na, da, ahs, ars = _hmd_strides(A, layout)
+    hb, nb, db, bhs, brs = _hmd_strides(B, layout)
+    if ha != hb or da != db:
+        raise ValueError("A and B disagree in heads or dim")
+    out_dtype = out_dtype or A.dtype
+    out = (torch.empty if covered else torch.zeros)((ha, nnz), dtype=out_dtype, device=dev)
+    rc = lib().attgnn_sddmm_edges(_ptr(rp), _ptr(col), _ptr(block_base), _ptr(edge_ids), _ptr(positions),
+                                  _ptr(window_of_part), window_of_part.numel(), _ptr(A), ahs, ars, _ptr(B), bhs, brs,
+                                  _TYPE_OF[A.dtype], ha, na, da, _ptr(out), _TYPE_OF[out_dtype], nnz, window,
+                                  _mode(mode, A.dtype), _stream(dev))
+    check(rc, "attgnn_sddmm_edges")
     return out
 
 

@@ -107,18 +107,20 @@ int attgnn_blocks_export(const attgnn_blocks *b, int32_t *rp, int32_t *col, void
 void attgnn_blocks_free(attgnn_blocks *b);
 
 /*
- * Compact tiles of a value structure built with wide = 32 and window <= 16:
+ * Compact tiles of a structure built with wide = 32 and window <= 16:
  * masks[C] (uint16, bit r = window row r has an entry in that condensed
  * column), block_base[C/32 + 1] (int64 offsets of each 32-column block's
- * values) and values[n_values] (tile type) packed per block, per 8-column
- * group, row-major within a group.  An entry is a tile value whose bits are
- * not all zero.
+ * values), values[n_values] (tile type, or int32 edge ids for a map) packed
+ * per block, per 8-column group, row-major within a group, and optionally
+ * positions[n_values] (uint16 r*32 + c of each entry in its block).  An entry
+ * is a tile value whose bits are not all zero, or a map id >= 0.
  *
  * No counterpart in the reference, which stores dense window x wide tiles
  * (gcn/Block_gcn/example.cpp:84-106); the reference format stays available.
  */
 int attgnn_blocks_compact_sizes(const attgnn_blocks *b, int64_t *n_blocks, int64_t *n_values);
-int attgnn_blocks_compact_export(const attgnn_blocks *b, uint16_t *masks, int64_t *block_base, void *values);
+int attgnn_blocks_compact_export(const attgnn_blocks *b, uint16_t *masks, int64_t *block_base, void *values,
+                                 uint16_t *positions);
 
 /* ====================================================================== */
 /* Window-block SpMM (device)                                              */
@@ -245,6 +247,21 @@ int attgnn_sddmm_block(const int32_t *rp, const int32_t *col, const int32_t *map
                        int64_t b_head_stride, int64_t b_row_stride, int in_type,
                        int64_t heads, int64_t n_rows, int64_t dim, void *out, int out_type,
                        int64_t nnz, int window, int wide, int mode, attgnn_stream_t stream);
+
+/*
+ * attgnn_sddmm_block over a compact edge list (attgnn_blocks_compact_export of
+ * an edge-id map built with wide = 32: block_base, edge ids, positions): all
+ * heads in one pass, the dense map is not read.  Same results.
+ *
+ * Replaces ATT_SDDMM_v{1,3}.sddmm_{fp16,tf32}_h_nnz (see attgnn_sddmm_block).
+ */
+int attgnn_sddmm_edges(const int32_t *rp, const int32_t *col, const int64_t *block_base,
+                       const int32_t *edge_ids, const uint16_t *positions,
+                       const int32_t *window_of_part, int64_t parts, const void *A,
+                       int64_t a_head_stride, int64_t a_row_stride, const void *B,
+                       int64_t b_head_stride, int64_t b_row_stride, int in_type, int64_t heads,
+                       int64_t n_rows, int64_t dim, void *out, int out_type, int64_t nnz, int window,
+                       int mode, attgnn_stream_t stream);
 
 /*
  * scores[h*n_rows + i] = sum_d X[h*x_head_stride + i*x_row_stride + d] * vec[h*dim + d]

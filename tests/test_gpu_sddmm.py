@@ -137,3 +137,22 @@ def test_edge_map_spmm_random_vs_oracle(cuda, heads, dim):
     rref = O.rowsum(rp, ev)
     rref[:, n // 16 * 16:] = 0
     assert O.rel_fro(rs.cpu().numpy(), rref) < 1e-6
+
+
+@pytest.mark.parametrize("heads,dim", [(1, 64), (4, 64), (8, 32), (2, 40), (3, 13)])
+@pytest.mark.parametrize("dt", [torch.bfloat16, torch.float16, torch.float32])
+def test_sddmm_edge_list_equals_dense_map(cuda, heads, dim, dt):
+    """The compact edge-list SDDMM writes exactly the dense-map SDDMM's values."""
+    rng = np.random.default_rng(heads * 7 + dim)
+    n = 700
+    rp, col = rand_csr(rng, n, 40, hub_rows=3)
+    for window in (16, 8):
+        b = ops.blocks_build(rp, col, "map", None, torch.int32, window, 32, 2, emit_empty=True, compact=True)
+        d = {k: b[k].to(cuda) for k in ("rp", "col", "val", "window", "block_base", "values", "positions")}
+        A = torch.from_numpy(rng.standard_normal((n, heads, dim)).astype(np.float32)).to(dt).to(cuda)
+        B = torch.from_numpy(rng.standard_normal((n, heads, dim)).astype(np.float32)).to(dt).to(cuda)
+        ref = ops.sddmm_block(d["rp"], d["col"], d["val"], d["window"], A, B, col.size, window, 32, layout="mhd",
+                              out_dtype=torch.float32)
+        got = ops.sddmm_edges(d["rp"], d["col"], d["block_base"], d["values"], d["positions"], d["window"], A, B,
+                              col.size, window, layout="mhd", out_dtype=torch.float32)
+        assert torch.equal(ref, got), (window, dt)
