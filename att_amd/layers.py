@@ -189,7 +189,9 @@ class GATGraph:
         padded = csr.padded(window)
         self.num_nodes = padded.n
         self.num_edges = csr.nnz
-        self.part_size = part_size or default_part_size(_blocks_estimate(padded, wide))
+        # at least 32 blocks per part: the heads multiply the partial-tile
+        # traffic of split windows
+        self.part_size = part_size or max(default_part_size(_blocks_estimate(padded, wide)), 32)
         self.rowptr = torch.from_numpy(csr.rowptr32()).to(device)
         self.col = torch.from_numpy(csr.indices).to(device)
         b = ops.blocks_build(padded.rowptr32(), padded.indices, "map", None, torch.int32, window, wide,
