@@ -50,6 +50,12 @@ _SIGNATURES = {
     "attgnn_edge_softmax_bwd": (_int, [_vp, _vp, _i64, _i64, _i64, _vp, _vp, _i64, _i64, _vp, _vp, _int,
                                        _c.c_float, _c.c_float, _c.c_uint64, _vp, _vp, _vp]),
     "attgnn_gather_rowsum": (_int, [_vp, _vp, _i64, _i64, _i64, _vp, _vp, _vp]),
+    "attgnn_blocks_device_workspace_bytes": (_i64, [_i64, _i64, _int]),
+    "attgnn_blocks_plan_device": (_int, [_vp, _i64, _vp, _i64, _int, _int, _int, _int, _vp, _i64, _vp, _vp]),
+    "attgnn_blocks_fill_device": (_int, [_vp, _i64, _vp, _i64, _int, _vp, _int, _int, _int, _int, _vp, _i64, _vp,
+                                         _vp, _vp, _vp, _vp, _vp, _vp]),
+    "attgnn_blocks_compact_count_device": (_int, [_vp, _int, _i64, _int, _vp, _vp, _vp]),
+    "attgnn_blocks_compact_fill_device": (_int, [_vp, _int, _i64, _int, _vp, _vp, _vp, _vp, _vp]),
     "attgnn_pack_segments": (_int, [_vp, _i64, _vp]),
     "attgnn_pack_segment_bytes": (_i64, []),
 }

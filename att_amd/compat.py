@@ -90,6 +90,34 @@ def blockProcess_sddmm_balance_gnn_trans(row, column, window, wide, partSize):
     return [r["rp"], r["col"], r["val"], r["map_t"], r["window"], r["atomic"]]
 
 
+# ---------------------------------------------------------------- ATT_Block_gpu
+# Block_gpu/setup.py:10-14 (module sources not in the reference tree); caller
+# end2end_overhead/gat_final/mgat_csr_ov/mdataset_fp16.py:53-58, which passes
+# the CPU CSR of the padded graph and reads back seven values, the last being
+# the preprocessing time it reports (test_mgat_csr_ov.py:22-27).
+
+def preprocess_gpu_fs(row, column, num_nodes, num_edges, window, wide, partSize):
+    """Edge-id window blocks built on the current GPU: ``[rp, col, map, map_T,
+    window, automic, pre]`` -- the outputs of
+    ``blockProcess_sddmm_balance_gnn_trans`` as device tensors, and ``pre``,
+    the wall time of the build in milliseconds (upload of row / column
+    included, as the caller hands over CPU tensors)."""
+    import time
+    dev = torch.device("cuda", torch.cuda.current_device())
+    if int(num_nodes) + 1 != row.numel() or int(num_edges) != column.numel():
+        raise ValueError(f"num_nodes={num_nodes} / num_edges={num_edges} disagree with row ({row.numel()}) "
+                         f"and column ({column.numel()})")
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    r_d = row.to(device=dev, dtype=torch.int32).contiguous()
+    c_d = column.to(device=dev, dtype=torch.int32).contiguous()
+    r = ops.blocks_build_device(r_d, c_d, "map", None, torch.int32, int(window), int(wide), int(partSize),
+                                emit_empty=False, want_trans=True)
+    torch.cuda.synchronize(dev)
+    pre = (time.perf_counter() - t0) * 1e3
+    return [r["rp"], r["col"], r["val"], r["map_t"], r["window"], r["atomic"], pre]
+
+
 # ---------------------------------------------------------------- ATT_SpMM_v*
 # callers end2end_head/gat_final/mgat_csr/mgat_conv.py:42-59,79-86,99-106,126-132;
 # tests benchmark-v3/spmm-benchmark/test/spmm_{fp16,tf32}/*.py
@@ -218,6 +246,10 @@ def _make_modules():
         m.blockProcess_sddmm_balance_gnn = blockProcess_sddmm_balance_gnn
         m.blockProcess_sddmm_balance_gnn_trans = blockProcess_sddmm_balance_gnn_trans
         mods[name] = m
+
+    m = types.ModuleType("ATT_Block_gpu", "window-block preprocessing on the GPU")
+    m.preprocess_gpu_fs = preprocess_gpu_fs
+    mods["ATT_Block_gpu"] = m
 
     for ver in (1, 3, 4):
         m = types.ModuleType(f"ATT_SpMM_v{ver}", "edge-map window-block SpMM")

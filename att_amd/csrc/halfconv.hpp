@@ -1,15 +1,21 @@
-// Host-side IEEE half / bfloat16 conversions (round to nearest even).
+// IEEE half / bfloat16 conversions (round to nearest even), written with
+// integer operations so that host code and device code produce the same bits.
 #pragma once
 #include <cstdint>
-#include <cstring>
+
+#if defined(__HIPCC__)
+#define ATTGNN_HD __host__ __device__
+#else
+#define ATTGNN_HD
+#endif
 
 namespace attgnn {
 
-inline float bits_f32(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
-inline uint32_t f32_bits(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
+ATTGNN_HD inline float bits_f32(uint32_t u) { return __builtin_bit_cast(float, u); }
+ATTGNN_HD inline uint32_t f32_bits(float f) { return __builtin_bit_cast(uint32_t, f); }
 
 // binary16 -> binary32, exact.
-inline float f16_to_f32(uint16_t h)
+ATTGNN_HD inline float f16_to_f32(uint16_t h)
 {
     const uint32_t s = uint32_t(h & 0x8000u) << 16;
     const uint32_t e = (h >> 10) & 0x1fu;
@@ -23,7 +29,7 @@ inline float f16_to_f32(uint16_t h)
 }
 
 // binary32 -> binary16, round to nearest even; NaN -> quiet NaN.
-inline uint16_t f32_to_f16(float f)
+ATTGNN_HD inline uint16_t f32_to_f16(float f)
 {
     const uint32_t x = f32_bits(f);
     const uint16_t s = uint16_t((x >> 16) & 0x8000u);
@@ -43,13 +49,13 @@ inline uint16_t f32_to_f16(float f)
 }
 
 // binary32 -> bfloat16, round to nearest even; NaN stays NaN.
-inline uint16_t f32_to_bf16(float f)
+ATTGNN_HD inline uint16_t f32_to_bf16(float f)
 {
     const uint32_t x = f32_bits(f);
     if ((x & 0x7fffffffu) > 0x7f800000u) return uint16_t((x >> 16) | 0x40u);
     return uint16_t((x + 0x7fffu + ((x >> 16) & 1u)) >> 16);
 }
 
-inline float bf16_to_f32(uint16_t b) { return bits_f32(uint32_t(b) << 16); }
+ATTGNN_HD inline float bf16_to_f32(uint16_t b) { return bits_f32(uint32_t(b) << 16); }
 
 }  // namespace attgnn

@@ -1,0 +1,687 @@
+// Window-block preprocessing on the device (gfx950).
+//
+// Reference counterpart: ATT_Block_gpu.preprocess_gpu_fs (built from
+// Block_gpu/DTCSpMM.cpp + DTCSpMM_kernel.cu, Block_gpu/setup.py:10-14; those
+// sources are not in the reference tree; caller
+// end2end_overhead/gat_final/mgat_csr_ov/mdataset_fp16.py:53-58), which returns
+// the structure of blockProcess_sddmm_balance_gnn_trans, and the host windowing
+// of gcn/Block_gcn/example.cpp:48-348.  The result is bit-identical to
+// attgnn_blocks_build (host) for every source, window, wide, part size and
+// empty-window rule, including unsorted rows and duplicate entries.
+//
+// Pipeline (one stream, no host round trip until the caller needs sizes):
+//  plan  1. rows_check   row pointers monotone, row[0] = 0, row[rows] = nnz;
+//        2. cols_check   one wave per row: column ids in range, and a flag for
+//                        rows whose columns are not ascending;
+//        3. window_unique one workgroup per window: its distinct columns in
+//                        ascending order and, for every entry, the rank of its
+//                        column among them -- a bitonic sort in LDS for small
+//                        windows, or a bitmap of the window's column span in LDS
+//                        with per-word prefix counts (O(1) rank) for large ones;
+//        4. window_scan  one workgroup: prefix sums of padded column counts and
+//                        part counts over windows, totals for the caller;
+//  fill  5. window_fill  one workgroup per window: condensed columns, tile
+//                        initialisation and scatter, parts;
+//  compact 6. count (row masks + entry counts per 32-column block) + scan,
+//          7. pack (values and positions in block / 8-column group / row order).
+#include <hip/hip_runtime.h>
+
+#include <climits>
+
+#include "attgnn.h"
+#include "halfconv.hpp"
+
+namespace {
+
+constexpr int THREADS = 256;
+constexpr int SORT_CAP = 4096;   // windows up to this many entries may be sorted in LDS
+constexpr int BM_WORDS = 4096;   // bitmap chunk: 131,072 columns per pass
+constexpr int ST_FORMAT = 1, ST_UNSUPPORTED = 2;
+
+// Workspace carve-up (256-byte aligned sections).
+struct Ws {
+    int32_t *status;   // [1]  bit flags ST_*
+    int32_t *nuniq;    // [nwin]
+    int64_t *ncol;     // [nwin + 1] exclusive prefix of padded column counts
+    int64_t *nparts;   // [nwin + 1] exclusive prefix of part counts
+    int32_t *rowflag;  // [rows] 1 = columns of the row not ascending
+    int32_t *rank;     // [nnz] rank of the entry's column among its window's distinct columns
+    int32_t *uniq;     // [nnz] window i's distinct columns at [row[i*W], ...)
+};
+
+inline int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
+
+int64_t ws_bytes(int64_t rows, int64_t nnz, int window)
+{
+    const int64_t nwin = rows / window;
+    return align256(64) + align256(4 * nwin + 4) + 2 * align256(8 * (nwin + 1)) + align256(4 * rows + 4) +
+           2 * align256(4 * nnz + 4);
+}
+
+Ws ws_carve(void *base, int64_t rows, int64_t nnz, int window)
+{
+    const int64_t nwin = rows / window;
+    uint8_t *p = static_cast<uint8_t *>(base);
+    Ws w;
+    w.status = reinterpret_cast<int32_t *>(p); p += align256(64);
+    w.nuniq = reinterpret_cast<int32_t *>(p); p += align256(4 * nwin + 4);
+    w.ncol = reinterpret_cast<int64_t *>(p); p += align256(8 * (nwin + 1));
+    w.nparts = reinterpret_cast<int64_t *>(p); p += align256(8 * (nwin + 1));
+    w.rowflag = reinterpret_cast<int32_t *>(p); p += align256(4 * rows + 4);
+    w.rank = reinterpret_cast<int32_t *>(p); p += align256(4 * nnz + 4);
+    w.uniq = reinterpret_cast<int32_t *>(p);
+    return w;
+}
+
+inline int type_size(int t)
+{
+    switch (t) {
+    case ATTGNN_F32: case ATTGNN_I32: return 4;
+    case ATTGNN_F16: case ATTGNN_BF16: return 2;
+    default: return 0;
+    }
+}
+
+inline unsigned grid_for(int64_t n, int64_t per_block = 1, int64_t cap = 1 << 20)
+{
+    int64_t g = (n + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return unsigned(g);
+}
+
+// ---- block-wide helpers (256 threads = 4 waves) ---------------------------
+
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const T n = __shfl_up(v, d, 64);
+        if (lane >= d) v += n;
+    }
+    return v;
+}
+
+// Exclusive scan of v over the workgroup; `total` receives the sum.  `red`
+// holds 4 elements of LDS; the call contains two barriers.
+template <typename T>
+__device__ __forceinline__ T block_excl_scan(T v, T *red, T &total)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const T incl = wave_incl_scan(v);
+    if (lane == 63) red[wave] = incl;
+    __syncthreads();
+    T off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < THREADS / 64; ++w) {
+        if (w < wave) off += red[w];
+        tot += red[w];
+    }
+    __syncthreads();
+    total = tot;
+    return off + incl - v;
+}
+
+__device__ __forceinline__ void block_minmax(int &mn, int &mx, int *red)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        mn = min(mn, __shfl_xor(mn, d, 64));
+        mx = max(mx, __shfl_xor(mx, d, 64));
+    }
+    if (lane == 0) { red[2 * wave] = mn; red[2 * wave + 1] = mx; }
+    __syncthreads();
+    mn = red[0]; mx = red[1];
+#pragma unroll
+    for (int w = 1; w < THREADS / 64; ++w) { mn = min(mn, red[2 * w]); mx = max(mx, red[2 * w + 1]); }
+    __syncthreads();
+}
+
+// Blocks per part split (example.cpp:112-152) in closed form: part k of a
+// window with `blocks` blocks of `wide` columns, `v` real columns.
+__device__ __forceinline__ int64_t part_count_d(int64_t blocks, int part_size, int emit_empty)
+{
+    if (blocks == 0) return emit_empty ? 1 : 0;
+    if (blocks <= part_size) return 1;
+    return blocks / part_size + (blocks % part_size ? 1 : 0);
+}
+
+// ---- 1. row pointers -------------------------------------------------------
+
+__global__ __launch_bounds__(THREADS) void init_kernel(Ws ws, int64_t rows)
+{
+    const int64_t t = int64_t(blockIdx.x) * THREADS + threadIdx.x;
+    if (t == 0) ws.status[0] = 0;
+    for (int64_t j = t; j < rows; j += int64_t(gridDim.x) * THREADS) ws.rowflag[j] = 0;
+}
+
+__global__ __launch_bounds__(THREADS) void rows_check_kernel(const int32_t *row, int64_t rows, int64_t nnz, Ws ws)
+{
+    const int64_t t = int64_t(blockIdx.x) * THREADS + threadIdx.x;
+    if (t == 0 && (row[0] != 0 || int64_t(row[rows]) != nnz)) atomicOr(ws.status, ST_FORMAT);
+    for (int64_t j = t; j < rows; j += int64_t(gridDim.x) * THREADS)
+        if (row[j + 1] < row[j]) atomicOr(ws.status, ST_FORMAT);
+}
+
+// ---- 2. column ids and per-row order flags (one wave per row) -------------
+
+__global__ __launch_bounds__(THREADS) void cols_check_kernel(const int32_t *row, const int32_t *col, int64_t rows, Ws ws)
+{
+    if (ws.status[0]) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = int64_t(gridDim.x) * (THREADS / 64);
+    for (int64_t j = int64_t(blockIdx.x) * (THREADS / 64) + (threadIdx.x >> 6); j < rows; j += nw) {
+        const int64_t b = row[j], e = row[j + 1];
+        int bad = 0, unsorted = 0;
+        for (int64_t m = b + lane; m < e; m += 64) {
+            const int32_t c = col[m];
+            bad |= (c < 0 || int64_t(c) >= rows);
+            if (m > b && col[m - 1] > c) unsorted = 1;
+        }
+        if (__any(bad) && lane == 0) atomicOr(ws.status, ST_FORMAT);
+        if (__any(unsorted) && lane == 0) ws.rowflag[j] = 1;
+    }
+}
+
+// ---- 3. distinct columns per window ----------------------------------------
+
+struct UniqLds {
+    union {
+        struct { int32_t key[SORT_CAP]; int32_t idx[SORT_CAP]; } s;
+        struct { uint32_t bits[BM_WORDS]; int32_t pre[BM_WORDS]; } b;
+    };
+    int32_t red[8];
+};
+
+__global__ __launch_bounds__(THREADS) void window_unique_kernel(const int32_t *row, const int32_t *col, int64_t nwin,
+                                                                int window, Ws ws)
+{
+    __shared__ UniqLds L;
+    const int t = threadIdx.x;
+    for (int64_t i = blockIdx.x; i < nwin; i += gridDim.x) {
+        if (ws.status[0]) return;
+        const int64_t lo = row[i * window], hi = row[i * window + window];
+        const int E = int(hi - lo);
+        if (E == 0) {
+            if (t == 0) ws.nuniq[i] = 0;
+            continue;
+        }
+        int mn = INT_MAX, mx = -1;
+        for (int64_t k = lo + t; k < hi; k += THREADS) {
+            const int c = col[k];
+            mn = min(mn, c);
+            mx = max(mx, c);
+        }
+        block_minmax(mn, mx, L.red);
+        const int64_t span = int64_t(mx >> 5) - int64_t(mn >> 5) + 1;   // words
+        const bool bitmap = E > SORT_CAP || span <= 8 * int64_t(E);
+        if (!bitmap) {
+            // Bitonic sort of the window's column ids (padding INT_MAX sorts last).
+            int P2 = 64;
+            while (P2 < E) P2 <<= 1;
+            for (int k = t; k < P2; k += THREADS) {
+                L.s.key[k] = k < E ? col[lo + k] : INT_MAX;
+                L.s.idx[k] = k;
+            }
+            __syncthreads();
+            for (int size = 2; size <= P2; size <<= 1) {
+                for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                    for (int k = t; k < (P2 >> 1); k += THREADS) {
+                        const int a = (k / stride) * 2 * stride + (k % stride), b = a + stride;
+                        const bool up = (a & size) == 0;
+                        const int ka = L.s.key[a], kb = L.s.key[b];
+                        if ((ka > kb) == up) {
+                            L.s.key[a] = kb; L.s.key[b] = ka;
+                            const int ia = L.s.idx[a];
+                            L.s.idx[a] = L.s.idx[b]; L.s.idx[b] = ia;
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+            // Each thread walks a contiguous run of sorted slots.
+            const int per = (P2 + THREADS - 1) / THREADS;
+            const int k0 = min(t * per, P2), k1 = min(k0 + per, E);
+            int cnt = 0;
+            for (int k = k0; k < k1; ++k) cnt += (k == 0 || L.s.key[k] != L.s.key[k - 1]);
+            int total;
+            int r = block_excl_scan(cnt, L.red, total) - 1;
+            for (int k = k0; k < k1; ++k) {
+                if (k == 0 || L.s.key[k] != L.s.key[k - 1]) {
+                    ++r;
+                    ws.uniq[lo + r] = L.s.key[k];
+                }
+                ws.rank[lo + L.s.idx[k]] = r;
+            }
+            if (t == 0) ws.nuniq[i] = total;
+        } else {
+            // Bitmap over the window's column span, BM_WORDS words per pass.
+            const int64_t w_first = mn >> 5;
+            int run = 0;
+            for (int64_t c0 = 0; c0 < span; c0 += BM_WORDS) {
+                const int64_t w0 = w_first + c0;
+                const int nw = int(span - c0 < BM_WORDS ? span - c0 : BM_WORDS);
+                for (int w = t; w < nw; w += THREADS) L.b.bits[w] = 0;
+                __syncthreads();
+                for (int64_t k = lo + t; k < hi; k += THREADS) {
+                    const int c = col[k];
+                    const int64_t w = int64_t(c >> 5) - w0;
+                    if (w >= 0 && w < nw) atomicOr(&L.b.bits[w], 1u << (c & 31));
+                }
+                __syncthreads();
+                constexpr int PER = BM_WORDS / THREADS;
+                const int wa = min(t * PER, nw), wb = min(wa + PER, nw);
+                int cnt = 0;
+                for (int w = wa; w < wb; ++w) cnt += __popc(L.b.bits[w]);
+                int total;
+                int acc = run + block_excl_scan(cnt, L.red, total);
+                for (int w = wa; w < wb; ++w) {
+                    L.b.pre[w] = acc;
+                    uint32_t bits = L.b.bits[w];
+                    while (bits) {
+                        const int bit = __ffs(bits) - 1;
+                        bits &= bits - 1;
+                        ws.uniq[lo + acc] = int32_t((w0 + w) * 32 + bit);
+                        ++acc;
+                    }
+                }
+                __syncthreads();
+                for (int64_t k = lo + t; k < hi; k += THREADS) {
+                    const int c = col[k];
+                    const int64_t w = int64_t(c >> 5) - w0;
+                    if (w >= 0 && w < nw)
+                        ws.rank[k] = L.b.pre[w] + __popc(L.b.bits[w] & ((1u << (c & 31)) - 1u));
+                }
+                run += total;
+                __syncthreads();
+            }
+            if (t == 0) ws.nuniq[i] = run;
+        }
+        __syncthreads();
+    }
+}
+
+// ---- 4. prefix sums over windows (one workgroup) ---------------------------
+
+__global__ __launch_bounds__(THREADS) void window_scan_kernel(int64_t nwin, int window, int wide, int part_size,
+                                                              int emit_empty, Ws ws, int64_t *totals)
+{
+    __shared__ int64_t red[8];
+    const int t = threadIdx.x;
+    const int st = ws.status[0];
+    if (st) {
+        if (t == 0) {
+            for (int k = 1; k < 8; ++k) totals[k] = 0;
+            totals[0] = (st & ST_FORMAT) ? ATTGNN_E_FORMAT : ATTGNN_E_UNSUPPORTED;
+        }
+        return;
+    }
+    constexpr int ITEMS = 8;
+    int64_t carry_c = 0, carry_p = 0, carry_u = 0;
+    for (int64_t base = 0; base < nwin; base += int64_t(THREADS) * ITEMS) {
+        int64_t cc[ITEMS], pp[ITEMS];
+        int64_t sc = 0, sp = 0, su = 0;
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+            const int64_t i = base + int64_t(t) * ITEMS + k;
+            cc[k] = pp[k] = 0;
+            if (i < nwin) {
+                const int64_t v = ws.nuniq[i];
+                const int64_t pad = (v + wide - 1) / wide * wide;
+                const int64_t np = part_count_d(pad / wide, part_size, emit_empty);
+                cc[k] = np ? pad : 0;
+                pp[k] = np;
+                su += v;
+            }
+            sc += cc[k];
+            sp += pp[k];
+        }
+        int64_t tc, tp, tu;
+        int64_t oc = block_excl_scan(sc, red, tc) + carry_c;
+        int64_t op = block_excl_scan(sp, red, tp) + carry_p;
+        block_excl_scan(su, red, tu);
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+            const int64_t i = base + int64_t(t) * ITEMS + k;
+            if (i < nwin) {
+                ws.ncol[i] = oc;
+                ws.nparts[i] = op;
+            }
+            oc += cc[k];
+            op += pp[k];
+        }
+        carry_c += tc;
+        carry_p += tp;
+        carry_u += tu;
+    }
+    if (t == 0) {
+        ws.ncol[nwin] = carry_c;
+        ws.nparts[nwin] = carry_p;
+        const bool too_big = carry_c > INT32_MAX;
+        if (too_big) ws.status[0] = ST_UNSUPPORTED;
+        totals[0] = too_big ? ATTGNN_E_UNSUPPORTED : ATTGNN_OK;
+        totals[1] = 2 * carry_p + 1;
+        totals[2] = carry_c;
+        totals[3] = carry_c * window;
+        totals[4] = carry_p;
+        totals[5] = carry_u;
+        totals[6] = 0;
+        totals[7] = 0;
+    }
+}
+
+// ---- 5. fill ---------------------------------------------------------------
+
+__device__ __forceinline__ void store_val(void *base, int64_t idx, int out_type, float v)
+{
+    switch (out_type) {
+    case ATTGNN_F32: static_cast<float *>(base)[idx] = v; break;
+    case ATTGNN_F16: static_cast<uint16_t *>(base)[idx] = attgnn::f32_to_f16(v); break;
+    default: static_cast<uint16_t *>(base)[idx] = attgnn::f32_to_bf16(v); break;
+    }
+}
+
+// First CSR position of column j in row r (-1 if absent).
+__device__ __forceinline__ int32_t find_in_row(const int32_t *row, const int32_t *col, const int32_t *rowflag,
+                                               int32_t r, int64_t j)
+{
+    int64_t b = row[r], e = row[r + 1];
+    if (!rowflag[r]) {
+        while (b < e) {
+            const int64_t m = (b + e) >> 1;
+            if (col[m] < j) b = m + 1; else e = m;
+        }
+        return (b < row[r + 1] && col[b] == j) ? int32_t(b) : -1;
+    }
+    for (int64_t m = b; m < e; ++m)
+        if (col[m] == j) return int32_t(m);
+    return -1;
+}
+
+struct FillArgs {
+    const int32_t *row, *col;
+    int64_t nwin;
+    int src, out_type, window, wide, part_size;
+    const void *src_ptr;
+    int32_t *rp, *col_out, *map_t, *win, *atomic;
+    void *val;
+};
+
+__global__ __launch_bounds__(THREADS) void window_fill_kernel(FillArgs a, Ws ws)
+{
+    const int t = threadIdx.x;
+    if (blockIdx.x == 0 && t == 0) a.rp[0] = 0;
+    const int W = a.window, wide = a.wide;
+    const bool is_map = a.src == ATTGNN_SRC_MAP;
+    for (int64_t i = blockIdx.x; i < a.nwin; i += gridDim.x) {
+        if (ws.status[0]) return;
+        const int64_t p0 = ws.nparts[i], np = ws.nparts[i + 1] - p0;
+        if (np == 0) continue;
+        const int64_t c0 = ws.ncol[i], pad = ws.ncol[i + 1] - c0;
+        const int64_t v = ws.nuniq[i];
+        const int64_t lo = a.row[i * W];
+        for (int64_t k = t; k < pad; k += THREADS) a.col_out[c0 + k] = k < v ? ws.uniq[lo + k] : -1;
+        const int64_t s0 = c0 * W, ns = pad * W;
+        if (is_map) {
+            int32_t *m = static_cast<int32_t *>(a.val);
+            for (int64_t k = t; k < ns; k += THREADS) m[s0 + k] = -1;
+            if (a.map_t)
+                for (int64_t k = t; k < ns; k += THREADS) a.map_t[s0 + k] = -1;
+        } else if (a.out_type == ATTGNN_F32) {
+            float *m = static_cast<float *>(a.val);
+            for (int64_t k = t; k < ns; k += THREADS) m[s0 + k] = 0.f;
+        } else {
+            uint16_t *m = static_cast<uint16_t *>(a.val);
+            for (int64_t k = t; k < ns; k += THREADS) m[s0 + k] = 0;
+        }
+        // parts (example.cpp:112-152); every part but a window's last spans
+        // part_size * wide columns
+        const int64_t blocks = pad / wide;
+        for (int64_t k = t; k < np; k += THREADS) {
+            int64_t real, padc;
+            int at;
+            if (blocks == 0) { real = 0; padc = 0; at = 0; }
+            else if (blocks <= a.part_size) { real = v; padc = pad - v; at = 0; }
+            else {
+                const int64_t full = blocks / a.part_size, res = blocks % a.part_size, tailpad = pad - v;
+                const int64_t span = int64_t(a.part_size) * wide;
+                if (k < full) {
+                    const bool last = (res == 0 && k == full - 1);
+                    real = last ? span - tailpad : span;
+                    padc = last ? tailpad : 0;
+                } else {
+                    real = res * wide - tailpad;
+                    padc = tailpad;
+                }
+                at = 1;
+            }
+            const int64_t start = c0 + k * int64_t(a.part_size) * wide;
+            const int64_t pi = p0 + k;
+            a.rp[2 * pi + 1] = int32_t(start + real);
+            a.rp[2 * pi + 2] = int32_t(start + real + padc);
+            a.win[pi] = int32_t(i);
+            a.atomic[pi] = at;
+        }
+        __syncthreads();   // tile initialisation before the scatter
+        for (int p = 0; p < W; ++p) {
+            const int64_t j = i * W + p;
+            const int64_t e = a.row[j + 1];
+            for (int64_t m = a.row[j] + t; m < e; m += THREADS) {
+                const int32_t c = ws.rank[m];
+                const int32_t cj = a.col[m];
+                const int64_t o = s0 + int64_t(c / wide) * W * wide + int64_t(p) * wide + c % wide;
+                switch (a.src) {
+                case ATTGNN_SRC_DEG_F16: {
+                    const uint16_t *d = static_cast<const uint16_t *>(a.src_ptr);
+                    store_val(a.val, o, a.out_type, attgnn::f16_to_f32(d[j]) * attgnn::f16_to_f32(d[cj]));
+                    break;
+                }
+                case ATTGNN_SRC_DEG_F32: {
+                    const float *d = static_cast<const float *>(a.src_ptr);
+                    store_val(a.val, o, a.out_type, d[j] * d[cj]);
+                    break;
+                }
+                case ATTGNN_SRC_EDGE_F32: {
+                    // duplicate (row, column) entries: the last one wins, as on the host
+                    bool last = true;
+                    if (!ws.rowflag[j]) {
+                        last = (m + 1 == e) || a.col[m + 1] != cj;
+                    } else {
+                        for (int64_t q = m + 1; q < e; ++q)
+                            if (a.col[q] == cj) { last = false; break; }
+                    }
+                    if (last) store_val(a.val, o, a.out_type, static_cast<const float *>(a.src_ptr)[m]);
+                    break;
+                }
+                default:
+                    atomicMax(static_cast<int32_t *>(a.val) + o, int32_t(m));
+                    if (a.map_t) a.map_t[o] = find_in_row(a.row, a.col, ws.rowflag, cj, j);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---- 6/7. compact tiles ----------------------------------------------------
+
+__device__ __forceinline__ bool tile_nz(const void *val, int64_t i, int es, bool is_map)
+{
+    if (is_map) return static_cast<const int32_t *>(val)[i] >= 0;
+    if (es == 4) return static_cast<const uint32_t *>(val)[i] != 0;
+    return static_cast<const uint16_t *>(val)[i] != 0;
+}
+
+// One wave per 32-column block: row masks, entry count into block_base[blk + 1].
+__global__ __launch_bounds__(THREADS) void compact_count_kernel(const void *val, int es, int is_map, int64_t nb,
+                                                                int W, uint16_t *masks, int64_t *block_base)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = int64_t(gridDim.x) * (THREADS / 64);
+    for (int64_t blk = int64_t(blockIdx.x) * (THREADS / 64) + (threadIdx.x >> 6); blk < nb; blk += nw) {
+        uint32_t m = 0;
+        if (lane < 32)
+            for (int r = 0; r < W; ++r)
+                if (tile_nz(val, (blk * W + r) * 32 + lane, es, is_map != 0)) m |= 1u << r;
+        if (lane < 32) masks[blk * 32 + lane] = uint16_t(m);
+        int cnt = __popc(m);
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) cnt += __shfl_xor(cnt, d, 64);
+        if (lane == 0) block_base[blk + 1] = cnt;
+    }
+}
+
+// In-place inclusive scan of a[1..n] with a[0] = 0 (one workgroup).
+__global__ __launch_bounds__(THREADS) void scan_inplace_kernel(int64_t *a, int64_t n)
+{
+    __shared__ int64_t red[8];
+    const int t = threadIdx.x;
+    constexpr int ITEMS = 8;
+    int64_t carry = 0;
+    if (t == 0) a[0] = 0;
+    for (int64_t base = 1; base <= n; base += int64_t(THREADS) * ITEMS) {
+        int64_t x[ITEMS], s = 0;
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+            const int64_t i = base + int64_t(t) * ITEMS + k;
+            x[k] = i <= n ? a[i] : 0;
+            s += x[k];
+        }
+        int64_t tot;
+        int64_t o = block_excl_scan(s, red, tot) + carry;
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+            const int64_t i = base + int64_t(t) * ITEMS + k;
+            o += x[k];
+            if (i <= n) a[i] = o;
+        }
+        carry += tot;
+    }
+}
+
+// One wave per block; lane (g, r) = (lane / 16, lane % 16) packs the entries of
+// window row r in the 8-column group g, in column order.
+__global__ __launch_bounds__(THREADS) void compact_pack_kernel(const void *val, int es, int is_map, int64_t nb, int W,
+                                                               const uint16_t *masks, const int64_t *block_base,
+                                                               void *values, uint16_t *positions)
+{
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, r = lane & 15;
+    const int64_t nw = int64_t(gridDim.x) * (THREADS / 64);
+    for (int64_t blk = int64_t(blockIdx.x) * (THREADS / 64) + (threadIdx.x >> 6); blk < nb; blk += nw) {
+        uint32_t bits = 0;   // bit c-8g set: (r, c) is an entry
+        if (r < W)
+            for (int c = 0; c < 8; ++c) bits |= ((uint32_t(masks[blk * 32 + 8 * g + c]) >> r) & 1u) << c;
+        const int cnt = __popc(bits);
+        int64_t o = block_base[blk] + (wave_incl_scan(cnt) - cnt);
+        while (bits) {
+            const int c = 8 * g + __ffs(bits) - 1;
+            bits &= bits - 1;
+            const int64_t i = (blk * W + r) * 32 + c;
+            if (es == 4) static_cast<uint32_t *>(values)[o] = static_cast<const uint32_t *>(val)[i];
+            else static_cast<uint16_t *>(values)[o] = static_cast<const uint16_t *>(val)[i];
+            if (positions) positions[o] = uint16_t(r * 32 + c);
+            ++o;
+        }
+    }
+}
+
+inline int launch_status()
+{
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? ATTGNN_OK : int(e);
+}
+
+}  // namespace
+
+extern "C" int64_t attgnn_blocks_device_workspace_bytes(int64_t rows_plus1, int64_t nnz, int window)
+{
+    if (rows_plus1 < 1 || nnz < 0 || window <= 0) return -1;
+    return ws_bytes(rows_plus1 - 1, nnz, window);
+}
+
+extern "C" int attgnn_blocks_plan_device(const int32_t *row, int64_t rows_plus1, const int32_t *col, int64_t nnz,
+                                         int window, int wide, int part_size, int emit_empty, void *workspace,
+                                         int64_t workspace_bytes, int64_t *totals, attgnn_stream_t stream)
+{
+    if (!row || !totals || !workspace || rows_plus1 < 1 || window <= 0 || wide <= 0 || part_size <= 0 || nnz < 0 ||
+        nnz > INT32_MAX || rows_plus1 - 1 > INT32_MAX)
+        return ATTGNN_E_ARG;
+    if (nnz > 0 && !col) return ATTGNN_E_ARG;
+    const int64_t rows = rows_plus1 - 1, nwin = rows / window;
+    if (workspace_bytes < ws_bytes(rows, nnz, window)) return ATTGNN_E_ARG;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const Ws ws = ws_carve(workspace, rows, nnz, window);
+    hipLaunchKernelGGL(init_kernel, dim3(grid_for(rows, THREADS)), dim3(THREADS), 0, s, ws, rows);
+    hipLaunchKernelGGL(rows_check_kernel, dim3(grid_for(rows, THREADS)), dim3(THREADS), 0, s, row, rows, nnz, ws);
+    hipLaunchKernelGGL(cols_check_kernel, dim3(grid_for(rows, THREADS / 64)), dim3(THREADS), 0, s, row, col, rows,
+                       ws);
+    if (nwin > 0)
+        hipLaunchKernelGGL(window_unique_kernel, dim3(grid_for(nwin)), dim3(THREADS), 0, s, row, col, nwin, window,
+                           ws);
+    hipLaunchKernelGGL(window_scan_kernel, dim3(1), dim3(THREADS), 0, s, nwin, window, wide, part_size, emit_empty,
+                       ws, totals);
+    return launch_status();
+}
+
+extern "C" int attgnn_blocks_fill_device(const int32_t *row, int64_t rows_plus1, const int32_t *col, int64_t nnz,
+                                         int src, const void *src_ptr, int out_type, int window, int wide,
+                                         int part_size, const void *workspace, int64_t workspace_bytes,
+                                         int32_t *rp, int32_t *col_out, void *val, int32_t *map_t,
+                                         int32_t *window_of_part, int32_t *atomic, attgnn_stream_t stream)
+{
+    if (!row || !workspace || !rp || rows_plus1 < 1 || window <= 0 || wide <= 0 || part_size <= 0 || nnz < 0)
+        return ATTGNN_E_ARG;
+    if (src < ATTGNN_SRC_DEG_F16 || src > ATTGNN_SRC_MAP) return ATTGNN_E_ARG;
+    const bool is_map = src == ATTGNN_SRC_MAP;
+    if (is_map ? out_type != ATTGNN_I32
+               : !(out_type == ATTGNN_F32 || out_type == ATTGNN_F16 || out_type == ATTGNN_BF16))
+        return ATTGNN_E_ARG;
+    if (!is_map && !src_ptr) return ATTGNN_E_ARG;
+    if (map_t && !is_map) return ATTGNN_E_ARG;
+    const int64_t rows = rows_plus1 - 1, nwin = rows / window;
+    if (workspace_bytes < ws_bytes(rows, nnz, window)) return ATTGNN_E_ARG;
+    FillArgs a;
+    a.row = row; a.col = col; a.nwin = nwin; a.src = src; a.out_type = out_type; a.window = window;
+    a.wide = wide; a.part_size = part_size; a.src_ptr = src_ptr; a.rp = rp; a.col_out = col_out;
+    a.map_t = map_t; a.win = window_of_part; a.atomic = atomic; a.val = val;
+    hipLaunchKernelGGL(window_fill_kernel, dim3(grid_for(nwin)), dim3(THREADS), 0,
+                       reinterpret_cast<hipStream_t>(stream), a,
+                       ws_carve(const_cast<void *>(workspace), rows, nnz, window));
+    return launch_status();
+}
+
+extern "C" int attgnn_blocks_compact_count_device(const void *val, int tile_type, int64_t n_cols, int window,
+                                                  uint16_t *masks, int64_t *block_base, attgnn_stream_t stream)
+{
+    if (n_cols < 0 || n_cols % 32 || window <= 0 || window > 16 || !block_base) return ATTGNN_E_ARG;
+    const int es = type_size(tile_type);
+    if (!es) return ATTGNN_E_ARG;
+    const int64_t nb = n_cols / 32;
+    if (nb > 0 && (!val || !masks)) return ATTGNN_E_ARG;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (nb > 0)
+        hipLaunchKernelGGL(compact_count_kernel, dim3(grid_for(nb, THREADS / 64)), dim3(THREADS), 0, s, val, es,
+                           int(tile_type == ATTGNN_I32), nb, window, masks, block_base);
+    hipLaunchKernelGGL(scan_inplace_kernel, dim3(1), dim3(THREADS), 0, s, block_base, nb);
+    return launch_status();
+}
+
+extern "C" int attgnn_blocks_compact_fill_device(const void *val, int tile_type, int64_t n_cols, int window,
+                                                 const uint16_t *masks, const int64_t *block_base, void *values,
+                                                 uint16_t *positions, attgnn_stream_t stream)
+{
+    if (n_cols < 0 || n_cols % 32 || window <= 0 || window > 16 || !block_base) return ATTGNN_E_ARG;
+    const int es = type_size(tile_type);
+    if (!es) return ATTGNN_E_ARG;
+    const int64_t nb = n_cols / 32;
+    if (nb == 0) return ATTGNN_OK;
+    if (!val || !masks || !values) return ATTGNN_E_ARG;
+    hipLaunchKernelGGL(compact_pack_kernel, dim3(grid_for(nb, THREADS / 64)), dim3(THREADS), 0,
+                       reinterpret_cast<hipStream_t>(stream), val, es, int(tile_type == ATTGNN_I32), nb, window,
+                       masks, block_base, values, positions);
+    return launch_status();
+}

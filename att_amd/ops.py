@@ -134,6 +134,87 @@ def blocks_build(row, col, src, src_values, out_dtype, window, wide, part_size,
     return out
 
 
+_SRCS = {"deg_f16": _lib.SRC_DEG_F16, "deg_f32": _lib.SRC_DEG_F32, "edge_f32": _lib.SRC_EDGE_F32,
+         "map": _lib.SRC_MAP}
+
+
+def blocks_build_device(row, col, src, src_values, out_dtype, window, wide, part_size, emit_empty,
+                        want_trans=False, compact=False):
+    """Window-block structure built on the GPU from a device-resident CSR.
+
+    Same arguments and same result keys as :func:`blocks_build`, bit-identical
+    values, but every array is a device tensor.  ``row`` / ``col`` are int32
+    device tensors; ``src_values`` (degrees or edge values) a device tensor.
+    Reads the structure's sizes back once (and the compact value count once).
+    """
+    if src not in _SRCS:
+        raise ValueError(f"unknown src {src!r}")
+    if want_trans and src != "map":
+        raise ValueError("want_trans needs src='map'")
+    dev = _need_gpu("blocks_build_device", row, col, src_values if src != "map" else None)
+    _need_i32("blocks_build_device", row, col)
+    if row.dim() != 1 or col.dim() != 1 or row.numel() < 1:
+        raise ValueError("row and col must be 1-D, row non-empty")
+    if src == "map":
+        out_dtype, sp, keep = torch.int32, ctypes.c_void_p(0), None
+    else:
+        want = {"deg_f16": torch.float16, "deg_f32": torch.float32, "edge_f32": torch.float32}[src]
+        if src_values.dtype != want:
+            raise TypeError(f"{src}: values must be {want}, got {src_values.dtype}")
+        expect = row.numel() - 1 if src.startswith("deg") else col.numel()
+        if src_values.numel() < expect:
+            raise ValueError(f"{src}: need {expect} values, got {src_values.numel()}")
+        if out_dtype not in (torch.float32, torch.float16, torch.bfloat16):
+            raise TypeError(f"tile type {out_dtype} is not supported")
+        keep = src_values
+        sp = _ptr(keep)
+    rows1, nnz = row.numel(), col.numel()
+    L = lib()
+    ws_n = L.attgnn_blocks_device_workspace_bytes(rows1, nnz, window)
+    if ws_n < 0:
+        raise ValueError("invalid sizes for the device preprocessing")
+    ws = torch.empty(max(ws_n, 1), dtype=torch.uint8, device=dev)
+    totals = torch.empty(8, dtype=torch.int64, device=dev)
+    st = _stream(dev)
+    check(L.attgnn_blocks_plan_device(_ptr(row), rows1, _ptr(col), nnz, window, wide, part_size,
+                                      int(bool(emit_empty)), _ptr(ws), ws_n, _ptr(totals), st),
+          "attgnn_blocks_plan_device")
+    status, n_rp, n_col, n_val, n_parts, n_unique = [int(x) for x in totals[:6].cpu()]
+    check(status, "attgnn_blocks_plan_device")
+    out = {
+        "rp": torch.empty(n_rp, dtype=torch.int32, device=dev),
+        "col": torch.empty(n_col, dtype=torch.int32, device=dev),
+        "val": torch.empty(n_val, dtype=out_dtype, device=dev),
+        "window": torch.empty(n_parts, dtype=torch.int32, device=dev),
+        "atomic": torch.empty(n_parts, dtype=torch.int32, device=dev),
+        "map_t": torch.empty(n_val, dtype=torch.int32, device=dev) if want_trans else None,
+        "n_unique": n_unique,
+    }
+    check(L.attgnn_blocks_fill_device(_ptr(row), rows1, _ptr(col), nnz, _SRCS[src], sp, _TYPE_OF[out_dtype],
+                                      window, wide, part_size, _ptr(ws), ws_n, _ptr(out["rp"]), _ptr(out["col"]),
+                                      _ptr(out["val"]), _ptr(out["map_t"]), _ptr(out["window"]),
+                                      _ptr(out["atomic"]), st),
+          "attgnn_blocks_fill_device")
+    if compact:
+        if wide != 32 or window > 16:
+            raise ValueError("compact tiles need wide = 32 and window <= 16")
+        nb = n_col // 32
+        masks = torch.empty(max(n_col, 8), dtype=torch.int16, device=dev)
+        base = torch.empty(nb + 1, dtype=torch.int64, device=dev)
+        check(L.attgnn_blocks_compact_count_device(_ptr(out["val"]), _TYPE_OF[out_dtype], n_col, window,
+                                                   _ptr(masks), _ptr(base), st),
+              "attgnn_blocks_compact_count_device")
+        nv = int(base[nb].item())
+        values = torch.empty(max(nv, 1), dtype=out_dtype, device=dev)
+        positions = torch.empty(max(nv, 1), dtype=torch.int16, device=dev)
+        check(L.attgnn_blocks_compact_fill_device(_ptr(out["val"]), _TYPE_OF[out_dtype], n_col, window,
+                                                  _ptr(masks), _ptr(base), _ptr(values), _ptr(positions), st),
+              "attgnn_blocks_compact_fill_device")
+        out.update(masks=masks, block_base=base, values=values, positions=positions)
+    del keep
+    return out
+
+
 # --------------------------------------------------------------------------
 # SpMM
 # --------------------------------------------------------------------------

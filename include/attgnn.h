@@ -123,6 +123,63 @@ int attgnn_blocks_compact_export(const attgnn_blocks *b, uint16_t *masks, int64_
                                  uint16_t *positions);
 
 /* ====================================================================== */
+/* Window-block preprocessing (device)                                     */
+/* ====================================================================== */
+
+/*
+ * The same structure as attgnn_blocks_build, built on the GPU from a CSR that
+ * is already in device memory; every output is bit-identical to the host
+ * build (same sources, element types, window / wide / part_size, empty-window
+ * rule, unsorted rows and duplicate entries).
+ *
+ * Replaces ATT_Block_gpu.preprocess_gpu_fs  (Block_gpu/setup.py:10-14; caller
+ *            end2end_overhead/gat_final/mgat_csr_ov/mdataset_fp16.py:53-58),
+ *          and on the device the host windowing of
+ *            Block_sr_bcrs.blockProcess_{fp16,tf32}_balance (gcn/Block_gcn/example.cpp:48,203)
+ *            ATT_Block[_v3].blockProcess_sddmm_balance_gnn[_trans] (Block/include/example.h:4-6)
+ *
+ * Two phases, both asynchronous on `stream`:
+ *  plan: checks the CSR and finds every window's distinct columns, writing
+ *    into totals (device int64[8]): [0] status (ATTGNN_OK, ATTGNN_E_FORMAT for a
+ *    malformed CSR, ATTGNN_E_UNSUPPORTED if the column count exceeds int32),
+ *    [1] n_rp, [2] n_col, [3] n_val, [4] n_parts, [5] n_unique (as
+ *    attgnn_blocks_sizes).  The caller reads totals, allocates the outputs
+ *    and calls
+ *  fill: with the same row, col, window, wide and part_size and the same
+ *    workspace (unchanged since the plan); map_t (MAP only, may be NULL)
+ *    receives the transposed-edge map.
+ * workspace: device memory of attgnn_blocks_device_workspace_bytes() bytes.
+ * row[rows_plus1], col[nnz], src_ptr: device pointers.
+ */
+int64_t attgnn_blocks_device_workspace_bytes(int64_t rows_plus1, int64_t nnz, int window);
+
+int attgnn_blocks_plan_device(const int32_t *row, int64_t rows_plus1, const int32_t *col, int64_t nnz,
+                              int window, int wide, int part_size, int emit_empty, void *workspace,
+                              int64_t workspace_bytes, int64_t *totals, attgnn_stream_t stream);
+
+int attgnn_blocks_fill_device(const int32_t *row, int64_t rows_plus1, const int32_t *col, int64_t nnz,
+                              int src, const void *src_ptr, int out_type, int window, int wide,
+                              int part_size, const void *workspace, int64_t workspace_bytes,
+                              int32_t *rp, int32_t *col_out, void *val, int32_t *map_t,
+                              int32_t *window_of_part, int32_t *atomic, attgnn_stream_t stream);
+
+/*
+ * Compact tiles on the device, bit-identical to attgnn_blocks_compact_export:
+ * count writes masks[n_cols] and block_base[n_cols/32 + 1] (block_base[n_cols/32]
+ * = n_values, which the caller reads to size values / positions); fill packs
+ * values[n_values] and (if not NULL) positions[n_values].  val holds the
+ * n_cols * window tiles of a structure built with wide = 32 (tile_type
+ * F16 / BF16 / F32, or I32 for an edge-id map).
+ *
+ * No counterpart in the reference (see attgnn_blocks_compact_sizes).
+ */
+int attgnn_blocks_compact_count_device(const void *val, int tile_type, int64_t n_cols, int window,
+                                       uint16_t *masks, int64_t *block_base, attgnn_stream_t stream);
+int attgnn_blocks_compact_fill_device(const void *val, int tile_type, int64_t n_cols, int window,
+                                      const uint16_t *masks, const int64_t *block_base, void *values,
+                                      uint16_t *positions, attgnn_stream_t stream);
+
+/* ====================================================================== */
 /* Window-block SpMM (device)                                              */
 /* ====================================================================== */
 
