@@ -589,6 +589,16 @@ __global__ __launch_bounds__(THREADS) void compact_pack_kernel(const void *val, 
     }
 }
 
+// perm[map[s]] = map_t[s] for every slot with map[s] >= 0.
+__global__ __launch_bounds__(THREADS) void map_scatter_kernel(const int32_t *map, const int32_t *map_t, int64_t n,
+                                                              int64_t nnz, int64_t *perm)
+{
+    for (int64_t s = int64_t(blockIdx.x) * THREADS + threadIdx.x; s < n; s += int64_t(gridDim.x) * THREADS) {
+        const int32_t e = map[s];
+        if (e >= 0 && e < nnz) perm[e] = map_t[s];
+    }
+}
+
 inline int launch_status()
 {
     const hipError_t e = hipGetLastError();
@@ -683,5 +693,15 @@ extern "C" int attgnn_blocks_compact_fill_device(const void *val, int tile_type,
     hipLaunchKernelGGL(compact_pack_kernel, dim3(grid_for(nb, THREADS / 64)), dim3(THREADS), 0,
                        reinterpret_cast<hipStream_t>(stream), val, es, int(tile_type == ATTGNN_I32), nb, window,
                        masks, block_base, values, positions);
+    return launch_status();
+}
+
+extern "C" int attgnn_map_transpose_perm_device(const int32_t *map, const int32_t *map_t, int64_t n_val, int64_t nnz,
+                                                int64_t *perm, attgnn_stream_t stream)
+{
+    if (n_val < 0 || nnz < 0 || (n_val > 0 && (!map || !map_t || !perm))) return ATTGNN_E_ARG;
+    if (n_val == 0) return ATTGNN_OK;
+    hipLaunchKernelGGL(map_scatter_kernel, dim3(grid_for(n_val, THREADS)), dim3(THREADS), 0,
+                       reinterpret_cast<hipStream_t>(stream), map, map_t, n_val, nnz, perm);
     return launch_status();
 }

@@ -62,7 +62,10 @@ class GCNGraph:
     """
 
     def __init__(self, csr, device, window=16, wide=32, part_size=None, tile_dtype=torch.float16,
-                 symmetric=None, values=None, compact="auto"):
+                 symmetric=None, values=None, compact="auto", build="host"):
+        if build not in ("host", "device"):
+            raise ValueError(f"build must be 'host' or 'device', got {build!r}")
+        self.build = build
         self.window, self.wide = window, wide
         # compact tiles win once a gathered row is >= 256 bytes; "auto" keeps
         # both forms and picks per call
@@ -91,11 +94,18 @@ class GCNGraph:
          self.t_atomicTensor) = self.fwd["arrays"]
 
     def _build(self, csr, values, device):
-        b = ops.blocks_build(csr.rowptr32(), csr.indices, "edge_f32", np.asarray(values, np.float32),
-                             self.tile_dtype, self.window, self.wide, self.part_size, emit_empty=True,
-                             compact=self.compact)
+        if self.build == "device":
+            b = ops.blocks_build_device(torch.from_numpy(csr.rowptr32()).to(device),
+                                        torch.from_numpy(csr.indices).to(device), "edge_f32",
+                                        torch.from_numpy(np.asarray(values, np.float32)).to(device),
+                                        self.tile_dtype, self.window, self.wide, self.part_size, emit_empty=True,
+                                        compact=bool(self.compact))
+        else:
+            b = ops.blocks_build(csr.rowptr32(), csr.indices, "edge_f32", np.asarray(values, np.float32),
+                                 self.tile_dtype, self.window, self.wide, self.part_size, emit_empty=True,
+                                 compact=self.compact)
         keys = ("rp", "col", "val", "window", "atomic")
-        if self.compact is True:   # compact only: the dense tiles are not uploaded
+        if self.compact is True:   # compact only: the dense tiles are not kept
             b["val"] = torch.zeros(1, dtype=self.tile_dtype)
         arrays = [b[k].to(device) for k in keys]
         self.n_unique = b["n_unique"]
@@ -183,7 +193,10 @@ class GATGraph:
     sums).  Attribute names follow the reference dataset
     (end2end_head/gat_final/mgat_csr/mdataset_fp16.py:53-65)."""
 
-    def __init__(self, csr, device, window=16, wide=32, part_size=None):
+    def __init__(self, csr, device, window=16, wide=32, part_size=None, build="host"):
+        if build not in ("host", "device"):
+            raise ValueError(f"build must be 'host' or 'device', got {build!r}")
+        self.build = build
         self.window, self.wide = window, wide
         self.num_nodes_ori = csr.n
         padded = csr.padded(window)
@@ -194,12 +207,29 @@ class GATGraph:
         self.part_size = part_size or max(default_part_size(_blocks_estimate(padded, wide)), 32)
         self.rowptr = torch.from_numpy(csr.rowptr32()).to(device)
         self.col = torch.from_numpy(csr.indices).to(device)
-        b = ops.blocks_build(padded.rowptr32(), padded.indices, "map", None, torch.int32, window, wide,
-                             self.part_size, emit_empty=True, compact=(wide == 32))
+        if build == "device":
+            # one device build of A with its transposed map: on a symmetric
+            # pattern the structure of A^T is A's, with map_T as its edge ids
+            b = ops.blocks_build_device(torch.from_numpy(padded.rowptr32()).to(device), self.col, "map", None,
+                                        torch.int32, window, wide, self.part_size, emit_empty=True,
+                                        want_trans=True, compact=(wide == 32))
+            perm = ops.map_transpose_perm(b["val"], b["map_t"], csr.nnz)
+            symmetric = csr.nnz == 0 or int(perm.min().item()) >= 0
+        else:
+            b = ops.blocks_build(padded.rowptr32(), padded.indices, "map", None, torch.int32, window, wide,
+                                 self.part_size, emit_empty=True, compact=(wide == 32))
+            symmetric = False
         self.n_unique = b["n_unique"]
         self.fwd = [b[k].to(device) for k in ("rp", "col", "val", "window", "atomic")]
         # compact edge list of A (block offsets, edge ids, packed positions) for the SDDMM
         self.edges = [b[k].to(device) for k in ("block_base", "values", "positions")] if wide == 32 else None
+        if symmetric:
+            self.bwd = [b["rp"], b["col"], b["map_t"], b["window"], b["atomic"]]
+            self.t_rowptr = self.rowptr
+            self.t_perm = perm
+            self.row_pointers, self.column_index, self.degrees, self.window_t, self.automic = self.fwd
+            self.degrees_trans = self.bwd[2]
+            return
         t, order = transpose_csr(padded)
         bt = ops.blocks_build(t.rowptr32(), t.indices, "map", None, torch.int32, window, wide, self.part_size,
                               emit_empty=True)

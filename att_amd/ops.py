@@ -215,6 +215,19 @@ def blocks_build_device(row, col, src, src_values, out_dtype, window, wide, part
     return out
 
 
+def map_transpose_perm(emap, emap_t, nnz):
+    """int64 [nnz]: CSR position of the transpose of every entry, from an edge-id
+    map and its transposed map (entries no slot covers stay -1)."""
+    dev = _need_gpu("map_transpose_perm", emap, emap_t)
+    _need_i32("map_transpose_perm", emap, emap_t)
+    if emap.numel() != emap_t.numel():
+        raise ValueError("map and transposed map differ in size")
+    perm = torch.full((nnz,), -1, dtype=torch.int64, device=dev)
+    check(lib().attgnn_map_transpose_perm_device(_ptr(emap), _ptr(emap_t), emap.numel(), nnz, _ptr(perm),
+                                                 _stream(dev)), "attgnn_map_transpose_perm_device")
+    return perm
+
+
 # --------------------------------------------------------------------------
 # SpMM
 # --------------------------------------------------------------------------

@@ -81,11 +81,39 @@ def prepare_host(csr, features, part_size=SERVE_PART_SIZE):
             "X": torch.from_numpy(X).to(torch.bfloat16)}
 
 
+def prepare_device(csr, features, device, stream, part_size=SERVE_PART_SIZE):
+    """Device-side preprocessing of one graph for both models: the padded CSR,
+    its normalised values and the features are uploaded on `stream` and the
+    window blocks are built there (bit-identical to :func:`prepare_host`)."""
+    p = csr.padded(WINDOW)
+    d = G.gcn_norm_degree(p)
+    d[~np.isfinite(d)] = 0.0
+    vals = np.asarray(G.gcn_norm_values(p, d), np.float32)
+    X = np.zeros((p.n, F_IN), np.float32)
+    X[: csr.n] = features
+    with torch.cuda.stream(stream):
+        up = lambda t: t.pin_memory().to(device, non_blocking=True)
+        rowptr = up(torch.from_numpy(p.rowptr32()))
+        ccol = up(torch.from_numpy(p.indices))
+        gb = ops.blocks_build_device(rowptr, ccol, "edge_f32", up(torch.from_numpy(vals)), torch.bfloat16, WINDOW,
+                                     WIDE, part_size, emit_empty=True)
+        mb = ops.blocks_build_device(rowptr, ccol, "map", None, torch.int32, WINDOW, WIDE, part_size,
+                                     emit_empty=True)
+        Xd = up(torch.from_numpy(X).to(torch.bfloat16))
+    return {"n": csr.n, "n_pad": p.n, "nnz": csr.nnz,
+            "gcn": {k: gb[k] for k in ("rp", "col", "val", "window", "atomic")},
+            "gat": {"rowptr": rowptr, "ccol": ccol, "rp": mb["rp"], "col": mb["col"], "map": mb["val"],
+                    "window": mb["window"], "atomic": mb["atomic"]},
+            "X": Xd}
+
+
 class CachedGraph:
     def __init__(self, key, host, device, stream):
+        """`host`: the dict of prepare_host (uploaded here) or of prepare_device
+        (already resident; built on `stream`)."""
         self.key, self.n, self.n_pad, self.nnz = key, host["n"], host["n_pad"], host["nnz"]
         with torch.cuda.stream(stream):
-            up = lambda t: t.pin_memory().to(device, non_blocking=True)
+            up = lambda t: t if t.is_cuda else t.pin_memory().to(device, non_blocking=True)
             self.gcn = {k: up(v) for k, v in host["gcn"].items()}
             self.gat = {k: up(v) for k, v in host["gat"].items()}
             self.X = up(host["X"])
@@ -364,9 +392,12 @@ class Request:
 class Server:
     def __init__(self, device, models, graph_source, hbm_budget=200 << 30,
                  capacities=(1 << 12, 1 << 14, 1 << 16, 1 << 18), use_graph=True, keep_results=False,
-                 max_in_flight=2, part_size=SERVE_PART_SIZE):
+                 max_in_flight=2, part_size=SERVE_PART_SIZE, preprocess="device"):
+        if preprocess not in ("host", "device"):
+            raise ValueError(f"preprocess must be 'host' or 'device', got {preprocess!r}")
         self.device, self.models, self.source = device, models, graph_source
         self.part_size = part_size
+        self.preprocess = preprocess
         self.cache = GraphCache(hbm_budget)
         self.stream = torch.cuda.Stream(device)        # compute (default priority)
         self.copy_stream = torch.cuda.Stream(device)   # uploads of new graphs
@@ -388,7 +419,11 @@ class Server:
         csr, feats = self.source(req.key)
         if not self.cache.make_room(estimate_bytes(csr), self.reserved):
             return None
-        cg = CachedGraph(req.key, prepare_host(csr, feats, self.part_size), self.device, self.copy_stream)
+        if self.preprocess == "device":
+            built = prepare_device(csr, feats, self.device, self.copy_stream, self.part_size)
+        else:
+            built = prepare_host(csr, feats, self.part_size)
+        cg = CachedGraph(req.key, built, self.device, self.copy_stream)
         self.cache.put(cg)
         return cg
 

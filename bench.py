@@ -86,6 +86,8 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="graph size fraction (testing only)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--part-size", type=int, default=0)
+    ap.add_argument("--build", choices=["host", "device"], default="device",
+                    help="where the window blocks are built (results are bit-identical)")
     ap.add_argument("--mode", choices=["train", "kernels"], default="train",
                     help="kernels: only the width-256 SpMM and a 1 GiB copy (for counter runs)")
     args = ap.parse_args()
@@ -110,10 +112,11 @@ def main():
     cpu = None if args.no_cpu_baseline else cpu_baseline(g, vals)
     if cpu:
         log(f"cpu baseline: {cpu}")
+    t = time.perf_counter()
     graph = GCNGraph(g, dev, window=16, wide=32, part_size=args.part_size or None, tile_dtype=torch.float16,
-                     symmetric=True, values=np.concatenate([vals, np.zeros(0, np.float32)]))
-    log(f"window blocks: U={graph.n_unique} parts={graph.fwd['parts']} cols={graph.fwd['n_cols']} "
-        f"part_size={graph.part_size} ({time.perf_counter() - t:.1f}s)")
+                     symmetric=True, values=np.concatenate([vals, np.zeros(0, np.float32)]), build=args.build)
+    log(f"window blocks ({args.build} build): U={graph.n_unique} parts={graph.fwd['parts']} "
+        f"cols={graph.fwd['n_cols']} part_size={graph.part_size} ({time.perf_counter() - t:.1f}s)")
 
     rng = torch.Generator().manual_seed(1)
     if args.mode == "kernels":

@@ -164,6 +164,19 @@ int attgnn_blocks_fill_device(const int32_t *row, int64_t rows_plus1, const int3
                               int32_t *window_of_part, int32_t *atomic, attgnn_stream_t stream);
 
 /*
+ * perm[map[s]] = map_t[s] for every tile slot s with map[s] >= 0: from the
+ * edge-id map and transposed map of a structure that covers every CSR entry
+ * once (rows padded to the window, no duplicate entries), the CSR position of
+ * the transpose of each entry (-1 where the transposed entry is absent);
+ * entries no slot covers are left untouched.
+ *
+ * Replaces the column-sum indexing of ATT_SpMM_v{1,3}.spmm_*_h_m_rowsum called
+ * with degrees_trans (end2end_head/gat_final/mgat_csr/mgat_conv.py:42-48).
+ */
+int attgnn_map_transpose_perm_device(const int32_t *map, const int32_t *map_t, int64_t n_val, int64_t nnz,
+                                     int64_t *perm, attgnn_stream_t stream);
+
+/*
  * Compact tiles on the device, bit-identical to attgnn_blocks_compact_export:
  * count writes masks[n_cols] and block_base[n_cols/32 + 1] (block_base[n_cols/32]
  * = n_values, which the caller reads to size values / positions); fill packs

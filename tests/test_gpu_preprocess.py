@@ -167,3 +167,61 @@ def test_reference_module_preprocess_gpu_fs(cuda):
     for a, b in zip(got[:6], want):
         assert a.device.type == "cuda"
         assert torch.equal(a.cpu(), b)
+
+
+def _tensors_equal(a, b, where):
+    a, b = a.detach().cpu(), b.detach().cpu()
+    if a.dtype in (torch.float16, torch.bfloat16):
+        a, b = a.view(torch.int16), b.view(torch.int16)
+    assert a.shape == b.shape and torch.equal(a, b), where
+
+
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_gcn_graph_device_build_equals_host(symmetric, cuda):
+    from att_amd import graphs as G
+    from att_amd.layers import GCNGraph
+    if symmetric:
+        g = G.chung_lu(3000, 9, 2.1, seed=4)
+    else:
+        rng = np.random.default_rng(4)
+        rp, col = rand_csr(rng, 2500, 14, hub_rows=2)
+        g = G.CSR(rp.astype(np.int64), col, rp.size - 1)
+    host = GCNGraph(g, cuda, tile_dtype=torch.bfloat16, build="host")
+    dev = GCNGraph(g, cuda, tile_dtype=torch.bfloat16, build="device")
+    assert host.symmetric == dev.symmetric == symmetric
+    for which in ("fwd", "bwd"):
+        h, d = getattr(host, which), getattr(dev, which)
+        for i, (a, b) in enumerate(zip(h["arrays"] + h["compact"], d["arrays"] + d["compact"])):
+            _tensors_equal(a, b, (which, i))
+    X = torch.randn(g.n, 64, device=cuda).bfloat16()
+    _tensors_equal(host.spmm(X), dev.spmm(X), "spmm")
+    _tensors_equal(host.spmm(X, transpose=True), dev.spmm(X, transpose=True), "spmm^T")
+
+
+def test_gat_graph_device_build_equals_host(cuda):
+    from att_amd import graphs as G
+    from att_amd.layers import GATGraph
+    g = G.directed_chung_lu_symmetrized(5000, 60_000, seed=2)
+    host = GATGraph(g, cuda, build="host")
+    dev = GATGraph(g, cuda, build="device")
+    for which in ("fwd", "edges", "bwd"):
+        for i, (a, b) in enumerate(zip(getattr(host, which), getattr(dev, which))):
+            _tensors_equal(a, b, (which, i))
+    _tensors_equal(host.t_rowptr, dev.t_rowptr, "t_rowptr")
+    _tensors_equal(host.t_perm, dev.t_perm, "t_perm")
+
+
+def test_serving_prepare_device_equals_host(cuda):
+    from att_amd import serve as S
+    pool = S.GraphPool(6, seed=5, n_max=5000)
+    stream = torch.cuda.Stream(cuda)
+    for key in range(6):
+        csr, feats = pool(key)
+        h = S.prepare_host(csr, feats)
+        d = S.prepare_device(csr, feats, cuda, stream)
+        stream.synchronize()
+        assert (h["n"], h["n_pad"], h["nnz"]) == (d["n"], d["n_pad"], d["nnz"])
+        for part in ("gcn", "gat"):
+            for k in h[part]:
+                _tensors_equal(h[part][k], d[part][k], (key, part, k))
+        _tensors_equal(h["X"], d["X"], (key, "X"))
