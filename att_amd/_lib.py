@@ -57,6 +57,9 @@ _SIGNATURES = {
     "attgnn_blocks_compact_count_device": (_int, [_vp, _int, _i64, _int, _vp, _vp, _vp]),
     "attgnn_blocks_compact_fill_device": (_int, [_vp, _int, _i64, _int, _vp, _vp, _vp, _vp, _vp]),
     "attgnn_map_transpose_perm_device": (_int, [_vp, _vp, _i64, _i64, _vp, _vp]),
+    "attgnn_attention_node_bwd_workspace_elems": (_i64, [_i64, _i64, _i64]),
+    "attgnn_attention_node_bwd": (_int, [_vp, _vp, _int, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                         _vp, _vp, _i64, _vp]),
     "attgnn_pack_segments": (_int, [_vp, _i64, _vp]),
     "attgnn_pack_segment_bytes": (_i64, []),
 }

@@ -378,6 +378,158 @@ __global__ __launch_bounds__(256) void rowsum_map_kernel(const int32_t *rp, cons
     }
 }
 
+
+// Backward of the per-node attention terms s0 = Xs.a0, s1 = Xt.a1 over
+// [n][heads][dim] features (C = heads*dim columns per row), fused with the
+// aggregation gradient:
+//   dxs[i,c] = ds0[h,i] a0[c]                         (dxs != null)
+//   dxt[i,c] = base[i,c] + ds1[h,i] a1[c] (+ ds0[h,i] a0[c] when dxs == null)
+//   partial[y][0][c] = sum_{i in chunk y} ds0[h,i] xs[i,c],  [y][1][c] likewise
+// with h = c / dim.  A 256-thread workgroup is 64 column lanes (V columns
+// each, one head) x 4 row lanes; rows are walked in order per lane and the 4
+// row lanes are combined in order, so the partials are deterministic.
+struct NodeBwdArgs {
+    const uint8_t *xs, *xt;
+    const float *a0, *a1, *ds0, *ds1, *base;
+    uint8_t *dxs, *dxt;
+    float *partial;
+    int64_t n, heads, dim, C;
+    int x_type;
+};
+
+template <int V>
+__device__ __forceinline__ void load_vec(const uint8_t *p, int type, float (&v)[V])
+{
+    if constexpr (V == 4) {
+        if (type == ATTGNN_F32) {
+            const f32x4 x = *reinterpret_cast<const f32x4 *>(p);
+            v[0] = x[0]; v[1] = x[1]; v[2] = x[2]; v[3] = x[3];
+            return;
+        }
+        const u32x2 x = *reinterpret_cast<const u32x2 *>(p);
+        if (type == ATTGNN_BF16) {
+            v[0] = __uint_as_float(x[0] << 16); v[1] = __uint_as_float(x[0] & 0xffff0000u);
+            v[2] = __uint_as_float(x[1] << 16); v[3] = __uint_as_float(x[1] & 0xffff0000u);
+        } else {
+            const _Float16 *h = reinterpret_cast<const _Float16 *>(&x);
+            v[0] = float(h[0]); v[1] = float(h[1]); v[2] = float(h[2]); v[3] = float(h[3]);
+        }
+    } else {
+        const int eb = type_bytes(type);
+#pragma unroll
+        for (int k = 0; k < V; ++k) v[k] = load_elt(p + k * eb, type);
+    }
+}
+
+template <int V>
+__device__ __forceinline__ void store_vec(uint8_t *p, int type, const float (&v)[V])
+{
+    if constexpr (V == 4) {
+        if (type == ATTGNN_F32) {
+            *reinterpret_cast<f32x4 *>(p) = f32x4{v[0], v[1], v[2], v[3]};
+            return;
+        }
+        uint32_t h[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            h[k] = type == ATTGNN_BF16 ? bf16_bits_rne(v[k]) : uint32_t(__builtin_bit_cast(uint16_t, _Float16(v[k])));
+        *reinterpret_cast<u32x2 *>(p) = u32x2{h[0] | h[1] << 16, h[2] | h[3] << 16};
+    } else {
+        const int eb = type_bytes(type);
+#pragma unroll
+        for (int k = 0; k < V; ++k) store_elt(p + k * eb, type, v[k]);
+    }
+}
+
+template <int V>
+__global__ __launch_bounds__(256) void attn_node_bwd_kernel(NodeBwdArgs a)
+{
+    __shared__ float red[2][4][64 * V];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int64_t c0 = (int64_t(blockIdx.x) * 64 + tx) * V;
+    const bool active = c0 < a.C;
+    const int64_t h = active ? c0 / a.dim : 0;
+    const int eb = type_bytes(a.x_type);
+    float av0[V], av1[V], acc0[V], acc1[V];
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        av0[k] = active ? a.a0[c0 + k] : 0.f;
+        av1[k] = active ? a.a1[c0 + k] : 0.f;
+        acc0[k] = acc1[k] = 0.f;
+    }
+    const int64_t per = (a.n + gridDim.y - 1) / gridDim.y;
+    const int64_t r0 = int64_t(blockIdx.y) * per, r1 = min(a.n, r0 + per);
+    if (active) {
+        const bool same = a.xs == a.xt;
+        for (int64_t i = r0 + ty; i < r1; i += 4) {
+            const float g0 = a.ds0[h * a.n + i], g1 = a.ds1[h * a.n + i];
+            const int64_t off = i * a.C + c0;
+            float xs[V], xt[V], b[V], o[V];
+            load_vec<V>(a.xs + off * eb, a.x_type, xs);
+            if (same) {
+#pragma unroll
+                for (int k = 0; k < V; ++k) xt[k] = xs[k];
+            } else {
+                load_vec<V>(a.xt + off * eb, a.x_type, xt);
+            }
+            if (a.base) load_vec<V>(reinterpret_cast<const uint8_t *>(a.base + off), ATTGNN_F32, b);
+            else {
+#pragma unroll
+                for (int k = 0; k < V; ++k) b[k] = 0.f;
+            }
+#pragma unroll
+            for (int k = 0; k < V; ++k) {
+                acc0[k] += g0 * xs[k];
+                acc1[k] += g1 * xt[k];
+            }
+            if (a.dxs) {
+#pragma unroll
+                for (int k = 0; k < V; ++k) o[k] = g0 * av0[k];
+                store_vec<V>(a.dxs + off * eb, a.x_type, o);
+#pragma unroll
+                for (int k = 0; k < V; ++k) o[k] = b[k] + g1 * av1[k];
+            } else {
+#pragma unroll
+                for (int k = 0; k < V; ++k) o[k] = b[k] + g1 * av1[k] + g0 * av0[k];
+            }
+            store_vec<V>(a.dxt + off * eb, a.x_type, o);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        red[0][ty][tx * V + k] = acc0[k];
+        red[1][ty][tx * V + k] = acc1[k];
+    }
+    __syncthreads();
+    // 256 threads combine 2 x 64V values over the 4 row lanes, in order
+    for (int e = threadIdx.x; e < 2 * 64 * V; e += 256) {
+        const int which = e / (64 * V), c = e - which * 64 * V;
+        const int64_t col = int64_t(blockIdx.x) * 64 * V + c;
+        if (col >= a.C) continue;
+        const float sum = ((red[which][0][c] + red[which][1][c]) + red[which][2][c]) + red[which][3][c];
+        a.partial[(int64_t(blockIdx.y) * 2 + which) * a.C + col] = sum;
+    }
+}
+
+// da[which][c] = sum over chunks y (in order) of partial[y][which][c].
+__global__ __launch_bounds__(256) void attn_node_bwd_reduce_kernel(const float *partial, int64_t chunks, int64_t C,
+                                                                   float *da0, float *da1)
+{
+    const int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (e >= 2 * C) return;
+    const int which = int(e / C);
+    const int64_t c = e - which * C;
+    float s = 0.f;
+    for (int64_t y = 0; y < chunks; ++y) s += partial[(y * 2 + which) * C + c];
+    (which ? da1 : da0)[c] = s;
+}
+
+void node_bwd_grid(int64_t n, int64_t C, int64_t &gx, int64_t &gy)
+{
+    gx = (C + 255) / 256;   // columns of a workgroup: 64 lanes x 4 (one lane's worth for V = 1)
+    gy = std::max<int64_t>(1, std::min<int64_t>((n + 63) / 64, std::max<int64_t>(1, 2048 / gx)));
+}
+
 __global__ void fill_f32_kernel(float *p, int64_t n, float v)
 {
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
@@ -546,6 +698,46 @@ extern "C" int attgnn_sddmm_edges(const int32_t *rp, const int32_t *col, const i
         ATTGNN_EDGES_CASE(3)
     }
 #undef ATTGNN_EDGES_CASE
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? ATTGNN_OK : int(e);
+}
+
+extern "C" int64_t attgnn_attention_node_bwd_workspace_elems(int64_t n_rows, int64_t heads, int64_t dim)
+{
+    if (n_rows < 0 || heads <= 0 || dim <= 0) return -1;
+    int64_t gx, gy;
+    node_bwd_grid(n_rows, heads * dim, gx, gy);
+    return 2 * gy * heads * dim;
+}
+
+extern "C" int attgnn_attention_node_bwd(const void *xs, const void *xt, int x_type, int64_t n_rows, int64_t heads,
+                                         int64_t dim, const float *a0, const float *a1, const float *ds0,
+                                         const float *ds1, const float *base, void *dxs, void *dxt, float *da0,
+                                         float *da1, float *workspace, int64_t workspace_elems,
+                                         attgnn_stream_t stream)
+{
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (n_rows < 0 || heads <= 0 || dim <= 0 || !is_float_type(x_type)) return ATTGNN_E_ARG;
+    if (!a0 || !a1 || !da0 || !da1 || !workspace) return ATTGNN_E_ARG;
+    if (n_rows > 0 && (!xs || !xt || !ds0 || !ds1 || !dxt)) return ATTGNN_E_ARG;
+    if (!dxs && xs != xt) return ATTGNN_E_ARG;   // combined output needs one shared input
+    if (workspace_elems < attgnn_attention_node_bwd_workspace_elems(n_rows, heads, dim)) return ATTGNN_E_ARG;
+    const int64_t C = heads * dim;
+    int64_t gx, gy;
+    node_bwd_grid(n_rows, C, gx, gy);
+    NodeBwdArgs a{static_cast<const uint8_t *>(xs), static_cast<const uint8_t *>(xt), a0, a1, ds0, ds1, base,
+                  static_cast<uint8_t *>(dxs), static_cast<uint8_t *>(dxt), workspace, n_rows, heads, dim, C,
+                  x_type};
+    const int eb = type_bytes(x_type);
+    auto al = [&](const void *p, int bytes) { return p == nullptr || reinterpret_cast<uintptr_t>(p) % bytes == 0; };
+    const bool vec = dim % 4 == 0 && al(xs, 4 * eb) && al(xt, 4 * eb) && al(dxs, 4 * eb) && al(dxt, 4 * eb) &&
+                     al(base, 16);
+    if (vec)
+        hipLaunchKernelGGL(attn_node_bwd_kernel<4>, dim3(unsigned((C + 255) / 256), unsigned(gy)), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(attn_node_bwd_kernel<1>, dim3(unsigned((C + 63) / 64), unsigned(gy)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(attn_node_bwd_reduce_kernel, dim3(unsigned((2 * C + 255) / 256)), dim3(256), 0, s,
+                       static_cast<const float *>(workspace), gy, C, da0, da1);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? ATTGNN_OK : int(e);
 }

@@ -144,8 +144,9 @@ class MGCNFunction(torch.autograd.Function):
         X, weights = ctx.saved_tensors
         g = ctx.inputInfo
         d_input_prime = g.spmm(d_output.to(X.dtype).contiguous(), transpose=True)
-        d_input = torch.mm(d_input_prime, weights.t())
-        d_weights = torch.mm(X.t(), d_input_prime)
+        # dX only when the input needs it (the first layer's features do not)
+        d_input = torch.mm(d_input_prime, weights.t()) if ctx.needs_input_grad[0] else None
+        d_weights = torch.mm(X.t(), d_input_prime) if ctx.needs_input_grad[1] else None
         return d_input, d_weights, None
 
 
@@ -273,6 +274,7 @@ class GATAttention(torch.autograd.Function):
                               x_layout="mhd")
         ctx.save_for_backward(Xs, Xt, a0, a1, s0, s1, lse, p)
         ctx.graph, ctx.slope, ctx.drop_p, ctx.seed = g, slope, drop_p, seed
+        ctx.shared = Xs is Xt
         return h.to(Xt.dtype)
 
     @staticmethod
@@ -289,14 +291,13 @@ class GATAttention(torch.autograd.Function):
         dp = g.sddmm(dh, Xt)
         dz, ds0 = ops.edge_softmax_bwd(g.rowptr, g.col, s0, s1, lse, dp, ctx.slope, ctx.drop_p, ctx.seed)
         ds1 = ops.gather_rowsum(g.t_rowptr, g.t_perm, dz, n)
-        Xs32, Xt32 = Xs.float(), Xt.float()
-        # s0 = Xs.a0, s1 = Xt.a1
-        da0 = (ds0.unsqueeze(1) @ Xs32.permute(1, 0, 2).contiguous()).squeeze(1)   # [H, D]
-        da1 = (ds1.unsqueeze(1) @ Xt32.permute(1, 0, 2).contiguous()).squeeze(1)
-        dXs = ds0.t().unsqueeze(-1) * a0.float().unsqueeze(0)
-        dXt = dXt + ds1.t().unsqueeze(-1) * a1.float().unsqueeze(0)
-        return (dXs.to(Xs.dtype), dXt.to(Xt.dtype), da0.to(a0.dtype).reshape(a0.shape),
-                da1.to(a1.dtype).reshape(a1.shape), None, None, None, None)
+        # s0 = Xs.a0, s1 = Xt.a1: node-term gradients fused with the aggregation
+        # gradient (one pass over X; GAT's shared X W gets the summed gradient)
+        shared = ctx.shared and Xs.data_ptr() == Xt.data_ptr()
+        dXs, dXt, da0, da1 = ops.attention_node_bwd(Xs.contiguous(), Xt.contiguous(), a0, a1, ds0, ds1,
+                                                    base=dXt, combine=shared)
+        return (dXs, dXt, da0.to(a0.dtype).reshape(a0.shape), da1.to(a1.dtype).reshape(a1.shape),
+                None, None, None, None)
 
 
 class GATConv_multi(nn.Module):

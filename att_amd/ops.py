@@ -410,6 +410,41 @@ def node_scores(X, vec, layout="hmd"):
     return s
 
 
+def attention_node_bwd(Xs, Xt, a0, a1, ds0, ds1, base=None, combine=False):
+    """Fused backward of s0 = Xs.a0, s1 = Xt.a1 plus the aggregation gradient.
+
+    Xs, Xt [N, H, D] (same dtype), a0 / a1 [H, D], ds0 / ds1 fp32 [H, N],
+    base fp32 [N, H, D] or None.  Returns (dXs, dXt, da0, da1): dXs = ds0 a0,
+    dXt = base + ds1 a1 (in X's dtype), da0 / da1 fp32 [H, D].  combine=True
+    (Xs is Xt): dXs is None and dXt holds the sum of both gradients.
+    """
+    dev = _need_gpu("attention_node_bwd", Xs, Xt, ds0, ds1, base)
+    if Xs.dtype != Xt.dtype or Xs.shape != Xt.shape or Xs.dim() != 3:
+        raise ValueError("Xs and Xt must be [N, H, D] tensors of one dtype")
+    if combine and Xs.data_ptr() != Xt.data_ptr():
+        raise ValueError("combine=True needs Xs and Xt to be the same tensor")
+    n, heads, dim = Xs.shape
+    for t, nm in ((ds0, "ds0"), (ds1, "ds1")):
+        if t.dtype != torch.float32 or tuple(t.shape) != (heads, n):
+            raise ValueError(f"{nm} must be fp32 [H, N]")
+    if base is not None and (base.dtype != torch.float32 or tuple(base.shape) != (n, heads, dim)):
+        raise ValueError("base must be fp32 [N, H, D]")
+    a0f = a0.reshape(heads, dim).float().contiguous()
+    a1f = a1.reshape(heads, dim).float().contiguous()
+    dXt = torch.empty_like(Xt)
+    dXs = None if combine else torch.empty_like(Xs)
+    da0 = torch.empty((heads, dim), dtype=torch.float32, device=dev)
+    da1 = torch.empty((heads, dim), dtype=torch.float32, device=dev)
+    L = lib()
+    ws_n = L.attgnn_attention_node_bwd_workspace_elems(n, heads, dim)
+    ws = torch.empty(max(ws_n, 1), dtype=torch.float32, device=dev)
+    check(L.attgnn_attention_node_bwd(_ptr(Xs), _ptr(Xt), _TYPE_OF[Xs.dtype], n, heads, dim, _ptr(a0f), _ptr(a1f),
+                                      _ptr(ds0), _ptr(ds1), _ptr(base), _ptr(dXs), _ptr(dXt), _ptr(da0), _ptr(da1),
+                                      _ptr(ws), ws_n, _stream(dev)),
+          "attgnn_attention_node_bwd")
+    return dXs, dXt, da0, da1
+
+
 def edge_from_nodes(rp, col, emap, window_of_part, nnz, heads, n_rows, window, wide, out_dtype,
                     s0=None, s1=None, g0=None, g0_strides=None):
     """out[h, e=(i,j)] = S0[h, i] + S1[h, j] over the map's edges ([H, nnz])."""

@@ -362,6 +362,30 @@ int attgnn_edge_from_nodes(const int32_t *rp, const int32_t *col, const int32_t 
                            int g0_type, int64_t heads, int64_t n_rows, void *out, int out_type,
                            int64_t nnz, int window, int wide, attgnn_stream_t stream);
 
+/*
+ * Backward of the per-node attention terms s0 = Xs.a0 and s1 = Xt.a1, fused
+ * with the aggregation gradient, over [n_rows][heads][dim] features (x_type
+ * F32 / F16 / BF16, contiguous) with a0, a1 [heads][dim] fp32 and
+ * ds0, ds1 [heads][n_rows] fp32 (dL/ds0, dL/ds1):
+ *   dxs[i,h,:] = ds0[h,i] a0[h,:]
+ *   dxt[i,h,:] = base[i,h,:] + ds1[h,i] a1[h,:]       (base fp32, NULL = 0)
+ *   da0[h,:]   = sum_i ds0[h,i] xs[i,h,:],  da1[h,:] = sum_i ds1[h,i] xt[i,h,:]
+ * dxs == NULL (only when xs == xt, as in GAT where both are X W): the two
+ * gradients are summed into dxt.  dxs / dxt in x_type, rounded once;
+ * da0 / da1 fp32, reduced in a fixed order (deterministic).
+ * workspace: attgnn_attention_node_bwd_workspace_elems() floats.
+ *
+ * Replaces the attention-vector and feature gradients of the reference's
+ * backward (end2end_head/gat_final/mgat_csr/mgat_conv.py:34-69, 91-151,
+ * where a0 is paired with column sums and cross terms are added), with the
+ * exact gradient.
+ */
+int64_t attgnn_attention_node_bwd_workspace_elems(int64_t n_rows, int64_t heads, int64_t dim);
+int attgnn_attention_node_bwd(const void *xs, const void *xt, int x_type, int64_t n_rows, int64_t heads,
+                              int64_t dim, const float *a0, const float *a1, const float *ds0, const float *ds1,
+                              const float *base, void *dxs, void *dxt, float *da0, float *da1, float *workspace,
+                              int64_t workspace_elems, attgnn_stream_t stream);
+
 /* ====================================================================== */
 /* Fused edge softmax over CSR rows (device)                               */
 /* ====================================================================== */

@@ -169,3 +169,42 @@ def test_gat_dropout_mask_regenerated(cuda):
     assert rel(layer.fc.weight.grad, W.grad) < 1e-3
     assert rel(layer.a0.grad.view(heads, dim), a0.grad) < 1e-3
     assert torch.allclose(p_drop.cpu()[kept], p_keep.cpu()[kept] / (1 - q), rtol=1e-6)
+
+
+@pytest.mark.parametrize("dtype,dim,shared", [(torch.float32, 64, False), (torch.bfloat16, 64, True),
+                                              (torch.float16, 32, False), (torch.bfloat16, 6, False),
+                                              (torch.float32, 10, True)])
+def test_attention_node_bwd_fused(dtype, dim, shared, cuda):
+    """attgnn_attention_node_bwd against float64 torch: feature gradients (with the
+    aggregation gradient folded in) and attention-vector gradients; dim 6 / 10
+    take the scalar path.  Deterministic across runs."""
+    gen = torch.Generator().manual_seed(dim)
+    n, heads = 3001, 3
+    Xs = torch.randn(n, heads, dim, generator=gen).to(dtype)
+    Xt = Xs if shared else torch.randn(n, heads, dim, generator=gen).to(dtype)
+    a0, a1 = torch.randn(heads, dim, generator=gen), torch.randn(heads, dim, generator=gen)
+    ds0, ds1 = torch.randn(heads, n, generator=gen), torch.randn(heads, n, generator=gen)
+    base = torch.randn(n, heads, dim, generator=gen)
+    Xs_d = Xs.to(cuda)
+    Xt_d = Xs_d if shared else Xt.to(cuda)
+    out = ops.attention_node_bwd(Xs_d, Xt_d, a0.to(cuda), a1.to(cuda), ds0.to(cuda), ds1.to(cuda),
+                                 base=base.to(cuda), combine=shared)
+    again = ops.attention_node_bwd(Xs_d, Xt_d, a0.to(cuda), a1.to(cuda), ds0.to(cuda), ds1.to(cuda),
+                                   base=base.to(cuda), combine=shared)
+    xs, xt = Xs.double(), Xt.double()
+    g0 = ds0.double().t().unsqueeze(-1) * a0.double().unsqueeze(0)
+    g1 = ds1.double().t().unsqueeze(-1) * a1.double().unsqueeze(0)
+    want_da0 = torch.einsum("hi,ihd->hd", ds0.double(), xs)
+    want_da1 = torch.einsum("hi,ihd->hd", ds1.double(), xt)
+    want_dxt = base.double() + g1 + (g0 if shared else 0)
+    rel = lambda a, b: ((a.double().cpu() - b).norm() / b.norm()).item()
+    tol_x = 1e-6 if dtype == torch.float32 else (4e-3 if dtype == torch.float16 else 8e-3)
+    assert rel(out[1], want_dxt) < tol_x
+    if not shared:
+        assert rel(out[0], g0) < tol_x
+    else:
+        assert out[0] is None
+    assert rel(out[2], want_da0) < 1e-5
+    assert rel(out[3], want_da1) < 1e-5
+    for a, b in zip(out, again):
+        assert (a is None and b is None) or torch.equal(a, b)
