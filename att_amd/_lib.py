@@ -37,7 +37,7 @@ _SIGNATURES = {
     "attgnn_sddmm_edges": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _int, _i64,
                                   _i64, _i64, _vp, _int, _i64, _int, _int, _vp]),
     "attgnn_spmm_edge_map": (_int, [_vp, _vp, _vp, _vp, _int, _i64, _vp, _vp, _i64, _vp, _int, _int, _i64, _i64,
-                                    _vp, _i64, _i64, _int, _int, _int, _int, _vp, _i64, _vp]),
+                                    _vp, _int, _i64, _i64, _int, _int, _int, _int, _vp, _i64, _int, _vp]),
     "attgnn_rowsum_edge_map": (_int, [_vp, _vp, _vp, _i64, _vp, _int, _i64, _i64, _vp, _int, _i64, _int, _int,
                                       _vp]),
     "attgnn_sddmm_block": (_int, [_vp, _vp, _vp, _vp, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _int, _i64, _i64,

@@ -511,17 +511,24 @@ __global__ __launch_bounds__(256) void attn_node_bwd_kernel(NodeBwdArgs a)
     }
 }
 
-// da[which][c] = sum over chunks y (in order) of partial[y][which][c].
-__global__ __launch_bounds__(256) void attn_node_bwd_reduce_kernel(const float *partial, int64_t chunks, int64_t C,
-                                                                   float *da0, float *da1)
+// Sum of consecutive chunk partials, in order: out[g][e] = sum over
+// y in [g*G, min(g*G + G, Y)) of in[y][e] (E = 2C values per chunk).  When one
+// group remains the sums go straight to da0 (e < C) / da1 (e >= C).
+constexpr int CHUNK_GROUP = 64;
+
+__global__ __launch_bounds__(256) void chunk_sum_kernel(const float *in, int64_t Y, int64_t E, float *out,
+                                                        float *da0, float *da1, int64_t C)
 {
-    const int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x;
-    if (e >= 2 * C) return;
-    const int which = int(e / C);
-    const int64_t c = e - which * C;
+    const int64_t groups = (Y + CHUNK_GROUP - 1) / CHUNK_GROUP;
+    const int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (t >= groups * E) return;
+    const int64_t g = t / E, e = t - g * E;
+    const int64_t y1 = min(Y, (g + 1) * CHUNK_GROUP);
     float s = 0.f;
-    for (int64_t y = 0; y < chunks; ++y) s += partial[(y * 2 + which) * C + c];
-    (which ? da1 : da0)[c] = s;
+    for (int64_t y = g * CHUNK_GROUP; y < y1; ++y) s += in[y * E + e];
+    if (groups > 1) out[g * E + e] = s;
+    else if (e < C) da0[e] = s;
+    else da1[e - C] = s;
 }
 
 void node_bwd_grid(int64_t n, int64_t C, int64_t &gx, int64_t &gy)
@@ -707,7 +714,8 @@ extern "C" int64_t attgnn_attention_node_bwd_workspace_elems(int64_t n_rows, int
     if (n_rows < 0 || heads <= 0 || dim <= 0) return -1;
     int64_t gx, gy;
     node_bwd_grid(n_rows, heads * dim, gx, gy);
-    return 2 * gy * heads * dim;
+    // chunk partials, then the partials of groups of CHUNK_GROUP chunks
+    return 2 * heads * dim * (gy + (gy + CHUNK_GROUP - 1) / CHUNK_GROUP);
 }
 
 extern "C" int attgnn_attention_node_bwd(const void *xs, const void *xt, int x_type, int64_t n_rows, int64_t heads,
@@ -736,8 +744,20 @@ extern "C" int attgnn_attention_node_bwd(const void *xs, const void *xt, int x_t
         hipLaunchKernelGGL(attn_node_bwd_kernel<4>, dim3(unsigned((C + 255) / 256), unsigned(gy)), dim3(256), 0, s, a);
     else
         hipLaunchKernelGGL(attn_node_bwd_kernel<1>, dim3(unsigned((C + 63) / 64), unsigned(gy)), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(attn_node_bwd_reduce_kernel, dim3(unsigned((2 * C + 255) / 256)), dim3(256), 0, s,
-                       static_cast<const float *>(workspace), gy, C, da0, da1);
+    // two fixed-order levels (gy <= 2048 <= CHUNK_GROUP^2)
+    const int64_t E = 2 * C;
+    float *level2 = workspace + gy * E;
+    int64_t Y = gy;
+    const float *src = workspace;
+    while (true) {
+        const int64_t groups = (Y + CHUNK_GROUP - 1) / CHUNK_GROUP;
+        hipLaunchKernelGGL(chunk_sum_kernel, dim3(unsigned((groups * E + 255) / 256)), dim3(256), 0, s, src, Y, E,
+                           level2, da0, da1, C);
+        if (groups == 1) break;
+        src = level2;
+        Y = groups;
+        level2 = workspace;   // the chunk partials are consumed; reuse their space
+    }
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? ATTGNN_OK : int(e);
 }

@@ -642,8 +642,8 @@ extern "C" int attgnn_spmm_edge_map(const int32_t *rp, const int32_t *col, const
                                     const void *edge_val, int val_type, int64_t nnz,
                                     const int32_t *window_of_part, const int32_t *atomic, int64_t parts,
                                     const void *X, int x_type, int x_layout, int64_t heads, int64_t n_x,
-                                    float *Y, int64_t m_ori, int64_t dim_n, int out_layout, int window,
-                                    int wide, int mode, float *workspace, int64_t workspace_elems,
+                                    void *Y, int y_type, int64_t m_ori, int64_t dim_n, int out_layout, int window,
+                                    int wide, int mode, float *workspace, int64_t workspace_elems, int flags,
                                     attgnn_stream_t stream)
 {
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
@@ -655,7 +655,7 @@ extern "C" int attgnn_spmm_edge_map(const int32_t *rp, const int32_t *col, const
     if (wl < 0) return ATTGNN_E_ARG;
     const int kind = kind_of(mode, val_type, x_type);
     if (kind < 0 || kind == K_FP8) return ATTGNN_E_UNSUPPORTED;
-    if (!Y) return ATTGNN_E_ARG;
+    if (!Y || !(y_type == ATTGNN_F32 || y_type == ATTGNN_F16 || y_type == ATTGNN_BF16)) return ATTGNN_E_ARG;
     if (parts > 0 && (!rp || !col || !map || !edge_val || !window_of_part || !atomic || !X || !workspace))
         return ATTGNN_E_ARG;
     if (parts > 0 && workspace_elems < heads * attgnn_spmm_workspace_elems(parts, window, dim_n)) return ATTGNN_E_ARG;
@@ -676,9 +676,9 @@ extern "C" int attgnn_spmm_edge_map(const int32_t *rp, const int32_t *col, const
     if (out_layout == ATTGNN_LAYOUT_HMD) { a.y_head_stride = m_ori * dim_n; a.y_row_stride = dim_n; }
     else { a.y_head_stride = dim_n; a.y_row_stride = heads * dim_n; }
     a.slices = int((dim_n + 16 * t - 1) / (16 * t));
-    a.heads = int(heads); a.window = window; a.wide_log2 = wl; a.y_type = ATTGNN_F32; a.pipe = 1;
+    a.heads = int(heads); a.window = window; a.wide_log2 = wl; a.y_type = y_type; a.pipe = 1;
     a.xcd_slices = a.slices > 1 && 8 % a.slices == 0;
-    const hipError_t e = run_spmm(kind, A_MAPPED, t, a, false, s);
+    const hipError_t e = run_spmm(kind, A_MAPPED, t, a, (flags & ATTGNN_SPMM_ROWS_COVERED) != 0, s);
     return e == hipSuccess ? ATTGNN_OK : int(e);
 }
 

@@ -271,11 +271,11 @@ class GATAttention(torch.autograd.Function):
         p, lse = ops.edge_softmax_fwd(g.rowptr, g.col, s0, s1, slope, drop_p, seed, p_dtype=Xt.dtype)
         rp, col, emap, win, at = g.fwd
         h = ops.spmm_edge_map(rp, col, emap, p, win, at, Xt, "mhd", g.window, g.wide, m_ori=g.num_nodes_ori,
-                              x_layout="mhd")
+                              x_layout="mhd", out_dtype=Xt.dtype, rows_covered=True)
         ctx.save_for_backward(Xs, Xt, a0, a1, s0, s1, lse, p)
         ctx.graph, ctx.slope, ctx.drop_p, ctx.seed = g, slope, drop_p, seed
         ctx.shared = Xs is Xt
-        return h.to(Xt.dtype)
+        return h
 
     @staticmethod
     def backward(ctx, dh):
@@ -286,7 +286,7 @@ class GATAttention(torch.autograd.Function):
         # aggregation: dXt[j] = sum_i p[(i,j)] dh[i]  (A^T edge-map SpMM)
         rp, col, emap_t, win, at = g.bwd
         dXt = ops.spmm_edge_map(rp, col, emap_t, p, win, at, dh, "mhd", g.window, g.wide, m_ori=n,
-                                x_layout="mhd")
+                                x_layout="mhd", rows_covered=True)
         # dp[(i,j)] = <dh[i], Xt[j]>  (SDDMM over A)
         dp = g.sddmm(dh, Xt)
         dz, ds0 = ops.edge_softmax_bwd(g.rowptr, g.col, s0, s1, lse, dp, ctx.slope, ctx.drop_p, ctx.seed)

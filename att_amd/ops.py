@@ -289,9 +289,11 @@ def spmm_compact(rp, col, masks, block_base, values, window_of_part, atomic, X, 
 
 
 def spmm_edge_map(rp, col, emap, edge_val, window_of_part, atomic, X, out_layout, window, wide,
-                  m_ori=None, mode=None, x_layout="hmd"):
+                  m_ori=None, mode=None, x_layout="hmd", out_dtype=torch.float32, rows_covered=False):
     """Edge-map SpMM.  X [H, N, D] ("hmd") or [N, H, D] ("mhd"); edge_val [H, nnz]
-    (or [nnz]); Y fp32 [H, M, D] ("hmd") or [M, H, D] ("mhd")."""
+    (or [nnz]); Y [H, M, D] ("hmd") or [M, H, D] ("mhd") in ``out_dtype`` (FP32
+    accumulation).  ``rows_covered``: every window up to M has a part, so Y
+    needs no zero-fill."""
     dev = _need_gpu("spmm_edge_map", rp, col, emap, edge_val, window_of_part, atomic, X)
     _need_i32("spmm_edge_map", rp, col, emap, window_of_part, atomic)
     if X.dim() == 2:
@@ -307,7 +309,9 @@ def spmm_edge_map(rp, col, emap, edge_val, window_of_part, atomic, X, out_layout
     m = n_x if m_ori is None else m_ori
     layout = {"hmd": _lib.LAYOUT_HMD, "mhd": _lib.LAYOUT_MHD}[out_layout]
     shape = (heads, m, dim) if layout == _lib.LAYOUT_HMD else (m, heads, dim)
-    y = torch.empty(shape, dtype=torch.float32, device=dev)
+    if out_dtype not in (torch.float32, torch.float16, torch.bfloat16):
+        raise TypeError(f"output type {out_dtype} is not supported")
+    y = torch.empty(shape, dtype=out_dtype, device=dev)
     parts = window_of_part.numel()
     ws_n = heads * lib().attgnn_spmm_workspace_elems(parts, window, dim) if parts else 0
     ws = torch.empty(max(ws_n, 1), dtype=torch.float32, device=dev)
@@ -317,8 +321,9 @@ def spmm_edge_map(rp, col, emap, edge_val, window_of_part, atomic, X, out_layout
     xl = {"hmd": _lib.LAYOUT_HMD, "mhd": _lib.LAYOUT_MHD}[x_layout]
     rc = lib().attgnn_spmm_edge_map(_ptr(rp), _ptr(col), _ptr(emap), _ptr(ev), _TYPE_OF[ev.dtype], nnz,
                                     _ptr(window_of_part), _ptr(atomic), parts, _ptr(X), _TYPE_OF[X.dtype], xl,
-                                    heads, n_x, _ptr(y), m, dim, layout, window, wide, _mode(mode, X.dtype), _ptr(ws),
-                                    ws_n, _stream(dev))
+                                    heads, n_x, _ptr(y), _TYPE_OF[out_dtype], m, dim, layout, window, wide,
+                                    _mode(mode, X.dtype), _ptr(ws), ws_n,
+                                    _lib.SPMM_ROWS_COVERED if rows_covered else 0, _stream(dev))
     check(rc, "attgnn_spmm_edge_map")
     return y
 

@@ -269,7 +269,10 @@ int attgnn_spmm_compact(const int32_t *rp, const int32_t *col, const uint16_t *m
  *            test_spmm_m_h.py:49-84)
  *
  * edge_val[heads][nnz] (F16/BF16/F32, matching x_type), X in x_layout
- * (HMD [heads][n_x][dim_n] or MHD [n_x][heads][dim_n]), Y fp32 in out_layout (HMD: [heads][m_ori][dim_n], MHD: [m_ori][heads][dim_n]).
+ * (HMD [heads][n_x][dim_n] or MHD [n_x][heads][dim_n]), Y of y_type (F32 /
+ * F16 / BF16, FP32 accumulation rounded once) in out_layout (HMD:
+ * [heads][m_ori][dim_n], MHD: [m_ori][heads][dim_n]).  flags: 0 or
+ * ATTGNN_SPMM_ROWS_COVERED (see attgnn_spmm_block).
  * mode: NATIVE for F16/BF16; BF16X3 or F32 for F32.  workspace: at least
  * heads * attgnn_spmm_workspace_elems(parts, window, dim_n) floats.
  */
@@ -277,8 +280,8 @@ int attgnn_spmm_edge_map(const int32_t *rp, const int32_t *col, const int32_t *m
                          const void *edge_val, int val_type, int64_t nnz,
                          const int32_t *window_of_part, const int32_t *atomic, int64_t parts,
                          const void *X, int x_type, int x_layout, int64_t heads, int64_t n_x,
-                         float *Y, int64_t m_ori, int64_t dim_n, int out_layout, int window,
-                         int wide, int mode, float *workspace, int64_t workspace_elems,
+                         void *Y, int y_type, int64_t m_ori, int64_t dim_n, int out_layout, int window,
+                         int wide, int mode, float *workspace, int64_t workspace_elems, int flags,
                          attgnn_stream_t stream);
 
 /*
