@@ -128,6 +128,46 @@ class GCNGraph:
                               out_dtype=out_dtype or X.dtype, rows_covered=True)
 
 
+def weight_grad(X, dY, min_rows_per_split=2048, max_splits=64):
+    """X^T dY (fp32, or fp64 for fp64 inputs) for a long row dimension N: a plain GEMM gets a handful of
+    output tiles that each walk all N rows, so the rows are split into S chunks
+    run as one batched product (fp16 / bf16 operands, FP32 accumulation) whose
+    S partial products are summed in FP32 in a fixed order."""
+    acc = torch.promote_types(X.dtype, torch.float32)
+    n = X.shape[0]
+    S = max(1, min(max_splits, n // min_rows_per_split))
+    if S == 1:
+        return torch.mm(X.t(), dY).to(acc)
+    m = (n // S) * S
+    part = torch.bmm(X[:m].view(S, m // S, X.shape[1]).transpose(1, 2),
+                     dY[:m].contiguous().view(S, m // S, dY.shape[1]))
+    out = part.to(acc).sum(0)
+    if m < n:
+        out += torch.mm(X[m:].t(), dY[m:]).to(acc)
+    return out
+
+
+class LinearSplitK(torch.autograd.Function):
+    """Y = X W^T with W cast to X's dtype (nn.Linear without bias); the weight
+    gradient uses :func:`weight_grad` and is returned in W's dtype."""
+
+    @staticmethod
+    def forward(ctx, X, W):
+        Wc = W.to(X.dtype)
+        ctx.save_for_backward(X, Wc)
+        ctx.w_dtype = W.dtype
+        return F.linear(X, Wc)
+
+    @staticmethod
+    def backward(ctx, dY):
+        X, Wc = ctx.saved_tensors
+        dY = dY.to(X.dtype)
+        dX = dY.reshape(-1, dY.shape[-1]) @ Wc if ctx.needs_input_grad[0] else None
+        dW = weight_grad(X.reshape(-1, X.shape[-1]), dY.reshape(-1, dY.shape[-1])).t().to(ctx.w_dtype) \
+            if ctx.needs_input_grad[1] else None
+        return (dX.view(X.shape) if dX is not None else None), dW
+
+
 class MGCNFunction(torch.autograd.Function):
     """Y = A_hat . (X W); backward dXW = A_hat^T dY, dX = dXW W^T, dW = X^T dXW
     (end2end_ori/gcn_final/mgat_csr/mgat_conv.py:27-74)."""
@@ -146,7 +186,7 @@ class MGCNFunction(torch.autograd.Function):
         d_input_prime = g.spmm(d_output.to(X.dtype).contiguous(), transpose=True)
         # dX only when the input needs it (the first layer's features do not)
         d_input = torch.mm(d_input_prime, weights.t()) if ctx.needs_input_grad[0] else None
-        d_weights = torch.mm(X.t(), d_input_prime) if ctx.needs_input_grad[1] else None
+        d_weights = weight_grad(X, d_input_prime).to(weights.dtype) if ctx.needs_input_grad[1] else None
         return d_input, d_weights, None
 
 
@@ -317,7 +357,7 @@ class GATConv_multi(nn.Module):
         self._step = 0
 
     def forward(self, X, inputInfo, seed=None):
-        Xp = F.linear(X.to(self.dtype), self.fc.weight.to(self.dtype)).view(-1, self.head, self.output_dim)
+        Xp = LinearSplitK.apply(X.to(self.dtype), self.fc.weight).view(-1, self.head, self.output_dim)
         drop = self.dropout if self.training else 0.0
         if seed is None:
             seed = self._step
@@ -346,10 +386,8 @@ class GATv2Conv_multi(nn.Module):
 
     def forward(self, X, inputInfo, seed=None):
         Xd = X.to(self.dtype)
-        X0 = F.leaky_relu(F.linear(Xd, self.fc0.weight.to(self.dtype)), self.alpha).view(-1, self.head,
-                                                                                         self.output_dim)
-        X1 = F.leaky_relu(F.linear(Xd, self.fc1.weight.to(self.dtype)), self.alpha).view(-1, self.head,
-                                                                                         self.output_dim)
+        X0 = F.leaky_relu(LinearSplitK.apply(Xd, self.fc0.weight), self.alpha).view(-1, self.head, self.output_dim)
+        X1 = F.leaky_relu(LinearSplitK.apply(Xd, self.fc1.weight), self.alpha).view(-1, self.head, self.output_dim)
         drop = self.dropout if self.training else 0.0
         if seed is None:
             seed = self._step
