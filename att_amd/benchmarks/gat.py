@@ -40,6 +40,7 @@ def main():
     ap.add_argument("--reps", type=int, default=10)
     ap.add_argument("--nodes", type=int, default=169_343)
     ap.add_argument("--nnz", type=int, default=2_480_000)
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP32 path and the SDDMM variants")
     args = ap.parse_args()
     dev = torch.device("cuda", 0)
     t = time.perf_counter()
@@ -56,7 +57,7 @@ def main():
     # columns summed over 16-row windows), flops 2*nnz*D*H.
     from att_amd import ops
     rp, col, emap, win, _ = graph.fwd
-    for heads in [int(h) for h in args.heads.split(",")]:
+    for heads in ([] if args.no_plain else [int(h) for h in args.heads.split(",")]):
         A = torch.randn(g.n, heads, dim, device=dev).bfloat16()
         B = torch.randn(g.n, heads, dim, device=dev).bfloat16()
         variants = {"dense_map": lambda: ops.sddmm_block(rp, col, emap, win, A, B, g.nnz, graph.window, graph.wide,
@@ -102,6 +103,10 @@ def main():
                     out.backward(go)
 
             t_ours = timeit(ours, args.reps)
+            if args.no_plain:
+                print(json.dumps({"layer": kind, "heads": heads, "dim": dim, "nodes": g.n, "nnz": g.nnz,
+                                  "ms_fwd_bwd": round(t_ours, 3)}), flush=True)
+                continue
             t_plain = timeit(plain, max(args.reps // 2, 3))
             # accuracy of the forward output against the FP32 path on the same inputs
             with torch.no_grad():
