@@ -19,6 +19,15 @@ SPMM_ROWS_COVERED = 1
 _c = ctypes
 _vp, _i64, _int = _c.c_void_p, _c.c_int64, _c.c_int
 
+
+class Segments(ctypes.Structure):
+    """attgnn_segments (include/attgnn.h)."""
+    _fields_ = [("seg_ptr", _c.c_void_p), ("seg_row", _c.c_void_p), ("seg_slot", _c.c_void_p),
+                ("row_seg", _c.c_void_p), ("n_seg", _c.c_int64), ("n_slots", _c.c_int64)]
+
+
+_SEGP = _c.POINTER(Segments)
+
 _SIGNATURES = {
     "attgnn_status_string": (_c.c_char_p, [_int]),
     "attgnn_blocks_build": (_int, [_vp, _i64, _vp, _i64, _int, _vp, _int, _int, _int, _int, _int, _int, _int,
@@ -60,6 +69,12 @@ _SIGNATURES = {
     "attgnn_attention_node_bwd_workspace_elems": (_i64, [_i64, _i64, _i64]),
     "attgnn_attention_node_bwd": (_int, [_vp, _vp, _int, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                          _vp, _vp, _i64, _vp]),
+    "attgnn_gatv2_fwd": (_int, [_vp, _vp, _SEGP, _i64, _i64, _i64, _i64, _vp, _vp, _int, _vp, _c.c_float,
+                                _c.c_float, _c.c_uint64, _vp, _vp, _vp, _int, _vp]),
+    "attgnn_gatv2_bwd_workspace_elems": (_i64, [_i64, _i64, _SEGP, _SEGP]),
+    "attgnn_gatv2_bwd": (_int, [_vp, _vp, _SEGP, _vp, _vp, _SEGP, _i64, _i64, _i64, _i64, _vp, _vp, _int, _vp,
+                                _c.c_float, _c.c_float, _c.c_uint64, _vp, _vp, _vp, _int, _vp, _vp, _vp, _vp, _vp,
+                                _vp, _i64, _vp]),
     "attgnn_pack_segments": (_int, [_vp, _i64, _vp]),
     "attgnn_pack_segment_bytes": (_i64, []),
 }

@@ -15,7 +15,7 @@ import numpy as np
 import torch
 
 from att_amd import graphs as G
-from att_amd.benchmarks.plain import gat_fp32
+from att_amd.benchmarks.plain import gat_fp32, gatv2_true_fp32
 from att_amd.layers import GATConv_multi, GATGraph, GATv2Conv_multi
 
 
@@ -45,7 +45,7 @@ def main():
     dev = torch.device("cuda", 0)
     t = time.perf_counter()
     g = G.directed_chung_lu_symmetrized(args.nodes, args.nnz, seed=0)
-    graph = GATGraph(g, dev)
+    graph = GATGraph(g, dev, build="device")
     print(f"graph N={g.n} nnz={g.nnz} U16={graph.n_unique} ({time.perf_counter() - t:.1f}s)", file=sys.stderr)
     rows = torch.from_numpy(g.row_ids().astype(np.int64)).to(dev)
     cols = torch.from_numpy(g.indices.astype(np.int64)).to(dev)
@@ -71,13 +71,14 @@ def main():
                               "ms": round(ms, 4), "gflops": round(2.0 * g.nnz * dim * heads / ms / 1e6, 1),
                               "algorithmic_gbs": round(nbytes / ms / 1e6, 1),
                               "hbm_frac": round(nbytes / ms / 1e6 / 8000.0, 3)}), flush=True)
-    for kind in ("gat", "gatv2"):
+    for kind in ("gat", "gatv2", "gatv2_true"):
         for heads in [int(h) for h in args.heads.split(",")]:
             torch.manual_seed(heads)
             if kind == "gat":
                 layer = GATConv_multi(128, dim, 0.0, 0.2, heads, dtype=torch.bfloat16).to(dev)
             else:
-                layer = GATv2Conv_multi(128, dim, 0.0, 0.2, heads, dtype=torch.bfloat16).to(dev)
+                layer = GATv2Conv_multi(128, dim, 0.0, 0.2, heads, dtype=torch.bfloat16,
+                                        true_gatv2=(kind == "gatv2_true")).to(dev)
             go = torch.randn(g.n, heads, dim, device=dev)
 
             def ours():
@@ -93,13 +94,21 @@ def main():
                 def plain():
                     out = gat_fp32(X, W, a0, a1, rows, cols, g.n, heads, dim)
                     out.backward(go)
-            else:
+            elif kind == "gatv2":
                 W0 = layer.fc0.weight.detach().clone().requires_grad_(True)
                 W1 = layer.fc1.weight.detach().clone().requires_grad_(True)
                 a = layer.a0.detach().clone().requires_grad_(True)
 
                 def plain():
                     out = gat_fp32(X, W0, a, a, rows, cols, g.n, heads, dim, slope=1.0, Wt=W1, v2_alpha=0.2)
+                    out.backward(go)
+            else:
+                W0 = layer.fc0.weight.detach().clone().requires_grad_(True)
+                W1 = layer.fc1.weight.detach().clone().requires_grad_(True)
+                a = layer.a0.detach().clone().view(heads, dim).requires_grad_(True)
+
+                def plain():
+                    out = gatv2_true_fp32(X, W0, W1, a, rows, cols, g.n, heads, dim, slope=0.2)
                     out.backward(go)
 
             t_ours = timeit(ours, args.reps)
@@ -113,9 +122,12 @@ def main():
                 out = layer(Xb, graph).float()
                 if kind == "gat":
                     ref = gat_fp32(Xb.float(), W.bfloat16().float(), a0, a1, rows, cols, g.n, heads, dim)
-                else:
+                elif kind == "gatv2":
                     ref = gat_fp32(Xb.float(), W0.bfloat16().float(), a, a, rows, cols, g.n, heads, dim, slope=1.0,
                                    Wt=W1.bfloat16().float(), v2_alpha=0.2)
+                else:
+                    ref = gatv2_true_fp32(Xb.float(), W0.bfloat16().float(), W1.bfloat16().float(), a, rows, cols,
+                                          g.n, heads, dim)
                 err = ((out - ref).norm() / ref.norm()).item()
             print(json.dumps({"layer": kind, "heads": heads, "dim": dim, "nodes": g.n, "nnz": g.nnz,
                               "ms_fwd_bwd": round(t_ours, 3), "plain_fp32_ms": round(t_plain, 3),

@@ -28,3 +28,16 @@ def gat_fp32(X, W, a0, a1, rows, cols, n, heads, dim, slope=0.2, Wt=None, v2_alp
     s = torch.zeros(n, heads, device=X.device).index_add(0, rows, ex)
     p = ex / s[rows]
     return torch.zeros(n, heads, dim, device=X.device).index_add(0, rows, p.unsqueeze(-1) * Xt[cols])
+
+
+def gatv2_true_fp32(X, W0, W1, a, rows, cols, n, heads, dim, slope=0.2):
+    """GATv2 as its paper defines it, in FP32 message passing: [N, H, D]."""
+    X0 = (X @ W0.t()).view(n, heads, dim)
+    X1 = (X @ W1.t()).view(n, heads, dim)
+    z = (F.leaky_relu(X0[rows] + X1[cols], slope) * a.view(1, heads, dim)).sum(-1)   # [E, H]
+    idx = rows.unsqueeze(1).expand(-1, heads)
+    m = torch.full((n, heads), float("-inf"), device=X.device).scatter_reduce(0, idx, z, "amax", include_self=True)
+    ex = torch.exp(z - m[rows])
+    s = torch.zeros(n, heads, device=X.device).index_add(0, rows, ex)
+    p = ex / s[rows]
+    return torch.zeros(n, heads, dim, device=X.device).index_add(0, rows, p.unsqueeze(-1) * X1[cols])

@@ -173,3 +173,27 @@ def window_unique_columns(csr, window):
     rows = csr.row_ids().astype(np.int64) // window
     keys = rows * csr.n + csr.indices.astype(np.int64)
     return int(np.unique(keys).size)
+
+
+def csr_segments(indptr, max_len=32):
+    """Segments of CSR rows of at most `max_len` consecutive entries, in the
+    layout of ``attgnn_segments`` (include/attgnn.h): int32 arrays seg_ptr
+    [n_seg+1], seg_row [n_seg], seg_slot [n_seg] (-1 for a row's only segment,
+    else its index among the segments of rows that have several) and row_seg
+    [n+1], plus n_seg and n_slots."""
+    indptr = np.asarray(indptr, np.int64)
+    n = indptr.size - 1
+    nseg = (np.diff(indptr) + max_len - 1) // max_len          # 0 for an empty row
+    row_seg = np.zeros(n + 1, np.int64)
+    row_seg[1:] = np.cumsum(nseg)
+    n_seg = int(row_seg[-1])
+    seg_row = np.repeat(np.arange(n, dtype=np.int64), nseg)
+    within = np.arange(n_seg, dtype=np.int64) - row_seg[seg_row]
+    seg_ptr = np.append(indptr[seg_row] + within * max_len, indptr[-1])
+    multi = nseg[seg_row] > 1
+    seg_slot = np.where(multi, np.cumsum(multi) - 1, -1)
+    if n_seg + 1 >= 2**31 or indptr[-1] >= 2**31:
+        raise OverflowError("segments do not fit int32")
+    return {"seg_ptr": seg_ptr.astype(np.int32), "seg_row": seg_row.astype(np.int32),
+            "seg_slot": seg_slot.astype(np.int32), "row_seg": row_seg.astype(np.int32),
+            "n_seg": n_seg, "n_slots": int(multi.sum())}

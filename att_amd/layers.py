@@ -267,6 +267,7 @@ class GATGraph:
         if symmetric:
             self.bwd = [b["rp"], b["col"], b["map_t"], b["window"], b["atomic"]]
             self.t_rowptr = self.rowptr
+            self.t_col = self.col
             self.t_perm = perm
             self.row_pointers, self.column_index, self.degrees, self.window_t, self.automic = self.fwd
             self.degrees_trans = self.bwd[2]
@@ -280,9 +281,21 @@ class GATGraph:
                     bt["window"].to(device), bt["atomic"].to(device)]
         self.t_rowptr = torch.from_numpy(t.rowptr32()[: csr.n + 1].copy()).to(device)
         self.t_perm = torch.from_numpy(order[: int(t.indptr[csr.n])]).to(device)
+        self.t_col = torch.from_numpy(t.indices[: int(t.indptr[csr.n])].copy()).to(device)
         # reference names
         self.row_pointers, self.column_index, self.degrees, self.window_t, self.automic = self.fwd
         self.degrees_trans = self.bwd[2]
+
+    def segments(self):
+        """Row segments of A and of A^T for the per-edge attention kernels
+        (built on first use, kept on the device)."""
+        if getattr(self, "_segments", None) is None:
+            rp = self.rowptr.cpu().numpy()
+            trp = self.t_rowptr.cpu().numpy()
+            dev = self.rowptr.device
+            self._segments = (ops.DeviceSegments(G.csr_segments(rp), dev),
+                              ops.DeviceSegments(G.csr_segments(trp), dev))
+        return self._segments
 
     def sddmm(self, A, B, out_dtype=torch.float32):
         """out[h, e=(i,j)] = <A[i,h], B[j,h]> over A's edges ([N, H, D] inputs, [H, nnz] out)."""
@@ -340,6 +353,40 @@ class GATAttention(torch.autograd.Function):
                 None, None, None, None)
 
 
+class GATv2Attention(torch.autograd.Function):
+    """GATv2 as its paper defines it: h[i,h,:] = sum_j softmax_j(a_h .
+    LeakyReLU(X0[i,h] + X1[j,h])) X1[j,h,:], dropout on the probabilities.
+    X0 (target side) and X1 (source side, the messages) [N, H, D]; a [H, D]."""
+
+    @staticmethod
+    def forward(ctx, X0, X1, a, graph, slope, drop_p, seed):
+        g = graph
+        segs, _ = g.segments()
+        z, lse, p = ops.gatv2_fwd(g.rowptr, g.col, segs, X0, X1, a, slope, drop_p, seed, p_dtype=X1.dtype)
+        rp, col, emap, win, at = g.fwd
+        h = ops.spmm_edge_map(rp, col, emap, p, win, at, X1, "mhd", g.window, g.wide, m_ori=g.num_nodes_ori,
+                              x_layout="mhd", out_dtype=X1.dtype, rows_covered=True)
+        ctx.save_for_backward(X0, X1, a, z, lse, p)
+        ctx.graph, ctx.slope, ctx.drop_p, ctx.seed = g, slope, drop_p, seed
+        return h
+
+    @staticmethod
+    def backward(ctx, dh):
+        X0, X1, a, z, lse, p = ctx.saved_tensors
+        g = ctx.graph
+        n = X1.shape[0]
+        dh = dh.to(X1.dtype).contiguous()
+        rp, col, emap_t, win, at = g.bwd
+        # messages: dX1[j] = sum_i p[(i,j)] dh[i]  (A^T edge-map SpMM)
+        base = ops.spmm_edge_map(rp, col, emap_t, p, win, at, dh, "mhd", g.window, g.wide, m_ori=n,
+                                 x_layout="mhd", rows_covered=True)
+        dp = g.sddmm(dh, X1)            # dL/dp[(i,j)] = <dh[i], X1[j]>
+        segs, t_segs = g.segments()
+        _, dX0, dX1, da = ops.gatv2_bwd(g.rowptr, g.col, segs, g.t_col, g.t_perm, t_segs, X0, X1, a, ctx.slope, z,
+                                        lse, dp, ctx.drop_p, ctx.seed, base=base)
+        return dX0, dX1, da.to(a.dtype).reshape(a.shape), None, None, None, None
+
+
 class GATConv_multi(nn.Module):
     """end2end_head/gat_final/mgat_csr/mgat_conv.py:154-199 (forward returns [N, H, D])."""
 
@@ -369,10 +416,15 @@ class GATConv_multi(nn.Module):
 class GATv2Conv_multi(nn.Module):
     """end2end_head/gatv2_final/mgat_csr/mgat_conv.py:151-198 with the single-head
     file's logits (end2end_ori/gatv2_final/mgat_csr/mgat_conv.py:174-196):
-    X0 = LReLU(X W0), X1 = LReLU(X W1), logit = a.X0[i] + a.X1[j], h = sum p X1."""
+    X0 = LReLU(X W0), X1 = LReLU(X W1), logit = a.X0[i] + a.X1[j], h = sum p X1.
 
-    def __init__(self, input_dim, output_dim, dropout, alpha, head=1, dtype=torch.bfloat16):
+    ``true_gatv2=True``: the attention of the GATv2 paper instead,
+    X0 = X W0, X1 = X W1, logit = a . LReLU(X0[i] + X1[j]), h = sum p X1
+    (fused per-edge kernels; no counterpart in the reference)."""
+
+    def __init__(self, input_dim, output_dim, dropout, alpha, head=1, dtype=torch.bfloat16, true_gatv2=False):
         super().__init__()
+        self.true_gatv2 = true_gatv2
         self.alpha, self.head, self.output_dim, self.dtype = alpha, head, output_dim, dtype
         gain = nn.init.calculate_gain("relu")
         self.fc0 = nn.Linear(input_dim, output_dim * head, bias=False)
@@ -386,11 +438,15 @@ class GATv2Conv_multi(nn.Module):
 
     def forward(self, X, inputInfo, seed=None):
         Xd = X.to(self.dtype)
-        X0 = F.leaky_relu(LinearSplitK.apply(Xd, self.fc0.weight), self.alpha).view(-1, self.head, self.output_dim)
-        X1 = F.leaky_relu(LinearSplitK.apply(Xd, self.fc1.weight), self.alpha).view(-1, self.head, self.output_dim)
         drop = self.dropout if self.training else 0.0
         if seed is None:
             seed = self._step
             self._step += 1
         a = self.a0.view(self.head, -1)
+        if self.true_gatv2:
+            X0 = LinearSplitK.apply(Xd, self.fc0.weight).view(-1, self.head, self.output_dim)
+            X1 = LinearSplitK.apply(Xd, self.fc1.weight).view(-1, self.head, self.output_dim)
+            return GATv2Attention.apply(X0, X1, a, inputInfo, self.alpha, drop, seed)
+        X0 = F.leaky_relu(LinearSplitK.apply(Xd, self.fc0.weight), self.alpha).view(-1, self.head, self.output_dim)
+        X1 = F.leaky_relu(LinearSplitK.apply(Xd, self.fc1.weight), self.alpha).view(-1, self.head, self.output_dim)
         return GATAttention.apply(X0, X1, a, a, inputInfo, 1.0, drop, seed)

@@ -508,6 +508,74 @@ def edge_softmax_bwd(rowptr, col, s0, s1, lse, dp, slope, drop_p=0.0, seed=0):
     return dz, ds0
 
 
+class DeviceSegments:
+    """Device copy of :func:`att_amd.graphs.csr_segments` plus the C struct."""
+
+    def __init__(self, seg, device):
+        self.arrays = {k: torch.from_numpy(seg[k]).to(device) for k in ("seg_ptr", "seg_row", "seg_slot", "row_seg")}
+        self.n_seg, self.n_slots = seg["n_seg"], seg["n_slots"]
+        self.rows = self.arrays["row_seg"].numel() - 1
+        self.struct = _lib.Segments(*[_ptr(self.arrays[k]).value for k in ("seg_ptr", "seg_row", "seg_slot",
+                                                                             "row_seg")],
+                                    self.n_seg, self.n_slots)
+
+
+def gatv2_fwd(rowptr, col, segs, X0, X1, att, slope, drop_p=0.0, seed=0, p_dtype=None):
+    """GATv2 logits z [H, nnz] (fp32), row log-sum-exp [H, N] and probabilities
+    p [H, nnz] (dropout applied) from X0, X1 [N, H, D] and att [H, D]; `segs`:
+    :class:`DeviceSegments` of the rows."""
+    dev = _need_gpu("gatv2_fwd", rowptr, col, X0, X1)
+    _need_i32("gatv2_fwd", rowptr, col)
+    if X0.shape != X1.shape or X0.dtype != X1.dtype or X0.dim() != 3:
+        raise ValueError("X0 and X1 must be [N, H, D] of one dtype")
+    n, heads, dim = X0.shape
+    nnz = col.numel()
+    if rowptr.numel() != n + 1 or segs.rows != n:
+        raise ValueError("rowptr / segments must cover the N rows")
+    a = att.reshape(heads, dim).float().contiguous()
+    p_dtype = p_dtype or X0.dtype
+    z = torch.empty((heads, nnz), dtype=torch.float32, device=dev)
+    lse = torch.empty((heads, n), dtype=torch.float32, device=dev)
+    p = torch.empty((heads, nnz), dtype=p_dtype, device=dev)
+    check(lib().attgnn_gatv2_fwd(_ptr(rowptr), _ptr(col), ctypes.byref(segs.struct), n, nnz, heads, dim, _ptr(X0),
+                                 _ptr(X1), _TYPE_OF[X0.dtype], _ptr(a), float(slope), float(drop_p),
+                                 int(seed) & (2**64 - 1), _ptr(z), _ptr(lse), _ptr(p), _TYPE_OF[p_dtype],
+                                 _stream(dev)), "attgnn_gatv2_fwd")
+    return z, lse, p
+
+
+def gatv2_bwd(rowptr, col, segs, t_col, t_perm, t_segs, X0, X1, att, slope, z, lse, dp, drop_p=0.0, seed=0,
+              base=None):
+    """Backward of :func:`gatv2_fwd`: (dz [H, nnz] fp32, dX0, dX1 in X's dtype,
+    datt [H, D] fp32); dX1 includes ``base`` (fp32 [N, H, D]) when given.
+    ``t_col`` / ``t_perm`` / ``t_segs``: the transposed CSR's entries (row of A,
+    A edge id) and its segments."""
+    dev = _need_gpu("gatv2_bwd", rowptr, col, t_col, t_perm, X0, X1, z, lse, dp, base)
+    _need_i32("gatv2_bwd", rowptr, col, t_col)
+    if t_perm.dtype != torch.int64:
+        raise TypeError("t_perm must be int64")
+    n, heads, dim = X0.shape
+    nnz = col.numel()
+    if t_col.numel() != nnz or t_perm.numel() != nnz or segs.rows != n or t_segs.rows != n:
+        raise ValueError("transposed CSR / segments do not match A")
+    if base is not None and (base.dtype != torch.float32 or tuple(base.shape) != (n, heads, dim)):
+        raise ValueError("base must be fp32 [N, H, D]")
+    a = att.reshape(heads, dim).float().contiguous()
+    dz = torch.empty((heads, nnz), dtype=torch.float32, device=dev)
+    dX0 = torch.empty_like(X0)
+    dX1 = torch.empty_like(X1)
+    datt = torch.empty((heads, dim), dtype=torch.float32, device=dev)
+    L = lib()
+    ws_n = L.attgnn_gatv2_bwd_workspace_elems(heads, dim, ctypes.byref(segs.struct), ctypes.byref(t_segs.struct))
+    ws = torch.empty(max(ws_n, 1), dtype=torch.float32, device=dev)
+    check(L.attgnn_gatv2_bwd(_ptr(rowptr), _ptr(col), ctypes.byref(segs.struct), _ptr(t_col), _ptr(t_perm),
+                             ctypes.byref(t_segs.struct), n, nnz, heads, dim, _ptr(X0), _ptr(X1), _TYPE_OF[X0.dtype],
+                             _ptr(a), float(slope), float(drop_p), int(seed) & (2**64 - 1), _ptr(z), _ptr(lse),
+                             _ptr(dp), _TYPE_OF[dp.dtype], _ptr(base), _ptr(dz), _ptr(dX0), _ptr(dX1), _ptr(datt),
+                             _ptr(ws), ws_n, _stream(dev)), "attgnn_gatv2_bwd")
+    return dz, dX0, dX1, datt
+
+
 def gather_rowsum(rowptr, perm, vals, n):
     """out[h, j] = sum_{e in row j} vals[h, perm[e]] (column sums via the transpose permutation)."""
     dev = _need_gpu("gather_rowsum", rowptr, perm, vals)

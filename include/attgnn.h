@@ -389,6 +389,62 @@ int attgnn_attention_node_bwd(const void *xs, const void *xt, int x_type, int64_
                               const float *base, void *dxs, void *dxt, float *da0, float *da1, float *workspace,
                               int64_t workspace_elems, attgnn_stream_t stream);
 
+/*
+ * Segments of a CSR's rows for the per-edge attention kernels: every row is
+ * cut into pieces of at most 32 consecutive entries, so a long row becomes
+ * many short tasks.  Segment s covers entries [seg_ptr[s], seg_ptr[s+1]) of
+ * row seg_row[s]; the segments of row i are [row_seg[i], row_seg[i+1]) (none
+ * for an empty row); seg_slot[s] = -1 when s is its row's only segment, else
+ * its index 0..n_slots-1 among the segments of rows that have several.  All
+ * arrays int32 in device memory; att_amd.graphs.csr_segments builds them.
+ */
+typedef struct attgnn_segments {
+    const int32_t *seg_ptr;   /* [n_seg + 1] */
+    const int32_t *seg_row;   /* [n_seg]     */
+    const int32_t *seg_slot;  /* [n_seg]     */
+    const int32_t *row_seg;   /* [rows + 1]  */
+    int64_t n_seg, n_slots;
+} attgnn_segments;
+
+/*
+ * GATv2 attention as the GATv2 paper defines it, over CSR rows of A:
+ *   z[h*nnz + e] = sum_d att[h,d] LeakyReLU_slope(x0[i,h,d] + x1[j,h,d]), e = (i, j)
+ *   lse[h*n + i] = log sum_{e in row i} exp(z),  p[h*nnz + e] = exp(z - lse)
+ * with dropout on p as attgnn_edge_softmax_fwd (drop_p, seed).  x0, x1
+ * [n][heads][dim] of x_type (F32 / F16 / BF16, contiguous), att fp32
+ * [heads][dim]; dim in {16, 32, 64, 128} or any dim <= 128.  row_segments:
+ * the segments of A's rows.
+ *
+ * No counterpart in the reference: its GATv2 (end2end_ori/gatv2_final/mgat_csr/mgat_conv.py:174-196,
+ * ATT_SDDMM_v*.attv2_*) applies the LeakyReLU before a separable dot product.
+ */
+int attgnn_gatv2_fwd(const int32_t *rowptr, const int32_t *col, const attgnn_segments *row_segments, int64_t n,
+                     int64_t nnz, int64_t heads, int64_t dim, const void *x0, const void *x1, int x_type,
+                     const float *att, float slope, float drop_p, uint64_t seed, float *z, float *lse, void *p,
+                     int p_type, attgnn_stream_t stream);
+
+/*
+ * Backward of attgnn_gatv2_fwd given dp = dL/dp (after dropout, dp_type):
+ *   dz[h*nnz + e]  (softmax backward, fp32)
+ *   dx0[i,h,:] = sum_{e in row i} dz_e att[h,:] * LeakyReLU'(x0[i] + x1[j])
+ *   dx1[j,h,:] = base[j,h,:] + sum_{e = (i, j)} dz_e att[h,:] * LeakyReLU'(x0[i] + x1[j])
+ *   datt[h,:]  = sum_e dz_e LeakyReLU(x0[i] + x1[j])   (fixed-order reduction)
+ * The column sums walk the CSR of A^T given by col_segments (segments of A^T's
+ * rows), t_col (row of A of each A^T entry) and t_perm (A edge id of each A^T
+ * entry, int64).  base fp32 [n][heads][dim] or NULL; dx0 / dx1 of x_type.
+ * workspace: attgnn_gatv2_bwd_workspace_elems() floats.  Deterministic.
+ *
+ * No counterpart in the reference (see attgnn_gatv2_fwd).
+ */
+int64_t attgnn_gatv2_bwd_workspace_elems(int64_t heads, int64_t dim, const attgnn_segments *row_segments,
+                                         const attgnn_segments *col_segments);
+int attgnn_gatv2_bwd(const int32_t *rowptr, const int32_t *col, const attgnn_segments *row_segments,
+                     const int32_t *t_col, const int64_t *t_perm, const attgnn_segments *col_segments, int64_t n,
+                     int64_t nnz, int64_t heads, int64_t dim, const void *x0, const void *x1, int x_type,
+                     const float *att, float slope, float drop_p, uint64_t seed, const float *z, const float *lse,
+                     const void *dp, int dp_type, const float *base, float *dz, void *dx0, void *dx1, float *datt,
+                     float *workspace, int64_t workspace_elems, attgnn_stream_t stream);
+
 /* ====================================================================== */
 /* Fused edge softmax over CSR rows (device)                               */
 /* ====================================================================== */

@@ -208,3 +208,69 @@ def test_attention_node_bwd_fused(dtype, dim, shared, cuda):
     assert rel(out[3], want_da1) < 1e-5
     for a, b in zip(out, again):
         assert (a is None and b is None) or torch.equal(a, b)
+
+
+def gatv2_true_reference(X, W0, W1, a, A, heads, dim, slope):
+    """FP64 dense GATv2 (the paper's attention): [N, H, D]."""
+    n = X.shape[0]
+    X0 = (X @ W0.t()).view(n, heads, dim)
+    X1 = (X @ W1.t()).view(n, heads, dim)
+    U = X0.unsqueeze(1) + X1.unsqueeze(0)                            # [i, j, H, D]
+    z = (F.leaky_relu(U, slope) * a.view(1, 1, heads, dim)).sum(-1)  # [i, j, H]
+    e = z.permute(2, 0, 1).masked_fill(~A.unsqueeze(0), float("-inf"))
+    P = torch.nan_to_num(torch.softmax(e, dim=-1))
+    return torch.einsum("hij,jhd->ihd", P, X1)
+
+
+@pytest.mark.parametrize("heads,dim,symmetric,dtype,tol", [(1, 64, True, torch.float32, 2e-4),
+                                                           (4, 16, False, torch.float32, 2e-4),
+                                                           (2, 24, True, torch.float32, 2e-4),
+                                                           (2, 128, False, torch.float32, 2e-4),
+                                                           (4, 32, True, torch.bfloat16, 3e-2)])
+def test_true_gatv2_vs_fp64(heads, dim, symmetric, dtype, tol, cuda):
+    g = small_graph(200, 5, seed=11, symmetric=symmetric)
+    graph = GATGraph(g, cuda, build="device" if symmetric else "host")
+    torch.manual_seed(4)
+    layer = GATv2Conv_multi(20, dim, 0.0, 0.2, heads, dtype=dtype, true_gatv2=True).to(cuda)
+    X = torch.randn(g.n, 20, device=cuda)
+    out = layer(X, graph)
+    go = torch.randn(out.shape, device=cuda)
+    (out.float() * go).sum().backward()
+    A = dense_adj(g)
+    Xr = X.to(dtype).double().cpu()
+    W0 = layer.fc0.weight.detach().to(dtype).double().cpu().requires_grad_(True)
+    W1 = layer.fc1.weight.detach().to(dtype).double().cpu().requires_grad_(True)
+    a = layer.a0.detach().double().cpu().view(heads, dim).requires_grad_(True)
+    ref = gatv2_true_reference(Xr, W0, W1, a, A, heads, dim, 0.2)
+    (ref * go.double().cpu()).sum().backward()
+    rel = lambda x, y: ((x.double().cpu() - y).norm() / y.norm()).item()
+    assert rel(out.float(), ref) < tol
+    gtol = 1e-3 if dtype == torch.float32 else 5e-2
+    assert rel(layer.fc0.weight.grad, W0.grad) < gtol
+    assert rel(layer.fc1.weight.grad, W1.grad) < gtol
+    assert rel(layer.a0.grad.view(heads, dim), a.grad) < gtol
+
+
+def test_true_gatv2_logits_and_dropout(cuda):
+    g = small_graph(300, 6, seed=12)
+    graph = GATGraph(g, cuda)
+    heads, dim = 2, 16
+    X0 = torch.randn(g.n, heads, dim, device=cuda)
+    X1 = torch.randn(g.n, heads, dim, device=cuda)
+    att = torch.randn(heads, dim, device=cuda)
+    segs, _ = graph.segments()
+    z, lse, p = ops.gatv2_fwd(graph.rowptr, graph.col, segs, X0, X1, att, 0.2)
+    rows = torch.from_numpy(g.row_ids().astype(np.int64)).to(cuda)
+    cols = torch.from_numpy(g.indices.astype(np.int64)).to(cuda)
+    want = (F.leaky_relu(X0[rows].double() + X1[cols].double(), 0.2) * att.double()).sum(-1).t()
+    assert ((z.double() - want).abs().max() / want.abs().max()).item() < 1e-6
+    # probabilities sum to one per row; dropout keeps a deterministic subset scaled by 1/(1-q)
+    sums = torch.zeros(heads, g.n, device=cuda, dtype=torch.float64).index_add_(1, rows, p.double())
+    nonempty = torch.from_numpy(np.diff(g.indptr) > 0).to(cuda)
+    assert (sums[:, nonempty] - 1).abs().max().item() < 1e-5
+    _, _, pd = ops.gatv2_fwd(graph.rowptr, graph.col, segs, X0, X1, att, 0.2, drop_p=0.4, seed=7)
+    _, _, pd2 = ops.gatv2_fwd(graph.rowptr, graph.col, segs, X0, X1, att, 0.2, drop_p=0.4, seed=7)
+    assert torch.equal(pd, pd2)
+    kept = pd != 0
+    assert 0.5 < kept.float().mean().item() < 0.7
+    assert torch.allclose(pd[kept], p[kept] / 0.6, rtol=1e-5)

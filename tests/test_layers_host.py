@@ -28,3 +28,25 @@ def test_linear_split_k_gradients():
     (F.linear(X2, W2) * g).sum().backward()
     assert torch.allclose(X.grad, X2.grad, atol=1e-12)
     assert torch.allclose(W.grad, W2.grad, atol=1e-10)
+
+
+def test_csr_segments_partition_rows():
+    import numpy as np
+    from att_amd.graphs import csr_segments
+    rng = np.random.default_rng(3)
+    deg = rng.integers(0, 5, 400)
+    deg[[3, 4, 200]] = [100, 33, 64]            # long rows, and an exact multiple of the length
+    deg[[7, 8, 9]] = 0
+    indptr = np.concatenate([[0], np.cumsum(deg)])
+    s = csr_segments(indptr, max_len=32)
+    n_seg, sp, sr, ss, rs = s["n_seg"], s["seg_ptr"], s["seg_row"], s["seg_slot"], s["row_seg"]
+    assert sp[0] == 0 and sp[-1] == indptr[-1] and np.all(np.diff(sp) > 0) and np.all(np.diff(sp) <= 32)
+    for i in range(400):
+        segs = range(rs[i], rs[i + 1])
+        assert len(segs) == -(-deg[i] // 32)
+        if len(segs):
+            assert sp[segs[0]] == indptr[i] and sp[segs[-1] + 1] == indptr[i + 1]
+            assert all(sr[k] == i for k in segs)
+            assert all((ss[k] == -1) == (len(segs) == 1) for k in segs)
+    slots = ss[ss >= 0]
+    assert np.array_equal(slots, np.arange(s["n_slots"])) and s["n_slots"] == 4 + 2 + 2
