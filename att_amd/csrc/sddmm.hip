@@ -275,6 +275,114 @@ __global__ __launch_bounds__(256) void sddmm_edges_kernel(SddmmArgs a, const int
     }
 }
 
+// 16-bit inputs with 16-byte aligned rows and dim <= 32*NDC: the gathers of a
+// block's (head, 32-feature chunk) pairs are issued PAIRS at a time before
+// their MFMAs (whole heads per batch), and the next block's column ids and
+// edge list load while this block computes.  Same tiles, same results as
+// sddmm_edges_kernel.
+constexpr int PAIRS = 8;
+
+template <int KIND, int NDC>
+__global__ __launch_bounds__(256) void sddmm_edges_batched_kernel(SddmmArgs a, const int64_t *bbase,
+                                                                  const int32_t *eids, const uint16_t *epos)
+{
+    constexpr int HPB = PAIRS / NDC;   // heads per batch
+    __shared__ float tile[4][16 * 32];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t part = int64_t(blockIdx.x) * 4 + wave;
+    if (part >= a.parts) return;
+    const int c_begin = a.rp[2 * part];
+    const int nreal = a.rp[2 * part + 1] - c_begin;
+    const int g = lane >> 4, n = lane & 15;
+    const int W = a.window;
+    const int64_t row = int64_t(a.win[part]) * W + n;
+    const bool row_ok = n < W && row < a.n_rows;
+    float *S = tile[wave];
+    const int ob = type_bytes(a.out_type);
+
+    // block metadata: column ids of the lane's two columns, the block's edge
+    // range and this lane's first edge
+    int cid[2] = {-1, -1}, cnt = 0, pos = -1, eid = -1;
+    int64_t e0 = 0;
+    auto meta = [&](int kb, int (&c)[2], int64_t &b0, int &bc, int &bp, int &be) {
+        const int64_t blk = (int64_t(c_begin) + kb) >> 5;
+        b0 = bbase[blk];
+        bc = int(bbase[blk + 1] - b0);
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int k = kb + 16 * t + n;
+            c[t] = k < nreal ? a.col[c_begin + k] : -1;
+        }
+        bp = -1;
+        be = -1;
+        if (lane < bc) {
+            bp = epos[b0 + lane];
+            be = eids[b0 + lane];
+        }
+    };
+    if (nreal > 0) meta(0, cid, e0, cnt, pos, eid);
+    for (int kb = 0; kb < nreal; kb += 32) {
+        int ncid[2] = {-1, -1}, ncnt = 0, npos = -1, neid = -1;
+        int64_t ne0 = 0;
+        if (kb + 32 < nreal) meta(kb + 32, ncid, ne0, ncnt, npos, neid);
+        if (cnt > 0) {
+            for (int h0 = 0; h0 < a.heads; h0 += HPB) {
+                const int hn = a.heads - h0 < HPB ? a.heads - h0 : HPB;
+                u32x4 ar[PAIRS], br[PAIRS][2];
+#pragma unroll
+                for (int p = 0; p < PAIRS; ++p) {
+                    const int hl = p / NDC;
+                    const int64_t d = (p % NDC) * 32 + 8 * g;
+                    const int64_t h = h0 + hl;
+                    ar[p] = u32x4{0, 0, 0, 0};
+                    br[p][0] = br[p][1] = u32x4{0, 0, 0, 0};
+                    if (hl < hn && d < a.dim) {
+                        if (row_ok)
+                            ar[p] = *reinterpret_cast<const u32x4 *>(a.a + (h * a.a_hs + row * a.a_rs + d) * 2);
+#pragma unroll
+                        for (int t = 0; t < 2; ++t)
+                            if (cid[t] >= 0)
+                                br[p][t] = *reinterpret_cast<const u32x4 *>(
+                                    a.b + (h * a.b_hs + int64_t(cid[t]) * a.b_rs + d) * 2);
+                    }
+                }
+                f32x4 acc[HPB][2];
+#pragma unroll
+                for (int hl = 0; hl < HPB; ++hl) acc[hl][0] = acc[hl][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int p = 0; p < PAIRS; ++p)
+#pragma unroll
+                    for (int t = 0; t < 2; ++t)
+                        acc[p / NDC][t] = KIND == 0 ? mfma_f16(ar[p], br[p][t], acc[p / NDC][t])
+                                                    : mfma_bf16(ar[p], br[p][t], acc[p / NDC][t]);
+#pragma unroll
+                for (int hl = 0; hl < HPB; ++hl) {
+                    if (hl >= hn) break;
+                    const int64_t h = h0 + hl;
+#pragma unroll
+                    for (int t = 0; t < 2; ++t)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) S[(4 * g + q) * 32 + 16 * t + n] = acc[hl][t][q];
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    if (eid >= 0) store_elt(a.out + (h * a.nnz + eid) * ob, a.out_type, S[pos]);
+                    for (int j = 64 + lane; j < cnt; j += 64) {
+                        const int e = eids[e0 + j];
+                        store_elt(a.out + (h * a.nnz + e) * ob, a.out_type, S[epos[e0 + j]]);
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                }
+            }
+        }
+        cid[0] = ncid[0]; cid[1] = ncid[1];
+        e0 = ne0; cnt = ncnt; pos = npos; eid = neid;
+    }
+}
+
 // s[h*n_rows + i] = sum_d X[h, i, d] * vec[h*dim + d]   (fp32 accumulate)
 // 16 lanes per (h, i).
 __global__ __launch_bounds__(256) void node_score_kernel(const uint8_t *x, int x_type, int64_t x_hs,
@@ -691,6 +799,17 @@ extern "C" int attgnn_sddmm_edges(const int32_t *rp, const int32_t *col, const i
                 static_cast<uint8_t *>(out), parts, nnz, n_rows, dim, a_head_stride, a_row_stride,
                 b_head_stride, b_row_stride, int(heads), window, 5, in_type, out_type};
     const dim3 grid{static_cast<unsigned>((parts + 3) / 4)}, block{256};
+    if (vec && eb == 2 && dim <= 128) {
+        const int ndc = int((dim + 31) / 32);
+#define ATTGNN_BATCHED(K, N) hipLaunchKernelGGL((sddmm_edges_batched_kernel<K, N>), grid, block, 0, s, a, block_base, \
+                                                edge_ids, positions)
+        if (ndc == 1) { if (kind == 0) ATTGNN_BATCHED(0, 1); else ATTGNN_BATCHED(1, 1); }
+        else if (ndc == 2) { if (kind == 0) ATTGNN_BATCHED(0, 2); else ATTGNN_BATCHED(1, 2); }
+        else { if (kind == 0) ATTGNN_BATCHED(0, 4); else ATTGNN_BATCHED(1, 4); }
+#undef ATTGNN_BATCHED
+        const hipError_t e = hipGetLastError();
+        return e == hipSuccess ? ATTGNN_OK : int(e);
+    }
 #define ATTGNN_EDGES_CASE(K)                                                                               \
     case K:                                                                                                \
         if (vec) hipLaunchKernelGGL((sddmm_edges_kernel<K, true>), grid, block, 0, s, a, block_base,       \
