@@ -178,10 +178,11 @@ class MGAT_dataset_csr:
 
 
 def synthetic_npz(path, n, mean_degree, num_features=128, num_classes=16, seed=0, gamma=2.1, train_frac=0.6,
-                  val_frac=0.2):
-    """A Chung-Lu graph (symmetric, self-loops) written as a reference-format
-    .npz with N(0,1) features, random labels and random split masks."""
-    g = G.chung_lu(n, mean_degree, gamma, seed=seed)
+                  val_frac=0.2, target_nnz=None):
+    """A Chung-Lu graph (symmetric, self-loops; ~target_nnz entries if given)
+    written as a reference-format .npz with N(0,1) features, random labels and
+    random split masks."""
+    g = G.chung_lu(n, mean_degree, gamma, seed=seed, target_nnz=target_nnz)
     rng = np.random.Generator(np.random.PCG64(seed + 1))
     feats = rng.standard_normal((n, num_features)).astype(np.float32)
     labels = rng.integers(0, num_classes, n)

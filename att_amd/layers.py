@@ -80,30 +80,46 @@ class GCNGraph:
             values = G.gcn_norm_values(padded, d)
         self.part_size = part_size or default_part_size(_blocks_estimate(padded, wide))
         self.tile_dtype = tile_dtype
-        self.fwd = self._build(padded, values, device)
-        if symmetric is None:
-            symmetric = bool(np.array_equal(np.sort(csr.transpose_perm()), np.arange(csr.nnz)))
-        self.symmetric = symmetric
-        if symmetric:
-            self.bwd = self.fwd
+        if build == "device":
+            # structure, symmetry test and A^T all on the device
+            rp_d = torch.from_numpy(padded.rowptr32()).to(device)
+            col_d = torch.from_numpy(padded.indices).to(device)
+            vals_d = torch.from_numpy(np.asarray(values, np.float32)).to(device)
+            self.fwd = self._build_device(rp_d, col_d, vals_d, device)
+            t_rowptr = t_col = t_perm = None
+            if symmetric is None or not symmetric:
+                t_rowptr, t_col, t_perm = ops.csr_transpose_device(rp_d, col_d, padded.n)
+                if symmetric is None:
+                    symmetric = bool(torch.equal(t_rowptr, rp_d) and torch.equal(t_col, col_d))
+            self.symmetric = symmetric
+            self.bwd = self.fwd if symmetric else \
+                self._build_device(t_rowptr, t_col, vals_d.index_select(0, t_perm), device)
         else:
-            t, order = transpose_csr(padded)
-            self.bwd = self._build(t, np.asarray(values)[order], device)
+            self.fwd = self._build(padded, values, device)
+            if symmetric is None:
+                symmetric = bool(np.array_equal(np.sort(csr.transpose_perm()), np.arange(csr.nnz)))
+            self.symmetric = symmetric
+            if symmetric:
+                self.bwd = self.fwd
+            else:
+                t, order = transpose_csr(padded)
+                self.bwd = self._build(t, np.asarray(values)[order], device)
         # reference attribute names
         (self.row_pointers, self.column_index, self.degrees, self.t_window_rowTensor,
          self.t_atomicTensor) = self.fwd["arrays"]
 
+    def _build_device(self, rp_d, col_d, vals_d, device):
+        b = ops.blocks_build_device(rp_d, col_d, "edge_f32", vals_d, self.tile_dtype, self.window, self.wide,
+                                    self.part_size, emit_empty=True, compact=bool(self.compact))
+        return self._finish(b, device)
+
     def _build(self, csr, values, device):
-        if self.build == "device":
-            b = ops.blocks_build_device(torch.from_numpy(csr.rowptr32()).to(device),
-                                        torch.from_numpy(csr.indices).to(device), "edge_f32",
-                                        torch.from_numpy(np.asarray(values, np.float32)).to(device),
-                                        self.tile_dtype, self.window, self.wide, self.part_size, emit_empty=True,
-                                        compact=bool(self.compact))
-        else:
-            b = ops.blocks_build(csr.rowptr32(), csr.indices, "edge_f32", np.asarray(values, np.float32),
-                                 self.tile_dtype, self.window, self.wide, self.part_size, emit_empty=True,
-                                 compact=self.compact)
+        b = ops.blocks_build(csr.rowptr32(), csr.indices, "edge_f32", np.asarray(values, np.float32),
+                             self.tile_dtype, self.window, self.wide, self.part_size, emit_empty=True,
+                             compact=self.compact)
+        return self._finish(b, device)
+
+    def _finish(self, b, device):
         keys = ("rp", "col", "val", "window", "atomic")
         if self.compact is True:   # compact only: the dense tiles are not kept
             b["val"] = torch.zeros(1, dtype=self.tile_dtype)
@@ -269,6 +285,17 @@ class GATGraph:
             self.t_rowptr = self.rowptr
             self.t_col = self.col
             self.t_perm = perm
+        elif build == "device":
+            # A^T on the device, its edge-id map re-expressed in A's edge order
+            t_rowptr, t_col, t_perm = ops.csr_transpose_device(torch.from_numpy(padded.rowptr32()).to(device),
+                                                               self.col, padded.n)
+            bt = ops.blocks_build_device(t_rowptr, t_col, "map", None, torch.int32, window, wide, self.part_size,
+                                         emit_empty=True)
+            self.bwd = [bt["rp"], bt["col"], ops.map_remap(bt["val"], t_perm), bt["window"], bt["atomic"]]
+            self.t_rowptr = t_rowptr[: csr.n + 1].contiguous()
+            self.t_col = t_col
+            self.t_perm = t_perm
+        if symmetric or build == "device":
             self.row_pointers, self.column_index, self.degrees, self.window_t, self.automic = self.fwd
             self.degrees_trans = self.bwd[2]
             return

@@ -228,6 +228,41 @@ def map_transpose_perm(emap, emap_t, nnz):
     return perm
 
 
+def csr_transpose_device(rowptr, col, n_cols):
+    """CSR of A^T on the device: (t_rowptr int32 [n_cols+1], t_col int32 [nnz]
+    (source rows, ascending per row of A^T), t_perm int64 [nnz] (A positions));
+    raises on a malformed CSR."""
+    dev = _need_gpu("csr_transpose_device", rowptr, col)
+    _need_i32("csr_transpose_device", rowptr, col)
+    n_rows, nnz = rowptr.numel() - 1, col.numel()
+    L = lib()
+    ws_n = L.attgnn_csr_transpose_workspace_bytes(n_rows, n_cols, nnz)
+    if ws_n < 0:
+        raise ValueError("sizes out of range for the device transpose")
+    ws = torch.empty(max(ws_n, 1), dtype=torch.uint8, device=dev)
+    t_rowptr = torch.empty(n_cols + 1, dtype=torch.int32, device=dev)
+    t_col = torch.empty(max(nnz, 1), dtype=torch.int32, device=dev)[:nnz]
+    t_perm = torch.empty(max(nnz, 1), dtype=torch.int64, device=dev)[:nnz]
+    status = torch.empty(1, dtype=torch.int64, device=dev)
+    check(L.attgnn_csr_transpose_device(_ptr(rowptr), _ptr(col), n_rows, n_cols, nnz, _ptr(t_rowptr), _ptr(t_col),
+                                        _ptr(t_perm), _ptr(status), _ptr(ws), ws_n, _stream(dev)),
+          "attgnn_csr_transpose_device")
+    check(int(status.item()), "attgnn_csr_transpose_device")
+    return t_rowptr, t_col, t_perm
+
+
+def map_remap(emap, remap):
+    """map entries re-expressed through `remap` (int64); -1 stays -1."""
+    dev = _need_gpu("map_remap", emap, remap)
+    _need_i32("map_remap", emap)
+    if remap.dtype != torch.int64:
+        raise TypeError("remap must be int64")
+    out = torch.empty_like(emap)
+    check(lib().attgnn_map_remap_device(_ptr(emap), emap.numel(), _ptr(remap), remap.numel(), _ptr(out),
+                                        _stream(dev)), "attgnn_map_remap_device")
+    return out
+
+
 # --------------------------------------------------------------------------
 # SpMM
 # --------------------------------------------------------------------------

@@ -18,6 +18,7 @@ scatter message passing for GAT) run the same architecture, loop and timing.
 import argparse
 import json
 import os
+import sys
 import tempfile
 import time
 
@@ -84,6 +85,12 @@ def test1(model, inputInfo):
         return correct.item() * 1.0 / max(len(labels), 1)
 
 
+def nll_loss(logp, y):
+    """Mean negative log-likelihood of the labels (F.nll_loss's value), as a
+    gather and a mean."""
+    return -logp.float().gather(1, y[:, None]).mean()
+
+
 def train(model, inputInfo, epoches):
     """`epoches` full-graph training steps; returns the last loss (a tensor)."""
     optimizer = torch.optim.Adam(model.parameters(), lr=1e-2, weight_decay=5e-4)
@@ -91,7 +98,7 @@ def train(model, inputInfo, epoches):
     for _ in range(epoches):
         model.train()
         logits = model(inputInfo)
-        loss = F.nll_loss(logits.float(), inputInfo.y)
+        loss = nll_loss(logits, inputInfo.y)
         optimizer.zero_grad()
         loss.backward()
         optimizer.step()
@@ -205,12 +212,14 @@ def main():
     ap.add_argument("--model", choices=["gcn", "gat"], default="gcn")
     ap.add_argument("--npz", help="graph in the reference's .npz format")
     ap.add_argument("--synthetic", default="169343,14",
-                    help="nodes,mean_degree of a Chung-Lu graph written to a temporary .npz (no --npz)")
+                    help="nodes,mean_degree[,nnz] of a Chung-Lu graph written to a temporary .npz (no --npz)")
     ap.add_argument("--layers", type=int, default=3)
     ap.add_argument("--hidden", type=int, default=64)
     ap.add_argument("--epochs", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--plain", action="store_true", help="also time the plain FP32 PyTorch model")
+    ap.add_argument("--plain-epochs", type=int, default=0,
+                    help="epochs of the plain model (default: --epochs); its time is reported per epoch too")
     ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args()
     dev = torch.device("cuda", 0)
@@ -218,15 +227,21 @@ def main():
     tmp = None
     path = args.npz
     if path is None:
-        n, deg = args.synthetic.split(",")
+        parts = args.synthetic.split(",")
+        n, deg = int(parts[0]), float(parts[1])
+        nnz = int(parts[2]) if len(parts) > 2 else None
         tmp = tempfile.TemporaryDirectory()
         path = os.path.join(tmp.name, "synthetic.npz")
-        D.synthetic_npz(path, int(n), float(deg), num_classes=16 if args.model == "gcn" else 10, seed=args.seed)
+        t = time.perf_counter()
+        D.synthetic_npz(path, n, deg, num_classes=16 if args.model == "gcn" else 10, seed=args.seed, target_nnz=nnz)
+        print(f"synthetic graph written ({time.perf_counter() - t:.1f}s)", file=sys.stderr, flush=True)
     t = time.perf_counter()
     ds = D.MGAT_dataset_csr(path, kind=args.model, device=dev).to(dev)
     prep = time.perf_counter() - t
+    print(f"dataset: {ds.num_nodes_ori} nodes, {ds.csr.nnz} entries ({prep:.1f}s)", file=sys.stderr, flush=True)
     model = build_model(args.model, ds, args.layers, args.hidden).to(dev)
     secs, loss = timed_train(ds, model, args.epochs, args.warmup)
+    print(f"trained {args.epochs} epochs in {secs:.2f}s", file=sys.stderr, flush=True)
     rec = {"model": args.model, "graph": os.path.basename(path), "nodes": ds.num_nodes_ori, "nnz": ds.csr.nnz,
            "layers": args.layers, "hidden": args.hidden, "epochs": args.epochs, "seconds": secs,
            "ms_per_epoch": round(secs / args.epochs * 1e3, 3), "final_loss": round(loss, 4),
@@ -237,8 +252,10 @@ def main():
         torch.manual_seed(args.seed)
         info = PlainInfo(ds)
         pm = build_model(args.model, ds, args.layers, args.hidden, plain=True).to(dev)
-        psecs, ploss = timed_train(info, pm, args.epochs, args.warmup)
-        rec.update(plain_fp32_seconds=psecs, plain_fp32_final_loss=round(ploss, 4), speedup=round(psecs / secs, 2))
+        pe = args.plain_epochs or args.epochs
+        psecs, ploss = timed_train(info, pm, pe, min(args.warmup, 3))
+        rec.update(plain_fp32_epochs=pe, plain_fp32_seconds=psecs, plain_fp32_ms_per_epoch=round(psecs / pe * 1e3, 3),
+                   plain_fp32_final_loss=round(ploss, 4), speedup=round(psecs / pe / (secs / args.epochs), 2))
     print(json.dumps(rec), flush=True)
     if tmp is not None:
         tmp.cleanup()

@@ -177,6 +177,29 @@ int attgnn_map_transpose_perm_device(const int32_t *map, const int32_t *map_t, i
                                      int64_t *perm, attgnn_stream_t stream);
 
 /*
+ * CSR of A^T on the device: t_rowptr[n_cols+1], t_col[nnz] (the row of A of
+ * each entry, ascending within a row of A^T) and t_perm[nnz] (the CSR position
+ * in A of each entry), equal to a stable host sort of A's entries by (column,
+ * row).  status (device int64[1]) receives ATTGNN_OK or ATTGNN_E_FORMAT (row
+ * pointers not a partition of [0, nnz), or a column outside [0, n_cols)).
+ * workspace: attgnn_csr_transpose_workspace_bytes() bytes.
+ *
+ * No counterpart in the reference: its layers reuse A for A^T
+ * (end2end_ori/gcn_final/mgat_csr/mgat_conv.py:57-74), exact only for a
+ * symmetric A.
+ */
+int64_t attgnn_csr_transpose_workspace_bytes(int64_t n_rows, int64_t n_cols, int64_t nnz);
+int attgnn_csr_transpose_device(const int32_t *rowptr, const int32_t *col, int64_t n_rows, int64_t n_cols,
+                                int64_t nnz, int32_t *t_rowptr, int32_t *t_col, int64_t *t_perm, int64_t *status,
+                                void *workspace, int64_t workspace_bytes, attgnn_stream_t stream);
+
+/* out[s] = map[s] < 0 ? -1 : (int32) remap[map[s]]: edge ids of a transposed
+ * structure's map re-expressed in A's edge order (remap = t_perm). */
+int attgnn_map_remap_device(const int32_t *map, int64_t n, const int64_t *remap, int64_t remap_n, int32_t *out,
+                            attgnn_stream_t stream);
+
+/*
+ * Compact tiles on the device, bit-identical/*
  * Compact tiles on the device, bit-identical to attgnn_blocks_compact_export:
  * count writes masks[n_cols] and block_base[n_cols/32 + 1] (block_base[n_cols/32]
  * = n_values, which the caller reads to size values / positions); fill packs

@@ -198,17 +198,48 @@ def test_gcn_graph_device_build_equals_host(symmetric, cuda):
     _tensors_equal(host.spmm(X, transpose=True), dev.spmm(X, transpose=True), "spmm^T")
 
 
-def test_gat_graph_device_build_equals_host(cuda):
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_gat_graph_device_build_equals_host(symmetric, cuda):
     from att_amd import graphs as G
     from att_amd.layers import GATGraph
-    g = G.directed_chung_lu_symmetrized(5000, 60_000, seed=2)
+    if symmetric:
+        g = G.directed_chung_lu_symmetrized(5000, 60_000, seed=2)
+    else:
+        rng = np.random.default_rng(6)
+        rp, col = rand_csr(rng, 3000, 20, hub_rows=3)
+        g = G.CSR(rp.astype(np.int64), col, rp.size - 1)
     host = GATGraph(g, cuda, build="host")
     dev = GATGraph(g, cuda, build="device")
     for which in ("fwd", "edges", "bwd"):
         for i, (a, b) in enumerate(zip(getattr(host, which), getattr(dev, which))):
             _tensors_equal(a, b, (which, i))
-    _tensors_equal(host.t_rowptr, dev.t_rowptr, "t_rowptr")
-    _tensors_equal(host.t_perm, dev.t_perm, "t_perm")
+    for name in ("t_rowptr", "t_col", "t_perm"):
+        _tensors_equal(getattr(host, name), getattr(dev, name), name)
+
+
+def test_csr_transpose_device_matches_stable_host_sort(cuda):
+    """Rectangular, unsorted rows, repeated entries, empty rows and columns:
+    the device transpose equals a stable host sort by (column, row)."""
+    rng = np.random.default_rng(8)
+    for n_rows, n_cols in [(700, 700), (300, 1000), (1000, 50), (5, 1)]:
+        rows = [rng.integers(0, n_cols, int(rng.integers(0, 30))) for _ in range(n_rows)]
+        rows[0] = rng.integers(0, n_cols, 3000)                 # a long unsorted row with repeats
+        if n_rows > 3:
+            rows[3] = np.zeros(0, np.int64)
+        rp = np.concatenate([[0], np.cumsum([r.size for r in rows])]).astype(np.int32)
+        col = np.concatenate(rows).astype(np.int32)
+        t_rowptr, t_col, t_perm = ops.csr_transpose_device(torch.from_numpy(rp).to(cuda), torch.from_numpy(col).to(cuda),
+                                                           n_cols)
+        src = np.repeat(np.arange(n_rows), np.diff(rp))
+        order = np.lexsort((src, col))                           # stable: ties keep CSR order
+        want_rp = np.concatenate([[0], np.cumsum(np.bincount(col, minlength=n_cols))])
+        assert np.array_equal(t_rowptr.cpu().numpy(), want_rp)
+        assert np.array_equal(t_perm.cpu().numpy(), order)
+        assert np.array_equal(t_col.cpu().numpy(), src[order])
+    for rp, col in ([0, 2, 1], [0, 1]), ([0, 1, 2], [0, 9]), ([0, 1, 3], [0, 1]):
+        with pytest.raises(AttGnnError):
+            ops.csr_transpose_device(torch.tensor(rp, dtype=torch.int32, device=cuda),
+                                     torch.tensor(col, dtype=torch.int32, device=cuda), 2)
 
 
 def test_serving_prepare_device_equals_host(cuda):

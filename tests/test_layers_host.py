@@ -50,3 +50,11 @@ def test_csr_segments_partition_rows():
             assert all((ss[k] == -1) == (len(segs) == 1) for k in segs)
     slots = ss[ss >= 0]
     assert np.array_equal(slots, np.arange(s["n_slots"])) and s["n_slots"] == 4 + 2 + 2
+
+
+def test_nll_loss_equals_torch():
+    from att_amd.train import nll_loss
+    gen = torch.Generator().manual_seed(5)
+    logp = torch.log_softmax(torch.randn(1000, 7, generator=gen, dtype=torch.float64), 1)
+    y = torch.randint(0, 7, (1000,), generator=gen)
+    assert torch.allclose(nll_loss(logp, y).double(), F.nll_loss(logp, y), rtol=1e-6)
