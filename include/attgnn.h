@@ -199,7 +199,6 @@ int attgnn_map_remap_device(const int32_t *map, int64_t n, const int64_t *remap,
                             attgnn_stream_t stream);
 
 /*
- * Compact tiles on the device, bit-identical/*
  * Compact tiles on the device, bit-identical to attgnn_blocks_compact_export:
  * count writes masks[n_cols] and block_base[n_cols/32 + 1] (block_base[n_cols/32]
  * = n_values, which the caller reads to size values / positions); fill packs
