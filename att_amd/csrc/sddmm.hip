@@ -383,12 +383,12 @@ __global__ __launch_bounds__(256) void sddmm_edges_batched_kernel(SddmmArgs a, c
     }
 }
 
-// s[h*n_rows + i] = sum_d X[h, i, d] * vec[h*dim + d]   (fp32 accumulate)
-// 16 lanes per (h, i).
+// s[h*s_hs + i*s_rs] = sum_d X[h, i, d] * vec[h*dim + d]   (fp32 accumulate)
+// Generic form: 16 lanes per (h, i), one element per lane per step.
 __global__ __launch_bounds__(256) void node_score_kernel(const uint8_t *x, int x_type, int64_t x_hs,
                                                          int64_t x_rs, const uint8_t *vec, int v_type,
                                                          int64_t heads, int64_t n_rows, int64_t dim,
-                                                         float *s)
+                                                         float *s, int64_t s_hs, int64_t s_rs)
 {
     const int64_t idx = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 4;
     const int l = threadIdx.x & 15;
@@ -405,7 +405,74 @@ __global__ __launch_bounds__(256) void node_score_kernel(const uint8_t *x, int x
     }
 #pragma unroll
     for (int off = 8; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 16);
-    if (ok && l == 0) s[h * n_rows + i] = acc;
+    if (ok && l == 0) s[h * s_hs + i * s_rs] = acc;
+}
+
+// Vector form: 16 lanes per node i with all H heads.  The node's H x dim
+// values are cut into 16-byte chunks (CE elements); lane l takes chunks
+// l, l+16, ...  Each chunk's dot product is summed in element order and added
+// to its head's partial, then the 16 partials of each head are reduced.
+template <int EB, int H>
+__global__ __launch_bounds__(256) void node_score_vec_kernel(const uint8_t *x, int x_type, int64_t x_hs,
+                                                             int64_t x_rs, const uint8_t *vec, int v_type,
+                                                             int64_t n_rows, int64_t dim, float *s,
+                                                             int64_t s_hs, int64_t s_rs)
+{
+    constexpr int CE = 16 / EB;
+    const int64_t i = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 4;
+    const int l = threadIdx.x & 15;
+    if (i >= n_rows) return;          // whole 16-lane group exits together
+    const int64_t cph = dim / CE, chunks = cph * H;
+    const bool same = v_type == x_type;
+    float acc[H];
+#pragma unroll
+    for (int h = 0; h < H; ++h) acc[h] = 0.f;
+    for (int64_t c = l; c < chunks; c += 16) {
+        const int h = int(c / cph);
+        const int64_t d0 = (c - int64_t(h) * cph) * CE;
+        const u32x4 xv = *reinterpret_cast<const u32x4 *>(x + (int64_t(h) * x_hs + i * x_rs + d0) * EB);
+        float xf[CE], vf[CE];
+        if constexpr (EB == 4) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) xf[k] = __uint_as_float(xv[k]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t b = (xv[k >> 1] >> (16 * (k & 1))) & 0xffffu;
+                xf[k] = x_type == ATTGNN_F16 ? float(__builtin_bit_cast(_Float16, uint16_t(b))) : __uint_as_float(b << 16);
+            }
+        }
+        const uint8_t *vp = vec + (int64_t(h) * dim + d0) * type_bytes(v_type);
+        if (same) {
+            const u32x4 vv = *reinterpret_cast<const u32x4 *>(vp);
+            if constexpr (EB == 4) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) vf[k] = __uint_as_float(vv[k]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const uint32_t b = (vv[k >> 1] >> (16 * (k & 1))) & 0xffffu;
+                    vf[k] = v_type == ATTGNN_F16 ? float(__builtin_bit_cast(_Float16, uint16_t(b))) : __uint_as_float(b << 16);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < CE; ++k) vf[k] = load_elt(vp + k * type_bytes(v_type), v_type);
+        }
+        float p = 0.f;
+#pragma unroll
+        for (int k = 0; k < CE; ++k) p += xf[k] * vf[k];
+#pragma unroll
+        for (int hh = 0; hh < H; ++hh)
+            if (hh == h) acc[hh] += p;
+    }
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+        float v = acc[h];
+#pragma unroll
+        for (int off = 8; off >= 1; off >>= 1) v += __shfl_xor(v, off, 16);
+        if (l == 0) s[h * s_hs + i * s_rs] = v;
+    }
 }
 
 struct EdgeArgs {
@@ -704,17 +771,35 @@ extern "C" int attgnn_sddmm_block(const int32_t *rp, const int32_t *col, const i
 
 extern "C" int attgnn_node_scores(const void *X, int x_type, int64_t x_head_stride, int64_t x_row_stride,
                                   const void *vec, int vec_type, int64_t heads, int64_t n_rows, int64_t dim,
-                                  float *scores, attgnn_stream_t stream)
+                                  float *scores, int64_t s_head_stride, int64_t s_row_stride,
+                                  attgnn_stream_t stream)
 {
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     if (heads <= 0 || n_rows < 0 || dim <= 0 || !is_float_type(x_type) || !is_float_type(vec_type))
         return ATTGNN_E_ARG;
     if (n_rows == 0) return ATTGNN_OK;
     if (!X || !vec || !scores) return ATTGNN_E_ARG;
-    const int64_t threads = heads * n_rows * 16;
-    hipLaunchKernelGGL(node_score_kernel, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, s,
-                       static_cast<const uint8_t *>(X), x_type, x_head_stride, x_row_stride,
-                       static_cast<const uint8_t *>(vec), vec_type, heads, n_rows, dim, scores);
+    const uint8_t *xb = static_cast<const uint8_t *>(X), *vb = static_cast<const uint8_t *>(vec);
+    const int eb = type_bytes(x_type), ce = 16 / eb;
+    const bool aligned16 = (reinterpret_cast<uintptr_t>(X) & 15) == 0 &&
+                           (vec_type != x_type || (reinterpret_cast<uintptr_t>(vec) & 15) == 0);
+    const bool vec_ok = aligned16 && dim % ce == 0 && x_head_stride % ce == 0 && x_row_stride % ce == 0 &&
+                        (heads == 1 || heads == 2 || heads == 4 || heads == 8);
+    if (vec_ok) {
+        const dim3 grid(unsigned((n_rows * 16 + 255) / 256)), block(256);
+#define ATTGNN_NS_CASE(EBV, HV)                                                                                \
+        if (eb == EBV && heads == HV)                                                                          \
+            hipLaunchKernelGGL((node_score_vec_kernel<EBV, HV>), grid, block, 0, s, xb, x_type, x_head_stride, \
+                               x_row_stride, vb, vec_type, n_rows, dim, scores, s_head_stride, s_row_stride);
+        ATTGNN_NS_CASE(2, 1) ATTGNN_NS_CASE(2, 2) ATTGNN_NS_CASE(2, 4) ATTGNN_NS_CASE(2, 8)
+        ATTGNN_NS_CASE(4, 1) ATTGNN_NS_CASE(4, 2) ATTGNN_NS_CASE(4, 4) ATTGNN_NS_CASE(4, 8)
+#undef ATTGNN_NS_CASE
+    } else {
+        const int64_t threads = heads * n_rows * 16;
+        hipLaunchKernelGGL(node_score_kernel, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, s, xb, x_type,
+                           x_head_stride, x_row_stride, vb, vec_type, heads, n_rows, dim, scores, s_head_stride,
+                           s_row_stride);
+    }
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? ATTGNN_OK : int(e);
 }

@@ -346,9 +346,10 @@ class GATAttention(torch.autograd.Function):
     @staticmethod
     def forward(ctx, Xs, Xt, a0, a1, graph, slope, drop_p, seed):
         g = graph
-        s0 = ops.node_scores(Xs, a0.to(Xs.dtype).contiguous(), "mhd")
-        s1 = ops.node_scores(Xt, a1.to(Xt.dtype).contiguous(), "mhd")
-        p, lse = ops.edge_softmax_fwd(g.rowptr, g.col, s0, s1, slope, drop_p, seed, p_dtype=Xt.dtype)
+        # node-major [N, H] scores: one load per neighbour for all heads
+        s0 = ops.node_scores(Xs, a0.to(Xs.dtype).contiguous(), "mhd", out_layout="nh")
+        s1 = ops.node_scores(Xt, a1.to(Xt.dtype).contiguous(), "mhd", out_layout="nh")
+        p, lse = ops.edge_softmax_fwd(g.rowptr, g.col, s0, s1, slope, drop_p, seed, p_dtype=Xt.dtype, s_layout="nh")
         rp, col, emap, win, at = g.fwd
         h = ops.spmm_edge_map(rp, col, emap, p, win, at, Xt, "mhd", g.window, g.wide, m_ori=g.num_nodes_ori,
                               x_layout="mhd", out_dtype=Xt.dtype, rows_covered=True)
@@ -369,8 +370,9 @@ class GATAttention(torch.autograd.Function):
                                 x_layout="mhd", rows_covered=True)
         # dp[(i,j)] = <dh[i], Xt[j]>  (SDDMM over A)
         dp = g.sddmm(dh, Xt)
-        dz, ds0 = ops.edge_softmax_bwd(g.rowptr, g.col, s0, s1, lse, dp, ctx.slope, ctx.drop_p, ctx.seed)
-        ds1 = ops.gather_rowsum(g.t_rowptr, g.t_perm, dz, n)
+        # row and column sums of dL/dz (columns through the transpose permutation)
+        ds0, ds1 = ops.edge_softmax_bwd_sums(g.rowptr, g.col, s0, s1, lse, dp, ctx.slope, ctx.drop_p, ctx.seed,
+                                             g.t_rowptr, g.t_perm, s_layout="nh")
         # s0 = Xs.a0, s1 = Xt.a1: node-term gradients fused with the aggregation
         # gradient (one pass over X; GAT's shared X W gets the summed gradient)
         shared = ctx.shared and Xs.data_ptr() == Xt.data_ptr()

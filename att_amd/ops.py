@@ -437,15 +437,23 @@ def sddmm_edges(rp, col, block_base, edge_ids, positions, window_of_part, A, B, 
     return out
 
 
-def node_scores(X, vec, layout="hmd"):
-    """s[h, i] = X_h[i] . vec_h  (fp32 [H, N])."""
+def node_scores(X, vec, layout="hmd", out_layout="hn"):
+    """s[h, i] = X_h[i] . vec_h in fp32: [H, N] ("hn") or node-major [N, H]
+    ("nh", the layout the edge softmax reads with one load per neighbour)."""
     dev = _need_gpu("node_scores", X, vec)
     h, n, d, hs, rs = _hmd_strides(X, layout)
     if vec.numel() != h * d:
         raise ValueError(f"vector has {vec.numel()} elements, expected {h}*{d}")
-    s = torch.empty((h, n), dtype=torch.float32, device=dev)
+    if out_layout == "hn":
+        s = torch.empty((h, n), dtype=torch.float32, device=dev)
+        shs, srs = n, 1
+    elif out_layout == "nh":
+        s = torch.empty((n, h), dtype=torch.float32, device=dev)
+        shs, srs = 1, h
+    else:
+        raise ValueError(f"out_layout must be 'hn' or 'nh', got {out_layout!r}")
     rc = lib().attgnn_node_scores(_ptr(X), _TYPE_OF[X.dtype], hs, rs, _ptr(vec), _TYPE_OF[vec.dtype], h, n, d,
-                                  _ptr(s), _stream(dev))
+                                  _ptr(s), shs, srs, _stream(dev))
     check(rc, "attgnn_node_scores")
     return s
 
@@ -513,34 +521,75 @@ def dtype_code(dtype):
 # fused edge softmax (CSR rows)
 # --------------------------------------------------------------------------
 
-def edge_softmax_fwd(rowptr, col, s0, s1, slope, drop_p=0.0, seed=0, p_dtype=torch.float32):
-    """p [H, nnz] and lse [H, N] of the fused LeakyReLU + softmax + dropout."""
+def _score_layout(s0, s1, s_layout):
+    """(heads, n, head_stride, row_stride) of [H, N] ("hn") or [N, H] ("nh") scores."""
+    if s0.shape != s1.shape or s0.dim() != 2 or s0.dtype != torch.float32 or s1.dtype != torch.float32:
+        raise ValueError("s0 and s1 must be fp32 tensors of one 2-D shape")
+    if not (s0.is_contiguous() and s1.is_contiguous()):
+        raise ValueError("s0 and s1 must be contiguous")
+    if s_layout == "hn":
+        heads, n = s0.shape
+        return heads, n, n, 1
+    if s_layout == "nh":
+        n, heads = s0.shape
+        return heads, n, 1, heads
+    raise ValueError(f"s_layout must be 'hn' or 'nh', got {s_layout!r}")
+
+
+def edge_softmax_fwd(rowptr, col, s0, s1, slope, drop_p=0.0, seed=0, p_dtype=torch.float32, s_layout="hn"):
+    """p [H, nnz] and lse [H, N] of the fused LeakyReLU + softmax + dropout;
+    s0 / s1 [H, N] ("hn") or [N, H] ("nh")."""
     dev = _need_gpu("edge_softmax_fwd", rowptr, col, s0, s1)
     _need_i32("edge_softmax_fwd", rowptr, col)
-    heads, n = s0.shape
+    heads, n, shs, srs = _score_layout(s0, s1, s_layout)
     nnz = col.numel()
     p = torch.empty((heads, nnz), dtype=p_dtype, device=dev)
     lse = torch.empty((heads, n), dtype=torch.float32, device=dev)
-    rc = lib().attgnn_edge_softmax_fwd(_ptr(rowptr), _ptr(col), n, nnz, heads, _ptr(s0), _ptr(s1), n, 1,
+    rc = lib().attgnn_edge_softmax_fwd(_ptr(rowptr), _ptr(col), n, nnz, heads, _ptr(s0), _ptr(s1), shs, srs,
                                        float(slope), float(drop_p), int(seed) & (2**64 - 1), _ptr(p),
                                        _TYPE_OF[p_dtype], _ptr(lse), _stream(dev))
     check(rc, "attgnn_edge_softmax_fwd")
     return p, lse
 
 
-def edge_softmax_bwd(rowptr, col, s0, s1, lse, dp, slope, drop_p=0.0, seed=0):
+def edge_softmax_bwd(rowptr, col, s0, s1, lse, dp, slope, drop_p=0.0, seed=0, s_layout="hn"):
     """dz [H, nnz] (fp32) and ds0 [H, N] (row sums of dz)."""
     dev = _need_gpu("edge_softmax_bwd", rowptr, col, s0, s1, lse, dp)
     _need_i32("edge_softmax_bwd", rowptr, col)
-    heads, n = s0.shape
+    heads, n, shs, srs = _score_layout(s0, s1, s_layout)
     nnz = col.numel()
     dz = torch.empty((heads, nnz), dtype=torch.float32, device=dev)
     ds0 = torch.empty((heads, n), dtype=torch.float32, device=dev)
-    rc = lib().attgnn_edge_softmax_bwd(_ptr(rowptr), _ptr(col), n, nnz, heads, _ptr(s0), _ptr(s1), n, 1,
+    rc = lib().attgnn_edge_softmax_bwd(_ptr(rowptr), _ptr(col), n, nnz, heads, _ptr(s0), _ptr(s1), shs, srs,
                                        _ptr(lse), _ptr(dp), _TYPE_OF[dp.dtype], float(slope), float(drop_p),
                                        int(seed) & (2**64 - 1), _ptr(dz), _ptr(ds0), _stream(dev))
     check(rc, "attgnn_edge_softmax_bwd")
     return dz, ds0
+
+
+def edge_softmax_bwd_sums(rowptr, col, s0, s1, lse, dp, slope, drop_p, seed, t_rowptr, t_perm, s_layout="hn"):
+    """(ds0, ds1) fp32 [H, N]: the row and column sums of the softmax backward's
+    dz, without returning dz (bitwise equal to edge_softmax_bwd followed by
+    gather_rowsum over the transpose)."""
+    dev = _need_gpu("edge_softmax_bwd_sums", rowptr, col, s0, s1, lse, dp, t_rowptr, t_perm)
+    _need_i32("edge_softmax_bwd_sums", rowptr, col, t_rowptr)
+    if t_perm.dtype != torch.int64:
+        raise TypeError("t_perm must be int64")
+    heads, n, shs, srs = _score_layout(s0, s1, s_layout)
+    nnz = col.numel()
+    if rowptr.numel() != n + 1 or t_rowptr.numel() != n + 1 or t_perm.numel() != nnz:
+        raise ValueError("rowptr / t_rowptr / t_perm sizes disagree with the scores and col")
+    ds0 = torch.empty((heads, n), dtype=torch.float32, device=dev)
+    ds1 = torch.empty((heads, n), dtype=torch.float32, device=dev)
+    L = lib()
+    ws_n = L.attgnn_edge_softmax_bwd_sums_workspace_elems(nnz, heads)
+    ws = torch.empty(max(ws_n, 1), dtype=torch.float32, device=dev)
+    check(L.attgnn_edge_softmax_bwd_sums(_ptr(rowptr), _ptr(col), n, nnz, heads, _ptr(s0), _ptr(s1), shs, srs,
+                                         _ptr(lse), _ptr(dp), _TYPE_OF[dp.dtype], float(slope), float(drop_p),
+                                         int(seed) & (2**64 - 1), _ptr(t_rowptr), _ptr(t_perm), _ptr(ds0),
+                                         _ptr(ds1), _ptr(ws), ws_n, _stream(dev)),
+          "attgnn_edge_softmax_bwd_sums")
+    return ds0, ds1
 
 
 class DeviceSegments:

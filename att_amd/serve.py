@@ -339,9 +339,9 @@ class Bucket:
             return ops.spmm_block(self.rp, self.col, self.val, self.win, self.atomic, h.contiguous(), self.rows_cap,
                                   WINDOW, WIDE, out_dtype=torch.float32, rows_covered=True, workspace=self.ws)
         xp = (self.X @ m.Wg).view(self.rows_cap, GAT_HEADS, GAT_DIM)
-        s0 = ops.node_scores(xp, m.a0, "mhd")
-        s1 = ops.node_scores(xp, m.a1, "mhd")
-        p, _ = ops.edge_softmax_fwd(self.rowptr, self.ccol, s0, s1, 0.2, 0.0, 0, p_dtype=torch.bfloat16)
+        s0 = ops.node_scores(xp, m.a0, "mhd", out_layout="nh")
+        s1 = ops.node_scores(xp, m.a1, "mhd", out_layout="nh")
+        p, _ = ops.edge_softmax_fwd(self.rowptr, self.ccol, s0, s1, 0.2, 0.0, 0, p_dtype=torch.bfloat16, s_layout="nh")
         return ops.spmm_edge_map(self.rp, self.col, self.map, p, self.win, self.atomic, xp, "mhd", WINDOW, WIDE,
                                  m_ori=self.rows_cap, x_layout="mhd")
 
@@ -515,9 +515,9 @@ class Server:
             else:
                 s = cg.gat
                 xp = (cg.X @ m.Wg).view(cg.n_pad, GAT_HEADS, GAT_DIM)
-                s0 = ops.node_scores(xp, m.a0, "mhd")
-                s1 = ops.node_scores(xp, m.a1, "mhd")
-                p, _ = ops.edge_softmax_fwd(s["rowptr"], s["ccol"], s0, s1, 0.2, 0.0, 0, p_dtype=torch.bfloat16)
+                s0 = ops.node_scores(xp, m.a0, "mhd", out_layout="nh")
+                s1 = ops.node_scores(xp, m.a1, "mhd", out_layout="nh")
+                p, _ = ops.edge_softmax_fwd(s["rowptr"], s["ccol"], s0, s1, 0.2, 0.0, 0, p_dtype=torch.bfloat16, s_layout="nh")
                 out = ops.spmm_edge_map(s["rp"], s["col"], s["map"], p, s["window"], s["atomic"], xp, "mhd", WINDOW,
                                         WIDE, m_ori=cg.n_pad, x_layout="mhd")
         self.stream.synchronize()

@@ -99,41 +99,63 @@ def _(rp, col, block_base, edge_ids, positions, window_of_part, A, B, nnz, windo
 
 
 @torch.library.custom_op(f"{_LIB}::node_scores", mutates_args=())
-def node_scores(X: torch.Tensor, vec: torch.Tensor, layout: str) -> torch.Tensor:
-    """s[h, i] = X_h[i] . vec_h (attgnn_node_scores)."""
-    return ops.node_scores(X, vec, layout)
+def node_scores(X: torch.Tensor, vec: torch.Tensor, layout: str, out_layout: str = "hn") -> torch.Tensor:
+    """s[h, i] = X_h[i] . vec_h (attgnn_node_scores): [H, N] ("hn") or [N, H] ("nh")."""
+    return ops.node_scores(X, vec, layout, out_layout=out_layout)
 
 
 @node_scores.register_fake
-def _(X, vec, layout):
+def _(X, vec, layout, out_layout="hn"):
     heads, n, _ = _hmd(X, layout)
-    return X.new_empty((heads, n), dtype=torch.float32)
+    return X.new_empty((heads, n) if out_layout == "hn" else (n, heads), dtype=torch.float32)
+
+
+def _sdims(s0, s_layout):
+    return tuple(s0.shape) if s_layout == "hn" else tuple(s0.shape)[::-1]
 
 
 @torch.library.custom_op(f"{_LIB}::edge_softmax_fwd", mutates_args=())
 def edge_softmax_fwd(rowptr: torch.Tensor, col: torch.Tensor, s0: torch.Tensor, s1: torch.Tensor, slope: float,
-                     drop_p: float, seed: int, p_dtype: torch.dtype) -> tuple[torch.Tensor, torch.Tensor]:
+                     drop_p: float, seed: int, p_dtype: torch.dtype,
+                     s_layout: str = "hn") -> tuple[torch.Tensor, torch.Tensor]:
     """Fused LeakyReLU + row softmax + dropout (attgnn_edge_softmax_fwd): (p, lse)."""
-    return ops.edge_softmax_fwd(rowptr, col, s0, s1, slope, drop_p, seed, p_dtype=p_dtype)
+    return ops.edge_softmax_fwd(rowptr, col, s0, s1, slope, drop_p, seed, p_dtype=p_dtype, s_layout=s_layout)
 
 
 @edge_softmax_fwd.register_fake
-def _(rowptr, col, s0, s1, slope, drop_p, seed, p_dtype):
-    heads, n = s0.shape
+def _(rowptr, col, s0, s1, slope, drop_p, seed, p_dtype, s_layout="hn"):
+    heads, n = _sdims(s0, s_layout)
     return s0.new_empty((heads, col.shape[0]), dtype=p_dtype), s0.new_empty((heads, n))
 
 
 @torch.library.custom_op(f"{_LIB}::edge_softmax_bwd", mutates_args=())
 def edge_softmax_bwd(rowptr: torch.Tensor, col: torch.Tensor, s0: torch.Tensor, s1: torch.Tensor, lse: torch.Tensor,
-                     dp: torch.Tensor, slope: float, drop_p: float, seed: int) -> tuple[torch.Tensor, torch.Tensor]:
+                     dp: torch.Tensor, slope: float, drop_p: float, seed: int,
+                     s_layout: str = "hn") -> tuple[torch.Tensor, torch.Tensor]:
     """Backward of edge_softmax_fwd (attgnn_edge_softmax_bwd): (dz, ds0)."""
-    return ops.edge_softmax_bwd(rowptr, col, s0, s1, lse, dp, slope, drop_p, seed)
+    return ops.edge_softmax_bwd(rowptr, col, s0, s1, lse, dp, slope, drop_p, seed, s_layout=s_layout)
 
 
 @edge_softmax_bwd.register_fake
-def _(rowptr, col, s0, s1, lse, dp, slope, drop_p, seed):
-    heads, n = s0.shape
+def _(rowptr, col, s0, s1, lse, dp, slope, drop_p, seed, s_layout="hn"):
+    heads, n = _sdims(s0, s_layout)
     return s0.new_empty((heads, col.shape[0])), s0.new_empty((heads, n))
+
+
+@torch.library.custom_op(f"{_LIB}::edge_softmax_bwd_sums", mutates_args=())
+def edge_softmax_bwd_sums(rowptr: torch.Tensor, col: torch.Tensor, s0: torch.Tensor, s1: torch.Tensor,
+                          lse: torch.Tensor, dp: torch.Tensor, slope: float, drop_p: float, seed: int,
+                          t_rowptr: torch.Tensor, t_perm: torch.Tensor,
+                          s_layout: str = "hn") -> tuple[torch.Tensor, torch.Tensor]:
+    """Row and column sums of the softmax backward's dz (attgnn_edge_softmax_bwd_sums): (ds0, ds1)."""
+    return ops.edge_softmax_bwd_sums(rowptr, col, s0, s1, lse, dp, slope, drop_p, seed, t_rowptr, t_perm,
+                                     s_layout=s_layout)
+
+
+@edge_softmax_bwd_sums.register_fake
+def _(rowptr, col, s0, s1, lse, dp, slope, drop_p, seed, t_rowptr, t_perm, s_layout="hn"):
+    heads, n = _sdims(s0, s_layout)
+    return s0.new_empty((heads, n)), s0.new_empty((heads, n))
 
 
 @torch.library.custom_op(f"{_LIB}::gather_rowsum", mutates_args=())
@@ -166,4 +188,4 @@ def _(Xs, Xt, a0, a1, ds0, ds1, base, combine):
 
 
 OPS = ("spmm_block", "spmm_compact", "spmm_edge_map", "sddmm_edges", "node_scores", "edge_softmax_fwd",
-       "edge_softmax_bwd", "gather_rowsum", "attention_node_bwd")
+       "edge_softmax_bwd", "edge_softmax_bwd_sums", "gather_rowsum", "attention_node_bwd")

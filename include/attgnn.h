@@ -359,8 +359,11 @@ int attgnn_sddmm_edges(const int32_t *rp, const int32_t *col, const int64_t *blo
                        int mode, attgnn_stream_t stream);
 
 /*
- * scores[h*n_rows + i] = sum_d X[h*x_head_stride + i*x_row_stride + d] * vec[h*dim + d]
+ * scores[h*s_head_stride + i*s_row_stride] =
+ *     sum_d X[h*x_head_stride + i*x_row_stride + d] * vec[h*dim + d]
  * (fp32 accumulate) -- the per-node half of the attention projections.
+ * (s_head_stride, s_row_stride) = (n_rows, 1) gives [H, N] scores, (1, heads)
+ * node-major [N, H] scores, the layout the edge softmax reads fastest.
  *
  * Part of ATT_SDDMM_v{1,3}.att_{fp16,tf32}_h_nnz / attv2_{fp16,tf32}_h_nnz
  *            (compute_utils.h:1144-1218; callers end2end_head/gat_final/mgat_csr/mgat_conv.py:21-30,
@@ -368,7 +371,7 @@ int attgnn_sddmm_edges(const int32_t *rp, const int32_t *col, const int64_t *blo
  */
 int attgnn_node_scores(const void *X, int x_type, int64_t x_head_stride, int64_t x_row_stride,
                        const void *vec, int vec_type, int64_t heads, int64_t n_rows, int64_t dim,
-                       float *scores, attgnn_stream_t stream);
+                       float *scores, int64_t s_head_stride, int64_t s_row_stride, attgnn_stream_t stream);
 
 /*
  * out[h*nnz + e] = S0(h, i) + S1(h, j) for every edge e = (i, j) of the map, with
@@ -503,6 +506,28 @@ int attgnn_edge_softmax_bwd(const int32_t *rowptr, const int32_t *col, int64_t n
                             int64_t s_row_stride, const float *lse, const void *dp, int dp_type,
                             float slope, float drop_p, uint64_t seed, float *dz, float *ds0,
                             attgnn_stream_t stream);
+
+/*
+ * Backward of attgnn_edge_softmax_fwd reduced to the two node gradients the
+ * attention layers need: ds0[h*n + i] = sum_j dz[h, (i, j)] (row sums) and
+ * ds1[h*n + j] = sum_i dz[h, (i, j)] (column sums, through the transpose:
+ * t_rowptr [n+1] holds the rows of A^T and t_perm[q] the CSR position of A^T
+ * entry q, as attgnn_csr_transpose_device writes them).  dz is kept in the
+ * workspace only (fp32, attgnn_edge_softmax_bwd_sums_workspace_elems
+ * elements), edge-major for 1, 2, 4 or 8 heads.  Results are bitwise equal to
+ * attgnn_edge_softmax_bwd followed by attgnn_gather_rowsum.
+ *
+ * Replaces the backward of the softmax chain together with the column-sum
+ * call ATT_SpMM_v{1,3}.spmm_*_h_m_rowsum over the transposed map
+ *   (end2end_head/gat_final/mgat_csr/mgat_conv.py:42-48, 91-151).
+ */
+int64_t attgnn_edge_softmax_bwd_sums_workspace_elems(int64_t nnz, int64_t heads);
+int attgnn_edge_softmax_bwd_sums(const int32_t *rowptr, const int32_t *col, int64_t n, int64_t nnz,
+                                 int64_t heads, const float *s0, const float *s1, int64_t s_head_stride,
+                                 int64_t s_row_stride, const float *lse, const void *dp, int dp_type,
+                                 float slope, float drop_p, uint64_t seed, const int32_t *t_rowptr,
+                                 const int64_t *t_perm, float *ds0, float *ds1, float *workspace,
+                                 int64_t workspace_elems, attgnn_stream_t stream);
 
 /*
  * out[h*n + j] = sum over CSR row j of vals[h*nnz + perm[e]] (perm[e] = -1 skips):

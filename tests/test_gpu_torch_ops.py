@@ -76,6 +76,12 @@ def test_attention_ops_match_wrappers(cuda, gat):
     _same(dz, dz2)
     _same(ds0, ds02)
     _same(A.gather_rowsum(graph.t_rowptr, graph.t_perm, dz, n), ops.gather_rowsum(graph.t_rowptr, graph.t_perm, dz, n))
+    s0n, s1n = A.node_scores(X, a0, "mhd", "nh"), A.node_scores(X, a1, "mhd", "nh")
+    _same(s0n.t(), s0)
+    f0, f1 = A.edge_softmax_bwd_sums(graph.rowptr, graph.col, s0n, s1n, lse, dp, 0.2, 0.25, 11, graph.t_rowptr,
+                                     graph.t_perm, "nh")
+    _same(f0, ds0)
+    _same(f1, ops.gather_rowsum(graph.t_rowptr, graph.t_perm, dz, n))
     ds1 = ops.gather_rowsum(graph.t_rowptr, graph.t_perm, dz, n)
     gbase = torch.randn(n, heads, dim, device=cuda)
     got = A.attention_node_bwd(X, X, a0, a1, ds0, ds1, gbase, True)

@@ -55,6 +55,14 @@ def test_attention_shapes(fake):
     vec = torch.empty(heads, dim, dtype=torch.float32, device="cuda")
     s = torch.ops.attgnn.node_scores(X, vec, "mhd")
     assert s.shape == (heads, n) and s.dtype == torch.float32
+    sn = torch.ops.attgnn.node_scores(X, vec, "mhd", "nh")
+    assert sn.shape == (n, heads)
+    p, lse = torch.ops.attgnn.edge_softmax_fwd(_i32(n + 1), _i32(nnz), sn, sn, 0.2, 0.0, 0, torch.float16, "nh")
+    assert p.shape == (heads, nnz) and lse.shape == (heads, n)
+    t_perm = torch.empty(nnz, dtype=torch.int64, device="cuda")
+    d0, d1 = torch.ops.attgnn.edge_softmax_bwd_sums(_i32(n + 1), _i32(nnz), sn, sn, lse, p, 0.2, 0.0, 0,
+                                                    _i32(n + 1), t_perm, "nh")
+    assert d0.shape == (heads, n) and d1.shape == (heads, n) and d1.dtype == torch.float32
     e = torch.ops.attgnn.sddmm_edges(_i32(5), _i32(32), torch.empty(3, dtype=torch.int64, device="cuda"),
                                      _i32(nnz), torch.empty(nnz, dtype=torch.int16, device="cuda"), _i32(2), X, X,
                                      nnz, 16, "mhd", torch.float32, True)
