@@ -54,15 +54,16 @@ _SIGNATURES = {
     "attgnn_node_scores": (_int, [_vp, _int, _i64, _i64, _vp, _int, _i64, _i64, _i64, _vp, _i64, _i64, _vp]),
     "attgnn_edge_from_nodes": (_int, [_vp, _vp, _vp, _vp, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _int,
                                       _i64, _i64, _vp, _int, _i64, _int, _int, _vp]),
+    "attgnn_long_rows": (_i64, [_vp, _i64, _vp]),
     "attgnn_edge_softmax_fwd": (_int, [_vp, _vp, _i64, _i64, _i64, _vp, _vp, _i64, _i64, _c.c_float, _c.c_float,
-                                       _c.c_uint64, _vp, _int, _vp, _vp]),
+                                       _c.c_uint64, _vp, _int, _vp, _vp, _i64, _vp]),
     "attgnn_edge_softmax_bwd": (_int, [_vp, _vp, _i64, _i64, _i64, _vp, _vp, _i64, _i64, _vp, _vp, _int,
-                                       _c.c_float, _c.c_float, _c.c_uint64, _vp, _vp, _vp]),
+                                       _c.c_float, _c.c_float, _c.c_uint64, _vp, _vp, _vp, _i64, _vp]),
     "attgnn_edge_softmax_bwd_sums_workspace_elems": (_i64, [_i64, _i64]),
     "attgnn_edge_softmax_bwd_sums": (_int, [_vp, _vp, _i64, _i64, _i64, _vp, _vp, _i64, _i64, _vp, _vp, _int,
-                                            _c.c_float, _c.c_float, _c.c_uint64, _vp, _vp, _vp, _vp, _vp, _i64,
-                                            _vp]),
-    "attgnn_gather_rowsum": (_int, [_vp, _vp, _i64, _i64, _i64, _vp, _vp, _vp]),
+                                            _c.c_float, _c.c_float, _c.c_uint64, _vp, _vp, _vp, _i64, _vp, _i64,
+                                            _vp, _vp, _vp, _i64, _vp]),
+    "attgnn_gather_rowsum": (_int, [_vp, _vp, _i64, _i64, _i64, _vp, _vp, _vp, _i64, _vp]),
     "attgnn_blocks_device_workspace_bytes": (_i64, [_i64, _i64, _int]),
     "attgnn_blocks_plan_device": (_int, [_vp, _i64, _vp, _i64, _int, _int, _int, _int, _vp, _i64, _vp, _vp]),
     "attgnn_blocks_fill_device": (_int, [_vp, _i64, _vp, _i64, _int, _vp, _int, _int, _int, _int, _vp, _i64, _vp,

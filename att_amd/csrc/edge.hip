@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
+#include <cstdint>
 
 #include "attgnn.h"
 #include "mfma.hpp"
@@ -76,7 +78,11 @@ struct SoftmaxArgs {
     float *dz;                 // backward output [heads][nnz]
     float *ds0;                // backward output [heads][n]
     int64_t n, nnz, s_hs, s_rs;
+    int64_t long_min;          // rows with more entries are left to the workgroup kernels
+    const int32_t *long_rows;  // those rows (n_long of them)
+    int64_t n_long;
     int heads, p_type, dp_type;
+    int dz_edge_major;         // backward: dz[e*heads + h] instead of dz[h*nnz + e]
     float slope, keep_scale;   // keep_scale = 1/(1-q)
     uint32_t thresh;           // q * 2^32
     uint64_t seed;
@@ -90,6 +96,7 @@ __global__ __launch_bounds__(256) void edge_softmax_fwd_kernel(SoftmaxArgs a)
     const int64_t i = gid / a.heads;
     const int h = int(gid - i * a.heads);
     const int64_t e0 = a.rowptr[i], e1 = a.rowptr[i + 1];
+    if (e1 - e0 > a.long_min) return;          // a workgroup of its own takes it
     const float si = a.s0[h * a.s_hs + i * a.s_rs];
     float m = -INFINITY, s = 0.f;
     for (int64_t e = e0 + l; e < e1; e += 16) {
@@ -119,6 +126,7 @@ __global__ __launch_bounds__(256) void edge_softmax_bwd_kernel(SoftmaxArgs a)
     const int64_t i = gid / a.heads;
     const int h = int(gid - i * a.heads);
     const int64_t e0 = a.rowptr[i], e1 = a.rowptr[i + 1];
+    if (e1 - e0 > a.long_min) return;
     const float si = a.s0[h * a.s_hs + i * a.s_rs];
     const float lse = a.lse[h * a.n + i];
     float acc = 0.f;  // sum_k p_k g_k, g = dL/dp before dropout
@@ -175,6 +183,165 @@ __device__ __forceinline__ void load_heads(const float *s, int64_t hs, int64_t r
 // entries is ~90 steps of a 16-lane group, each a dependent col -> score load
 // pair, so the loads of U steps are issued before their updates, which run in
 // the same edge order as before.
+// ---- rows with many entries: one 256-thread workgroup per (row, head) -------
+// A 16-lane group walks a row serially, so a hub row of ~1,500 entries (power-
+// law graphs) outlasts every other row by far; rows above long_min entries are
+// skipped by the group kernels and taken by workgroups: workgroup b of the
+// long part handles row long_rows[b / heads], head b % heads.  Each thread
+// folds a 256-strided slice of the row; the block reduction is a butterfly
+// inside each wave, then the four wave results in a fixed order, so the result
+// is deterministic.  The all-heads kernels run these workgroups first in the
+// same launch (so the hub rows overlap the short ones); the per-head kernels
+// launch them separately.  Either way the same function computes them.
+
+constexpr int LONG_THREADS = 256;
+
+__device__ __forceinline__ float wave_max(float v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ float wave_sum(float v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+// MAX: block maximum, else block sum; every thread gets the result.
+template <bool MAX>
+__device__ __forceinline__ float block_reduce(float v, float *red)
+{
+    v = MAX ? wave_max(v) : wave_sum(v);
+    __syncthreads();                      // red[] free from the previous reduction
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    float r = red[0];
+#pragma unroll
+    for (int w = 1; w < LONG_THREADS / 64; ++w) r = MAX ? fmaxf(r, red[w]) : r + red[w];
+    return r;
+}
+
+// (row, head) of long-part workgroup b; false when the row is not long (then
+// the group kernel has written it).  Uniform over the workgroup.
+__device__ __forceinline__ bool long_task(const SoftmaxArgs &a, int64_t b, int64_t &i, int &h, int64_t &e0,
+                                          int64_t &e1)
+{
+    i = a.long_rows[b / a.heads];
+    h = int(b % a.heads);
+    if (i < 0 || i >= a.n) return false;
+    e0 = a.rowptr[i];
+    e1 = a.rowptr[i + 1];
+    return e1 - e0 > a.long_min;
+}
+
+__device__ __forceinline__ void softmax_fwd_long(const SoftmaxArgs &a, int64_t b, float *red)
+{
+    int64_t i, e0 = 0, e1 = 0;
+    int h;
+    if (!long_task(a, b, i, h, e0, e1)) return;
+    const int t = threadIdx.x;
+    const float si = a.s0[h * a.s_hs + i * a.s_rs];
+    float m = -INFINITY, s = 0.f;
+    for (int64_t e = e0 + t; e < e1; e += LONG_THREADS) {
+        const float v = lrelu(si + a.s1[h * a.s_hs + int64_t(a.col[e]) * a.s_rs], a.slope);
+        const float mn = fmaxf(m, v);
+        s = s * __expf(m - mn) + __expf(v - mn);
+        m = mn;
+    }
+    const float M = block_reduce<true>(m, red);
+    const float S = block_reduce<false>((m == -INFINITY) ? 0.f : s * __expf(m - M), red);
+    const float lse = M + __logf(S);
+    if (t == 0) a.lse_out[h * a.n + i] = lse;
+    for (int64_t e = e0 + t; e < e1; e += LONG_THREADS) {
+        const float v = lrelu(si + a.s1[h * a.s_hs + int64_t(a.col[e]) * a.s_rs], a.slope);
+        float p = __expf(v - lse);
+        if (a.thresh) p = keep_edge(a.seed, h, e, a.thresh) ? p * a.keep_scale : 0.f;
+        st(a.p, a.p_type, int64_t(h) * a.nnz + e, p);
+    }
+}
+
+__device__ __forceinline__ void softmax_bwd_long(const SoftmaxArgs &a, int64_t b, float *red)
+{
+    int64_t i, e0 = 0, e1 = 0;
+    int h;
+    if (!long_task(a, b, i, h, e0, e1)) return;
+    const int t = threadIdx.x;
+    const float si = a.s0[h * a.s_hs + i * a.s_rs];
+    const float lse = a.lse[h * a.n + i];
+    float acc = 0.f;
+    for (int64_t e = e0 + t; e < e1; e += LONG_THREADS) {
+        const float p = __expf(lrelu(si + a.s1[h * a.s_hs + int64_t(a.col[e]) * a.s_rs], a.slope) - lse);
+        float g = ld(a.dp, a.dp_type, int64_t(h) * a.nnz + e);
+        if (a.thresh) g = keep_edge(a.seed, h, e, a.thresh) ? g * a.keep_scale : 0.f;
+        acc += p * g;
+    }
+    const float S = block_reduce<false>(acc, red);
+    float ds = 0.f;
+    for (int64_t e = e0 + t; e < e1; e += LONG_THREADS) {
+        const float z = si + a.s1[h * a.s_hs + int64_t(a.col[e]) * a.s_rs];
+        const float p = __expf(lrelu(z, a.slope) - lse);
+        float g = ld(a.dp, a.dp_type, int64_t(h) * a.nnz + e);
+        if (a.thresh) g = keep_edge(a.seed, h, e, a.thresh) ? g * a.keep_scale : 0.f;
+        const float dz = p * (g - S) * (z > 0.f ? 1.f : a.slope);
+        a.dz[a.dz_edge_major ? e * a.heads + h : int64_t(h) * a.nnz + e] = dz;
+        ds += dz;
+    }
+    ds = block_reduce<false>(ds, red);
+    if (t == 0) a.ds0[h * a.n + i] = ds;
+}
+
+__global__ __launch_bounds__(LONG_THREADS) void softmax_fwd_long_kernel(SoftmaxArgs a)
+{
+    __shared__ float red[LONG_THREADS / 64];
+    softmax_fwd_long(a, blockIdx.x, red);
+}
+
+__global__ __launch_bounds__(LONG_THREADS) void softmax_bwd_long_kernel(SoftmaxArgs a)
+{
+    __shared__ float red[LONG_THREADS / 64];
+    softmax_bwd_long(a, blockIdx.x, red);
+}
+
+// Column sums of the listed long rows: out[h*n + j] with vals(h, q) at
+// vals[q*heads + h] (edge_major) or vals[h*nnz + q].
+struct GatherArgs {
+    const int32_t *rowptr;
+    const int64_t *perm;
+    const float *vals;
+    float *out;
+    const int32_t *long_rows;
+    int64_t n, nnz, long_min, n_long;
+    int heads, edge_major;
+};
+
+__device__ __forceinline__ void gather_long(const GatherArgs &g, int64_t b, float *red)
+{
+    const int64_t j = g.long_rows[b / g.heads];
+    const int h = int(b % g.heads);
+    if (j < 0 || j >= g.n) return;
+    const int64_t e0 = g.rowptr[j], e1 = g.rowptr[j + 1];
+    if (e1 - e0 <= g.long_min) return;
+    float s = 0.f;
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += LONG_THREADS) {
+        const int64_t q = g.perm[e];
+        if (q >= 0 && q < g.nnz) s += g.vals[g.edge_major ? q * g.heads + h : int64_t(h) * g.nnz + q];
+    }
+    s = block_reduce<false>(s, red);
+    if (threadIdx.x == 0) g.out[h * g.n + j] = s;
+}
+
+__global__ __launch_bounds__(LONG_THREADS) void gather_rowsum_long_kernel(GatherArgs g)
+{
+    __shared__ float red[LONG_THREADS / 64];
+    gather_long(g, blockIdx.x, red);
+}
+
+// ---- all heads of a row in one 16-lane group (short rows) -------------------
+// Edges a lane handles per step (loads issued together): the loads of U steps
+// go out before their updates, which run in the same edge order as before.  A
+// row of at most 16*U entries takes one step, and its second pass reuses the
+// values loaded by the first instead of loading them again.
 template <int H>
 constexpr int unroll_of() { return H >= 8 ? 4 : 8; }
 
@@ -182,16 +349,24 @@ template <int H, bool NM>
 __global__ __launch_bounds__(256) void softmax_fwd_heads_kernel(SoftmaxArgs a)
 {
     constexpr int U = unroll_of<H>();
-    const int64_t i = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 4;
+    __shared__ float red[LONG_THREADS / 64];
+    const int64_t lb = a.n_long * a.heads;
+    if (int64_t(blockIdx.x) < lb) {            // long rows first, a workgroup per (row, head)
+        softmax_fwd_long(a, blockIdx.x, red);
+        return;
+    }
+    const int64_t i = ((int64_t(blockIdx.x) - lb) * blockDim.x + threadIdx.x) >> 4;
     const int l = threadIdx.x & 15;
     if (i >= a.n) return;                      // whole 16-lane group exits together
     const int64_t e0 = a.rowptr[i], e1 = a.rowptr[i + 1];
+    if (e1 - e0 > a.long_min) return;          // a workgroup per head takes it
+    const bool one_step = e1 - e0 <= 16 * U;
     float si[H], m[H], s[H];
     load_heads<H, NM>(a.s0, a.s_hs, a.s_rs, i, si);
 #pragma unroll
     for (int h = 0; h < H; ++h) { m[h] = -INFINITY; s[h] = 0.f; }
+    float sj[U][H];
     for (int64_t eb = e0 + l; eb < e1; eb += 16 * U) {
-        float sj[U][H];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int64_t e = eb + 16 * u;
@@ -218,11 +393,12 @@ __global__ __launch_bounds__(256) void softmax_fwd_heads_kernel(SoftmaxArgs a)
         if (l == 0) a.lse_out[h * a.n + i] = (e1 > e0) ? lse[h] : 0.f;
     }
     for (int64_t eb = e0 + l; eb < e1; eb += 16 * U) {
-        float sj[U][H];
+        if (!one_step) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int64_t e = eb + 16 * u;
-            load_heads<H, NM>(a.s1, a.s_hs, a.s_rs, e < e1 ? int64_t(a.col[e]) : 0, sj[u]);
+            for (int u = 0; u < U; ++u) {
+                const int64_t e = eb + 16 * u;
+                load_heads<H, NM>(a.s1, a.s_hs, a.s_rs, e < e1 ? int64_t(a.col[e]) : 0, sj[u]);
+            }
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -243,16 +419,24 @@ template <int H, bool NM, bool EDGE_MAJOR>
 __global__ __launch_bounds__(256) void softmax_bwd_heads_kernel(SoftmaxArgs a)
 {
     constexpr int U = unroll_of<H>();
-    const int64_t i = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 4;
+    __shared__ float red[LONG_THREADS / 64];
+    const int64_t lb = a.n_long * a.heads;
+    if (int64_t(blockIdx.x) < lb) {
+        softmax_bwd_long(a, blockIdx.x, red);
+        return;
+    }
+    const int64_t i = ((int64_t(blockIdx.x) - lb) * blockDim.x + threadIdx.x) >> 4;
     const int l = threadIdx.x & 15;
     if (i >= a.n) return;
     const int64_t e0 = a.rowptr[i], e1 = a.rowptr[i + 1];
+    if (e1 - e0 > a.long_min) return;
+    const bool one_step = e1 - e0 <= 16 * U;
     float si[H], lse[H], acc[H];
     load_heads<H, NM>(a.s0, a.s_hs, a.s_rs, i, si);
 #pragma unroll
     for (int h = 0; h < H; ++h) { lse[h] = a.lse[h * a.n + i]; acc[h] = 0.f; }
+    float sj[U][H], g[U][H];
     for (int64_t eb = e0 + l; eb < e1; eb += 16 * U) {
-        float sj[U][H], g[U][H];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int64_t e = eb + 16 * u;
@@ -278,14 +462,15 @@ __global__ __launch_bounds__(256) void softmax_bwd_heads_kernel(SoftmaxArgs a)
 #pragma unroll
     for (int h = 0; h < H; ++h) { S[h] = group_sum(acc[h]); ds[h] = 0.f; }
     for (int64_t eb = e0 + l; eb < e1; eb += 16 * U) {
-        float sj[U][H], g[U][H];
+        if (!one_step) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int64_t e = eb + 16 * u;
-            const bool ok = e < e1;
-            load_heads<H, NM>(a.s1, a.s_hs, a.s_rs, ok ? int64_t(a.col[e]) : 0, sj[u]);
+            for (int u = 0; u < U; ++u) {
+                const int64_t e = eb + 16 * u;
+                const bool ok = e < e1;
+                load_heads<H, NM>(a.s1, a.s_hs, a.s_rs, ok ? int64_t(a.col[e]) : 0, sj[u]);
 #pragma unroll
-            for (int h = 0; h < H; ++h) g[u][h] = ok ? ld(a.dp, a.dp_type, int64_t(h) * a.nnz + e) : 0.f;
+                for (int h = 0; h < H; ++h) g[u][h] = ok ? ld(a.dp, a.dp_type, int64_t(h) * a.nnz + e) : 0.f;
+            }
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -326,33 +511,39 @@ __global__ __launch_bounds__(256) void softmax_bwd_heads_kernel(SoftmaxArgs a)
 // out[h*n + j] = sum over row j of vals(h, perm[e]) (perm -1 or out of range =
 // skip), vals(h, q) at vals[q*H + h] (EDGE_MAJOR) or vals[h*nnz + q]: the
 // column sums of dz through the transpose permutation, with the lane order of
-// gather_rowsum_kernel and the permutation read once for all heads.
+// gather_rowsum_kernel and the permutation read once for all heads.  The first
+// n_long*H workgroups take the long rows.
 template <int H, bool EDGE_MAJOR>
-__global__ __launch_bounds__(256) void gather_rowsum_heads_kernel(const int32_t *rowptr, const int64_t *perm,
-                                                                  int64_t n, int64_t nnz, const float *vals,
-                                                                  float *out)
+__global__ __launch_bounds__(256) void gather_rowsum_heads_kernel(GatherArgs ga)
 {
     constexpr int U = unroll_of<H>();
-    const int64_t j = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 4;
+    __shared__ float red[LONG_THREADS / 64];
+    const int64_t lb = ga.n_long * H;
+    if (int64_t(blockIdx.x) < lb) {
+        gather_long(ga, blockIdx.x, red);
+        return;
+    }
+    const int64_t j = ((int64_t(blockIdx.x) - lb) * blockDim.x + threadIdx.x) >> 4;
     const int l = threadIdx.x & 15;
-    if (j >= n) return;
+    if (j >= ga.n) return;
     float s[H];
 #pragma unroll
     for (int h = 0; h < H; ++h) s[h] = 0.f;
-    const int64_t b = rowptr[j], e1 = rowptr[j + 1];
+    const int64_t b = ga.rowptr[j], e1 = ga.rowptr[j + 1];
+    if (e1 - b > ga.long_min) return;
     for (int64_t eb = b + l; eb < e1; eb += 16 * U) {
         int64_t q[U];
         float v[U][H];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int64_t e = eb + 16 * u;
-            q[u] = e < e1 ? perm[e] : -1;
-            const bool ok = q[u] >= 0 && q[u] < nnz;
-            load_heads<H, EDGE_MAJOR>(vals, nnz, 1, ok ? q[u] : 0, v[u]);
+            q[u] = e < e1 ? ga.perm[e] : -1;
+            const bool ok = q[u] >= 0 && q[u] < ga.nnz;
+            load_heads<H, EDGE_MAJOR>(ga.vals, ga.nnz, 1, ok ? q[u] : 0, v[u]);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            if (q[u] >= 0 && q[u] < nnz) {
+            if (q[u] >= 0 && q[u] < ga.nnz) {
 #pragma unroll
                 for (int h = 0; h < H; ++h) s[h] += v[u][h];
             }
@@ -361,19 +552,21 @@ __global__ __launch_bounds__(256) void gather_rowsum_heads_kernel(const int32_t 
 #pragma unroll
     for (int h = 0; h < H; ++h) {
         const float v = group_sum(s[h]);
-        if (l == 0) out[h * n + j] = v;
+        if (l == 0) ga.out[h * ga.n + j] = v;
     }
 }
 
 // out[h*n + j] = sum over row j of vals[h*nnz + perm[e]] (perm -1 = skip).
 __global__ __launch_bounds__(256) void gather_rowsum_kernel(const int32_t *rowptr, const int64_t *perm, int64_t n,
-                                                            int64_t nnz, int heads, const float *vals, float *out)
+                                                            int64_t nnz, int heads, const float *vals, float *out,
+                                                            int64_t long_min)
 {
     const int64_t gid = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 4;
     const int l = threadIdx.x & 15;
     if (gid >= n * heads) return;
     const int64_t j = gid / heads;
     const int h = int(gid - j * heads);
+    if (int64_t(rowptr[j + 1]) - rowptr[j] > long_min) return;
     float s = 0.f;
     for (int64_t e = rowptr[j] + l; e < rowptr[j + 1]; e += 16) {
         const int64_t q = perm[e];
@@ -399,9 +592,12 @@ bool node_major(int64_t heads, int64_t hs, int64_t rs, const float *s0, const fl
            (reinterpret_cast<uintptr_t>(s1) & al) == 0;
 }
 
+// Grid of the all-heads kernels: the long-row workgroups, then 16 rows per workgroup.
+dim3 heads_grid(const SoftmaxArgs &a) { return dim3(unsigned(a.n_long * a.heads + (a.n * 16 + 255) / 256)); }
+
 hipError_t launch_fwd_heads(const SoftmaxArgs &a, bool nm, hipStream_t s)
 {
-    const dim3 grid(unsigned((a.n * 16 + 255) / 256)), block(256);
+    const dim3 grid = heads_grid(a), block(256);
     switch (a.heads * 2 + int(nm)) {
     case 2: hipLaunchKernelGGL((softmax_fwd_heads_kernel<1, false>), grid, block, 0, s, a); break;
     case 4: hipLaunchKernelGGL((softmax_fwd_heads_kernel<2, false>), grid, block, 0, s, a); break;
@@ -418,7 +614,7 @@ hipError_t launch_fwd_heads(const SoftmaxArgs &a, bool nm, hipStream_t s)
 template <bool EM>
 hipError_t launch_bwd_heads(const SoftmaxArgs &a, bool nm, hipStream_t s)
 {
-    const dim3 grid(unsigned((a.n * 16 + 255) / 256)), block(256);
+    const dim3 grid = heads_grid(a), block(256);
     switch (a.heads * 2 + int(nm)) {
     case 2: hipLaunchKernelGGL((softmax_bwd_heads_kernel<1, false, EM>), grid, block, 0, s, a); break;
     case 4: hipLaunchKernelGGL((softmax_bwd_heads_kernel<2, false, EM>), grid, block, 0, s, a); break;
@@ -432,19 +628,42 @@ hipError_t launch_bwd_heads(const SoftmaxArgs &a, bool nm, hipStream_t s)
     return hipGetLastError();
 }
 
-template <bool EM>
-hipError_t launch_gather_heads(const int32_t *rowptr, const int64_t *perm, int64_t n, int64_t nnz, int heads,
-                               const float *vals, float *out, hipStream_t s)
+// Column sums: the all-heads kernel (1, 2, 4 or 8 heads; long rows in the same
+// launch) or the per-head kernel followed by a workgroup per (long row, head).
+hipError_t launch_gather(const int32_t *rowptr, const int64_t *perm, int64_t n, int64_t nnz, int heads,
+                         const float *vals, bool edge_major, float *out, const int32_t *long_rows, int64_t n_long,
+                         int64_t long_min, hipStream_t s)
 {
-    const dim3 grid(unsigned((n * 16 + 255) / 256)), block(256);
-    switch (heads) {
-    case 1: hipLaunchKernelGGL((gather_rowsum_heads_kernel<1, EM>), grid, block, 0, s, rowptr, perm, n, nnz, vals, out); break;
-    case 2: hipLaunchKernelGGL((gather_rowsum_heads_kernel<2, EM>), grid, block, 0, s, rowptr, perm, n, nnz, vals, out); break;
-    case 4: hipLaunchKernelGGL((gather_rowsum_heads_kernel<4, EM>), grid, block, 0, s, rowptr, perm, n, nnz, vals, out); break;
-    case 8: hipLaunchKernelGGL((gather_rowsum_heads_kernel<8, EM>), grid, block, 0, s, rowptr, perm, n, nnz, vals, out); break;
-    default: return hipErrorInvalidValue;
+    GatherArgs ga{rowptr, perm, vals, out, long_rows, n, nnz, long_min, n_long, heads, edge_major ? 1 : 0};
+    const dim3 block(256);
+    if (heads == 1 || heads == 2 || heads == 4 || heads == 8) {
+        const dim3 grid(unsigned(n_long * heads + (n * 16 + 255) / 256));
+#define ATTGNN_G_CASE(HV, EMV)                                                                                   \
+        if (heads == HV && edge_major == EMV)                                                                    \
+            hipLaunchKernelGGL((gather_rowsum_heads_kernel<HV, EMV>), grid, block, 0, s, ga);
+        ATTGNN_G_CASE(1, false) ATTGNN_G_CASE(2, false) ATTGNN_G_CASE(4, false) ATTGNN_G_CASE(8, false)
+        ATTGNN_G_CASE(1, true) ATTGNN_G_CASE(2, true) ATTGNN_G_CASE(4, true) ATTGNN_G_CASE(8, true)
+#undef ATTGNN_G_CASE
+        return hipGetLastError();
     }
+    if (edge_major) return hipErrorInvalidValue;
+    const int64_t threads = n * heads * 16;
+    hipLaunchKernelGGL(gather_rowsum_kernel, dim3(unsigned((threads + 255) / 256)), block, 0, s, rowptr, perm, n, nnz,
+                       heads, vals, out, long_min);
+    if (n_long > 0)
+        hipLaunchKernelGGL(gather_rowsum_long_kernel, dim3(unsigned(n_long * heads)), dim3(LONG_THREADS), 0, s, ga);
     return hipGetLastError();
+}
+
+// long_min for a row list: rows above ATTGNN_LONG_ROW entries when a list is given.
+int64_t long_min_of(const int32_t *long_rows, int64_t n_long)
+{
+    return (long_rows && n_long > 0) ? int64_t(ATTGNN_LONG_ROW) : INT64_MAX;
+}
+
+bool long_args_ok(const int32_t *long_rows, int64_t n_long)
+{
+    return n_long >= 0 && n_long <= INT32_MAX && (n_long == 0 || long_rows);
 }
 
 }  // namespace
@@ -452,23 +671,31 @@ hipError_t launch_gather_heads(const int32_t *rowptr, const int64_t *perm, int64
 extern "C" int attgnn_edge_softmax_fwd(const int32_t *rowptr, const int32_t *col, int64_t n, int64_t nnz,
                                        int64_t heads, const float *s0, const float *s1, int64_t s_head_stride,
                                        int64_t s_row_stride, float slope, float drop_p, uint64_t seed, void *p,
-                                       int p_type, float *lse, attgnn_stream_t stream)
+                                       int p_type, float *lse, const int32_t *long_rows, int64_t n_long,
+                                       attgnn_stream_t stream)
 {
-    if (n < 0 || nnz < 0 || heads <= 0 || !float_type(p_type) || drop_p < 0.f || drop_p >= 1.f) return ATTGNN_E_ARG;
+    if (n < 0 || nnz < 0 || heads <= 0 || heads > 65535 || !float_type(p_type) || drop_p < 0.f || drop_p >= 1.f)
+        return ATTGNN_E_ARG;
+    if (!long_args_ok(long_rows, n_long)) return ATTGNN_E_ARG;
     if (n == 0) return ATTGNN_OK;
     if (!rowptr || !s0 || !s1 || !lse || (nnz > 0 && (!col || !p))) return ATTGNN_E_ARG;
     SoftmaxArgs a{};
     a.rowptr = rowptr; a.col = col; a.s0 = s0; a.s1 = s1; a.p = static_cast<uint8_t *>(p); a.lse_out = lse;
     a.n = n; a.nnz = nnz; a.s_hs = s_head_stride; a.s_rs = s_row_stride; a.heads = int(heads); a.p_type = p_type;
     a.slope = slope; a.thresh = drop_threshold(drop_p); a.keep_scale = 1.f / (1.f - drop_p); a.seed = seed;
+    a.long_min = long_min_of(long_rows, n_long);
+    a.long_rows = long_rows; a.n_long = long_rows ? n_long : 0;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    hipError_t e;
     if (heads_kernel_ok(heads)) {
-        const hipError_t e = launch_fwd_heads(a, node_major(heads, s_head_stride, s_row_stride, s0, s1), s);
-        return e == hipSuccess ? ATTGNN_OK : int(e);
+        e = launch_fwd_heads(a, node_major(heads, s_head_stride, s_row_stride, s0, s1), s);
+    } else {
+        const int64_t threads = n * heads * 16;
+        hipLaunchKernelGGL(edge_softmax_fwd_kernel, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, s, a);
+        if (a.n_long > 0)
+            hipLaunchKernelGGL(softmax_fwd_long_kernel, dim3(unsigned(a.n_long * heads)), dim3(LONG_THREADS), 0, s, a);
+        e = hipGetLastError();
     }
-    const int64_t threads = n * heads * 16;
-    hipLaunchKernelGGL(edge_softmax_fwd_kernel, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, s, a);
-    const hipError_t e = hipGetLastError();
     return e == hipSuccess ? ATTGNN_OK : int(e);
 }
 
@@ -476,14 +703,32 @@ namespace {
 
 SoftmaxArgs bwd_args(const int32_t *rowptr, const int32_t *col, int64_t n, int64_t nnz, int64_t heads,
                      const float *s0, const float *s1, int64_t s_head_stride, int64_t s_row_stride, const float *lse,
-                     const void *dp, int dp_type, float slope, float drop_p, uint64_t seed, float *dz, float *ds0)
+                     const void *dp, int dp_type, float slope, float drop_p, uint64_t seed, float *dz, float *ds0,
+                     const int32_t *long_rows, int64_t n_long)
 {
     SoftmaxArgs a{};
     a.rowptr = rowptr; a.col = col; a.s0 = s0; a.s1 = s1; a.lse = lse; a.dp = static_cast<const uint8_t *>(dp);
     a.dz = dz; a.ds0 = ds0; a.n = n; a.nnz = nnz; a.s_hs = s_head_stride; a.s_rs = s_row_stride;
     a.heads = int(heads); a.dp_type = dp_type; a.slope = slope; a.thresh = drop_threshold(drop_p);
     a.keep_scale = 1.f / (1.f - drop_p); a.seed = seed;
+    a.long_min = long_min_of(long_rows, n_long);
+    a.long_rows = long_rows; a.n_long = long_rows ? n_long : 0;
     return a;
+}
+
+// Softmax backward over the rows (long rows included).
+hipError_t launch_bwd(SoftmaxArgs a, bool edge_major, hipStream_t s)
+{
+    a.dz_edge_major = edge_major ? 1 : 0;
+    const bool nm = node_major(a.heads, a.s_hs, a.s_rs, a.s0, a.s1);
+    if (heads_kernel_ok(a.heads))
+        return edge_major ? launch_bwd_heads<true>(a, nm, s) : launch_bwd_heads<false>(a, nm, s);
+    if (edge_major) return hipErrorInvalidValue;
+    const int64_t threads = a.n * a.heads * 16;
+    hipLaunchKernelGGL(edge_softmax_bwd_kernel, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, s, a);
+    if (a.n_long > 0)
+        hipLaunchKernelGGL(softmax_bwd_long_kernel, dim3(unsigned(a.n_long * a.heads)), dim3(LONG_THREADS), 0, s, a);
+    return hipGetLastError();
 }
 
 }  // namespace
@@ -492,21 +737,16 @@ extern "C" int attgnn_edge_softmax_bwd(const int32_t *rowptr, const int32_t *col
                                        int64_t heads, const float *s0, const float *s1, int64_t s_head_stride,
                                        int64_t s_row_stride, const float *lse, const void *dp, int dp_type,
                                        float slope, float drop_p, uint64_t seed, float *dz, float *ds0,
-                                       attgnn_stream_t stream)
+                                       const int32_t *long_rows, int64_t n_long, attgnn_stream_t stream)
 {
-    if (n < 0 || nnz < 0 || heads <= 0 || !float_type(dp_type) || drop_p < 0.f || drop_p >= 1.f) return ATTGNN_E_ARG;
+    if (n < 0 || nnz < 0 || heads <= 0 || heads > 65535 || !float_type(dp_type) || drop_p < 0.f || drop_p >= 1.f)
+        return ATTGNN_E_ARG;
+    if (!long_args_ok(long_rows, n_long)) return ATTGNN_E_ARG;
     if (n == 0) return ATTGNN_OK;
     if (!rowptr || !s0 || !s1 || !lse || !ds0 || (nnz > 0 && (!col || !dp || !dz))) return ATTGNN_E_ARG;
     const SoftmaxArgs a = bwd_args(rowptr, col, n, nnz, heads, s0, s1, s_head_stride, s_row_stride, lse, dp, dp_type,
-                                   slope, drop_p, seed, dz, ds0);
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    if (heads_kernel_ok(heads)) {
-        const hipError_t e = launch_bwd_heads<false>(a, node_major(heads, s_head_stride, s_row_stride, s0, s1), s);
-        return e == hipSuccess ? ATTGNN_OK : int(e);
-    }
-    const int64_t threads = n * heads * 16;
-    hipLaunchKernelGGL(edge_softmax_bwd_kernel, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, s, a);
-    const hipError_t e = hipGetLastError();
+                                   slope, drop_p, seed, dz, ds0, long_rows, n_long);
+    const hipError_t e = launch_bwd(a, false, reinterpret_cast<hipStream_t>(stream));
     return e == hipSuccess ? ATTGNN_OK : int(e);
 }
 
@@ -520,47 +760,50 @@ extern "C" int attgnn_edge_softmax_bwd_sums(const int32_t *rowptr, const int32_t
                                             int64_t heads, const float *s0, const float *s1,
                                             int64_t s_head_stride, int64_t s_row_stride, const float *lse,
                                             const void *dp, int dp_type, float slope, float drop_p, uint64_t seed,
-                                            const int32_t *t_rowptr, const int64_t *t_perm, float *ds0, float *ds1,
-                                            float *workspace, int64_t workspace_elems, attgnn_stream_t stream)
+                                            const int32_t *t_rowptr, const int64_t *t_perm,
+                                            const int32_t *long_rows, int64_t n_long, const int32_t *t_long_rows,
+                                            int64_t n_t_long, float *ds0, float *ds1, float *workspace,
+                                            int64_t workspace_elems, attgnn_stream_t stream)
 {
-    if (n < 0 || nnz < 0 || heads <= 0 || !float_type(dp_type) || drop_p < 0.f || drop_p >= 1.f) return ATTGNN_E_ARG;
+    if (n < 0 || nnz < 0 || heads <= 0 || heads > 65535 || !float_type(dp_type) || drop_p < 0.f || drop_p >= 1.f)
+        return ATTGNN_E_ARG;
+    if (!long_args_ok(long_rows, n_long) || !long_args_ok(t_long_rows, n_t_long)) return ATTGNN_E_ARG;
     if (n == 0) return ATTGNN_OK;
     if (!rowptr || !t_rowptr || !s0 || !s1 || !lse || !ds0 || !ds1) return ATTGNN_E_ARG;
     if (nnz > 0 && (!col || !dp || !t_perm || !workspace)) return ATTGNN_E_ARG;
     if (workspace_elems < nnz * heads) return ATTGNN_E_ARG;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    // dz in the workspace (edge-major for the all-heads kernels), then the column sums
     const bool em = heads_kernel_ok(heads) && (reinterpret_cast<uintptr_t>(workspace) & 15) == 0;
     const SoftmaxArgs a = bwd_args(rowptr, col, n, nnz, heads, s0, s1, s_head_stride, s_row_stride, lse, dp, dp_type,
-                                   slope, drop_p, seed, workspace, ds0);
-    hipError_t e;
-    if (em) {
-        // dz edge-major in the workspace, then column sums reading one record per edge
-        e = launch_bwd_heads<true>(a, node_major(heads, s_head_stride, s_row_stride, s0, s1), s);
-        if (e == hipSuccess) e = launch_gather_heads<true>(t_rowptr, t_perm, n, nnz, int(heads), workspace, ds1, s);
-    } else {
-        const int64_t threads = n * heads * 16;
-        hipLaunchKernelGGL(edge_softmax_bwd_kernel, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, s, a);
-        hipLaunchKernelGGL(gather_rowsum_kernel, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, s, t_rowptr,
-                           t_perm, n, nnz, int(heads), static_cast<const float *>(workspace), ds1);
-        e = hipGetLastError();
-    }
+                                   slope, drop_p, seed, workspace, ds0, long_rows, n_long);
+    hipError_t e = launch_bwd(a, em, s);
+    if (e == hipSuccess)
+        e = launch_gather(t_rowptr, t_perm, n, nnz, int(heads), workspace, em, ds1, t_long_rows, n_t_long,
+                          long_min_of(t_long_rows, n_t_long), s);
     return e == hipSuccess ? ATTGNN_OK : int(e);
 }
 
 extern "C" int attgnn_gather_rowsum(const int32_t *rowptr, const int64_t *perm, int64_t n, int64_t nnz,
-                                    int64_t heads, const float *vals, float *out, attgnn_stream_t stream)
+                                    int64_t heads, const float *vals, float *out, const int32_t *long_rows,
+                                    int64_t n_long, attgnn_stream_t stream)
 {
-    if (n < 0 || nnz < 0 || heads <= 0) return ATTGNN_E_ARG;
+    if (n < 0 || nnz < 0 || heads <= 0 || heads > 65535 || !long_args_ok(long_rows, n_long)) return ATTGNN_E_ARG;
     if (n == 0) return ATTGNN_OK;
     if (!rowptr || !out || (nnz > 0 && (!perm || !vals))) return ATTGNN_E_ARG;
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    if (heads_kernel_ok(heads)) {
-        const hipError_t e = launch_gather_heads<false>(rowptr, perm, n, nnz, int(heads), vals, out, s);
-        return e == hipSuccess ? ATTGNN_OK : int(e);
-    }
-    const int64_t threads = n * heads * 16;
-    hipLaunchKernelGGL(gather_rowsum_kernel, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, s, rowptr, perm,
-                       n, nnz, int(heads), vals, out);
-    const hipError_t e = hipGetLastError();
+    const hipError_t e = launch_gather(rowptr, perm, n, nnz, int(heads), vals, false, out, long_rows, n_long,
+                                       long_min_of(long_rows, n_long), reinterpret_cast<hipStream_t>(stream));
     return e == hipSuccess ? ATTGNN_OK : int(e);
+}
+
+extern "C" int64_t attgnn_long_rows(const int32_t *rowptr, int64_t n, int32_t *rows)
+{
+    if (n < 0 || (n > 0 && !rowptr) || n > INT32_MAX) return -1;
+    int64_t k = 0;
+    for (int64_t i = 0; i < n; ++i)
+        if (int64_t(rowptr[i + 1]) - rowptr[i] > ATTGNN_LONG_ROW) {
+            if (rows) rows[k] = int32_t(i);
+            ++k;
+        }
+    return k;
 }

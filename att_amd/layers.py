@@ -324,6 +324,16 @@ class GATGraph:
                               ops.DeviceSegments(G.csr_segments(trp), dev))
         return self._segments
 
+    def long_row_lists(self):
+        """Rows of A and of A^T with more than 256 entries (int32, on the
+        device), which the edge-softmax kernels give a workgroup each; built on
+        first use from a host copy of the row pointers."""
+        if getattr(self, "_long_rows", None) is None:
+            lr = ops.long_rows(self.rowptr)
+            tl = lr if self.t_rowptr is self.rowptr else ops.long_rows(self.t_rowptr)
+            self._long_rows = (lr, tl)
+        return self._long_rows
+
     def sddmm(self, A, B, out_dtype=torch.float32):
         """out[h, e=(i,j)] = <A[i,h], B[j,h]> over A's edges ([N, H, D] inputs, [H, nnz] out)."""
         rp, col, emap, win, _ = self.fwd
@@ -349,7 +359,9 @@ class GATAttention(torch.autograd.Function):
         # node-major [N, H] scores: one load per neighbour for all heads
         s0 = ops.node_scores(Xs, a0.to(Xs.dtype).contiguous(), "mhd", out_layout="nh")
         s1 = ops.node_scores(Xt, a1.to(Xt.dtype).contiguous(), "mhd", out_layout="nh")
-        p, lse = ops.edge_softmax_fwd(g.rowptr, g.col, s0, s1, slope, drop_p, seed, p_dtype=Xt.dtype, s_layout="nh")
+        long_rows, _ = g.long_row_lists()
+        p, lse = ops.edge_softmax_fwd(g.rowptr, g.col, s0, s1, slope, drop_p, seed, p_dtype=Xt.dtype, s_layout="nh",
+                                      long_rows=long_rows)
         rp, col, emap, win, at = g.fwd
         h = ops.spmm_edge_map(rp, col, emap, p, win, at, Xt, "mhd", g.window, g.wide, m_ori=g.num_nodes_ori,
                               x_layout="mhd", out_dtype=Xt.dtype, rows_covered=True)
@@ -371,8 +383,10 @@ class GATAttention(torch.autograd.Function):
         # dp[(i,j)] = <dh[i], Xt[j]>  (SDDMM over A)
         dp = g.sddmm(dh, Xt)
         # row and column sums of dL/dz (columns through the transpose permutation)
+        long_rows, t_long_rows = g.long_row_lists()
         ds0, ds1 = ops.edge_softmax_bwd_sums(g.rowptr, g.col, s0, s1, lse, dp, ctx.slope, ctx.drop_p, ctx.seed,
-                                             g.t_rowptr, g.t_perm, s_layout="nh")
+                                             g.t_rowptr, g.t_perm, s_layout="nh", long_rows=long_rows,
+                                             t_long_rows=t_long_rows)
         # s0 = Xs.a0, s1 = Xt.a1: node-term gradients fused with the aggregation
         # gradient (one pass over X; GAT's shared X W gets the summed gradient)
         shared = ctx.shared and Xs.data_ptr() == Xt.data_ptr()

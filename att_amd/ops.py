@@ -521,6 +521,31 @@ def dtype_code(dtype):
 # fused edge softmax (CSR rows)
 # --------------------------------------------------------------------------
 
+def long_rows(rowptr):
+    """int32 device tensor of the rows with more than ATTGNN_LONG_ROW (256)
+    entries, the list the row kernels take so a workgroup handles each of them
+    (one host copy of the row pointers)."""
+    rp = torch.as_tensor(rowptr).to(torch.int32).contiguous().cpu()
+    n = rp.numel() - 1
+    if n < 0:
+        raise ValueError("rowptr must have at least one element")
+    k = lib().attgnn_long_rows(_ptr(rp), n, None)
+    out = torch.empty(max(k, 0), dtype=torch.int32)
+    if k > 0:
+        lib().attgnn_long_rows(_ptr(rp), n, _ptr(out))
+    dev = rowptr.device if isinstance(rowptr, torch.Tensor) else torch.device("cpu")
+    return out.to(dev)
+
+
+def _rows_arg(name, rows, dev):
+    """(pointer, count) of an optional long-row list."""
+    if rows is None or rows.numel() == 0:
+        return None, 0
+    if rows.dtype != torch.int32 or rows.device != dev or not rows.is_contiguous():
+        raise ValueError(f"{name} must be a contiguous int32 tensor on {dev}")
+    return _ptr(rows), rows.numel()
+
+
 def _score_layout(s0, s1, s_layout):
     """(heads, n, head_stride, row_stride) of [H, N] ("hn") or [N, H] ("nh") scores."""
     if s0.shape != s1.shape or s0.dim() != 2 or s0.dtype != torch.float32 or s1.dtype != torch.float32:
@@ -536,38 +561,43 @@ def _score_layout(s0, s1, s_layout):
     raise ValueError(f"s_layout must be 'hn' or 'nh', got {s_layout!r}")
 
 
-def edge_softmax_fwd(rowptr, col, s0, s1, slope, drop_p=0.0, seed=0, p_dtype=torch.float32, s_layout="hn"):
+def edge_softmax_fwd(rowptr, col, s0, s1, slope, drop_p=0.0, seed=0, p_dtype=torch.float32, s_layout="hn",
+                     long_rows=None):
     """p [H, nnz] and lse [H, N] of the fused LeakyReLU + softmax + dropout;
-    s0 / s1 [H, N] ("hn") or [N, H] ("nh")."""
+    s0 / s1 [H, N] ("hn") or [N, H] ("nh").  ``long_rows``: the list from
+    :func:`long_rows` (rows above 256 entries get a workgroup each)."""
     dev = _need_gpu("edge_softmax_fwd", rowptr, col, s0, s1)
     _need_i32("edge_softmax_fwd", rowptr, col)
     heads, n, shs, srs = _score_layout(s0, s1, s_layout)
+    lp, ln = _rows_arg("long_rows", long_rows, dev)
     nnz = col.numel()
     p = torch.empty((heads, nnz), dtype=p_dtype, device=dev)
     lse = torch.empty((heads, n), dtype=torch.float32, device=dev)
     rc = lib().attgnn_edge_softmax_fwd(_ptr(rowptr), _ptr(col), n, nnz, heads, _ptr(s0), _ptr(s1), shs, srs,
                                        float(slope), float(drop_p), int(seed) & (2**64 - 1), _ptr(p),
-                                       _TYPE_OF[p_dtype], _ptr(lse), _stream(dev))
+                                       _TYPE_OF[p_dtype], _ptr(lse), lp, ln, _stream(dev))
     check(rc, "attgnn_edge_softmax_fwd")
     return p, lse
 
 
-def edge_softmax_bwd(rowptr, col, s0, s1, lse, dp, slope, drop_p=0.0, seed=0, s_layout="hn"):
+def edge_softmax_bwd(rowptr, col, s0, s1, lse, dp, slope, drop_p=0.0, seed=0, s_layout="hn", long_rows=None):
     """dz [H, nnz] (fp32) and ds0 [H, N] (row sums of dz)."""
     dev = _need_gpu("edge_softmax_bwd", rowptr, col, s0, s1, lse, dp)
     _need_i32("edge_softmax_bwd", rowptr, col)
     heads, n, shs, srs = _score_layout(s0, s1, s_layout)
+    lp, ln = _rows_arg("long_rows", long_rows, dev)
     nnz = col.numel()
     dz = torch.empty((heads, nnz), dtype=torch.float32, device=dev)
     ds0 = torch.empty((heads, n), dtype=torch.float32, device=dev)
     rc = lib().attgnn_edge_softmax_bwd(_ptr(rowptr), _ptr(col), n, nnz, heads, _ptr(s0), _ptr(s1), shs, srs,
                                        _ptr(lse), _ptr(dp), _TYPE_OF[dp.dtype], float(slope), float(drop_p),
-                                       int(seed) & (2**64 - 1), _ptr(dz), _ptr(ds0), _stream(dev))
+                                       int(seed) & (2**64 - 1), _ptr(dz), _ptr(ds0), lp, ln, _stream(dev))
     check(rc, "attgnn_edge_softmax_bwd")
     return dz, ds0
 
 
-def edge_softmax_bwd_sums(rowptr, col, s0, s1, lse, dp, slope, drop_p, seed, t_rowptr, t_perm, s_layout="hn"):
+def edge_softmax_bwd_sums(rowptr, col, s0, s1, lse, dp, slope, drop_p, seed, t_rowptr, t_perm, s_layout="hn",
+                          long_rows=None, t_long_rows=None):
     """(ds0, ds1) fp32 [H, N]: the row and column sums of the softmax backward's
     dz, without returning dz (bitwise equal to edge_softmax_bwd followed by
     gather_rowsum over the transpose)."""
@@ -579,6 +609,8 @@ def edge_softmax_bwd_sums(rowptr, col, s0, s1, lse, dp, slope, drop_p, seed, t_r
     nnz = col.numel()
     if rowptr.numel() != n + 1 or t_rowptr.numel() != n + 1 or t_perm.numel() != nnz:
         raise ValueError("rowptr / t_rowptr / t_perm sizes disagree with the scores and col")
+    lp, ln = _rows_arg("long_rows", long_rows, dev)
+    tp, tn = _rows_arg("t_long_rows", t_long_rows, dev)
     ds0 = torch.empty((heads, n), dtype=torch.float32, device=dev)
     ds1 = torch.empty((heads, n), dtype=torch.float32, device=dev)
     L = lib()
@@ -586,8 +618,8 @@ def edge_softmax_bwd_sums(rowptr, col, s0, s1, lse, dp, slope, drop_p, seed, t_r
     ws = torch.empty(max(ws_n, 1), dtype=torch.float32, device=dev)
     check(L.attgnn_edge_softmax_bwd_sums(_ptr(rowptr), _ptr(col), n, nnz, heads, _ptr(s0), _ptr(s1), shs, srs,
                                          _ptr(lse), _ptr(dp), _TYPE_OF[dp.dtype], float(slope), float(drop_p),
-                                         int(seed) & (2**64 - 1), _ptr(t_rowptr), _ptr(t_perm), _ptr(ds0),
-                                         _ptr(ds1), _ptr(ws), ws_n, _stream(dev)),
+                                         int(seed) & (2**64 - 1), _ptr(t_rowptr), _ptr(t_perm), lp, ln, tp, tn,
+                                         _ptr(ds0), _ptr(ds1), _ptr(ws), ws_n, _stream(dev)),
           "attgnn_edge_softmax_bwd_sums")
     return ds0, ds1
 
@@ -660,13 +692,15 @@ def gatv2_bwd(rowptr, col, segs, t_col, t_perm, t_segs, X0, X1, att, slope, z, l
     return dz, dX0, dX1, datt
 
 
-def gather_rowsum(rowptr, perm, vals, n):
+def gather_rowsum(rowptr, perm, vals, n, long_rows=None):
     """out[h, j] = sum_{e in row j} vals[h, perm[e]] (column sums via the transpose permutation)."""
     dev = _need_gpu("gather_rowsum", rowptr, perm, vals)
     heads, nnz = vals.shape
     if perm.dtype != torch.int64:
         raise TypeError("perm must be int64")
+    lp, ln = _rows_arg("long_rows", long_rows, dev)
     out = torch.empty((heads, n), dtype=torch.float32, device=dev)
-    rc = lib().attgnn_gather_rowsum(_ptr(rowptr), _ptr(perm), n, nnz, heads, _ptr(vals), _ptr(out), _stream(dev))
+    rc = lib().attgnn_gather_rowsum(_ptr(rowptr), _ptr(perm), n, nnz, heads, _ptr(vals), _ptr(out), lp, ln,
+                                    _stream(dev))
     check(rc, "attgnn_gather_rowsum")
     return out

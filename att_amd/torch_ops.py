@@ -116,28 +116,30 @@ def _sdims(s0, s_layout):
 
 @torch.library.custom_op(f"{_LIB}::edge_softmax_fwd", mutates_args=())
 def edge_softmax_fwd(rowptr: torch.Tensor, col: torch.Tensor, s0: torch.Tensor, s1: torch.Tensor, slope: float,
-                     drop_p: float, seed: int, p_dtype: torch.dtype,
-                     s_layout: str = "hn") -> tuple[torch.Tensor, torch.Tensor]:
+                     drop_p: float, seed: int, p_dtype: torch.dtype, s_layout: str = "hn",
+                     long_rows: Optional[torch.Tensor] = None) -> tuple[torch.Tensor, torch.Tensor]:
     """Fused LeakyReLU + row softmax + dropout (attgnn_edge_softmax_fwd): (p, lse)."""
-    return ops.edge_softmax_fwd(rowptr, col, s0, s1, slope, drop_p, seed, p_dtype=p_dtype, s_layout=s_layout)
+    return ops.edge_softmax_fwd(rowptr, col, s0, s1, slope, drop_p, seed, p_dtype=p_dtype, s_layout=s_layout,
+                                long_rows=long_rows)
 
 
 @edge_softmax_fwd.register_fake
-def _(rowptr, col, s0, s1, slope, drop_p, seed, p_dtype, s_layout="hn"):
+def _(rowptr, col, s0, s1, slope, drop_p, seed, p_dtype, s_layout="hn", long_rows=None):
     heads, n = _sdims(s0, s_layout)
     return s0.new_empty((heads, col.shape[0]), dtype=p_dtype), s0.new_empty((heads, n))
 
 
 @torch.library.custom_op(f"{_LIB}::edge_softmax_bwd", mutates_args=())
 def edge_softmax_bwd(rowptr: torch.Tensor, col: torch.Tensor, s0: torch.Tensor, s1: torch.Tensor, lse: torch.Tensor,
-                     dp: torch.Tensor, slope: float, drop_p: float, seed: int,
-                     s_layout: str = "hn") -> tuple[torch.Tensor, torch.Tensor]:
+                     dp: torch.Tensor, slope: float, drop_p: float, seed: int, s_layout: str = "hn",
+                     long_rows: Optional[torch.Tensor] = None) -> tuple[torch.Tensor, torch.Tensor]:
     """Backward of edge_softmax_fwd (attgnn_edge_softmax_bwd): (dz, ds0)."""
-    return ops.edge_softmax_bwd(rowptr, col, s0, s1, lse, dp, slope, drop_p, seed, s_layout=s_layout)
+    return ops.edge_softmax_bwd(rowptr, col, s0, s1, lse, dp, slope, drop_p, seed, s_layout=s_layout,
+                                long_rows=long_rows)
 
 
 @edge_softmax_bwd.register_fake
-def _(rowptr, col, s0, s1, lse, dp, slope, drop_p, seed, s_layout="hn"):
+def _(rowptr, col, s0, s1, lse, dp, slope, drop_p, seed, s_layout="hn", long_rows=None):
     heads, n = _sdims(s0, s_layout)
     return s0.new_empty((heads, col.shape[0])), s0.new_empty((heads, n))
 
@@ -145,27 +147,30 @@ def _(rowptr, col, s0, s1, lse, dp, slope, drop_p, seed, s_layout="hn"):
 @torch.library.custom_op(f"{_LIB}::edge_softmax_bwd_sums", mutates_args=())
 def edge_softmax_bwd_sums(rowptr: torch.Tensor, col: torch.Tensor, s0: torch.Tensor, s1: torch.Tensor,
                           lse: torch.Tensor, dp: torch.Tensor, slope: float, drop_p: float, seed: int,
-                          t_rowptr: torch.Tensor, t_perm: torch.Tensor,
-                          s_layout: str = "hn") -> tuple[torch.Tensor, torch.Tensor]:
+                          t_rowptr: torch.Tensor, t_perm: torch.Tensor, s_layout: str = "hn",
+                          long_rows: Optional[torch.Tensor] = None,
+                          t_long_rows: Optional[torch.Tensor] = None) -> tuple[torch.Tensor, torch.Tensor]:
     """Row and column sums of the softmax backward's dz (attgnn_edge_softmax_bwd_sums): (ds0, ds1)."""
     return ops.edge_softmax_bwd_sums(rowptr, col, s0, s1, lse, dp, slope, drop_p, seed, t_rowptr, t_perm,
-                                     s_layout=s_layout)
+                                     s_layout=s_layout, long_rows=long_rows, t_long_rows=t_long_rows)
 
 
 @edge_softmax_bwd_sums.register_fake
-def _(rowptr, col, s0, s1, lse, dp, slope, drop_p, seed, t_rowptr, t_perm, s_layout="hn"):
+def _(rowptr, col, s0, s1, lse, dp, slope, drop_p, seed, t_rowptr, t_perm, s_layout="hn", long_rows=None,
+      t_long_rows=None):
     heads, n = _sdims(s0, s_layout)
     return s0.new_empty((heads, n)), s0.new_empty((heads, n))
 
 
 @torch.library.custom_op(f"{_LIB}::gather_rowsum", mutates_args=())
-def gather_rowsum(rowptr: torch.Tensor, perm: torch.Tensor, vals: torch.Tensor, n: int) -> torch.Tensor:
+def gather_rowsum(rowptr: torch.Tensor, perm: torch.Tensor, vals: torch.Tensor, n: int,
+                  long_rows: Optional[torch.Tensor] = None) -> torch.Tensor:
     """Column sums through the transpose permutation (attgnn_gather_rowsum)."""
-    return ops.gather_rowsum(rowptr, perm, vals, n)
+    return ops.gather_rowsum(rowptr, perm, vals, n, long_rows=long_rows)
 
 
 @gather_rowsum.register_fake
-def _(rowptr, perm, vals, n):
+def _(rowptr, perm, vals, n, long_rows=None):
     return vals.new_empty((vals.shape[0], n), dtype=torch.float32)
 
 

@@ -475,6 +475,21 @@ int attgnn_gatv2_bwd(const int32_t *rowptr, const int32_t *col, const attgnn_seg
 /* ====================================================================== */
 
 /*
+ * Long rows.  The row kernels below give each row a 16-lane group, so on a
+ * power-law graph a hub row of a few thousand entries outlasts all the others.
+ * Every row entry point takes an optional list of the rows with more than
+ * ATTGNN_LONG_ROW entries (long_rows, n_long; NULL / 0 = none): the group
+ * kernel then skips those rows and a 256-thread workgroup per (listed row,
+ * head) takes them.  The list must hold every row above the threshold (any
+ * order; listed rows at or below it are ignored).  attgnn_long_rows fills it
+ * from a host copy of the row pointers and returns its length (rows = NULL:
+ * count only; -1 on bad arguments).  Results depend on the list only through
+ * the summation order of the long rows, and are deterministic either way.
+ */
+#define ATTGNN_LONG_ROW 256
+int64_t attgnn_long_rows(const int32_t *rowptr_host, int64_t n, int32_t *rows);
+
+/*
  * Attention probabilities of one layer, fused:
  *   z = s0[h,i] + s1[h,j];  v = LeakyReLU_slope(z);  p = exp(v - lse_i) with
  *   lse_i = log sum_j exp(v) (max subtracted, FP32);  dropout with probability
@@ -491,7 +506,8 @@ int attgnn_gatv2_bwd(const int32_t *rowptr, const int32_t *col, const attgnn_seg
 int attgnn_edge_softmax_fwd(const int32_t *rowptr, const int32_t *col, int64_t n, int64_t nnz,
                             int64_t heads, const float *s0, const float *s1, int64_t s_head_stride,
                             int64_t s_row_stride, float slope, float drop_p, uint64_t seed, void *p,
-                            int p_type, float *lse, attgnn_stream_t stream);
+                            int p_type, float *lse, const int32_t *long_rows, int64_t n_long,
+                            attgnn_stream_t stream);
 
 /*
  * Backward of attgnn_edge_softmax_fwd: p is recomputed from (s0, s1, lse) and
@@ -505,14 +521,15 @@ int attgnn_edge_softmax_bwd(const int32_t *rowptr, const int32_t *col, int64_t n
                             int64_t heads, const float *s0, const float *s1, int64_t s_head_stride,
                             int64_t s_row_stride, const float *lse, const void *dp, int dp_type,
                             float slope, float drop_p, uint64_t seed, float *dz, float *ds0,
-                            attgnn_stream_t stream);
+                            const int32_t *long_rows, int64_t n_long, attgnn_stream_t stream);
 
 /*
  * Backward of attgnn_edge_softmax_fwd reduced to the two node gradients the
  * attention layers need: ds0[h*n + i] = sum_j dz[h, (i, j)] (row sums) and
  * ds1[h*n + j] = sum_i dz[h, (i, j)] (column sums, through the transpose:
  * t_rowptr [n+1] holds the rows of A^T and t_perm[q] the CSR position of A^T
- * entry q, as attgnn_csr_transpose_device writes them).  dz is kept in the
+ * entry q, as attgnn_csr_transpose_device writes them; long_rows / t_long_rows
+ * list the long rows of A and of A^T).  dz is kept in the
  * workspace only (fp32, attgnn_edge_softmax_bwd_sums_workspace_elems
  * elements), edge-major for 1, 2, 4 or 8 heads.  Results are bitwise equal to
  * attgnn_edge_softmax_bwd followed by attgnn_gather_rowsum.
@@ -526,8 +543,9 @@ int attgnn_edge_softmax_bwd_sums(const int32_t *rowptr, const int32_t *col, int6
                                  int64_t heads, const float *s0, const float *s1, int64_t s_head_stride,
                                  int64_t s_row_stride, const float *lse, const void *dp, int dp_type,
                                  float slope, float drop_p, uint64_t seed, const int32_t *t_rowptr,
-                                 const int64_t *t_perm, float *ds0, float *ds1, float *workspace,
-                                 int64_t workspace_elems, attgnn_stream_t stream);
+                                 const int64_t *t_perm, const int32_t *long_rows, int64_t n_long,
+                                 const int32_t *t_long_rows, int64_t n_t_long, float *ds0, float *ds1,
+                                 float *workspace, int64_t workspace_elems, attgnn_stream_t stream);
 
 /*
  * out[h*n + j] = sum over CSR row j of vals[h*nnz + perm[e]] (perm[e] = -1 skips):
@@ -537,7 +555,8 @@ int attgnn_edge_softmax_bwd_sums(const int32_t *rowptr, const int32_t *col, int6
  *   (end2end_head/gat_final/mgat_csr/mgat_conv.py:42-48).
  */
 int attgnn_gather_rowsum(const int32_t *rowptr, const int64_t *perm, int64_t n, int64_t nnz,
-                         int64_t heads, const float *vals, float *out, attgnn_stream_t stream);
+                         int64_t heads, const float *vals, float *out, const int32_t *long_rows,
+                         int64_t n_long, attgnn_stream_t stream);
 
 /* ====================================================================== */
 /* Block-diagonal packing for batched inference (device)                   */

@@ -51,27 +51,70 @@ def _softmax_fp64(rp, col, s0, s1, slope):
     return p, lse
 
 
+@pytest.mark.parametrize("use_long", [False, True])
 @pytest.mark.parametrize("symmetric", [True, False])
-def test_heads_kernels_equal_per_head_kernels(cuda, symmetric):
+def test_heads_kernels_equal_per_head_kernels(cuda, symmetric, use_long):
     n = 700
     rp, col, rowptr, colt, t_rowptr, t_perm = _graph(cuda, n, 20, 3, seed=5, symmetric=symmetric)
+    lr = ops.long_rows(rowptr) if use_long else None
+    tlr = ops.long_rows(t_rowptr) if use_long else None
+    if use_long:
+        assert lr.numel() == 3     # the hub rows (350+ entries)
     nnz = col.size
     g = torch.Generator(device="cpu").manual_seed(0)
     s0 = torch.randn(3, n, generator=g).to(cuda)
     s1 = torch.randn(3, n, generator=g).to(cuda)
     dp = torch.randn(3, nnz, generator=g).to(cuda)
     for drop in (0.0, 0.3):
-        p3, lse3 = ops.edge_softmax_fwd(rowptr, colt, s0, s1, 0.2, drop, 9, p_dtype=torch.float32)
+        p3, lse3 = ops.edge_softmax_fwd(rowptr, colt, s0, s1, 0.2, drop, 9, p_dtype=torch.float32, long_rows=lr)
         p2, lse2 = ops.edge_softmax_fwd(rowptr, colt, s0[:2].contiguous(), s1[:2].contiguous(), 0.2, drop, 9,
-                                        p_dtype=torch.float32)
+                                        p_dtype=torch.float32, long_rows=lr)
         assert torch.equal(p3[:2], p2) and torch.equal(lse3[:2], lse2)
-        dz3, ds03 = ops.edge_softmax_bwd(rowptr, colt, s0, s1, lse3, dp, 0.2, drop, 9)
+        dz3, ds03 = ops.edge_softmax_bwd(rowptr, colt, s0, s1, lse3, dp, 0.2, drop, 9, long_rows=lr)
         dz2, ds02 = ops.edge_softmax_bwd(rowptr, colt, s0[:2].contiguous(), s1[:2].contiguous(), lse2,
-                                         dp[:2].contiguous(), 0.2, drop, 9)
+                                         dp[:2].contiguous(), 0.2, drop, 9, long_rows=lr)
         assert torch.equal(dz3[:2], dz2) and torch.equal(ds03[:2], ds02)
-        c3 = ops.gather_rowsum(t_rowptr, t_perm, dz3, n)
-        c2 = ops.gather_rowsum(t_rowptr, t_perm, dz2, n)
+        c3 = ops.gather_rowsum(t_rowptr, t_perm, dz3, n, long_rows=tlr)
+        c2 = ops.gather_rowsum(t_rowptr, t_perm, dz2, n, long_rows=tlr)
         assert torch.equal(c3[:2], c2)
+
+
+def test_long_rows_list_and_workgroup_path(cuda):
+    """ops.long_rows lists the rows above 256 entries; with the list, the short
+    rows' results are unchanged bit for bit, the long rows' agree to FP32
+    rounding, and extra short or repeated rows in the list change nothing."""
+    n = 2000
+    rp, col, rowptr, colt, t_rowptr, t_perm = _graph(cuda, n, 12, 6, seed=11, symmetric=True)
+    deg = np.diff(rp)
+    lr = ops.long_rows(rowptr)
+    assert lr.dtype == torch.int32 and np.array_equal(lr.cpu().numpy(), np.flatnonzero(deg > 256))
+    assert lr.numel() >= 6
+    heads, nnz = 4, col.size
+    g = torch.Generator(device="cpu").manual_seed(1)
+    s0 = torch.randn(n, heads, generator=g).to(cuda)
+    s1 = torch.randn(n, heads, generator=g).to(cuda)
+    dp = torch.randn(heads, nnz, generator=g).to(cuda)
+    tlr = ops.long_rows(t_rowptr)
+    padded = torch.cat([lr, lr[:2], torch.tensor([0, 1, 2], dtype=torch.int32, device=cuda)])
+    short = torch.from_numpy(deg <= 256).to(cuda)
+    short_e = torch.from_numpy(np.repeat(deg <= 256, deg)).to(cuda)
+    for drop in (0.0, 0.2):
+        p0, l0 = ops.edge_softmax_fwd(rowptr, colt, s0, s1, 0.2, drop, 4, s_layout="nh")
+        p1, l1 = ops.edge_softmax_fwd(rowptr, colt, s0, s1, 0.2, drop, 4, s_layout="nh", long_rows=lr)
+        p2, l2 = ops.edge_softmax_fwd(rowptr, colt, s0, s1, 0.2, drop, 4, s_layout="nh", long_rows=padded)
+        assert torch.equal(p1, p2) and torch.equal(l1, l2)
+        assert torch.equal(p0[:, short_e], p1[:, short_e]) and torch.equal(l0[:, short], l1[:, short])
+        assert torch.allclose(p0, p1, rtol=1e-5, atol=1e-7) and torch.allclose(l0, l1, rtol=1e-5, atol=1e-5)
+        d0, r0 = ops.edge_softmax_bwd(rowptr, colt, s0, s1, l1, dp, 0.2, drop, 4, s_layout="nh")
+        d1, r1 = ops.edge_softmax_bwd(rowptr, colt, s0, s1, l1, dp, 0.2, drop, 4, s_layout="nh", long_rows=lr)
+        assert torch.equal(d0[:, short_e], d1[:, short_e]) and torch.equal(r0[:, short], r1[:, short])
+        assert torch.allclose(d0, d1, rtol=1e-4, atol=1e-6) and torch.allclose(r0, r1, rtol=1e-4, atol=1e-5)
+        c1 = ops.gather_rowsum(t_rowptr, t_perm, d1, n, long_rows=tlr)
+        f0, f1 = ops.edge_softmax_bwd_sums(rowptr, colt, s0, s1, l1, dp, 0.2, drop, 4, t_rowptr, t_perm,
+                                           s_layout="nh", long_rows=lr, t_long_rows=tlr)
+        assert torch.equal(f0, r1) and torch.equal(f1, c1)
+        c0 = ops.gather_rowsum(t_rowptr, t_perm, d1, n)
+        assert torch.allclose(c0, c1, rtol=1e-4, atol=1e-5)
 
 
 @pytest.mark.parametrize("heads", [1, 2, 3, 4, 8])
@@ -79,6 +122,7 @@ def test_heads_kernels_equal_per_head_kernels(cuda, symmetric):
 def test_layouts_and_fused_sums(cuda, heads, dp_dtype):
     n = 900
     rp, col, rowptr, colt, t_rowptr, t_perm = _graph(cuda, n, 24, 2, seed=heads, symmetric=heads % 2 == 0)
+    lr, tlr = ops.long_rows(rowptr), ops.long_rows(t_rowptr)
     nnz = col.size
     g = torch.Generator(device="cpu").manual_seed(heads)
     s0 = torch.randn(heads, n, generator=g).to(cuda)
@@ -97,13 +141,25 @@ def test_layouts_and_fused_sums(cuda, heads, dp_dtype):
             f0, f1 = ops.edge_softmax_bwd_sums(rowptr, colt, a, b, lse, dp, 0.2, drop, 3, t_rowptr, t_perm,
                                                s_layout=layout)
             assert torch.equal(f0, ds0) and torch.equal(f1, ds1), layout
-    # FP64 on the host (no dropout)
-    p, lse = ops.edge_softmax_fwd(rowptr, colt, s0n, s1n, 0.2, 0.0, 0, p_dtype=torch.float32, s_layout="nh")
-    pr, lr = _softmax_fp64(rp, col, s0.double().cpu().numpy(), s1.double().cpu().numpy(), 0.2)
+        # with the long-row lists: head-major and node-major scores, split and fused backward, same bits
+        pl, lsel = ops.edge_softmax_fwd(rowptr, colt, s0, s1, 0.2, drop, 3, p_dtype=torch.bfloat16, long_rows=lr)
+        pln, lseln = ops.edge_softmax_fwd(rowptr, colt, s0n, s1n, 0.2, drop, 3, p_dtype=torch.bfloat16,
+                                          s_layout="nh", long_rows=lr)
+        assert torch.equal(pl, pln) and torch.equal(lsel, lseln)
+        dzl, ds0l = ops.edge_softmax_bwd(rowptr, colt, s0, s1, lsel, dp, 0.2, drop, 3, long_rows=lr)
+        ds1l = ops.gather_rowsum(t_rowptr, t_perm, dzl, n, long_rows=tlr)
+        for layout, a, b in (("hn", s0, s1), ("nh", s0n, s1n)):
+            f0, f1 = ops.edge_softmax_bwd_sums(rowptr, colt, a, b, lsel, dp, 0.2, drop, 3, t_rowptr, t_perm,
+                                               s_layout=layout, long_rows=lr, t_long_rows=tlr)
+            assert torch.equal(f0, ds0l) and torch.equal(f1, ds1l), layout
+    # FP64 on the host (no dropout), through the long-row path
+    p, lse = ops.edge_softmax_fwd(rowptr, colt, s0n, s1n, 0.2, 0.0, 0, p_dtype=torch.float32, s_layout="nh",
+                                  long_rows=lr)
+    pr, lse64 = _softmax_fp64(rp, col, s0.double().cpu().numpy(), s1.double().cpu().numpy(), 0.2)
     assert np.abs(p.cpu().numpy() - pr).max() < 1e-5
-    assert np.abs(lse.cpu().numpy() - lr).max() < 1e-4
+    assert np.abs(lse.cpu().numpy() - lse64).max() < 1e-4
     ds0, ds1 = ops.edge_softmax_bwd_sums(rowptr, colt, s0n, s1n, lse, dp, 0.2, 0.0, 0, t_rowptr, t_perm,
-                                         s_layout="nh")
+                                         s_layout="nh", long_rows=lr, t_long_rows=tlr)
     g64 = dp.double().cpu().numpy()
     rows = csr_rows(rp)
     z = s0.double().cpu().numpy()[:, rows] + s1.double().cpu().numpy()[:, col]
