@@ -80,11 +80,12 @@ def main():
                 layer = GATv2Conv_multi(128, dim, 0.0, 0.2, heads, dtype=torch.bfloat16,
                                         true_gatv2=(kind == "gatv2_true")).to(dev)
             go = torch.randn(g.n, heads, dim, device=dev)
+            go_b = go.bfloat16()            # the upstream gradient in the layer's output type
 
             def ours():
                 layer.zero_grad(set_to_none=True)
                 out = layer(Xb, graph)
-                out.backward(go.to(out.dtype))
+                out.backward(go_b)
 
             if kind == "gat":
                 W = layer.fc.weight.detach().clone().requires_grad_(True)
