@@ -255,6 +255,53 @@ struct KBlock {
         }
     }
 
+    // Compact tiles, prefetch form: the lane's 8 row masks of k-block kb and
+    // the block's value offset, loaded one k-block ahead so the value loads of
+    // a block depend only on registers.
+    __device__ __forceinline__ void masks(int kb, u32x4 &mv, int64_t &base) const
+    {
+        const int kg = kb + 8 * g;
+        mv = u32x4{0u, 0u, 0u, 0u};
+        base = 0;
+        if (kb < nreal) base = a.bbase[(int64_t(c_begin) + kb) >> 5];
+        if (kg < plen) mv = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(a.masks + int64_t(c_begin) + kg));
+    }
+
+    // tile_compact from prefetched masks (every lane of the wave must call it).
+    __device__ __forceinline__ void tile_compact_pf(int kb, const u32x4 &mv, int64_t base, uint32_t (&av)[AW]) const
+    {
+        const int kg = kb + 8 * g;
+#pragma unroll
+        for (int w = 0; w < AW; ++w) av[w] = 0;
+        uint32_t m[8];
+        int tot = 0, below = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            m[i] = (mv[i >> 1] >> (16 * (i & 1))) & 0xffffu;
+            tot += __builtin_popcount(m[i]);
+            below += __builtin_popcount(m[i] & ((1u << n) - 1u));
+        }
+        const int t0 = __shfl(tot, 0), t1 = __shfl(tot, 16), t2 = __shfl(tot, 32);
+        const int prefix = (g > 0 ? t0 : 0) + (g > 1 ? t1 : 0) + (g > 2 ? t2 : 0);
+        if (kg >= nreal) return;
+        int64_t idx = base + prefix + below;
+        uint32_t e[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            e[i] = 0;
+            if ((m[i] >> n) & 1u) {
+                load_bytes<EB>(a.val + idx * EB, &e[i]);
+                ++idx;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if constexpr (EB == 4) av[i] = e[i];
+            else if constexpr (EB == 2) av[i >> 1] |= e[i] << (16 * (i & 1));
+            else av[i >> 2] |= e[i] << (8 * (i & 3));
+        }
+    }
+
     __device__ __forceinline__ void load_a(int kb, uint32_t (&av)[AW]) const
     {
         if constexpr (AMODE == A_COMPACT) tile_compact(kb, av);
@@ -376,6 +423,29 @@ __global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
             k.rows(cid, xv);
             KB::mma(av, xv, acc);
         }
+    } else if constexpr (PIPE == 1 && AMODE == A_COMPACT) {
+        // column ids, row masks and value offset of the next k-block load
+        // while this one multiplies
+        int cid[8];
+        u32x4 mv;
+        int64_t mb;
+        k.cols(0, cid);
+        k.masks(0, mv, mb);
+        for (int kb = 0; kb < nreal; kb += 32) {
+            uint32_t av[AW], xv[8][XW];
+            int nid[8];
+            u32x4 nmv;
+            int64_t nmb;
+            k.tile_compact_pf(kb, mv, mb, av);
+            k.rows(cid, xv);
+            k.cols(kb + 32, nid);
+            k.masks(kb + 32, nmv, nmb);
+            KB::mma(av, xv, acc);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) cid[i] = nid[i];
+            mv = nmv;
+            mb = nmb;
+        }
     } else if constexpr (PIPE == 1) {
         int cid[8];
         k.cols(0, cid);
@@ -388,6 +458,35 @@ __global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
             KB::mma(av, xv, acc);
 #pragma unroll
             for (int i = 0; i < 8; ++i) cid[i] = nid[i];
+        }
+    } else if constexpr (AMODE == A_COMPACT) {
+        // two k-blocks per iteration; the next pair's column ids, row masks
+        // and value offsets load while this pair multiplies
+        int c0[8], c1[8];
+        u32x4 m0, m1;
+        int64_t b0, b1;
+        k.cols(0, c0);
+        k.cols(32, c1);
+        k.masks(0, m0, b0);
+        k.masks(32, m1, b1);
+        for (int kb = 0; kb < nreal; kb += 64) {
+            uint32_t av0[AW], av1[AW], xv0[8][XW], xv1[8][XW];
+            int n0[8], n1[8];
+            u32x4 nm0, nm1;
+            int64_t nb0, nb1;
+            k.tile_compact_pf(kb, m0, b0, av0);
+            k.tile_compact_pf(kb + 32, m1, b1, av1);
+            k.rows(c0, xv0);
+            k.rows(c1, xv1);
+            k.cols(kb + 64, n0);
+            k.cols(kb + 96, n1);
+            k.masks(kb + 64, nm0, nb0);
+            k.masks(kb + 96, nm1, nb1);
+            KB::mma(av0, xv0, acc);
+            KB::mma(av1, xv1, acc);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) { c0[i] = n0[i]; c1[i] = n1[i]; }
+            m0 = nm0; m1 = nm1; b0 = nb0; b1 = nb1;
         }
     } else {
         int c0[8], c1[8];
@@ -486,6 +585,22 @@ int kind_of(int mode, int tile_type, int x_type)
     if (mode == ATTGNN_MODE_BF16X3) return K_BF16X3;
     if (mode == ATTGNN_MODE_F32) return K_F32;
     return -1;
+}
+
+// k-loop schedule from the ATTGNN_SPMM_PIPE_* bits.  Default: two k-blocks per
+// iteration for 8/16-bit dense or compact tiles (measured on a 114.6M-entry
+// graph: width 64 dense 1.62 -> 1.56 ms, width 256 compact 4.91 -> 4.79 ms);
+// one block with prefetched column ids for fp32 data, whose operands take
+// twice the registers, and for edge-map tiles (4 heads x 64 on 2.48M entries:
+// 0.271 ms with one block, 0.288 ms with two).
+int pipe_of(int flags, int eb, bool mapped)
+{
+    switch ((flags >> 4) & 3) {
+    case 3: return 0;
+    case 2: return 2;
+    case 1: return 1;
+    default: return (eb <= 2 && !mapped) ? 2 : 1;
+    }
 }
 
 // Features per lane: the widest power of two dividing dim_n that keeps a lane
@@ -629,10 +744,7 @@ extern "C" int attgnn_spmm_block(const int32_t *rp, const int32_t *col, const vo
     a.val_head_stride = 0;
     a.slices = int((dim_n + 16 * t - 1) / (16 * t));
     a.heads = 1; a.window = window; a.wide_log2 = wl; a.y_type = y_type;
-    {
-        const int sel = (flags >> 4) & 3;  // ATTGNN_SPMM_PIPE_*
-        a.pipe = sel == 3 ? 0 : sel == 2 ? 2 : 1;
-    }
+    a.pipe = pipe_of(flags, eb, false);
     a.xcd_slices = !(flags & ATTGNN_SPMM_NO_XCD_SLICES) && a.slices > 1 && 8 % a.slices == 0;
     const hipError_t e = run_spmm(kind, A_DENSE, t, a, (flags & ATTGNN_SPMM_ROWS_COVERED) != 0, s);
     return e == hipSuccess ? ATTGNN_OK : int(e);
@@ -676,7 +788,7 @@ extern "C" int attgnn_spmm_edge_map(const int32_t *rp, const int32_t *col, const
     if (out_layout == ATTGNN_LAYOUT_HMD) { a.y_head_stride = m_ori * dim_n; a.y_row_stride = dim_n; }
     else { a.y_head_stride = dim_n; a.y_row_stride = heads * dim_n; }
     a.slices = int((dim_n + 16 * t - 1) / (16 * t));
-    a.heads = int(heads); a.window = window; a.wide_log2 = wl; a.y_type = y_type; a.pipe = 1;
+    a.heads = int(heads); a.window = window; a.wide_log2 = wl; a.y_type = y_type; a.pipe = pipe_of(flags, eb, true);
     a.xcd_slices = a.slices > 1 && 8 % a.slices == 0;
     const hipError_t e = run_spmm(kind, A_MAPPED, t, a, (flags & ATTGNN_SPMM_ROWS_COVERED) != 0, s);
     return e == hipSuccess ? ATTGNN_OK : int(e);
@@ -721,10 +833,7 @@ extern "C" int attgnn_spmm_compact(const int32_t *rp, const int32_t *col, const 
     a.val_head_stride = 0;
     a.slices = int((dim_n + 16 * t - 1) / (16 * t));
     a.heads = 1; a.window = window; a.wide_log2 = 5; a.y_type = y_type;
-    {
-        const int sel = (flags >> 4) & 3;
-        a.pipe = sel == 3 ? 0 : sel == 2 ? 2 : 1;
-    }
+    a.pipe = pipe_of(flags, eb, false);
     a.xcd_slices = !(flags & ATTGNN_SPMM_NO_XCD_SLICES) && a.slices > 1 && 8 % a.slices == 0;
     const hipError_t e = run_spmm(kind, A_COMPACT, t, a, (flags & ATTGNN_SPMM_ROWS_COVERED) != 0, s);
     return e == hipSuccess ? ATTGNN_OK : int(e);

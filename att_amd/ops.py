@@ -267,9 +267,13 @@ def map_remap(emap, remap):
 # SpMM
 # --------------------------------------------------------------------------
 
+# k-loop schedules (ATTGNN_SPMM_PIPE_*); results are bitwise identical across them
+_PIPES = {None: 0, "default": 0, "one": 1 << 4, "pair": 2 << 4, "none": 3 << 4}
+
+
 def spmm_block(rp, col, val, window_of_part, atomic, X, m_ori, window, wide, mode=None,
                row_scale=None, col_scale=None, out=None, out_dtype=torch.float32, rows_covered=False,
-               workspace=None):
+               workspace=None, pipe=None):
     """Y[m_ori, D] = A . X over a window-block structure (all on the GPU).
 
     FP32 accumulation; Y is written in ``out_dtype`` (fp32 / fp16 / bf16).
@@ -292,7 +296,7 @@ def spmm_block(rp, col, val, window_of_part, atomic, X, m_ori, window, wide, mod
         ws = workspace
     else:
         ws = torch.empty(max(ws_n, 1), dtype=torch.float32, device=dev)
-    flags = _lib.SPMM_ROWS_COVERED if rows_covered else 0
+    flags = (_lib.SPMM_ROWS_COVERED if rows_covered else 0) | _PIPES[pipe]
     rc = lib().attgnn_spmm_block(_ptr(rp), _ptr(col), _ptr(val), _TYPE_OF[val.dtype], _ptr(window_of_part),
                                  _ptr(atomic), parts, _ptr(X), _TYPE_OF[X.dtype], n_x, _ptr(y), _TYPE_OF[y.dtype],
                                  m_ori, dim, window, wide, _mode(mode, X.dtype), _ptr(row_scale), _ptr(col_scale), _ptr(ws),
@@ -302,7 +306,8 @@ def spmm_block(rp, col, val, window_of_part, atomic, X, m_ori, window, wide, mod
 
 
 def spmm_compact(rp, col, masks, block_base, values, window_of_part, atomic, X, m_ori, window, mode=None,
-                 row_scale=None, col_scale=None, out_dtype=torch.float32, rows_covered=False, workspace=None):
+                 row_scale=None, col_scale=None, out_dtype=torch.float32, rows_covered=False, workspace=None,
+                 pipe=None):
     """attgnn_spmm_compact: spmm_block over compact tiles (wide = 32)."""
     dev = _need_gpu("spmm_compact", rp, col, masks, block_base, values, window_of_part, atomic, X, row_scale,
                     col_scale)
@@ -313,7 +318,7 @@ def spmm_compact(rp, col, masks, block_base, values, window_of_part, atomic, X, 
     ws_n = lib().attgnn_spmm_workspace_elems(parts, window, dim) if parts else 0
     ws = workspace if workspace is not None and workspace.numel() >= ws_n else \
         torch.empty(max(ws_n, 1), dtype=torch.float32, device=dev)
-    flags = _lib.SPMM_ROWS_COVERED if rows_covered else 0
+    flags = (_lib.SPMM_ROWS_COVERED if rows_covered else 0) | _PIPES[pipe]
     rc = lib().attgnn_spmm_compact(_ptr(rp), _ptr(col), _ptr(masks), _ptr(block_base), _ptr(values),
                                    _TYPE_OF[values.dtype], _ptr(window_of_part), _ptr(atomic), parts, _ptr(X),
                                    _TYPE_OF[X.dtype], n_x, _ptr(y), _TYPE_OF[y.dtype], m_ori, dim, window,
@@ -324,7 +329,7 @@ def spmm_compact(rp, col, masks, block_base, values, window_of_part, atomic, X, 
 
 
 def spmm_edge_map(rp, col, emap, edge_val, window_of_part, atomic, X, out_layout, window, wide,
-                  m_ori=None, mode=None, x_layout="hmd", out_dtype=torch.float32, rows_covered=False):
+                  m_ori=None, mode=None, x_layout="hmd", out_dtype=torch.float32, rows_covered=False, pipe=None):
     """Edge-map SpMM.  X [H, N, D] ("hmd") or [N, H, D] ("mhd"); edge_val [H, nnz]
     (or [nnz]); Y [H, M, D] ("hmd") or [M, H, D] ("mhd") in ``out_dtype`` (FP32
     accumulation).  ``rows_covered``: every window up to M has a part, so Y
@@ -358,7 +363,7 @@ def spmm_edge_map(rp, col, emap, edge_val, window_of_part, atomic, X, out_layout
                                     _ptr(window_of_part), _ptr(atomic), parts, _ptr(X), _TYPE_OF[X.dtype], xl,
                                     heads, n_x, _ptr(y), _TYPE_OF[out_dtype], m, dim, layout, window, wide,
                                     _mode(mode, X.dtype), _ptr(ws), ws_n,
-                                    _lib.SPMM_ROWS_COVERED if rows_covered else 0, _stream(dev))
+                                    (_lib.SPMM_ROWS_COVERED if rows_covered else 0) | _PIPES[pipe], _stream(dev))
     check(rc, "attgnn_spmm_edge_map")
     return y
 

@@ -249,9 +249,12 @@ enum {
                                      part (built with emit_empty=1 and m_ori <=
                                      windows*window): Y is not pre-zeroed       */
     /* k-loop schedule (bits 4-5); results are bitwise identical across them */
-    ATTGNN_SPMM_PIPE_DEFAULT = 0 << 4, /* column ids of the next block prefetched */
-    ATTGNN_SPMM_PIPE_NONE = 3 << 4,    /* no prefetch                            */
+    ATTGNN_SPMM_PIPE_DEFAULT = 0 << 4, /* PAIR for 8/16-bit dense or compact
+                                          tiles, else ONE                        */
+    ATTGNN_SPMM_PIPE_ONE = 1 << 4,     /* next block's column ids (and compact
+                                          masks) prefetched                      */
     ATTGNN_SPMM_PIPE_PAIR = 2 << 4,    /* two blocks per iteration + prefetch    */
+    ATTGNN_SPMM_PIPE_NONE = 3 << 4,    /* no prefetch                            */
     ATTGNN_SPMM_NO_XCD_SLICES = 1 << 6, /* by default, when the feature-slice count
                                           divides 8, every XCD works on a single
                                           slice so its L2 caches one slice of the

@@ -123,3 +123,38 @@ def test_empty_and_degenerate(cuda):
     y = ops.spmm_block(*[b[k].to(cuda) for k in ("rp", "col", "val", "window", "atomic")], X, 32, 8, 4,
                        mode="f32")
     assert torch.count_nonzero(y) == 0
+
+
+@pytest.mark.parametrize("window,wide,ps", [(16, 32, 1), (16, 32, 3), (16, 32, 1000), (8, 8, 2), (16, 8, 5)])
+@pytest.mark.parametrize("dim", [8, 48, 64, 256, 66])
+def test_pipeline_schedules_bitwise_equal(cuda, window, wide, ps, dim):
+    """The k-loop schedules (none / one block / two blocks per iteration, with
+    their column-id and compact-mask prefetches) only reorder loads: dense,
+    compact and edge-map results are bit for bit the same, for every
+    arithmetic kind, odd block counts and split windows included."""
+    rng = np.random.default_rng(window * 100 + wide + ps * 7 + dim)
+    n = 700
+    rp, col = rand_csr(rng, n, 60, hub_rows=3)
+    vals = rng.standard_normal(col.size).astype(np.float32)
+    for dt, mode in ((torch.float16, None), (torch.bfloat16, None), (torch.float32, "f32"),
+                     (torch.float32, "bf16x3")):
+        b = ops.blocks_build(rp, col, "edge_f32", vals, dt, window, wide, ps, emit_empty=True,
+                             compact=(wide == 32))
+        args = [b[k].to(cuda) for k in ("rp", "col", "val", "window", "atomic")]
+        X = torch.from_numpy(rng.uniform(-1, 1, (n, dim)).astype(np.float32)).to(dt).to(cuda)
+        ys = [ops.spmm_block(*args, X, n, window, wide, mode=mode, pipe=p) for p in ("none", "one", "pair")]
+        assert all(torch.equal(ys[0], y) for y in ys[1:]), (dt, mode)
+        if wide == 32:
+            cargs = [b[k].to(cuda) for k in ("masks", "block_base", "values")]
+            cs = [ops.spmm_compact(args[0], args[1], *cargs, args[3], args[4], X, n, window, mode=mode, pipe=p)
+                  for p in ("none", "one", "pair")]
+            assert all(torch.equal(ys[0], c) for c in cs), (dt, mode, "compact")
+    # edge-map SpMM, 4 heads
+    m = ops.blocks_build(rp, col, "map", None, torch.int32, window, wide, ps, emit_empty=True)
+    margs = [m[k].to(cuda) for k in ("rp", "col", "val", "window", "atomic")]
+    ev = torch.rand(4, col.size, device=cuda).bfloat16()
+    Xh = torch.randn(n, 4, dim, device=cuda).bfloat16()
+    # 700 rows leave a partial last window, so Y is zero-filled (rows_covered=False)
+    es = [ops.spmm_edge_map(margs[0], margs[1], margs[2], ev, margs[3], margs[4], Xh, "mhd", window, wide,
+                            x_layout="mhd", pipe=p) for p in ("none", "one", "pair")]
+    assert all(torch.equal(es[0], e) for e in es[1:])
