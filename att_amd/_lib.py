@@ -68,7 +68,8 @@ _SIGNATURES = {
     "attgnn_blocks_plan_device": (_int, [_vp, _i64, _vp, _i64, _int, _int, _int, _int, _vp, _i64, _vp, _vp]),
     "attgnn_blocks_fill_device": (_int, [_vp, _i64, _vp, _i64, _int, _vp, _int, _int, _int, _int, _vp, _i64, _vp,
                                          _vp, _vp, _vp, _vp, _vp, _vp]),
-    "attgnn_blocks_compact_count_device": (_int, [_vp, _int, _i64, _int, _vp, _vp, _vp]),
+    "attgnn_blocks_compact_workspace_bytes": (_i64, [_i64]),
+    "attgnn_blocks_compact_count_device": (_int, [_vp, _int, _i64, _int, _vp, _vp, _vp, _i64, _vp]),
     "attgnn_blocks_compact_fill_device": (_int, [_vp, _int, _i64, _int, _vp, _vp, _vp, _vp, _vp]),
     "attgnn_map_transpose_perm_device": (_int, [_vp, _vp, _i64, _i64, _vp, _vp]),
     "attgnn_attention_node_bwd_workspace_elems": (_i64, [_i64, _i64, _i64]),

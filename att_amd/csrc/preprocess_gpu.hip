@@ -28,6 +28,8 @@
 
 #include <climits>
 
+#include <rocprim/device/device_scan.hpp>
+
 #include "attgnn.h"
 #include "halfconv.hpp"
 
@@ -515,12 +517,14 @@ __device__ __forceinline__ bool tile_nz(const void *val, int64_t i, int es, bool
     return static_cast<const uint16_t *>(val)[i] != 0;
 }
 
-// One wave per 32-column block: row masks, entry count into block_base[blk + 1].
+// One wave per 32-column block: row masks, entry count into counts[blk];
+// counts[nb] = 0 so an exclusive scan of counts[0..nb] ends with the total.
 __global__ __launch_bounds__(THREADS) void compact_count_kernel(const void *val, int es, int is_map, int64_t nb,
-                                                                int W, uint16_t *masks, int64_t *block_base)
+                                                                int W, uint16_t *masks, int64_t *counts)
 {
     const int lane = threadIdx.x & 63;
     const int64_t nw = int64_t(gridDim.x) * (THREADS / 64);
+    if (blockIdx.x == 0 && threadIdx.x == 0) counts[nb] = 0;
     for (int64_t blk = int64_t(blockIdx.x) * (THREADS / 64) + (threadIdx.x >> 6); blk < nb; blk += nw) {
         uint32_t m = 0;
         if (lane < 32)
@@ -530,37 +534,26 @@ __global__ __launch_bounds__(THREADS) void compact_count_kernel(const void *val,
         int cnt = __popc(m);
 #pragma unroll
         for (int d = 32; d > 0; d >>= 1) cnt += __shfl_xor(cnt, d, 64);
-        if (lane == 0) block_base[blk + 1] = cnt;
+        if (lane == 0) counts[blk] = cnt;
     }
 }
 
-// In-place inclusive scan of a[1..n] with a[0] = 0 (one workgroup).
-__global__ __launch_bounds__(THREADS) void scan_inplace_kernel(int64_t *a, int64_t n)
+__global__ void zero_first_kernel(int64_t *a)
 {
-    __shared__ int64_t red[8];
-    const int t = threadIdx.x;
-    constexpr int ITEMS = 8;
-    int64_t carry = 0;
-    if (t == 0) a[0] = 0;
-    for (int64_t base = 1; base <= n; base += int64_t(THREADS) * ITEMS) {
-        int64_t x[ITEMS], s = 0;
-#pragma unroll
-        for (int k = 0; k < ITEMS; ++k) {
-            const int64_t i = base + int64_t(t) * ITEMS + k;
-            x[k] = i <= n ? a[i] : 0;
-            s += x[k];
-        }
-        int64_t tot;
-        int64_t o = block_excl_scan(s, red, tot) + carry;
-#pragma unroll
-        for (int k = 0; k < ITEMS; ++k) {
-            const int64_t i = base + int64_t(t) * ITEMS + k;
-            o += x[k];
-            if (i <= n) a[i] = o;
-        }
-        carry += tot;
-    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) a[0] = 0;
 }
+
+// Workspace of the compact count: the block counts, then rocPRIM's scan storage.
+size_t compact_scan_temp_bytes(int64_t nb)
+{
+    size_t bytes = 0;
+    const hipError_t e = rocprim::exclusive_scan(nullptr, bytes, static_cast<const int64_t *>(nullptr),
+                                                 static_cast<int64_t *>(nullptr), int64_t(0), size_t(nb + 1),
+                                                 rocprim::plus<int64_t>());
+    return e == hipSuccess ? bytes : size_t(1) << 40;
+}
+
+int64_t compact_ws_bytes(int64_t nb) { return (8 * (nb + 1) + 255) / 256 * 256 + int64_t(compact_scan_temp_bytes(nb)) + 256; }
 
 // One wave per block; lane (g, r) = (lane / 16, lane % 16) packs the entries of
 // window row r in the 8-column group g, in column order.
@@ -664,19 +657,36 @@ extern "C" int attgnn_blocks_fill_device(const int32_t *row, int64_t rows_plus1,
     return launch_status();
 }
 
+extern "C" int64_t attgnn_blocks_compact_workspace_bytes(int64_t n_cols)
+{
+    if (n_cols < 0 || n_cols % 32) return -1;
+    return compact_ws_bytes(n_cols / 32);
+}
+
 extern "C" int attgnn_blocks_compact_count_device(const void *val, int tile_type, int64_t n_cols, int window,
-                                                  uint16_t *masks, int64_t *block_base, attgnn_stream_t stream)
+                                                  uint16_t *masks, int64_t *block_base, void *workspace,
+                                                  int64_t workspace_bytes, attgnn_stream_t stream)
 {
     if (n_cols < 0 || n_cols % 32 || window <= 0 || window > 16 || !block_base) return ATTGNN_E_ARG;
     const int es = type_size(tile_type);
     if (!es) return ATTGNN_E_ARG;
     const int64_t nb = n_cols / 32;
-    if (nb > 0 && (!val || !masks)) return ATTGNN_E_ARG;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    if (nb > 0)
-        hipLaunchKernelGGL(compact_count_kernel, dim3(grid_for(nb, THREADS / 64)), dim3(THREADS), 0, s, val, es,
-                           int(tile_type == ATTGNN_I32), nb, window, masks, block_base);
-    hipLaunchKernelGGL(scan_inplace_kernel, dim3(1), dim3(THREADS), 0, s, block_base, nb);
+    if (nb == 0) {
+        hipLaunchKernelGGL(zero_first_kernel, dim3(1), dim3(64), 0, s, block_base);
+        return launch_status();
+    }
+    if (!val || !masks || !workspace || workspace_bytes < compact_ws_bytes(nb)) return ATTGNN_E_ARG;
+    // block counts into the workspace, then block_base = their exclusive scan
+    // over nb + 1 entries (rocPRIM, device-wide), so block_base[nb] = n_values
+    int64_t *counts = static_cast<int64_t *>(workspace);
+    uint8_t *temp = static_cast<uint8_t *>(workspace) + (8 * (nb + 1) + 255) / 256 * 256;
+    size_t tb = compact_scan_temp_bytes(nb);
+    hipLaunchKernelGGL(compact_count_kernel, dim3(grid_for(nb, THREADS / 64)), dim3(THREADS), 0, s, val, es,
+                       int(tile_type == ATTGNN_I32), nb, window, masks, counts);
+    const hipError_t e = rocprim::exclusive_scan(temp, tb, counts, block_base, int64_t(0), size_t(nb + 1),
+                                                 rocprim::plus<int64_t>(), s);
+    if (e != hipSuccess) return int(e);
     return launch_status();
 }
 

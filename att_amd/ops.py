@@ -201,8 +201,12 @@ def blocks_build_device(row, col, src, src_values, out_dtype, window, wide, part
         nb = n_col // 32
         masks = torch.empty(max(n_col, 8), dtype=torch.int16, device=dev)
         base = torch.empty(nb + 1, dtype=torch.int64, device=dev)
+        cws_n = L.attgnn_blocks_compact_workspace_bytes(n_col)
+        if cws_n > ws_n:
+            ws = torch.empty(cws_n, dtype=torch.uint8, device=dev)
+            ws_n = cws_n
         check(L.attgnn_blocks_compact_count_device(_ptr(out["val"]), _TYPE_OF[out_dtype], n_col, window,
-                                                   _ptr(masks), _ptr(base), st),
+                                                   _ptr(masks), _ptr(base), _ptr(ws), ws_n, st),
               "attgnn_blocks_compact_count_device")
         nv = int(base[nb].item())
         values = torch.empty(max(nv, 1), dtype=out_dtype, device=dev)

@@ -204,12 +204,16 @@ int attgnn_map_remap_device(const int32_t *map, int64_t n, const int64_t *remap,
  * = n_values, which the caller reads to size values / positions); fill packs
  * values[n_values] and (if not NULL) positions[n_values].  val holds the
  * n_cols * window tiles of a structure built with wide = 32 (tile_type
- * F16 / BF16 / F32, or I32 for an edge-id map).
+ * F16 / BF16 / F32, or I32 for an edge-id map).  count takes a device
+ * workspace of attgnn_blocks_compact_workspace_bytes(n_cols) bytes (the block
+ * counts and the device-wide scan's storage).
  *
  * No counterpart in the reference (see attgnn_blocks_compact_sizes).
  */
+int64_t attgnn_blocks_compact_workspace_bytes(int64_t n_cols);
 int attgnn_blocks_compact_count_device(const void *val, int tile_type, int64_t n_cols, int window,
-                                       uint16_t *masks, int64_t *block_base, attgnn_stream_t stream);
+                                       uint16_t *masks, int64_t *block_base, void *workspace,
+                                       int64_t workspace_bytes, attgnn_stream_t stream);
 int attgnn_blocks_compact_fill_device(const void *val, int tile_type, int64_t n_cols, int window,
                                       const uint16_t *masks, const int64_t *block_base, void *values,
                                       uint16_t *positions, attgnn_stream_t stream);
