@@ -261,19 +261,29 @@ extern "C" int attgnn_blocks_sizes(const attgnn_blocks *b, int64_t *n_rp, int64_
     return ATTGNN_OK;
 }
 
+namespace {
+
+// Copy a result vector out; an empty vector copies nothing (its data() may be
+// NULL, which memcpy may not be given even for zero bytes).
+template <typename T>
+void copy_out(void *dst, const std::vector<T> &v)
+{
+    if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(T));
+}
+
+}  // namespace
+
 extern "C" int attgnn_blocks_export(const attgnn_blocks *b, int32_t *rp, int32_t *col, void *val,
                                     int32_t *map_t, int32_t *window_of_part, int32_t *atomic)
 {
     if (!b) return ATTGNN_E_ARG;
-    if (rp) std::memcpy(rp, b->rp.data(), b->rp.size() * 4);
-    if (col) std::memcpy(col, b->col.data(), b->col.size() * 4);
-    if (val) std::memcpy(val, b->val.data(), b->val.size());
-    if (map_t) {
-        if (!b->has_map_t) return ATTGNN_E_ARG;
-        std::memcpy(map_t, b->map_t.data(), b->map_t.size() * 4);
-    }
-    if (window_of_part) std::memcpy(window_of_part, b->win.data(), b->win.size() * 4);
-    if (atomic) std::memcpy(atomic, b->atomic.data(), b->atomic.size() * 4);
+    if (map_t && !b->has_map_t) return ATTGNN_E_ARG;
+    copy_out(rp, b->rp);
+    copy_out(col, b->col);
+    copy_out(val, b->val);
+    copy_out(map_t, b->map_t);
+    copy_out(window_of_part, b->win);
+    copy_out(atomic, b->atomic);
     return ATTGNN_OK;
 }
 
