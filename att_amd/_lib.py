@@ -28,6 +28,17 @@ class Segments(ctypes.Structure):
 
 _SEGP = _c.POINTER(Segments)
 
+
+class AttnTerms(ctypes.Structure):
+    """attgnn_attn_terms (include/attgnn.h)."""
+    _fields_ = [("row_logit", _c.c_void_p), ("col_logit", _c.c_void_p), ("row_offset", _c.c_void_p),
+                ("col_offset", _c.c_void_p), ("logit_head_stride", _c.c_int64), ("logit_node_stride", _c.c_int64),
+                ("offset_head_stride", _c.c_int64), ("offset_node_stride", _c.c_int64), ("n_nodes", _c.c_int64),
+                ("slope", _c.c_float), ("drop_p", _c.c_float), ("seed", _c.c_uint64)]
+
+
+_ATTNP = _c.POINTER(AttnTerms)
+
 _SIGNATURES = {
     "attgnn_status_string": (_c.c_char_p, [_int]),
     "attgnn_blocks_build": (_int, [_vp, _i64, _vp, _i64, _int, _vp, _int, _int, _int, _int, _int, _int, _int,
@@ -47,6 +58,8 @@ _SIGNATURES = {
                                   _i64, _i64, _vp, _int, _i64, _int, _int, _vp]),
     "attgnn_spmm_edge_map": (_int, [_vp, _vp, _vp, _vp, _int, _i64, _vp, _vp, _i64, _vp, _int, _int, _i64, _i64,
                                     _vp, _int, _i64, _i64, _int, _int, _int, _int, _vp, _i64, _int, _vp]),
+    "attgnn_spmm_attention": (_int, [_vp, _vp, _vp, _vp, _vp, _i64, _ATTNP, _vp, _int, _int, _i64, _i64, _vp, _int,
+                                     _i64, _i64, _int, _int, _int, _int, _vp, _i64, _int, _vp]),
     "attgnn_rowsum_edge_map": (_int, [_vp, _vp, _vp, _i64, _vp, _int, _i64, _i64, _vp, _int, _i64, _int, _int,
                                       _vp]),
     "attgnn_sddmm_block": (_int, [_vp, _vp, _vp, _vp, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _int, _i64, _i64,

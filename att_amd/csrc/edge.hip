@@ -18,24 +18,12 @@
 #include <cstdint>
 
 #include "attgnn.h"
+#include "attn.hpp"
 #include "mfma.hpp"
 
 using namespace attgnn;
 
 namespace {
-
-__device__ __forceinline__ float lrelu(float z, float slope) { return z > 0.f ? z : z * slope; }
-
-// Counter-based keep decision for dropout: deterministic in (seed, head, edge),
-// so the backward pass regenerates the forward mask without storing it.
-__device__ __forceinline__ bool keep_edge(uint64_t seed, int64_t h, int64_t e, uint32_t thresh)
-{
-    uint64_t x = seed ^ (uint64_t(h) * 0x9E3779B97F4A7C15ull) ^ (uint64_t(e) * 0xC2B2AE3D27D4EB4Full);
-    x ^= x >> 31; x *= 0x7FB5D329728EA185ull;
-    x ^= x >> 27; x *= 0x81DADEF4BC2DD44Dull;
-    x ^= x >> 33;
-    return uint32_t(x) >= thresh;
-}
 
 __device__ __forceinline__ float ld(const uint8_t *p, int type, int64_t i)
 {
@@ -110,10 +98,10 @@ __global__ __launch_bounds__(256) void edge_softmax_fwd_kernel(SoftmaxArgs a)
     const float S = group_sum(s);
     const float lse = M + __logf(S);
     if (l == 0) a.lse_out[h * a.n + i] = (e1 > e0) ? lse : 0.f;
+    if (!a.p) return;                          // log-sum-exp only
     for (int64_t e = e0 + l; e < e1; e += 16) {
-        const float v = lrelu(si + a.s1[h * a.s_hs + int64_t(a.col[e]) * a.s_rs], a.slope);
-        float p = __expf(v - lse);
-        if (a.thresh) p = keep_edge(a.seed, h, e, a.thresh) ? p * a.keep_scale : 0.f;
+        const float p = attn_prob(si + a.s1[h * a.s_hs + int64_t(a.col[e]) * a.s_rs], lse, a.slope, a.thresh,
+                                  a.keep_scale, a.seed, h, e);
         st(a.p, a.p_type, int64_t(h) * a.nnz + e, p);
     }
 }
@@ -253,10 +241,10 @@ __device__ __forceinline__ void softmax_fwd_long(const SoftmaxArgs &a, int64_t b
     const float S = block_reduce<false>((m == -INFINITY) ? 0.f : s * __expf(m - M), red);
     const float lse = M + __logf(S);
     if (t == 0) a.lse_out[h * a.n + i] = lse;
+    if (!a.p) return;
     for (int64_t e = e0 + t; e < e1; e += LONG_THREADS) {
-        const float v = lrelu(si + a.s1[h * a.s_hs + int64_t(a.col[e]) * a.s_rs], a.slope);
-        float p = __expf(v - lse);
-        if (a.thresh) p = keep_edge(a.seed, h, e, a.thresh) ? p * a.keep_scale : 0.f;
+        const float p = attn_prob(si + a.s1[h * a.s_hs + int64_t(a.col[e]) * a.s_rs], lse, a.slope, a.thresh,
+                                  a.keep_scale, a.seed, h, e);
         st(a.p, a.p_type, int64_t(h) * a.nnz + e, p);
     }
 }
@@ -392,6 +380,7 @@ __global__ __launch_bounds__(256) void softmax_fwd_heads_kernel(SoftmaxArgs a)
         lse[h] = M + __logf(group_sum(sh));
         if (l == 0) a.lse_out[h * a.n + i] = (e1 > e0) ? lse[h] : 0.f;
     }
+    if (!a.p) return;                          // log-sum-exp only
     for (int64_t eb = e0 + l; eb < e1; eb += 16 * U) {
         if (!one_step) {
 #pragma unroll
@@ -406,8 +395,7 @@ __global__ __launch_bounds__(256) void softmax_fwd_heads_kernel(SoftmaxArgs a)
             if (e >= e1) break;
 #pragma unroll
             for (int h = 0; h < H; ++h) {
-                float p = __expf(lrelu(si[h] + sj[u][h], a.slope) - lse[h]);
-                if (a.thresh) p = keep_edge(a.seed, h, e, a.thresh) ? p * a.keep_scale : 0.f;
+                const float p = attn_prob(si[h] + sj[u][h], lse[h], a.slope, a.thresh, a.keep_scale, a.seed, h, e);
                 st(a.p, a.p_type, int64_t(h) * a.nnz + e, p);
             }
         }
@@ -678,7 +666,7 @@ extern "C" int attgnn_edge_softmax_fwd(const int32_t *rowptr, const int32_t *col
         return ATTGNN_E_ARG;
     if (!long_args_ok(long_rows, n_long)) return ATTGNN_E_ARG;
     if (n == 0) return ATTGNN_OK;
-    if (!rowptr || !s0 || !s1 || !lse || (nnz > 0 && (!col || !p))) return ATTGNN_E_ARG;
+    if (!rowptr || !s0 || !s1 || !lse || (nnz > 0 && !col)) return ATTGNN_E_ARG;
     SoftmaxArgs a{};
     a.rowptr = rowptr; a.col = col; a.s0 = s0; a.s1 = s1; a.p = static_cast<uint8_t *>(p); a.lse_out = lse;
     a.n = n; a.nnz = nnz; a.s_hs = s_head_stride; a.s_rs = s_row_stride; a.heads = int(heads); a.p_type = p_type;

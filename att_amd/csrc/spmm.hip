@@ -22,6 +22,7 @@
 #include <algorithm>
 
 #include "attgnn.h"
+#include "attn.hpp"
 #include "mfma.hpp"
 
 using namespace attgnn;
@@ -74,6 +75,13 @@ struct SpmmArgs {
     int64_t x_head_stride, x_row_stride, y_head_stride, y_row_stride, val_head_stride;
     int slices, heads, window, wide_log2, y_type, pipe;
     int xcd_slices;         // 1: slice-per-XCD task mapping (slices divides 8)
+    // attention variant: tile value = exp(LeakyReLU(r1[row] + c1[col]) - r2[row] - c2[col])
+    // with dropout on the edge id; element (h, v) at [h*hs + v*ns]
+    const float *r1, *r2, *c1, *c2;   // r2 / c2 may be null (0)
+    int64_t lg_hs, lg_ns, of_hs, of_ns, n_nodes;
+    float slope, keep_scale;
+    uint32_t thresh;
+    uint64_t seed;
 };
 
 // Store n consecutive fp32 values as y_type (F32 / F16 / BF16).
@@ -132,7 +140,7 @@ __device__ __forceinline__ void load_bytes_nt(const uint8_t *p, uint32_t *d)
 
 // Per-wave state and the three steps of one k-block (32 condensed columns):
 // column ids of the lane's 8 k slots, the A operand (tile row n), the X rows.
-enum AMode { A_DENSE = 0, A_MAPPED = 1, A_COMPACT = 2 };
+enum AMode { A_DENSE = 0, A_MAPPED = 1, A_COMPACT = 2, A_ATTN = 3 };
 
 template <int KIND, int T, int AMODE>
 struct KBlock {
@@ -146,6 +154,7 @@ struct KBlock {
     int c_begin, nreal, plen, g, n, W, wide, head;
     int64_t f0;
     bool f_ok, a_row_ok;
+    float r1v, r2v;   // attention variant: the lane's window-row terms
 
     __device__ __forceinline__ void cols(int kb, int (&cid)[8]) const
     {
@@ -302,9 +311,66 @@ struct KBlock {
         }
     }
 
-    __device__ __forceinline__ void load_a(int kb, uint32_t (&av)[AW]) const
+    // Attention variant: the A operand is rebuilt from node terms where it is
+    // used -- exp(LeakyReLU(r1[row] + c1[col]) - r2[row] - c2[col]), dropout
+    // keyed on the edge id from the map -- with the edge softmax's own
+    // expression and rounding, so it equals the stored probabilities bit for bit.
+    __device__ __forceinline__ void tile_attn(int kb, const int (&cid)[8], uint32_t (&av)[AW]) const
+    {
+        const int kg = kb + 8 * g;
+#pragma unroll
+        for (int w = 0; w < AW; ++w) av[w] = 0;
+        if (!a_row_ok || kg >= nreal) return;
+        int m[8];
+        if (wide >= 8) {
+            const int64_t cc = int64_t(c_begin) + kg;
+            const int32_t *mp = a.map + (cc >> a.wide_log2) * W * wide + int64_t(n) * wide + (cc & (wide - 1));
+            const u32x4 m0 = *reinterpret_cast<const u32x4 *>(mp);
+            const u32x4 m1 = *reinterpret_cast<const u32x4 *>(mp + 4);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) { m[i] = int(m0[i]); m[i + 4] = int(m1[i]); }
+        } else {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int64_t cc = int64_t(c_begin) + kg + 4 * h;
+                u32x4 mv = {~0u, ~0u, ~0u, ~0u};
+                if (kg + 4 * h < plen)
+                    mv = *reinterpret_cast<const u32x4 *>(a.map + (cc >> a.wide_log2) * W * wide + int64_t(n) * wide);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) m[4 * h + i] = int(mv[i]);
+            }
+        }
+        float c1v[8], c2v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const bool ok = m[i] >= 0 && cid[i] >= 0 && cid[i] < a.n_nodes;
+            c1v[i] = ok ? a.c1[int64_t(head) * a.lg_hs + int64_t(cid[i]) * a.lg_ns] : 0.f;
+            c2v[i] = (ok && a.c2) ? a.c2[int64_t(head) * a.of_hs + int64_t(cid[i]) * a.of_ns] : 0.f;
+        }
+        uint32_t e[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            e[i] = 0;
+            if (m[i] >= 0 && cid[i] >= 0 && cid[i] < a.n_nodes) {
+                const float p = attn_prob(r1v + c1v[i], r2v + c2v[i], a.slope, a.thresh, a.keep_scale, a.seed, head,
+                                          m[i]);
+                if constexpr (KIND == K_F16) e[i] = __builtin_bit_cast(uint16_t, _Float16(p));
+                else if constexpr (KIND == K_BF16) e[i] = uint16_t(bf16_bits_rne(p));
+                else e[i] = __float_as_uint(p);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if constexpr (EB == 4) av[i] = e[i];
+            else if constexpr (EB == 2) av[i >> 1] |= e[i] << (16 * (i & 1));
+            else av[i >> 2] |= e[i] << (8 * (i & 3));
+        }
+    }
+
+    __device__ __forceinline__ void load_a(int kb, const int (&cid)[8], uint32_t (&av)[AW]) const
     {
         if constexpr (AMODE == A_COMPACT) tile_compact(kb, av);
+        else if constexpr (AMODE == A_ATTN) tile_attn(kb, cid, av);
         else tile(kb, av);
     }
 
@@ -407,6 +473,15 @@ __global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
     k.f0 = int64_t(slice) * 16 * T + int64_t(k.n) * T;
     k.f_ok = k.f0 < a.dim_n;
     k.a_row_ok = k.n < k.W;
+    k.r1v = 0.f;
+    k.r2v = 0.f;
+    if constexpr (AMODE == A_ATTN) {
+        const int64_t row = int64_t(a.win[part]) * k.W + k.n;
+        if (k.a_row_ok && row < a.n_nodes) {
+            k.r1v = a.r1[int64_t(head) * a.lg_hs + row * a.lg_ns];
+            if (a.r2) k.r2v = a.r2[int64_t(head) * a.of_hs + row * a.of_ns];
+        }
+    }
     const int g = k.g, W = k.W, nreal = k.nreal;
     const int64_t f0 = k.f0;
 
@@ -419,7 +494,7 @@ __global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
             int cid[8];
             uint32_t av[AW], xv[8][XW];
             k.cols(kb, cid);
-            k.load_a(kb, av);
+            k.load_a(kb, cid, av);
             k.rows(cid, xv);
             KB::mma(av, xv, acc);
         }
@@ -452,7 +527,7 @@ __global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
         for (int kb = 0; kb < nreal; kb += 32) {
             uint32_t av[AW], xv[8][XW];
             int nid[8];
-            k.load_a(kb, av);
+            k.load_a(kb, cid, av);
             k.rows(cid, xv);
             k.cols(kb + 32, nid);
             KB::mma(av, xv, acc);
@@ -495,8 +570,8 @@ __global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
         for (int kb = 0; kb < nreal; kb += 64) {
             uint32_t av0[AW], av1[AW], xv0[8][XW], xv1[8][XW];
             int n0[8], n1[8];
-            k.load_a(kb, av0);
-            k.load_a(kb + 32, av1);
+            k.load_a(kb, c0, av0);
+            k.load_a(kb + 32, c1, av1);
             k.rows(c0, xv0);
             k.rows(c1, xv1);
             k.cols(kb + 64, n0);
@@ -668,10 +743,11 @@ hipError_t run_spmm(int kind, int amode, int t, const SpmmArgs &a, bool covered,
     }
     if (a.parts == 0) return hipSuccess;
     hipError_t e;
-#define ATTGNN_BY_AMODE(K)                                                           \
-    (amode == A_MAPPED ? launch_main<K, A_MAPPED>(t, a, s)                       \
-                       : amode == A_COMPACT ? launch_main<K, A_COMPACT>(t, a, s) \
-                                            : launch_main<K, A_DENSE>(t, a, s))
+#define ATTGNN_BY_AMODE(K)                                                                              \
+    (amode == A_MAPPED ? launch_main<K, A_MAPPED>(t, a, s)                                          \
+                       : amode == A_COMPACT ? launch_main<K, A_COMPACT>(t, a, s)                    \
+                                            : amode == A_ATTN ? launch_main<K, A_ATTN>(t, a, s)     \
+                                                              : launch_main<K, A_DENSE>(t, a, s))
     switch (kind) {
     case K_F16: e = ATTGNN_BY_AMODE(K_F16); break;
     case K_BF16: e = ATTGNN_BY_AMODE(K_BF16); break;
@@ -791,6 +867,57 @@ extern "C" int attgnn_spmm_edge_map(const int32_t *rp, const int32_t *col, const
     a.heads = int(heads); a.window = window; a.wide_log2 = wl; a.y_type = y_type; a.pipe = pipe_of(flags, eb, true);
     a.xcd_slices = a.slices > 1 && 8 % a.slices == 0;
     const hipError_t e = run_spmm(kind, A_MAPPED, t, a, (flags & ATTGNN_SPMM_ROWS_COVERED) != 0, s);
+    return e == hipSuccess ? ATTGNN_OK : int(e);
+}
+
+extern "C" int attgnn_spmm_attention(const int32_t *rp, const int32_t *col, const int32_t *map,
+                                     const int32_t *window_of_part, const int32_t *atomic, int64_t parts,
+                                     const attgnn_attn_terms *terms, const void *X, int x_type, int x_layout,
+                                     int64_t heads, int64_t n_x, void *Y, int y_type, int64_t m_ori, int64_t dim_n,
+                                     int out_layout, int window, int wide, int mode, float *workspace,
+                                     int64_t workspace_elems, int flags, attgnn_stream_t stream)
+{
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (parts < 0 || m_ori < 0 || dim_n <= 0 || n_x < 0 || heads <= 0 || !terms) return ATTGNN_E_ARG;
+    if (!(window == 8 || window == 16)) return ATTGNN_E_ARG;
+    if (!(out_layout == ATTGNN_LAYOUT_HMD || out_layout == ATTGNN_LAYOUT_MHD)) return ATTGNN_E_ARG;
+    if (!(x_layout == ATTGNN_LAYOUT_HMD || x_layout == ATTGNN_LAYOUT_MHD)) return ATTGNN_E_ARG;
+    if (terms->n_nodes < 0 || !(terms->drop_p >= 0.f && terms->drop_p < 1.f)) return ATTGNN_E_ARG;
+    const int wl = wide_log2_of(wide);
+    if (wl < 0) return ATTGNN_E_ARG;
+    const int kind = kind_of(mode, x_type, x_type);
+    if (kind < 0 || kind == K_FP8) return ATTGNN_E_UNSUPPORTED;
+    if (!Y || !(y_type == ATTGNN_F32 || y_type == ATTGNN_F16 || y_type == ATTGNN_BF16)) return ATTGNN_E_ARG;
+    if (parts > 0 && (!rp || !col || !map || !window_of_part || !atomic || !X || !workspace || !terms->row_logit ||
+                      !terms->col_logit))
+        return ATTGNN_E_ARG;
+    if (parts > 0 && workspace_elems < heads * attgnn_spmm_workspace_elems(parts, window, dim_n)) return ATTGNN_E_ARG;
+    const int eb = elt_bytes(kind);
+    const int t = pick_t(dim_n, eb);
+    if (reinterpret_cast<uintptr_t>(X) % (t * eb) != 0) return ATTGNN_E_ARG;
+    if (reinterpret_cast<uintptr_t>(Y) % 16 != 0) return ATTGNN_E_ARG;
+
+    SpmmArgs a{};
+    a.rp = rp; a.col = col; a.win = window_of_part; a.atomic = atomic; a.map = map;
+    a.x = static_cast<const uint8_t *>(X);
+    a.y = Y; a.partial = workspace;
+    a.parts = parts; a.m_ori = m_ori; a.dim_n = dim_n;
+    if (x_layout == ATTGNN_LAYOUT_HMD) { a.x_head_stride = n_x * dim_n; a.x_row_stride = dim_n; }
+    else { a.x_head_stride = dim_n; a.x_row_stride = heads * dim_n; }
+    if (out_layout == ATTGNN_LAYOUT_HMD) { a.y_head_stride = m_ori * dim_n; a.y_row_stride = dim_n; }
+    else { a.y_head_stride = dim_n; a.y_row_stride = heads * dim_n; }
+    a.slices = int((dim_n + 16 * t - 1) / (16 * t));
+    a.heads = int(heads); a.window = window; a.wide_log2 = wl; a.y_type = y_type; a.pipe = pipe_of(flags, eb, true);
+    a.xcd_slices = a.slices > 1 && 8 % a.slices == 0;
+    a.r1 = terms->row_logit; a.r2 = terms->row_offset; a.c1 = terms->col_logit; a.c2 = terms->col_offset;
+    a.lg_hs = terms->logit_head_stride; a.lg_ns = terms->logit_node_stride;
+    a.of_hs = terms->offset_head_stride; a.of_ns = terms->offset_node_stride;
+    a.n_nodes = terms->n_nodes;
+    a.slope = terms->slope;
+    a.thresh = terms->drop_p <= 0.f ? 0u : uint32_t(std::min(double(terms->drop_p), 0.999999) * 4294967296.0);
+    a.keep_scale = 1.f / (1.f - terms->drop_p);
+    a.seed = terms->seed;
+    const hipError_t e = run_spmm(kind, A_ATTN, t, a, (flags & ATTGNN_SPMM_ROWS_COVERED) != 0, s);
     return e == hipSuccess ? ATTGNN_OK : int(e);
 }
 

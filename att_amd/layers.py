@@ -351,22 +351,38 @@ class GATAttention(torch.autograd.Function):
 
     GAT: Xs = Xt = X W, slope 0.2.  Reference GATv2: Xs = LReLU(X W0),
     Xt = LReLU(X W1), a0 = a1 = a, slope 1 (no LeakyReLU on the logits).
+    fused=True: the softmax writes only its log-sum-exp and both aggregations
+    rebuild the probabilities in the SpMM (attgnn_spmm_attention); bitwise
+    equal to the default fused=False, which stores p [H, nnz] between them.
+    Rebuilding costs time (4 heads x 64 on a 2.48M-entry graph: aggregation
+    282 -> 381 us forward, 357 -> 403 us backward) and saves the H * nnz
+    probabilities, so it is the choice when those do not fit.
     """
 
     @staticmethod
-    def forward(ctx, Xs, Xt, a0, a1, graph, slope, drop_p, seed):
+    def forward(ctx, Xs, Xt, a0, a1, graph, slope, drop_p, seed, fused=False):
         g = graph
         # node-major [N, H] scores: one load per neighbour for all heads
         s0 = ops.node_scores(Xs, a0.to(Xs.dtype).contiguous(), "mhd", out_layout="nh")
         s1 = ops.node_scores(Xt, a1.to(Xt.dtype).contiguous(), "mhd", out_layout="nh")
         long_rows, _ = g.long_row_lists()
-        p, lse = ops.edge_softmax_fwd(g.rowptr, g.col, s0, s1, slope, drop_p, seed, p_dtype=Xt.dtype, s_layout="nh",
-                                      long_rows=long_rows)
         rp, col, emap, win, at = g.fwd
-        h = ops.spmm_edge_map(rp, col, emap, p, win, at, Xt, "mhd", g.window, g.wide, m_ori=g.num_nodes_ori,
-                              x_layout="mhd", out_dtype=Xt.dtype, rows_covered=True)
+        if fused:
+            # the softmax only produces the row log-sum-exp; the aggregation
+            # rebuilds each probability where it multiplies (never stored)
+            p, lse = ops.edge_softmax_fwd(g.rowptr, g.col, s0, s1, slope, drop_p, seed, s_layout="nh",
+                                          long_rows=long_rows, lse_only=True)
+            h = ops.spmm_attention(rp, col, emap, win, at, Xt, "mhd", g.window, g.wide, s0, s1, row_offset=lse,
+                                   slope=slope, drop_p=drop_p, seed=seed, m_ori=g.num_nodes_ori, x_layout="mhd",
+                                   out_dtype=Xt.dtype, rows_covered=True)
+            p = lse.new_empty(0)
+        else:
+            p, lse = ops.edge_softmax_fwd(g.rowptr, g.col, s0, s1, slope, drop_p, seed, p_dtype=Xt.dtype,
+                                          s_layout="nh", long_rows=long_rows)
+            h = ops.spmm_edge_map(rp, col, emap, p, win, at, Xt, "mhd", g.window, g.wide, m_ori=g.num_nodes_ori,
+                                  x_layout="mhd", out_dtype=Xt.dtype, rows_covered=True)
         ctx.save_for_backward(Xs, Xt, a0, a1, s0, s1, lse, p)
-        ctx.graph, ctx.slope, ctx.drop_p, ctx.seed = g, slope, drop_p, seed
+        ctx.graph, ctx.slope, ctx.drop_p, ctx.seed, ctx.fused = g, slope, drop_p, seed, fused
         ctx.shared = Xs is Xt
         return h
 
@@ -376,10 +392,16 @@ class GATAttention(torch.autograd.Function):
         g = ctx.graph
         n, heads, dim = Xt.shape
         dh = dh.to(Xt.dtype).contiguous()
-        # aggregation: dXt[j] = sum_i p[(i,j)] dh[i]  (A^T edge-map SpMM)
+        # aggregation: dXt[j] = sum_i p[(i,j)] dh[i]  (A^T SpMM; with fused
+        # probabilities, row j's term is s1[j], column i's are s0[i] and lse[i])
         rp, col, emap_t, win, at = g.bwd
-        dXt = ops.spmm_edge_map(rp, col, emap_t, p, win, at, dh, "mhd", g.window, g.wide, m_ori=n,
-                                x_layout="mhd", rows_covered=True)
+        if ctx.fused:
+            dXt = ops.spmm_attention(rp, col, emap_t, win, at, dh, "mhd", g.window, g.wide, s1, s0, col_offset=lse,
+                                     slope=ctx.slope, drop_p=ctx.drop_p, seed=ctx.seed, m_ori=n, x_layout="mhd",
+                                     rows_covered=True)
+        else:
+            dXt = ops.spmm_edge_map(rp, col, emap_t, p, win, at, dh, "mhd", g.window, g.wide, m_ori=n,
+                                    x_layout="mhd", rows_covered=True)
         # dp[(i,j)] = <dh[i], Xt[j]>  (SDDMM over A)
         dp = g.sddmm(dh, Xt)
         # row and column sums of dL/dz (columns through the transpose permutation)
@@ -393,7 +415,7 @@ class GATAttention(torch.autograd.Function):
         dXs, dXt, da0, da1 = ops.attention_node_bwd(Xs.contiguous(), Xt.contiguous(), a0, a1, ds0, ds1,
                                                     base=dXt, combine=shared)
         return (dXs, dXt, da0.to(a0.dtype).reshape(a0.shape), da1.to(a1.dtype).reshape(a1.shape),
-                None, None, None, None)
+                None, None, None, None, None)
 
 
 class GATv2Attention(torch.autograd.Function):
@@ -445,6 +467,7 @@ class GATConv_multi(nn.Module):
         nn.init.xavier_normal_(self.a1.data, gain=gain)
         self.dropout = dropout
         self._step = 0
+        self.fused = False    # True: probabilities rebuilt in the aggregation, not stored (see GATAttention)
 
     def forward(self, X, inputInfo, seed=None):
         Xp = LinearSplitK.apply(X.to(self.dtype), self.fc.weight).view(-1, self.head, self.output_dim)
@@ -453,7 +476,7 @@ class GATConv_multi(nn.Module):
             seed = self._step
             self._step += 1
         return GATAttention.apply(Xp, Xp, self.a0.view(self.head, -1), self.a1.view(self.head, -1), inputInfo,
-                                  self.alpha, drop, seed)
+                                  self.alpha, drop, seed, self.fused)
 
 
 class GATv2Conv_multi(nn.Module):
@@ -478,6 +501,7 @@ class GATv2Conv_multi(nn.Module):
         nn.init.xavier_normal_(self.a0.data, gain=gain)
         self.dropout = dropout
         self._step = 0
+        self.fused = False    # reference logits; True: probabilities rebuilt in the aggregation
 
     def forward(self, X, inputInfo, seed=None):
         Xd = X.to(self.dtype)
@@ -492,4 +516,4 @@ class GATv2Conv_multi(nn.Module):
             return GATv2Attention.apply(X0, X1, a, inputInfo, self.alpha, drop, seed)
         X0 = F.leaky_relu(LinearSplitK.apply(Xd, self.fc0.weight), self.alpha).view(-1, self.head, self.output_dim)
         X1 = F.leaky_relu(LinearSplitK.apply(Xd, self.fc1.weight), self.alpha).view(-1, self.head, self.output_dim)
-        return GATAttention.apply(X0, X1, a, a, inputInfo, 1.0, drop, seed)
+        return GATAttention.apply(X0, X1, a, a, inputInfo, 1.0, drop, seed, self.fused)

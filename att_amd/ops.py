@@ -372,6 +372,69 @@ def spmm_edge_map(rp, col, emap, edge_val, window_of_part, atomic, X, out_layout
     return y
 
 
+def _node_terms(t, heads, layout, name):
+    """(pointer, head stride, node stride, nodes) of fp32 node terms, [N, H] ("nh") or [H, N] ("hn")."""
+    if t is None:
+        return None, 0, 0, None
+    if t.dtype != torch.float32 or t.dim() != 2 or not t.is_contiguous():
+        raise ValueError(f"{name} must be a contiguous fp32 2-D tensor")
+    if layout == "nh" and t.shape[1] == heads:
+        return _ptr(t), 1, heads, t.shape[0]
+    if layout == "hn" and t.shape[0] == heads:
+        return _ptr(t), t.shape[1], 1, t.shape[1]
+    raise ValueError(f"{name}: shape {tuple(t.shape)} is not {layout!r} with {heads} heads")
+
+
+def spmm_attention(rp, col, emap, window_of_part, atomic, X, out_layout, window, wide, row_logit, col_logit,
+                   row_offset=None, col_offset=None, slope=0.2, drop_p=0.0, seed=0, m_ori=None, mode=None,
+                   x_layout="hmd", out_dtype=torch.float32, rows_covered=False, pipe=None, logit_layout="nh",
+                   offset_layout="hn"):
+    """Attention aggregation with the probabilities rebuilt in the kernel:
+    entry (r, c) = exp(LeakyReLU(row_logit[r] + col_logit[c]) - row_offset[r] - col_offset[c]) with the edge
+    softmax's dropout; equal bit for bit to :func:`spmm_edge_map` over the p that
+    :func:`edge_softmax_fwd` writes.  Logit terms are fp32 [N, H] (``logit_layout="nh"``, the node scores'
+    node-major form) or [H, N]; offsets (the softmax's log-sum-exp) [H, N] (``offset_layout="hn"``) or
+    [N, H].  X / Y layouts and types as :func:`spmm_edge_map`."""
+    dev = _need_gpu("spmm_attention", rp, col, emap, window_of_part, atomic, X, row_logit, col_logit, row_offset,
+                    col_offset)
+    _need_i32("spmm_attention", rp, col, emap, window_of_part, atomic)
+    if X.dim() == 2:
+        X = X.unsqueeze(0 if x_layout == "hmd" else 1)
+    if x_layout == "hmd":
+        heads, n_x, dim = X.shape
+    else:
+        n_x, heads, dim = X.shape
+    r1, lhs, lns, n1 = _node_terms(row_logit, heads, logit_layout, "row_logit")
+    c1, _, _, n2 = _node_terms(col_logit, heads, logit_layout, "col_logit")
+    r2, ohs, ons, n3 = _node_terms(row_offset, heads, offset_layout, "row_offset")
+    c2, ohs2, ons2, n4 = _node_terms(col_offset, heads, offset_layout, "col_offset")
+    if r1 is None or c1 is None:
+        raise ValueError("row_logit and col_logit are required")
+    if r2 is None:
+        ohs, ons, n3 = ohs2, ons2, n4
+    if n1 != n2 or any(n is not None and n != n1 for n in (n3, n4)):
+        raise ValueError("the node terms cover different node counts")
+    m = n_x if m_ori is None else m_ori
+    layout = {"hmd": _lib.LAYOUT_HMD, "mhd": _lib.LAYOUT_MHD}[out_layout]
+    shape = (heads, m, dim) if layout == _lib.LAYOUT_HMD else (m, heads, dim)
+    if out_dtype not in (torch.float32, torch.float16, torch.bfloat16):
+        raise TypeError(f"output type {out_dtype} is not supported")
+    y = torch.empty(shape, dtype=out_dtype, device=dev)
+    parts = window_of_part.numel()
+    ws_n = heads * lib().attgnn_spmm_workspace_elems(parts, window, dim) if parts else 0
+    ws = torch.empty(max(ws_n, 1), dtype=torch.float32, device=dev)
+    terms = _lib.AttnTerms(r1, c1, r2, c2, lhs, lns, ohs, ons, n1, float(slope), float(drop_p),
+                           int(seed) & (2**64 - 1))
+    xl = {"hmd": _lib.LAYOUT_HMD, "mhd": _lib.LAYOUT_MHD}[x_layout]
+    rc = lib().attgnn_spmm_attention(_ptr(rp), _ptr(col), _ptr(emap), _ptr(window_of_part), _ptr(atomic), parts,
+                                     ctypes.byref(terms), _ptr(X), _TYPE_OF[X.dtype], xl, heads, n_x, _ptr(y),
+                                     _TYPE_OF[out_dtype], m, dim, layout, window, wide, _mode(mode, X.dtype),
+                                     _ptr(ws), ws_n, (_lib.SPMM_ROWS_COVERED if rows_covered else 0) | _PIPES[pipe],
+                                     _stream(dev))
+    check(rc, "attgnn_spmm_attention")
+    return y
+
+
 def rowsum_edge_map(rp, emap, window_of_part, edge_val, m_rows, out_layout, window, wide):
     """Row sums of per-edge values; fp32 [H, M] ("hmd") or [M, H] ("mhd")."""
     dev = _need_gpu("rowsum_edge_map", rp, emap, window_of_part, edge_val)
@@ -571,20 +634,22 @@ def _score_layout(s0, s1, s_layout):
 
 
 def edge_softmax_fwd(rowptr, col, s0, s1, slope, drop_p=0.0, seed=0, p_dtype=torch.float32, s_layout="hn",
-                     long_rows=None):
+                     long_rows=None, lse_only=False):
     """p [H, nnz] and lse [H, N] of the fused LeakyReLU + softmax + dropout;
     s0 / s1 [H, N] ("hn") or [N, H] ("nh").  ``long_rows``: the list from
-    :func:`long_rows` (rows above 256 entries get a workgroup each)."""
+    :func:`long_rows` (rows above 256 entries get a workgroup each).
+    ``lse_only``: returns (None, lse) without writing p (for :func:`spmm_attention`)."""
     dev = _need_gpu("edge_softmax_fwd", rowptr, col, s0, s1)
     _need_i32("edge_softmax_fwd", rowptr, col)
     heads, n, shs, srs = _score_layout(s0, s1, s_layout)
     lp, ln = _rows_arg("long_rows", long_rows, dev)
     nnz = col.numel()
-    p = torch.empty((heads, nnz), dtype=p_dtype, device=dev)
+    p = None if lse_only else torch.empty((heads, nnz), dtype=p_dtype, device=dev)
     lse = torch.empty((heads, n), dtype=torch.float32, device=dev)
     rc = lib().attgnn_edge_softmax_fwd(_ptr(rowptr), _ptr(col), n, nnz, heads, _ptr(s0), _ptr(s1), shs, srs,
                                        float(slope), float(drop_p), int(seed) & (2**64 - 1), _ptr(p),
-                                       _TYPE_OF[p_dtype], _ptr(lse), lp, ln, _stream(dev))
+                                       _TYPE_OF.get(p_dtype, _TYPE_OF[torch.float32]), _ptr(lse), lp, ln,
+                                       _stream(dev))
     check(rc, "attgnn_edge_softmax_fwd")
     return p, lse
 

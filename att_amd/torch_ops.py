@@ -81,6 +81,29 @@ def _(rp, col, emap, edge_val, window_of_part, atomic, X, out_layout, window, wi
     return X.new_empty(shape, dtype=out_dtype)
 
 
+@torch.library.custom_op(f"{_LIB}::spmm_attention", mutates_args=())
+def spmm_attention(rp: torch.Tensor, col: torch.Tensor, emap: torch.Tensor, window_of_part: torch.Tensor,
+                   atomic: torch.Tensor, X: torch.Tensor, out_layout: str, window: int, wide: int,
+                   row_logit: torch.Tensor, col_logit: torch.Tensor, row_offset: Optional[torch.Tensor],
+                   col_offset: Optional[torch.Tensor], slope: float, drop_p: float, seed: int, m_ori: int, mode: str,
+                   x_layout: str, out_dtype: torch.dtype, rows_covered: bool) -> torch.Tensor:
+    """Attention aggregation with the probabilities rebuilt in the kernel (attgnn_spmm_attention);
+    logits node-major [N, H], offsets [H, N]; m_ori < 0 means the rows of X."""
+    return ops.spmm_attention(rp, col, emap, window_of_part, atomic, X, out_layout, window, wide, row_logit,
+                              col_logit, row_offset=row_offset, col_offset=col_offset, slope=slope, drop_p=drop_p,
+                              seed=seed, m_ori=None if m_ori < 0 else m_ori, mode=_mode(mode), x_layout=x_layout,
+                              out_dtype=out_dtype, rows_covered=rows_covered)
+
+
+@spmm_attention.register_fake
+def _(rp, col, emap, window_of_part, atomic, X, out_layout, window, wide, row_logit, col_logit, row_offset,
+      col_offset, slope, drop_p, seed, m_ori, mode, x_layout, out_dtype, rows_covered):
+    heads, n, dim = _hmd(X, x_layout)
+    m = n if m_ori < 0 else m_ori
+    shape = (heads, m, dim) if out_layout == "hmd" else (m, heads, dim)
+    return X.new_empty(shape, dtype=out_dtype)
+
+
 # ---------------------------------------------------------------- SDDMM and attention
 
 @torch.library.custom_op(f"{_LIB}::sddmm_edges", mutates_args=())
@@ -192,5 +215,5 @@ def _(Xs, Xt, a0, a1, ds0, ds1, base, combine):
         Xs.new_empty((heads, dim), dtype=torch.float32)
 
 
-OPS = ("spmm_block", "spmm_compact", "spmm_edge_map", "sddmm_edges", "node_scores", "edge_softmax_fwd",
+OPS = ("spmm_block", "spmm_compact", "spmm_edge_map", "spmm_attention", "sddmm_edges", "node_scores", "edge_softmax_fwd",
        "edge_softmax_bwd", "edge_softmax_bwd_sums", "gather_rowsum", "attention_node_bwd")

@@ -314,6 +314,44 @@ int attgnn_spmm_edge_map(const int32_t *rp, const int32_t *col, const int32_t *m
                          attgnn_stream_t stream);
 
 /*
+ * Attention aggregation with the probabilities rebuilt in the kernel:
+ * Y_h = P_h . X_h over the edge-id map's tiles, where the entry of edge e =
+ * (r, c) (e = map[slot], A's CSR position) is
+ *   P_h[r, c] = exp(LeakyReLU_slope(row_logit(h, r) + col_logit(h, c))
+ *                   - row_offset(h, r) - col_offset(h, c)),
+ * then dropout with drop_p from the same (seed, h, e) hash as
+ * attgnn_edge_softmax_fwd, kept values scaled by 1/(1-drop_p); NULL offsets
+ * count as 0.  Element (h, v) of the logit terms is at
+ * [h*logit_head_stride + v*logit_node_stride], of the offsets at
+ * [h*offset_head_stride + v*offset_node_stride].  With the map of A,
+ * row_logit = s0, col_logit = s1, row_offset = the softmax's lse, this equals
+ * attgnn_spmm_edge_map over the p written by attgnn_edge_softmax_fwd bit for
+ * bit (p in X's type); with the transposed map, row_logit = s1, col_logit =
+ * s0, col_offset = lse it is the A^T aggregation of the backward -- and p is
+ * never stored.  Other arguments as attgnn_spmm_edge_map (X, Y, layouts,
+ * mode, workspace, flags).
+ *
+ * Replaces the aggregation calls ATT_SpMM_v{1,3}.spmm_*_{h_m,m_h} of
+ *   end2end_head/gat_final/mgat_csr/mgat_conv.py:79-86, 99-106 together with
+ *   the probability tensor they read (:180-197).
+ */
+typedef struct attgnn_attn_terms {
+    const float *row_logit, *col_logit;
+    const float *row_offset, *col_offset;  /* may be NULL */
+    int64_t logit_head_stride, logit_node_stride;
+    int64_t offset_head_stride, offset_node_stride;
+    int64_t n_nodes;                       /* terms cover nodes [0, n_nodes) */
+    float slope, drop_p;
+    uint64_t seed;
+} attgnn_attn_terms;
+int attgnn_spmm_attention(const int32_t *rp, const int32_t *col, const int32_t *map,
+                          const int32_t *window_of_part, const int32_t *atomic, int64_t parts,
+                          const attgnn_attn_terms *terms, const void *X, int x_type, int x_layout,
+                          int64_t heads, int64_t n_x, void *Y, int y_type, int64_t m_ori, int64_t dim_n,
+                          int out_layout, int window, int wide, int mode, float *workspace,
+                          int64_t workspace_elems, int flags, attgnn_stream_t stream);
+
+/*
  * Row sums of per-edge values through the edge-id map:
  * out(h, i) = sum_{slots of row i} edge_val[h*nnz + map[slot]] (fp32).
  *
@@ -501,7 +539,9 @@ int64_t attgnn_long_rows(const int32_t *rowptr_host, int64_t n, int32_t *rows);
  *   z = s0[h,i] + s1[h,j];  v = LeakyReLU_slope(z);  p = exp(v - lse_i) with
  *   lse_i = log sum_j exp(v) (max subtracted, FP32);  dropout with probability
  *   drop_p from a counter-based hash of (seed, h, e), kept values scaled by
- *   1/(1-drop_p).  Writes p[h*nnz + e] (p_type) and lse[h*n + i] (fp32).
+ *   1/(1-drop_p).  Writes p[h*nnz + e] (p_type) and lse[h*n + i] (fp32);
+ *   p = NULL writes the log-sum-exp only (for attgnn_spmm_attention, which
+ *   rebuilds p where it is used).
  * s0/s1 element (h, i) at s[h*s_head_stride + i*s_row_stride].
  *
  * Replaces the LeakyReLU / exp / rowsum / dropout / divide chain of

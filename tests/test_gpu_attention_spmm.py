@@ -1,0 +1,127 @@
+"""Attention aggregation with the probabilities rebuilt inside the SpMM
+(attgnn_spmm_attention) against the two-step path it replaces: the edge
+softmax writes p, the edge-map SpMM reads it.  Both must agree bit for bit --
+forward over A's map (row logit s0, column logit s1, row offset lse) and the
+backward's A^T aggregation over the transposed map (row logit s1, column
+logit s0, column offset lse) -- for every arithmetic kind, head count, slope
+and dropout, on symmetric and non-symmetric graphs with hub rows."""
+import numpy as np
+import pytest
+import torch
+
+from att_amd import graphs as G
+from att_amd import ops
+from att_amd.layers import GATGraph
+
+pytestmark = pytest.mark.gpu
+
+
+def _nonsymmetric(n, deg, seed):
+    g = G.chung_lu(n, deg, 2.1, seed=seed)
+    rng = np.random.default_rng(seed)
+    keep = rng.random(g.nnz) < 0.7
+    rows = g.row_ids()[keep].astype(np.int64)
+    cols = g.indices[keep].astype(np.int64)
+    keys = np.unique(np.concatenate([rows * n + cols, np.arange(n) * (n + 1)]))
+    ip = np.zeros(n + 1, np.int64)
+    ip[1:] = np.cumsum(np.bincount(keys // n, minlength=n))
+    return G.CSR(ip, (keys % n).astype(np.int32), n)
+
+
+@pytest.fixture(scope="module", params=["symmetric", "nonsymmetric"])
+def graph(request, cuda):
+    g = G.chung_lu(900, 10, 2.1, seed=7) if request.param == "symmetric" else _nonsymmetric(900, 10, 8)
+    return GATGraph(g, cuda, build="device"), g
+
+
+@pytest.mark.parametrize("heads,dim", [(1, 64), (2, 32), (4, 16), (8, 8)])
+@pytest.mark.parametrize("dtype,mode", [(torch.bfloat16, None), (torch.float16, None), (torch.float32, "f32"),
+                                        (torch.float32, "bf16x3")])
+def test_fused_equals_softmax_then_edge_map(cuda, graph, heads, dim, dtype, mode):
+    gg, g = graph
+    n = g.n
+    gen = torch.Generator(device="cpu").manual_seed(heads * 10 + dim)
+    s0 = torch.randn(n, heads, generator=gen).to(cuda)
+    s1 = torch.randn(n, heads, generator=gen).to(cuda)
+    X = torch.randn(n, heads, dim, generator=gen).to(dtype).to(cuda)
+    dh = torch.randn(n, heads, dim, generator=gen).to(dtype).to(cuda)
+    lr, _ = gg.long_row_lists()
+    rp, col, emap, win, at = gg.fwd
+    rpt, colt, emapt, wint, att = gg.bwd
+    for slope, drop in ((0.2, 0.0), (0.2, 0.3), (1.0, 0.1)):
+        p, lse = ops.edge_softmax_fwd(gg.rowptr, gg.col, s0, s1, slope, drop, 5, p_dtype=dtype, s_layout="nh",
+                                      long_rows=lr)
+        _, lse2 = ops.edge_softmax_fwd(gg.rowptr, gg.col, s0, s1, slope, drop, 5, s_layout="nh", long_rows=lr,
+                                       lse_only=True)
+        assert torch.equal(lse, lse2)
+        for out_dtype in (torch.float32, dtype):
+            ref = ops.spmm_edge_map(rp, col, emap, p, win, at, X, "mhd", gg.window, gg.wide, m_ori=n, mode=mode,
+                                    x_layout="mhd", out_dtype=out_dtype, rows_covered=True)
+            got = ops.spmm_attention(rp, col, emap, win, at, X, "mhd", gg.window, gg.wide, s0, s1, row_offset=lse,
+                                     slope=slope, drop_p=drop, seed=5, m_ori=n, mode=mode, x_layout="mhd",
+                                     out_dtype=out_dtype, rows_covered=True)
+            assert torch.equal(ref, got), ("forward", slope, drop, out_dtype)
+        ref_t = ops.spmm_edge_map(rpt, colt, emapt, p, wint, att, dh, "mhd", gg.window, gg.wide, m_ori=n, mode=mode,
+                                  x_layout="mhd", rows_covered=True)
+        got_t = ops.spmm_attention(rpt, colt, emapt, wint, att, dh, "mhd", gg.window, gg.wide, s1, s0,
+                                   col_offset=lse, slope=slope, drop_p=drop, seed=5, m_ori=n, mode=mode,
+                                   x_layout="mhd", rows_covered=True)
+        assert torch.equal(ref_t, got_t), ("transpose", slope, drop)
+
+
+def test_fused_against_fp64(cuda, graph):
+    """The fused aggregation is the softmax-weighted sum (FP64 on the host)."""
+    gg, g = graph
+    n, heads, dim = g.n, 4, 32
+    gen = torch.Generator(device="cpu").manual_seed(3)
+    s0 = torch.randn(n, heads, generator=gen).to(cuda)
+    s1 = torch.randn(n, heads, generator=gen).to(cuda)
+    X = torch.randn(n, heads, dim, generator=gen).to(cuda)
+    _, lse = ops.edge_softmax_fwd(gg.rowptr, gg.col, s0, s1, 0.2, 0.0, 0, s_layout="nh", lse_only=True)
+    rp, col, emap, win, at = gg.fwd
+    y = ops.spmm_attention(rp, col, emap, win, at, X, "mhd", gg.window, gg.wide, s0, s1, row_offset=lse, slope=0.2,
+                           m_ori=n, mode="f32", x_layout="mhd", rows_covered=True)
+    rows = torch.from_numpy(g.row_ids().astype(np.int64))
+    cols = torch.from_numpy(g.indices.astype(np.int64))
+    z = s0.double().cpu()[rows] + s1.double().cpu()[cols]
+    e = torch.nn.functional.leaky_relu(z, 0.2)
+    m = torch.full((n, heads), -np.inf, dtype=torch.float64).scatter_reduce(
+        0, rows[:, None].expand(-1, heads), e, "amax", include_self=True)
+    ex = torch.exp(e - m[rows])
+    p = ex / torch.zeros(n, heads, dtype=torch.float64).index_add(0, rows, ex)[rows]
+    ref = torch.zeros(n, heads, dim, dtype=torch.float64).index_add(0, rows, p[..., None] * X.double().cpu()[cols])
+    err = ((y.double().cpu() - ref).norm() / ref.norm()).item()
+    assert err < 1e-5, err
+
+
+def test_rejects_bad_terms(cuda, graph):
+    gg, g = graph
+    rp, col, emap, win, at = gg.fwd
+    X = torch.randn(g.n, 2, 16, device=cuda).bfloat16()
+    s = torch.randn(g.n, 2, device=cuda)
+    with pytest.raises(ValueError):
+        ops.spmm_attention(rp, col, emap, win, at, X, "mhd", 16, 32, s, s[:-1].contiguous(), x_layout="mhd")
+    with pytest.raises(ValueError):
+        ops.spmm_attention(rp, col, emap, win, at, X, "mhd", 16, 32, s, s.double(), x_layout="mhd")
+
+
+@pytest.mark.parametrize("kind", ["gat", "gatv2"])
+def test_layer_fused_equals_stored(cuda, graph, kind):
+    """A GAT / reference-GATv2 layer with the probabilities rebuilt in both
+    aggregations gives the same output and gradients, bit for bit, as with
+    them stored (dropout on, training mode)."""
+    from att_amd.layers import GATConv_multi, GATv2Conv_multi
+    gg, g = graph
+    torch.manual_seed(11)
+    layer = (GATConv_multi(48, 32, 0.3, 0.2, 4) if kind == "gat" else GATv2Conv_multi(48, 32, 0.3, 0.2, 4)).to(cuda)
+    X = torch.randn(g.n, 48, device=cuda)
+    go = torch.randn(g.n, 4, 32, device=cuda).bfloat16()
+    results = []
+    for fused in (False, True):
+        layer.fused = fused
+        layer.zero_grad(set_to_none=True)
+        out = layer(X, gg, seed=123)
+        out.backward(go)
+        results.append([out.detach().clone()] + [p.grad.clone() for p in layer.parameters()])
+    for a, b in zip(*results):
+        assert torch.equal(a, b)

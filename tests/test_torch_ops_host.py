@@ -47,6 +47,11 @@ def test_spmm_edge_map_shapes(fake, x_layout, out_layout):
     assert y.shape == ((heads, n, dim) if out_layout == "hmd" else (n, heads, dim))
     y = torch.ops.attgnn.spmm_edge_map(*args, 80, "auto", x_layout, torch.float16, True)
     assert y.shape[0 if out_layout == "mhd" else 1] == 80 and y.dtype == torch.float16
+    s = torch.empty(n, heads, device="cuda")
+    lse = torch.empty(heads, n, device="cuda")
+    y = torch.ops.attgnn.spmm_attention(*args[:3], args[4], args[5], X, out_layout, 16, 32, s, s, lse, None, 0.2,
+                                        0.0, 0, -1, "auto", x_layout, torch.bfloat16, True)
+    assert y.shape == ((heads, n, dim) if out_layout == "hmd" else (n, heads, dim)) and y.dtype == torch.bfloat16
 
 
 def test_attention_shapes(fake):
