@@ -75,6 +75,7 @@ struct SpmmArgs {
     int64_t x_head_stride, x_row_stride, y_head_stride, y_row_stride, val_head_stride;
     int slices, heads, window, wide_log2, y_type, pipe;
     int xcd_slices;         // 1: slice-per-XCD task mapping (slices divides 8)
+    int32_t *split_list;    // [0] = count, then (first, last) part of each split window
     // attention variant: tile value = exp(LeakyReLU(r1[row] + c1[col]) - r2[row] - c2[col])
     // with dropout on the edge id; element (h, v) at [h*hs + v*ns]
     const float *r1, *r2, *c1, *c2;   // r2 / c2 may be null (0)
@@ -459,6 +460,19 @@ __global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
         slice = hs - head * a.slices;
     }
 
+    // the wave of a split window's first part (head 0, slice 0) lists the
+    // window for the combine; list order varies, each window's sum does not
+    if (a.split_list && head == 0 && slice == 0 && lane == 0 && a.atomic[part]) {
+        const int w = a.win[part];
+        if (!(part > 0 && a.atomic[part - 1] && a.win[part - 1] == w)) {
+            int64_t last = part;
+            while (last + 1 < a.parts && a.atomic[last + 1] && a.win[last + 1] == w) ++last;
+            const int k = atomicAdd(a.split_list, 1);
+            a.split_list[1 + 2 * k] = int32_t(part);
+            a.split_list[2 + 2 * k] = int32_t(last);
+        }
+    }
+
     KB k{a};
     k.xh = a.x + int64_t(head) * a.x_head_stride * EB;
     k.vh = a.val + int64_t(head) * a.val_head_stride * EB;
@@ -616,28 +630,43 @@ __global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
     }
 }
 
-// Sum the partial tiles of every split window in part order.  Workgroups walk
-// the parts with a grid stride; only the first part of a split window does
-// work (the workgroup reduces rows x features of the window), so the launch
-// stays small however many parts there are.
-__global__ __launch_bounds__(256) void spmm_combine_kernel(SpmmArgs a)
+// Sum the partial tiles of the listed split windows in part order, V features
+// per thread over every (window, row, feature group): the sum of each output
+// element runs q = first..last from 0, whatever the list order.
+template <int V>
+__global__ __launch_bounds__(256) void spmm_combine_list_kernel(SpmmArgs a)
 {
     const int head = blockIdx.y;
-    for (int64_t p = blockIdx.x; p < a.parts; p += gridDim.x) {
-        if (!a.atomic[p]) continue;
-        const int w = a.win[p];
-        if (p > 0 && a.atomic[p - 1] && a.win[p - 1] == w) continue;
-        int64_t last = p;
-        while (last + 1 < a.parts && a.atomic[last + 1] && a.win[last + 1] == w) ++last;
-        const int64_t rows = min(int64_t(a.window), a.m_ori - int64_t(w) * a.window);
-        const int64_t n = rows * a.dim_n;
-        for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
-            const int64_t r = e / a.dim_n, f = e - r * a.dim_n;
-            float s[1] = {0.f};
-            for (int64_t q = p; q <= last; ++q) s[0] += a.partial[((q * a.heads + head) * a.window + r) * a.dim_n + f];
-            store_row<1>(a.y, int64_t(head) * a.y_head_stride + (int64_t(w) * a.window + r) * a.y_row_stride + f, a.y_type, s);
+    const int64_t n_split = a.split_list[0];
+    const int64_t groups = a.dim_n / V, per = int64_t(a.window) * groups;
+    const int64_t total = n_split * per;
+    for (int64_t it = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; it < total;
+         it += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t k = it / per, e = it - k * per;
+        const int64_t p = a.split_list[1 + 2 * k], last = a.split_list[2 + 2 * k];
+        const int64_t w = a.win[p];
+        const int64_t r = e / groups, f = (e - r * groups) * V;
+        if (w * a.window + r >= a.m_ori) continue;
+        float sv[V];
+#pragma unroll
+        for (int j = 0; j < V; ++j) sv[j] = 0.f;
+        for (int64_t q = p; q <= last; ++q) {
+            const float *src = a.partial + ((q * a.heads + head) * a.window + r) * a.dim_n + f;
+            if constexpr (V == 4) {
+                const f32x4 v = *reinterpret_cast<const f32x4 *>(src);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) sv[j] += v[j];
+            } else {
+                sv[0] += src[0];
+            }
         }
+        store_row<V>(a.y, int64_t(head) * a.y_head_stride + (w * a.window + r) * a.y_row_stride + f, a.y_type, sv);
     }
+}
+
+__global__ void zero_count_kernel(int32_t *count)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) count[0] = 0;
 }
 
 __global__ __launch_bounds__(256) void fill_zero_kernel(u32x4 *p, int64_t n4, uint8_t *tail, int64_t ntail)
@@ -742,23 +771,33 @@ hipError_t run_spmm(int kind, int amode, int t, const SpmmArgs &a, bool covered,
         if (e != hipSuccess) return e;
     }
     if (a.parts == 0) return hipSuccess;
+    // the split-window list sits after the partial tiles in the workspace
+    SpmmArgs b = a;
+    b.split_list = reinterpret_cast<int32_t *>(a.partial + a.parts * a.heads * int64_t(a.window) * a.dim_n);
+    hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(64), 0, s, b.split_list);
     hipError_t e;
 #define ATTGNN_BY_AMODE(K)                                                                              \
-    (amode == A_MAPPED ? launch_main<K, A_MAPPED>(t, a, s)                                          \
-                       : amode == A_COMPACT ? launch_main<K, A_COMPACT>(t, a, s)                    \
-                                            : amode == A_ATTN ? launch_main<K, A_ATTN>(t, a, s)     \
-                                                              : launch_main<K, A_DENSE>(t, a, s))
+    (amode == A_MAPPED ? launch_main<K, A_MAPPED>(t, b, s)                                          \
+                       : amode == A_COMPACT ? launch_main<K, A_COMPACT>(t, b, s)                    \
+                                            : amode == A_ATTN ? launch_main<K, A_ATTN>(t, b, s)     \
+                                                              : launch_main<K, A_DENSE>(t, b, s))
     switch (kind) {
     case K_F16: e = ATTGNN_BY_AMODE(K_F16); break;
     case K_BF16: e = ATTGNN_BY_AMODE(K_BF16); break;
-    case K_FP8: e = amode == A_COMPACT ? launch_main<K_FP8, A_COMPACT>(t, a, s) : launch_main<K_FP8, A_DENSE>(t, a, s); break;
+    case K_FP8: e = amode == A_COMPACT ? launch_main<K_FP8, A_COMPACT>(t, b, s) : launch_main<K_FP8, A_DENSE>(t, b, s); break;
     case K_BF16X3: e = ATTGNN_BY_AMODE(K_BF16X3); break;
     default: e = ATTGNN_BY_AMODE(K_F32); break;
     }
 #undef ATTGNN_BY_AMODE
     if (e != hipSuccess) return e;
-    const int64_t cblocks = std::min<int64_t>(a.parts, 2048);
-    hipLaunchKernelGGL(spmm_combine_kernel, dim3(unsigned(cblocks), unsigned(a.heads)), dim3(256), 0, s, a);
+    // at most parts / 2 split windows; enough workgroups for one element group per thread
+    const int v = a.dim_n % 4 == 0 ? 4 : 1;
+    const int64_t items = (a.parts / 2 + 1) * int64_t(a.window) * (a.dim_n / v);
+    const int64_t cblocks = std::max<int64_t>(1, std::min<int64_t>((items + 255) / 256, 4096));
+    if (v == 4)
+        hipLaunchKernelGGL(spmm_combine_list_kernel<4>, dim3(unsigned(cblocks), unsigned(a.heads)), dim3(256), 0, s, b);
+    else
+        hipLaunchKernelGGL(spmm_combine_list_kernel<1>, dim3(unsigned(cblocks), unsigned(a.heads)), dim3(256), 0, s, b);
     return hipGetLastError();
 }
 
@@ -766,7 +805,8 @@ hipError_t run_spmm(int kind, int amode, int t, const SpmmArgs &a, bool covered,
 
 extern "C" int64_t attgnn_spmm_workspace_elems(int64_t parts, int window, int64_t dim_n)
 {
-    return parts * window * dim_n;
+    // partial tiles of split windows, then the list of split windows
+    return parts * window * dim_n + parts + 2;
 }
 
 namespace {

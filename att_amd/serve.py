@@ -211,7 +211,8 @@ class Bucket:
             self.ccol = torch.zeros(self.nnz_cap, **i32)
             dmax = GAT_DIM
         heads = GAT_HEADS if kind == "gat" else 1
-        self.ws = torch.empty(self.parts_cap * WINDOW * dmax * heads, dtype=torch.float32, device=device)
+        self.ws = torch.empty(heads * _lib.lib().attgnn_spmm_workspace_elems(self.parts_cap, WINDOW, dmax),
+                              dtype=torch.float32, device=device)
         buf_bytes = sum(t.numel() * t.element_size() for t in self._buffers())
         self.max_segs = self.max_graphs * 10 + 10 + buf_bytes // CHUNK_BYTES + 16
         self.table_host = torch.zeros(self.max_segs * SEG_DTYPE.itemsize, dtype=torch.uint8).pin_memory()
