@@ -15,6 +15,7 @@ MODE_NATIVE, MODE_BF16X3, MODE_F32 = 0, 1, 2
 SRC_DEG_F16, SRC_DEG_F32, SRC_EDGE_F32, SRC_MAP = 0, 1, 2, 3
 LAYOUT_HMD, LAYOUT_MHD = 0, 1
 SPMM_ROWS_COVERED = 1
+SPMM_NO_XCD_SLICES = 1 << 6
 
 _c = ctypes
 _vp, _i64, _int = _c.c_void_p, _c.c_int64, _c.c_int
@@ -58,6 +59,9 @@ _SIGNATURES = {
                                   _i64, _i64, _vp, _int, _i64, _int, _int, _vp]),
     "attgnn_spmm_edge_map": (_int, [_vp, _vp, _vp, _vp, _int, _i64, _vp, _vp, _i64, _vp, _int, _int, _i64, _i64,
                                     _vp, _int, _i64, _i64, _int, _int, _int, _int, _vp, _i64, _int, _vp]),
+    "attgnn_spmm_edge_map_compact": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _int, _i64, _vp, _vp, _i64, _vp, _int,
+                                            _int, _i64, _i64, _vp, _int, _i64, _i64, _int, _int, _int, _vp, _i64,
+                                            _int, _vp]),
     "attgnn_spmm_attention": (_int, [_vp, _vp, _vp, _vp, _vp, _i64, _ATTNP, _vp, _int, _int, _i64, _i64, _vp, _int,
                                      _i64, _i64, _int, _int, _int, _int, _vp, _i64, _int, _vp]),
     "attgnn_rowsum_edge_map": (_int, [_vp, _vp, _vp, _i64, _vp, _int, _i64, _i64, _vp, _int, _i64, _int, _int,

@@ -11,9 +11,9 @@
 //   running offsets (part offsets rp, CSR row pointers): + offset;
 //   raw bytes (tile values, features): copied.
 // The tail of every buffer beyond the batch is filled with "empty" entries
-// (rp / row pointers repeat the total, window ids point at the reserved last
-// window), so kernels launched with capacity sizes do no work there -- which
-// lets one HIP graph per capacity bucket serve every batch.
+// (rp / row pointers repeat the total, window ids are -1, which marks a
+// padding part the SpMM skips), so kernels launched with capacity sizes do no
+// work there -- which lets one HIP graph per capacity bucket serve every batch.
 #include <hip/hip_runtime.h>
 
 #include <climits>

@@ -64,7 +64,7 @@ __device__ __forceinline__ void load_bytes(const uint8_t *p, uint32_t *d)
 struct SpmmArgs {
     const int32_t *rp, *col, *win, *atomic;
     const uint8_t *val;     // tiles, or per-edge values [heads][val_head_stride] (mapped)
-    const int32_t *map;     // mapped variant: CSR position per tile slot, -1 = none
+    const int32_t *map;     // mapped variant: CSR position per tile slot, -1 = none; compact map: packed ids
     const uint16_t *masks;  // compact variant: row mask per condensed column
     const int64_t *bbase;   // compact variant: value offset per 32-column block
     const uint8_t *x;       // X rows: element (h, c, f) at x[h*x_head_stride + c*x_row_stride + f]
@@ -139,16 +139,20 @@ __device__ __forceinline__ void load_bytes_nt(const uint8_t *p, uint32_t *d)
     }
 }
 
+// A operand: dense value tiles, edge-id map tiles (values gathered by edge
+// id), compact value tiles, map tiles with the attention probabilities
+// rebuilt, compact edge-id lists (values gathered by edge id).
+enum AMode { A_DENSE = 0, A_MAPPED = 1, A_COMPACT = 2, A_ATTN = 3, A_CMAP = 4 };
+
 // Per-wave state and the three steps of one k-block (32 condensed columns):
 // column ids of the lane's 8 k slots, the A operand (tile row n), the X rows.
-enum AMode { A_DENSE = 0, A_MAPPED = 1, A_COMPACT = 2, A_ATTN = 3 };
-
 template <int KIND, int T, int AMODE>
 struct KBlock {
     static constexpr int EB = KindTraits<KIND>::EB;
     static constexpr int AW = (8 * EB) / 4;
     static constexpr int XB = T * EB;
     static constexpr int XW = XB >= 4 ? XB / 4 : 1;
+    static constexpr bool CMP = AMODE == A_COMPACT || AMODE == A_CMAP;   // mask-addressed tiles
 
     const SpmmArgs &a;
     const uint8_t *xh, *vh;
@@ -170,7 +174,7 @@ struct KBlock {
 #pragma unroll
         for (int w = 0; w < AW; ++w) av[w] = 0;
         if (!a_row_ok || kg >= nreal) return;
-        if constexpr (AMODE == A_COMPACT) {
+        if constexpr (CMP) {
             return;  // built by tile_compact, which needs the whole wave
         } else if constexpr (AMODE == A_MAPPED) {
             // the 8 edge ids of row n (one or two vector loads), then the 8
@@ -222,6 +226,45 @@ struct KBlock {
         }
     }
 
+    // The lane's entries of row n in the 8 columns (bit n of m[i]) sit at
+    // consecutive packed positions from idx: values, or (A_CMAP) edge ids
+    // whose values are then gathered, all ids loaded before the gathers.
+    __device__ __forceinline__ void packed(const uint32_t (&m)[8], int64_t idx, uint32_t (&av)[AW]) const
+    {
+        uint32_t e[8];
+        if constexpr (AMODE == A_CMAP) {
+            int id[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                id[i] = -1;
+                if ((m[i] >> n) & 1u) {
+                    id[i] = a.map[idx];
+                    ++idx;
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                e[i] = 0;
+                if (id[i] >= 0) load_bytes<EB>(vh + int64_t(id[i]) * EB, &e[i]);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                e[i] = 0;
+                if ((m[i] >> n) & 1u) {
+                    load_bytes<EB>(a.val + idx * EB, &e[i]);
+                    ++idx;
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if constexpr (EB == 4) av[i] = e[i];
+            else if constexpr (EB == 2) av[i >> 1] |= e[i] << (16 * (i & 1));
+            else av[i >> 2] |= e[i] << (8 * (i & 3));
+        }
+    }
+
     // Compact tiles: every lane of the wave must call this (shuffles).
     __device__ __forceinline__ void tile_compact(int kb, uint32_t (&av)[AW]) const
     {
@@ -248,21 +291,7 @@ struct KBlock {
         const int prefix = (g > 0 ? t0 : 0) + (g > 1 ? t1 : 0) + (g > 2 ? t2 : 0);
         if (kg >= nreal) return;
         int64_t idx = a.bbase[(int64_t(c_begin) + kb) >> 5] + prefix + below;
-        uint32_t e[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            e[i] = 0;
-            if ((m[i] >> n) & 1u) {
-                load_bytes<EB>(a.val + idx * EB, &e[i]);
-                ++idx;
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            if constexpr (EB == 4) av[i] = e[i];
-            else if constexpr (EB == 2) av[i >> 1] |= e[i] << (16 * (i & 1));
-            else av[i >> 2] |= e[i] << (8 * (i & 3));
-        }
+        packed(m, idx, av);
     }
 
     // Compact tiles, prefetch form: the lane's 8 row masks of k-block kb and
@@ -295,21 +324,7 @@ struct KBlock {
         const int prefix = (g > 0 ? t0 : 0) + (g > 1 ? t1 : 0) + (g > 2 ? t2 : 0);
         if (kg >= nreal) return;
         int64_t idx = base + prefix + below;
-        uint32_t e[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            e[i] = 0;
-            if ((m[i] >> n) & 1u) {
-                load_bytes<EB>(a.val + idx * EB, &e[i]);
-                ++idx;
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            if constexpr (EB == 4) av[i] = e[i];
-            else if constexpr (EB == 2) av[i >> 1] |= e[i] << (16 * (i & 1));
-            else av[i >> 2] |= e[i] << (8 * (i & 3));
-        }
+        packed(m, idx, av);
     }
 
     // Attention variant: the A operand is rebuilt from node terms where it is
@@ -370,7 +385,7 @@ struct KBlock {
 
     __device__ __forceinline__ void load_a(int kb, const int (&cid)[8], uint32_t (&av)[AW]) const
     {
-        if constexpr (AMODE == A_COMPACT) tile_compact(kb, av);
+        if constexpr (CMP) tile_compact(kb, av);
         else if constexpr (AMODE == A_ATTN) tile_attn(kb, cid, av);
         else tile(kb, av);
     }
@@ -443,7 +458,9 @@ __global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
         // Workgroups are dispatched round-robin over the 8 XCDs.  Give every
         // XCD a single feature slice (slices divides 8) so each XCD's L2 caches
         // one slice of the gathered rows: xcd = b % 8 -> slice xcd % S; the
-        // other bits of b and the wave pick the (part, head).
+        // other bits of b and the wave pick the (part, head).  The heads of a
+        // part stay in one workgroup: they share column ids and edge ids
+        // through the CU's L1 (spreading heads over XCDs measured slower).
         const int64_t b = blockIdx.x;
         const int xcd = int(b & 7), S = a.slices, groups = 8 / S;
         slice = xcd % S;
@@ -459,6 +476,8 @@ __global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
         head = hs / a.slices;
         slice = hs - head * a.slices;
     }
+    // a negative window marks a padding part of a fixed-capacity structure
+    if (a.win[part] < 0) return;
 
     // the wave of a split window's first part (head 0, slice 0) lists the
     // window for the combine; list order varies, each window's sum does not
@@ -512,7 +531,7 @@ __global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
             k.rows(cid, xv);
             KB::mma(av, xv, acc);
         }
-    } else if constexpr (PIPE == 1 && AMODE == A_COMPACT) {
+    } else if constexpr (PIPE == 1 && KB::CMP) {
         // column ids, row masks and value offset of the next k-block load
         // while this one multiplies
         int cid[8];
@@ -548,7 +567,7 @@ __global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
 #pragma unroll
             for (int i = 0; i < 8; ++i) cid[i] = nid[i];
         }
-    } else if constexpr (AMODE == A_COMPACT) {
+    } else if constexpr (KB::CMP) {
         // two k-blocks per iteration; the next pair's column ids, row masks
         // and value offsets load while this pair multiplies
         int c0[8], c1[8];
@@ -707,6 +726,13 @@ int pipe_of(int flags, int eb, bool mapped)
     }
 }
 
+// One feature slice per XCD (see the kernel's task mapping) when the slice
+// count is above 1 and divides 8, unless the caller turns it off.
+int xcd_mapping(int flags, int slices)
+{
+    return !(flags & ATTGNN_SPMM_NO_XCD_SLICES) && slices > 1 && 8 % slices == 0;
+}
+
 // Features per lane: the widest power of two dividing dim_n that keeps a lane
 // vector <= 16 bytes, narrowed while half of it still covers dim_n.
 int pick_t(int64_t dim_n, int eb)
@@ -779,6 +805,7 @@ hipError_t run_spmm(int kind, int amode, int t, const SpmmArgs &a, bool covered,
 #define ATTGNN_BY_AMODE(K)                                                                              \
     (amode == A_MAPPED ? launch_main<K, A_MAPPED>(t, b, s)                                          \
                        : amode == A_COMPACT ? launch_main<K, A_COMPACT>(t, b, s)                    \
+                       : amode == A_CMAP ? launch_main<K, A_CMAP>(t, b, s)                          \
                                             : amode == A_ATTN ? launch_main<K, A_ATTN>(t, b, s)     \
                                                               : launch_main<K, A_DENSE>(t, b, s))
     switch (kind) {
@@ -861,20 +888,22 @@ extern "C" int attgnn_spmm_block(const int32_t *rp, const int32_t *col, const vo
     a.slices = int((dim_n + 16 * t - 1) / (16 * t));
     a.heads = 1; a.window = window; a.wide_log2 = wl; a.y_type = y_type;
     a.pipe = pipe_of(flags, eb, false);
-    a.xcd_slices = !(flags & ATTGNN_SPMM_NO_XCD_SLICES) && a.slices > 1 && 8 % a.slices == 0;
+    a.xcd_slices = xcd_mapping(flags, a.slices);
     const hipError_t e = run_spmm(kind, A_DENSE, t, a, (flags & ATTGNN_SPMM_ROWS_COVERED) != 0, s);
     return e == hipSuccess ? ATTGNN_OK : int(e);
 }
 
-extern "C" int attgnn_spmm_edge_map(const int32_t *rp, const int32_t *col, const int32_t *map,
-                                    const void *edge_val, int val_type, int64_t nnz,
-                                    const int32_t *window_of_part, const int32_t *atomic, int64_t parts,
-                                    const void *X, int x_type, int x_layout, int64_t heads, int64_t n_x,
-                                    void *Y, int y_type, int64_t m_ori, int64_t dim_n, int out_layout, int window,
-                                    int wide, int mode, float *workspace, int64_t workspace_elems, int flags,
-                                    attgnn_stream_t stream)
+namespace {
+
+// Edge-value SpMM over an edge-id map (dense map tiles, or compact masks,
+// block offsets and packed ids): argument checks and launch.
+int edge_map_run(int amode, const int32_t *rp, const int32_t *col, const int32_t *map, const uint16_t *masks,
+                 const int64_t *bbase, const void *edge_val, int val_type, int64_t nnz,
+                 const int32_t *window_of_part, const int32_t *atomic, int64_t parts, const void *X, int x_type,
+                 int x_layout, int64_t heads, int64_t n_x, void *Y, int y_type, int64_t m_ori, int64_t dim_n,
+                 int out_layout, int window, int wide, int mode, float *workspace, int64_t workspace_elems,
+                 int flags, hipStream_t s)
 {
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     if (parts < 0 || m_ori < 0 || dim_n <= 0 || n_x < 0 || heads <= 0 || nnz < 0) return ATTGNN_E_ARG;
     if (!(window == 8 || window == 16)) return ATTGNN_E_ARG;
     if (!(out_layout == ATTGNN_LAYOUT_HMD || out_layout == ATTGNN_LAYOUT_MHD)) return ATTGNN_E_ARG;
@@ -886,6 +915,7 @@ extern "C" int attgnn_spmm_edge_map(const int32_t *rp, const int32_t *col, const
     if (!Y || !(y_type == ATTGNN_F32 || y_type == ATTGNN_F16 || y_type == ATTGNN_BF16)) return ATTGNN_E_ARG;
     if (parts > 0 && (!rp || !col || !map || !edge_val || !window_of_part || !atomic || !X || !workspace))
         return ATTGNN_E_ARG;
+    if (amode == A_CMAP && parts > 0 && (!masks || !bbase)) return ATTGNN_E_ARG;
     if (parts > 0 && workspace_elems < heads * attgnn_spmm_workspace_elems(parts, window, dim_n)) return ATTGNN_E_ARG;
     const int eb = elt_bytes(kind);
     const int t = pick_t(dim_n, eb);
@@ -894,7 +924,7 @@ extern "C" int attgnn_spmm_edge_map(const int32_t *rp, const int32_t *col, const
 
     SpmmArgs a{};
     a.rp = rp; a.col = col; a.win = window_of_part; a.atomic = atomic;
-    a.val = static_cast<const uint8_t *>(edge_val); a.map = map;
+    a.val = static_cast<const uint8_t *>(edge_val); a.map = map; a.masks = masks; a.bbase = bbase;
     a.x = static_cast<const uint8_t *>(X);
     a.y = Y; a.partial = workspace;
     a.parts = parts; a.m_ori = m_ori; a.dim_n = dim_n;
@@ -905,9 +935,39 @@ extern "C" int attgnn_spmm_edge_map(const int32_t *rp, const int32_t *col, const
     else { a.y_head_stride = dim_n; a.y_row_stride = heads * dim_n; }
     a.slices = int((dim_n + 16 * t - 1) / (16 * t));
     a.heads = int(heads); a.window = window; a.wide_log2 = wl; a.y_type = y_type; a.pipe = pipe_of(flags, eb, true);
-    a.xcd_slices = a.slices > 1 && 8 % a.slices == 0;
-    const hipError_t e = run_spmm(kind, A_MAPPED, t, a, (flags & ATTGNN_SPMM_ROWS_COVERED) != 0, s);
+    a.xcd_slices = xcd_mapping(flags, a.slices);
+    const hipError_t e = run_spmm(kind, amode, t, a, (flags & ATTGNN_SPMM_ROWS_COVERED) != 0, s);
     return e == hipSuccess ? ATTGNN_OK : int(e);
+}
+
+}  // namespace
+
+extern "C" int attgnn_spmm_edge_map(const int32_t *rp, const int32_t *col, const int32_t *map,
+                                    const void *edge_val, int val_type, int64_t nnz,
+                                    const int32_t *window_of_part, const int32_t *atomic, int64_t parts,
+                                    const void *X, int x_type, int x_layout, int64_t heads, int64_t n_x,
+                                    void *Y, int y_type, int64_t m_ori, int64_t dim_n, int out_layout, int window,
+                                    int wide, int mode, float *workspace, int64_t workspace_elems, int flags,
+                                    attgnn_stream_t stream)
+{
+    return edge_map_run(A_MAPPED, rp, col, map, nullptr, nullptr, edge_val, val_type, nnz, window_of_part, atomic,
+                        parts, X, x_type, x_layout, heads, n_x, Y, y_type, m_ori, dim_n, out_layout, window, wide,
+                        mode, workspace, workspace_elems, flags, reinterpret_cast<hipStream_t>(stream));
+}
+
+extern "C" int attgnn_spmm_edge_map_compact(const int32_t *rp, const int32_t *col, const uint16_t *masks,
+                                            const int64_t *block_base, const int32_t *ids, const void *edge_val,
+                                            int val_type, int64_t nnz, const int32_t *window_of_part,
+                                            const int32_t *atomic, int64_t parts, const void *X, int x_type,
+                                            int x_layout, int64_t heads, int64_t n_x, void *Y, int y_type,
+                                            int64_t m_ori, int64_t dim_n, int out_layout, int window, int mode,
+                                            float *workspace, int64_t workspace_elems, int flags,
+                                            attgnn_stream_t stream)
+{
+    if (window > 16) return ATTGNN_E_ARG;
+    return edge_map_run(A_CMAP, rp, col, ids, masks, block_base, edge_val, val_type, nnz, window_of_part, atomic,
+                        parts, X, x_type, x_layout, heads, n_x, Y, y_type, m_ori, dim_n, out_layout, window, 32,
+                        mode, workspace, workspace_elems, flags, reinterpret_cast<hipStream_t>(stream));
 }
 
 extern "C" int attgnn_spmm_attention(const int32_t *rp, const int32_t *col, const int32_t *map,
@@ -948,7 +1008,7 @@ extern "C" int attgnn_spmm_attention(const int32_t *rp, const int32_t *col, cons
     else { a.y_head_stride = dim_n; a.y_row_stride = heads * dim_n; }
     a.slices = int((dim_n + 16 * t - 1) / (16 * t));
     a.heads = int(heads); a.window = window; a.wide_log2 = wl; a.y_type = y_type; a.pipe = pipe_of(flags, eb, true);
-    a.xcd_slices = a.slices > 1 && 8 % a.slices == 0;
+    a.xcd_slices = xcd_mapping(flags, a.slices);
     a.r1 = terms->row_logit; a.r2 = terms->row_offset; a.c1 = terms->col_logit; a.c2 = terms->col_offset;
     a.lg_hs = terms->logit_head_stride; a.lg_ns = terms->logit_node_stride;
     a.of_hs = terms->offset_head_stride; a.of_ns = terms->offset_node_stride;
@@ -1001,7 +1061,7 @@ extern "C" int attgnn_spmm_compact(const int32_t *rp, const int32_t *col, const 
     a.slices = int((dim_n + 16 * t - 1) / (16 * t));
     a.heads = 1; a.window = window; a.wide_log2 = 5; a.y_type = y_type;
     a.pipe = pipe_of(flags, eb, false);
-    a.xcd_slices = !(flags & ATTGNN_SPMM_NO_XCD_SLICES) && a.slices > 1 && 8 % a.slices == 0;
+    a.xcd_slices = xcd_mapping(flags, a.slices);
     const hipError_t e = run_spmm(kind, A_COMPACT, t, a, (flags & ATTGNN_SPMM_ROWS_COVERED) != 0, s);
     return e == hipSuccess ? ATTGNN_OK : int(e);
 }

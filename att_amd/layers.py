@@ -280,6 +280,9 @@ class GATGraph:
         self.fwd = [b[k].to(device) for k in ("rp", "col", "val", "window", "atomic")]
         # compact edge list of A (block offsets, edge ids, packed positions) for the SDDMM
         self.edges = [b[k].to(device) for k in ("block_base", "values", "positions")] if wide == 32 else None
+        # compact edge-id maps (row masks, block offsets, edge ids) of A and A^T for the aggregations
+        self.cmap_fwd = [b[k].to(device) for k in ("masks", "block_base", "values")] if wide == 32 else None
+        self.cmap_bwd = None
         if symmetric:
             self.bwd = [b["rp"], b["col"], b["map_t"], b["window"], b["atomic"]]
             self.t_rowptr = self.rowptr
@@ -298,6 +301,7 @@ class GATGraph:
         if symmetric or build == "device":
             self.row_pointers, self.column_index, self.degrees, self.window_t, self.automic = self.fwd
             self.degrees_trans = self.bwd[2]
+            self._compact_bwd()
             return
         t, order = transpose_csr(padded)
         bt = ops.blocks_build(t.rowptr32(), t.indices, "map", None, torch.int32, window, wide, self.part_size,
@@ -312,6 +316,26 @@ class GATGraph:
         # reference names
         self.row_pointers, self.column_index, self.degrees, self.window_t, self.automic = self.fwd
         self.degrees_trans = self.bwd[2]
+        self._compact_bwd()
+
+    def _compact_bwd(self):
+        if self.wide == 32:
+            c = ops.compact_tiles_device(self.bwd[2], self.bwd[1].numel(), self.window, positions=False)
+            self.cmap_bwd = [c["masks"], c["block_base"], c["values"]]
+
+    def aggregate(self, p, X, transpose=False, m_ori=None, out_dtype=torch.float32, compact=True):
+        """Y[i] = sum_j P[i, j] X[j] over A (``transpose``: over A^T) with P's
+        entries p[h, e] in A's CSR edge order; X and Y [N, H, D].  Runs over
+        the compact edge-id lists when the graph has them (bitwise equal to
+        the dense edge-id map tiles, which ``compact=False`` selects)."""
+        rp, col, emap, win, at = self.bwd if transpose else self.fwd
+        cm = self.cmap_bwd if transpose else self.cmap_fwd
+        m = self.num_nodes_ori if m_ori is None else m_ori
+        if compact and cm is not None:
+            return ops.spmm_edge_map_compact(rp, col, *cm, p, win, at, X, "mhd", self.window, m_ori=m,
+                                             x_layout="mhd", out_dtype=out_dtype, rows_covered=True)
+        return ops.spmm_edge_map(rp, col, emap, p, win, at, X, "mhd", self.window, self.wide, m_ori=m,
+                                 x_layout="mhd", out_dtype=out_dtype, rows_covered=True)
 
     def segments(self):
         """Row segments of A and of A^T for the per-edge attention kernels
@@ -379,8 +403,7 @@ class GATAttention(torch.autograd.Function):
         else:
             p, lse = ops.edge_softmax_fwd(g.rowptr, g.col, s0, s1, slope, drop_p, seed, p_dtype=Xt.dtype,
                                           s_layout="nh", long_rows=long_rows)
-            h = ops.spmm_edge_map(rp, col, emap, p, win, at, Xt, "mhd", g.window, g.wide, m_ori=g.num_nodes_ori,
-                                  x_layout="mhd", out_dtype=Xt.dtype, rows_covered=True)
+            h = g.aggregate(p, Xt, out_dtype=Xt.dtype)
         ctx.save_for_backward(Xs, Xt, a0, a1, s0, s1, lse, p)
         ctx.graph, ctx.slope, ctx.drop_p, ctx.seed, ctx.fused = g, slope, drop_p, seed, fused
         ctx.shared = Xs is Xt
@@ -400,8 +423,7 @@ class GATAttention(torch.autograd.Function):
                                      slope=ctx.slope, drop_p=ctx.drop_p, seed=ctx.seed, m_ori=n, x_layout="mhd",
                                      rows_covered=True)
         else:
-            dXt = ops.spmm_edge_map(rp, col, emap_t, p, win, at, dh, "mhd", g.window, g.wide, m_ori=n,
-                                    x_layout="mhd", rows_covered=True)
+            dXt = g.aggregate(p, dh, transpose=True, m_ori=n)
         # dp[(i,j)] = <dh[i], Xt[j]>  (SDDMM over A)
         dp = g.sddmm(dh, Xt)
         # row and column sums of dL/dz (columns through the transpose permutation)
@@ -428,9 +450,7 @@ class GATv2Attention(torch.autograd.Function):
         g = graph
         segs, _ = g.segments()
         z, lse, p = ops.gatv2_fwd(g.rowptr, g.col, segs, X0, X1, a, slope, drop_p, seed, p_dtype=X1.dtype)
-        rp, col, emap, win, at = g.fwd
-        h = ops.spmm_edge_map(rp, col, emap, p, win, at, X1, "mhd", g.window, g.wide, m_ori=g.num_nodes_ori,
-                              x_layout="mhd", out_dtype=X1.dtype, rows_covered=True)
+        h = g.aggregate(p, X1, out_dtype=X1.dtype)
         ctx.save_for_backward(X0, X1, a, z, lse, p)
         ctx.graph, ctx.slope, ctx.drop_p, ctx.seed = g, slope, drop_p, seed
         return h
@@ -441,10 +461,8 @@ class GATv2Attention(torch.autograd.Function):
         g = ctx.graph
         n = X1.shape[0]
         dh = dh.to(X1.dtype).contiguous()
-        rp, col, emap_t, win, at = g.bwd
         # messages: dX1[j] = sum_i p[(i,j)] dh[i]  (A^T edge-map SpMM)
-        base = ops.spmm_edge_map(rp, col, emap_t, p, win, at, dh, "mhd", g.window, g.wide, m_ori=n,
-                                 x_layout="mhd", rows_covered=True)
+        base = g.aggregate(p, dh, transpose=True, m_ori=n)
         dp = g.sddmm(dh, X1)            # dL/dp[(i,j)] = <dh[i], X1[j]>
         segs, t_segs = g.segments()
         _, dX0, dX1, da = ops.gatv2_bwd(g.rowptr, g.col, segs, g.t_col, g.t_perm, t_segs, X0, X1, a, ctx.slope, z,

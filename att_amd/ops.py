@@ -198,24 +198,37 @@ def blocks_build_device(row, col, src, src_values, out_dtype, window, wide, part
     if compact:
         if wide != 32 or window > 16:
             raise ValueError("compact tiles need wide = 32 and window <= 16")
-        nb = n_col // 32
-        masks = torch.empty(max(n_col, 8), dtype=torch.int16, device=dev)
-        base = torch.empty(nb + 1, dtype=torch.int64, device=dev)
-        cws_n = L.attgnn_blocks_compact_workspace_bytes(n_col)
-        if cws_n > ws_n:
-            ws = torch.empty(cws_n, dtype=torch.uint8, device=dev)
-            ws_n = cws_n
-        check(L.attgnn_blocks_compact_count_device(_ptr(out["val"]), _TYPE_OF[out_dtype], n_col, window,
-                                                   _ptr(masks), _ptr(base), _ptr(ws), ws_n, st),
-              "attgnn_blocks_compact_count_device")
-        nv = int(base[nb].item())
-        values = torch.empty(max(nv, 1), dtype=out_dtype, device=dev)
-        positions = torch.empty(max(nv, 1), dtype=torch.int16, device=dev)
-        check(L.attgnn_blocks_compact_fill_device(_ptr(out["val"]), _TYPE_OF[out_dtype], n_col, window,
-                                                  _ptr(masks), _ptr(base), _ptr(values), _ptr(positions), st),
-              "attgnn_blocks_compact_fill_device")
-        out.update(masks=masks, block_base=base, values=values, positions=positions)
+        out.update(compact_tiles_device(out["val"], n_col, window, workspace=ws))
     del keep
+    return out
+
+
+def compact_tiles_device(val, n_col, window, workspace=None, positions=True):
+    """Compact form of n_col condensed columns of window x 32 tiles on the
+    device (value tiles, or an int32 edge-id map): masks, block_base, values
+    and (optionally) positions, as attgnn_blocks_compact_export lays them out.
+    Reads the value count back once."""
+    dev = _need_gpu("compact_tiles_device", val)
+    L = lib()
+    st = _stream(dev)
+    nb = n_col // 32
+    masks = torch.empty(max(n_col, 8), dtype=torch.int16, device=dev)
+    base = torch.empty(nb + 1, dtype=torch.int64, device=dev)
+    ws_n = L.attgnn_blocks_compact_workspace_bytes(n_col)
+    ws = workspace if workspace is not None and workspace.numel() >= ws_n else \
+        torch.empty(ws_n, dtype=torch.uint8, device=dev)
+    check(L.attgnn_blocks_compact_count_device(_ptr(val), _TYPE_OF[val.dtype], n_col, window, _ptr(masks),
+                                               _ptr(base), _ptr(ws), ws.numel(), st),
+          "attgnn_blocks_compact_count_device")
+    nv = int(base[nb].item())
+    values = torch.empty(max(nv, 1), dtype=val.dtype, device=dev)
+    pos = torch.empty(max(nv, 1), dtype=torch.int16, device=dev) if positions else None
+    check(L.attgnn_blocks_compact_fill_device(_ptr(val), _TYPE_OF[val.dtype], n_col, window, _ptr(masks), _ptr(base),
+                                              _ptr(values), _ptr(pos), st),
+          "attgnn_blocks_compact_fill_device")
+    out = dict(masks=masks, block_base=base, values=values)
+    if positions:
+        out["positions"] = pos
     return out
 
 
@@ -275,9 +288,16 @@ def map_remap(emap, remap):
 _PIPES = {None: 0, "default": 0, "one": 1 << 4, "pair": 2 << 4, "none": 3 << 4}
 
 
+def _spmm_flags(rows_covered, pipe, xcd):
+    """SpMM flags: rows covered, k-loop schedule (``pipe``) and the per-XCD
+    feature-lane mapping (``xcd=False`` turns it off; results are the same)."""
+    return ((_lib.SPMM_ROWS_COVERED if rows_covered else 0) | _PIPES[pipe]
+            | (0 if xcd else _lib.SPMM_NO_XCD_SLICES))
+
+
 def spmm_block(rp, col, val, window_of_part, atomic, X, m_ori, window, wide, mode=None,
                row_scale=None, col_scale=None, out=None, out_dtype=torch.float32, rows_covered=False,
-               workspace=None, pipe=None):
+               workspace=None, pipe=None, xcd=True):
     """Y[m_ori, D] = A . X over a window-block structure (all on the GPU).
 
     FP32 accumulation; Y is written in ``out_dtype`` (fp32 / fp16 / bf16).
@@ -300,7 +320,7 @@ def spmm_block(rp, col, val, window_of_part, atomic, X, m_ori, window, wide, mod
         ws = workspace
     else:
         ws = torch.empty(max(ws_n, 1), dtype=torch.float32, device=dev)
-    flags = (_lib.SPMM_ROWS_COVERED if rows_covered else 0) | _PIPES[pipe]
+    flags = _spmm_flags(rows_covered, pipe, xcd)
     rc = lib().attgnn_spmm_block(_ptr(rp), _ptr(col), _ptr(val), _TYPE_OF[val.dtype], _ptr(window_of_part),
                                  _ptr(atomic), parts, _ptr(X), _TYPE_OF[X.dtype], n_x, _ptr(y), _TYPE_OF[y.dtype],
                                  m_ori, dim, window, wide, _mode(mode, X.dtype), _ptr(row_scale), _ptr(col_scale), _ptr(ws),
@@ -311,7 +331,7 @@ def spmm_block(rp, col, val, window_of_part, atomic, X, m_ori, window, wide, mod
 
 def spmm_compact(rp, col, masks, block_base, values, window_of_part, atomic, X, m_ori, window, mode=None,
                  row_scale=None, col_scale=None, out_dtype=torch.float32, rows_covered=False, workspace=None,
-                 pipe=None):
+                 pipe=None, xcd=True):
     """attgnn_spmm_compact: spmm_block over compact tiles (wide = 32)."""
     dev = _need_gpu("spmm_compact", rp, col, masks, block_base, values, window_of_part, atomic, X, row_scale,
                     col_scale)
@@ -322,7 +342,7 @@ def spmm_compact(rp, col, masks, block_base, values, window_of_part, atomic, X, 
     ws_n = lib().attgnn_spmm_workspace_elems(parts, window, dim) if parts else 0
     ws = workspace if workspace is not None and workspace.numel() >= ws_n else \
         torch.empty(max(ws_n, 1), dtype=torch.float32, device=dev)
-    flags = (_lib.SPMM_ROWS_COVERED if rows_covered else 0) | _PIPES[pipe]
+    flags = _spmm_flags(rows_covered, pipe, xcd)
     rc = lib().attgnn_spmm_compact(_ptr(rp), _ptr(col), _ptr(masks), _ptr(block_base), _ptr(values),
                                    _TYPE_OF[values.dtype], _ptr(window_of_part), _ptr(atomic), parts, _ptr(X),
                                    _TYPE_OF[X.dtype], n_x, _ptr(y), _TYPE_OF[y.dtype], m_ori, dim, window,
@@ -333,13 +353,34 @@ def spmm_compact(rp, col, masks, block_base, values, window_of_part, atomic, X, 
 
 
 def spmm_edge_map(rp, col, emap, edge_val, window_of_part, atomic, X, out_layout, window, wide,
-                  m_ori=None, mode=None, x_layout="hmd", out_dtype=torch.float32, rows_covered=False, pipe=None):
+                  m_ori=None, mode=None, x_layout="hmd", out_dtype=torch.float32, rows_covered=False, pipe=None,
+                  workspace=None, xcd=True):
     """Edge-map SpMM.  X [H, N, D] ("hmd") or [N, H, D] ("mhd"); edge_val [H, nnz]
     (or [nnz]); Y [H, M, D] ("hmd") or [M, H, D] ("mhd") in ``out_dtype`` (FP32
     accumulation).  ``rows_covered``: every window up to M has a part, so Y
     needs no zero-fill."""
     dev = _need_gpu("spmm_edge_map", rp, col, emap, edge_val, window_of_part, atomic, X)
     _need_i32("spmm_edge_map", rp, col, emap, window_of_part, atomic)
+    return _edge_map(None, rp, col, emap, edge_val, window_of_part, atomic, X, out_layout, window, wide, m_ori, mode,
+                     x_layout, out_dtype, rows_covered, pipe, workspace, xcd, dev)
+
+
+def spmm_edge_map_compact(rp, col, masks, block_base, ids, edge_val, window_of_part, atomic, X, out_layout, window,
+                          m_ori=None, mode=None, x_layout="hmd", out_dtype=torch.float32, rows_covered=False,
+                          pipe=None, workspace=None, xcd=True):
+    """:func:`spmm_edge_map` over the compact form of the map (wide = 32):
+    ``masks``, ``block_base`` and the packed edge ``ids`` of a map built with
+    ``compact=True``.  Bitwise equal to :func:`spmm_edge_map`."""
+    dev = _need_gpu("spmm_edge_map_compact", rp, col, masks, block_base, ids, edge_val, window_of_part, atomic, X)
+    _need_i32("spmm_edge_map_compact", rp, col, ids, window_of_part, atomic)
+    if masks.dtype not in (torch.int16, torch.uint16) or block_base.dtype != torch.int64:
+        raise TypeError("masks must be 16-bit and block_base int64")
+    return _edge_map((masks, block_base), rp, col, ids, edge_val, window_of_part, atomic, X, out_layout, window, 32,
+                     m_ori, mode, x_layout, out_dtype, rows_covered, pipe, workspace, xcd, dev)
+
+
+def _edge_map(compact, rp, col, emap, edge_val, window_of_part, atomic, X, out_layout, window, wide, m_ori, mode,
+              x_layout, out_dtype, rows_covered, pipe, workspace, xcd, dev):
     if X.dim() == 2:
         X = X.unsqueeze(0 if x_layout == "hmd" else 1)
     if x_layout == "hmd":
@@ -358,17 +399,23 @@ def spmm_edge_map(rp, col, emap, edge_val, window_of_part, atomic, X, out_layout
     y = torch.empty(shape, dtype=out_dtype, device=dev)
     parts = window_of_part.numel()
     ws_n = heads * lib().attgnn_spmm_workspace_elems(parts, window, dim) if parts else 0
-    ws = torch.empty(max(ws_n, 1), dtype=torch.float32, device=dev)
+    ws = workspace if workspace is not None and workspace.numel() >= ws_n else \
+        torch.empty(max(ws_n, 1), dtype=torch.float32, device=dev)
     if ev.dtype != X.dtype:
         ev = ev.to(X.dtype)
     ev = ev.contiguous()
     xl = {"hmd": _lib.LAYOUT_HMD, "mhd": _lib.LAYOUT_MHD}[x_layout]
-    rc = lib().attgnn_spmm_edge_map(_ptr(rp), _ptr(col), _ptr(emap), _ptr(ev), _TYPE_OF[ev.dtype], nnz,
-                                    _ptr(window_of_part), _ptr(atomic), parts, _ptr(X), _TYPE_OF[X.dtype], xl,
-                                    heads, n_x, _ptr(y), _TYPE_OF[out_dtype], m, dim, layout, window, wide,
-                                    _mode(mode, X.dtype), _ptr(ws), ws_n,
-                                    (_lib.SPMM_ROWS_COVERED if rows_covered else 0) | _PIPES[pipe], _stream(dev))
-    check(rc, "attgnn_spmm_edge_map")
+    flags = _spmm_flags(rows_covered, pipe, xcd)
+    tail = (_ptr(ev), _TYPE_OF[ev.dtype], nnz, _ptr(window_of_part), _ptr(atomic), parts, _ptr(X), _TYPE_OF[X.dtype],
+            xl, heads, n_x, _ptr(y), _TYPE_OF[out_dtype], m, dim, layout, window)
+    if compact is None:
+        rc = lib().attgnn_spmm_edge_map(_ptr(rp), _ptr(col), _ptr(emap), *tail, wide, _mode(mode, X.dtype), _ptr(ws),
+                                        ws_n, flags, _stream(dev))
+        check(rc, "attgnn_spmm_edge_map")
+    else:
+        rc = lib().attgnn_spmm_edge_map_compact(_ptr(rp), _ptr(col), _ptr(compact[0]), _ptr(compact[1]), _ptr(emap),
+                                                *tail, _mode(mode, X.dtype), _ptr(ws), ws_n, flags, _stream(dev))
+        check(rc, "attgnn_spmm_edge_map_compact")
     return y
 
 
@@ -388,7 +435,7 @@ def _node_terms(t, heads, layout, name):
 def spmm_attention(rp, col, emap, window_of_part, atomic, X, out_layout, window, wide, row_logit, col_logit,
                    row_offset=None, col_offset=None, slope=0.2, drop_p=0.0, seed=0, m_ori=None, mode=None,
                    x_layout="hmd", out_dtype=torch.float32, rows_covered=False, pipe=None, logit_layout="nh",
-                   offset_layout="hn"):
+                   offset_layout="hn", xcd=True):
     """Attention aggregation with the probabilities rebuilt in the kernel:
     entry (r, c) = exp(LeakyReLU(row_logit[r] + col_logit[c]) - row_offset[r] - col_offset[c]) with the edge
     softmax's dropout; equal bit for bit to :func:`spmm_edge_map` over the p that
@@ -429,7 +476,7 @@ def spmm_attention(rp, col, emap, window_of_part, atomic, X, out_layout, window,
     rc = lib().attgnn_spmm_attention(_ptr(rp), _ptr(col), _ptr(emap), _ptr(window_of_part), _ptr(atomic), parts,
                                      ctypes.byref(terms), _ptr(X), _TYPE_OF[X.dtype], xl, heads, n_x, _ptr(y),
                                      _TYPE_OF[out_dtype], m, dim, layout, window, wide, _mode(mode, X.dtype),
-                                     _ptr(ws), ws_n, (_lib.SPMM_ROWS_COVERED if rows_covered else 0) | _PIPES[pipe],
+                                     _ptr(ws), ws_n, _spmm_flags(rows_covered, pipe, xcd),
                                      _stream(dev))
     check(rc, "attgnn_spmm_attention")
     return y

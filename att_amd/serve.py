@@ -239,7 +239,7 @@ class Bucket:
             parts = sum(g.parts_gat for g in graphs)
             cols = sum(g.cols_gat for g in graphs)
             nnz = sum(g.nnz for g in graphs)
-        return (rows <= self.rows_cap - WINDOW and parts <= self.parts_cap and cols <= self.cols_cap
+        return (rows <= self.rows_cap and parts <= self.parts_cap and cols <= self.cols_cap
                 and nnz <= self.nnz_cap and len(graphs) <= self.max_graphs)
 
     # -- segment table -----------------------------------------------------------
@@ -289,7 +289,7 @@ class Bucket:
         tot = sizes.sum(axis=0)
         node, part, col, edge = (int(x) for x in tot)
         tails = [(self.rp, 2 * part, 2 * (self.parts_cap - part) + 1, col),
-                 (self.win, part, self.parts_cap - part, self.rows_cap // WINDOW - 1),
+                 (self.win, part, self.parts_cap - part, -1),
                  (self.atomic, part, self.parts_cap - part, 0)]
         if self.kind == "gat":
             tails.append((self.rowptr, node, self.rows_cap + 1 - node, edge))

@@ -239,6 +239,11 @@ int attgnn_blocks_compact_fill_device(const void *val, int tile_type, int64_t n_
  * (partial sums of split windows, reduced in part order); required when
  * parts > 0.  Every row of Y is written (rows no part covers are zero).
  * X must be aligned to its lane vector (<= 16 bytes) and Y to 16 bytes.
+ * A part whose window_of_part is negative is padding: it does no work and
+ * stores nothing (atomic must be 0 there).  Buffers sized for a capacity and
+ * filled per batch -- a captured graph replayed over batches of different
+ * sizes -- mark their unused parts so.  The same holds for every
+ * attgnn_spmm_* entry point below.  No counterpart in the reference.
  */
 int attgnn_spmm_block(const int32_t *rp, const int32_t *col, const void *val, int tile_type,
                       const int32_t *window_of_part, const int32_t *atomic, int64_t parts,
@@ -312,6 +317,24 @@ int attgnn_spmm_edge_map(const int32_t *rp, const int32_t *col, const int32_t *m
                          void *Y, int y_type, int64_t m_ori, int64_t dim_n, int out_layout, int window,
                          int wide, int mode, float *workspace, int64_t workspace_elems, int flags,
                          attgnn_stream_t stream);
+
+/*
+ * attgnn_spmm_edge_map over the compact form of the edge-id map (wide = 32,
+ * window <= 16): masks, block_base and ids as written by
+ * attgnn_blocks_compact_export / attgnn_blocks_compact_fill_device for the
+ * map (tile type I32).  A block reads its row masks and its edges' ids -- a
+ * few hundred bytes on sparse windows -- instead of the window x 32 int32 map
+ * tile.  Same arguments otherwise; bitwise equal to attgnn_spmm_edge_map.
+ * Replaces the same reference entry points as attgnn_spmm_edge_map (the
+ * compact form itself has no counterpart in the reference).
+ */
+int attgnn_spmm_edge_map_compact(const int32_t *rp, const int32_t *col, const uint16_t *masks,
+                                 const int64_t *block_base, const int32_t *ids, const void *edge_val,
+                                 int val_type, int64_t nnz, const int32_t *window_of_part, const int32_t *atomic,
+                                 int64_t parts, const void *X, int x_type, int x_layout, int64_t heads,
+                                 int64_t n_x, void *Y, int y_type, int64_t m_ori, int64_t dim_n, int out_layout,
+                                 int window, int mode, float *workspace, int64_t workspace_elems, int flags,
+                                 attgnn_stream_t stream);
 
 /*
  * Attention aggregation with the probabilities rebuilt in the kernel:

@@ -125,3 +125,20 @@ def test_layer_fused_equals_stored(cuda, graph, kind):
         results.append([out.detach().clone()] + [p.grad.clone() for p in layer.parameters()])
     for a, b in zip(*results):
         assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("build", ["host", "device"])
+def test_graph_aggregate_compact_equals_dense(cuda, graph, build):
+    """GATGraph.aggregate over the compact edge-id lists of A and of A^T
+    equals the dense edge-id map tiles bit for bit (host and device builds,
+    symmetric and non-symmetric patterns)."""
+    _, g = graph
+    gg = GATGraph(g, cuda, build=build)
+    assert gg.cmap_fwd is not None and gg.cmap_bwd is not None
+    gen = torch.Generator(device="cpu").manual_seed(4)
+    p = torch.rand(4, g.nnz, generator=gen).bfloat16().to(cuda)
+    X = torch.randn(g.n, 4, 32, generator=gen).bfloat16().to(cuda)
+    for transpose in (False, True):
+        ref = gg.aggregate(p, X, transpose=transpose, compact=False)
+        got = gg.aggregate(p, X, transpose=transpose)
+        assert torch.equal(ref, got), transpose

@@ -158,3 +158,131 @@ def test_pipeline_schedules_bitwise_equal(cuda, window, wide, ps, dim):
     es = [ops.spmm_edge_map(margs[0], margs[1], margs[2], ev, margs[3], margs[4], Xh, "mhd", window, wide,
                             x_layout="mhd", pipe=p) for p in ("none", "one", "pair")]
     assert all(torch.equal(es[0], e) for e in es[1:])
+
+
+def _with_padding_parts(rp, win, atomic, every):
+    """Insert a padding part (window -1, atomic 0, no columns) before every
+    `every`-th part that starts a new window, and three after the last."""
+    rp, win, atomic = rp.cpu().numpy(), win.cpu().numpy(), atomic.cpu().numpy()
+    nrp, nwin, nat = [rp[0]], [], []
+    starts = 0
+    for p in range(win.size):
+        if p == 0 or win[p] != win[p - 1]:
+            starts += 1
+            if starts % every == 0:
+                nrp += [rp[2 * p], rp[2 * p]]
+                nwin.append(-1)
+                nat.append(0)
+        nrp += [rp[2 * p + 1], rp[2 * p + 2]]
+        nwin.append(win[p])
+        nat.append(atomic[p])
+    for _ in range(3):
+        nrp += [rp[-1], rp[-1]]
+        nwin.append(-1)
+        nat.append(0)
+    return [torch.tensor(np.array(x, np.int32)) for x in (nrp, nwin, nat)]
+
+
+@pytest.mark.parametrize("ps", [1, 2, 1000])
+def test_padding_parts_skipped(cuda, ps):
+    """Parts with a negative window (the padding of capacity-sized buffers) do
+    no work and store nothing: dense, compact and edge-map results with
+    padding parts between windows and after the last equal the unpadded ones
+    bit for bit, with and without rows_covered, split windows included."""
+    rng = np.random.default_rng(ps)
+    n = 1024
+    rp, col = rand_csr(rng, n, 50, hub_rows=3)
+    vals = rng.standard_normal(col.size).astype(np.float32)
+    b = ops.blocks_build(rp, col, "edge_f32", vals, torch.bfloat16, 16, 32, ps, emit_empty=True, compact=True)
+    if ps == 1:
+        assert int(b["atomic"].sum()) > 0
+    prp, pwin, pat = (t.to(cuda) for t in _with_padding_parts(b["rp"], b["window"], b["atomic"], 3))
+    assert int((pwin < 0).sum()) > 3
+    brp, bcol, bval, bwin, bat = (b[k].to(cuda) for k in ("rp", "col", "val", "window", "atomic"))
+    X = torch.from_numpy(rng.uniform(-1, 1, (n, 96)).astype(np.float32)).bfloat16().to(cuda)
+    for cov in (False, True):
+        ref = ops.spmm_block(brp, bcol, bval, bwin, bat, X, n, 16, 32, rows_covered=cov)
+        got = ops.spmm_block(prp, bcol, bval, pwin, pat, X, n, 16, 32, rows_covered=cov)
+        assert torch.equal(ref, got), ("dense", cov)
+        cargs = [b[k].to(cuda) for k in ("masks", "block_base", "values")]
+        got = ops.spmm_compact(prp, bcol, *cargs, pwin, pat, X, n, 16, rows_covered=cov)
+        assert torch.equal(ref, got), ("compact", cov)
+    m = ops.blocks_build(rp, col, "map", None, torch.int32, 16, 32, ps, emit_empty=True)
+    mrp, mcol, mmap, mwin, mat = (m[k].to(cuda) for k in ("rp", "col", "val", "window", "atomic"))
+    qrp, qwin, qat = (t.to(cuda) for t in _with_padding_parts(m["rp"], m["window"], m["atomic"], 2))
+    ev = torch.rand(4, col.size, device=cuda).bfloat16()
+    Xh = torch.randn(n, 4, 32, device=cuda).bfloat16()
+    ref = ops.spmm_edge_map(mrp, mcol, mmap, ev, mwin, mat, Xh, "mhd", 16, 32, x_layout="mhd", rows_covered=True)
+    got = ops.spmm_edge_map(qrp, mcol, mmap, ev, qwin, qat, Xh, "mhd", 16, 32, x_layout="mhd", rows_covered=True)
+    assert torch.equal(ref, got)
+
+
+@pytest.mark.parametrize("window,ps", [(16, 1), (16, 4), (8, 2), (16, 1000)])
+@pytest.mark.parametrize("heads,dim", [(1, 64), (4, 32), (2, 48), (8, 8)])
+def test_edge_map_compact_equals_dense_map(cuda, window, ps, heads, dim):
+    """The compact edge map (row masks + packed edge ids) only changes where a
+    lane finds its edge ids: results equal the dense-map SpMM bit for bit for
+    every arithmetic kind, both layouts, every k-loop schedule, split windows
+    and padding parts, on host- and device-built maps."""
+    rng = np.random.default_rng(window * 1000 + ps * 10 + heads + dim)
+    n = 900
+    rp, col = rand_csr(rng, n, 45, hub_rows=3)
+    m = ops.blocks_build(rp, col, "map", None, torch.int32, window, 32, ps, emit_empty=True, compact=True)
+    md = ops.blocks_build_device(torch.from_numpy(rp).to(cuda), torch.from_numpy(col).to(cuda), "map", None,
+                                 torch.int32, window, 32, ps, emit_empty=True, compact=True)
+    for k in ("rp", "col", "val", "window", "atomic", "masks", "block_base"):
+        assert torch.equal(m[k], md[k].cpu()), k
+    nv = int(m["block_base"][-1])
+    assert torch.equal(m["values"][:nv], md["values"][:nv].cpu())
+    rp_, col_, emap, win, at, masks, base, ids = (m[k].to(cuda) for k in ("rp", "col", "val", "window", "atomic",
+                                                                          "masks", "block_base", "values"))
+    for dt, mode in ((torch.bfloat16, None), (torch.float16, None), (torch.float32, "f32"),
+                     (torch.float32, "bf16x3")):
+        ev = torch.rand(heads, col.size, device=cuda).to(dt)
+        for xl, ol in (("mhd", "mhd"), ("hmd", "hmd"), ("mhd", "hmd")):
+            Xs = (n, heads, dim) if xl == "mhd" else (heads, n, dim)
+            X = torch.randn(*Xs, device=cuda).to(dt)
+            ref = ops.spmm_edge_map(rp_, col_, emap, ev, win, at, X, ol, window, 32, mode=mode, x_layout=xl)
+            for pipe in (None, "none", "one", "pair"):
+                got = ops.spmm_edge_map_compact(rp_, col_, masks, base, ids, ev, win, at, X, ol, window, mode=mode,
+                                                x_layout=xl, pipe=pipe)
+                assert torch.equal(ref, got), (dt, mode, xl, ol, pipe)
+    prp, pwin, pat = (t.to(cuda) for t in _with_padding_parts(m["rp"], m["window"], m["atomic"], 2))
+    ev = torch.rand(heads, col.size, device=cuda).bfloat16()
+    X = torch.randn(n, heads, dim, device=cuda).bfloat16()
+    ref = ops.spmm_edge_map(rp_, col_, emap, ev, win, at, X, "mhd", window, 32, x_layout="mhd", out_dtype=torch.bfloat16)
+    got = ops.spmm_edge_map_compact(prp, col_, masks, base, ids, ev, pwin, pat, X, "mhd", window, x_layout="mhd",
+                                    out_dtype=torch.bfloat16)
+    assert torch.equal(ref, got)
+
+
+@pytest.mark.parametrize("heads,dim", [(1, 256), (2, 256), (4, 32), (8, 8), (3, 128), (1, 512)])
+def test_xcd_slice_mapping_bitwise_equal(cuda, heads, dim):
+    """The slice-per-XCD task mapping only reorders waves: every SpMM variant
+    gives the same bits with it on and off, for single and multi-slice heads."""
+    rng = np.random.default_rng(heads * 100 + dim)
+    n = 800
+    rp, col = rand_csr(rng, n, 50, hub_rows=2)
+    m = ops.blocks_build(rp, col, "map", None, torch.int32, 16, 32, 2, emit_empty=True, compact=True)
+    a = [m[k].to(cuda) for k in ("rp", "col", "val", "window", "atomic", "masks", "block_base", "values")]
+    ev = torch.rand(heads, col.size, device=cuda).bfloat16()
+    X = torch.randn(n, heads, dim, device=cuda).bfloat16()
+    for xl, ol in (("mhd", "mhd"), ("mhd", "hmd")):
+        on = ops.spmm_edge_map(*a[:3], ev, a[3], a[4], X, ol, 16, 32, x_layout=xl)
+        off = ops.spmm_edge_map(*a[:3], ev, a[3], a[4], X, ol, 16, 32, x_layout=xl, xcd=False)
+        assert torch.equal(on, off), (xl, ol)
+        c = ops.spmm_edge_map_compact(a[0], a[1], a[5], a[6], a[7], ev, a[3], a[4], X, ol, 16, x_layout=xl)
+        assert torch.equal(on, c), (xl, ol)
+    s0 = torch.randn(n, heads, device=cuda)
+    s1 = torch.randn(n, heads, device=cuda)
+    fa = [ops.spmm_attention(*a[:5], X, "mhd", 16, 32, s0, s1, x_layout="mhd", xcd=x) for x in (True, False)]
+    assert torch.equal(fa[0], fa[1])
+    if heads == 1:
+        vals = rng.standard_normal(col.size).astype(np.float32)
+        b = ops.blocks_build(rp, col, "edge_f32", vals, torch.float16, 16, 32, 2, emit_empty=True, compact=True)
+        args = [b[k].to(cuda) for k in ("rp", "col", "val", "window", "atomic")]
+        X2 = X.reshape(n, dim).half()
+        on = ops.spmm_block(*args, X2, n, 16, 32)
+        assert torch.equal(on, ops.spmm_block(*args, X2, n, 16, 32, xcd=False))
+        cargs = [b[k].to(cuda) for k in ("masks", "block_base", "values")]
+        assert torch.equal(on, ops.spmm_compact(args[0], args[1], *cargs, args[3], args[4], X2, n, 16, xcd=False))
