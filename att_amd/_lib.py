@@ -101,7 +101,7 @@ _SIGNATURES = {
     "attgnn_csr_transpose_workspace_bytes": (_i64, [_i64, _i64, _i64]),
     "attgnn_csr_transpose_device": (_int, [_vp, _vp, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _vp]),
     "attgnn_map_remap_device": (_int, [_vp, _i64, _vp, _i64, _vp, _vp]),
-    "attgnn_pack_segments": (_int, [_vp, _i64, _vp]),
+    "attgnn_pack_segments": (_int, [_vp, _i64, _vp, _i64, _i64, _vp]),
     "attgnn_pack_segment_bytes": (_i64, []),
 }
 

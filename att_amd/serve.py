@@ -3,8 +3,10 @@
 No counterpart in the reference, which times one graph per launch
 (end2end_ori/gcn_final/mgat_csr/test_mgat_csr.py:12-32).  Pieces:
 
-* ``CachedGraph``  a request graph's window-block structures (GCN tiles of
-  A_hat; GAT edge-id maps and CSR rows) and its features, resident in HBM.
+* ``CachedGraph``  a request graph's window-block structures (GCN: dense
+  tiles of A_hat; GAT: compact edge-id lists -- row masks per condensed
+  column, block offsets, packed edge ids -- and CSR rows) and its features,
+  resident in HBM.
 * ``GraphCache``   LRU cache of CachedGraph under an HBM byte budget; graphs
   used by in-flight batches are never evicted; a graph that cannot be made to
   fit waits in the queue (admission by HBM budget).
@@ -36,9 +38,10 @@ from . import ops
 WINDOW, WIDE = 16, 32
 F_IN, GCN_HID, GCN_OUT = 128, 128, 16
 GAT_HEADS, GAT_DIM = 4, 32
-SEG_DTYPE = np.dtype([("src", "<u8"), ("dst", "<u8"), ("count", "<i8"), ("offset", "<i4"), ("kind", "<i4")])
-KIND_IDX, KIND_ALL, KIND_BYTES = 0, 1, 2
-CHUNK_BYTES = 32 << 10       # a pack-table entry covers at most this much (one workgroup)
+SEG_DTYPE = np.dtype([("src", "<u8"), ("dst", "<u8"), ("count", "<i8"), ("first", "<i8"), ("offset", "<i4"),
+                      ("kind", "<i4")])
+KIND_IDX, KIND_ALL, KIND_BYTES, KIND_I64 = 0, 1, 2, 3
+CHUNK_BYTES = 32 << 10       # bytes one packing workgroup copies
 SERVE_PART_SIZE = 8          # blocks per part: short per-wave chains for small graphs
 
 
@@ -61,6 +64,20 @@ class Models:
 # per-graph structures
 # ----------------------------------------------------------------------------
 
+def _dense_parts(b):
+    return {k: b[k] for k in ("rp", "col", "val", "window", "atomic")}
+
+
+def _compact_parts(b):
+    """The arrays a cached graph keeps of a compact build: the part table, the
+    condensed columns, the row masks, the first n_blocks block offsets and the
+    packed values (exactly n_values of them)."""
+    nb = b["col"].numel() // WIDE
+    nv = int(b["block_base"][nb])
+    return {"rp": b["rp"], "col": b["col"], "window": b["window"], "atomic": b["atomic"],
+            "masks": b["masks"][: b["col"].numel()], "bbase": b["block_base"][:nb], "values": b["values"][:nv]}
+
+
 def prepare_host(csr, features, part_size=SERVE_PART_SIZE):
     """Host-side preprocessing of one graph for both models (numpy / C++)."""
     p = csr.padded(WINDOW)
@@ -70,14 +87,13 @@ def prepare_host(csr, features, part_size=SERVE_PART_SIZE):
     gb = ops.blocks_build(p.rowptr32(), p.indices, "edge_f32", vals, torch.bfloat16, WINDOW, WIDE, part_size,
                           emit_empty=True)
     mb = ops.blocks_build(p.rowptr32(), p.indices, "map", None, torch.int32, WINDOW, WIDE, part_size,
-                          emit_empty=True)
+                          emit_empty=True, compact=True)
     X = np.zeros((p.n, F_IN), np.float32)
     X[: csr.n] = features
     return {"n": csr.n, "n_pad": p.n, "nnz": csr.nnz,
-            "gcn": {k: gb[k] for k in ("rp", "col", "val", "window", "atomic")},
+            "gcn": _dense_parts(gb),
             "gat": {"rowptr": torch.from_numpy(p.rowptr32()), "ccol": torch.from_numpy(p.indices),
-                    "rp": mb["rp"], "col": mb["col"], "map": mb["val"], "window": mb["window"],
-                    "atomic": mb["atomic"]},
+                    **_compact_parts(mb)},
             "X": torch.from_numpy(X).to(torch.bfloat16)}
 
 
@@ -98,12 +114,10 @@ def prepare_device(csr, features, device, stream, part_size=SERVE_PART_SIZE):
         gb = ops.blocks_build_device(rowptr, ccol, "edge_f32", up(torch.from_numpy(vals)), torch.bfloat16, WINDOW,
                                      WIDE, part_size, emit_empty=True)
         mb = ops.blocks_build_device(rowptr, ccol, "map", None, torch.int32, WINDOW, WIDE, part_size,
-                                     emit_empty=True)
+                                     emit_empty=True, compact=True)
         Xd = up(torch.from_numpy(X).to(torch.bfloat16))
-    return {"n": csr.n, "n_pad": p.n, "nnz": csr.nnz,
-            "gcn": {k: gb[k] for k in ("rp", "col", "val", "window", "atomic")},
-            "gat": {"rowptr": rowptr, "ccol": ccol, "rp": mb["rp"], "col": mb["col"], "map": mb["val"],
-                    "window": mb["window"], "atomic": mb["atomic"]},
+        gcn, gat = _dense_parts(gb), _compact_parts(mb)
+    return {"n": csr.n, "n_pad": p.n, "nnz": csr.nnz, "gcn": gcn, "gat": {"rowptr": rowptr, "ccol": ccol, **gat},
             "X": Xd}
 
 
@@ -119,24 +133,43 @@ class CachedGraph:
             self.X = up(host["X"])
         self.ready = torch.cuda.Event()
         self.ready.record(stream)
-        self.parts_gcn = self.gcn["window"].numel()
-        self.cols_gcn = self.gcn["col"].numel()
-        self.parts_gat = self.gat["window"].numel()
-        self.cols_gat = self.gat["col"].numel()
+        self.describe()
+        self.pins = 0  # in-flight batches using this graph
+
+    def describe(self):
+        """Batch needs, pack-table descriptions and vectors of the resident arrays."""
+        vals_gat = -(-self.gat["values"].numel() // 8) * 8   # packed 16-byte aligned in a batch
+        # what the graph adds to a batch of each kind: rows, parts, condensed
+        # columns, edges / packed values (checked against a bucket's capacity)
+        self.need = {"gcn": (self.n_pad, self.gcn["window"].numel(), self.gcn["col"].numel(), 0),
+                     "gat": (self.n_pad, self.gat["window"].numel(), self.gat["col"].numel(),
+                             max(self.nnz, vals_gat))}
         self.nbytes = sum(t.numel() * t.element_size() for t in
                           list(self.gcn.values()) + list(self.gat.values()) + [self.X])
-        self.pins = 0  # in-flight batches using this graph
         # (device pointer, positions) of every array the pack table copies, a
-        # position being one part (rp: 2 ints), column, row, or edge; rp and the
-        # CSR row pointers drop their final entry (the next graph's start)
-        P, Pg = self.parts_gcn, self.parts_gat
+        # position being one part (rp: 2 ints), column, 32-column block, packed
+        # value, row, or edge; rp and the CSR row pointers drop their final
+        # entry (the next graph's start), block offsets keep one per block
+        P, Pg = self.need["gcn"][1], self.need["gat"][1]
         common = lambda s_, p_: {"rp": (s_["rp"].data_ptr(), p_), "col": (s_["col"].data_ptr(), s_["col"].numel()),
                                  "window": (s_["window"].data_ptr(), p_), "atomic": (s_["atomic"].data_ptr(), p_),
                                  "X": (self.X.data_ptr(), self.n_pad)}
-        self.info = {"gcn": {**common(self.gcn, P), "val": (self.gcn["val"].data_ptr(), self.cols_gcn)},
-                     "gat": {**common(self.gat, Pg), "map": (self.gat["map"].data_ptr(), self.cols_gat),
+        self.info = {"gcn": {**common(self.gcn, P), "val": (self.gcn["val"].data_ptr(), self.gcn["col"].numel())},
+                     "gat": {**common(self.gat, Pg),
+                             **{k: (self.gat[k].data_ptr(), self.gat[k].numel()) for k in ("masks", "bbase", "values")},
                              "rowptr": (self.gat["rowptr"].data_ptr(), self.n_pad),
                              "ccol": (self.gat["ccol"].data_ptr(), self.nnz)}}
+        # the same as vectors, stacked over a batch's graphs by the pack table:
+        # source pointers and counts per array, and the graph's running-total
+        # sizes (rows, parts, columns, edges, packed values rounded up to 8 so
+        # each graph's values start 16-byte aligned)
+        self.vec = {}
+        for kind, info in self.info.items():
+            nv = -(-info.get("values", (0, 0))[1] // 8) * 8
+            self.vec[kind] = (np.array([v[0] for v in info.values()], np.uint64),
+                              np.array([v[1] for v in info.values()], np.int64),
+                              np.array([self.n_pad, info["window"][1], info["col"][1], self.nnz, nv], np.int64))
+        self.uploaded = False  # the upload on the copy stream is known to have finished
 
 
 class GraphCache:
@@ -179,7 +212,10 @@ def estimate_bytes(csr):
     """Device bytes of one cached graph (used for admission before building it)."""
     n_pad = -(-csr.n // WINDOW) * WINDOW
     cols = csr.nnz + 2 * n_pad  # distinct columns per window plus padding, upper estimate
-    return cols * WINDOW * (2 + 4) + cols * 8 + csr.nnz * 4 + n_pad * (F_IN * 2 + 4) + (n_pad // WINDOW) * 32
+    # GCN: columns and dense tiles; GAT: columns (4 B), masks (2 B), block
+    # offsets (8 B per 32 columns), edge ids and CSR columns (4 B each per edge)
+    return (cols * (4 + WINDOW * 2) + cols * (4 + 2 + 1) + csr.nnz * (4 + 4) + n_pad * (F_IN * 2 + 4)
+            + (n_pad // WINDOW) * 32 + 64)
 
 
 # ----------------------------------------------------------------------------
@@ -191,145 +227,188 @@ class Bucket:
 
     def __init__(self, kind, rows_cap, device, models, use_graph=True):
         self.kind, self.rows_cap, self.device, self.models = kind, rows_cap, device, models
+        self.use_graph = use_graph
+        self.allocate(device)
+        heads = GAT_HEADS if kind == "gat" else 1
+        dmax = max(GCN_HID, GCN_OUT) if kind == "gcn" else GAT_DIM
+        self.ws = torch.empty(heads * _lib.lib().attgnn_spmm_workspace_elems(self.parts_cap, WINDOW, dmax),
+                              dtype=torch.float32, device=device)
+        self.table_host = self.table_host.pin_memory()
+        self.table_dev = self.table_host.to(device)
+        self.graph = None
+        self.out = None
+        self.done = None  # event of the last batch run on these buffers
+        self.copy_bytes = 0
+
+    def allocate(self, device):
+        """Capacities, batch buffers and the (unpinned) host pack tables."""
+        rows_cap = self.rows_cap
         self.parts_cap = rows_cap // WINDOW * 8
         self.cols_cap = rows_cap * 36
         self.nnz_cap = rows_cap * 34
         self.max_graphs = min(rows_cap // 112, 512)
-        self.use_graph = use_graph
         i32 = dict(dtype=torch.int32, device=device)
         self.rp = torch.zeros(2 * self.parts_cap + 1, **i32)
         self.col = torch.full((self.cols_cap,), -1, **i32)
         self.win = torch.zeros(self.parts_cap, **i32)
         self.atomic = torch.zeros(self.parts_cap, **i32)
         self.X = torch.zeros(rows_cap, F_IN, dtype=torch.bfloat16, device=device)
-        if kind == "gcn":
+        if self.kind == "gcn":
             self.val = torch.zeros(self.cols_cap * WINDOW, dtype=torch.bfloat16, device=device)
-            dmax = max(GCN_HID, GCN_OUT)
         else:
-            self.map = torch.full((self.cols_cap * WINDOW,), -1, **i32)
+            # compact edge-id lists: a row mask per condensed column, an offset
+            # per 32-column block, the edge ids (one per edge)
+            self.masks = torch.zeros(self.cols_cap, dtype=torch.int16, device=device)
+            self.bbase = torch.zeros(self.cols_cap // WIDE + 1, dtype=torch.int64, device=device)
+            self.values = torch.zeros(self.nnz_cap, **i32)
             self.rowptr = torch.zeros(rows_cap + 1, **i32)
             self.ccol = torch.zeros(self.nnz_cap, **i32)
-            dmax = GAT_DIM
-        heads = GAT_HEADS if kind == "gat" else 1
-        self.ws = torch.empty(heads * _lib.lib().attgnn_spmm_workspace_elems(self.parts_cap, WINDOW, dmax),
-                              dtype=torch.float32, device=device)
+        # pack tables (pinned host copy, device copy): max_segs segment entries,
+        # then the chunk -> segment map of max_chunks int32 (-1 = no chunk)
         buf_bytes = sum(t.numel() * t.element_size() for t in self._buffers())
-        self.max_segs = self.max_graphs * 10 + 10 + buf_bytes // CHUNK_BYTES + 16
-        self.table_host = torch.zeros(self.max_segs * SEG_DTYPE.itemsize, dtype=torch.uint8).pin_memory()
-        self.table_dev = torch.zeros_like(self.table_host, device=device)
-        self.graph = None
-        self.out = None
-        self.done = None  # event of the last batch run on these buffers
-        self.used = 0     # table entries written by the previous batch
-        self.copy_bytes = 0
+        self.max_segs = self.max_graphs * 12 + 16
+        self.max_chunks = buf_bytes // CHUNK_BYTES + self.max_segs + 1
+        self.map_at = self.max_segs * SEG_DTYPE.itemsize
+        self.table_host = torch.zeros(self.map_at + 4 * self.max_chunks, dtype=torch.uint8)
+        self.table_host[self.map_at:].view(torch.int32).fill_(-1)
+        self.used = 0     # chunk-map entries written by the previous batch
 
     def _buffers(self):
         bufs = [self.rp, self.col, self.win, self.atomic, self.X]
-        bufs += [self.val] if self.kind == "gcn" else [self.map, self.rowptr, self.ccol]
+        bufs += [self.val] if self.kind == "gcn" else [self.masks, self.bbase, self.values, self.rowptr, self.ccol]
         return bufs
+
+    def nbytes(self):
+        return sum(t.numel() * t.element_size() for t in self._buffers())
 
     # -- capacity checks -------------------------------------------------------
     def fits(self, graphs):
-        rows = sum(g.n_pad for g in graphs)
-        if self.kind == "gcn":
-            parts = sum(g.parts_gcn for g in graphs)
-            cols = sum(g.cols_gcn for g in graphs)
-            nnz = 0
-        else:
-            parts = sum(g.parts_gat for g in graphs)
-            cols = sum(g.cols_gat for g in graphs)
-            nnz = sum(g.nnz for g in graphs)
-        return (rows <= self.rows_cap and parts <= self.parts_cap and cols <= self.cols_cap
-                and nnz <= self.nnz_cap and len(graphs) <= self.max_graphs)
+        tot = [0, 0, 0, 0]
+        for g in graphs:
+            tot = [a + b for a, b in zip(tot, g.need[self.kind])]
+        return self.fits_totals(tot, len(graphs))
+
+    def fits_totals(self, tot, count):
+        """Whether a batch of `count` graphs whose needs sum to `tot` (rows,
+        parts, columns, edges / values) fits this bucket."""
+        return (tot[0] <= self.rows_cap and tot[1] <= self.parts_cap and tot[2] <= self.cols_cap
+                and tot[3] <= self.nnz_cap and count <= self.max_graphs)
 
     # -- segment table -----------------------------------------------------------
     # Segment types: (source array, destination buffer, destination position
     # (a running total), items per position, offset source, table kind).
-    # Running totals per graph: node (rows), part, col (condensed columns), edge.
+    # Running totals per graph: node (rows), part, col (condensed columns),
+    # edge, val (packed values); blk (32-column blocks) is col / 32.
     def _types(self):
-        W = WINDOW
         t = [("rp", self.rp, "part", 2, "col", KIND_ALL),
              ("col", self.col, "col", 1, "node", KIND_IDX),
              ("window", self.win, "part", 1, "win", KIND_ALL),
              ("atomic", self.atomic, "part", 1, None, KIND_ALL),
              ("X", self.X, "node", F_IN, None, KIND_BYTES)]
         if self.kind == "gcn":
-            t.append(("val", self.val, "col", W, None, KIND_BYTES))
+            t.append(("val", self.val, "col", WINDOW, None, KIND_BYTES))
         else:
-            t += [("map", self.map, "col", W, "edge", KIND_IDX),
+            t += [("masks", self.masks, "col", 1, None, KIND_BYTES),
+                  ("bbase", self.bbase, "blk", 1, "val", KIND_I64),
+                  ("values", self.values, "val", 1, "edge", KIND_IDX),
                   ("rowptr", self.rowptr, "node", 1, "edge", KIND_ALL),
                   ("ccol", self.ccol, "edge", 1, "node", KIND_ALL)]
         return t
 
+    # running-total columns of the pack table: node, part, col, edge, val, then
+    # derived blk (col / 32), win (node / 16) and a zero column
+    _COLS = {"node": 0, "part": 1, "col": 2, "edge": 3, "val": 4, "blk": 5, "win": 6, None: 7}
+
+    def _plan(self, names):
+        """Per-array constants of the table for the graphs' array order `names`:
+        destination base, destination position column and bytes per position,
+        count multiplier, offset column and segment kind (cached)."""
+        if getattr(self, "_plan_names", None) != names:
+            types = {t[0]: t for t in self._types()}
+            base, poscol, mult, cmult, offcol, kinds = [], [], [], [], [], []
+            for nm in names:
+                _, buf, pos, per, offk, kind = types[nm]
+                es = buf.element_size()
+                base.append(buf.data_ptr())
+                poscol.append(self._COLS[pos])
+                mult.append(per * es)
+                cmult.append(per * (es if kind == KIND_BYTES else 1))
+                offcol.append(self._COLS[offk])
+                kinds.append(kind)
+            self._plan_names = names
+            self._plan_arrays = (np.array(base, np.uint64), np.array(poscol), np.array(mult, np.int64),
+                                 np.array(cmult, np.int64), np.array(offcol), np.array(kinds, np.int64))
+        return self._plan_arrays
+
     def build_table(self, graphs):
-        """Write the pack table for `graphs` into the pinned host table (numpy,
-        vectorised over graphs and chunks); returns each graph's first row."""
+        """Write the pack tables for `graphs` into the pinned host buffer
+        (numpy, vectorised over graphs and arrays): one entry per segment and
+        the chunk -> segment map; returns each graph's first row."""
+        names = tuple(graphs[0].info[self.kind].keys())
+        base, poscol, mult, cmult, offcol, kinds = self._plan(names)
+        vecs = [g.vec[self.kind] for g in graphs]
+        src = np.stack([v[0] for v in vecs])      # [k, T]
+        cnt = np.stack([v[1] for v in vecs])      # [k, T]
+        sizes = np.stack([v[2] for v in vecs])    # [k, 5]
         k = len(graphs)
-        gi = graphs[0].info[self.kind]
-        names = list(gi.keys())
-        src = np.array([[g.info[self.kind][nm][0] for nm in names] for g in graphs], np.uint64)   # [k, T]
-        cnt = np.array([[g.info[self.kind][nm][1] for nm in names] for g in graphs], np.int64)    # [k, T]
-        sizes = np.array([[g.n_pad, g.info[self.kind]["window"][1], g.info[self.kind]["col"][1], g.nnz]
-                          for g in graphs], np.int64)
-        start = np.vstack([np.zeros((1, 4), np.int64), np.cumsum(sizes, axis=0)[:-1]])             # [k, 4]
-        pos_of = {"node": start[:, 0], "part": start[:, 1], "col": start[:, 2], "edge": start[:, 3]}
-        off_of = {"node": start[:, 0], "col": start[:, 2], "edge": start[:, 3], "win": start[:, 0] // WINDOW,
-                  None: np.zeros(k, np.int64)}
-        types = {t[0]: t for t in self._types()}
-        seg_src, seg_dst, seg_cnt, seg_off, seg_kind = [], [], [], [], []
-        for j, nm in enumerate(names):
-            _, buf, pos, per, offk, kind = types[nm]
-            es = buf.element_size()
-            unit = 1 if kind == KIND_BYTES else 4
-            seg_src.append(src[:, j])
-            seg_dst.append(np.uint64(buf.data_ptr()) + (pos_of[pos] * per * es).astype(np.uint64))
-            seg_cnt.append(cnt[:, j] * per * (es if kind == KIND_BYTES else 1))
-            seg_off.append(off_of[offk])
-            seg_kind.append(np.full(k, kind, np.int64))
-        tot = sizes.sum(axis=0)
-        node, part, col, edge = (int(x) for x in tot)
+        start = np.zeros((k, 8), np.int64)
+        np.cumsum(sizes[:-1], axis=0, out=start[1:, :5])
+        start[:, 5] = start[:, 2] // WIDE
+        start[:, 6] = start[:, 0] // WINDOW
+        tot = start[-1, :5] + sizes[-1]
+        node, part, col, edge = (int(x) for x in tot[:4])
+        # the tails beyond the batch: empty parts (window -1), CSR rows past the last
         tails = [(self.rp, 2 * part, 2 * (self.parts_cap - part) + 1, col),
                  (self.win, part, self.parts_cap - part, -1),
                  (self.atomic, part, self.parts_cap - part, 0)]
         if self.kind == "gat":
             tails.append((self.rowptr, node, self.rows_cap + 1 - node, edge))
-        for buf, at, n, val in tails:
-            seg_src.append(np.zeros(1, np.uint64))
-            seg_dst.append(np.array([buf.data_ptr() + at * 4], np.uint64))
-            seg_cnt.append(np.array([n], np.int64))
-            seg_off.append(np.array([val], np.int64))
-            seg_kind.append(np.array([KIND_ALL], np.int64))
-        s_src, s_dst, s_cnt = np.concatenate(seg_src), np.concatenate(seg_dst), np.concatenate(seg_cnt)
-        s_off, s_kind = np.concatenate(seg_off), np.concatenate(seg_kind)
+        nt = len(tails)
+        s_src = np.concatenate([src.ravel(), np.zeros(nt, np.uint64)])
+        s_dst = np.concatenate([(base[None, :] + (start[:, poscol] * mult[None, :]).astype(np.uint64)).ravel(),
+                                np.array([b.data_ptr() + at * 4 for b, at, _, _ in tails], np.uint64)])
+        s_cnt = np.concatenate([(cnt * cmult[None, :]).ravel(), np.array([n for _, _, n, _ in tails], np.int64)])
+        s_off = np.concatenate([start[:, offcol].ravel(), np.array([v for _, _, _, v in tails], np.int64)])
+        s_kind = np.concatenate([np.broadcast_to(kinds, (k, kinds.size)).ravel(), np.full(nt, KIND_ALL, np.int64)])
         keep = s_cnt > 0
-        s_src, s_dst, s_cnt, s_off, s_kind = s_src[keep], s_dst[keep], s_cnt[keep], s_off[keep], s_kind[keep]
-        # split into chunks of at most CHUNK_BYTES (counts are bytes or int32 items)
-        unit = np.where(s_kind == KIND_BYTES, 1, 4)
-        step = CHUNK_BYTES // unit
-        nch = (s_cnt + step - 1) // step
-        rep = np.repeat(np.arange(s_cnt.size), nch)
-        first = np.cumsum(nch) - nch
-        ci = np.arange(rep.size) - first[rep]
-        byte0 = (ci * step[rep] * unit[rep]).astype(np.uint64)
-        n_entries = rep.size
-        if n_entries > self.max_segs:
+        n_seg = int(keep.sum())
+        if n_seg > self.max_segs:
             raise RuntimeError("segment table overflow")
-        arr = np.zeros(self.max_segs, SEG_DTYPE)
-        arr["src"][:n_entries] = np.where(s_src[rep] != 0, s_src[rep] + byte0, 0)
-        arr["dst"][:n_entries] = s_dst[rep] + byte0
-        arr["count"][:n_entries] = np.minimum(step[rep], s_cnt[rep] - ci * step[rep])
-        arr["offset"][:n_entries] = s_off[rep]
-        arr["kind"][:n_entries] = s_kind[rep]
-        used = max(n_entries, self.used)
-        self.table_host.numpy()[: used * SEG_DTYPE.itemsize] = arr[:used].view(np.uint8)
-        self.used = n_entries
-        self.copy_bytes = used * SEG_DTYPE.itemsize
+        # chunks of at most CHUNK_BYTES (counts are bytes, int32 or int64 items)
+        s_cnt = s_cnt[keep]
+        s_kind = s_kind[keep]
+        unit = np.where(s_kind == KIND_BYTES, 1, np.where(s_kind == KIND_I64, 8, 4))
+        nch = (s_cnt * unit + CHUNK_BYTES - 1) // CHUNK_BYTES
+        first = np.cumsum(nch) - nch
+        n_ch = int(first[-1] + nch[-1]) if n_seg else 0
+        if n_ch > self.max_chunks:
+            raise RuntimeError("chunk table overflow")
+        arr = np.zeros(n_seg, SEG_DTYPE)
+        arr["src"] = s_src[keep]
+        arr["dst"] = s_dst[keep]
+        arr["count"] = s_cnt
+        arr["first"] = first
+        arr["offset"] = s_off[keep]
+        arr["kind"] = s_kind
+        host = self.table_host.numpy()
+        host[: n_seg * SEG_DTYPE.itemsize] = arr.view(np.uint8)
+        # chunk map; entries past this batch's up to the previous batch's count
+        # are reset to -1 so the captured launch skips them
+        used = max(n_ch, self.used)
+        cmap = host[self.map_at: self.map_at + 4 * used].view(np.int32)
+        cmap[:n_ch] = np.repeat(np.arange(n_seg, dtype=np.int32), nch)
+        cmap[n_ch:] = -1
+        self.used = n_ch
+        self.copy_bytes = self.map_at + 4 * used
         return [int(x) for x in start[:, 0]]
 
     # -- packed forward --------------------------------------------------------
     def _forward(self):
         L = _lib.lib()
-        _lib.check(L.attgnn_pack_segments(ops._ptr(self.table_dev), self.max_segs, ops._stream(self.device)),
+        _lib.check(L.attgnn_pack_segments(ops._ptr(self.table_dev), self.max_segs,
+                                          ctypes.c_void_p(self.table_dev.data_ptr() + self.map_at), self.max_chunks,
+                                          CHUNK_BYTES,
+                                          ops._stream(self.device)),
                    "attgnn_pack_segments")
         m = self.models
         if self.kind == "gcn":
@@ -343,8 +422,9 @@ class Bucket:
         s0 = ops.node_scores(xp, m.a0, "mhd", out_layout="nh")
         s1 = ops.node_scores(xp, m.a1, "mhd", out_layout="nh")
         p, _ = ops.edge_softmax_fwd(self.rowptr, self.ccol, s0, s1, 0.2, 0.0, 0, p_dtype=torch.bfloat16, s_layout="nh")
-        return ops.spmm_edge_map(self.rp, self.col, self.map, p, self.win, self.atomic, xp, "mhd", WINDOW, WIDE,
-                                 m_ori=self.rows_cap, x_layout="mhd")
+        return ops.spmm_edge_map_compact(self.rp, self.col, self.masks, self.bbase, self.values, p, self.win,
+                                         self.atomic, xp, "mhd", WINDOW, m_ori=self.rows_cap, x_layout="mhd",
+                                         rows_covered=True, workspace=self.ws)
 
     def launch(self, graphs, stream, timing=None):
         """Pack `graphs` and run the model on `stream`; returns (row offsets, done event)."""
@@ -358,7 +438,11 @@ class Bucket:
             timing["table"] += time.perf_counter() - t1
         with torch.cuda.stream(stream):
             for g in graphs:
-                stream.wait_event(g.ready)
+                if not g.uploaded:
+                    if g.ready.query():
+                        g.uploaded = True
+                    else:
+                        stream.wait_event(g.ready)
             nb = self.copy_bytes
             self.table_dev[:nb].copy_(self.table_host[:nb], non_blocking=True)
             if not self.use_graph:
@@ -405,8 +489,7 @@ class Server:
         self.buckets = {k: [[Bucket(k, c, device, models, use_graph) for _ in range(2)] for c in capacities]
                         for k in ("gcn", "gat")}
         self.turn = collections.defaultdict(int)
-        self.reserved = sum(b.X.numel() * 2 + b.col.numel() * 4 * (WINDOW + 1)
-                            for ks in self.buckets.values() for pair in ks for b in pair)
+        self.reserved = sum(b.nbytes() + b.ws.numel() * 4 for ks in self.buckets.values() for pair in ks for b in pair)
         self.keep_results = keep_results
         self.max_in_flight = max_in_flight
         self.batches = 0
@@ -462,6 +545,7 @@ class Server:
                 kind = min((k for k in queue if queue[k]), key=lambda k: queue[k][0].arrival)
                 batch, graphs = [], []
                 big = self.buckets[kind][-1][0]
+                tot = (0, 0, 0, 0)
                 while queue[kind]:
                     r = queue[kind][0]
                     cg = self._admit(r)
@@ -469,7 +553,8 @@ class Server:
                         if not in_flight and not graphs:
                             raise RuntimeError(f"request {r.rid}: graph does not fit the HBM budget")
                         break
-                    if not big.fits(graphs + [cg]):
+                    nxt = tuple(a + b for a, b in zip(tot, cg.need[kind]))
+                    if not big.fits_totals(nxt, len(graphs) + 1):
                         if graphs:
                             break
                         # a graph larger than every bucket runs alone, unpacked
@@ -482,6 +567,7 @@ class Server:
                     r.t_admit = time.perf_counter() - t0
                     batch.append(r)
                     graphs.append(cg)
+                    tot = nxt
                     cg.pins += 1            # not evictable while in the batch
                 if batch:
                     t1 = time.perf_counter()

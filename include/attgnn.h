@@ -633,19 +633,24 @@ int attgnn_gather_rowsum(const int32_t *rowptr, const int64_t *perm, int64_t n, 
 /* ====================================================================== */
 
 /*
- * Copy nsegs segments described by a device-resident table (attgnn_pack_segment_bytes()
- * bytes per entry: const void *src; void *dst; int64_t count; int32_t offset;
- * int32_t kind), one workgroup per entry (split long segments into chunks;
- * entries with count 0 are skipped).  kind 0: int32 dst[i] = src[i] < 0 ?
+ * Copy segments described by a device-resident table of nsegs entries
+ * (attgnn_pack_segment_bytes() bytes each: const void *src; void *dst;
+ * int64_t count; int64_t first; int32_t offset; int32_t kind), one workgroup
+ * per chunk of chunk_bytes (a multiple of 16): chunk c copies the part of
+ * segment chunk_seg[c] from item (c - first) * chunk_bytes / item size;
+ * chunk_seg[c] = -1 copies nothing.  kind 0: int32 dst[i] = src[i] < 0 ?
  * src[i] : src[i] + offset; kind 1: int32 dst[i] = (src ? src[i] : 0) + offset
- * (src NULL = fill); kind 2: copy count bytes.
+ * (src NULL = fill); kind 2: copy count bytes; kind 3: int64 dst[i] =
+ * (src ? src[i] : 0) + offset (compact block offsets).
  * One launch packs a whole batch of cached graphs into one block-diagonal
- * window-block structure; the table lives in device memory so the launch can
- * be captured in a HIP graph and replayed with new tables.
+ * window-block structure; the tables live in device memory so the launch can
+ * be captured in a HIP graph (fixed n_chunks, unused chunks -1) and replayed
+ * with new tables.
  *
  * No counterpart in the reference (it runs one graph per launch).
  */
-int attgnn_pack_segments(const void *segs, int64_t nsegs, attgnn_stream_t stream);
+int attgnn_pack_segments(const void *segs, int64_t nsegs, const int32_t *chunk_seg, int64_t n_chunks,
+                         int64_t chunk_bytes, attgnn_stream_t stream);
 int64_t attgnn_pack_segment_bytes(void);
 
 #ifdef __cplusplus

@@ -1,10 +1,15 @@
 """CPU: host-side serving logic -- LRU cache with byte budget and pinning,
-request generation, latency statistics, graph pool, and the admission
-estimate bounding the real footprint."""
+request generation, latency statistics, graph pool, the admission estimate
+bounding the real footprint, and the pack tables: applied to CPU buffers they
+produce the window-block structure of the block-diagonal batch graph."""
+import ctypes
+
 import numpy as np
 import pytest
 import torch
 
+from att_amd import graphs as G
+from att_amd import ops
 from att_amd import serve as S
 
 
@@ -61,3 +66,91 @@ def test_graph_pool_distribution_and_estimate():
         assert S.estimate_bytes(csr) >= real              # admission never under-counts
         assert host["n_pad"] % S.WINDOW == 0
         assert torch.equal(host["gat"]["rowptr"][-1], torch.tensor(csr.nnz, dtype=torch.int32))
+
+
+def _apply_tables(b):
+    """A NumPy stand-in for attgnn_pack_segments over CPU buffers: chunk c
+    copies its part of segment chunk_seg[c] (kinds as the C ABI documents)."""
+    host = b.table_host.numpy()
+    segs = host[: b.map_at].view(S.SEG_DTYPE)
+    cmap = host[b.map_at:].view(np.int32)
+    arr = lambda addr, ct, n: np.ctypeslib.as_array((ct * int(n)).from_address(int(addr)))
+    for c in np.nonzero(cmap >= 0)[0]:
+        seg = segs[cmap[c]]
+        unit = {S.KIND_BYTES: 1, S.KIND_I64: 8}.get(int(seg["kind"]), 4)
+        per = S.CHUNK_BYTES // unit
+        i0 = (int(c) - int(seg["first"])) * per
+        n = min(per, int(seg["count"]) - i0)
+        assert 0 <= i0 and n > 0
+        src, dst, off = int(seg["src"]), int(seg["dst"]), int(seg["offset"])
+        if seg["kind"] == S.KIND_BYTES:
+            ctypes.memmove(dst + i0, src + i0, n)
+            continue
+        ct = ctypes.c_int64 if seg["kind"] == S.KIND_I64 else ctypes.c_int32
+        v = arr(src + i0 * unit, ct, n).copy() if src else np.zeros(n, np.int64 if unit == 8 else np.int32)
+        out = v + off
+        if seg["kind"] == S.KIND_IDX:
+            out = np.where(v < 0, v, out)
+        arr(dst + i0 * unit, ct, n)[:] = out
+
+
+@pytest.mark.parametrize("kind", ["gcn", "gat"])
+def test_pack_tables_build_block_diagonal_structure(kind):
+    pool = S.GraphPool(24, seed=3, n_max=6000)
+    graphs = []
+    for key in (0, 3, 5, 7, 11):
+        csr, feats = pool(key)
+        h = S.prepare_host(csr, feats)
+        cg = object.__new__(S.CachedGraph)
+        cg.key, cg.n, cg.n_pad, cg.nnz = key, h["n"], h["n_pad"], h["nnz"]
+        cg.gcn, cg.gat, cg.X = h["gcn"], h["gat"], h["X"]
+        cg.describe()
+        graphs.append(cg)
+    b = object.__new__(S.Bucket)
+    b.kind, b.rows_cap = kind, 1 << 16
+    b.allocate("cpu")
+    assert b.fits(graphs)
+    for batch in (graphs, graphs[1:3]):     # a second, smaller batch rewrites the tails
+        offsets = b.build_table(batch)
+        _apply_tables(b)
+        # the block-diagonal batch graph, built directly
+        pads = [g.n_pad for g in batch]
+        assert offsets == list(np.cumsum([0] + pads[:-1]))
+        ip, idx, vals, node, edge = [0], [], [], 0, 0
+        for g in batch:
+            csr = pool(g.key)[0]
+            p = csr.padded(S.WINDOW)
+            d = G.gcn_norm_degree(p)
+            d[~np.isfinite(d)] = 0.0
+            vals.append(np.asarray(G.gcn_norm_values(p, d), np.float32))
+            ip += list(p.indptr[1:] + edge)
+            idx.append(p.indices + node)
+            node += p.n
+            edge += p.nnz
+        rowptr, col = np.array(ip, np.int32), np.concatenate(idx).astype(np.int32)
+        N, E = node, edge
+        if kind == "gcn":
+            e = ops.blocks_build(rowptr, col, "edge_f32", np.concatenate(vals), torch.bfloat16, S.WINDOW, S.WIDE,
+                                 S.SERVE_PART_SIZE, emit_empty=True)
+        else:
+            e = ops.blocks_build(rowptr, col, "map", None, torch.int32, S.WINDOW, S.WIDE, S.SERVE_PART_SIZE,
+                                 emit_empty=True, compact=True)
+        P, C = e["window"].numel(), e["col"].numel()
+        assert torch.equal(b.rp[: 2 * P + 1], e["rp"]) and bool((b.rp[2 * P + 1:] == C).all())
+        assert torch.equal(b.col[:C], e["col"])
+        assert torch.equal(b.win[:P], e["window"]) and bool((b.win[P:] == -1).all())
+        assert torch.equal(b.atomic[:P], e["atomic"]) and bool((b.atomic[P:] == 0).all())
+        X = torch.cat([g.X for g in batch])
+        assert torch.equal(b.X[:N], X)
+        if kind == "gcn":
+            assert torch.equal(b.val[: C * S.WINDOW].view(torch.int16), e["val"].view(torch.int16))
+            continue
+        assert torch.equal(b.masks[:C], e["masks"][:C])
+        # every block's packed ids (each graph's run of values starts 8-aligned in the batch)
+        nb = C // S.WIDE
+        ebase = e["block_base"].numpy()
+        per = np.diff(ebase)
+        got_ids = np.concatenate([b.values.numpy()[o: o + k] for o, k in zip(b.bbase.numpy()[:nb], per)])
+        assert np.array_equal(got_ids, e["values"].numpy()[: ebase[nb]])
+        assert torch.equal(b.rowptr[: N + 1], torch.from_numpy(rowptr)) and bool((b.rowptr[N + 1:] == E).all())
+        assert torch.equal(b.ccol[:E], torch.from_numpy(col))
