@@ -386,6 +386,27 @@ __device__ __forceinline__ void store_val(void *base, int64_t idx, int out_type,
     }
 }
 
+// Elements [s0, s0 + n) of an array of es-byte elements (es = 2 or 4, the
+// array es-aligned) set to the 32-bit pattern pat (0 or all ones, the same in
+// every byte pair), by the workgroup's THREADS threads: element stores up to
+// the first 16-byte boundary and after the last, 16-byte stores between.
+__device__ __forceinline__ void fill_range(void *base, int64_t s0, int64_t n, int es, uint32_t pat, int t)
+{
+    uint8_t *b = static_cast<uint8_t *>(base) + s0 * es;
+    const int64_t bytes = n * es;
+    int64_t head = int64_t((16 - (reinterpret_cast<uintptr_t>(b) & 15)) & 15);
+    if (head > bytes) head = bytes;
+    const int64_t nv = (bytes - head) >> 4, tail = head + (nv << 4);
+    for (int64_t k = t; k < (head + bytes - tail) / es; k += THREADS) {
+        const int64_t off = k < head / es ? k * es : tail + (k - head / es) * es;
+        if (es == 4) *reinterpret_cast<uint32_t *>(b + off) = pat;
+        else *reinterpret_cast<uint16_t *>(b + off) = uint16_t(pat);
+    }
+    uint4 *v = reinterpret_cast<uint4 *>(b + head);
+    const uint4 p4 = make_uint4(pat, pat, pat, pat);
+    for (int64_t k = t; k < nv; k += THREADS) v[k] = p4;
+}
+
 // First CSR position of column j in row r (-1 if absent).
 __device__ __forceinline__ int32_t find_in_row(const int32_t *row, const int32_t *col, const int32_t *rowflag,
                                                int32_t r, int64_t j)
@@ -428,16 +449,10 @@ __global__ __launch_bounds__(THREADS) void window_fill_kernel(FillArgs a, Ws ws)
         for (int64_t k = t; k < pad; k += THREADS) a.col_out[c0 + k] = k < v ? ws.uniq[lo + k] : -1;
         const int64_t s0 = c0 * W, ns = pad * W;
         if (is_map) {
-            int32_t *m = static_cast<int32_t *>(a.val);
-            for (int64_t k = t; k < ns; k += THREADS) m[s0 + k] = -1;
-            if (a.map_t)
-                for (int64_t k = t; k < ns; k += THREADS) a.map_t[s0 + k] = -1;
-        } else if (a.out_type == ATTGNN_F32) {
-            float *m = static_cast<float *>(a.val);
-            for (int64_t k = t; k < ns; k += THREADS) m[s0 + k] = 0.f;
+            fill_range(a.val, s0, ns, 4, 0xFFFFFFFFu, t);
+            if (a.map_t) fill_range(a.map_t, s0, ns, 4, 0xFFFFFFFFu, t);
         } else {
-            uint16_t *m = static_cast<uint16_t *>(a.val);
-            for (int64_t k = t; k < ns; k += THREADS) m[s0 + k] = 0;
+            fill_range(a.val, s0, ns, a.out_type == ATTGNN_F32 ? 4 : 2, 0u, t);
         }
         // parts (example.cpp:112-152); every part but a window's last spans
         // part_size * wide columns
@@ -510,30 +525,71 @@ __global__ __launch_bounds__(THREADS) void window_fill_kernel(FillArgs a, Ws ws)
 
 // ---- 6/7. compact tiles ----------------------------------------------------
 
-__device__ __forceinline__ bool tile_nz(const void *val, int64_t i, int es, bool is_map)
+// Bits 0, 4, 8, ..., 60 of x packed into bits 0..15.
+__device__ __forceinline__ uint32_t every_fourth_bit(uint64_t x)
 {
-    if (is_map) return static_cast<const int32_t *>(val)[i] >= 0;
-    if (es == 4) return static_cast<const uint32_t *>(val)[i] != 0;
-    return static_cast<const uint16_t *>(val)[i] != 0;
+    x &= 0x1111111111111111ull;
+    x = (x | (x >> 3)) & 0x0303030303030303ull;
+    x = (x | (x >> 6)) & 0x000F000F000F000Full;
+    x = (x | (x >> 12)) & 0x000000FF000000FFull;
+    x = (x | (x >> 24)) & 0xFFFFull;
+    return uint32_t(x);
 }
 
-// One wave per 32-column block: row masks, entry count into counts[blk];
-// counts[nb] = 0 so an exclusive scan of counts[0..nb] ends with the total.
-__global__ __launch_bounds__(THREADS) void compact_count_kernel(const void *val, int es, int is_map, int64_t nb,
-                                                                int W, uint16_t *masks, int64_t *counts)
+// Nonzero pattern of the 8 tile entries at val[i..i+8) (16-byte aligned):
+// bit c set when entry c is an entry of the graph (ES-byte tiles; an int32
+// edge-id map holds ids >= 0, value tiles nonzero bit patterns).
+template <int ES>
+__device__ __forceinline__ uint32_t nz8(const void *val, int64_t i, bool is_map)
 {
-    const int lane = threadIdx.x & 63;
+    uint32_t nz = 0;
+    if constexpr (ES == 2) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(static_cast<const uint16_t *>(val) + i);
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            nz |= (uint32_t((w[k] & 0xFFFFu) != 0) << (2 * k)) | (uint32_t((w[k] >> 16) != 0) << (2 * k + 1));
+    } else {
+        const uint4 *p = reinterpret_cast<const uint4 *>(static_cast<const uint32_t *>(val) + i);
+        const uint4 v0 = p[0], v1 = p[1];
+        const uint32_t w[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+#pragma unroll
+        for (int k = 0; k < 8; ++k) nz |= uint32_t(is_map ? int32_t(w[k]) >= 0 : w[k] != 0) << k;
+    }
+    return nz;
+}
+
+// One wave per 32-column block of window x 32 tiles (16-byte aligned): lane
+// (r, q) = (lane / 4, lane % 4) tests the 8 entries of row r, columns
+// 8q..8q+7, with one or two vector loads; ballot j of "column 8q + j is an
+// entry" holds column 8q+j's row bits at lanes 4r + q, so its row mask is
+// every fourth bit from q, and the block's entry count is the ballots'
+// popcount.  counts[nb] = 0 so an exclusive scan of counts[0..nb] ends with
+// the total.
+template <int ES>
+__global__ __launch_bounds__(THREADS) void compact_count_kernel(const void *val, int is_map, int64_t nb, int W,
+                                                                uint16_t *masks, int64_t *counts)
+{
+    const int lane = threadIdx.x & 63, r = lane >> 2, q = lane & 3;
     const int64_t nw = int64_t(gridDim.x) * (THREADS / 64);
     if (blockIdx.x == 0 && threadIdx.x == 0) counts[nb] = 0;
-    for (int64_t blk = int64_t(blockIdx.x) * (THREADS / 64) + (threadIdx.x >> 6); blk < nb; blk += nw) {
-        uint32_t m = 0;
-        if (lane < 32)
-            for (int r = 0; r < W; ++r)
-                if (tile_nz(val, (blk * W + r) * 32 + lane, es, is_map != 0)) m |= 1u << r;
-        if (lane < 32) masks[blk * 32 + lane] = uint16_t(m);
-        int cnt = __popc(m);
+    const int64_t first = int64_t(blockIdx.x) * (THREADS / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int64_t blk = first; blk < nb; blk += nw) {
+        const uint32_t nz = r < W ? nz8<ES>(val, (blk * W + r) * 32 + 8 * q, is_map != 0) : 0u;
+        uint64_t b[8];
+        int cnt = 0;
 #pragma unroll
-        for (int d = 32; d > 0; d >>= 1) cnt += __shfl_xor(cnt, d, 64);
+        for (int j = 0; j < 8; ++j) {
+            b[j] = __ballot((nz >> j) & 1u);
+            cnt += __popcll(b[j]);
+        }
+        if (lane < 32) {
+            const int j = lane & 7;
+            uint64_t x = b[0];
+#pragma unroll
+            for (int jj = 1; jj < 8; ++jj) x = j == jj ? b[jj] : x;
+            masks[blk * 32 + lane] = uint16_t(every_fourth_bit(x >> (lane >> 3)));
+        }
         if (lane == 0) counts[blk] = cnt;
     }
 }
@@ -556,28 +612,49 @@ size_t compact_scan_temp_bytes(int64_t nb)
 int64_t compact_ws_bytes(int64_t nb) { return (8 * (nb + 1) + 255) / 256 * 256 + int64_t(compact_scan_temp_bytes(nb)) + 256; }
 
 // One wave per block; lane (g, r) = (lane / 16, lane % 16) packs the entries of
-// window row r in the 8-column group g, in column order.
-__global__ __launch_bounds__(THREADS) void compact_pack_kernel(const void *val, int es, int is_map, int64_t nb, int W,
+// window row r in the 8-column group g, in column order: the group's 8 row
+// masks are one 16-byte load, the row's 8 tile entries one (16-bit tiles) or
+// two (32-bit) 16-byte loads, and the entries are picked from registers.
+template <int ES>
+__global__ __launch_bounds__(THREADS) void compact_pack_kernel(const void *val, int64_t nb, int W,
                                                                const uint16_t *masks, const int64_t *block_base,
                                                                void *values, uint16_t *positions)
 {
     const int lane = threadIdx.x & 63;
     const int g = lane >> 4, r = lane & 15;
     const int64_t nw = int64_t(gridDim.x) * (THREADS / 64);
-    for (int64_t blk = int64_t(blockIdx.x) * (THREADS / 64) + (threadIdx.x >> 6); blk < nb; blk += nw) {
-        uint32_t bits = 0;   // bit c-8g set: (r, c) is an entry
-        if (r < W)
-            for (int c = 0; c < 8; ++c) bits |= ((uint32_t(masks[blk * 32 + 8 * g + c]) >> r) & 1u) << c;
+    const int64_t first = int64_t(blockIdx.x) * (THREADS / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int64_t blk = first; blk < nb; blk += nw) {
+        const uint4 mv = *reinterpret_cast<const uint4 *>(masks + blk * 32 + 8 * g);
+        const uint32_t mw[4] = {mv.x, mv.y, mv.z, mv.w};
+        uint32_t bits = 0;   // bit c set: (r, 8g + c) is an entry
+#pragma unroll
+        for (int c = 0; c < 8; ++c) bits |= ((mw[c >> 1] >> (16 * (c & 1) + r)) & 1u) << c;
+        if (r >= W) bits = 0;
         const int cnt = __popc(bits);
         int64_t o = block_base[blk] + (wave_incl_scan(cnt) - cnt);
-        while (bits) {
-            const int c = 8 * g + __ffs(bits) - 1;
-            bits &= bits - 1;
-            const int64_t i = (blk * W + r) * 32 + c;
-            if (es == 4) static_cast<uint32_t *>(values)[o] = static_cast<const uint32_t *>(val)[i];
-            else static_cast<uint16_t *>(values)[o] = static_cast<const uint16_t *>(val)[i];
-            if (positions) positions[o] = uint16_t(r * 32 + c);
-            ++o;
+        if (!bits) continue;   // after the wave-wide scan; nothing else is shared
+        const int64_t i = (blk * W + r) * 32 + 8 * g;
+        uint32_t e[8];
+        if constexpr (ES == 2) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(static_cast<const uint16_t *>(val) + i);
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int c = 0; c < 8; ++c) e[c] = (w[c >> 1] >> (16 * (c & 1))) & 0xFFFFu;
+        } else {
+            const uint4 *p = reinterpret_cast<const uint4 *>(static_cast<const uint32_t *>(val) + i);
+            const uint4 v0 = p[0], v1 = p[1];
+            e[0] = v0.x; e[1] = v0.y; e[2] = v0.z; e[3] = v0.w;
+            e[4] = v1.x; e[5] = v1.y; e[6] = v1.z; e[7] = v1.w;
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            if ((bits >> c) & 1u) {
+                if constexpr (ES == 4) static_cast<uint32_t *>(values)[o] = e[c];
+                else static_cast<uint16_t *>(values)[o] = uint16_t(e[c]);
+                if (positions) positions[o] = uint16_t(r * 32 + 8 * g + c);
+                ++o;
+            }
         }
     }
 }
@@ -677,13 +754,18 @@ extern "C" int attgnn_blocks_compact_count_device(const void *val, int tile_type
         return launch_status();
     }
     if (!val || !masks || !workspace || workspace_bytes < compact_ws_bytes(nb)) return ATTGNN_E_ARG;
+    if ((reinterpret_cast<uintptr_t>(val) | reinterpret_cast<uintptr_t>(masks)) & 15) return ATTGNN_E_ARG;
     // block counts into the workspace, then block_base = their exclusive scan
     // over nb + 1 entries (rocPRIM, device-wide), so block_base[nb] = n_values
     int64_t *counts = static_cast<int64_t *>(workspace);
     uint8_t *temp = static_cast<uint8_t *>(workspace) + (8 * (nb + 1) + 255) / 256 * 256;
     size_t tb = compact_scan_temp_bytes(nb);
-    hipLaunchKernelGGL(compact_count_kernel, dim3(grid_for(nb, THREADS / 64)), dim3(THREADS), 0, s, val, es,
-                       int(tile_type == ATTGNN_I32), nb, window, masks, counts);
+    const dim3 grid(grid_for(nb, THREADS / 64));
+    if (es == 4)
+        hipLaunchKernelGGL(compact_count_kernel<4>, grid, dim3(THREADS), 0, s, val, int(tile_type == ATTGNN_I32), nb,
+                           window, masks, counts);
+    else
+        hipLaunchKernelGGL(compact_count_kernel<2>, grid, dim3(THREADS), 0, s, val, 0, nb, window, masks, counts);
     const hipError_t e = rocprim::exclusive_scan(temp, tb, counts, block_base, int64_t(0), size_t(nb + 1),
                                                  rocprim::plus<int64_t>(), s);
     if (e != hipSuccess) return int(e);
@@ -700,9 +782,15 @@ extern "C" int attgnn_blocks_compact_fill_device(const void *val, int tile_type,
     const int64_t nb = n_cols / 32;
     if (nb == 0) return ATTGNN_OK;
     if (!val || !masks || !values) return ATTGNN_E_ARG;
-    hipLaunchKernelGGL(compact_pack_kernel, dim3(grid_for(nb, THREADS / 64)), dim3(THREADS), 0,
-                       reinterpret_cast<hipStream_t>(stream), val, es, int(tile_type == ATTGNN_I32), nb, window,
-                       masks, block_base, values, positions);
+    if ((reinterpret_cast<uintptr_t>(val) | reinterpret_cast<uintptr_t>(masks)) & 15) return ATTGNN_E_ARG;
+    const dim3 grid(grid_for(nb, THREADS / 64));
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (es == 4)
+        hipLaunchKernelGGL(compact_pack_kernel<4>, grid, dim3(THREADS), 0, s, val, nb, window, masks, block_base,
+                           values, positions);
+    else
+        hipLaunchKernelGGL(compact_pack_kernel<2>, grid, dim3(THREADS), 0, s, val, nb, window, masks, block_base,
+                           values, positions);
     return launch_status();
 }
 

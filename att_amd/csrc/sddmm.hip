@@ -171,7 +171,8 @@ __global__ __launch_bounds__(256) void sddmm_block_kernel(SddmmArgs a)
 {
     constexpr int EB = KIND >= 2 ? 4 : 2;
     const int lane = threadIdx.x & 63;
-    const int64_t task = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    // wave index in a scalar register: the task stays wave-uniform
+    const int64_t task = int64_t(blockIdx.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (task >= a.parts * a.heads) return;
     const int64_t part = task / a.heads;
     const int head = int(task - part * a.heads);
@@ -223,7 +224,7 @@ __global__ __launch_bounds__(256) void sddmm_edges_kernel(SddmmArgs a, const int
     constexpr int EB = KIND >= 2 ? 4 : 2;
     __shared__ float tile[4][16 * 32];
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform part
     const int64_t part = int64_t(blockIdx.x) * 4 + wave;
     if (part >= a.parts) return;
     const int c_begin = a.rp[2 * part];
@@ -289,7 +290,7 @@ __global__ __launch_bounds__(256) void sddmm_edges_batched_kernel(SddmmArgs a, c
     constexpr int HPB = PAIRS / NDC;   // heads per batch
     __shared__ float tile[4][16 * 32];
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform part
     const int64_t part = int64_t(blockIdx.x) * 4 + wave;
     if (part >= a.parts) return;
     const int c_begin = a.rp[2 * part];

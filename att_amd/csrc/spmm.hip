@@ -452,6 +452,9 @@ __global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
     constexpr int EB = KB::EB, AW = KB::AW, XW = KB::XW;
 
     const int lane = threadIdx.x & 63;
+    // the wave's index in the workgroup, in a scalar register: the task
+    // (part, head, slice) and everything derived from it stay wave-uniform
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     int64_t part;
     int head, slice;
     if (a.xcd_slices) {
@@ -464,12 +467,12 @@ __global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
         const int64_t b = blockIdx.x;
         const int xcd = int(b & 7), S = a.slices, groups = 8 / S;
         slice = xcd % S;
-        const int64_t ph = ((b >> 3) * groups + xcd / S) * 4 + (threadIdx.x >> 6);
+        const int64_t ph = ((b >> 3) * groups + xcd / S) * 4 + wave;
         if (ph >= a.parts * a.heads) return;
         part = ph / a.heads;
         head = int(ph - part * a.heads);
     } else {
-        const int64_t task = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+        const int64_t task = int64_t(blockIdx.x) * 4 + wave;
         if (task >= a.parts * a.heads * a.slices) return;
         part = task / (int64_t(a.heads) * a.slices);
         const int hs = int(task - part * a.heads * a.slices);

@@ -206,7 +206,8 @@ int attgnn_map_remap_device(const int32_t *map, int64_t n, const int64_t *remap,
  * n_cols * window tiles of a structure built with wide = 32 (tile_type
  * F16 / BF16 / F32, or I32 for an edge-id map).  count takes a device
  * workspace of attgnn_blocks_compact_workspace_bytes(n_cols) bytes (the block
- * counts and the device-wide scan's storage).
+ * counts and the device-wide scan's storage).  val and masks must be 16-byte
+ * aligned (tiles and masks are read with 16-byte loads); ATTGNN_E_ARG if not.
  *
  * No counterpart in the reference (see attgnn_blocks_compact_sizes).
  */
