@@ -624,15 +624,27 @@ __global__ __launch_bounds__(THREADS) void compact_pack_kernel(const void *val, 
     const int g = lane >> 4, r = lane & 15;
     const int64_t nw = int64_t(gridDim.x) * (THREADS / 64);
     const int64_t first = int64_t(blockIdx.x) * (THREADS / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // the next block's masks and value offset load while this block packs
+    uint4 mv_next = make_uint4(0, 0, 0, 0);
+    int64_t base_next = 0;
+    if (first < nb) {
+        mv_next = *reinterpret_cast<const uint4 *>(masks + first * 32 + 8 * g);
+        base_next = block_base[first];
+    }
     for (int64_t blk = first; blk < nb; blk += nw) {
-        const uint4 mv = *reinterpret_cast<const uint4 *>(masks + blk * 32 + 8 * g);
+        const uint4 mv = mv_next;
+        const int64_t base = base_next;
+        if (blk + nw < nb) {
+            mv_next = *reinterpret_cast<const uint4 *>(masks + (blk + nw) * 32 + 8 * g);
+            base_next = block_base[blk + nw];
+        }
         const uint32_t mw[4] = {mv.x, mv.y, mv.z, mv.w};
         uint32_t bits = 0;   // bit c set: (r, 8g + c) is an entry
 #pragma unroll
         for (int c = 0; c < 8; ++c) bits |= ((mw[c >> 1] >> (16 * (c & 1) + r)) & 1u) << c;
         if (r >= W) bits = 0;
         const int cnt = __popc(bits);
-        int64_t o = block_base[blk] + (wave_incl_scan(cnt) - cnt);
+        int64_t o = base + (wave_incl_scan(cnt) - cnt);
         if (!bits) continue;   // after the wave-wide scan; nothing else is shared
         const int64_t i = (blk * W + r) * 32 + 8 * g;
         uint32_t e[8];

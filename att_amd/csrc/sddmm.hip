@@ -284,7 +284,7 @@ __global__ __launch_bounds__(256) void sddmm_edges_kernel(SddmmArgs a, const int
 constexpr int PAIRS = 8;
 
 template <int KIND, int NDC>
-__global__ __launch_bounds__(256) void sddmm_edges_batched_kernel(SddmmArgs a, const int64_t *bbase,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void sddmm_edges_batched_kernel(SddmmArgs a, const int64_t *bbase,
                                                                   const int32_t *eids, const uint16_t *epos)
 {
     constexpr int HPB = PAIRS / NDC;   // heads per batch
@@ -322,6 +322,21 @@ __global__ __launch_bounds__(256) void sddmm_edges_batched_kernel(SddmmArgs a, c
             be = eids[b0 + lane];
         }
     };
+    // The window rows (A operand) are the same for every k-block: when all
+    // heads fit in one batch they are loaded once for the part.
+    const bool one_batch = a.heads <= HPB;
+    u32x4 ar[PAIRS];
+    auto load_a = [&](int h0, int hn) {
+#pragma unroll
+        for (int p = 0; p < PAIRS; ++p) {
+            const int hl = p / NDC;
+            const int64_t d = (p % NDC) * 32 + 8 * g;
+            ar[p] = u32x4{0, 0, 0, 0};
+            if (hl < hn && d < a.dim && row_ok)
+                ar[p] = *reinterpret_cast<const u32x4 *>(a.a + ((h0 + hl) * a.a_hs + row * a.a_rs + d) * 2);
+        }
+    };
+    if (one_batch && nreal > 0) load_a(0, a.heads);
     if (nreal > 0) meta(0, cid, e0, cnt, pos, eid);
     for (int kb = 0; kb < nreal; kb += 32) {
         int ncid[2] = {-1, -1}, ncnt = 0, npos = -1, neid = -1;
@@ -330,17 +345,15 @@ __global__ __launch_bounds__(256) void sddmm_edges_batched_kernel(SddmmArgs a, c
         if (cnt > 0) {
             for (int h0 = 0; h0 < a.heads; h0 += HPB) {
                 const int hn = a.heads - h0 < HPB ? a.heads - h0 : HPB;
-                u32x4 ar[PAIRS], br[PAIRS][2];
+                if (!one_batch) load_a(h0, hn);
+                u32x4 br[PAIRS][2];
 #pragma unroll
                 for (int p = 0; p < PAIRS; ++p) {
                     const int hl = p / NDC;
                     const int64_t d = (p % NDC) * 32 + 8 * g;
                     const int64_t h = h0 + hl;
-                    ar[p] = u32x4{0, 0, 0, 0};
                     br[p][0] = br[p][1] = u32x4{0, 0, 0, 0};
                     if (hl < hn && d < a.dim) {
-                        if (row_ok)
-                            ar[p] = *reinterpret_cast<const u32x4 *>(a.a + (h * a.a_hs + row * a.a_rs + d) * 2);
 #pragma unroll
                         for (int t = 0; t < 2; ++t)
                             if (cid[t] >= 0)
