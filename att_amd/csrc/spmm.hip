@@ -719,13 +719,20 @@ int kind_of(int mode, int tile_type, int x_type)
 // one block with prefetched column ids for fp32 data, whose operands take
 // twice the registers, and for edge-map tiles (4 heads x 64 on 2.48M entries:
 // 0.271 ms with one block, 0.288 ms with two).
-int pipe_of(int flags, int eb, bool mapped)
+// Default k-loop schedule: two blocks per iteration for 8/16-bit value tiles;
+// for edge-map tiles only in launches of few waves (one head: the deeper
+// prefetch hides more latency per wave), one block otherwise (more waves per
+// SIMD fit).  Measured on 2.48M entries x 64 features, compact edge lists:
+// 1 head 94 -> 86 us with two blocks, 4 heads 222 -> 254 us.
+constexpr int64_t FEW_WAVES = 16384;
+
+int pipe_of(int flags, int eb, bool mapped, int64_t waves = FEW_WAVES)
 {
     switch ((flags >> 4) & 3) {
     case 3: return 0;
     case 2: return 2;
     case 1: return 1;
-    default: return (eb <= 2 && !mapped) ? 2 : 1;
+    default: return (eb <= 2 && (!mapped || waves < FEW_WAVES)) ? 2 : 1;
     }
 }
 
@@ -937,7 +944,8 @@ int edge_map_run(int amode, const int32_t *rp, const int32_t *col, const int32_t
     if (out_layout == ATTGNN_LAYOUT_HMD) { a.y_head_stride = m_ori * dim_n; a.y_row_stride = dim_n; }
     else { a.y_head_stride = dim_n; a.y_row_stride = heads * dim_n; }
     a.slices = int((dim_n + 16 * t - 1) / (16 * t));
-    a.heads = int(heads); a.window = window; a.wide_log2 = wl; a.y_type = y_type; a.pipe = pipe_of(flags, eb, true);
+    a.heads = int(heads); a.window = window; a.wide_log2 = wl; a.y_type = y_type;
+    a.pipe = pipe_of(flags, eb, true, parts * heads * a.slices);
     a.xcd_slices = xcd_mapping(flags, a.slices);
     const hipError_t e = run_spmm(kind, amode, t, a, (flags & ATTGNN_SPMM_ROWS_COVERED) != 0, s);
     return e == hipSuccess ? ATTGNN_OK : int(e);

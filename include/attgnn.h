@@ -260,7 +260,10 @@ enum {
                                      windows*window): Y is not pre-zeroed       */
     /* k-loop schedule (bits 4-5); results are bitwise identical across them */
     ATTGNN_SPMM_PIPE_DEFAULT = 0 << 4, /* PAIR for 8/16-bit dense or compact
-                                          tiles, else ONE                        */
+                                          tiles, and for 8/16-bit edge values of
+                                          an edge-id map when the launch has
+                                          fewer than 16,384 (part, head, slice)
+                                          tasks; else ONE                        */
     ATTGNN_SPMM_PIPE_ONE = 1 << 4,     /* next block's column ids (and compact
                                           masks) prefetched                      */
     ATTGNN_SPMM_PIPE_PAIR = 2 << 4,    /* two blocks per iteration + prefetch    */
