@@ -62,6 +62,9 @@ _SIGNATURES = {
     "attgnn_spmm_edge_map_compact": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _int, _i64, _vp, _vp, _i64, _vp, _int,
                                             _int, _i64, _i64, _vp, _int, _i64, _i64, _int, _int, _int, _vp, _i64,
                                             _int, _vp]),
+    "attgnn_spmm_edge_map_compact_dot": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _int, _i64, _vp, _vp, _i64, _vp,
+                                                 _int, _int, _i64, _i64, _vp, _int, _i64, _i64, _int, _int, _vp,
+                                                 _i64, _vp, _vp, _i64, _int, _vp]),
     "attgnn_spmm_attention": (_int, [_vp, _vp, _vp, _vp, _vp, _i64, _ATTNP, _vp, _int, _int, _i64, _i64, _vp, _int,
                                      _i64, _i64, _int, _int, _int, _int, _vp, _i64, _int, _vp]),
     "attgnn_rowsum_edge_map": (_int, [_vp, _vp, _vp, _i64, _vp, _int, _i64, _i64, _vp, _int, _i64, _int, _int,

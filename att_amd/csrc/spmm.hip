@@ -83,6 +83,11 @@ struct SpmmArgs {
     float slope, keep_scale;
     uint32_t thresh;
     uint64_t seed;
+    // A_CMAP_DOT: dot_out[h*val_head_stride + e] = <dot_x[h, window row], X[h, column]>
+    // for every edge e in the structure; dot_x has X's type and strides
+    const uint8_t *dot_x;
+    float *dot_out;
+    int64_t dot_rows;
 };
 
 // Store n consecutive fp32 values as y_type (F32 / F16 / BF16).
@@ -141,18 +146,36 @@ __device__ __forceinline__ void load_bytes_nt(const uint8_t *p, uint32_t *d)
 
 // A operand: dense value tiles, edge-id map tiles (values gathered by edge
 // id), compact value tiles, map tiles with the attention probabilities
-// rebuilt, compact edge-id lists (values gathered by edge id).
-enum AMode { A_DENSE = 0, A_MAPPED = 1, A_COMPACT = 2, A_ATTN = 3, A_CMAP = 4 };
+// rebuilt, compact edge-id lists (values gathered by edge id), and compact
+// edge-id lists with each slot's dot product of the window row of a second
+// matrix and the gathered row written per edge (A_CMAP_DOT).
+enum AMode { A_DENSE = 0, A_MAPPED = 1, A_COMPACT = 2, A_ATTN = 3, A_CMAP = 4, A_CMAP_DOT = 5 };
 
 // Per-wave state and the three steps of one k-block (32 condensed columns):
 // column ids of the lane's 8 k slots, the A operand (tile row n), the X rows.
+// A_CMAP_DOT staging in LDS, per wave: the k-block's 32 gathered rows (16T
+// 16-bit features each; row c at c * dot_pitch + 16 * ((c >> 3) & 3) bytes, so
+// the four 8-column groups of a lane position write to different banks), then
+// the 16 x 32 dot products in rows of 33 floats.
+template <int T> constexpr int dot_pitch() { return 32 * T + 16; }
+template <int T> constexpr int dot_rows_bytes() { return 32 * dot_pitch<T>() + 48; }
+template <int T> constexpr int dot_wave_bytes() { return dot_rows_bytes<T>() + 16 * 33 * 4; }
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int KIND, int T, int AMODE>
 struct KBlock {
     static constexpr int EB = KindTraits<KIND>::EB;
     static constexpr int AW = (8 * EB) / 4;
     static constexpr int XB = T * EB;
     static constexpr int XW = XB >= 4 ? XB / 4 : 1;
-    static constexpr bool CMP = AMODE == A_COMPACT || AMODE == A_CMAP;   // mask-addressed tiles
+    static constexpr bool CMP = AMODE == A_COMPACT || AMODE == A_CMAP || AMODE == A_CMAP_DOT;   // mask-addressed
+    static constexpr bool IDS = AMODE == A_CMAP || AMODE == A_CMAP_DOT;   // packed entries are edge ids
 
     const SpmmArgs &a;
     const uint8_t *xh, *vh;
@@ -160,6 +183,7 @@ struct KBlock {
     int64_t f0;
     bool f_ok, a_row_ok;
     float r1v, r2v;   // attention variant: the lane's window-row terms
+    mutable int dot_id[8];   // A_CMAP_DOT: edge ids of the lane's row in its 8 columns (-1: none)
 
     __device__ __forceinline__ void cols(int kb, int (&cid)[8]) const
     {
@@ -232,7 +256,7 @@ struct KBlock {
     __device__ __forceinline__ void packed(const uint32_t (&m)[8], int64_t idx, uint32_t (&av)[AW]) const
     {
         uint32_t e[8];
-        if constexpr (AMODE == A_CMAP) {
+        if constexpr (IDS) {
             int id[8];
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
@@ -241,6 +265,10 @@ struct KBlock {
                     id[i] = a.map[idx];
                     ++idx;
                 }
+            }
+            if constexpr (AMODE == A_CMAP_DOT) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) dot_id[i] = id[i];
             }
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
@@ -271,6 +299,10 @@ struct KBlock {
         const int kg = kb + 8 * g;
 #pragma unroll
         for (int w = 0; w < AW; ++w) av[w] = 0;
+        if constexpr (AMODE == A_CMAP_DOT) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) dot_id[i] = -1;
+        }
         uint32_t m[8];
         int tot = 0, below = 0;
         if (kg < plen) {
@@ -312,6 +344,10 @@ struct KBlock {
         const int kg = kb + 8 * g;
 #pragma unroll
         for (int w = 0; w < AW; ++w) av[w] = 0;
+        if constexpr (AMODE == A_CMAP_DOT) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) dot_id[i] = -1;
+        }
         uint32_t m[8];
         int tot = 0, below = 0;
 #pragma unroll
@@ -397,6 +433,48 @@ struct KBlock {
 #pragma unroll
             for (int w = 0; w < XW; ++w) xv[i][w] = 0;
             if (cid[i] >= 0 && f_ok) load_bytes<XB>(xh + (int64_t(cid[i]) * a.x_row_stride + f0) * EB, xv[i]);
+        }
+    }
+
+    // A_CMAP_DOT: S[r][c] = <xa row r, gathered row c> for the k-block's 32
+    // columns, K = the 16T features in chunks of 32 (chunk s2, features
+    // 32 s2 + 8g.. at lane group g, as the SDDMM assigns them), then written
+    // to dot_out per edge through the lane's ids.  The gathered rows are
+    // transposed through LDS: the aggregation holds 8 columns x T features per
+    // lane, the product needs 8 features of one column.
+    __device__ __forceinline__ void dot_block(const uint32_t (&xv)[8][XW], const u32x4 *xa, uint8_t *dg,
+                                              float *ds) const
+    {
+        if constexpr (AMODE == A_CMAP_DOT && T >= 2 && (KIND == K_F16 || KIND == K_BF16)) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int c = 8 * g + i;
+                uint32_t *d = reinterpret_cast<uint32_t *>(dg + c * dot_pitch<T>() + 16 * ((c >> 3) & 3) + n * T * 2);
+#pragma unroll
+                for (int w = 0; w < XW; ++w) d[w] = xv[i][w];
+            }
+            wave_sync();
+            f32x4 sacc[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int c = 16 * t + n;
+                const uint8_t *row = dg + c * dot_pitch<T>() + 16 * ((c >> 3) & 3);
+                sacc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int s2 = 0; s2 < T / 2; ++s2) {
+                    const u32x4 B = *reinterpret_cast<const u32x4 *>(row + (32 * s2 + 8 * g) * 2);
+                    sacc[t] = KIND == K_F16 ? mfma_f16(xa[s2], B, sacc[t]) : mfma_bf16(xa[s2], B, sacc[t]);
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) ds[(4 * g + q) * 33 + 16 * t + n] = sacc[t][q];
+            wave_sync();
+            float *out = a.dot_out + int64_t(head) * a.val_head_stride;
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                if (dot_id[i] >= 0) out[dot_id[i]] = ds[n * 33 + 8 * g + i];
         }
     }
 
@@ -521,6 +599,24 @@ __global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
     const int g = k.g, W = k.W, nreal = k.nreal;
     const int64_t f0 = k.f0;
 
+    // A_CMAP_DOT: the window rows of dot_x (MFMA A operand: row n, features
+    // 32 s2 + 8g..) for the whole part, and the wave's LDS staging
+    constexpr bool DOT = AMODE == A_CMAP_DOT && T >= 2 && (KIND == K_F16 || KIND == K_BF16);
+    extern __shared__ __align__(16) uint8_t spmm_lds[];
+    u32x4 xa[T >= 2 ? T / 2 : 1];
+    uint8_t *dg = nullptr;
+    float *ds = nullptr;
+    if constexpr (DOT) {
+        dg = spmm_lds + wave * dot_wave_bytes<T>();
+        ds = reinterpret_cast<float *>(dg + dot_rows_bytes<T>());
+        const int64_t drow = int64_t(a.win[part]) * W + k.n;
+        const bool ok = k.a_row_ok && drow < a.dot_rows;
+        const uint8_t *xr = a.dot_x + (int64_t(head) * a.x_head_stride + drow * a.x_row_stride) * 2;
+#pragma unroll
+        for (int s2 = 0; s2 < T / 2; ++s2)
+            xa[s2] = ok ? *reinterpret_cast<const u32x4 *>(xr + (32 * s2 + 8 * g) * 2) : u32x4{0u, 0u, 0u, 0u};
+    }
+
     f32x4 acc[T];
 #pragma unroll
     for (int j = 0; j < T; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -552,6 +648,7 @@ __global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
             k.cols(kb + 32, nid);
             k.masks(kb + 32, nmv, nmb);
             KB::mma(av, xv, acc);
+            if constexpr (DOT) k.dot_block(xv, xa, dg, ds);
 #pragma unroll
             for (int i = 0; i < 8; ++i) cid[i] = nid[i];
             mv = nmv;
@@ -753,9 +850,21 @@ int pick_t(int64_t dim_n, int eb)
     return t;
 }
 
+// Dynamic LDS of an A_CMAP_DOT launch (4 waves per workgroup).
+size_t dot_lds_bytes(int t)
+{
+    switch (t) {
+    case 2: return 4 * size_t(dot_wave_bytes<2>());
+    case 4: return 4 * size_t(dot_wave_bytes<4>());
+    case 8: return 4 * size_t(dot_wave_bytes<8>());
+    default: return 0;
+    }
+}
+
 template <int KIND, int AMODE, int PIPE>
 hipError_t launch_pipe(int t, const SpmmArgs &a, hipStream_t s)
 {
+    const size_t lds = AMODE == A_CMAP_DOT ? dot_lds_bytes(t) : 0;
     const int64_t tasks = a.parts * a.heads * a.slices;
     int64_t blocks = (tasks + 3) / 4;
     if (a.xcd_slices) {
@@ -764,12 +873,12 @@ hipError_t launch_pipe(int t, const SpmmArgs &a, hipStream_t s)
     }
     const dim3 grid{static_cast<unsigned>(blocks)}, block{256};
     switch (t) {
-    case 1: hipLaunchKernelGGL((spmm_block_kernel<KIND, 1, AMODE, PIPE>), grid, block, 0, s, a); break;
-    case 2: hipLaunchKernelGGL((spmm_block_kernel<KIND, 2, AMODE, PIPE>), grid, block, 0, s, a); break;
-    case 4: hipLaunchKernelGGL((spmm_block_kernel<KIND, 4, AMODE, PIPE>), grid, block, 0, s, a); break;
+    case 1: hipLaunchKernelGGL((spmm_block_kernel<KIND, 1, AMODE, PIPE>), grid, block, lds, s, a); break;
+    case 2: hipLaunchKernelGGL((spmm_block_kernel<KIND, 2, AMODE, PIPE>), grid, block, lds, s, a); break;
+    case 4: hipLaunchKernelGGL((spmm_block_kernel<KIND, 4, AMODE, PIPE>), grid, block, lds, s, a); break;
     case 8:
         if constexpr (KindTraits<KIND>::EB <= 2) {
-            hipLaunchKernelGGL((spmm_block_kernel<KIND, 8, AMODE, PIPE>), grid, block, 0, s, a);
+            hipLaunchKernelGGL((spmm_block_kernel<KIND, 8, AMODE, PIPE>), grid, block, lds, s, a);
             break;
         }
         return hipErrorInvalidValue;
@@ -819,8 +928,8 @@ hipError_t run_spmm(int kind, int amode, int t, const SpmmArgs &a, bool covered,
                                             : amode == A_ATTN ? launch_main<K, A_ATTN>(t, b, s)     \
                                                               : launch_main<K, A_DENSE>(t, b, s))
     switch (kind) {
-    case K_F16: e = ATTGNN_BY_AMODE(K_F16); break;
-    case K_BF16: e = ATTGNN_BY_AMODE(K_BF16); break;
+    case K_F16: e = amode == A_CMAP_DOT ? launch_pipe<K_F16, A_CMAP_DOT, 1>(t, b, s) : ATTGNN_BY_AMODE(K_F16); break;
+    case K_BF16: e = amode == A_CMAP_DOT ? launch_pipe<K_BF16, A_CMAP_DOT, 1>(t, b, s) : ATTGNN_BY_AMODE(K_BF16); break;
     case K_FP8: e = amode == A_COMPACT ? launch_main<K_FP8, A_COMPACT>(t, b, s) : launch_main<K_FP8, A_DENSE>(t, b, s); break;
     case K_BF16X3: e = ATTGNN_BY_AMODE(K_BF16X3); break;
     default: e = ATTGNN_BY_AMODE(K_F32); break;
@@ -912,7 +1021,8 @@ int edge_map_run(int amode, const int32_t *rp, const int32_t *col, const int32_t
                  const int32_t *window_of_part, const int32_t *atomic, int64_t parts, const void *X, int x_type,
                  int x_layout, int64_t heads, int64_t n_x, void *Y, int y_type, int64_t m_ori, int64_t dim_n,
                  int out_layout, int window, int wide, int mode, float *workspace, int64_t workspace_elems,
-                 int flags, hipStream_t s)
+                 int flags, hipStream_t s, const void *dot_x = nullptr, int64_t dot_rows = 0,
+                 float *dot_out = nullptr)
 {
     if (parts < 0 || m_ori < 0 || dim_n <= 0 || n_x < 0 || heads <= 0 || nnz < 0) return ATTGNN_E_ARG;
     if (!(window == 8 || window == 16)) return ATTGNN_E_ARG;
@@ -925,7 +1035,7 @@ int edge_map_run(int amode, const int32_t *rp, const int32_t *col, const int32_t
     if (!Y || !(y_type == ATTGNN_F32 || y_type == ATTGNN_F16 || y_type == ATTGNN_BF16)) return ATTGNN_E_ARG;
     if (parts > 0 && (!rp || !col || !map || !edge_val || !window_of_part || !atomic || !X || !workspace))
         return ATTGNN_E_ARG;
-    if (amode == A_CMAP && parts > 0 && (!masks || !bbase)) return ATTGNN_E_ARG;
+    if ((amode == A_CMAP || amode == A_CMAP_DOT) && parts > 0 && (!masks || !bbase)) return ATTGNN_E_ARG;
     if (parts > 0 && workspace_elems < heads * attgnn_spmm_workspace_elems(parts, window, dim_n)) return ATTGNN_E_ARG;
     const int eb = elt_bytes(kind);
     const int t = pick_t(dim_n, eb);
@@ -947,6 +1057,21 @@ int edge_map_run(int amode, const int32_t *rp, const int32_t *col, const int32_t
     a.heads = int(heads); a.window = window; a.wide_log2 = wl; a.y_type = y_type;
     a.pipe = pipe_of(flags, eb, true, parts * heads * a.slices);
     a.xcd_slices = xcd_mapping(flags, a.slices);
+    if (amode == A_CMAP_DOT) {
+        // 16-bit types, one feature slice of 32, 64 or 128 features, the
+        // one-block schedule; dot_x 16-byte aligned in X's type and layout
+        if (!(kind == K_F16 || kind == K_BF16) || !(dim_n == 32 || dim_n == 64 || dim_n == 128) ||
+            t != dim_n / 16 || a.slices != 1)
+            return ATTGNN_E_UNSUPPORTED;
+        if (dot_rows < 0 || (parts > 0 && (!dot_x || !dot_out)) ||
+            reinterpret_cast<uintptr_t>(dot_x) % 16 != 0)
+            return ATTGNN_E_ARG;
+        a.dot_x = static_cast<const uint8_t *>(dot_x);
+        a.dot_out = dot_out;
+        a.dot_rows = dot_rows;
+        a.pipe = 1;
+        a.xcd_slices = 0;
+    }
     const hipError_t e = run_spmm(kind, amode, t, a, (flags & ATTGNN_SPMM_ROWS_COVERED) != 0, s);
     return e == hipSuccess ? ATTGNN_OK : int(e);
 }
@@ -979,6 +1104,23 @@ extern "C" int attgnn_spmm_edge_map_compact(const int32_t *rp, const int32_t *co
     return edge_map_run(A_CMAP, rp, col, ids, masks, block_base, edge_val, val_type, nnz, window_of_part, atomic,
                         parts, X, x_type, x_layout, heads, n_x, Y, y_type, m_ori, dim_n, out_layout, window, 32,
                         mode, workspace, workspace_elems, flags, reinterpret_cast<hipStream_t>(stream));
+}
+
+extern "C" int attgnn_spmm_edge_map_compact_dot(const int32_t *rp, const int32_t *col, const uint16_t *masks,
+                                                const int64_t *block_base, const int32_t *ids, const void *edge_val,
+                                                int val_type, int64_t nnz, const int32_t *window_of_part,
+                                                const int32_t *atomic, int64_t parts, const void *X, int x_type,
+                                                int x_layout, int64_t heads, int64_t n_x, void *Y, int y_type,
+                                                int64_t m_ori, int64_t dim_n, int out_layout, int window,
+                                                const void *dot_x, int64_t dot_rows, float *dot_out,
+                                                float *workspace, int64_t workspace_elems, int flags,
+                                                attgnn_stream_t stream)
+{
+    if (window > 16) return ATTGNN_E_ARG;
+    return edge_map_run(A_CMAP_DOT, rp, col, ids, masks, block_base, edge_val, val_type, nnz, window_of_part,
+                        atomic, parts, X, x_type, x_layout, heads, n_x, Y, y_type, m_ori, dim_n, out_layout, window,
+                        32, ATTGNN_MODE_NATIVE, workspace, workspace_elems, flags,
+                        reinterpret_cast<hipStream_t>(stream), dot_x, dot_rows, dot_out);
 }
 
 extern "C" int attgnn_spmm_attention(const int32_t *rp, const int32_t *col, const int32_t *map,

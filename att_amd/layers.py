@@ -358,6 +358,26 @@ class GATGraph:
             self._long_rows = (lr, tl)
         return self._long_rows
 
+    # the A^T aggregation and the SDDMM of the attention backward in one kernel
+    fuse_backward = True
+
+    def aggregate_sddmm(self, p, dh, Xt, m_ori=None, out_dtype=torch.float32):
+        """(aggregate(p, dh, transpose=True), sddmm(dh, Xt)): dXt[j] = sum_i
+        p[(i,j)] dh[i] over A^T, and dp[h, e=(i,j)] = <dh[i,h], Xt[j,h]> in
+        A's edge order (fp32).  With compact edge lists and fp16 / bf16
+        features of 32, 64 or 128 per head, one kernel over A^T's structure
+        computes both from the same gathered rows of dh (bitwise equal to the
+        two separate products); otherwise the two products run separately."""
+        n, heads, dim = Xt.shape
+        m = self.num_nodes_ori if m_ori is None else m_ori
+        if (self.fuse_backward and self.cmap_bwd is not None and dh.dtype in (torch.float16, torch.bfloat16)
+                and Xt.dtype == dh.dtype and dh.shape == Xt.shape and dim in (32, 64, 128)):
+            rp, col, _, win, at = self.bwd
+            return ops.spmm_edge_map_compact_dot(rp, col, *self.cmap_bwd, p, win, at, dh.contiguous(),
+                                                 Xt.contiguous(), "mhd", self.window, m_ori=m, x_layout="mhd",
+                                                 out_dtype=out_dtype, rows_covered=True)
+        return self.aggregate(p, dh, transpose=True, m_ori=m, out_dtype=out_dtype), self.sddmm(dh, Xt)
+
     def sddmm(self, A, B, out_dtype=torch.float32):
         """out[h, e=(i,j)] = <A[i,h], B[j,h]> over A's edges ([N, H, D] inputs, [H, nnz] out)."""
         rp, col, emap, win, _ = self.fwd
@@ -422,10 +442,11 @@ class GATAttention(torch.autograd.Function):
             dXt = ops.spmm_attention(rp, col, emap_t, win, at, dh, "mhd", g.window, g.wide, s1, s0, col_offset=lse,
                                      slope=ctx.slope, drop_p=ctx.drop_p, seed=ctx.seed, m_ori=n, x_layout="mhd",
                                      rows_covered=True)
+            # dp[(i,j)] = <dh[i], Xt[j]>  (SDDMM over A)
+            dp = g.sddmm(dh, Xt)
         else:
-            dXt = g.aggregate(p, dh, transpose=True, m_ori=n)
-        # dp[(i,j)] = <dh[i], Xt[j]>  (SDDMM over A)
-        dp = g.sddmm(dh, Xt)
+            # with the SDDMM dp[(i,j)] = <dh[i], Xt[j]> from the same gathered rows
+            dXt, dp = g.aggregate_sddmm(p, dh, Xt, m_ori=n)
         # row and column sums of dL/dz (columns through the transpose permutation)
         long_rows, t_long_rows = g.long_row_lists()
         ds0, ds1 = ops.edge_softmax_bwd_sums(g.rowptr, g.col, s0, s1, lse, dp, ctx.slope, ctx.drop_p, ctx.seed,
@@ -461,9 +482,9 @@ class GATv2Attention(torch.autograd.Function):
         g = ctx.graph
         n = X1.shape[0]
         dh = dh.to(X1.dtype).contiguous()
-        # messages: dX1[j] = sum_i p[(i,j)] dh[i]  (A^T edge-map SpMM)
-        base = g.aggregate(p, dh, transpose=True, m_ori=n)
-        dp = g.sddmm(dh, X1)            # dL/dp[(i,j)] = <dh[i], X1[j]>
+        # messages: dX1[j] = sum_i p[(i,j)] dh[i]  (A^T edge-map SpMM), and
+        # dL/dp[(i,j)] = <dh[i], X1[j]> from the same gathered rows
+        base, dp = g.aggregate_sddmm(p, dh, X1, m_ori=n)
         segs, t_segs = g.segments()
         _, dX0, dX1, da = ops.gatv2_bwd(g.rowptr, g.col, segs, g.t_col, g.t_perm, t_segs, X0, X1, a, ctx.slope, z,
                                         lse, dp, ctx.drop_p, ctx.seed, base=base)

@@ -380,7 +380,7 @@ def spmm_edge_map_compact(rp, col, masks, block_base, ids, edge_val, window_of_p
 
 
 def _edge_map(compact, rp, col, emap, edge_val, window_of_part, atomic, X, out_layout, window, wide, m_ori, mode,
-              x_layout, out_dtype, rows_covered, pipe, workspace, xcd, dev):
+              x_layout, out_dtype, rows_covered, pipe, workspace, xcd, dev, dot_x=None):
     if X.dim() == 2:
         X = X.unsqueeze(0 if x_layout == "hmd" else 1)
     if x_layout == "hmd":
@@ -408,6 +408,15 @@ def _edge_map(compact, rp, col, emap, edge_val, window_of_part, atomic, X, out_l
     flags = _spmm_flags(rows_covered, pipe, xcd)
     tail = (_ptr(ev), _TYPE_OF[ev.dtype], nnz, _ptr(window_of_part), _ptr(atomic), parts, _ptr(X), _TYPE_OF[X.dtype],
             xl, heads, n_x, _ptr(y), _TYPE_OF[out_dtype], m, dim, layout, window)
+    if dot_x is not None:
+        if dot_x.shape != X.shape or dot_x.dtype != X.dtype or not dot_x.is_contiguous() or dot_x.device != X.device:
+            raise ValueError("dot_x must be a contiguous tensor of X's shape, type and device")
+        dot = torch.empty((heads, nnz), dtype=torch.float32, device=dev)
+        rc = lib().attgnn_spmm_edge_map_compact_dot(_ptr(rp), _ptr(col), _ptr(compact[0]), _ptr(compact[1]),
+                                                    _ptr(emap), *tail, _ptr(dot_x), n_x, _ptr(dot), _ptr(ws), ws_n,
+                                                    flags, _stream(dev))
+        check(rc, "attgnn_spmm_edge_map_compact_dot")
+        return y, dot
     if compact is None:
         rc = lib().attgnn_spmm_edge_map(_ptr(rp), _ptr(col), _ptr(emap), *tail, wide, _mode(mode, X.dtype), _ptr(ws),
                                         ws_n, flags, _stream(dev))
@@ -417,6 +426,27 @@ def _edge_map(compact, rp, col, emap, edge_val, window_of_part, atomic, X, out_l
                                                 *tail, _mode(mode, X.dtype), _ptr(ws), ws_n, flags, _stream(dev))
         check(rc, "attgnn_spmm_edge_map_compact")
     return y
+
+
+def spmm_edge_map_compact_dot(rp, col, masks, block_base, ids, edge_val, window_of_part, atomic, X, dot_x,
+                              out_layout, window, m_ori=None, x_layout="mhd", out_dtype=torch.float32,
+                              rows_covered=False, workspace=None):
+    """:func:`spmm_edge_map_compact` (native mode) that also returns, for every
+    edge e at window row r and column c of the structure, dot[h, e] =
+    <dot_x[h, r, :], X[h, c, :]> in fp32 (dot_x: X's shape, type and layout;
+    fp16 / bf16, 32, 64 or 128 features).  Over A^T's structure with A's edge
+    ids and X = dL/dh, dot_x = Xt this is the GAT backward's aggregation
+    gradient and the SDDMM <dh[i], Xt[j]> in one pass.  Returns (Y, dot);
+    bitwise equal to the two separate products."""
+    dev = _need_gpu("spmm_edge_map_compact_dot", rp, col, masks, block_base, ids, edge_val, window_of_part, atomic,
+                    X, dot_x)
+    _need_i32("spmm_edge_map_compact_dot", rp, col, ids, window_of_part, atomic)
+    if masks.dtype not in (torch.int16, torch.uint16) or block_base.dtype != torch.int64:
+        raise TypeError("masks must be 16-bit and block_base int64")
+    if X.dtype not in (torch.float16, torch.bfloat16):
+        raise TypeError("spmm_edge_map_compact_dot: X must be fp16 or bf16")
+    return _edge_map((masks, block_base), rp, col, ids, edge_val, window_of_part, atomic, X, out_layout, window, 32,
+                     m_ori, None, x_layout, out_dtype, rows_covered, None, workspace, True, dev, dot_x=dot_x)
 
 
 def _node_terms(t, heads, layout, name):

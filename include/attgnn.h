@@ -341,6 +341,33 @@ int attgnn_spmm_edge_map_compact(const int32_t *rp, const int32_t *col, const ui
                                  attgnn_stream_t stream);
 
 /*
+ * attgnn_spmm_edge_map_compact fused with the SDDMM over the same structure:
+ * besides Y, every edge e in the structure at window row r and column c gets
+ *   dot_out[h*nnz + e] = sum_f dot_x[h, r, f] * X[h, c, f]   (fp32),
+ * the products taken from the X rows the aggregation gathers anyway (dot_x
+ * has X's type and layout, dot_rows rows, 16-byte aligned).  Run over the
+ * structure of A^T with A's edge ids (GAT backward: X = dL/dh, dot_x = the
+ * projected features) it gives the aggregation gradient and the
+ * edge-probability gradient <dh[i], Xt[j]> in A's edge order in one pass over
+ * the gathered rows; bitwise equal to attgnn_spmm_edge_map_compact plus
+ * attgnn_sddmm_edges over A.  F16 / BF16 only (native mode), dim_n 32, 64 or
+ * 128; ATTGNN_E_UNSUPPORTED otherwise.  The k-loop schedule flags are ignored.
+ *
+ * Replaces the pair ATT_SpMM_v{1,3}.spmm_*_h_m (transposed map) +
+ * ATT_SDDMM_v{1,3}.sddmm_*_h_nnz of the attention backward
+ *   (end2end_head/gat_final/mgat_csr/mgat_conv.py:91-151).
+ */
+int attgnn_spmm_edge_map_compact_dot(const int32_t *rp, const int32_t *col, const uint16_t *masks,
+                                     const int64_t *block_base, const int32_t *ids, const void *edge_val,
+                                     int val_type, int64_t nnz, const int32_t *window_of_part,
+                                     const int32_t *atomic, int64_t parts, const void *X, int x_type,
+                                     int x_layout, int64_t heads, int64_t n_x, void *Y, int y_type,
+                                     int64_t m_ori, int64_t dim_n, int out_layout, int window,
+                                     const void *dot_x, int64_t dot_rows, float *dot_out,
+                                     float *workspace, int64_t workspace_elems, int flags,
+                                     attgnn_stream_t stream);
+
+/*
  * Attention aggregation with the probabilities rebuilt in the kernel:
  * Y_h = P_h . X_h over the edge-id map's tiles, where the entry of edge e =
  * (r, c) (e = map[slot], A's CSR position) is
