@@ -10,7 +10,7 @@ modules live in :mod:`att_amd.compat`.
     import att_amd.torch_ops
     y = torch.ops.attgnn.spmm_block(rp, col, val, win, atomic, X, m_ori, 16, 32, "auto", torch.float16, True)
 """
-from typing import Optional
+from typing import Optional, Tuple
 
 import torch
 
@@ -79,6 +79,50 @@ def _(rp, col, emap, edge_val, window_of_part, atomic, X, out_layout, window, wi
     m = n if m_ori < 0 else m_ori
     shape = (heads, m, dim) if out_layout == "hmd" else (m, heads, dim)
     return X.new_empty(shape, dtype=out_dtype)
+
+
+@torch.library.custom_op(f"{_LIB}::spmm_edge_map_compact", mutates_args=())
+def spmm_edge_map_compact(rp: torch.Tensor, col: torch.Tensor, masks: torch.Tensor, block_base: torch.Tensor,
+                          ids: torch.Tensor, edge_val: torch.Tensor, window_of_part: torch.Tensor, atomic: torch.Tensor,
+                          X: torch.Tensor, out_layout: str, window: int, m_ori: int, mode: str, x_layout: str,
+                          out_dtype: torch.dtype, rows_covered: bool) -> torch.Tensor:
+    """Edge-map SpMM over the compact edge-id list (attgnn_spmm_edge_map_compact);
+    m_ori < 0 means the rows of X."""
+    return ops.spmm_edge_map_compact(rp, col, masks, block_base, ids, edge_val, window_of_part, atomic, X, out_layout,
+                                     window, m_ori=None if m_ori < 0 else m_ori, mode=_mode(mode), x_layout=x_layout,
+                                     out_dtype=out_dtype, rows_covered=rows_covered)
+
+
+@spmm_edge_map_compact.register_fake
+def _(rp, col, masks, block_base, ids, edge_val, window_of_part, atomic, X, out_layout, window, m_ori, mode, x_layout,
+      out_dtype, rows_covered):
+    heads, n, dim = _hmd(X, x_layout)
+    m = n if m_ori < 0 else m_ori
+    shape = (heads, m, dim) if out_layout == "hmd" else (m, heads, dim)
+    return X.new_empty(shape, dtype=out_dtype)
+
+
+@torch.library.custom_op(f"{_LIB}::spmm_edge_map_compact_dot", mutates_args=())
+def spmm_edge_map_compact_dot(rp: torch.Tensor, col: torch.Tensor, masks: torch.Tensor, block_base: torch.Tensor,
+                              ids: torch.Tensor, edge_val: torch.Tensor, window_of_part: torch.Tensor,
+                              atomic: torch.Tensor, X: torch.Tensor, dot_x: torch.Tensor, out_layout: str,
+                              window: int, m_ori: int, x_layout: str, out_dtype: torch.dtype,
+                              rows_covered: bool) -> Tuple[torch.Tensor, torch.Tensor]:
+    """The compact edge-map SpMM with each edge's dot product <dot_x[window row], X[column]>
+    (attgnn_spmm_edge_map_compact_dot): (Y, dot [H, nnz] fp32); m_ori < 0 means the rows of X."""
+    return ops.spmm_edge_map_compact_dot(rp, col, masks, block_base, ids, edge_val, window_of_part, atomic, X, dot_x,
+                                         out_layout, window, m_ori=None if m_ori < 0 else m_ori, x_layout=x_layout,
+                                         out_dtype=out_dtype, rows_covered=rows_covered)
+
+
+@spmm_edge_map_compact_dot.register_fake
+def _(rp, col, masks, block_base, ids, edge_val, window_of_part, atomic, X, dot_x, out_layout, window, m_ori,
+      x_layout, out_dtype, rows_covered):
+    heads, n, dim = _hmd(X, x_layout)
+    m = n if m_ori < 0 else m_ori
+    shape = (heads, m, dim) if out_layout == "hmd" else (m, heads, dim)
+    nnz = edge_val.shape[-1]
+    return X.new_empty(shape, dtype=out_dtype), X.new_empty((heads, nnz), dtype=torch.float32)
 
 
 @torch.library.custom_op(f"{_LIB}::spmm_attention", mutates_args=())
@@ -215,5 +259,6 @@ def _(Xs, Xt, a0, a1, ds0, ds1, base, combine):
         Xs.new_empty((heads, dim), dtype=torch.float32)
 
 
-OPS = ("spmm_block", "spmm_compact", "spmm_edge_map", "spmm_attention", "sddmm_edges", "node_scores", "edge_softmax_fwd",
-       "edge_softmax_bwd", "edge_softmax_bwd_sums", "gather_rowsum", "attention_node_bwd")
+OPS = ("spmm_block", "spmm_compact", "spmm_edge_map", "spmm_edge_map_compact", "spmm_edge_map_compact_dot",
+       "spmm_attention", "sddmm_edges", "node_scores", "edge_softmax_fwd", "edge_softmax_bwd", "edge_softmax_bwd_sums",
+       "gather_rowsum", "attention_node_bwd")

@@ -116,6 +116,29 @@ def test_opcheck_fake_implementations(cuda, gcn, gat):
     rp2, col2, emap, win2, at2 = ggraph.fwd
     torch.library.opcheck(A.spmm_edge_map, (rp2, col2, emap, p, win2, at2, Xg, "mhd", 16, 32, -1, "auto", "mhd",
                                             torch.float32, True), test_utils=util)
+    rpt, colt, _, wint, att = ggraph.bwd
+    torch.library.opcheck(A.spmm_edge_map_compact_dot, (rpt, colt, *ggraph.cmap_bwd, p, wint, att, Xg, Xg, "mhd", 16,
+                                                        -1, "mhd", torch.float32, True), test_utils=util)
+
+
+def test_compact_edge_map_ops_match_wrappers(cuda, gat):
+    graph, n = gat
+    heads, dim = 4, 64
+    torch.manual_seed(5)
+    X = torch.randn(n, heads, dim, device=cuda).bfloat16()
+    Xt = torch.randn(n, heads, dim, device=cuda).bfloat16()
+    p = torch.rand(heads, graph.num_edges, device=cuda).bfloat16()
+    rp, col, _, win, at = graph.fwd
+    _same(A.spmm_edge_map_compact(rp, col, *graph.cmap_fwd, p, win, at, X, "mhd", 16, -1, "auto", "mhd",
+                                  torch.float32, True),
+          ops.spmm_edge_map_compact(rp, col, *graph.cmap_fwd, p, win, at, X, "mhd", 16, x_layout="mhd",
+                                    out_dtype=torch.float32, rows_covered=True))
+    rp, col, _, win, at = graph.bwd
+    y, dot = A.spmm_edge_map_compact_dot(rp, col, *graph.cmap_bwd, p, win, at, X, Xt, "mhd", 16, -1, "mhd",
+                                         torch.float32, True)
+    y2, dot2 = graph.aggregate_sddmm(p, X, Xt)
+    _same(y, y2)
+    _same(dot, dot2)
 
 
 def test_compiled_attention_forward(cuda, gat):

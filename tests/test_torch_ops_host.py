@@ -54,6 +54,20 @@ def test_spmm_edge_map_shapes(fake, x_layout, out_layout):
     assert y.shape == ((heads, n, dim) if out_layout == "hmd" else (n, heads, dim)) and y.dtype == torch.bfloat16
 
 
+def test_spmm_edge_map_compact_shapes(fake):
+    heads, n, dim, nnz, parts = 4, 100, 64, 500, 4
+    X = torch.empty(n, heads, dim, dtype=torch.bfloat16, device="cuda")
+    ev = torch.empty(heads, nnz, dtype=torch.bfloat16, device="cuda")
+    masks = torch.empty(64, dtype=torch.int16, device="cuda")
+    base = torch.empty(3, dtype=torch.int64, device="cuda")
+    args = (_i32(2 * parts + 1), _i32(64), masks, base, _i32(nnz), ev, _i32(parts), _i32(parts), X)
+    y = torch.ops.attgnn.spmm_edge_map_compact(*args, "mhd", 16, -1, "auto", "mhd", torch.float32, True)
+    assert y.shape == (n, heads, dim) and y.dtype == torch.float32
+    y, dot = torch.ops.attgnn.spmm_edge_map_compact_dot(*args, X, "hmd", 16, 80, "mhd", torch.bfloat16, True)
+    assert y.shape == (heads, 80, dim) and y.dtype == torch.bfloat16
+    assert dot.shape == (heads, nnz) and dot.dtype == torch.float32
+
+
 def test_attention_shapes(fake):
     heads, n, dim, nnz = 2, 50, 8, 300
     X = torch.empty(n, heads, dim, dtype=torch.bfloat16, device="cuda")
