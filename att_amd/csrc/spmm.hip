@@ -156,10 +156,10 @@ enum AMode { A_DENSE = 0, A_MAPPED = 1, A_COMPACT = 2, A_ATTN = 3, A_CMAP = 4, A
 // A_CMAP_DOT staging in LDS, per wave: the k-block's 32 gathered rows (16T
 // 16-bit features each; row c at c * dot_pitch + 16 * ((c >> 3) & 3) bytes, so
 // the four 8-column groups of a lane position write to different banks), then
-// the 16 x 32 dot products in rows of 33 floats.
+// the block's 16 x 32 edge ids (-1: no edge), rows of 32 ints.
 template <int T> constexpr int dot_pitch() { return 32 * T + 16; }
 template <int T> constexpr int dot_rows_bytes() { return 32 * dot_pitch<T>() + 48; }
-template <int T> constexpr int dot_wave_bytes() { return dot_rows_bytes<T>() + 16 * 33 * 4; }
+template <int T> constexpr int dot_wave_bytes() { return dot_rows_bytes<T>() + 16 * 32 * 4; }
 
 __device__ __forceinline__ void wave_sync()
 {
@@ -183,7 +183,7 @@ struct KBlock {
     int64_t f0;
     bool f_ok, a_row_ok;
     float r1v, r2v;   // attention variant: the lane's window-row terms
-    mutable int dot_id[8];   // A_CMAP_DOT: edge ids of the lane's row in its 8 columns (-1: none)
+    int *dot_ids = nullptr;  // A_CMAP_DOT: the wave's 16 x 32 edge-id tile in LDS
 
     __device__ __forceinline__ void cols(int kb, int (&cid)[8]) const
     {
@@ -267,8 +267,9 @@ struct KBlock {
                 }
             }
             if constexpr (AMODE == A_CMAP_DOT) {
-#pragma unroll
-                for (int i = 0; i < 8; ++i) dot_id[i] = id[i];
+                u32x4 *d = reinterpret_cast<u32x4 *>(dot_ids + n * 32 + 8 * g);
+                d[0] = u32x4{uint32_t(id[0]), uint32_t(id[1]), uint32_t(id[2]), uint32_t(id[3])};
+                d[1] = u32x4{uint32_t(id[4]), uint32_t(id[5]), uint32_t(id[6]), uint32_t(id[7])};
             }
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
@@ -299,10 +300,6 @@ struct KBlock {
         const int kg = kb + 8 * g;
 #pragma unroll
         for (int w = 0; w < AW; ++w) av[w] = 0;
-        if constexpr (AMODE == A_CMAP_DOT) {
-#pragma unroll
-            for (int i = 0; i < 8; ++i) dot_id[i] = -1;
-        }
         uint32_t m[8];
         int tot = 0, below = 0;
         if (kg < plen) {
@@ -321,7 +318,14 @@ struct KBlock {
         // entries of the earlier 8-column groups of this block
         const int t0 = __shfl(tot, 0), t1 = __shfl(tot, 16), t2 = __shfl(tot, 32);
         const int prefix = (g > 0 ? t0 : 0) + (g > 1 ? t1 : 0) + (g > 2 ? t2 : 0);
-        if (kg >= nreal) return;
+        if (kg >= nreal) {
+            if constexpr (AMODE == A_CMAP_DOT) {   // no entries: the id tile row stays empty
+                u32x4 *d = reinterpret_cast<u32x4 *>(dot_ids + n * 32 + 8 * g);
+                d[0] = u32x4{~0u, ~0u, ~0u, ~0u};
+                d[1] = u32x4{~0u, ~0u, ~0u, ~0u};
+            }
+            return;
+        }
         int64_t idx = a.bbase[(int64_t(c_begin) + kb) >> 5] + prefix + below;
         packed(m, idx, av);
     }
@@ -344,10 +348,6 @@ struct KBlock {
         const int kg = kb + 8 * g;
 #pragma unroll
         for (int w = 0; w < AW; ++w) av[w] = 0;
-        if constexpr (AMODE == A_CMAP_DOT) {
-#pragma unroll
-            for (int i = 0; i < 8; ++i) dot_id[i] = -1;
-        }
         uint32_t m[8];
         int tot = 0, below = 0;
 #pragma unroll
@@ -358,7 +358,14 @@ struct KBlock {
         }
         const int t0 = __shfl(tot, 0), t1 = __shfl(tot, 16), t2 = __shfl(tot, 32);
         const int prefix = (g > 0 ? t0 : 0) + (g > 1 ? t1 : 0) + (g > 2 ? t2 : 0);
-        if (kg >= nreal) return;
+        if (kg >= nreal) {
+            if constexpr (AMODE == A_CMAP_DOT) {   // no entries: the id tile row stays empty
+                u32x4 *d = reinterpret_cast<u32x4 *>(dot_ids + n * 32 + 8 * g);
+                d[0] = u32x4{~0u, ~0u, ~0u, ~0u};
+                d[1] = u32x4{~0u, ~0u, ~0u, ~0u};
+            }
+            return;
+        }
         int64_t idx = base + prefix + below;
         packed(m, idx, av);
     }
@@ -442,8 +449,7 @@ struct KBlock {
     // to dot_out per edge through the lane's ids.  The gathered rows are
     // transposed through LDS: the aggregation holds 8 columns x T features per
     // lane, the product needs 8 features of one column.
-    __device__ __forceinline__ void dot_block(const uint32_t (&xv)[8][XW], const u32x4 *xa, uint8_t *dg,
-                                              float *ds) const
+    __device__ __forceinline__ void dot_block(const uint32_t (&xv)[8][XW], const u32x4 *xa, uint8_t *dg) const
     {
         if constexpr (AMODE == A_CMAP_DOT && T >= 2 && (KIND == K_F16 || KIND == K_BF16)) {
 #pragma unroll
@@ -466,15 +472,16 @@ struct KBlock {
                     sacc[t] = KIND == K_F16 ? mfma_f16(xa[s2], B, sacc[t]) : mfma_bf16(xa[s2], B, sacc[t]);
                 }
             }
+            // product (4g + q, 16t + n) of this lane goes to the edge in that slot
+            float *out = a.dot_out + int64_t(head) * a.val_head_stride;
 #pragma unroll
             for (int t = 0; t < 2; ++t)
 #pragma unroll
-                for (int q = 0; q < 4; ++q) ds[(4 * g + q) * 33 + 16 * t + n] = sacc[t][q];
-            wave_sync();
-            float *out = a.dot_out + int64_t(head) * a.val_head_stride;
-#pragma unroll
-            for (int i = 0; i < 8; ++i)
-                if (dot_id[i] >= 0) out[dot_id[i]] = ds[n * 33 + 8 * g + i];
+                for (int q = 0; q < 4; ++q) {
+                    const int e = dot_ids[(4 * g + q) * 32 + 16 * t + n];
+                    if (e >= 0) out[e] = sacc[t][q];
+                }
+            wave_sync();   // id tile and gathered rows read before the next block writes them
         }
     }
 
@@ -524,7 +531,8 @@ struct KBlock {
 // PIPE 1: the next k-block's column ids load while this block multiplies.
 // PIPE 2: two k-blocks per iteration, the next pair's column ids prefetched.
 template <int KIND, int T, int AMODE, int PIPE>
-__global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((AMODE == A_CMAP_DOT && T == 4) ? 5 : 1)))
+void spmm_block_kernel(SpmmArgs a)
 {
     using KB = KBlock<KIND, T, AMODE>;
     constexpr int EB = KB::EB, AW = KB::AW, XW = KB::XW;
@@ -605,10 +613,9 @@ __global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
     extern __shared__ __align__(16) uint8_t spmm_lds[];
     u32x4 xa[T >= 2 ? T / 2 : 1];
     uint8_t *dg = nullptr;
-    float *ds = nullptr;
     if constexpr (DOT) {
         dg = spmm_lds + wave * dot_wave_bytes<T>();
-        ds = reinterpret_cast<float *>(dg + dot_rows_bytes<T>());
+        k.dot_ids = reinterpret_cast<int *>(dg + dot_rows_bytes<T>());
         const int64_t drow = int64_t(a.win[part]) * W + k.n;
         const bool ok = k.a_row_ok && drow < a.dot_rows;
         const uint8_t *xr = a.dot_x + (int64_t(head) * a.x_head_stride + drow * a.x_row_stride) * 2;
@@ -648,7 +655,7 @@ __global__ __launch_bounds__(256) void spmm_block_kernel(SpmmArgs a)
             k.cols(kb + 32, nid);
             k.masks(kb + 32, nmv, nmb);
             KB::mma(av, xv, acc);
-            if constexpr (DOT) k.dot_block(xv, xa, dg, ds);
+            if constexpr (DOT) k.dot_block(xv, xa, dg);
 #pragma unroll
             for (int i = 0; i < 8; ++i) cid[i] = nid[i];
             mv = nmv;
