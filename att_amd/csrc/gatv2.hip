@@ -183,8 +183,9 @@ struct V2Args {
 // ---- forward --------------------------------------------------------------
 
 // z for the edges of one segment of row i.
+// 4-head instance: 8 waves per SIMD without spilling when asked (68 VGPRs, 7 waves, otherwise)
 template <int V>
-__global__ __launch_bounds__(256) void gatv2_logits_kernel(V2Args a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(V == 4 ? 8 : 1))) void gatv2_logits_kernel(V2Args a)
 {
     const int64_t gid = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 4;
     const int l = threadIdx.x & 15;
