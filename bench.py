@@ -16,8 +16,12 @@ HIP events on the stream it runs on; its algorithmic bytes per launch are
 4(M+1) + nnz*(4+2) + U*256*2 + M*256*2 with U the number of distinct columns
 summed over 16-row windows.  ``cpu_baseline`` times torch.sparse.mm (CSR,
 FP32) on the host cores over a bounded row sample of the same A_hat.
+``attention`` reports the SDDMM half of the metric and the GAT layer against
+plain FP32 on the attention workload (169,343 nodes, 2.48M entries, 4 heads
+x 64, BF16; see attention_leg); it is measured after the timed steps and is
+not part of ``value``.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-attention]
 """
 import argparse
 import json
@@ -78,6 +82,89 @@ def cpu_baseline(g, vals, dim=256, budget_s=12.0):
                       f"median of {len(times)}"}
 
 
+def attention_leg(dev, heads=4, dim=64, reps=20):
+    """The SDDMM half of the metric and the GAT layer against plain FP32, on
+    the attention workload: directed Chung-Lu symmetrised to 169,343 nodes and
+    2,480,223 entries with self-loops (seed 0), BF16 features, 4 heads x 64.
+
+    SDDMM out[h, (i,j)] = <A[i,h], B[j,h]> (compact edge lists, fp32 out):
+    HIP-event time on its stream, algorithmic bytes 4(M+1) + nnz*(4+4) +
+    M*D*H*2 + U*D*H*2 (U = distinct columns summed over 16-row windows), and
+    the relative error of every 97th edge against float64.  GAT fwd+bwd
+    (GATConv_multi, BF16 storage, FP32 accumulation) against plain FP32
+    PyTorch message passing with the same weights, medians of event-timed
+    iterations; rel. error of the layer output vs the FP32 path."""
+    import torch
+    from att_amd import graphs as G
+    from att_amd.benchmarks.plain import gat_fp32
+    from att_amd.layers import GATConv_multi, GATGraph
+
+    g = G.directed_chung_lu_symmetrized(169_343, 2_480_000, seed=0)
+    graph = GATGraph(g, dev, build="device")
+    gen = torch.Generator().manual_seed(4)
+    A = torch.randn(g.n, heads, dim, generator=gen).bfloat16().to(dev)
+    B = torch.randn(g.n, heads, dim, generator=gen).bfloat16().to(dev)
+    stream = torch.cuda.current_stream(dev)
+
+    def timed(fn, n):
+        for _ in range(2):
+            fn()
+        ts = []
+        for _ in range(n):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        return float(np.median(ts))
+
+    t_sddmm = timed(lambda: graph.sddmm(A, B), reps)
+    out = graph.sddmm(A, B)
+    idx = np.arange(0, g.nnz, 97)
+    rows_s = torch.from_numpy(g.row_ids()[idx].astype(np.int64)).to(dev)
+    cols_s = torch.from_numpy(g.indices[idx].astype(np.int64)).to(dev)
+    want = (A.double()[rows_s] * B.double()[cols_s]).sum(-1)
+    got = out[:, torch.from_numpy(idx).to(dev)].t().double()
+    sddmm_err = float((got - want).norm() / want.norm())
+    M = g.n
+    nbytes = 4 * (M + 1) + g.nnz * 8 + M * dim * heads * 2 + graph.n_unique * dim * heads * 2
+
+    torch.manual_seed(heads)
+    layer = GATConv_multi(128, dim, 0.0, 0.2, heads, dtype=torch.bfloat16).to(dev)
+    X = torch.randn(g.n, 128, generator=gen).to(dev)
+    Xb = X.bfloat16()
+    go = torch.randn(g.n, heads, dim, generator=gen).to(dev)
+    go_b = go.bfloat16()
+    W = layer.fc.weight.detach().clone().requires_grad_(True)
+    a0 = layer.a0.detach().clone().requires_grad_(True)
+    a1 = layer.a1.detach().clone().requires_grad_(True)
+    rows = torch.from_numpy(g.row_ids().astype(np.int64)).to(dev)
+    cols = torch.from_numpy(g.indices.astype(np.int64)).to(dev)
+
+    def ours():
+        layer.zero_grad(set_to_none=True)
+        layer(Xb, graph).backward(go_b)
+
+    def plain():
+        gat_fp32(X, W, a0, a1, rows, cols, g.n, heads, dim).backward(go)
+
+    t_ours = timed(ours, 10)
+    t_plain = timed(plain, 3)
+    with torch.no_grad():
+        o = layer(Xb, graph).float()
+        ref = gat_fp32(Xb.float(), W.bfloat16().float(), a0, a1, rows, cols, g.n, heads, dim)
+        gat_err = float((o - ref).norm() / ref.norm())
+    return {"workload": f"GAT graph: directed Chung-Lu symmetrised, {g.n} nodes, {g.nnz} entries, "
+                        f"{heads} heads x {dim}, bf16 features, fp32 accumulation",
+            "sddmm": {"ms": round(t_sddmm, 4), "gflops": round(2.0 * g.nnz * dim * heads / (t_sddmm * 1e-3) / 1e9, 1),
+                      "algorithmic_gbs": round(nbytes / (t_sddmm * 1e-3) / 1e9, 1),
+                      "hbm_frac": round(nbytes / (t_sddmm * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                      "rel_fro_vs_fp64": sddmm_err},
+            "gat_fwd_bwd": {"ms": round(t_ours, 4), "plain_fp32_ms": round(t_plain, 4),
+                            "speedup": round(t_plain / t_ours, 2), "rel_err_vs_fp32": gat_err}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +172,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--scale", type=float, default=1.0, help="graph size fraction (testing only)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-attention", action="store_true", help="skip the SDDMM / GAT leg")
     ap.add_argument("--part-size", type=int, default=0)
     ap.add_argument("--build", choices=["host", "device"], default="device",
                     help="where the window blocks are built (results are bit-identical)")
@@ -216,6 +304,10 @@ def main():
     torch.cuda.synchronize()
     t_plain = ev0.elapsed_time(ev1) / 3
     del A32, X32
+    attention = None
+    if not args.no_attention and args.scale == 1.0:
+        attention = attention_leg(dev)
+        log(f"attention leg: {attention}")
 
     traffic = None
     tfile = os.path.join(HERE, "profiles", "spmm256_traffic.json")
@@ -250,6 +342,7 @@ def main():
         "plain_fp32_gpu": {"spmm256_ms": round(t_plain, 4),
                            "gflops": round(2.0 * g.nnz * F_HID / (t_plain * 1e-3) / 1e9, 1),
                            "speedup_window_block": round(t_plain / t_spmm, 3)},
+        "attention": attention,
     }
     print(json.dumps(out), flush=True)
 
