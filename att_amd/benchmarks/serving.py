@@ -59,6 +59,7 @@ def main():
               f"{srv.cache.bytes / 2**30:.2f} GiB resident", file=sys.stderr)
         if where == args.preprocess:
             server = srv
+            server.prime()   # every bucket's HIP graph captured before the timed runs
         else:
             del srv
     if args.saturated_only:

@@ -586,6 +586,26 @@ class Server:
                 time.sleep(2e-5)
         return requests
 
+    def prime(self):
+        """Capture every bucket's HIP graph before traffic arrives: a bucket's
+        first launch runs the model eagerly, synchronises and captures, which
+        stalls the loop for milliseconds -- under load that lands on the p99
+        of whatever requests queue meanwhile.  Launches the smallest cached
+        graph once through each bucket (both instances of every capacity and
+        model); a no-op without cached graphs or HIP graphs."""
+        if not self.cache.entries:
+            return
+        g = min(self.cache.entries.values(), key=lambda c: c.n_pad)
+        for kind in ("gcn", "gat"):
+            for pair in self.buckets[kind]:
+                for b in pair:
+                    if not b.use_graph or b.graph is not None or not b.fits([g]):
+                        continue
+                    g.pins += 1
+                    _, ev = b.launch([g], self.stream)
+                    ev.synchronize()
+                    g.pins -= 1
+
     def _run_alone(self, r, cg, t0):
         """Unpacked path for a graph above the largest bucket."""
         m = self.models

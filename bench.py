@@ -18,10 +18,11 @@ summed over 16-row windows.  ``cpu_baseline`` times torch.sparse.mm (CSR,
 FP32) on the host cores over a bounded row sample of the same A_hat.
 ``attention`` reports the SDDMM half of the metric and the GAT layer against
 plain FP32 on the attention workload (169,343 nodes, 2.48M entries, 4 heads
-x 64, BF16; see attention_leg); it is measured after the timed steps and is
-not part of ``value``.
+x 64, BF16; see attention_leg), ``serving`` the req/s and p99 of batched
+inference against per-graph plain FP32 (see serving_leg); both are measured
+after the timed steps and are not part of ``value``.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-attention]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-attention] [--no-serving]
 """
 import argparse
 import json
@@ -165,6 +166,49 @@ def attention_leg(dev, heads=4, dim=64, reps=20):
                             "speedup": round(t_plain / t_ours, 2), "rel_err_vs_fp32": gat_err}}
 
 
+def serving_leg(dev, pool_size=1024, requests=4096, plain_requests=1024):
+    """The req/s + p99 half of the metric: Poisson-arrival inference requests
+    (50/50 2-layer GCN / 1-layer 4-head GAT, 128 BF16 features) on a pool of
+    Chung-Lu graphs with N ~ lognormal(median 2000, sigma 1.2) clipped to
+    [100, 200000] and mean degree U[4, 32], the structure cache warmed first.
+    The per-graph plain FP32 server's saturated throughput is its capacity and
+    its p99 at 70% of capacity the latency bound; the batched server's
+    saturated throughput and its p99 at 3x and 32x the plain capacity are
+    reported against that bound."""
+    import torch
+    from att_amd import serve as S
+
+    models = S.Models(dev)
+    pool = S.GraphPool(pool_size, seed=0)
+    for k in range(pool_size):
+        pool(k)
+    server = S.Server(dev, models, pool, use_graph=True, part_size=S.SERVE_PART_SIZE, preprocess="device")
+    server.run([S.Request(i, 0.0, "gcn" if i % 2 else "gat", i) for i in range(pool_size)])
+    server.prime()   # every bucket's HIP graph captured before the timed runs
+    plain = S.PlainServer(dev, models, pool)
+    for k in range(pool_size):
+        plain._get(k)
+
+    def run(srv, n, rate, seed):
+        reqs = S.make_requests(n, rate, pool_size, seed=seed)
+        torch.cuda.synchronize()
+        srv.run(reqs)
+        return S.latency_stats(reqs)
+
+    cap_plain = run(plain, plain_requests, None, 1)["req_per_s"]
+    sat = run(server, requests, None, 2)
+    bound = run(plain, plain_requests, 0.7 * cap_plain, 3)["p99_ms"]
+    at = {}
+    for mult in (3, 32):
+        st = run(server, requests, mult * cap_plain, 10 + mult)
+        at[f"{mult}x_plain_capacity"] = {"req_per_s": round(st["req_per_s"], 1), "p99_ms": round(st["p99_ms"], 3),
+                                         "within_p99_bound": st["p99_ms"] <= bound}
+    return {"workload": f"{requests} requests, 50/50 GCN 2-layer / GAT 4-head, pool of {pool_size} Chung-Lu graphs "
+                        "(N ~ lognormal(2000, 1.2) in [100, 200000]), 128 bf16 features, warm cache",
+            "plain_fp32_req_per_s": round(cap_plain, 1), "batched_req_per_s": round(sat["req_per_s"], 1),
+            "capacity_ratio": round(sat["req_per_s"] / cap_plain, 2), "p99_bound_ms": round(bound, 3), **at}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +217,7 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="graph size fraction (testing only)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-attention", action="store_true", help="skip the SDDMM / GAT leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--part-size", type=int, default=0)
     ap.add_argument("--build", choices=["host", "device"], default="device",
                     help="where the window blocks are built (results are bit-identical)")
@@ -308,6 +353,11 @@ def main():
     if not args.no_attention and args.scale == 1.0:
         attention = attention_leg(dev)
         log(f"attention leg: {attention}")
+    serving = None
+    if not args.no_serving and args.scale == 1.0:
+        t = time.perf_counter()
+        serving = serving_leg(dev)
+        log(f"serving leg ({time.perf_counter() - t:.1f}s): {serving}")
 
     traffic = None
     tfile = os.path.join(HERE, "profiles", "spmm256_traffic.json")
@@ -343,6 +393,7 @@ def main():
                            "gflops": round(2.0 * g.nnz * F_HID / (t_plain * 1e-3) / 1e9, 1),
                            "speedup_window_block": round(t_plain / t_spmm, 3)},
         "attention": attention,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

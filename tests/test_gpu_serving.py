@@ -79,3 +79,22 @@ def test_budget_admission(cuda, setup):
     assert all(r.t_done is not None for r in reqs)
     assert srv.cache.bytes + srv.reserved <= budget
     assert srv.cache.evictions > 0
+
+
+def test_prime_captures_every_bucket_and_keeps_results(cuda, setup):
+    """Server.prime captures each bucket's HIP graph ahead of traffic (both
+    instances of every capacity and model); the served results afterwards
+    equal those of eager launches."""
+    pool, models = setup
+    srv = S.Server(cuda, models, pool, capacities=(1 << 14, 1 << 16), use_graph=True, keep_results=True)
+    srv.run([S.Request(i, 0.0, "gcn" if i % 2 else "gat", i) for i in range(24)])
+    srv.prime()
+    for kind in ("gcn", "gat"):
+        for pair in srv.buckets[kind]:
+            assert all(b.graph is not None for b in pair), kind
+    reqs = S.make_requests(40, None, 24, seed=13)
+    srv.run(reqs)
+    eager = S.make_requests(40, None, 24, seed=13)
+    run_server(cuda, models, pool, eager, use_graph=False)
+    for r, e in zip(reqs, eager):
+        assert torch.equal(r.result, e.result), r.rid
