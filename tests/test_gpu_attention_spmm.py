@@ -182,3 +182,21 @@ def test_aggregate_sddmm_rejects_unsupported(cuda, graph):
     with pytest.raises(ValueError):
         ops.spmm_edge_map_compact_dot(rp, col, *gg.cmap_bwd, p, win, at, X, X[:-1], "mhd", gg.window,
                                       m_ori=g.n, x_layout="mhd", rows_covered=True)
+
+
+@pytest.mark.parametrize("heads,dim", [(1, 64), (4, 32), (2, 128)])
+def test_aggregate_sddmm_fused_split_windows(cuda, heads, dim):
+    """One block per part splits every window with more than 32 condensed
+    columns: the fused kernel's aggregation goes through the split-window
+    partials and the ordered combine, its dot products straight to the edges;
+    both equal the separate kernels bit for bit."""
+    g = G.chung_lu(900, 10, 2.1, seed=7)
+    gg = GATGraph(g, cuda, build="device", part_size=1)
+    assert int(gg.fwd[4].sum().item()) > 0          # some parts belong to split windows
+    gen = torch.Generator(device="cpu").manual_seed(dim)
+    p = torch.rand(heads, g.nnz, generator=gen).bfloat16().to(cuda)
+    dh = torch.randn(g.n, heads, dim, generator=gen).bfloat16().to(cuda)
+    Xt = torch.randn(g.n, heads, dim, generator=gen).bfloat16().to(cuda)
+    y, dp = gg.aggregate_sddmm(p, dh, Xt)
+    assert torch.equal(y, gg.aggregate(p, dh, transpose=True))
+    assert torch.equal(dp, gg.sddmm(dh, Xt))
