@@ -445,10 +445,11 @@ struct KBlock {
 
     // A_CMAP_DOT: S[r][c] = <xa row r, gathered row c> for the k-block's 32
     // columns, K = the 16T features in chunks of 32 (chunk s2, features
-    // 32 s2 + 8g.. at lane group g, as the SDDMM assigns them), then written
-    // to dot_out per edge through the lane's ids.  The gathered rows are
-    // transposed through LDS: the aggregation holds 8 columns x T features per
-    // lane, the product needs 8 features of one column.
+    // 32 s2 + 8g.. at lane group g, as the SDDMM assigns them), stored to
+    // dot_out from the MFMA output layout through the wave's id tile (filled by
+    // packed() for this block).  The gathered rows are transposed through
+    // LDS: the aggregation holds 8 columns x T features per lane, the product
+    // needs 8 features of one column.
     __device__ __forceinline__ void dot_block(const uint32_t (&xv)[8][XW], const u32x4 *xa, uint8_t *dg) const
     {
         if constexpr (AMODE == A_CMAP_DOT && T >= 2 && (KIND == K_F16 || KIND == K_BF16)) {
@@ -530,6 +531,8 @@ struct KBlock {
 // PIPE 0: each k-block loads its column ids, then its rows, then multiplies.
 // PIPE 1: the next k-block's column ids load while this block multiplies.
 // PIPE 2: two k-blocks per iteration, the next pair's column ids prefetched.
+// The 64-feature A_CMAP_DOT instance is compiled for 5 waves per SIMD (86
+// VGPRs, no spills; 4 otherwise); every other instance as the compiler picks.
 template <int KIND, int T, int AMODE, int PIPE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((AMODE == A_CMAP_DOT && T == 4) ? 5 : 1)))
 void spmm_block_kernel(SpmmArgs a)
