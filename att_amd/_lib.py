@@ -72,6 +72,8 @@ _SIGNATURES = {
     "attgnn_sddmm_block": (_int, [_vp, _vp, _vp, _vp, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _int, _i64, _i64,
                                   _i64, _vp, _int, _i64, _int, _int, _int, _vp]),
     "attgnn_node_scores": (_int, [_vp, _int, _i64, _i64, _vp, _int, _i64, _i64, _i64, _vp, _i64, _i64, _vp]),
+    "attgnn_node_scores_pair": (_int, [_vp, _int, _i64, _i64, _vp, _vp, _int, _i64, _i64, _i64, _vp, _vp, _i64, _i64,
+                                       _vp]),
     "attgnn_edge_from_nodes": (_int, [_vp, _vp, _vp, _vp, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _int,
                                       _i64, _i64, _vp, _int, _i64, _int, _int, _vp]),
     "attgnn_long_rows": (_i64, [_vp, _i64, _vp]),

@@ -426,11 +426,14 @@ __global__ __launch_bounds__(256) void node_score_kernel(const uint8_t *x, int x
 // values are cut into 16-byte chunks (CE elements); lane l takes chunks
 // l, l+16, ...  Each chunk's dot product is summed in element order and added
 // to its head's partial, then the 16 partials of each head are reduced.
-template <int EB, int H>
+// NV = 2: a second vector (vec2 -> s2) from the same loads of X, each score
+// summed in the same order as alone.
+template <int EB, int H, int NV>
 __global__ __launch_bounds__(256) void node_score_vec_kernel(const uint8_t *x, int x_type, int64_t x_hs,
                                                              int64_t x_rs, const uint8_t *vec, int v_type,
                                                              int64_t n_rows, int64_t dim, float *s,
-                                                             int64_t s_hs, int64_t s_rs)
+                                                             int64_t s_hs, int64_t s_rs, const uint8_t *vec2,
+                                                             float *s2)
 {
     constexpr int CE = 16 / EB;
     const int64_t i = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 4;
@@ -438,14 +441,14 @@ __global__ __launch_bounds__(256) void node_score_vec_kernel(const uint8_t *x, i
     if (i >= n_rows) return;          // whole 16-lane group exits together
     const int64_t cph = dim / CE, chunks = cph * H;
     const bool same = v_type == x_type;
-    float acc[H];
+    float acc[H], acc2[H];
 #pragma unroll
-    for (int h = 0; h < H; ++h) acc[h] = 0.f;
+    for (int h = 0; h < H; ++h) acc[h] = acc2[h] = 0.f;
     for (int64_t c = l; c < chunks; c += 16) {
         const int h = int(c / cph);
         const int64_t d0 = (c - int64_t(h) * cph) * CE;
         const u32x4 xv = *reinterpret_cast<const u32x4 *>(x + (int64_t(h) * x_hs + i * x_rs + d0) * EB);
-        float xf[CE], vf[CE];
+        float xf[CE];
         if constexpr (EB == 4) {
 #pragma unroll
             for (int k = 0; k < 4; ++k) xf[k] = __uint_as_float(xv[k]);
@@ -456,29 +459,40 @@ __global__ __launch_bounds__(256) void node_score_vec_kernel(const uint8_t *x, i
                 xf[k] = x_type == ATTGNN_F16 ? float(__builtin_bit_cast(_Float16, uint16_t(b))) : __uint_as_float(b << 16);
             }
         }
-        const uint8_t *vp = vec + (int64_t(h) * dim + d0) * type_bytes(v_type);
-        if (same) {
-            const u32x4 vv = *reinterpret_cast<const u32x4 *>(vp);
-            if constexpr (EB == 4) {
+        auto chunk_dot = [&](const uint8_t *vbase) {
+            float vf[CE];
+            const uint8_t *vp = vbase + (int64_t(h) * dim + d0) * type_bytes(v_type);
+            if (same) {
+                const u32x4 vv = *reinterpret_cast<const u32x4 *>(vp);
+                if constexpr (EB == 4) {
 #pragma unroll
-                for (int k = 0; k < 4; ++k) vf[k] = __uint_as_float(vv[k]);
+                    for (int k = 0; k < 4; ++k) vf[k] = __uint_as_float(vv[k]);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        const uint32_t b = (vv[k >> 1] >> (16 * (k & 1))) & 0xffffu;
+                        vf[k] = v_type == ATTGNN_F16 ? float(__builtin_bit_cast(_Float16, uint16_t(b))) : __uint_as_float(b << 16);
+                    }
+                }
             } else {
 #pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    const uint32_t b = (vv[k >> 1] >> (16 * (k & 1))) & 0xffffu;
-                    vf[k] = v_type == ATTGNN_F16 ? float(__builtin_bit_cast(_Float16, uint16_t(b))) : __uint_as_float(b << 16);
-                }
+                for (int k = 0; k < CE; ++k) vf[k] = load_elt(vp + k * type_bytes(v_type), v_type);
             }
-        } else {
+            float p = 0.f;
 #pragma unroll
-            for (int k = 0; k < CE; ++k) vf[k] = load_elt(vp + k * type_bytes(v_type), v_type);
-        }
-        float p = 0.f;
-#pragma unroll
-        for (int k = 0; k < CE; ++k) p += xf[k] * vf[k];
+            for (int k = 0; k < CE; ++k) p += xf[k] * vf[k];
+            return p;
+        };
+        const float p = chunk_dot(vec);
 #pragma unroll
         for (int hh = 0; hh < H; ++hh)
             if (hh == h) acc[hh] += p;
+        if constexpr (NV == 2) {
+            const float p2 = chunk_dot(vec2);
+#pragma unroll
+            for (int hh = 0; hh < H; ++hh)
+                if (hh == h) acc2[hh] += p2;
+        }
     }
 #pragma unroll
     for (int h = 0; h < H; ++h) {
@@ -486,6 +500,12 @@ __global__ __launch_bounds__(256) void node_score_vec_kernel(const uint8_t *x, i
 #pragma unroll
         for (int off = 8; off >= 1; off >>= 1) v += __shfl_xor(v, off, 16);
         if (l == 0) s[h * s_hs + i * s_rs] = v;
+        if constexpr (NV == 2) {
+            float v2 = acc2[h];
+#pragma unroll
+            for (int off = 8; off >= 1; off >>= 1) v2 += __shfl_xor(v2, off, 16);
+            if (l == 0) s2[h * s_hs + i * s_rs] = v2;
+        }
     }
 }
 
@@ -803,8 +823,9 @@ extern "C" int attgnn_node_scores(const void *X, int x_type, int64_t x_head_stri
         const dim3 grid(unsigned((n_rows * 16 + 255) / 256)), block(256);
 #define ATTGNN_NS_CASE(EBV, HV)                                                                                \
         if (eb == EBV && heads == HV)                                                                          \
-            hipLaunchKernelGGL((node_score_vec_kernel<EBV, HV>), grid, block, 0, s, xb, x_type, x_head_stride, \
-                               x_row_stride, vb, vec_type, n_rows, dim, scores, s_head_stride, s_row_stride);
+            hipLaunchKernelGGL((node_score_vec_kernel<EBV, HV, 1>), grid, block, 0, s, xb, x_type, x_head_stride, \
+                               x_row_stride, vb, vec_type, n_rows, dim, scores, s_head_stride, s_row_stride,  \
+                               nullptr, nullptr);
         ATTGNN_NS_CASE(2, 1) ATTGNN_NS_CASE(2, 2) ATTGNN_NS_CASE(2, 4) ATTGNN_NS_CASE(2, 8)
         ATTGNN_NS_CASE(4, 1) ATTGNN_NS_CASE(4, 2) ATTGNN_NS_CASE(4, 4) ATTGNN_NS_CASE(4, 8)
 #undef ATTGNN_NS_CASE
@@ -814,6 +835,45 @@ extern "C" int attgnn_node_scores(const void *X, int x_type, int64_t x_head_stri
                            x_head_stride, x_row_stride, vb, vec_type, heads, n_rows, dim, scores, s_head_stride,
                            s_row_stride);
     }
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? ATTGNN_OK : int(e);
+}
+
+extern "C" int attgnn_node_scores_pair(const void *X, int x_type, int64_t x_head_stride, int64_t x_row_stride,
+                                       const void *vec0, const void *vec1, int vec_type, int64_t heads,
+                                       int64_t n_rows, int64_t dim, float *scores0, float *scores1,
+                                       int64_t s_head_stride, int64_t s_row_stride, attgnn_stream_t stream)
+{
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (heads <= 0 || n_rows < 0 || dim <= 0 || !is_float_type(x_type) || !is_float_type(vec_type))
+        return ATTGNN_E_ARG;
+    if (n_rows == 0) return ATTGNN_OK;
+    if (!X || !vec0 || !vec1 || !scores0 || !scores1) return ATTGNN_E_ARG;
+    const uint8_t *xb = static_cast<const uint8_t *>(X);
+    const uint8_t *v0 = static_cast<const uint8_t *>(vec0), *v1 = static_cast<const uint8_t *>(vec1);
+    const int eb = type_bytes(x_type), ce = 16 / eb;
+    const bool aligned16 = (reinterpret_cast<uintptr_t>(X) & 15) == 0 &&
+                           (vec_type != x_type || ((reinterpret_cast<uintptr_t>(vec0) |
+                                                    reinterpret_cast<uintptr_t>(vec1)) & 15) == 0);
+    const bool vec_ok = aligned16 && dim % ce == 0 && x_head_stride % ce == 0 && x_row_stride % ce == 0 &&
+                        (heads == 1 || heads == 2 || heads == 4 || heads == 8);
+    if (!vec_ok) {
+        // the generic kernel, once per vector
+        int rc = attgnn_node_scores(X, x_type, x_head_stride, x_row_stride, vec0, vec_type, heads, n_rows, dim,
+                                    scores0, s_head_stride, s_row_stride, stream);
+        if (rc != ATTGNN_OK) return rc;
+        return attgnn_node_scores(X, x_type, x_head_stride, x_row_stride, vec1, vec_type, heads, n_rows, dim,
+                                  scores1, s_head_stride, s_row_stride, stream);
+    }
+    const dim3 grid(unsigned((n_rows * 16 + 255) / 256)), block(256);
+#define ATTGNN_NS2_CASE(EBV, HV)                                                                                \
+    if (eb == EBV && heads == HV)                                                                              \
+        hipLaunchKernelGGL((node_score_vec_kernel<EBV, HV, 2>), grid, block, 0, s, xb, x_type, x_head_stride,  \
+                           x_row_stride, v0, vec_type, n_rows, dim, scores0, s_head_stride, s_row_stride, v1,  \
+                           scores1);
+    ATTGNN_NS2_CASE(2, 1) ATTGNN_NS2_CASE(2, 2) ATTGNN_NS2_CASE(2, 4) ATTGNN_NS2_CASE(2, 8)
+    ATTGNN_NS2_CASE(4, 1) ATTGNN_NS2_CASE(4, 2) ATTGNN_NS2_CASE(4, 4) ATTGNN_NS2_CASE(4, 8)
+#undef ATTGNN_NS2_CASE
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? ATTGNN_OK : int(e);
 }

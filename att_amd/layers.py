@@ -406,9 +406,14 @@ class GATAttention(torch.autograd.Function):
     @staticmethod
     def forward(ctx, Xs, Xt, a0, a1, graph, slope, drop_p, seed, fused=False):
         g = graph
-        # node-major [N, H] scores: one load per neighbour for all heads
-        s0 = ops.node_scores(Xs, a0.to(Xs.dtype).contiguous(), "mhd", out_layout="nh")
-        s1 = ops.node_scores(Xt, a1.to(Xt.dtype).contiguous(), "mhd", out_layout="nh")
+        # node-major [N, H] scores: one load per neighbour for all heads; one
+        # pass over X when both terms read the same features (GAT)
+        if Xs is Xt:
+            s0, s1 = ops.node_scores_pair(Xs, a0.to(Xs.dtype).contiguous(), a1.to(Xs.dtype).contiguous(), "mhd",
+                                          out_layout="nh")
+        else:
+            s0 = ops.node_scores(Xs, a0.to(Xs.dtype).contiguous(), "mhd", out_layout="nh")
+            s1 = ops.node_scores(Xt, a1.to(Xt.dtype).contiguous(), "mhd", out_layout="nh")
         long_rows, _ = g.long_row_lists()
         rp, col, emap, win, at = g.fwd
         if fused:

@@ -607,6 +607,27 @@ def node_scores(X, vec, layout="hmd", out_layout="hn"):
     return s
 
 
+def node_scores_pair(X, vec0, vec1, layout="hmd", out_layout="hn"):
+    """(node_scores(X, vec0), node_scores(X, vec1)) from one pass over X,
+    each bitwise equal to its single call."""
+    dev = _need_gpu("node_scores_pair", X, vec0, vec1)
+    h, n, d, hs, rs = _hmd_strides(X, layout)
+    if vec0.numel() != h * d or vec1.numel() != h * d or vec0.dtype != vec1.dtype:
+        raise ValueError(f"vectors must have {h}*{d} elements of one dtype")
+    if out_layout == "hn":
+        shape, shs, srs = (h, n), n, 1
+    elif out_layout == "nh":
+        shape, shs, srs = (n, h), 1, h
+    else:
+        raise ValueError(f"out_layout must be 'hn' or 'nh', got {out_layout!r}")
+    s0 = torch.empty(shape, dtype=torch.float32, device=dev)
+    s1 = torch.empty(shape, dtype=torch.float32, device=dev)
+    rc = lib().attgnn_node_scores_pair(_ptr(X), _TYPE_OF[X.dtype], hs, rs, _ptr(vec0), _ptr(vec1),
+                                       _TYPE_OF[vec0.dtype], h, n, d, _ptr(s0), _ptr(s1), shs, srs, _stream(dev))
+    check(rc, "attgnn_node_scores_pair")
+    return s0, s1
+
+
 def attention_node_bwd(Xs, Xt, a0, a1, ds0, ds1, base=None, combine=False):
     """Fused backward of s0 = Xs.a0, s1 = Xt.a1 plus the aggregation gradient.
 

@@ -473,6 +473,20 @@ int attgnn_node_scores(const void *X, int x_type, int64_t x_head_stride, int64_t
                        float *scores, int64_t s_head_stride, int64_t s_row_stride, attgnn_stream_t stream);
 
 /*
+ * attgnn_node_scores for two vectors over the same X in one pass:
+ * scores0[h, i] = X_h[i] . vec0_h and scores1[h, i] = X_h[i] . vec1_h, each
+ * bitwise equal to its attgnn_node_scores call (GAT, where both attention
+ * terms read the same projected features).
+ *
+ * Replaces the pair of ATT_SDDMM_v{1,3}.att_*_h_nnz node terms
+ *   (end2end_head/gat_final/mgat_csr/mgat_conv.py:20-31).
+ */
+int attgnn_node_scores_pair(const void *X, int x_type, int64_t x_head_stride, int64_t x_row_stride,
+                            const void *vec0, const void *vec1, int vec_type, int64_t heads, int64_t n_rows,
+                            int64_t dim, float *scores0, float *scores1, int64_t s_head_stride,
+                            int64_t s_row_stride, attgnn_stream_t stream);
+
+/*
  * out[h*nnz + e] = S0(h, i) + S1(h, j) for every edge e = (i, j) of the map, with
  * S0(h,i) = s0[h*s0_head_stride + i*s0_row_stride] (or the typed g0 at the same
  * strides when g0 != NULL) and S1 likewise from s1 (NULL = 0).

@@ -200,3 +200,18 @@ def test_aggregate_sddmm_fused_split_windows(cuda, heads, dim):
     y, dp = gg.aggregate_sddmm(p, dh, Xt)
     assert torch.equal(y, gg.aggregate(p, dh, transpose=True))
     assert torch.equal(dp, gg.sddmm(dh, Xt))
+
+
+@pytest.mark.parametrize("heads,dim", [(1, 64), (4, 64), (8, 32), (3, 16), (4, 12)])
+@pytest.mark.parametrize("dtype", [torch.bfloat16, torch.float16, torch.float32])
+@pytest.mark.parametrize("out_layout", ["hn", "nh"])
+def test_node_scores_pair_equals_two_calls(cuda, heads, dim, dtype, out_layout):
+    """attgnn_node_scores_pair (one pass over X, vector kernel; generic kernel
+    twice for unsupported shapes) equals two attgnn_node_scores calls bit for bit."""
+    gen = torch.Generator(device="cpu").manual_seed(heads * 7 + dim)
+    X = torch.randn(333, heads, dim, generator=gen).to(dtype).to(cuda)
+    a0 = torch.randn(heads, dim, generator=gen).to(dtype).to(cuda)
+    a1 = torch.randn(heads, dim, generator=gen).to(dtype).to(cuda)
+    s0, s1 = ops.node_scores_pair(X, a0, a1, "mhd", out_layout=out_layout)
+    assert torch.equal(s0, ops.node_scores(X, a0, "mhd", out_layout=out_layout))
+    assert torch.equal(s1, ops.node_scores(X, a1, "mhd", out_layout=out_layout))
