@@ -3,6 +3,7 @@
 Usage: ``python -m att_amd.build``.  The library lands in ``att_amd/_build/``;
 it is git-ignored but travels with the working tree.
 """
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -39,18 +40,22 @@ def _run(cmd):
     return out
 
 
-def build(verbose=False):
-    """Compile every source and link libattgnn.so (always rebuilds)."""
+def build(verbose=False, jobs=None):
+    """Compile every source (``jobs`` at a time, default: all) and link
+    libattgnn.so (always rebuilds)."""
     os.makedirs(OUT_DIR, exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for src in HOST_SOURCES:
         obj = os.path.join(OUT_DIR, src + ".o")
-        _run([CXX] + HOST_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
+        cmds.append([CXX] + HOST_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     for src in HIP_SOURCES:
         obj = os.path.join(OUT_DIR, src + ".o")
-        _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
+        cmds.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    workers = jobs or min(len(cmds), os.cpu_count() or 1)
+    with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as pool:
+        list(pool.map(_run, cmds))
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lgomp"])
     shutil.move(tmp, LIB)

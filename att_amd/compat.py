@@ -16,8 +16,13 @@ Differences from the reference (all on the side of strictness):
   * split windows are reduced in a fixed order (bitwise reproducible).
 The window height of a structure is inferred from its array sizes; the block
 width (``wide``) is not recoverable from the arrays, so edge-map ops take it
-as a keyword (default 8, the value the reference's GAT datasets build with,
-end2end_ori/gat_final/mgat_csr/mdataset_fp16.py:62).
+as a keyword defaulting to the width the reference's datasets build with for
+that precision: 8 for the fp16 ops (end2end_ori/gat_final/mgat_csr/mdataset_fp16.py:62)
+and 4 for the tf32 ops (end2end_head/gat_final/mgat32_csr/mdataset_tf32.py:56),
+which is also the block width of the reference kernels' map addressing
+(benchmark-v3/sddmm-benchmark/include/output_tile.h:20 rows of 8, :58-61 rows
+of 4).  A structure whose part spans are not multiples of the width in use is
+rejected (ValueError) instead of being read at the wrong slots.
 """
 import sys
 import types
@@ -25,6 +30,14 @@ import types
 import torch
 
 from . import ops
+
+
+# block width the reference's datasets build edge-id maps with, per precision
+DEFAULT_WIDE = {"fp16": 8, "tf32": 4}
+
+
+def _wide_of(prec, wide):
+    return DEFAULT_WIDE[prec] if wide is None else int(wide)
 
 
 def _window_of(col, tiles):
@@ -74,6 +87,19 @@ def forward_tf32_gnn(row_offsets, col_indices, values, t_window_row, t_atomic, r
     y = ops.spmm_block(row_offsets, col_indices, values, t_window_row, t_atomic, rhs.contiguous(), int(mOri),
                        window, wide, mode="bf16x3")
     return [y]
+
+
+def forward_fp16_gnn_acc(row_offsets, col_indices, values, t_window_row, t_atomic, rhs, dimM1, dimN, mOri, wide=8):
+    """The reference's second binding of the same product (benchmark.cpp:259-288,
+    494; kernel spmmKernel.cu:2067-2200): Y = A.X with the same arguments and
+    output, launched with another grid and with feature columns past dimN
+    masked.  Here both names run the one kernel, which handles any width."""
+    return forward_fp16_gnn(row_offsets, col_indices, values, t_window_row, t_atomic, rhs, dimM1, dimN, mOri, wide)
+
+
+def forward_tf32_gnn_acc(row_offsets, col_indices, values, t_window_row, t_atomic, rhs, dimM1, dimN, mOri, wide=4):
+    """benchmark.cpp:305-334, 495: the tf32 counterpart of forward_fp16_gnn_acc."""
+    return forward_tf32_gnn(row_offsets, col_indices, values, t_window_row, t_atomic, rhs, dimM1, dimN, mOri, wide)
 
 
 # ---------------------------------------------------------------- ATT_Block(_v3)
@@ -126,7 +152,8 @@ def _spmm_map(version, prec, layout):
     mode = "native" if prec == "fp16" else "bf16x3"
     want = torch.float16 if prec == "fp16" else torch.float32
 
-    def op(row_pointers, column_index, degrees, values, window, automic, rhs, wide=8):
+    def op(row_pointers, column_index, degrees, values, window, automic, rhs, wide=None):
+        wide = _wide_of(prec, wide)
         if rhs.dtype != want:
             raise TypeError(f"expected {want} rhs, got {rhs.dtype}")
         w = _window_of(column_index, degrees)
@@ -143,7 +170,8 @@ def _spmm_map(version, prec, layout):
 
 
 def _rowsum_map(version, prec, layout):
-    def op(row_pointers, degrees, values, window, automic, M, wide=8, win_rows=None):
+    def op(row_pointers, degrees, values, window, automic, M, wide=None, win_rows=None):
+        wide = _wide_of(prec, wide)
         if version == 4 and layout == "mhd" and values.dim() == 2:
             vals = values.t().contiguous()       # v4 keeps per-edge values as [nnz, H]
         else:
@@ -179,7 +207,8 @@ def _prec_dtype(prec):
 
 
 def _sddmm(version, prec):
-    def op(row_pointers, column_index, degrees, window, rhs, rhs1, max, nnz, wide=8):
+    def op(row_pointers, column_index, degrees, window, rhs, rhs1, max, nnz, wide=None):
+        wide = _wide_of(prec, wide)
         w = _window_of(column_index, degrees)
         layout = "mhd" if version == 4 else "hmd"
         out = ops.sddmm_block(row_pointers, column_index, degrees, window, rhs.contiguous(), rhs1.contiguous(),
@@ -191,7 +220,8 @@ def _sddmm(version, prec):
 
 
 def _att(version, prec, v2):
-    def op(row_pointers, column_index, degrees, window, X, a0_or_X1, a1_or_a, max, nnz, wide=8):
+    def op(row_pointers, column_index, degrees, window, X, a0_or_X1, a1_or_a, max, nnz, wide=None):
+        wide = _wide_of(prec, wide)
         w = _window_of(column_index, degrees)
         layout = "mhd" if version == 4 else "hmd"
         if v2:
@@ -211,7 +241,8 @@ def _att(version, prec, v2):
 
 
 def _ones(version, prec):
-    def op(row_pointers, column_index, degrees, window, rhs, max, nnz, wide=8):
+    def op(row_pointers, column_index, degrees, window, rhs, max, nnz, wide=None):
+        wide = _wide_of(prec, wide)
         w = _window_of(column_index, degrees)
         g = rhs.contiguous()
         if g.dim() == 3:          # [N, H, 1]
@@ -239,6 +270,8 @@ def _make_modules():
     m = types.ModuleType("SpMM_sr_bcrs", "window-block SpMM (gcn/SpMM/src/benchmark.cpp)")
     m.forward_fp16_gnn = forward_fp16_gnn
     m.forward_tf32_gnn = forward_tf32_gnn
+    m.forward_fp16_gnn_acc = forward_fp16_gnn_acc
+    m.forward_tf32_gnn_acc = forward_tf32_gnn_acc
     mods["SpMM_sr_bcrs"] = m
 
     for name in ("ATT_Block", "ATT_Block_v3"):

@@ -795,3 +795,145 @@ extern "C" int64_t attgnn_long_rows(const int32_t *rowptr, int64_t n, int32_t *r
         }
     return k;
 }
+
+// ---------------------------------------------------------------------------
+// Per-head range of the logits z[h, e=(i,j)] = s0[h,i] + s1[h,j] over the CSR's
+// edges: the minimum, the maximum and the first edge attaining each.  The
+// reference's multi-head GATv2 divides its logits by this range before the
+// exponential (end2end_head/gatv2_final/mgat_csr/mgat_conv.py:184-186), a
+// per-head temperature; the backward needs the edges that set it.
+// Each (value, edge) pair is one 64-bit key -- the value's order-preserving
+// bits above the edge index (for the maximum, the index complemented so the
+// first edge wins a tie) -- reduced with 64-bit integer min / max atomics, so
+// the result does not depend on the order the edges are visited in.
+namespace {
+
+__device__ __forceinline__ uint32_t order_bits(float f)
+{
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ __forceinline__ float from_order_bits(uint32_t o)
+{
+    return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
+}
+
+struct RangeArgs {
+    const int32_t *rowptr, *col;
+    const float *s0, *s1;
+    int64_t n, s_hs, s_rs;
+    unsigned long long *keys;   // [heads][2]: min key, max key
+    float *range;               // [heads][3]: min, max, 1/(max - min) (0 when max == min or no edges)
+    int32_t *where;             // [heads][4]: row, column of the minimum; row, column of the maximum (-1: none)
+    int heads;
+};
+
+__global__ void range_init_kernel(RangeArgs a)
+{
+    const int h = threadIdx.x;
+    if (h < a.heads) {
+        a.keys[2 * h] = ~0ull;
+        a.keys[2 * h + 1] = 0ull;
+    }
+}
+
+__device__ __forceinline__ unsigned long long umin64(unsigned long long x, unsigned long long y) { return x < y ? x : y; }
+__device__ __forceinline__ unsigned long long umax64(unsigned long long x, unsigned long long y) { return x > y ? x : y; }
+
+__device__ __forceinline__ unsigned long long shfl_xor64(unsigned long long v, int o)
+{
+    const uint32_t lo = __shfl_xor(uint32_t(v), o);
+    const uint32_t hi = __shfl_xor(uint32_t(v >> 32), o);
+    return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+
+// grid (blocks, heads); 16-lane groups walk rows (grid-stride), a lane takes
+// every 16th edge of its group's row; one atomic pair per wave and head.
+__global__ __launch_bounds__(256) void range_kernel(RangeArgs a)
+{
+    const int h = blockIdx.y;
+    const int lane16 = threadIdx.x & 15;
+    const int64_t group = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 4;
+    const int64_t groups = (int64_t(gridDim.x) * blockDim.x) >> 4;
+    unsigned long long kmin = ~0ull, kmax = 0ull;
+    for (int64_t i = group; i < a.n; i += groups) {
+        const float r = a.s0[h * a.s_hs + i * a.s_rs];
+        const int32_t e0 = a.rowptr[i], e1 = a.rowptr[i + 1];
+        for (int32_t e = e0 + lane16; e < e1; e += 16) {
+            const float z = r + a.s1[h * a.s_hs + int64_t(a.col[e]) * a.s_rs];
+            const unsigned long long hi = static_cast<unsigned long long>(order_bits(z)) << 32;
+            kmin = umin64(kmin, hi | uint32_t(e));
+            kmax = umax64(kmax, hi | (0xffffffffu - uint32_t(e)));
+        }
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        kmin = umin64(kmin, shfl_xor64(kmin, o));
+        kmax = umax64(kmax, shfl_xor64(kmax, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (kmin != ~0ull) atomicMin(a.keys + 2 * h, kmin);
+        if (kmax != 0ull) atomicMax(a.keys + 2 * h + 1, kmax);
+    }
+}
+
+// Row of edge e: the last i with rowptr[i] <= e (rows are non-decreasing).
+__device__ int32_t row_of_edge(const int32_t *rowptr, int64_t n, int32_t e)
+{
+    int64_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) >> 1;
+        if (rowptr[mid] <= e) lo = mid; else hi = mid - 1;
+    }
+    return int32_t(lo);
+}
+
+__global__ void range_finish_kernel(RangeArgs a)
+{
+    const int h = threadIdx.x;
+    if (h >= a.heads) return;
+    const unsigned long long kmin = a.keys[2 * h], kmax = a.keys[2 * h + 1];
+    float *r = a.range + 3 * h;
+    int32_t *w = a.where + 4 * h;
+    if (kmin == ~0ull) {           // no edges
+        r[0] = 0.f; r[1] = 0.f; r[2] = 0.f;
+        w[0] = w[1] = w[2] = w[3] = -1;
+        return;
+    }
+    const float zmin = from_order_bits(uint32_t(kmin >> 32)), zmax = from_order_bits(uint32_t(kmax >> 32));
+    const int32_t emin = int32_t(uint32_t(kmin)), emax = int32_t(0xffffffffu - uint32_t(kmax));
+    r[0] = zmin;
+    r[1] = zmax;
+    r[2] = zmax > zmin ? 1.f / (zmax - zmin) : 0.f;
+    w[0] = row_of_edge(a.rowptr, a.n, emin);
+    w[1] = a.col[emin];
+    w[2] = row_of_edge(a.rowptr, a.n, emax);
+    w[3] = a.col[emax];
+}
+
+}  // namespace
+
+extern "C" int attgnn_edge_score_range(const int32_t *rowptr, const int32_t *col, int64_t n, int64_t nnz,
+                                       int64_t heads, const float *s0, const float *s1, int64_t s_head_stride,
+                                       int64_t s_row_stride, unsigned long long *keys, float *range, int32_t *where,
+                                       attgnn_stream_t stream)
+{
+    if (n < 0 || nnz < 0 || nnz > INT32_MAX || heads <= 0 || heads > 256) return ATTGNN_E_ARG;
+    if (!keys || !range || !where) return ATTGNN_E_ARG;
+    if (nnz > 0 && (n == 0 || !rowptr || !col || !s0 || !s1)) return ATTGNN_E_ARG;
+    RangeArgs a{};
+    a.rowptr = rowptr; a.col = col; a.s0 = s0; a.s1 = s1; a.n = n; a.s_hs = s_head_stride; a.s_rs = s_row_stride;
+    a.keys = keys; a.range = range; a.where = where; a.heads = int(heads);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(range_init_kernel, dim3(1), dim3(256), 0, s, a);
+    if (nnz > 0) {
+        // about 16 rows per group: enough blocks to fill the chip without
+        // one atomic pair per handful of rows
+        const int64_t blocks = std::min<int64_t>(std::max<int64_t>((n + 255) / 256, 1), 2048);
+        hipLaunchKernelGGL(range_kernel, dim3(unsigned(blocks), unsigned(heads)), dim3(256), 0, s, a);
+    }
+    hipLaunchKernelGGL(range_finish_kernel, dim3(1), dim3(256), 0, s, a);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? ATTGNN_OK : int(e);
+}

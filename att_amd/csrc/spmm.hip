@@ -84,10 +84,11 @@ struct SpmmArgs {
     uint32_t thresh;
     uint64_t seed;
     // A_CMAP_DOT: dot_out[h*val_head_stride + e] = <dot_x[h, window row], X[h, column]>
-    // for every edge e in the structure; dot_x has X's type and strides
+    // for every edge e in the structure; dot_x has X's type and layout and
+    // dot_rows rows: element (h, r, f) at dot_x[h*dot_head_stride + r*dot_row_stride + f]
     const uint8_t *dot_x;
     float *dot_out;
-    int64_t dot_rows;
+    int64_t dot_rows, dot_head_stride, dot_row_stride;
 };
 
 // Store n consecutive fp32 values as y_type (F32 / F16 / BF16).
@@ -621,7 +622,7 @@ void spmm_block_kernel(SpmmArgs a)
         k.dot_ids = reinterpret_cast<int *>(dg + dot_rows_bytes<T>());
         const int64_t drow = int64_t(a.win[part]) * W + k.n;
         const bool ok = k.a_row_ok && drow < a.dot_rows;
-        const uint8_t *xr = a.dot_x + (int64_t(head) * a.x_head_stride + drow * a.x_row_stride) * 2;
+        const uint8_t *xr = a.dot_x + (int64_t(head) * a.dot_head_stride + drow * a.dot_row_stride) * 2;
 #pragma unroll
         for (int s2 = 0; s2 < T / 2; ++s2)
             xa[s2] = ok ? *reinterpret_cast<const u32x4 *>(xr + (32 * s2 + 8 * g) * 2) : u32x4{0u, 0u, 0u, 0u};
@@ -1079,6 +1080,10 @@ int edge_map_run(int amode, const int32_t *rp, const int32_t *col, const int32_t
         a.dot_x = static_cast<const uint8_t *>(dot_x);
         a.dot_out = dot_out;
         a.dot_rows = dot_rows;
+        // dot_x has X's layout with its own row count: in [H, rows, D] the head
+        // stride is dot_rows * D, in [rows, H, D] it is D as for X
+        if (x_layout == ATTGNN_LAYOUT_HMD) { a.dot_head_stride = dot_rows * dim_n; a.dot_row_stride = dim_n; }
+        else { a.dot_head_stride = dim_n; a.dot_row_stride = heads * dim_n; }
         a.pipe = 1;
         a.xcd_slices = 0;
     }

@@ -401,10 +401,19 @@ class GATAttention(torch.autograd.Function):
     Rebuilding costs time (4 heads x 64 on a 2.48M-entry graph: aggregation
     282 -> 381 us forward, 357 -> 403 us backward) and saves the H * nnz
     probabilities, so it is the choice when those do not fit.
+
+    minmax=True: the logits of each head are first mapped to
+    (z - min_h) / (max_h - min_h), min and max over all edges of the head, as
+    the reference's multi-head GATv2 does before its exponential
+    (end2end_head/gatv2_final/mgat_csr/mgat_conv.py:184-188).  The softmax is
+    shift-invariant, so this is a per-head temperature 1 / (max_h - min_h):
+    s0 and s1 are scaled by it (LeakyReLU commutes with a positive scale) and
+    the gradient through the temperature reaches the two edges that set min_h
+    and max_h.
     """
 
     @staticmethod
-    def forward(ctx, Xs, Xt, a0, a1, graph, slope, drop_p, seed, fused=False):
+    def forward(ctx, Xs, Xt, a0, a1, graph, slope, drop_p, seed, fused=False, minmax=False):
         g = graph
         # node-major [N, H] scores: one load per neighbour for all heads; one
         # pass over X when both terms read the same features (GAT)
@@ -414,6 +423,12 @@ class GATAttention(torch.autograd.Function):
         else:
             s0 = ops.node_scores(Xs, a0.to(Xs.dtype).contiguous(), "mhd", out_layout="nh")
             s1 = ops.node_scores(Xt, a1.to(Xt.dtype).contiguous(), "mhd", out_layout="nh")
+        raw = None
+        if minmax:
+            rng, where = ops.edge_score_range(g.rowptr, g.col, s0, s1, s_layout="nh")
+            scale = rng[:, 2].contiguous()
+            raw = (s0, s1, scale, where)
+            s0, s1 = s0 * scale, s1 * scale
         long_rows, _ = g.long_row_lists()
         rp, col, emap, win, at = g.fwd
         if fused:
@@ -429,14 +444,15 @@ class GATAttention(torch.autograd.Function):
             p, lse = ops.edge_softmax_fwd(g.rowptr, g.col, s0, s1, slope, drop_p, seed, p_dtype=Xt.dtype,
                                           s_layout="nh", long_rows=long_rows)
             h = g.aggregate(p, Xt, out_dtype=Xt.dtype)
-        ctx.save_for_backward(Xs, Xt, a0, a1, s0, s1, lse, p)
+        ctx.save_for_backward(Xs, Xt, a0, a1, s0, s1, lse, p, *(raw or ()))
         ctx.graph, ctx.slope, ctx.drop_p, ctx.seed, ctx.fused = g, slope, drop_p, seed, fused
         ctx.shared = Xs is Xt
+        ctx.minmax = minmax
         return h
 
     @staticmethod
     def backward(ctx, dh):
-        Xs, Xt, a0, a1, s0, s1, lse, p = ctx.saved_tensors
+        Xs, Xt, a0, a1, s0, s1, lse, p = ctx.saved_tensors[:8]
         g = ctx.graph
         n, heads, dim = Xt.shape
         dh = dh.to(Xt.dtype).contiguous()
@@ -457,13 +473,39 @@ class GATAttention(torch.autograd.Function):
         ds0, ds1 = ops.edge_softmax_bwd_sums(g.rowptr, g.col, s0, s1, lse, dp, ctx.slope, ctx.drop_p, ctx.seed,
                                              g.t_rowptr, g.t_perm, s_layout="nh", long_rows=long_rows,
                                              t_long_rows=t_long_rows)
+        if ctx.minmax:
+            ds0, ds1 = _minmax_score_grads(*ctx.saved_tensors[8:], ds0, ds1)
         # s0 = Xs.a0, s1 = Xt.a1: node-term gradients fused with the aggregation
         # gradient (one pass over X; GAT's shared X W gets the summed gradient)
         shared = ctx.shared and Xs.data_ptr() == Xt.data_ptr()
         dXs, dXt, da0, da1 = ops.attention_node_bwd(Xs.contiguous(), Xt.contiguous(), a0, a1, ds0, ds1,
                                                     base=dXt, combine=shared)
         return (dXs, dXt, da0.to(a0.dtype).reshape(a0.shape), da1.to(a1.dtype).reshape(a1.shape),
-                None, None, None, None, None)
+                None, None, None, None, None, None)
+
+
+def _minmax_score_grads(s0, s1, scale, where, ds0s, ds1s):
+    """Gradients of the raw node terms under the per-head temperature c_h =
+    1 / (max_h - min_h), from those of the scaled terms s0 c, s1 c (ds0s /
+    ds1s, fp32 [H, N]).  With U_h = sum_i s0[i,h] ds0s[h,i] + s1[i,h] ds1s[h,i]
+    = dL/dc_h and dc/dmax = -c^2, dc/dmin = c^2:  ds0 = c ds0s, ds1 = c ds1s,
+    then -U c^2 at the row and column of the maximum's edge and +U c^2 at
+    those of the minimum's (each update one element per head, applied in a
+    fixed order, so the result is deterministic).  c = 0 (all logits equal,
+    or no edges: then the edge indices are -1 and the updates add zero to the
+    last node) leaves zero gradients."""
+    c = scale.view(-1, 1)
+    U = (s0.t() * ds0s).sum(1) + (s1.t() * ds1s).sum(1)          # [H]
+    k = U * scale * scale
+    ds0 = ds0s * c
+    ds1 = ds1s * c
+    w = where.to(torch.int64)
+    hs = torch.arange(scale.numel(), device=scale.device)
+    ds0[hs, w[:, 2]] -= k
+    ds1[hs, w[:, 3]] -= k
+    ds0[hs, w[:, 0]] += k
+    ds1[hs, w[:, 1]] += k
+    return ds0, ds1
 
 
 class GATv2Attention(torch.autograd.Function):
@@ -520,20 +562,31 @@ class GATConv_multi(nn.Module):
             seed = self._step
             self._step += 1
         return GATAttention.apply(Xp, Xp, self.a0.view(self.head, -1), self.a1.view(self.head, -1), inputInfo,
-                                  self.alpha, drop, seed, self.fused)
+                                  self.alpha, drop, seed, self.fused, False)
 
 
 class GATv2Conv_multi(nn.Module):
-    """end2end_head/gatv2_final/mgat_csr/mgat_conv.py:151-198 with the single-head
-    file's logits (end2end_ori/gatv2_final/mgat_csr/mgat_conv.py:174-196):
-    X0 = LReLU(X W0), X1 = LReLU(X W1), logit = a.X0[i] + a.X1[j], h = sum p X1.
+    """The reference's GATv2 layers: X0 = LReLU(X W0), X1 = LReLU(X W1),
+    logit z = a.X0[i] + a.X1[j], h = sum_j p X1[j] with p the row softmax.
+
+    Two reference files define a class of this name.  Constructed with a head
+    count (``GATv2Conv_multi(in, out, dropout, alpha, head)``) it is the
+    multi-head one, end2end_head/gatv2_final/mgat_csr/mgat_conv.py:151-198,
+    which maps each head's logits to (z - min) / (max - min) over all its edges
+    before the exponential (:184-186; ``minmax=True``).  Without a head count
+    it is the single-head one, end2end_ori/gatv2_final/mgat_csr/mgat_conv.py:
+    150-202, whose logits go to the exponential as they are (``minmax=False``).
+    ``minmax`` overrides either default.
 
     ``true_gatv2=True``: the attention of the GATv2 paper instead,
     X0 = X W0, X1 = X W1, logit = a . LReLU(X0[i] + X1[j]), h = sum p X1
     (fused per-edge kernels; no counterpart in the reference)."""
 
-    def __init__(self, input_dim, output_dim, dropout, alpha, head=1, dtype=torch.bfloat16, true_gatv2=False):
+    def __init__(self, input_dim, output_dim, dropout, alpha, head=None, dtype=torch.bfloat16, true_gatv2=False,
+                 minmax=None):
         super().__init__()
+        self.minmax = (head is not None) if minmax is None else bool(minmax)
+        head = 1 if head is None else head
         self.true_gatv2 = true_gatv2
         self.alpha, self.head, self.output_dim, self.dtype = alpha, head, output_dim, dtype
         gain = nn.init.calculate_gain("relu")
@@ -560,4 +613,4 @@ class GATv2Conv_multi(nn.Module):
             return GATv2Attention.apply(X0, X1, a, inputInfo, self.alpha, drop, seed)
         X0 = F.leaky_relu(LinearSplitK.apply(Xd, self.fc0.weight), self.alpha).view(-1, self.head, self.output_dim)
         X1 = F.leaky_relu(LinearSplitK.apply(Xd, self.fc1.weight), self.alpha).view(-1, self.head, self.output_dim)
-        return GATAttention.apply(X0, X1, a, a, inputInfo, 1.0, drop, seed, self.fused)
+        return GATAttention.apply(X0, X1, a, a, inputInfo, 1.0, drop, seed, self.fused, self.minmax)

@@ -63,6 +63,39 @@ def _need_i32(name, *tensors):
             raise TypeError(f"{name}: index arrays must be int32 (got {t.dtype})")
 
 
+def check_part_spans(rp, wide, name="structure"):
+    """Raise unless every part's condensed-column span rp[2p+2] - rp[2p] is a
+    multiple of ``wide``: tiles and edge-id maps are laid out in blocks of
+    ``wide`` columns, so a structure built with another block width would be
+    read at the wrong slots (and, for a span that is not a multiple, past its
+    end).  One device-to-host copy of ``rp`` per structure: the result is kept
+    on the tensor with its version counter and reused until ``rp`` is written
+    in place.  Skipped while a stream is being captured into a graph (no host
+    copy is possible there; captures replay structures checked when eager)."""
+    if rp.numel() < 3:
+        return
+    try:
+        version = rp._version
+    except RuntimeError:          # inference tensors carry no version counter
+        version = None
+    tag = (int(wide), version, rp.numel())
+    if version is not None and getattr(rp, "_attgnn_spans_ok", None) == tag:
+        return
+    if rp.is_cuda and torch.cuda.is_current_stream_capturing():
+        return
+    r = rp.detach().to("cpu", torch.int64)
+    bad = torch.remainder(r[2::2] - r[:-2:2], int(wide)).nonzero()
+    if bad.numel():
+        p = int(bad[0, 0])
+        raise ValueError(f"{name}: part {p} spans {int(r[2 * p + 2] - r[2 * p])} condensed columns, not a multiple "
+                         f"of wide={wide}; was the structure built with another block width?")
+    if version is not None:
+        try:
+            rp._attgnn_spans_ok = tag
+        except (AttributeError, RuntimeError):
+            pass
+
+
 # --------------------------------------------------------------------------
 # preprocessing (host)
 # --------------------------------------------------------------------------
@@ -310,6 +343,7 @@ def spmm_block(rp, col, val, window_of_part, atomic, X, m_ori, window, wide, mod
         raise ValueError("X must be 2-D [rows, dim]")
     if rp.numel() != 2 * window_of_part.numel() + 1 or atomic.numel() != window_of_part.numel():
         raise ValueError("rp / window / atomic sizes disagree")
+    check_part_spans(rp, wide, "spmm_block")
     parts = window_of_part.numel()
     n_x, dim = X.shape
     y = out if out is not None else torch.empty((m_ori, dim), dtype=out_dtype, device=dev)
@@ -409,15 +443,20 @@ def _edge_map(compact, rp, col, emap, edge_val, window_of_part, atomic, X, out_l
     tail = (_ptr(ev), _TYPE_OF[ev.dtype], nnz, _ptr(window_of_part), _ptr(atomic), parts, _ptr(X), _TYPE_OF[X.dtype],
             xl, heads, n_x, _ptr(y), _TYPE_OF[out_dtype], m, dim, layout, window)
     if dot_x is not None:
-        if dot_x.shape != X.shape or dot_x.dtype != X.dtype or not dot_x.is_contiguous() or dot_x.device != X.device:
-            raise ValueError("dot_x must be a contiguous tensor of X's shape, type and device")
+        if dot_x.dim() == 2:
+            dot_x = dot_x.unsqueeze(0 if x_layout == "hmd" else 1)
+        dh, dot_rows, dd = dot_x.shape if x_layout == "hmd" else (dot_x.shape[1], dot_x.shape[0], dot_x.shape[2])
+        if (dh, dd) != (heads, dim) or dot_x.dtype != X.dtype or not dot_x.is_contiguous() or \
+                dot_x.device != X.device:
+            raise ValueError("dot_x must be a contiguous tensor of X's heads, width, type, layout and device")
         dot = torch.empty((heads, nnz), dtype=torch.float32, device=dev)
         rc = lib().attgnn_spmm_edge_map_compact_dot(_ptr(rp), _ptr(col), _ptr(compact[0]), _ptr(compact[1]),
-                                                    _ptr(emap), *tail, _ptr(dot_x), n_x, _ptr(dot), _ptr(ws), ws_n,
-                                                    flags, _stream(dev))
+                                                    _ptr(emap), *tail, _ptr(dot_x), dot_rows, _ptr(dot), _ptr(ws),
+                                                    ws_n, flags, _stream(dev))
         check(rc, "attgnn_spmm_edge_map_compact_dot")
         return y, dot
     if compact is None:
+        check_part_spans(rp, wide, "spmm_edge_map")
         rc = lib().attgnn_spmm_edge_map(_ptr(rp), _ptr(col), _ptr(emap), *tail, wide, _mode(mode, X.dtype), _ptr(ws),
                                         ws_n, flags, _stream(dev))
         check(rc, "attgnn_spmm_edge_map")
@@ -433,8 +472,9 @@ def spmm_edge_map_compact_dot(rp, col, masks, block_base, ids, edge_val, window_
                               rows_covered=False, workspace=None):
     """:func:`spmm_edge_map_compact` (native mode) that also returns, for every
     edge e at window row r and column c of the structure, dot[h, e] =
-    <dot_x[h, r, :], X[h, c, :]> in fp32 (dot_x: X's shape, type and layout;
-    fp16 / bf16, 32, 64 or 128 features).  Over A^T's structure with A's edge
+    <dot_x[h, r, :], X[h, c, :]> in fp32 (dot_x: X's heads, width, type and
+    layout, any row count -- rows past it read as zero; fp16 / bf16, 32, 64 or
+    128 features).  Over A^T's structure with A's edge
     ids and X = dL/dh, dot_x = Xt this is the GAT backward's aggregation
     gradient and the SDDMM <dh[i], Xt[j]> in one pass.  Returns (Y, dot);
     bitwise equal to the two separate products."""
@@ -475,6 +515,7 @@ def spmm_attention(rp, col, emap, window_of_part, atomic, X, out_layout, window,
     dev = _need_gpu("spmm_attention", rp, col, emap, window_of_part, atomic, X, row_logit, col_logit, row_offset,
                     col_offset)
     _need_i32("spmm_attention", rp, col, emap, window_of_part, atomic)
+    check_part_spans(rp, wide, "spmm_attention")
     if X.dim() == 2:
         X = X.unsqueeze(0 if x_layout == "hmd" else 1)
     if x_layout == "hmd":
@@ -516,6 +557,7 @@ def rowsum_edge_map(rp, emap, window_of_part, edge_val, m_rows, out_layout, wind
     """Row sums of per-edge values; fp32 [H, M] ("hmd") or [M, H] ("mhd")."""
     dev = _need_gpu("rowsum_edge_map", rp, emap, window_of_part, edge_val)
     _need_i32("rowsum_edge_map", rp, emap, window_of_part)
+    check_part_spans(rp, wide, "rowsum_edge_map")
     ev = edge_val if edge_val.dim() == 2 else edge_val.reshape(1, -1)
     heads, nnz = ev.shape
     layout = {"hmd": _lib.LAYOUT_HMD, "mhd": _lib.LAYOUT_MHD}[out_layout]
@@ -548,6 +590,7 @@ def sddmm_block(rp, col, emap, window_of_part, A, B, nnz, window, wide, layout="
     """out[h, e=(i,j)] = <A_h[i], B_h[j]> in CSR edge order ([H, nnz])."""
     dev = _need_gpu("sddmm_block", rp, col, emap, window_of_part, A, B)
     _need_i32("sddmm_block", rp, col, emap, window_of_part)
+    check_part_spans(rp, wide, "sddmm_block")
     if A.dtype != B.dtype:
         raise TypeError("A and B must share a dtype")
     ha, na, da, ahs, ars = _hmd_strides(A, layout)
@@ -668,6 +711,7 @@ def edge_from_nodes(rp, col, emap, window_of_part, nnz, heads, n_rows, window, w
     """out[h, e=(i,j)] = S0[h, i] + S1[h, j] over the map's edges ([H, nnz])."""
     dev = _need_gpu("edge_from_nodes", rp, col, emap, window_of_part, s0, s1, g0)
     _need_i32("edge_from_nodes", rp, col, emap, window_of_part)
+    check_part_spans(rp, wide, "edge_from_nodes")
     out = torch.zeros((heads, nnz), dtype=out_dtype, device=dev)
     if g0 is not None:
         hs0, rs0 = g0_strides
@@ -690,6 +734,30 @@ def dtype_code(dtype):
 # --------------------------------------------------------------------------
 # fused edge softmax (CSR rows)
 # --------------------------------------------------------------------------
+
+def edge_score_range(rowptr, col, s0, s1, s_layout="nh"):
+    """Per-head range of the logits s0[i] + s1[j] over the CSR's edges:
+    (range fp32 [H, 3] = min, max, 1/(max - min) or 0; where int32 [H, 4] =
+    row, column of the minimum's edge, then of the maximum's; -1 without
+    edges).  s0 / s1 fp32 node terms, [N, H] ("nh") or [H, N] ("hn")."""
+    dev = _need_gpu("edge_score_range", rowptr, col, s0, s1)
+    _need_i32("edge_score_range", rowptr, col)
+    if s0.shape != s1.shape or s0.dtype != torch.float32 or s1.dtype != torch.float32 or s0.dim() != 2:
+        raise ValueError("s0 and s1 must be fp32 [N, H] / [H, N] tensors of one shape")
+    heads = s0.shape[1] if s_layout == "nh" else s0.shape[0]
+    n_nodes = s0.shape[0] if s_layout == "nh" else s0.shape[1]
+    n = rowptr.numel() - 1
+    if n > n_nodes:
+        raise ValueError(f"{n} CSR rows but node terms for {n_nodes} nodes")
+    hs, rs = (1, heads) if s_layout == "nh" else (n_nodes, 1)
+    keys = torch.empty(2 * heads, dtype=torch.int64, device=dev)
+    rng = torch.empty((heads, 3), dtype=torch.float32, device=dev)
+    where = torch.empty((heads, 4), dtype=torch.int32, device=dev)
+    check(lib().attgnn_edge_score_range(_ptr(rowptr), _ptr(col), n, col.numel(), heads, _ptr(s0), _ptr(s1), hs, rs,
+                                        _ptr(keys), _ptr(rng), _ptr(where), _stream(dev)),
+          "attgnn_edge_score_range")
+    return rng, where
+
 
 def long_rows(rowptr):
     """int32 device tensor of the rows with more than ATTGNN_LONG_ROW (256)

@@ -673,6 +673,25 @@ int attgnn_gather_rowsum(const int32_t *rowptr, const int64_t *perm, int64_t n, 
                          int64_t heads, const float *vals, float *out, const int32_t *long_rows,
                          int64_t n_long, attgnn_stream_t stream);
 
+/*
+ * Per-head range of the logits z[h, e=(i,j)] = s0[h,i] + s1[h,j] over the
+ * CSR's nnz edges (s0 / s1 strided as above): range[3h..3h+2] = (min, max,
+ * 1/(max - min)), the last 0 when max == min or there are no edges; where[4h..
+ * 4h+3] = (row, column) of the first edge in CSR order attaining the minimum,
+ * then of the one attaining the maximum (-1 without edges).  keys: a device
+ * workspace of 2*heads 64-bit words, overwritten.  Deterministic (integer
+ * min / max reductions).  heads <= 256, nnz < 2^31.
+ *
+ * Replaces the per-head torch.max / torch.min over the logits and the
+ * division by their difference of the multi-head GATv2 layer
+ *   (end2end_head/gatv2_final/mgat_csr/mgat_conv.py:184-186); the layer scales
+ *   s0 and s1 by range[3h+2] before attgnn_edge_softmax_fwd (the softmax is
+ *   shift-invariant, so subtracting the minimum changes nothing).
+ */
+int attgnn_edge_score_range(const int32_t *rowptr, const int32_t *col, int64_t n, int64_t nnz, int64_t heads,
+                            const float *s0, const float *s1, int64_t s_head_stride, int64_t s_row_stride,
+                            unsigned long long *keys, float *range, int32_t *where, attgnn_stream_t stream);
+
 /* ====================================================================== */
 /* Block-diagonal packing for batched inference (device)                   */
 /* ====================================================================== */

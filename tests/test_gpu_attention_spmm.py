@@ -179,9 +179,38 @@ def test_aggregate_sddmm_rejects_unsupported(cuda, graph):
     with pytest.raises(RuntimeError):
         ops.spmm_edge_map_compact_dot(rp, col, *gg.cmap_bwd, p, win, at, X, X, "mhd", gg.window, m_ori=g.n,
                                       x_layout="mhd", rows_covered=True)
-    with pytest.raises(ValueError):
-        ops.spmm_edge_map_compact_dot(rp, col, *gg.cmap_bwd, p, win, at, X, X[:-1], "mhd", gg.window,
-                                      m_ori=g.n, x_layout="mhd", rows_covered=True)
+    with pytest.raises(ValueError):          # dot_x of another width
+        ops.spmm_edge_map_compact_dot(rp, col, *gg.cmap_bwd, p, win, at, X, X[:, :, :32].contiguous(), "mhd",
+                                      gg.window, m_ori=g.n, x_layout="mhd", rows_covered=True)
+
+
+@pytest.mark.parametrize("x_layout", ["mhd", "hmd"])
+@pytest.mark.parametrize("heads,dim", [(1, 64), (3, 32), (2, 128)])
+def test_aggregate_sddmm_dot_rows_differ(cuda, graph, x_layout, heads, dim):
+    """dot_x with fewer rows than X (A^T of a rectangular A): its rows are
+    addressed with its own head stride in [H, rows, D] and rows past its end
+    read as zero, so the products equal those against dot_x padded with zero
+    rows, and the aggregation is unchanged."""
+    gg, g = graph
+    rp, col, _, win, at = gg.bwd
+    gen = torch.Generator(device="cpu").manual_seed(heads * 17 + dim)
+    cut = g.n - 37
+    p = torch.rand(heads, g.nnz, generator=gen).bfloat16().to(cuda)
+    dh = torch.randn(g.n, heads, dim, generator=gen).bfloat16().to(cuda)
+    Xt = torch.randn(cut, heads, dim, generator=gen).bfloat16().to(cuda)
+    pad = torch.cat([Xt, torch.zeros(g.n - cut, heads, dim, dtype=Xt.dtype, device=cuda)])
+    if x_layout == "hmd":
+        dh, Xt, pad = (t.permute(1, 0, 2).contiguous() for t in (dh, Xt, pad))
+    args = (rp, col, *gg.cmap_bwd, p, win, at, dh)
+    kw = dict(m_ori=g.n, x_layout=x_layout, rows_covered=True)
+    y_cut, dp_cut = ops.spmm_edge_map_compact_dot(*args, Xt, "mhd", gg.window, **kw)
+    y_pad, dp_pad = ops.spmm_edge_map_compact_dot(*args, pad, "mhd", gg.window, **kw)
+    assert torch.equal(y_cut, y_pad)
+    assert torch.equal(dp_cut, dp_pad)
+    # the window rows of A^T are A's columns: products of edges (i, j) with j >= cut are zero
+    cols = torch.from_numpy(g.indices.astype(np.int64)).to(cuda)
+    assert torch.count_nonzero(dp_cut[:, cols >= cut]) == 0
+    assert torch.count_nonzero(dp_cut[:, cols < cut]) > 0
 
 
 @pytest.mark.parametrize("heads,dim", [(1, 64), (4, 32), (2, 128)])

@@ -33,11 +33,17 @@ def dense_adj(g):
     return A
 
 
-def gat_reference(X, Ws, Wt, a0, a1, A, heads, dim, slope, mask=None, q=0.0, v2_alpha=None):
+def _round_through(x, dtype):
+    """x rounded to ``dtype`` in the forward, identity in the backward (the
+    projection's output is stored rounded; its gradient is taken at that point)."""
+    return x if dtype is None else x + (x.to(dtype).double() - x).detach()
+
+
+def gat_reference(X, Ws, Wt, a0, a1, A, heads, dim, slope, mask=None, q=0.0, v2_alpha=None, round_to=None):
     """FP64 dense reference of the attention layer; returns [N, H, D]."""
     n = X.shape[0]
-    Xs = (X @ Ws.t()).view(n, heads, dim)
-    Xt = Xs if Wt is None else (X @ Wt.t()).view(n, heads, dim)
+    Xs = _round_through(X @ Ws.t(), round_to).view(n, heads, dim)
+    Xt = Xs if Wt is None else _round_through(X @ Wt.t(), round_to).view(n, heads, dim)
     if v2_alpha is not None:
         Xs = F.leaky_relu(Xs, v2_alpha)
         Xt = F.leaky_relu(Xt, v2_alpha)
@@ -76,7 +82,7 @@ def test_gcn_layer_forward_backward(cuda):
         assert gerr < 1e-5, (symmetric, gerr)
 
 
-@pytest.mark.parametrize("heads,dim", [(1, 64), (2, 32), (4, 16), (8, 8)])
+@pytest.mark.parametrize("heads,dim", [(1, 64), (2, 32), (4, 16), (8, 8), (2, 64), (4, 64), (8, 64)])
 @pytest.mark.parametrize("dtype,tol", [(torch.float32, 2e-4), (torch.bfloat16, 3e-2)])
 def test_gat_layer_vs_fp64(cuda, heads, dim, dtype, tol):
     g = small_graph(350, 6, seed=heads)
@@ -89,14 +95,16 @@ def test_gat_layer_vs_fp64(cuda, heads, dim, dtype, tol):
     (out.float() * go).sum().backward()
 
     A = dense_adj(g)
-    W = layer.fc.weight.detach().double().cpu().requires_grad_(True)
-    a0 = layer.a0.detach().double().cpu().view(heads, dim).requires_grad_(True)
-    a1 = layer.a1.detach().double().cpu().view(heads, dim).requires_grad_(True)
-    Xd = X.double().cpu()
-    if dtype == torch.bfloat16:   # same rounding of the inputs the GPU path sees
-        Xd = Xd.to(torch.bfloat16).double()
-    ref = gat_reference(Xd, W, None, a0, a1, A, heads, dim, 0.2)
-    (ref * go.double().cpu()).sum().backward()
+    # the float64 reference sees the roundings of the GPU path's inputs and
+    # stored intermediates: X, W, a0, a1 and the upstream gradient in bf16,
+    # and X W stored in bf16 (for fp32 all of these are exact)
+    rd = dtype if dtype != torch.float32 else None
+    r = (lambda t: t.to(rd).double()) if rd else (lambda t: t.double())
+    W = r(layer.fc.weight.detach()).cpu().requires_grad_(True)
+    a0 = r(layer.a0.detach()).cpu().view(heads, dim).requires_grad_(True)
+    a1 = r(layer.a1.detach()).cpu().view(heads, dim).requires_grad_(True)
+    ref = gat_reference(r(X).cpu(), W, None, a0, a1, A, heads, dim, 0.2, round_to=rd)
+    (ref * r(go).cpu()).sum().backward()
     rel = lambda a, b: ((a.double().cpu() - b).norm() / b.norm()).item()
     assert rel(out, ref) < tol
     assert rel(layer.fc.weight.grad, W.grad) < tol * 3
@@ -104,30 +112,179 @@ def test_gat_layer_vs_fp64(cuda, heads, dim, dtype, tol):
     assert rel(layer.a1.grad.view(heads, dim), a1.grad) < tol * 3
 
 
-@pytest.mark.parametrize("heads,dim", [(1, 32), (4, 16)])
-def test_gatv2_layer_vs_fp64(cuda, heads, dim):
+@pytest.mark.parametrize("heads,dim,dtype,tol", [(1, 32, torch.float32, 2e-4), (4, 16, torch.float32, 2e-4),
+                                                 (1, 64, torch.bfloat16, 3e-2), (4, 64, torch.bfloat16, 3e-2)])
+def test_gatv2_layer_vs_fp64(cuda, heads, dim, dtype, tol):
+    """The single-head reference file's logits (no min-max map)."""
     g = small_graph(300, 5, seed=7, symmetric=False)
     graph = GATGraph(g, cuda)
     torch.manual_seed(3)
-    layer = GATv2Conv_multi(24, dim, 0.0, 0.2, heads, dtype=torch.float32).to(cuda)
+    layer = GATv2Conv_multi(24, dim, 0.0, 0.2, heads, dtype=dtype, minmax=False).to(cuda)
     X = torch.randn(g.n, 24, device=cuda)
     out = layer(X, graph)
     go = torch.randn(out.shape, device=cuda)
-    (out * go).sum().backward()
+    (out.float() * go).sum().backward()
     A = dense_adj(g)
-    W0 = layer.fc0.weight.detach().double().cpu().requires_grad_(True)
-    W1 = layer.fc1.weight.detach().double().cpu().requires_grad_(True)
+    W0 = layer.fc0.weight.detach().to(dtype).double().cpu().requires_grad_(True)
+    W1 = layer.fc1.weight.detach().to(dtype).double().cpu().requires_grad_(True)
     a = layer.a0.detach().double().cpu().view(heads, dim).requires_grad_(True)
-    ref = gat_reference(X.double().cpu(), W0, W1, a, a, A, heads, dim, 1.0, v2_alpha=0.2)
+    ref = gat_reference(X.to(dtype).double().cpu(), W0, W1, a, a, A, heads, dim, 1.0, v2_alpha=0.2)
     (ref * go.double().cpu()).sum().backward()
     rel = lambda x, y: ((x.double().cpu() - y).norm() / y.norm()).item()
-    assert rel(out, ref) < 2e-4
+    assert rel(out, ref) < tol
     # a.X0[i] is constant along row i, so the softmax cancels it: dL/dW0 = 0
     # exactly (in the reference's formulation as well)
     assert W0.grad.norm() < 1e-10 * W1.grad.norm()
     assert layer.fc0.weight.grad.norm() < 1e-5 * layer.fc1.weight.grad.norm()
-    assert rel(layer.fc1.weight.grad, W1.grad) < 1e-3
-    assert rel(layer.a0.grad.view(heads, dim), a.grad) < 1e-3
+    gtol = 1e-3 if dtype == torch.float32 else 3 * tol
+    assert rel(layer.fc1.weight.grad, W1.grad) < gtol
+    assert rel(layer.a0.grad.view(heads, dim), a.grad) < gtol
+
+
+def gatv2_minmax_reference(X, W0, W1, a, rows, cols, n, heads, dim, alpha, at=None):
+    """float64 restatement of the multi-head GATv2 forward
+    (end2end_head/gatv2_final/mgat_csr/mgat_conv.py:171-197, no dropout):
+    X0 = LReLU(X W0), X1 = LReLU(X W1), att = a.X0[i] + a.X1[j] per edge,
+    att = (att - min_h) / (max_h - min_h) over each head's edges, p = exp(att),
+    h = (sum_j p X1[j]) / (sum_j p); [N, H, D].  ``at``: (edge of the minimum,
+    edge of the maximum) per head to take them at instead of searching (a
+    subgradient choice at near-ties); returns (h, z)."""
+    X0 = F.leaky_relu(X @ W0.t(), alpha).view(n, heads, dim)
+    X1 = F.leaky_relu(X @ W1.t(), alpha).view(n, heads, dim)
+    s0 = (X0 * a.view(1, heads, dim)).sum(-1)
+    s1 = (X1 * a.view(1, heads, dim)).sum(-1)
+    z = s0[rows] + s1[cols]                                    # [E, H]
+    hs = torch.arange(heads)
+    if at is None:
+        zmin, zmax = z.min(0).values, z.max(0).values
+    else:
+        zmin, zmax = z[at[0], hs], z[at[1], hs]
+    p = torch.exp((z - zmin) / (zmax - zmin))
+    S = torch.zeros(n, heads, dtype=X.dtype).index_add(0, rows, p)
+    h = torch.zeros(n, heads, dim, dtype=X.dtype).index_add(0, rows, p.unsqueeze(-1) * X1[cols])
+    return h / S.unsqueeze(-1), z
+
+
+@pytest.mark.parametrize("heads,dim", [(1, 64), (2, 64), (4, 64), (2, 32)])
+@pytest.mark.parametrize("dtype,tol", [(torch.float32, 2e-4), (torch.bfloat16, 3e-2)])
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_gatv2_minmax_layer_vs_fp64(cuda, heads, dim, dtype, tol, symmetric):
+    """GATv2Conv_multi built with a head count maps each head's logits to
+    [0, 1] by their global min and max, as the reference's multi-head layer
+    does; forward and the gradients of W0, W1 and a (W0 reaches the loss only
+    through the row terms of the two edges that set min and max) against the
+    float64 restatement, with the extreme edges the kernel found (which the
+    float64 logits must confirm as extremes up to the input rounding)."""
+    g = small_graph(400, 6, seed=heads + dim, symmetric=symmetric)
+    graph = GATGraph(g, cuda, build="device")
+    torch.manual_seed(heads)
+    layer = GATv2Conv_multi(24, dim, 0.0, 0.2, heads, dtype=dtype).to(cuda)
+    assert layer.minmax
+    X = torch.randn(g.n, 24, device=cuda)
+    out = layer(X, graph)
+    go = torch.randn(out.shape, device=cuda)
+    (out.float() * go).sum().backward()
+
+    # the extreme edges the layer used (same deterministic kernels as inside it)
+    with torch.no_grad():
+        Xd = X.to(dtype)
+        X0 = F.leaky_relu(F.linear(Xd, layer.fc0.weight.to(dtype)), 0.2).view(-1, heads, dim)
+        X1 = F.leaky_relu(F.linear(Xd, layer.fc1.weight.to(dtype)), 0.2).view(-1, heads, dim)
+        av = layer.a0.view(heads, dim).to(dtype).contiguous()
+        s0 = ops.node_scores(X0, av, "mhd", out_layout="nh")
+        s1 = ops.node_scores(X1, av, "mhd", out_layout="nh")
+        rng, where = ops.edge_score_range(graph.rowptr, graph.col, s0, s1)
+    where = where.cpu().numpy()
+    rows_np, cols_np = g.row_ids().astype(np.int64), g.indices.astype(np.int64)
+
+    def edge_id(r, c):
+        lo, hi = g.indptr[r], g.indptr[r + 1]
+        return lo + int(np.searchsorted(g.indices[lo:hi], c))
+
+    e_min = torch.tensor([edge_id(*where[h, 0:2]) for h in range(heads)])
+    e_max = torch.tensor([edge_id(*where[h, 2:4]) for h in range(heads)])
+    rows, cols = torch.from_numpy(rows_np), torch.from_numpy(cols_np)
+    W0 = layer.fc0.weight.detach().to(dtype).double().cpu().requires_grad_(True)
+    W1 = layer.fc1.weight.detach().to(dtype).double().cpu().requires_grad_(True)
+    a = layer.a0.detach().to(dtype).double().cpu().view(heads, dim).requires_grad_(True)
+    ref, z = gatv2_minmax_reference(X.to(dtype).double().cpu(), W0, W1, a, rows, cols, g.n, heads, dim, 0.2,
+                                    at=(e_min, e_max))
+    (ref * go.double().cpu()).sum().backward()
+    zr = z.detach()
+    span = (zr.max(0).values - zr.min(0).values)
+    hs = torch.arange(heads)
+    assert torch.all(zr[e_min, hs] - zr.min(0).values <= tol * span)
+    assert torch.all(zr.max(0).values - zr[e_max, hs] <= tol * span)
+    assert torch.allclose(rng[:, 2].double().cpu(), 1.0 / (zr[e_max, hs] - zr[e_min, hs]), rtol=3 * tol)
+    rel = lambda x, y: ((x.double().cpu() - y).norm() / y.norm()).item()
+    assert rel(out.float(), ref) < tol
+    gtol = 1e-3 if dtype == torch.float32 else 3 * tol
+    assert rel(layer.fc0.weight.grad, W0.grad) < gtol
+    assert rel(layer.fc1.weight.grad, W1.grad) < gtol
+    assert rel(layer.a0.grad.view(heads, dim), a.grad) < gtol
+    assert W0.grad.norm() > 1e-3 * W1.grad.norm()        # the min / max edges do reach W0
+
+
+def test_edge_score_range_vs_torch(cuda):
+    """attgnn_edge_score_range: min, max, temperature and the first extreme
+    edge of each head against torch on the per-edge logits; both layouts;
+    deterministic; a graph with no edges gives -1 and a zero temperature."""
+    g = small_graph(700, 7, seed=3, symmetric=False)
+    rowptr = torch.from_numpy(g.rowptr32()).to(cuda)
+    col = torch.from_numpy(g.indices).to(cuda)
+    gen = torch.Generator().manual_seed(5)
+    heads = 3
+    s0 = torch.randn(g.n, heads, generator=gen).to(cuda)
+    s1 = torch.randn(g.n, heads, generator=gen).to(cuda)
+    s1[17, 1] = 1e3        # one column term dominates head 1's maximum (several edges share it)
+    rows = torch.from_numpy(g.row_ids().astype(np.int64)).to(cuda)
+    z = s0[rows] + s1[col.long()]                              # [E, H]
+    for layout in ("nh", "hn"):
+        a0, a1 = (s0, s1) if layout == "nh" else (s0.t().contiguous(), s1.t().contiguous())
+        rng, where = ops.edge_score_range(rowptr, col, a0, a1, s_layout=layout)
+        rng2, where2 = ops.edge_score_range(rowptr, col, a0, a1, s_layout=layout)
+        assert torch.equal(rng, rng2) and torch.equal(where, where2)
+        assert torch.equal(rng[:, 0], z.min(0).values) and torch.equal(rng[:, 1], z.max(0).values)
+        assert torch.allclose(rng[:, 2], 1.0 / (rng[:, 1] - rng[:, 0]))
+        for h in range(heads):
+            first_min = int((z[:, h] == z[:, h].min()).nonzero()[0])
+            first_max = int((z[:, h] == z[:, h].max()).nonzero()[0])
+            assert where[h, 0].item() == rows[first_min].item() and where[h, 1].item() == g.indices[first_min]
+            assert where[h, 2].item() == rows[first_max].item() and where[h, 3].item() == g.indices[first_max]
+    empty = torch.zeros(5, dtype=torch.int32, device=cuda)
+    rng, where = ops.edge_score_range(empty, col[:0], s0[:4], s1[:4])
+    assert torch.all(where.cpu() == -1) and torch.all(rng[:, 2].cpu() == 0)
+
+
+def test_gcn_fp16_two_layers_vs_fp64(cuda):
+    """The 2-layer GCN of the benchmark in its storage type -- FP16 tiles,
+    features and weights, FP32 accumulation; 128 hidden features (compact
+    tiles) and 48 outputs padded to 64 (dense tiles), ReLU -- on a
+    20,000-node Chung-Lu graph: the output and both weight gradients against
+    float64 on the same FP16-rounded inputs."""
+    g = G.chung_lu(20000, 10, 2.1, seed=4)
+    graph = GCNGraph(g, cuda, tile_dtype=torch.float16, build="device")
+    torch.manual_seed(1)
+    c1, c2 = GCNConv(96, 128).to(cuda), GCNConv(128, 48).to(cuda)
+    X = torch.randn(g.n, 96, device=cuda).half()
+    y = c2(F.relu(c1(X, graph)), graph)
+    assert y.dtype == torch.float16 and y.shape == (g.n, 48)
+    go = torch.randn(y.shape, device=cuda)
+    (y.float() * go).sum().backward()
+
+    d = G.gcn_norm_degree(g)
+    vals = torch.from_numpy(G.gcn_norm_values(g, d).astype(np.float32)).half().double()
+    idx = torch.from_numpy(np.stack([g.row_ids(), g.indices]).astype(np.int64))
+    A = torch.sparse_coo_tensor(idx, vals, (g.n, g.n)).coalesce()
+    W1 = c1.weights.detach().half().double().cpu().requires_grad_(True)
+    W2 = c2.weights.detach().half().double().cpu().requires_grad_(True)
+    h = F.relu(torch.sparse.mm(A, X.double().cpu() @ W1))
+    ref = torch.sparse.mm(A, h @ W2)
+    (ref * go.double().cpu()).sum().backward()
+    rel = lambda x, r: ((x.double().cpu() - r).norm() / r.norm()).item()
+    assert rel(y, ref) < 5e-3
+    assert rel(c1.weights.grad, W1.grad) < 1e-2
+    assert rel(c2.weights.grad, W2.grad) < 1e-2
 
 
 def test_gat_dropout_mask_regenerated(cuda):

@@ -27,6 +27,26 @@ def test_reference_balance_known_answer(cuda):
         ref = O.spmm(TOY_ROW, TOY_COL, np.ones(38), rhs.double().numpy(), n_rows=30)
         assert y.shape == (30, dimN)
         assert np.array_equal(y.cpu().double().numpy(), ref)
+        # forward_fp16_gnn_acc (benchmark.cpp:494): the same product under the second name
+        y2 = m["SpMM_sr_bcrs"].forward_fp16_gnn_acc(rp.to(cuda), col.to(cuda), val.to(cuda), win.to(cuda),
+                                                    at.to(cuda), rhs.to(cuda), 32, dimN, 30)[0]
+        assert torch.equal(y, y2)
+
+
+def test_reference_balance_known_answer_tf32(cuda):
+    """gcn/SpMM/test/mgcntf32/test_balance.py:20-62 (4-wide tf32 blocks, no
+    width keyword), through forward_tf32_gnn and forward_tf32_gnn_acc."""
+    m = compat.MODULES
+    dd = torch.ones(32, dtype=torch.float32)
+    rp, col, val, win, at = m["Block_sr_bcrs"].blockProcess_tf32_balance(
+        torch.from_numpy(TOY_ROW), torch.from_numpy(TOY_COL), dd, 8, 4, 2)
+    for dimN in (32, 65):
+        rhs = torch.randint(1, 3, (30, dimN), generator=torch.Generator().manual_seed(dimN)).float()
+        args = (rp.to(cuda), col.to(cuda), val.to(cuda), win.to(cuda), at.to(cuda), rhs.to(cuda), 32, dimN, 30)
+        y = m["SpMM_sr_bcrs"].forward_tf32_gnn(*args)[0]
+        ref = O.spmm(TOY_ROW, TOY_COL, np.ones(38), rhs.double().numpy(), n_rows=30)
+        assert np.array_equal(y.cpu().double().numpy(), ref)
+        assert torch.equal(y, m["SpMM_sr_bcrs"].forward_tf32_gnn_acc(*args)[0])
 
 
 def _tile_values_f64(rp_csr, col_csr, d, dtype):

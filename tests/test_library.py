@@ -44,3 +44,17 @@ def test_spmm_rejects_bad_arguments_without_device():
     rc = L.attgnn_sddmm_block(nul, nul, nul, nul, 1, nul, 0, 0, nul, 0, 0, _lib.F16, 1, 1, 8, nul, _lib.F16, 1,
                               8, 7, 0, nul)
     assert rc == -1
+
+
+def test_fused_dot_rejects_bad_arguments_without_device():
+    """attgnn_spmm_edge_map_compact_dot validates on the host: a negative
+    dot_rows is an argument error, an unsupported width (48) is unsupported."""
+    L = _lib.lib()
+    nul = ctypes.c_void_p(0)
+
+    def call(dim, dot_rows):
+        return L.attgnn_spmm_edge_map_compact_dot(nul, nul, nul, nul, nul, nul, _lib.BF16, 0, nul, nul, 0, nul,
+                                                  _lib.BF16, _lib.LAYOUT_HMD, 1, 0, ctypes.c_void_p(256), _lib.F32,
+                                                  0, dim, _lib.LAYOUT_HMD, 16, nul, dot_rows, nul, nul, 0, 0, nul)
+    assert call(64, -1) == -1
+    assert call(48, 0) == -4
