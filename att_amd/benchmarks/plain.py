@@ -6,8 +6,13 @@ import torch.nn.functional as F
 
 
 def csr_tensor(graph_csr, values, device):
+    """FP32 CSR tensor with int32 row pointers and column indices (the index
+    type of the window-block path; the nnz must stay below 2^31)."""
     import numpy as np
-    return torch.sparse_csr_tensor(torch.from_numpy(graph_csr.indptr), torch.from_numpy(graph_csr.indices.astype(np.int64)),
+    if graph_csr.nnz >= 2 ** 31:
+        raise ValueError("int32 CSR indices need nnz < 2^31")
+    return torch.sparse_csr_tensor(torch.from_numpy(graph_csr.indptr.astype(np.int32)),
+                                   torch.from_numpy(graph_csr.indices.astype(np.int32)),
                                    torch.from_numpy(np.asarray(values, np.float32)), (graph_csr.n, graph_csr.n)).to(device)
 
 

@@ -24,7 +24,7 @@ _KMODE = {"f32": "f32", "bf16x3": "bf16x3", "bf16": "native", "fp16": "native", 
 
 
 class BlockMatrix:
-    def __init__(self, csr, values, device, mode="bf16", window=16, wide=32, part_size=None):
+    def __init__(self, csr, values, device, mode="bf16", window=16, wide=32, part_size=None, build="host"):
         if mode not in _TILE:
             raise ValueError(f"unknown mode {mode!r}")
         from .layers import _blocks_estimate, default_part_size
@@ -39,8 +39,14 @@ class BlockMatrix:
             values = values / r[csr.row_ids()]
             self.row_scale = torch.from_numpy(r).to(device)
         padded = csr.padded(window)
-        b = ops.blocks_build(padded.rowptr32(), padded.indices, "edge_f32", values, _TILE[mode], window, wide,
-                             self.part_size, emit_empty=True)
+        if build == "device":      # bit-identical to the host build
+            b = ops.blocks_build_device(torch.from_numpy(padded.rowptr32()).to(device),
+                                        torch.from_numpy(padded.indices).to(device), "edge_f32",
+                                        torch.from_numpy(values).to(device), _TILE[mode], window, wide,
+                                        self.part_size, emit_empty=True)
+        else:
+            b = ops.blocks_build(padded.rowptr32(), padded.indices, "edge_f32", values, _TILE[mode], window, wide,
+                                 self.part_size, emit_empty=True)
         self.n_unique = b["n_unique"]
         arrs = [b[k].to(device) for k in ("rp", "col", "val", "window", "atomic")]
         if mode == "fp8":

@@ -400,16 +400,34 @@ class Bucket:
         cmap[n_ch:] = -1
         self.used = n_ch
         self.copy_bytes = self.map_at + 4 * used
+        self.last_edges = [int(x) for x in start[:, 3]]
         return [int(x) for x in start[:, 0]]
 
     # -- packed forward --------------------------------------------------------
-    def _forward(self):
+    def _pack(self):
         L = _lib.lib()
         _lib.check(L.attgnn_pack_segments(ops._ptr(self.table_dev), self.max_segs,
                                           ctypes.c_void_p(self.table_dev.data_ptr() + self.map_at), self.max_chunks,
                                           CHUNK_BYTES,
                                           ops._stream(self.device)),
                    "attgnn_pack_segments")
+
+    def pack(self, graphs):
+        """Pack `graphs` into the bucket's buffers on the current stream without
+        running a model; returns each graph's first row and first edge in the
+        batch (for inspection and tests)."""
+        if self.done is not None:
+            self.done.synchronize()
+        rows = self.build_table(graphs)
+        nb = self.copy_bytes
+        self.table_dev[:nb].copy_(self.table_host[:nb])
+        for g in graphs:
+            torch.cuda.current_stream(self.device).wait_event(g.ready)
+        self._pack()
+        return rows, list(self.last_edges)
+
+    def _forward(self):
+        self._pack()
         m = self.models
         if self.kind == "gcn":
             h = self.X @ m.W1

@@ -12,19 +12,32 @@ A step is one full training step.  Its hot path is four window-block SpMMs
 (feature widths 256 and 48, forward and backward); ``value`` is their
 algorithmic FLOPs (2 * nnz * width each) divided by the wall time of the whole
 step.  ``roofline`` covers the dominant kernel, the width-256 SpMM, timed with
-HIP events on the stream it runs on; its algorithmic bytes per launch are
-4(M+1) + nnz*(4+2) + U*256*2 + M*256*2 with U the number of distinct columns
-summed over 16-row windows.  ``cpu_baseline`` times torch.sparse.mm (CSR,
-FP32) on the host cores over a bounded row sample of the same A_hat.
-``attention`` reports the SDDMM half of the metric and the GAT layer against
-plain FP32 on the attention workload (169,343 nodes, 2.48M entries, 4 heads
-x 64, BF16; see attention_leg), ``serving`` the req/s and p99 of batched
-inference against per-graph plain FP32 (see serving_leg); both are measured
-after the timed steps and are not part of ``value``.
+HIP events on the stream it runs on.  Its ``achieved`` counts the compulsory
+HBM bytes of one launch -- X read once (N*256*2), the structure as the kernel
+reads it (part table, condensed columns, row masks, block offsets, the nnz
+FP16 values) and Y written once (M*256*2) -- so ``frac`` is a true fraction of
+the HBM roof.  The per-window gather figure 4(M+1) + nnz*(4+2) + U*256*2 +
+M*256*2 (U = distinct columns summed over 16-row windows, the rows the kernel
+gathers) is reported beside it as ``gather_bytes``: most of those gathers are
+served by the L2s and the Infinity Cache, so it can exceed the HBM roof.
+``traffic`` is the measured L2-to-fabric bytes per launch from the committed
+rocprofv3 FETCH_SIZE / WRITE_SIZE passes (copy-calibrated), used only when
+they were taken on the same kernel sources (a digest of them is stored with
+the counters), else null.  ``cpu_baseline`` times torch.sparse.mm (CSR, FP32)
+on the host cores over a strided row sample of the same A_hat (3 warm-ups,
+median of 10).  After the timed steps, and not part of ``value``:
+``attention`` (SDDMM and the GAT layer against plain FP32 on the attention
+workload: 169,343 nodes, 2.48M entries, 4 heads x 64, BF16), ``precision``
+(the SpMM in five arithmetic modes on a 2^20-node, 20M-entry matrix against
+FP64), ``small_graph`` (plain FP32 SpMM on host and GPU against the
+window-block modes on the 16,384-node graph, 16-row preprocessing checked
+bit for bit) and ``serving`` (batched inference against per-graph plain FP32,
+warm and cold cache).
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-attention] [--no-serving]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-attention] [--no-serving] [--no-precision]
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -55,32 +68,71 @@ def build_graph(scale):
     return g
 
 
-def cpu_baseline(g, vals, dim=256, budget_s=12.0):
-    """torch.sparse.mm (CSR FP32) on the host over a leading row sample."""
+def cpu_baseline(g, vals, dim=256, stride=1, warmups=3, runs=10):
+    """torch.sparse.mm (CSR FP32, int32 indices) on the host cores over every
+    ``stride``-th row of A_hat (a strided sample has the matrix's degree mix;
+    a leading block would be the hub rows): 3 warm-ups, median of 10."""
     import torch
     cores = min(len(os.sched_getaffinity(0)), 16)
     torch.set_num_threads(cores)
-    # sample rows so that one product is ~1/4 of the budget at ~5 GFLOP/s
-    target_nnz = int(budget_s / 4 * 5e9 / (2 * dim))
-    rows = int(np.searchsorted(g.indptr, min(target_nnz, g.nnz)))
-    rows = max(min(rows, g.n), 1)
-    e = int(g.indptr[rows])
-    A = torch.sparse_csr_tensor(torch.from_numpy(g.indptr[: rows + 1].copy()),
-                                torch.from_numpy(g.indices[:e].astype(np.int64)),
-                                torch.from_numpy(vals[:e].copy()), (rows, g.n))
+    rows = np.arange(0, g.n, stride)
+    counts = np.diff(g.indptr)[rows]
+    ip = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    take = np.concatenate([np.arange(g.indptr[r], g.indptr[r + 1]) for r in rows])
+    e = int(ip[-1])
+    A = torch.sparse_csr_tensor(torch.from_numpy(ip), torch.from_numpy(g.indices[take].astype(np.int32)),
+                                torch.from_numpy(vals[take].copy()), (rows.size, g.n))
     X = torch.from_numpy(np.random.default_rng(5).uniform(-1, 1, (g.n, dim)).astype(np.float32))
-    torch.sparse.mm(A, X)  # warm-up
+    for _ in range(warmups):
+        torch.sparse.mm(A, X)
     times = []
-    t_end = time.perf_counter() + budget_s
-    while time.perf_counter() < t_end and len(times) < 5:
+    for _ in range(runs):
         t = time.perf_counter()
         torch.sparse.mm(A, X)
         times.append(time.perf_counter() - t)
     tmed = float(np.median(times))
     gflops = 2.0 * e * dim / tmed / 1e9
     return {"value": round(gflops, 3), "unit": "GFLOP/s", "cores": cores, "kind": "port",
-            "sample": f"torch.sparse.mm CSR fp32 on the first {rows} rows ({e} nnz) of A_hat x X[{g.n},{dim}], "
-                      f"median of {len(times)}"}
+            "sample": f"torch.sparse.mm CSR fp32 (int32 indices) on one row in {stride} of A_hat "
+                      f"({rows.size} rows, {e} nnz) x X[{g.n},{dim}], {warmups} warm-ups, median of {runs} "
+                      f"({sum(times):.1f} s timed)"}
+
+
+KERNEL_SOURCES = ("att_amd/csrc/spmm.hip", "att_amd/csrc/mfma.hpp", "att_amd/csrc/halfconv.hpp",
+                  "att_amd/csrc/attn.hpp", "include/attgnn.h")
+
+
+def kernel_sources_digest():
+    """sha256 (16 hex digits) of the SpMM kernel's sources: the counter file
+    carries the digest of the sources it was measured on."""
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        with open(os.path.join(HERE, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def measured_traffic():
+    """(bytes per launch, note) from profiles/spmm256_traffic.json when its
+    digest matches the current kernel sources, else (None, reason)."""
+    tfile = os.path.join(HERE, "profiles", "spmm256_traffic.json")
+    if not os.path.exists(tfile):
+        return None, "no counter file"
+    t = json.load(open(tfile))
+    if t.get("sources_sha256_16") != kernel_sources_digest():
+        return None, "counters were taken on other kernel sources"
+    b = int((t["fetch_kib"] * t["fetch_factor"] + t["write_kib"] * t["write_factor"]) * 1024)
+    return b, f"rocprofv3 FETCH_SIZE x {t['fetch_factor']} + WRITE_SIZE x {t['write_factor']} ({t['source']})"
+
+
+def precision_leg(dev):
+    from att_amd.benchmarks import precision
+    return precision.run(dev, log=log)
+
+
+def small_graph_leg(dev):
+    from att_amd.benchmarks import spmm_small
+    return spmm_small.run(dev)
 
 
 def attention_leg(dev, heads=4, dim=64, reps=20):
@@ -166,15 +218,19 @@ def attention_leg(dev, heads=4, dim=64, reps=20):
                             "speedup": round(t_plain / t_ours, 2), "rel_err_vs_fp32": gat_err}}
 
 
-def serving_leg(dev, pool_size=1024, requests=4096, plain_requests=1024):
+def serving_leg(dev, pool_size=1024, requests=4096, plain_requests=1024, cold_fraction=0.25):
     """The req/s + p99 half of the metric: Poisson-arrival inference requests
     (50/50 2-layer GCN / 1-layer 4-head GAT, 128 BF16 features) on a pool of
     Chung-Lu graphs with N ~ lognormal(median 2000, sigma 1.2) clipped to
-    [100, 200000] and mean degree U[4, 32], the structure cache warmed first.
-    The per-graph plain FP32 server's saturated throughput is its capacity and
-    its p99 at 70% of capacity the latency bound; the batched server's
-    saturated throughput and its p99 at 3x and 32x the plain capacity are
-    reported against that bound."""
+    [100, 200000] and mean degree U[4, 32].
+    Warm: the structure cache is filled first.  The per-graph plain FP32
+    server's saturated throughput is its capacity and its p99 at 70% of
+    capacity the latency bound; the batched server's saturated throughput and
+    its p99 at 3x and 32x the plain capacity are reported against that bound.
+    Cold: a server whose HBM budget holds only ``cold_fraction`` of the pool's
+    structures starts with an empty cache, so admission, device preprocessing
+    and LRU eviction all run inside the timed requests; saturated and at 3x
+    the plain capacity, with its cache misses and evictions."""
     import torch
     from att_amd import serve as S
 
@@ -203,10 +259,28 @@ def serving_leg(dev, pool_size=1024, requests=4096, plain_requests=1024):
         st = run(server, requests, mult * cap_plain, 10 + mult)
         at[f"{mult}x_plain_capacity"] = {"req_per_s": round(st["req_per_s"], 1), "p99_ms": round(st["p99_ms"], 3),
                                          "within_p99_bound": st["p99_ms"] <= bound}
+    # cold cache under a budget that holds a quarter of the pool
+    total = sum(S.estimate_bytes(pool(k)[0]) for k in range(pool_size))
+    cold = S.Server(dev, models, pool, use_graph=True, part_size=S.SERVE_PART_SIZE, preprocess="device")
+    budget = cold.reserved + int(cold_fraction * total)
+    cold.cache.budget = budget
+    cold.run([S.Request(i, 0.0, "gcn" if i % 2 else "gat", i) for i in range(64)])
+    cold.prime()                               # bucket graphs captured; then the cache starts empty
+    cold_res = {}
+    for name, rate, seed in (("saturated", None, 21), ("3x_plain_capacity", 3 * cap_plain, 22)):
+        cold.cache = S.GraphCache(budget)
+        st = run(cold, requests, rate, seed)
+        c = cold.cache
+        cold_res[name] = {"req_per_s": round(st["req_per_s"], 1), "p50_ms": round(st["p50_ms"], 3),
+                          "p99_ms": round(st["p99_ms"], 3), "within_p99_bound": st["p99_ms"] <= bound,
+                          "cache_hits": c.hits, "cache_misses": c.misses, "evictions": c.evictions}
     return {"workload": f"{requests} requests, 50/50 GCN 2-layer / GAT 4-head, pool of {pool_size} Chung-Lu graphs "
-                        "(N ~ lognormal(2000, 1.2) in [100, 200000]), 128 bf16 features, warm cache",
+                        "(N ~ lognormal(2000, 1.2) in [100, 200000]), 128 bf16 features",
             "plain_fp32_req_per_s": round(cap_plain, 1), "batched_req_per_s": round(sat["req_per_s"], 1),
-            "capacity_ratio": round(sat["req_per_s"] / cap_plain, 2), "p99_bound_ms": round(bound, 3), **at}
+            "capacity_ratio": round(sat["req_per_s"] / cap_plain, 2), "p99_bound_ms": round(bound, 3),
+            "warm_cache": at,
+            "cold_cache": {"budget_bytes": budget, "pool_structure_bytes": int(total),
+                           "budget_holds_fraction": cold_fraction, **cold_res}}
 
 
 def main():
@@ -218,6 +292,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-attention", action="store_true", help="skip the SDDMM / GAT leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-precision", action="store_true", help="skip the precision and 16,384-node legs")
     ap.add_argument("--part-size", type=int, default=0)
     ap.add_argument("--build", choices=["host", "device"], default="device",
                     help="where the window blocks are built (results are bit-identical)")
@@ -324,8 +399,20 @@ def main():
     torch.cuda.synchronize()
     t_spmm = ev0.elapsed_time(ev1) / reps  # ms
     M = g.n
-    algo_bytes = 4 * (M + 1) + g.nnz * (4 + 2) + graph.n_unique * F_HID * 2 + M * F_HID * 2
-    achieved = algo_bytes / (t_spmm * 1e-3) / 1e9
+    P, C = graph.fwd["parts"], graph.fwd["n_cols"]
+    # compulsory bytes of one launch: X once, the compact structure as read
+    # (rp, window, atomic; columns 4 B + row mask 2 B each; an 8-byte offset
+    # per 32-column block; the FP16 values), Y once
+    struct_bytes = 4 * (2 * P + 1) + 8 * P + C * (4 + 2) + (C // 32) * 8 + g.nnz * 2
+    compulsory = g.n * F_HID * 2 + struct_bytes + M * F_HID * 2
+    gather_bytes = 4 * (M + 1) + g.nnz * (4 + 2) + graph.n_unique * F_HID * 2 + M * F_HID * 2
+    achieved = compulsory / (t_spmm * 1e-3) / 1e9
+    # the compulsory bytes of the step's four SpMMs (two at width 256 over
+    # the compact tiles, two at width 64 -- the 48 outputs padded -- over the
+    # dense FP16 tiles, C*16*2 bytes) per step time: a lower bound of the
+    # step's HBM traffic, below the roof
+    compulsory64 = g.n * 64 * 2 + 4 * (2 * P + 1) + 8 * P + C * (4 + 16 * 2) + M * 64 * 2
+    step_gbs = 2 * (compulsory + compulsory64) / (ms * 1e-3) / 1e9
     # accuracy of that SpMM vs FP64 on a row sample (same fp16-rounded A_hat and X)
     rows = np.arange(0, g.n, max(g.n // 2000, 1))
     sub_ip = np.concatenate([[0], np.cumsum(np.diff(g.indptr)[rows])])
@@ -336,8 +423,8 @@ def main():
     got = y[torch.from_numpy(rows).to(dev)].double().cpu().numpy()
     rel_err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
 
-    # plain FP32 GPU SpMM (torch.sparse on the same A_hat) for comparison
-    A32 = torch.sparse_csr_tensor(torch.from_numpy(g.indptr), torch.from_numpy(g.indices.astype(np.int64)),
+    # plain FP32 GPU SpMM (torch.sparse on the same A_hat, int32 indices) for comparison
+    A32 = torch.sparse_csr_tensor(torch.from_numpy(g.indptr.astype(np.int32)), torch.from_numpy(g.indices),
                                   torch.from_numpy(vals), (g.n, g.n)).to(dev)
     X32 = Xs.float()
     torch.sparse.mm(A32, X32)
@@ -353,19 +440,21 @@ def main():
     if not args.no_attention and args.scale == 1.0:
         attention = attention_leg(dev)
         log(f"attention leg: {attention}")
+    precision = small = None
+    if not args.no_precision and args.scale == 1.0:
+        t = time.perf_counter()
+        precision = precision_leg(dev)
+        log(f"precision leg ({time.perf_counter() - t:.1f}s): {precision}")
+        t = time.perf_counter()
+        small = small_graph_leg(dev)
+        log(f"16,384-node leg ({time.perf_counter() - t:.1f}s): {small}")
     serving = None
     if not args.no_serving and args.scale == 1.0:
         t = time.perf_counter()
         serving = serving_leg(dev)
         log(f"serving leg ({time.perf_counter() - t:.1f}s): {serving}")
 
-    traffic = None
-    tfile = os.path.join(HERE, "profiles", "spmm256_traffic.json")
-    if args.scale == 1.0 and os.path.exists(tfile):
-        # HBM-side bytes per launch from the committed rocprofv3 FETCH_SIZE /
-        # WRITE_SIZE passes of this kernel, scaled by the copy calibration
-        t = json.load(open(tfile))
-        traffic = int((t["fetch_kib"] * t["fetch_factor"] + t["write_kib"] * t["write_factor"]) * 1024)
+    traffic, traffic_note = measured_traffic() if args.scale == 1.0 else (None, "reduced graph")
     out = {
         "metric": "SpMM/SDDMM GFLOP/s & %HBM roofline; GNN req/s + p99 vs plain FP32 at stated err",
         "value": round(value, 2),
@@ -385,7 +474,12 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "window-block SpMM, width 256, fp16, compact tiles (main + split-window combine)",
-                     "ms_per_launch": round(t_spmm, 4), "algorithmic_bytes": int(algo_bytes),
+                     "ms_per_launch": round(t_spmm, 4), "compulsory_bytes": int(compulsory),
+                     "gather_bytes": int(gather_bytes),
+                     "gather_gbs": round(gather_bytes / (t_spmm * 1e-3) / 1e9, 1),
+                     "traffic_gbs": None if traffic is None else round(traffic / (t_spmm * 1e-3) / 1e9, 1),
+                     "traffic_note": traffic_note, "kernel_sources_sha256_16": kernel_sources_digest(),
+                     "step_compulsory_gbs": round(step_gbs, 1),
                      "gflops": round(2.0 * g.nnz * F_HID / (t_spmm * 1e-3) / 1e9, 1)},
         "cpu_baseline": cpu,
         "accuracy": {"spmm256_rel_fro_vs_fp64": rel_err},
@@ -393,8 +487,12 @@ def main():
                            "gflops": round(2.0 * g.nnz * F_HID / (t_plain * 1e-3) / 1e9, 1),
                            "speedup_window_block": round(t_plain / t_spmm, 3)},
         "attention": attention,
+        "precision": precision,
+        "small_graph": small,
         "serving": serving,
     }
+    if achieved > HBM_PEAK_GBS or step_gbs > HBM_PEAK_GBS:
+        raise RuntimeError("compulsory bytes per second above the HBM peak: the byte count is wrong")
     print(json.dumps(out), flush=True)
 
 

@@ -44,6 +44,58 @@ def test_alone_equals_mix(cuda, setup):
     print(f"{exact}/12 results bitwise equal alone and in the mix")
 
 
+@pytest.mark.parametrize("kind", ["gcn", "gat"])
+def test_packed_sparse_aggregation_bitwise_equals_alone(cuda, setup, kind):
+    """The sparse part of a packed batch is each graph's own result bit for
+    bit: rows are padded to the 16-row window so no window straddles two
+    graphs, every kernel works per window, and the split windows are combined
+    in a fixed order.  Same inputs per graph (random features / edge values)
+    through the bucket's packed structure and through the graph's own."""
+    pool, models = setup
+    srv = S.Server(cuda, models, pool, capacities=(1 << 14, 1 << 16), use_graph=False)
+    graphs = [srv._admit(S.Request(0, 0.0, kind, k)) for k in (3, 0, 7, 11, 5, 2)]
+    torch.cuda.synchronize()
+    bucket = srv._pick_bucket(kind, graphs)
+    assert bucket is not None
+    rows, edges = bucket.pack(graphs)
+    gen = torch.Generator().manual_seed(17)
+    if kind == "gcn":
+        D = S.GCN_HID
+        Z = torch.zeros(bucket.rows_cap, D, dtype=torch.bfloat16, device=cuda)
+        ins = []
+        for g, r0 in zip(graphs, rows):
+            z = torch.randn(g.n_pad, D, generator=gen).bfloat16().to(cuda)
+            Z[r0:r0 + g.n_pad] = z
+            ins.append(z)
+        Yp = S.ops.spmm_block(bucket.rp, bucket.col, bucket.val, bucket.win, bucket.atomic, Z, bucket.rows_cap,
+                              S.WINDOW, S.WIDE, out_dtype=torch.float32, rows_covered=True)
+        for g, r0, z in zip(graphs, rows, ins):
+            s = g.gcn
+            Ya = S.ops.spmm_block(s["rp"], s["col"], s["val"], s["window"], s["atomic"], z, g.n_pad, S.WINDOW,
+                                  S.WIDE, out_dtype=torch.float32, rows_covered=True)
+            assert torch.equal(Yp[r0:r0 + g.n_pad], Ya), g.key
+    else:
+        H, D = S.GAT_HEADS, S.GAT_DIM
+        Xp = torch.zeros(bucket.rows_cap, H, D, dtype=torch.bfloat16, device=cuda)
+        P = torch.zeros(H, bucket.nnz_cap, dtype=torch.bfloat16, device=cuda)
+        ins = []
+        for g, r0, e0 in zip(graphs, rows, edges):
+            x = torch.randn(g.n_pad, H, D, generator=gen).bfloat16().to(cuda)
+            p = torch.rand(H, g.nnz, generator=gen).bfloat16().to(cuda)
+            Xp[r0:r0 + g.n_pad] = x
+            P[:, e0:e0 + g.nnz] = p
+            ins.append((x, p.contiguous()))
+        Yp = S.ops.spmm_edge_map_compact(bucket.rp, bucket.col, bucket.masks, bucket.bbase, bucket.values, P,
+                                         bucket.win, bucket.atomic, Xp, "mhd", S.WINDOW, m_ori=bucket.rows_cap,
+                                         x_layout="mhd", rows_covered=True)
+        for g, r0, (x, p) in zip(graphs, rows, ins):
+            s = g.gat
+            Ya = S.ops.spmm_edge_map_compact(s["rp"], s["col"], s["masks"], s["bbase"], s["values"], p,
+                                             s["window"], s["atomic"], x, "mhd", S.WINDOW, m_ori=g.n_pad,
+                                             x_layout="mhd", rows_covered=True)
+            assert torch.equal(Yp[r0:r0 + g.n_pad], Ya), g.key
+
+
 def test_graph_replay_matches_eager(cuda, setup):
     pool, models = setup
     for seed in (7, 8, 9):     # several batches through the same captured graphs
