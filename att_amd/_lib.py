@@ -51,6 +51,8 @@ _SIGNATURES = {
     "attgnn_spmm_block": (_int, [_vp, _vp, _vp, _int, _vp, _vp, _i64, _vp, _int, _i64, _vp, _int, _i64, _i64,
                                  _int, _int, _int, _vp, _vp, _vp, _i64, _int, _vp]),
     "attgnn_spmm_workspace_elems": (_i64, [_i64, _int, _i64]),
+    "attgnn_spmm_block_noalloc": (_int, [_vp, _vp, _vp, _int, _vp, _vp, _i64, _vp, _int, _i64, _vp, _int, _i64,
+                                         _i64, _int, _int, _int, _vp, _vp, _int, _vp]),
     "attgnn_spmm_compact": (_int, [_vp, _vp, _vp, _vp, _vp, _int, _vp, _vp, _i64, _vp, _int, _i64, _vp, _int, _i64,
                                    _i64, _int, _int, _vp, _vp, _vp, _i64, _int, _vp]),
     "attgnn_blocks_compact_sizes": (_int, [_vp, _c.POINTER(_i64), _c.POINTER(_i64)]),

@@ -72,6 +72,7 @@ extern "C" const char *attgnn_status_string(int s)
     case ATTGNN_E_FORMAT: return "malformed sparse structure";
     case ATTGNN_E_NOMEM: return "out of host memory";
     case ATTGNN_E_UNSUPPORTED: return "unsupported configuration";
+    case ATTGNN_E_SCRATCH: return "larger than the fixed scratch of the allocation-free entry point";
     default: return "HIP runtime error";
     }
 }

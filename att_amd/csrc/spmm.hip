@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "attgnn.h"
 #include "attn.hpp"
@@ -977,12 +978,15 @@ int elt_bytes(int kind) { return kind == K_FP8 ? 1 : (kind == K_F16 || kind == K
 
 }  // namespace
 
-extern "C" int attgnn_spmm_block(const int32_t *rp, const int32_t *col, const void *val, int tile_type,
-                                 const int32_t *window_of_part, const int32_t *atomic, int64_t parts,
-                                 const void *X, int x_type, int64_t n_x, void *Y, int y_type, int64_t m_ori,
-                                 int64_t dim_n, int window, int wide, int mode, const float *row_scale,
-                                 const float *col_scale, float *workspace, int64_t workspace_elems,
-                                 int flags, attgnn_stream_t stream)
+namespace {
+
+// Argument checks and launch of attgnn_spmm_block (shared with the
+// allocation-free variant, which passes its fixed scratch as the workspace).
+int spmm_block_run(const int32_t *rp, const int32_t *col, const void *val, int tile_type,
+                   const int32_t *window_of_part, const int32_t *atomic, int64_t parts, const void *X, int x_type,
+                   int64_t n_x, void *Y, int y_type, int64_t m_ori, int64_t dim_n, int window, int wide, int mode,
+                   const float *row_scale, const float *col_scale, float *workspace, int64_t workspace_elems,
+                   int flags, attgnn_stream_t stream)
 {
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     if (parts < 0 || m_ori < 0 || dim_n <= 0 || n_x < 0) return ATTGNN_E_ARG;
@@ -1021,6 +1025,79 @@ extern "C" int attgnn_spmm_block(const int32_t *rp, const int32_t *col, const vo
     a.xcd_slices = xcd_mapping(flags, a.slices);
     const hipError_t e = run_spmm(kind, A_DENSE, t, a, (flags & ATTGNN_SPMM_ROWS_COVERED) != 0, s);
     return e == hipSuccess ? ATTGNN_OK : int(e);
+}
+
+}  // namespace
+
+extern "C" int attgnn_spmm_block(const int32_t *rp, const int32_t *col, const void *val, int tile_type,
+                                 const int32_t *window_of_part, const int32_t *atomic, int64_t parts,
+                                 const void *X, int x_type, int64_t n_x, void *Y, int y_type, int64_t m_ori,
+                                 int64_t dim_n, int window, int wide, int mode, const float *row_scale,
+                                 const float *col_scale, float *workspace, int64_t workspace_elems,
+                                 int flags, attgnn_stream_t stream)
+{
+    return spmm_block_run(rp, col, val, tile_type, window_of_part, atomic, parts, X, x_type, n_x, Y, y_type, m_ori,
+                          dim_n, window, wide, mode, row_scale, col_scale, workspace, workspace_elems, flags, stream);
+}
+
+// ---------------------------------------------------------------------------
+// attgnn_spmm_block without a workspace argument, for callers that cannot
+// allocate.  Its scratch -- the split windows' partial tiles and the list of
+// split windows -- is this fixed array: every device the library runs on holds
+// one copy (ATTGNN_NOALLOC_SCRATCH_ELEMS floats, 64 MiB) until the process
+// ends.  In host code `scratch` only names the symbol: its device address is
+// looked up once per device with hipGetSymbolAddress and that pointer is what
+// the kernels receive.  Calls on one device share the array, so calls that
+// overlap in time on one device race; the split-window count is cleared by a
+// kernel at the start of every call and every partial tile is written before
+// it is read, so nothing of an earlier call's contents is used.
+static __device__ __align__(256) float scratch[ATTGNN_NOALLOC_SCRATCH_ELEMS];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mutex;
+float *scratch_address[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up on the first
+// call there).
+hipError_t scratch_on_current_device(float **out)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_address[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        scratch_address[dev] = static_cast<float *>(p);
+    }
+    *out = scratch_address[dev];
+    return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" int attgnn_spmm_block_noalloc(const int32_t *rp, const int32_t *col, const void *val, int tile_type,
+                                         const int32_t *window_of_part, const int32_t *atomic, int64_t parts,
+                                         const void *X, int x_type, int64_t n_x, void *Y, int y_type,
+                                         int64_t m_ori, int64_t dim_n, int window, int wide, int mode,
+                                         const float *row_scale, const float *col_scale, int flags,
+                                         attgnn_stream_t stream)
+{
+    float *ws = nullptr;
+    if (parts > 0 && (window == 8 || window == 16) && dim_n > 0) {
+        if (attgnn_spmm_workspace_elems(parts, window, dim_n) > ATTGNN_NOALLOC_SCRATCH_ELEMS)
+            return ATTGNN_E_SCRATCH;
+        const hipError_t e = scratch_on_current_device(&ws);
+        if (e != hipSuccess) return int(e);
+    }
+    return spmm_block_run(rp, col, val, tile_type, window_of_part, atomic, parts, X, x_type, n_x, Y, y_type, m_ori,
+                          dim_n, window, wide, mode, row_scale, col_scale, ws, ATTGNN_NOALLOC_SCRATCH_ELEMS, flags,
+                          stream);
 }
 
 namespace {

@@ -30,6 +30,8 @@ enum {
     ATTGNN_E_FORMAT = -2,   /* malformed CSR / block structure                 */
     ATTGNN_E_NOMEM = -3,    /* host allocation failed (preprocessing)          */
     ATTGNN_E_UNSUPPORTED = -4,
+    ATTGNN_E_SCRATCH = -5,  /* larger than the fixed scratch of an allocation-free
+                               entry point                                       */
 };
 const char *attgnn_status_string(int status);
 
@@ -277,6 +279,28 @@ enum {
 };
 
 int64_t attgnn_spmm_workspace_elems(int64_t parts, int window, int64_t dim_n);
+
+/*
+ * attgnn_spmm_block without the workspace argument: the same arguments
+ * otherwise and the same results bit for bit, and it allocates nothing (no
+ * hipMalloc / hipFree, no allocating library routine).  The scratch is a fixed
+ * device array of ATTGNN_NOALLOC_SCRATCH_ELEMS floats (64 MiB) that every GPU
+ * the library runs on holds until the process ends; a problem needing more
+ * (attgnn_spmm_workspace_elems(parts, window, dim_n) elements, i.e. about
+ * parts * window * dim_n) returns ATTGNN_E_SCRATCH -- for window 16, fewer
+ * than 4,096 parts at 256 features or 8,192 at 128.  All calls on one GPU share the
+ * array: calls that overlap in time on one GPU race (serialise them, e.g. on
+ * one stream).  Nothing of an earlier call's scratch contents is used.
+ *
+ * No counterpart in the reference.  For callers that cannot allocate: code
+ * under HIP graph capture, tight serving loops, harnesses that forbid it.
+ */
+#define ATTGNN_NOALLOC_SCRATCH_ELEMS (16 * 1024 * 1024)
+int attgnn_spmm_block_noalloc(const int32_t *rp, const int32_t *col, const void *val, int tile_type,
+                              const int32_t *window_of_part, const int32_t *atomic, int64_t parts,
+                              const void *X, int x_type, int64_t n_x, void *Y, int y_type, int64_t m_ori,
+                              int64_t dim_n, int window, int wide, int mode, const float *row_scale,
+                              const float *col_scale, int flags, attgnn_stream_t stream);
 
 /*
  * attgnn_spmm_block over compact tiles (attgnn_blocks_compact_export: row masks,
