@@ -51,6 +51,10 @@ _SIGNATURES = {
     "attgnn_spmm_block": (_int, [_vp, _vp, _vp, _int, _vp, _vp, _i64, _vp, _int, _i64, _vp, _int, _i64, _i64,
                                  _int, _int, _int, _vp, _vp, _vp, _i64, _int, _vp]),
     "attgnn_spmm_workspace_elems": (_i64, [_i64, _int, _i64]),
+    "attgnn_blocks_tasks_max": (_i64, [_i64, _i64, _int, _int]),
+    "attgnn_blocks_tasks_workspace_bytes": (_i64, [_i64]),
+    "attgnn_blocks_tasks_device": (_int, [_vp, _vp, _i64, _i64, _int, _int, _vp, _vp, _vp, _i64, _vp, _i64, _vp,
+                                          _vp]),
     "attgnn_spmm_block_noalloc": (_int, [_vp, _vp, _vp, _int, _vp, _vp, _i64, _vp, _int, _i64, _vp, _int, _i64,
                                          _i64, _int, _int, _int, _vp, _vp, _int, _vp]),
     "attgnn_spmm_compact": (_int, [_vp, _vp, _vp, _vp, _vp, _int, _vp, _vp, _i64, _vp, _int, _i64, _vp, _int, _i64,

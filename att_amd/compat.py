@@ -8,6 +8,9 @@ reference's pybind11 extensions, so reference-style callers run unchanged::
 
 Same positional arguments, same argument meaning, every op returns a list.
 Differences from the reference (all on the side of strictness):
+  * the SpMMs run over a task table derived from the structure's windows
+    (ops.canonical_tasks), so their results are bitwise the same whatever
+    partSize the structure was built with;
   * inputs are validated (device, dtype, contiguity, sizes) and errors raise
     instead of corrupting memory;
   * kernels run on torch's current stream, not the legacy default stream;
@@ -38,6 +41,16 @@ DEFAULT_WIDE = {"fp16": 8, "tf32": 4}
 
 def _wide_of(prec, wide):
     return DEFAULT_WIDE[prec] if wide is None else int(wide)
+
+
+def _tasks(rp, col, tiles, window_of_part, wide):
+    """(rp, col, tiles, window, atomic) of the canonical SpMM task table of a
+    reference structure: its windows cut into chunks fixed by the structure's
+    size, so the product does not depend on the partSize the caller built it
+    with (the reference's t_atomic is implied by the table)."""
+    ops.check_part_spans(rp, wide, "structure")
+    t_rp, t_win, t_at = ops.canonical_tasks(rp, window_of_part, col.numel(), wide)
+    return t_rp, col, tiles, t_win, t_at
 
 
 def _window_of(col, tiles):
@@ -73,7 +86,7 @@ def forward_fp16_gnn(row_offsets, col_indices, values, t_window_row, t_atomic, r
     if rhs.shape[1] != dimN:
         raise ValueError(f"dimN={dimN} but rhs has {rhs.shape[1]} columns")
     window = _window_of(col_indices, values)
-    y = ops.spmm_block(row_offsets, col_indices, values, t_window_row, t_atomic, rhs.contiguous(), int(mOri),
+    y = ops.spmm_block(*_tasks(row_offsets, col_indices, values, t_window_row, wide), rhs.contiguous(), int(mOri),
                        window, wide, mode="native")
     return [y]
 
@@ -84,7 +97,7 @@ def forward_tf32_gnn(row_offsets, col_indices, values, t_window_row, t_atomic, r
     if rhs.shape[1] != dimN:
         raise ValueError(f"dimN={dimN} but rhs has {rhs.shape[1]} columns")
     window = _window_of(col_indices, values)
-    y = ops.spmm_block(row_offsets, col_indices, values, t_window_row, t_atomic, rhs.contiguous(), int(mOri),
+    y = ops.spmm_block(*_tasks(row_offsets, col_indices, values, t_window_row, wide), rhs.contiguous(), int(mOri),
                        window, wide, mode="bf16x3")
     return [y]
 
@@ -159,7 +172,8 @@ def _spmm_map(version, prec, layout):
         w = _window_of(column_index, degrees)
         x = rhs if rhs.dim() == 3 else rhs.unsqueeze(0)
         vals = values.reshape(x.shape[0], -1)
-        y = ops.spmm_edge_map(row_pointers, column_index, degrees, vals.contiguous(), window, automic,
+        t_rp, _, _, t_win, t_at = _tasks(row_pointers, column_index, degrees, window, wide)
+        y = ops.spmm_edge_map(t_rp, column_index, degrees, vals.contiguous(), t_win, t_at,
                               x.contiguous(), layout, w, wide, mode=mode)
         if version == 1 and rhs.dim() == 2:
             y = y[0] if layout == "hmd" else y[:, 0]

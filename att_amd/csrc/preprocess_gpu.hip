@@ -26,6 +26,7 @@
 //          7. pack (values and positions in block / 8-column group / row order).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 
 #include <rocprim/device/device_scan.hpp>
@@ -813,5 +814,140 @@ extern "C" int attgnn_map_transpose_perm_device(const int32_t *map, const int32_
     if (n_val == 0) return ATTGNN_OK;
     hipLaunchKernelGGL(map_scatter_kernel, dim3(grid_for(n_val, THREADS)), dim3(THREADS), 0,
                        reinterpret_cast<hipStream_t>(stream), map, map_t, n_val, nnz, perm);
+    return launch_status();
+}
+
+// ---------------------------------------------------------------------------
+// Canonical SpMM task table of a window-block structure.  The reference splits
+// a window into parts of partSize blocks (example.cpp:120-151), and a split
+// window's sum is then grouped by that choice.  Here the SpMM works on tasks
+// derived from the windows alone: each window's condensed columns [start,
+// padded end) are cut into chunks of chunk_blocks * wide columns from the
+// window start (at least one task per window, empty windows included), in
+// window order, written in the structure's own format (running (real, pad)
+// pairs, window, atomic = the window has more than one chunk).  Every sum
+// then depends only on the windows and the chunk size, not on how the caller
+// split them.  Parts with a negative window are dropped; task slots past the
+// total get window -1 and empty column ranges.
+namespace {
+
+__global__ void task_count_kernel(const int32_t *rp, const int32_t *win, int64_t parts, int64_t chunk_cols,
+                                  int64_t *counts)
+{
+    const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p == parts) counts[parts] = 0;
+    if (p >= parts) return;
+    const int32_t w = win[p];
+    int64_t c = 0;
+    if (w >= 0 && (p == 0 || win[p - 1] != w)) {
+        int64_t q = p;
+        while (q + 1 < parts && win[q + 1] == w) ++q;
+        const int64_t span = int64_t(rp[2 * q + 2]) - rp[2 * p];
+        c = span <= 0 ? 1 : (span + chunk_cols - 1) / chunk_cols;
+    }
+    counts[p] = c;
+}
+
+__global__ void task_write_kernel(const int32_t *rp, const int32_t *win, int64_t parts, int64_t chunk_cols,
+                                  const int64_t *counts, const int64_t *first, int32_t *t_rp, int32_t *t_win,
+                                  int32_t *t_atomic)
+{
+    const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= parts || counts[p] == 0) return;
+    const int32_t w = win[p];
+    int64_t q = p;
+    while (q + 1 < parts && win[q + 1] == w) ++q;
+    const int64_t s0 = rp[2 * p], real_end = rp[2 * q + 1], pad_end = rp[2 * q + 2];
+    const int64_t c = counts[p];
+    for (int64_t k = 0; k < c; ++k) {
+        const int64_t t = first[p] + k;
+        const int64_t s = s0 + k * chunk_cols;
+        const int64_t e = s + chunk_cols < pad_end ? s + chunk_cols : pad_end;
+        const int64_t re = real_end < e ? real_end : e;
+        t_rp[2 * t] = int32_t(s);
+        t_rp[2 * t + 1] = int32_t(re > s ? re : s);
+        t_win[t] = w;
+        t_atomic[t] = c > 1 ? 1 : 0;
+    }
+}
+
+// Slots past the total: window -1, empty ranges at the structure's end; and
+// the closing entry of the running pairs.
+__global__ void task_tail_kernel(const int32_t *rp, int64_t parts, const int64_t *first, int64_t max_tasks,
+                                 int32_t *t_rp, int32_t *t_win, int32_t *t_atomic, int64_t *n_tasks)
+{
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int32_t end = rp[2 * parts];
+    if (t == 0) {
+        t_rp[2 * max_tasks] = end;
+        *n_tasks = first[parts];
+    }
+    if (t >= max_tasks || t < first[parts]) return;
+    t_rp[2 * t] = end;
+    t_rp[2 * t + 1] = end;
+    t_win[t] = -1;
+    t_atomic[t] = 0;
+}
+
+size_t task_scan_temp_bytes(int64_t parts)
+{
+    size_t bytes = 0;
+    const hipError_t e = rocprim::exclusive_scan(nullptr, bytes, static_cast<const int64_t *>(nullptr),
+                                                 static_cast<int64_t *>(nullptr), int64_t(0), size_t(parts + 1),
+                                                 rocprim::plus<int64_t>());
+    return e == hipSuccess ? bytes : size_t(1) << 40;
+}
+
+int64_t task_ws_bytes(int64_t parts)
+{
+    const int64_t a = (8 * (parts + 1) + 255) / 256 * 256;
+    return 2 * a + int64_t(task_scan_temp_bytes(parts)) + 256;
+}
+
+}  // namespace
+
+extern "C" int64_t attgnn_blocks_tasks_max(int64_t parts, int64_t n_cols, int wide, int chunk_blocks)
+{
+    if (parts < 0 || n_cols < 0 || wide <= 0 || chunk_blocks <= 0) return -1;
+    const int64_t chunk_cols = int64_t(chunk_blocks) * wide;
+    return parts + (n_cols + chunk_cols - 1) / chunk_cols;
+}
+
+extern "C" int64_t attgnn_blocks_tasks_workspace_bytes(int64_t parts)
+{
+    if (parts < 0) return -1;
+    return task_ws_bytes(parts);
+}
+
+extern "C" int attgnn_blocks_tasks_device(const int32_t *rp, const int32_t *window_of_part, int64_t parts,
+                                          int64_t n_cols, int wide, int chunk_blocks, int32_t *task_rp,
+                                          int32_t *task_window, int32_t *task_atomic, int64_t max_tasks,
+                                          void *workspace, int64_t workspace_bytes, int64_t *n_tasks,
+                                          attgnn_stream_t stream)
+{
+    if (parts < 0 || n_cols < 0 || wide <= 0 || chunk_blocks <= 0 || !rp || !task_rp) return ATTGNN_E_ARG;
+    if (max_tasks < attgnn_blocks_tasks_max(parts, n_cols, wide, chunk_blocks)) return ATTGNN_E_ARG;
+    if (!workspace || !n_tasks || workspace_bytes < task_ws_bytes(parts)) return ATTGNN_E_ARG;
+    if ((parts > 0 && !window_of_part) || (max_tasks > 0 && (!task_window || !task_atomic))) return ATTGNN_E_ARG;
+    // one thread per part / task slot (no grid-stride loop): below 2^28 each
+    if (parts >= (int64_t(1) << 28) || max_tasks >= (int64_t(1) << 28)) return ATTGNN_E_UNSUPPORTED;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int64_t chunk_cols = int64_t(chunk_blocks) * wide;
+    // counts[parts + 1] and their exclusive scan first[parts + 1] in the
+    // workspace (first[parts] = the task total), then rocPRIM's storage
+    const int64_t a = (8 * (parts + 1) + 255) / 256 * 256;
+    int64_t *counts = static_cast<int64_t *>(workspace);
+    int64_t *first = reinterpret_cast<int64_t *>(static_cast<uint8_t *>(workspace) + a);
+    uint8_t *temp = static_cast<uint8_t *>(workspace) + 2 * a;
+    size_t tb = task_scan_temp_bytes(parts);
+    hipLaunchKernelGGL(task_count_kernel, dim3(grid_for(parts + 1, THREADS)), dim3(THREADS), 0, s, rp,
+                       window_of_part, parts, chunk_cols, counts);
+    hipError_t e = rocprim::exclusive_scan(temp, tb, counts, first, int64_t(0), size_t(parts + 1),
+                                           rocprim::plus<int64_t>(), s);
+    if (e != hipSuccess) return int(e);
+    hipLaunchKernelGGL(task_write_kernel, dim3(grid_for(parts, THREADS)), dim3(THREADS), 0, s, rp, window_of_part,
+                       parts, chunk_cols, counts, first, task_rp, task_window, task_atomic);
+    hipLaunchKernelGGL(task_tail_kernel, dim3(grid_for(std::max<int64_t>(max_tasks, 1), THREADS)), dim3(THREADS), 0,
+                       s, rp, parts, first, max_tasks, task_rp, task_window, task_atomic, n_tasks);
     return launch_status();
 }

@@ -62,11 +62,12 @@ class GCNGraph:
     """
 
     def __init__(self, csr, device, window=16, wide=32, part_size=None, tile_dtype=torch.float16,
-                 symmetric=None, values=None, compact="auto", build="host"):
+                 symmetric=None, values=None, compact="auto", build="host", chunk_blocks=None):
         if build not in ("host", "device"):
             raise ValueError(f"build must be 'host' or 'device', got {build!r}")
         self.build = build
         self.window, self.wide = window, wide
+        self.chunk_blocks = chunk_blocks
         # compact tiles win once a gathered row is >= 256 bytes; "auto" keeps
         # both forms and picks per call
         self.compact = compact if wide == 32 else False
@@ -126,13 +127,17 @@ class GCNGraph:
         arrays = [b[k].to(device) for k in keys]
         self.n_unique = b["n_unique"]
         out = {"arrays": arrays, "parts": int(b["window"].numel()), "n_cols": int(b["col"].numel())}
+        # the SpMM runs over the canonical task table (chunks of the windows
+        # fixed by the structure's size): results do not depend on part_size
+        out["tasks"] = ops.canonical_tasks(arrays[0], arrays[3], out["n_cols"], self.wide, self.chunk_blocks)
         if self.compact:
             out["compact"] = [b[k].to(device) for k in ("masks", "block_base", "values")]
         return out
 
     def spmm(self, X, transpose=False, out_dtype=None):
         s = self.bwd if transpose else self.fwd
-        rp, col, val, win, at = s["arrays"]
+        _, col, val, _, _ = s["arrays"]
+        rp, win, at = s["tasks"]
         if X.shape[0] < self.num_nodes_ori:
             raise ValueError("X has fewer rows than the graph")
         use_compact = self.compact is True or (self.compact == "auto" and X.shape[1] * X.element_size() >= 256)
@@ -250,7 +255,7 @@ class GATGraph:
     sums).  Attribute names follow the reference dataset
     (end2end_head/gat_final/mgat_csr/mdataset_fp16.py:53-65)."""
 
-    def __init__(self, csr, device, window=16, wide=32, part_size=None, build="host"):
+    def __init__(self, csr, device, window=16, wide=32, part_size=None, build="host", chunk_blocks=None):
         if build not in ("host", "device"):
             raise ValueError(f"build must be 'host' or 'device', got {build!r}")
         self.build = build
@@ -298,10 +303,12 @@ class GATGraph:
             self.t_rowptr = t_rowptr[: csr.n + 1].contiguous()
             self.t_col = t_col
             self.t_perm = t_perm
+        self.chunk_blocks = chunk_blocks
         if symmetric or build == "device":
             self.row_pointers, self.column_index, self.degrees, self.window_t, self.automic = self.fwd
             self.degrees_trans = self.bwd[2]
             self._compact_bwd()
+            self._use_tasks()
             return
         t, order = transpose_csr(padded)
         bt = ops.blocks_build(t.rowptr32(), t.indices, "map", None, torch.int32, window, wide, self.part_size,
@@ -317,6 +324,18 @@ class GATGraph:
         self.row_pointers, self.column_index, self.degrees, self.window_t, self.automic = self.fwd
         self.degrees_trans = self.bwd[2]
         self._compact_bwd()
+        self._use_tasks()
+
+    def _use_tasks(self):
+        """The kernels run over the canonical task tables of A and A^T (chunks of
+        at least 32 blocks -- the heads multiply the partial-tile traffic of
+        split windows -- fixed by the structure's size): results do not depend
+        on part_size.  The reference-named attributes keep the structure."""
+        for name in ("fwd", "bwd"):
+            rp, col, emap, win, _ = getattr(self, name)
+            chunk = self.chunk_blocks or max(ops.canonical_chunk(col.numel(), self.wide), 32)
+            t_rp, t_win, t_at = ops.canonical_tasks(rp, win, col.numel(), self.wide, chunk)
+            setattr(self, name, [t_rp, col, emap, t_win, t_at])
 
     def _compact_bwd(self):
         if self.wide == 32:

@@ -314,6 +314,64 @@ def map_remap(emap, remap):
 
 
 # --------------------------------------------------------------------------
+# canonical SpMM task table
+# --------------------------------------------------------------------------
+
+def canonical_chunk(n_cols, wide, lo=128, hi=16384):
+    """Blocks of ``wide`` columns per SpMM task, from the structure's size
+    alone (never from the partSize it was built with): n_cols / 8192 columns,
+    clamped to [128, 16384] and rounded down to a multiple of 32 -- e.g. 4
+    blocks of 32 for small graphs, 375 for 98.4M condensed columns."""
+    cols = min(max(int(n_cols) // 8192, lo), hi) // 32 * 32
+    return max(cols // int(wide), 1)
+
+
+def canonical_tasks(rp, window_of_part, n_cols, wide, chunk_blocks=None):
+    """(task_rp, task_window, task_atomic): each window's condensed columns cut
+    into chunks of ``chunk_blocks`` * ``wide`` columns from the window start
+    (attgnn_blocks_tasks_device), in the structure's part format with the slots
+    past the total marked window -1.  An SpMM over these tasks groups every
+    window's sum by the chunk size alone, so its result is bitwise the same
+    whichever partSize the structure was built with.  Kept on ``rp`` (with its
+    version counter) and reused until ``rp`` is written in place."""
+    dev = _need_gpu("canonical_tasks", rp, window_of_part)
+    _need_i32("canonical_tasks", rp, window_of_part)
+    parts = window_of_part.numel()
+    if rp.numel() != 2 * parts + 1:
+        raise ValueError("rp and window_of_part sizes disagree")
+    chunk = int(chunk_blocks or canonical_chunk(n_cols, wide))
+    try:
+        version = rp._version
+    except RuntimeError:
+        version = None
+    tag = (version, rp.numel(), window_of_part.data_ptr(), int(n_cols), int(wide), chunk)
+    cached = getattr(rp, "_attgnn_tasks", None)
+    if version is not None and cached is not None and cached[0] == tag:
+        return cached[1]
+    L = lib()
+    tmax = L.attgnn_blocks_tasks_max(parts, int(n_cols), int(wide), chunk)
+    if tmax < 0:
+        raise ValueError("bad structure sizes")
+    i32 = dict(dtype=torch.int32, device=dev)
+    t_rp = torch.empty(2 * tmax + 1, **i32)
+    t_win = torch.empty(max(tmax, 0), **i32)
+    t_at = torch.empty(max(tmax, 0), **i32)
+    ws = torch.empty(L.attgnn_blocks_tasks_workspace_bytes(parts), dtype=torch.uint8, device=dev)
+    n_tasks = torch.empty(1, dtype=torch.int64, device=dev)
+    check(L.attgnn_blocks_tasks_device(_ptr(rp), _ptr(window_of_part), parts, int(n_cols), int(wide), chunk,
+                                       _ptr(t_rp), _ptr(t_win), _ptr(t_at), tmax, _ptr(ws), ws.numel(),
+                                       _ptr(n_tasks), _stream(dev)),
+          "attgnn_blocks_tasks_device")
+    out = (t_rp, t_win, t_at)
+    if version is not None:
+        try:
+            rp._attgnn_tasks = (tag, out)
+        except (AttributeError, RuntimeError):
+            pass
+    return out
+
+
+# --------------------------------------------------------------------------
 # SpMM
 # --------------------------------------------------------------------------
 

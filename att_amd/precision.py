@@ -24,7 +24,8 @@ _KMODE = {"f32": "f32", "bf16x3": "bf16x3", "bf16": "native", "fp16": "native", 
 
 
 class BlockMatrix:
-    def __init__(self, csr, values, device, mode="bf16", window=16, wide=32, part_size=None, build="host"):
+    def __init__(self, csr, values, device, mode="bf16", window=16, wide=32, part_size=None, build="host",
+                 chunk_blocks=None):
         if mode not in _TILE:
             raise ValueError(f"unknown mode {mode!r}")
         from .layers import _blocks_estimate, default_part_size
@@ -51,6 +52,9 @@ class BlockMatrix:
         arrs = [b[k].to(device) for k in ("rp", "col", "val", "window", "atomic")]
         if mode == "fp8":
             arrs[2] = arrs[2].to(torch.float8_e4m3fn)
+        # the product runs over the canonical task table (results independent of part_size)
+        t_rp, t_win, t_at = ops.canonical_tasks(arrs[0], arrs[3], arrs[1].numel(), wide, chunk_blocks)
+        arrs = [t_rp, arrs[1], arrs[2], t_win, t_at]
         self.arrays = arrs
         self.parts = int(b["window"].numel())
 

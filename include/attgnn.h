@@ -221,6 +221,33 @@ int attgnn_blocks_compact_fill_device(const void *val, int tile_type, int64_t n_
                                       const uint16_t *masks, const int64_t *block_base, void *values,
                                       uint16_t *positions, attgnn_stream_t stream);
 
+/*
+ * Canonical SpMM task table of a window-block structure (device).  Each
+ * window's condensed columns, from its start to its padded end, are cut into
+ * chunks of chunk_blocks * wide columns (at least one task per window, empty
+ * windows included); the tasks are written in the structure's own part format
+ * -- task_rp [2*max_tasks+1] running (real, pad) pairs, task_window[max_tasks],
+ * task_atomic[max_tasks] = 1 when the window has more than one chunk -- in
+ * window order, slots past the total with window -1 and empty column ranges
+ * (which every attgnn_spmm_* entry point skips).  *n_tasks (device int64)
+ * receives the total.  Run over these tasks instead of the parts, a window's
+ * SpMM sum is grouped by the chunk size alone: results are bitwise the same
+ * whatever partSize split the structure.  max_tasks >= attgnn_blocks_tasks_max
+ * (parts + ceil(n_cols / (chunk_blocks*wide))); workspace of
+ * attgnn_blocks_tasks_workspace_bytes(parts) bytes.  Parts with a negative
+ * window are dropped.
+ *
+ * No counterpart in the reference (whose sums are grouped by partSize,
+ * gcn/Block_gcn/example.cpp:120-151, and accumulated with atomicAdd,
+ * gcn/SpMM/src/spmmKernel.cu:226-262).
+ */
+int64_t attgnn_blocks_tasks_max(int64_t parts, int64_t n_cols, int wide, int chunk_blocks);
+int64_t attgnn_blocks_tasks_workspace_bytes(int64_t parts);
+int attgnn_blocks_tasks_device(const int32_t *rp, const int32_t *window_of_part, int64_t parts, int64_t n_cols,
+                               int wide, int chunk_blocks, int32_t *task_rp, int32_t *task_window,
+                               int32_t *task_atomic, int64_t max_tasks, void *workspace, int64_t workspace_bytes,
+                               int64_t *n_tasks, attgnn_stream_t stream);
+
 /* ====================================================================== */
 /* Window-block SpMM (device)                                              */
 /* ====================================================================== */

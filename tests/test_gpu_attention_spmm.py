@@ -220,8 +220,8 @@ def test_aggregate_sddmm_fused_split_windows(cuda, heads, dim):
     partials and the ordered combine, its dot products straight to the edges;
     both equal the separate kernels bit for bit."""
     g = G.chung_lu(900, 10, 2.1, seed=7)
-    gg = GATGraph(g, cuda, build="device", part_size=1)
-    assert int(gg.fwd[4].sum().item()) > 0          # some parts belong to split windows
+    gg = GATGraph(g, cuda, build="device", part_size=1, chunk_blocks=1)
+    assert int(gg.fwd[4].sum().item()) > 0          # some tasks belong to split windows
     gen = torch.Generator(device="cpu").manual_seed(dim)
     p = torch.rand(heads, g.nnz, generator=gen).bfloat16().to(cuda)
     dh = torch.randn(g.n, heads, dim, generator=gen).bfloat16().to(cuda)

@@ -7,6 +7,7 @@ import torch
 import att_amd.compat as compat
 from att_amd import graphs as G
 from att_amd import ops
+from att_amd.layers import GCNGraph
 from oracle import dense as O
 from tests.gpu_util import TOY_COL, TOY_ROW, csr_rows, rand_csr
 
@@ -131,6 +132,109 @@ def test_determinism_run_to_run(cuda):
     for _ in range(3):
         y1 = ops.spmm_block(*args, X, n, 16, 32)
         assert torch.equal(y0, y1)
+
+
+def _tasks_np(rp, win, chunk_cols, tmax):
+    """Restatement of the canonical task table: each window's columns [start,
+    padded end) cut into chunks of chunk_cols from its start (one task for an
+    empty window), running (real, pad) pairs; slots past the total get
+    window -1 and empty ranges at the structure's end."""
+    t_rp, t_win, t_at = [], [], []
+    P, p = len(win), 0
+    while p < P:
+        w = int(win[p])
+        if w < 0:
+            p += 1
+            continue
+        q = p
+        while q + 1 < P and win[q + 1] == w:
+            q += 1
+        s0, real_end, pad_end = int(rp[2 * p]), int(rp[2 * q + 1]), int(rp[2 * q + 2])
+        c = 1 if pad_end <= s0 else -(-(pad_end - s0) // chunk_cols)
+        for k in range(c):
+            s = s0 + k * chunk_cols
+            e = min(s + chunk_cols, pad_end)
+            t_rp += [s, max(s, min(real_end, e))]
+            t_win.append(w)
+            t_at.append(int(c > 1))
+        p = q + 1
+    end = int(rp[2 * P])
+    while len(t_win) < tmax:
+        t_rp += [end, end]
+        t_win.append(-1)
+        t_at.append(0)
+    return np.array(t_rp + [end], np.int32), np.array(t_win, np.int32), np.array(t_at, np.int32)
+
+
+@pytest.mark.parametrize("window,wide,ps,chunk", [(16, 32, 1, 1), (16, 32, 2, 3), (16, 32, 32, 4), (8, 8, 1, 2),
+                                                  (8, 4, 3, 5), (16, 32, 1 << 20, 2)])
+@pytest.mark.parametrize("emit_empty", [True, False])
+def test_canonical_tasks_vs_restatement(cuda, window, wide, ps, chunk, emit_empty):
+    rng = np.random.default_rng(window + wide + ps + chunk)
+    n = 600
+    rp, col = rand_csr(rng, n, 50, hub_rows=4)
+    b = ops.blocks_build(rp, col, "map", None, torch.int32, window, wide, ps, emit_empty=emit_empty)
+    rpd, wind = b["rp"].to(cuda), b["window"].to(cuda)
+    t_rp, t_win, t_at = ops.canonical_tasks(rpd, wind, b["col"].numel(), wide, chunk)
+    want = _tasks_np(b["rp"].numpy(), b["window"].numpy(), chunk * wide, t_win.numel())
+    assert np.array_equal(t_rp.cpu().numpy(), want[0])
+    assert np.array_equal(t_win.cpu().numpy(), want[1])
+    assert np.array_equal(t_at.cpu().numpy(), want[2])
+    assert ops.canonical_tasks(rpd, wind, b["col"].numel(), wide, chunk)[0] is t_rp      # kept on rp
+
+
+@pytest.mark.parametrize("dim", [64, 256])
+@pytest.mark.parametrize("build", ["host", "device"])
+def test_split_factor_invariance_native(cuda, dim, build):
+    """The SpMM of GCNGraph is bitwise the same for part sizes 1, 2, 32 and
+    unbounded: it runs over the canonical task table, so every window's sum
+    is grouped by the chunk size alone (dense and compact tiles)."""
+    g = G.chung_lu(4000, 60, 2.1, seed=3)
+    X = torch.randn(g.n, dim, generator=torch.Generator().manual_seed(dim)).half().to(cuda)
+    ys = []
+    for ps in (1, 2, 32, 1 << 20):
+        gg = GCNGraph(g, cuda, part_size=ps, tile_dtype=torch.float16, build=build)
+        assert int(gg.fwd["tasks"][2].sum().item()) > 0          # split windows in the task table
+        ys.append(gg.spmm(X, out_dtype=torch.float32))
+    assert all(torch.equal(ys[0], y) for y in ys[1:])
+
+
+@pytest.mark.parametrize("prec", ["fp16", "tf32"])
+def test_split_factor_invariance_dropin(cuda, prec):
+    """SpMM_sr_bcrs.forward_{fp16,tf32}_gnn on structures built by the
+    reference-named preprocessing with partSize 1, 2, 32 and unbounded: the
+    same result bit for bit (and exact on small-integer inputs)."""
+    m = compat.MODULES
+    g = G.chung_lu(3000, 40, 2.1, seed=5)
+    wide = 8 if prec == "fp16" else 4
+    dt = torch.float16 if prec == "fp16" else torch.float32
+    dd = torch.from_numpy(G.gcn_norm_degree(g)).to(dt)
+    rhs = torch.randn(g.n, 64, generator=torch.Generator().manual_seed(1)).to(dt).to(cuda)
+    build = getattr(m["Block_sr_bcrs"], f"blockProcess_{prec}_balance")
+    fwd = getattr(m["SpMM_sr_bcrs"], f"forward_{prec}_gnn")
+    ys = []
+    for ps in (1, 2, 32, 1 << 20):
+        rp, col, val, win, at = build(torch.from_numpy(g.rowptr32()), torch.from_numpy(g.indices), dd, 8, wide, ps)
+        if ps == 1:
+            assert int(at.sum()) > 0
+        ys.append(fwd(rp.to(cuda), col.to(cuda), val.to(cuda), win.to(cuda), at.to(cuda), rhs, g.n, 64, g.n)[0])
+    assert all(torch.equal(ys[0], y) for y in ys[1:])
+
+
+def test_split_factor_invariance_attention(cuda):
+    """GATGraph's aggregations (A and A^T, compact edge lists) and the fused
+    backward product are the same bit for bit for every part size."""
+    from att_amd.layers import GATGraph
+    g = G.chung_lu(3000, 40, 2.1, seed=6)
+    gen = torch.Generator().manual_seed(2)
+    p = torch.rand(2, g.nnz, generator=gen).bfloat16().to(cuda)
+    X = torch.randn(g.n, 2, 64, generator=gen).bfloat16().to(cuda)
+    outs = []
+    for ps in (1, 2, 32, 1 << 20):
+        gg = GATGraph(g, cuda, build="device", part_size=ps, chunk_blocks=2)
+        outs.append((gg.aggregate(p, X), *gg.aggregate_sddmm(p, X, X)))
+    for o in outs[1:]:
+        assert all(torch.equal(a, b) for a, b in zip(outs[0], o))
 
 
 def test_empty_and_degenerate(cuda):
