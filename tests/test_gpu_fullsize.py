@@ -1,0 +1,150 @@
+"""The benchmark workloads at their stated sizes, checked through properties
+that do not need a full-size reference product:
+
+* GCN graph (232,965 nodes, ~114.6M entries, FP16 tiles): the width-256 SpMM
+  against FP64 on a strided row sample; A . 1 against the FP64 row sums of the
+  FP16 tile values on every row; exact homogeneity A . (2X) = 2 (A . X) and
+  run-to-run bitwise equality; one 2-layer training step with finite loss
+  and gradients.
+* Attention graph (169,343 nodes, ~2.48M entries): the GAT layer at 1, 2, 4
+  and 8 heads of 64 (BF16) against plain FP32 message passing with the same
+  BF16-rounded inputs, forward and the weight gradient; the multi-head GATv2
+  with the min-max logit map against a float32 edge-list restatement.
+* Precision matrix (2^20 nodes, 20M entries): every arithmetic mode against
+  FP64 on a strided row sample, within the stated bounds.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+import torch.nn.functional as F
+
+from att_amd import graphs as G
+from att_amd.benchmarks.plain import gat_fp32
+from att_amd.layers import GATConv_multi, GATGraph, GATv2Conv_multi, GCNConv, GCNGraph
+from att_amd.precision import BlockMatrix
+
+pytestmark = pytest.mark.gpu
+
+
+def _row_sample(indptr, indices, values, rows):
+    counts = np.diff(indptr)[rows]
+    ip = np.concatenate([[0], np.cumsum(counts)])
+    take = np.concatenate([np.arange(indptr[r], indptr[r + 1]) for r in rows])
+    return ip, indices[take], np.asarray(values)[take]
+
+
+@pytest.fixture(scope="module")
+def gcn(cuda):
+    g = G.chung_lu(232_965, (114_600_000 - 232_965) / 232_965, 2.1, seed=0, target_nnz=114_600_000)
+    vals = G.gcn_norm_values(g, G.gcn_norm_degree(g))
+    graph = GCNGraph(g, cuda, tile_dtype=torch.float16, symmetric=True, values=vals, build="device")
+    return g, vals, graph
+
+
+def test_gcn_fullsize_spmm(cuda, gcn):
+    g, vals, graph = gcn
+    assert abs(g.nnz - 114_600_000) < 0.005 * 114_600_000
+    X = torch.randn(g.n, 256, generator=torch.Generator().manual_seed(1)).half().to(cuda)
+    y = graph.spmm(X, out_dtype=torch.float32)
+    # FP64 on every 100th row, same FP16-rounded values and X
+    rows = np.arange(0, g.n, 100)
+    ip, ix, v = _row_sample(g.indptr, g.indices, vals.astype(np.float16).astype(np.float64), rows)
+    ref = sp.csr_matrix((v, ix, ip), shape=(rows.size, g.n)) @ X.double().cpu().numpy()
+    got = y[torch.from_numpy(rows).to(cuda)].double().cpu().numpy()
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-5
+    # homogeneity under an exact scaling, and run-to-run equality
+    assert torch.equal(graph.spmm(X * 2, out_dtype=torch.float32), y * 2)
+    assert torch.equal(graph.spmm(X, out_dtype=torch.float32), y)
+    # A . 1 on every row against the FP64 row sums of the FP16 values
+    ones = torch.ones(g.n, 64, dtype=torch.float16, device=cuda)
+    s = graph.spmm(ones, out_dtype=torch.float32)[:, 0].double().cpu().numpy()
+    want = np.add.reduceat(vals.astype(np.float16).astype(np.float64), g.indptr[:-1])
+    rel = np.abs(s - want) / np.abs(want)
+    # FP32 accumulation over up to a few thousand 32-column blocks per row
+    assert np.median(rel) < 1e-6 and rel.max() < 1e-4
+
+
+def test_gcn_fullsize_training_step(cuda, gcn):
+    g, vals, graph = gcn
+    torch.manual_seed(3)
+    c1, c2 = GCNConv(602, 256).to(cuda), GCNConv(256, 48).to(cuda)
+    X = torch.randn(g.n, 602, generator=torch.Generator().manual_seed(1)).half().to(cuda)
+    labels = torch.randint(0, 41, (g.n,), generator=torch.Generator().manual_seed(2)).to(cuda)
+    out = c2(F.dropout(F.relu(c1(X, graph)), 0.5), graph)
+    loss = F.nll_loss(F.log_softmax(out[:, :41].float(), 1), labels)
+    loss.backward()
+    assert torch.isfinite(loss)
+    for c in (c1, c2):
+        assert torch.isfinite(c.weights.grad).all() and c.weights.grad.abs().sum() > 0
+
+
+@pytest.fixture(scope="module")
+def att(cuda):
+    g = G.directed_chung_lu_symmetrized(169_343, 2_480_000, seed=0)
+    return g, GATGraph(g, cuda, build="device")
+
+
+@pytest.mark.parametrize("heads", [1, 2, 4, 8])
+def test_gat_fullsize_vs_plain_fp32(cuda, att, heads):
+    g, graph = att
+    assert abs(g.nnz - 2_480_000) < 0.005 * 2_480_000
+    dim = 64
+    torch.manual_seed(heads)
+    layer = GATConv_multi(128, dim, 0.0, 0.2, heads, dtype=torch.bfloat16).to(cuda)
+    gen = torch.Generator().manual_seed(heads)
+    X = torch.randn(g.n, 128, generator=gen).bfloat16().to(cuda)
+    go = torch.randn(g.n, heads, dim, generator=gen).bfloat16().to(cuda)
+    out = layer(X, graph)
+    out.backward(go)
+    W = layer.fc.weight.detach().bfloat16().float().clone().requires_grad_(True)
+    a0 = layer.a0.detach().bfloat16().float().clone()
+    a1 = layer.a1.detach().bfloat16().float().clone()
+    rows = torch.from_numpy(g.row_ids().astype(np.int64)).to(cuda)
+    cols = torch.from_numpy(g.indices.astype(np.int64)).to(cuda)
+    ref = gat_fp32(X.float(), W, a0, a1, rows, cols, g.n, heads, dim)
+    ref.backward(go.float())
+    rel = lambda a, b: ((a.float() - b).norm() / b.norm()).item()
+    assert rel(out, ref.detach()) < 1e-2
+    assert rel(layer.fc.weight.grad, W.grad) < 5e-2
+
+
+def test_gatv2_minmax_fullsize_vs_fp32(cuda, att):
+    g, graph = att
+    heads, dim = 4, 64
+    torch.manual_seed(7)
+    layer = GATv2Conv_multi(128, dim, 0.0, 0.2, heads, dtype=torch.bfloat16).to(cuda)
+    X = torch.randn(g.n, 128, generator=torch.Generator().manual_seed(8)).bfloat16().to(cuda)
+    out = layer(X, graph).float()
+    with torch.no_grad():
+        X0 = F.leaky_relu(X.float() @ layer.fc0.weight.bfloat16().float().t(), 0.2).view(g.n, heads, dim)
+        X1 = F.leaky_relu(X.float() @ layer.fc1.weight.bfloat16().float().t(), 0.2).view(g.n, heads, dim)
+        a = layer.a0.bfloat16().float().view(1, heads, dim)
+        rows = torch.from_numpy(g.row_ids().astype(np.int64)).to(cuda)
+        cols = torch.from_numpy(g.indices.astype(np.int64)).to(cuda)
+        z = (X0 * a).sum(-1)[rows] + (X1 * a).sum(-1)[cols]
+        z = (z - z.min(0).values) / (z.max(0).values - z.min(0).values)
+        p = torch.exp(z)
+        S = torch.zeros(g.n, heads, device=cuda).index_add_(0, rows, p)
+        ref = torch.zeros(g.n, heads, dim, device=cuda).index_add_(0, rows, p.unsqueeze(-1) * X1[cols])
+        ref = ref / S.unsqueeze(-1)
+    assert ((out - ref).norm() / ref.norm()).item() < 1e-2
+
+
+BOUNDS = {"f32": 1e-6, "bf16x3": 1e-5, "bf16": 1e-2, "fp16": 2e-3, "fp8": 8e-2}
+
+
+def test_precision_modes_fullsize(cuda):
+    g = G.band_random(1 << 20, 20_000_000, 2, seed=0)
+    X = np.random.default_rng(1).uniform(-1, 1, (g.n, 128)).astype(np.float32)
+    rows = np.arange(0, g.n, 128)
+    ip, ix, v = _row_sample(g.indptr, g.indices, g.values.astype(np.float64), rows)
+    ref = sp.csr_matrix((v, ix, ip), shape=(rows.size, g.n)) @ X.astype(np.float64)
+    Xt = torch.from_numpy(X).to(cuda)
+    rows_d = torch.from_numpy(rows).to(cuda)
+    for mode, bound in BOUNDS.items():
+        A = BlockMatrix(g, g.values, cuda, mode, build="device")
+        y = A.matmul(Xt)
+        err = np.linalg.norm(y[rows_d].double().cpu().numpy() - ref) / np.linalg.norm(ref)
+        assert err < bound, (mode, err)
+        del A, y
