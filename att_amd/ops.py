@@ -171,6 +171,81 @@ _SRCS = {"deg_f16": _lib.SRC_DEG_F16, "deg_f32": _lib.SRC_DEG_F32, "edge_f32": _
          "map": _lib.SRC_MAP}
 
 
+class DevicePlan:
+    """The planning phase of a device build (attgnn_blocks_plan_device) of one
+    CSR: its workspace and the structure sizes read back once.  Every fill of
+    the same (row, col, window, wide, part_size, emit_empty) reuses it -- the
+    fill only reads the plan -- so a graph needing several tile types (values
+    and an edge-id map) pays the read-back once."""
+
+    def __init__(self, row, col, window, wide, part_size, emit_empty):
+        dev = _need_gpu("blocks_plan_device", row, col)
+        _need_i32("blocks_plan_device", row, col)
+        if row.dim() != 1 or col.dim() != 1 or row.numel() < 1:
+            raise ValueError("row and col must be 1-D, row non-empty")
+        self.row, self.col, self.dev = row, col, dev
+        self.window, self.wide, self.part_size, self.emit_empty = window, wide, part_size, bool(emit_empty)
+        rows1, nnz = row.numel(), col.numel()
+        L = lib()
+        ws_n = L.attgnn_blocks_device_workspace_bytes(rows1, nnz, window)
+        if ws_n < 0:
+            raise ValueError("invalid sizes for the device preprocessing")
+        self.ws = torch.empty(max(ws_n, 1), dtype=torch.uint8, device=dev)
+        self.ws_n = ws_n
+        totals = torch.empty(8, dtype=torch.int64, device=dev)
+        check(L.attgnn_blocks_plan_device(_ptr(row), rows1, _ptr(col), nnz, window, wide, part_size,
+                                          int(self.emit_empty), _ptr(self.ws), ws_n, _ptr(totals), _stream(dev)),
+              "attgnn_blocks_plan_device")
+        status, self.n_rp, self.n_col, self.n_val, self.n_parts, self.n_unique = [int(x) for x in totals[:6].cpu()]
+        check(status, "attgnn_blocks_plan_device")
+
+    def fill(self, src, src_values, out_dtype, want_trans=False, compact=False, n_values=None):
+        """One structure of this plan (keys as :func:`blocks_build`).
+        ``n_values``: the compact value count when the caller knows it (an
+        edge-id map over a CSR whose rows all lie in windows holds each edge
+        once: nnz), which saves its read-back."""
+        if src not in _SRCS:
+            raise ValueError(f"unknown src {src!r}")
+        if want_trans and src != "map":
+            raise ValueError("want_trans needs src='map'")
+        dev = self.dev
+        if src == "map":
+            out_dtype, sp, keep = torch.int32, ctypes.c_void_p(0), None
+        else:
+            _need_gpu("blocks_fill_device", src_values)
+            want = {"deg_f16": torch.float16, "deg_f32": torch.float32, "edge_f32": torch.float32}[src]
+            if src_values.dtype != want:
+                raise TypeError(f"{src}: values must be {want}, got {src_values.dtype}")
+            expect = self.row.numel() - 1 if src.startswith("deg") else self.col.numel()
+            if src_values.numel() < expect:
+                raise ValueError(f"{src}: need {expect} values, got {src_values.numel()}")
+            if out_dtype not in (torch.float32, torch.float16, torch.bfloat16):
+                raise TypeError(f"tile type {out_dtype} is not supported")
+            keep = src_values
+            sp = _ptr(keep)
+        out = {
+            "rp": torch.empty(self.n_rp, dtype=torch.int32, device=dev),
+            "col": torch.empty(self.n_col, dtype=torch.int32, device=dev),
+            "val": torch.empty(self.n_val, dtype=out_dtype, device=dev),
+            "window": torch.empty(self.n_parts, dtype=torch.int32, device=dev),
+            "atomic": torch.empty(self.n_parts, dtype=torch.int32, device=dev),
+            "map_t": torch.empty(self.n_val, dtype=torch.int32, device=dev) if want_trans else None,
+            "n_unique": self.n_unique,
+        }
+        check(lib().attgnn_blocks_fill_device(_ptr(self.row), self.row.numel(), _ptr(self.col), self.col.numel(),
+                                              _SRCS[src], sp, _TYPE_OF[out_dtype], self.window, self.wide,
+                                              self.part_size, _ptr(self.ws), self.ws_n, _ptr(out["rp"]),
+                                              _ptr(out["col"]), _ptr(out["val"]), _ptr(out["map_t"]),
+                                              _ptr(out["window"]), _ptr(out["atomic"]), _stream(dev)),
+              "attgnn_blocks_fill_device")
+        if compact:
+            if self.wide != 32 or self.window > 16:
+                raise ValueError("compact tiles need wide = 32 and window <= 16")
+            out.update(compact_tiles_device(out["val"], self.n_col, self.window, n_values=n_values))
+        del keep
+        return out
+
+
 def blocks_build_device(row, col, src, src_values, out_dtype, window, wide, part_size, emit_empty,
                         want_trans=False, compact=False):
     """Window-block structure built on the GPU from a device-resident CSR.
@@ -184,63 +259,17 @@ def blocks_build_device(row, col, src, src_values, out_dtype, window, wide, part
         raise ValueError(f"unknown src {src!r}")
     if want_trans and src != "map":
         raise ValueError("want_trans needs src='map'")
-    dev = _need_gpu("blocks_build_device", row, col, src_values if src != "map" else None)
-    _need_i32("blocks_build_device", row, col)
-    if row.dim() != 1 or col.dim() != 1 or row.numel() < 1:
-        raise ValueError("row and col must be 1-D, row non-empty")
-    if src == "map":
-        out_dtype, sp, keep = torch.int32, ctypes.c_void_p(0), None
-    else:
-        want = {"deg_f16": torch.float16, "deg_f32": torch.float32, "edge_f32": torch.float32}[src]
-        if src_values.dtype != want:
-            raise TypeError(f"{src}: values must be {want}, got {src_values.dtype}")
-        expect = row.numel() - 1 if src.startswith("deg") else col.numel()
-        if src_values.numel() < expect:
-            raise ValueError(f"{src}: need {expect} values, got {src_values.numel()}")
-        if out_dtype not in (torch.float32, torch.float16, torch.bfloat16):
-            raise TypeError(f"tile type {out_dtype} is not supported")
-        keep = src_values
-        sp = _ptr(keep)
-    rows1, nnz = row.numel(), col.numel()
-    L = lib()
-    ws_n = L.attgnn_blocks_device_workspace_bytes(rows1, nnz, window)
-    if ws_n < 0:
-        raise ValueError("invalid sizes for the device preprocessing")
-    ws = torch.empty(max(ws_n, 1), dtype=torch.uint8, device=dev)
-    totals = torch.empty(8, dtype=torch.int64, device=dev)
-    st = _stream(dev)
-    check(L.attgnn_blocks_plan_device(_ptr(row), rows1, _ptr(col), nnz, window, wide, part_size,
-                                      int(bool(emit_empty)), _ptr(ws), ws_n, _ptr(totals), st),
-          "attgnn_blocks_plan_device")
-    status, n_rp, n_col, n_val, n_parts, n_unique = [int(x) for x in totals[:6].cpu()]
-    check(status, "attgnn_blocks_plan_device")
-    out = {
-        "rp": torch.empty(n_rp, dtype=torch.int32, device=dev),
-        "col": torch.empty(n_col, dtype=torch.int32, device=dev),
-        "val": torch.empty(n_val, dtype=out_dtype, device=dev),
-        "window": torch.empty(n_parts, dtype=torch.int32, device=dev),
-        "atomic": torch.empty(n_parts, dtype=torch.int32, device=dev),
-        "map_t": torch.empty(n_val, dtype=torch.int32, device=dev) if want_trans else None,
-        "n_unique": n_unique,
-    }
-    check(L.attgnn_blocks_fill_device(_ptr(row), rows1, _ptr(col), nnz, _SRCS[src], sp, _TYPE_OF[out_dtype],
-                                      window, wide, part_size, _ptr(ws), ws_n, _ptr(out["rp"]), _ptr(out["col"]),
-                                      _ptr(out["val"]), _ptr(out["map_t"]), _ptr(out["window"]),
-                                      _ptr(out["atomic"]), st),
-          "attgnn_blocks_fill_device")
-    if compact:
-        if wide != 32 or window > 16:
-            raise ValueError("compact tiles need wide = 32 and window <= 16")
-        out.update(compact_tiles_device(out["val"], n_col, window, workspace=ws))
-    del keep
-    return out
+    if src != "map":
+        _need_gpu("blocks_build_device", src_values)
+    return DevicePlan(row, col, window, wide, part_size, emit_empty).fill(src, src_values, out_dtype, want_trans,
+                                                                          compact)
 
 
-def compact_tiles_device(val, n_col, window, workspace=None, positions=True):
+def compact_tiles_device(val, n_col, window, workspace=None, positions=True, n_values=None):
     """Compact form of n_col condensed columns of window x 32 tiles on the
     device (value tiles, or an int32 edge-id map): masks, block_base, values
     and (optionally) positions, as attgnn_blocks_compact_export lays them out.
-    Reads the value count back once."""
+    Reads the value count back once, unless the caller gives it (``n_values``)."""
     dev = _need_gpu("compact_tiles_device", val)
     L = lib()
     st = _stream(dev)
@@ -253,7 +282,7 @@ def compact_tiles_device(val, n_col, window, workspace=None, positions=True):
     check(L.attgnn_blocks_compact_count_device(_ptr(val), _TYPE_OF[val.dtype], n_col, window, _ptr(masks),
                                                _ptr(base), _ptr(ws), ws.numel(), st),
           "attgnn_blocks_compact_count_device")
-    nv = int(base[nb].item())
+    nv = int(base[nb].item()) if n_values is None else int(n_values)
     values = torch.empty(max(nv, 1), dtype=val.dtype, device=dev)
     pos = torch.empty(max(nv, 1), dtype=torch.int16, device=dev) if positions else None
     check(L.attgnn_blocks_compact_fill_device(_ptr(val), _TYPE_OF[val.dtype], n_col, window, _ptr(masks), _ptr(base),

@@ -68,12 +68,12 @@ def _dense_parts(b):
     return {k: b[k] for k in ("rp", "col", "val", "window", "atomic")}
 
 
-def _compact_parts(b):
+def _compact_parts(b, n_values=None):
     """The arrays a cached graph keeps of a compact build: the part table, the
     condensed columns, the row masks, the first n_blocks block offsets and the
-    packed values (exactly n_values of them)."""
+    packed values (exactly n_values of them; read back when not given)."""
     nb = b["col"].numel() // WIDE
-    nv = int(b["block_base"][nb])
+    nv = int(b["block_base"][nb]) if n_values is None else int(n_values)
     return {"rp": b["rp"], "col": b["col"], "window": b["window"], "atomic": b["atomic"],
             "masks": b["masks"][: b["col"].numel()], "bbase": b["block_base"][:nb], "values": b["values"][:nv]}
 
@@ -97,26 +97,75 @@ def prepare_host(csr, features, part_size=SERVE_PART_SIZE):
             "X": torch.from_numpy(X).to(torch.bfloat16)}
 
 
-def prepare_device(csr, features, device, stream, part_size=SERVE_PART_SIZE):
-    """Device-side preprocessing of one graph for both models: the padded CSR,
-    its normalised values and the features are uploaded on `stream` and the
-    window blocks are built there (bit-identical to :func:`prepare_host`)."""
+class PinnedRing:
+    """Page-locked staging for host-to-device uploads: one pinned buffer
+    allocated once (pinning per upload costs more than the copy), handed out
+    round-robin in 256-byte aligned pieces.  A piece is reused only after the
+    uploads that read it have finished (an event recorded after them)."""
+
+    def __init__(self, nbytes=64 << 20):
+        self.buf = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
+        self.pos = 0
+        self.fences = collections.deque()      # (start, end, event) of pieces in flight
+
+    def _take(self, n):
+        n = (n + 255) // 256 * 256
+        if n > self.buf.numel():
+            return None
+        if self.pos + n > self.buf.numel():
+            self.pos = 0
+        start, end = self.pos, self.pos + n
+        # wait for the pieces this one overlaps
+        while self.fences and self.fences[0][0] < end and self.fences[0][1] > start:
+            self.fences.popleft()[2].synchronize()
+        self.pos = end
+        return start
+
+    def upload(self, t, device, stream):
+        """Copy the host tensor ``t`` to ``device`` on ``stream`` through the ring
+        (a pinned copy of its own if it exceeds the ring)."""
+        t = t.contiguous()
+        n = t.numel() * t.element_size()
+        start = self._take(n)
+        if start is None:
+            return t.pin_memory().to(device, non_blocking=True)
+        view = self.buf[start:start + n].view(t.dtype).view(t.shape)
+        view.copy_(t)
+        with torch.cuda.stream(stream):
+            out = view.to(device, non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(stream)
+        self.fences.append((start, start + (n + 255) // 256 * 256, ev))
+        return out
+
+
+def prepare_device(csr, features, device, stream, part_size=SERVE_PART_SIZE, ring=None):
+    """Device-side preprocessing of one graph for both models on `stream`: the
+    padded CSR and the normalising degrees are uploaded (through ``ring``, a
+    :class:`PinnedRing`, when given), the edge values d_i d_j are formed on the
+    device (the same float32 products as the host's), both window-block forms
+    are built from one device plan (one read-back of its sizes), and the
+    features are padded on the device.  Bit-identical to :func:`prepare_host`."""
     p = csr.padded(WINDOW)
     d = G.gcn_norm_degree(p)
     d[~np.isfinite(d)] = 0.0
-    vals = np.asarray(G.gcn_norm_values(p, d), np.float32)
-    X = np.zeros((p.n, F_IN), np.float32)
-    X[: csr.n] = features
+    feats = torch.from_numpy(np.asarray(features, np.float32)).to(torch.bfloat16)
+    up = (lambda t: ring.upload(t, device, stream)) if ring is not None else \
+        (lambda t: t.pin_memory().to(device, non_blocking=True))
     with torch.cuda.stream(stream):
-        up = lambda t: t.pin_memory().to(device, non_blocking=True)
         rowptr = up(torch.from_numpy(p.rowptr32()))
         ccol = up(torch.from_numpy(p.indices))
-        gb = ops.blocks_build_device(rowptr, ccol, "edge_f32", up(torch.from_numpy(vals)), torch.bfloat16, WINDOW,
-                                     WIDE, part_size, emit_empty=True)
-        mb = ops.blocks_build_device(rowptr, ccol, "map", None, torch.int32, WINDOW, WIDE, part_size,
-                                     emit_empty=True, compact=True)
-        Xd = up(torch.from_numpy(X).to(torch.bfloat16))
-        gcn, gat = _dense_parts(gb), _compact_parts(mb)
+        dd = up(torch.from_numpy(np.ascontiguousarray(d, np.float32)))
+        rows = torch.repeat_interleave(torch.arange(p.n, device=device), rowptr[1:] - rowptr[:-1],
+                                       output_size=p.nnz)
+        vals = (dd[rows] * dd[ccol.long()]).contiguous()
+        plan = ops.DevicePlan(rowptr, ccol, WINDOW, WIDE, part_size, emit_empty=True)
+        gb = plan.fill("edge_f32", vals, torch.bfloat16)
+        # every row lies in a window, so the edge-id map holds each edge once
+        mb = plan.fill("map", None, torch.int32, compact=True, n_values=p.nnz)
+        Xd = torch.zeros((p.n, F_IN), dtype=torch.bfloat16, device=device)
+        Xd[: csr.n].copy_(up(feats))
+        gcn, gat = _dense_parts(gb), _compact_parts(mb, n_values=p.nnz)
     return {"n": csr.n, "n_pad": p.n, "nnz": csr.nnz, "gcn": gcn, "gat": {"rowptr": rowptr, "ccol": ccol, **gat},
             "X": Xd}
 
@@ -504,6 +553,7 @@ class Server:
         self.cache = GraphCache(hbm_budget)
         self.stream = torch.cuda.Stream(device)        # compute (default priority)
         self.copy_stream = torch.cuda.Stream(device)   # uploads of new graphs
+        self.ring = PinnedRing() if preprocess == "device" else None
         self.buckets = {k: [[Bucket(k, c, device, models, use_graph) for _ in range(2)] for c in capacities]
                         for k in ("gcn", "gat")}
         self.turn = collections.defaultdict(int)
@@ -522,7 +572,7 @@ class Server:
         if not self.cache.make_room(estimate_bytes(csr), self.reserved):
             return None
         if self.preprocess == "device":
-            built = prepare_device(csr, feats, self.device, self.copy_stream, self.part_size)
+            built = prepare_device(csr, feats, self.device, self.copy_stream, self.part_size, self.ring)
         else:
             built = prepare_host(csr, feats, self.part_size)
         cg = CachedGraph(req.key, built, self.device, self.copy_stream)

@@ -242,14 +242,19 @@ def test_csr_transpose_device_matches_stable_host_sort(cuda):
                                      torch.tensor(col, dtype=torch.int32, device=cuda), 2)
 
 
-def test_serving_prepare_device_equals_host(cuda):
+@pytest.mark.parametrize("ring_bytes", [None, 1 << 16, 64 << 20])
+def test_serving_prepare_device_equals_host(cuda, ring_bytes):
+    """Device preparation (values formed on the device, one shared plan) is
+    bitwise the host's; through a pinned staging ring too, including a ring so
+    small that it wraps and hands larger arrays their own pinned copy."""
     from att_amd import serve as S
     pool = S.GraphPool(6, seed=5, n_max=5000)
     stream = torch.cuda.Stream(cuda)
+    ring = S.PinnedRing(ring_bytes) if ring_bytes else None
     for key in range(6):
         csr, feats = pool(key)
         h = S.prepare_host(csr, feats)
-        d = S.prepare_device(csr, feats, cuda, stream)
+        d = S.prepare_device(csr, feats, cuda, stream, ring=ring)
         stream.synchronize()
         assert (h["n"], h["n_pad"], h["nnz"]) == (d["n"], d["n_pad"], d["nnz"])
         for part in ("gcn", "gat"):
