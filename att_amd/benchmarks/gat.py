@@ -71,14 +71,16 @@ def main():
                               "ms": round(ms, 4), "gflops": round(2.0 * g.nnz * dim * heads / ms / 1e6, 1),
                               "algorithmic_gbs": round(nbytes / ms / 1e6, 1),
                               "hbm_frac": round(nbytes / ms / 1e6 / 8000.0, 3)}), flush=True)
-    for kind in ("gat", "gatv2", "gatv2_true"):
+    # gatv2: the single-head reference file's logits; gatv2_minmax: the
+    # multi-head file's per-head min-max logit map; gatv2_true: the paper's GATv2
+    for kind in ("gat", "gatv2", "gatv2_minmax", "gatv2_true"):
         for heads in [int(h) for h in args.heads.split(",")]:
             torch.manual_seed(heads)
             if kind == "gat":
                 layer = GATConv_multi(128, dim, 0.0, 0.2, heads, dtype=torch.bfloat16).to(dev)
             else:
                 layer = GATv2Conv_multi(128, dim, 0.0, 0.2, heads, dtype=torch.bfloat16,
-                                        true_gatv2=(kind == "gatv2_true")).to(dev)
+                                        true_gatv2=(kind == "gatv2_true"), minmax=(kind == "gatv2_minmax")).to(dev)
             go = torch.randn(g.n, heads, dim, device=dev)
             go_b = go.bfloat16()            # the upstream gradient in the layer's output type
 
@@ -95,13 +97,15 @@ def main():
                 def plain():
                     out = gat_fp32(X, W, a0, a1, rows, cols, g.n, heads, dim)
                     out.backward(go)
-            elif kind == "gatv2":
+            elif kind in ("gatv2", "gatv2_minmax"):
                 W0 = layer.fc0.weight.detach().clone().requires_grad_(True)
                 W1 = layer.fc1.weight.detach().clone().requires_grad_(True)
                 a = layer.a0.detach().clone().requires_grad_(True)
+                mm = kind == "gatv2_minmax"
 
                 def plain():
-                    out = gat_fp32(X, W0, a, a, rows, cols, g.n, heads, dim, slope=1.0, Wt=W1, v2_alpha=0.2)
+                    out = gat_fp32(X, W0, a, a, rows, cols, g.n, heads, dim, slope=1.0, Wt=W1, v2_alpha=0.2,
+                                   minmax=mm)
                     out.backward(go)
             else:
                 W0 = layer.fc0.weight.detach().clone().requires_grad_(True)
@@ -129,9 +133,9 @@ def main():
                 out = layer(Xb, graph).float()
                 if kind == "gat":
                     ref = gat_fp32(Xb.float(), W.bfloat16().float(), a0, a1, rows, cols, g.n, heads, dim)
-                elif kind == "gatv2":
+                elif kind in ("gatv2", "gatv2_minmax"):
                     ref = gat_fp32(Xb.float(), W0.bfloat16().float(), a, a, rows, cols, g.n, heads, dim, slope=1.0,
-                                   Wt=W1.bfloat16().float(), v2_alpha=0.2)
+                                   Wt=W1.bfloat16().float(), v2_alpha=0.2, minmax=(kind == "gatv2_minmax"))
                 else:
                     ref = gatv2_true_fp32(Xb.float(), W0.bfloat16().float(), W1.bfloat16().float(), a, rows, cols,
                                           g.n, heads, dim)

@@ -8,11 +8,16 @@ device in its tile type and runs Y = A_hat X with FP32 output:
   bf16x3       FP32 tiles and X, each split into bf16 hi + lo (3 MFMA terms)
   f32          FP32 tiles and X, FP32 MFMA (16x16x4)
 
-Reported per mode: time per SpMM (HIP events), TFLOP/s (2 nnz D), algorithmic
-bytes 4(M+1) + nnz (4 + s_A) + U D s_X + M D 4 (U = distinct columns summed
-over 16-row windows) per second as a fraction of the 8 TB/s HBM peak, and the
-relative Frobenius error against an FP64 product of the unrounded FP32
-A_hat and X on a sample of 2,000 rows (scipy)."""
+Reported per mode: time per SpMM (HIP events), TFLOP/s (2 nnz D); the
+compulsory bytes per second (X read once, the compact structure as read --
+per part its rp pair, window and flag, per condensed column a 4-byte id and a
+2-byte row mask, per 32-column block an 8-byte offset, per entry its value --
+and Y written once) as a fraction of the 8 TB/s HBM peak; beside it the
+per-window gather bytes 4(M+1) + nnz (4 + s_A) + U D s_X + M D 4 (U = distinct
+columns summed over 16-row windows) per second, which the L2s and the
+Infinity Cache mostly serve; and the relative Frobenius error against an
+FP64 product of the unrounded FP32 A_hat and X on a sample of 2,000 rows
+(scipy)."""
 import argparse
 import json
 import sys
@@ -62,7 +67,8 @@ def main():
         tile_dt, x_dt, kmode = MODES[name]
         graph = GCNGraph(g, dev, window=16, wide=32, tile_dtype=tile_dt, symmetric=True, values=vals,
                          compact=True, build="device")
-        rp, col, _, win, at = graph.fwd["arrays"]
+        _, col, _, _, _ = graph.fwd["arrays"]
+        rp, win, at = graph.fwd["tasks"]
         masks, base, values = graph.fwd["compact"]
         Xq = X.to(x_dt).to(dev)
 
@@ -81,12 +87,16 @@ def main():
         ms = e0.elapsed_time(e1) / args.reps
         s_a = torch.empty(0, dtype=tile_dt).element_size()
         s_x = Xq.element_size()
-        nbytes = 4 * (g.n + 1) + g.nnz * (4 + s_a) + graph.n_unique * DIM * s_x + g.n * DIM * 4
+        P, C = win.numel(), col.numel()
+        compulsory = (g.n * DIM * s_x + 4 * (2 * P + 1) + 8 * P + C * (4 + 2) + (C // 32) * 8 + g.nnz * s_a
+                      + g.n * DIM * 4)
+        gather = 4 * (g.n + 1) + g.nnz * (4 + s_a) + graph.n_unique * DIM * s_x + g.n * DIM * 4
         err = float(np.linalg.norm(y[rows_d].double().cpu().numpy() - ref) / nref)
         print(json.dumps({"mode": name, "nodes": g.n, "nnz": g.nnz, "dim": DIM, "U16": graph.n_unique,
                           "ms": round(ms, 4), "tflops": round(2.0 * g.nnz * DIM / ms / 1e9, 2),
-                          "algorithmic_gbs": round(nbytes / ms / 1e6, 1),
-                          "hbm_frac": round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 3),
+                          "compulsory_gbs": round(compulsory / ms / 1e6, 1),
+                          "hbm_frac": round(compulsory / ms / 1e6 / HBM_PEAK_GBS, 4),
+                          "gather_gbs": round(gather / ms / 1e6, 1),
                           "rel_fro_vs_fp64": err}), flush=True)
         del graph, masks, base, values, rp, col, win, at, Xq, y
         torch.cuda.empty_cache()

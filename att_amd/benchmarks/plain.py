@@ -16,8 +16,10 @@ def csr_tensor(graph_csr, values, device):
                                    torch.from_numpy(np.asarray(values, np.float32)), (graph_csr.n, graph_csr.n)).to(device)
 
 
-def gat_fp32(X, W, a0, a1, rows, cols, n, heads, dim, slope=0.2, Wt=None, v2_alpha=None):
-    """One attention layer in FP32: returns [N, H, D]."""
+def gat_fp32(X, W, a0, a1, rows, cols, n, heads, dim, slope=0.2, Wt=None, v2_alpha=None, minmax=False):
+    """One attention layer in FP32: returns [N, H, D].  ``minmax``: each head's
+    logits mapped to (z - min) / (max - min) over all its edges before the
+    softmax (the reference's multi-head GATv2)."""
     Xs = (X @ W.t()).view(n, heads, dim)
     Xt = Xs if Wt is None else (X @ Wt.t()).view(n, heads, dim)
     if v2_alpha is not None:
@@ -27,6 +29,9 @@ def gat_fp32(X, W, a0, a1, rows, cols, n, heads, dim, slope=0.2, Wt=None, v2_alp
     s1 = (Xt * a1.view(1, heads, dim)).sum(-1)
     z = s0[rows] + s1[cols]
     e = F.leaky_relu(z, slope) if slope != 1.0 else z
+    if minmax:
+        lo, hi = e.min(0).values, e.max(0).values
+        e = (e - lo) / (hi - lo)
     idx = rows.unsqueeze(1).expand(-1, heads)
     m = torch.full((n, heads), float("-inf"), device=X.device).scatter_reduce(0, idx, e, "amax", include_self=True)
     ex = torch.exp(e - m[rows])

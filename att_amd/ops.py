@@ -837,11 +837,12 @@ def edge_score_range(rowptr, col, s0, s1, s_layout="nh"):
     if n > n_nodes:
         raise ValueError(f"{n} CSR rows but node terms for {n_nodes} nodes")
     hs, rs = (1, heads) if s_layout == "nh" else (n_nodes, 1)
-    keys = torch.empty(2 * heads, dtype=torch.int64, device=dev)
+    ws_n = lib().attgnn_edge_score_range_workspace_elems(heads)
+    ws = torch.empty(ws_n, dtype=torch.int64, device=dev)
     rng = torch.empty((heads, 3), dtype=torch.float32, device=dev)
     where = torch.empty((heads, 4), dtype=torch.int32, device=dev)
     check(lib().attgnn_edge_score_range(_ptr(rowptr), _ptr(col), n, col.numel(), heads, _ptr(s0), _ptr(s1), hs, rs,
-                                        _ptr(keys), _ptr(rng), _ptr(where), _stream(dev)),
+                                        _ptr(ws), ws_n, _ptr(rng), _ptr(where), _stream(dev)),
           "attgnn_edge_score_range")
     return rng, where
 

@@ -93,8 +93,9 @@ def cpu_baseline(g, vals, dim=256, stride=1, warmups=3, runs=10):
     tmed = float(np.median(times))
     gflops = 2.0 * e * dim / tmed / 1e9
     return {"value": round(gflops, 3), "unit": "GFLOP/s", "cores": cores, "kind": "port",
-            "sample": f"torch.sparse.mm CSR fp32 (int32 indices) on one row in {stride} of A_hat "
-                      f"({rows.size} rows, {e} nnz) x X[{g.n},{dim}], {warmups} warm-ups, median of {runs} "
+            "sample": f"torch.sparse.mm CSR fp32 (int32 indices) on "
+                      + ("all of A_hat" if stride == 1 else f"one row in {stride} of A_hat")
+                      + f" ({rows.size} rows, {e} nnz) x X[{g.n},{dim}], {warmups} warm-ups, median of {runs} "
                       f"({sum(times):.1f} s timed)"}
 
 

@@ -729,9 +729,10 @@ int attgnn_gather_rowsum(const int32_t *rowptr, const int64_t *perm, int64_t n, 
  * CSR's nnz edges (s0 / s1 strided as above): range[3h..3h+2] = (min, max,
  * 1/(max - min)), the last 0 when max == min or there are no edges; where[4h..
  * 4h+3] = (row, column) of the first edge in CSR order attaining the minimum,
- * then of the one attaining the maximum (-1 without edges).  keys: a device
- * workspace of 2*heads 64-bit words, overwritten.  Deterministic (integer
- * min / max reductions).  heads <= 256, nnz < 2^31.
+ * then of the one attaining the maximum (-1 without edges).  workspace: a
+ * device buffer of attgnn_edge_score_range_workspace_elems(heads) 64-bit words
+ * (per-workgroup partial minima and maxima, overwritten).  Deterministic
+ * (integer min / max reductions, no atomics).  heads <= 256, nnz < 2^31.
  *
  * Replaces the per-head torch.max / torch.min over the logits and the
  * division by their difference of the multi-head GATv2 layer
@@ -739,9 +740,12 @@ int attgnn_gather_rowsum(const int32_t *rowptr, const int64_t *perm, int64_t n, 
  *   s0 and s1 by range[3h+2] before attgnn_edge_softmax_fwd (the softmax is
  *   shift-invariant, so subtracting the minimum changes nothing).
  */
+#define ATTGNN_RANGE_BLOCKS 1024
+int64_t attgnn_edge_score_range_workspace_elems(int64_t heads);
 int attgnn_edge_score_range(const int32_t *rowptr, const int32_t *col, int64_t n, int64_t nnz, int64_t heads,
                             const float *s0, const float *s1, int64_t s_head_stride, int64_t s_row_stride,
-                            unsigned long long *keys, float *range, int32_t *where, attgnn_stream_t stream);
+                            unsigned long long *workspace, int64_t workspace_elems, float *range, int32_t *where,
+                            attgnn_stream_t stream);
 
 /* ====================================================================== */
 /* Block-diagonal packing for batched inference (device)                   */
