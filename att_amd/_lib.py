@@ -93,7 +93,11 @@ _SIGNATURES = {
                                             _vp, _vp, _vp, _i64, _vp]),
     "attgnn_gather_rowsum": (_int, [_vp, _vp, _i64, _i64, _i64, _vp, _vp, _vp, _i64, _vp]),
     "attgnn_edge_score_range_workspace_elems": (_i64, [_i64]),
-    "attgnn_edge_score_range": (_int, [_vp, _vp, _i64, _i64, _i64, _vp, _vp, _i64, _i64, _vp, _i64, _vp, _vp, _vp]),
+    "attgnn_minmax_grads_workspace_elems": (_i64, [_i64]),
+    "attgnn_minmax_score_grads": (_int, [_vp, _vp, _i64, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64,
+                                         _vp]),
+    "attgnn_edge_score_range": (_int, [_vp, _vp, _i64, _i64, _i64, _vp, _vp, _i64, _i64, _vp, _i64, _vp, _vp, _vp,
+                                       _i64, _vp]),
     "attgnn_blocks_device_workspace_bytes": (_i64, [_i64, _i64, _int]),
     "attgnn_blocks_plan_device": (_int, [_vp, _i64, _vp, _i64, _int, _int, _int, _int, _vp, _i64, _vp, _vp]),
     "attgnn_blocks_fill_device": (_int, [_vp, _i64, _vp, _i64, _int, _vp, _int, _int, _int, _int, _vp, _i64, _vp,

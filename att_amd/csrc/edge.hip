@@ -832,7 +832,9 @@ struct RangeArgs {
     unsigned long long *part;   // [heads][ATTGNN_RANGE_BLOCKS][2]: each block's min key, max key
     float *range;               // [heads][3]: min, max, 1/(max - min) (0 when max == min or no edges)
     int32_t *where;             // [heads][4]: row, column of the minimum; row, column of the maximum (-1: none)
-    int heads, blocks;
+    const int32_t *long_rows;   // rows above long_min entries, taken by the first long_blocks workgroups
+    int64_t n_long, long_min;
+    int heads, blocks, long_blocks;
 };
 
 __device__ __forceinline__ unsigned long long umin64(unsigned long long x, unsigned long long y) { return x < y ? x : y; }
@@ -845,52 +847,84 @@ __device__ __forceinline__ unsigned long long shfl_xor64(unsigned long long v, i
     return (static_cast<unsigned long long>(hi) << 32) | lo;
 }
 
-// grid (blocks, ceil(heads / 8)); 16-lane groups walk rows (grid-stride), a
-// lane takes every 16th edge of its group's row for up to 8 heads (the
-// column id read once); each block writes its partial keys per head.
+// grid (blocks, ceil(heads / 8)), up to 8 heads per grid row (a column id
+// read once for all of them); each block writes its partial keys per head.
 __global__ __launch_bounds__(256) void range_kernel(RangeArgs a)
 {
     const int h0 = blockIdx.y * RANGE_HB;
     const int nh = min(RANGE_HB, a.heads - h0);
-    const int lane16 = threadIdx.x & 15;
-    const int64_t group = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 4;
-    const int64_t groups = (int64_t(gridDim.x) * blockDim.x) >> 4;
     unsigned long long kmin[RANGE_HB], kmax[RANGE_HB];
 #pragma unroll
     for (int k = 0; k < RANGE_HB; ++k) {
         kmin[k] = ~0ull;
         kmax[k] = 0ull;
     }
-    for (int64_t i = group; i < a.n; i += groups) {
+    // the keys of row i's edges eb, eb + stride, ... below e1, four per lane
+    // per iteration with their column ids and column terms loaded before any
+    // is used
+    auto visit = [&](int64_t i, int32_t eb, int32_t e1, int32_t stride) {
         float r[RANGE_HB];
 #pragma unroll
         for (int k = 0; k < RANGE_HB; ++k) r[k] = k < nh ? a.s0[(h0 + k) * a.s_hs + i * a.s_rs] : 0.f;
-        const int32_t e0 = a.rowptr[i], e1 = a.rowptr[i + 1];
-        for (int32_t e = e0 + lane16; e < e1; e += 16) {
-            const int64_t c = a.col[e];
+        for (; eb < e1; eb += 4 * stride) {
+            int64_t c[4];
 #pragma unroll
-            for (int k = 0; k < RANGE_HB; ++k) {
-                if (k < nh) {
-                    const float z = r[k] + a.s1[(h0 + k) * a.s_hs + c * a.s_rs];
-                    const unsigned long long hi = static_cast<unsigned long long>(order_bits(z)) << 32;
-                    kmin[k] = umin64(kmin[k], hi | uint32_t(e));
-                    kmax[k] = umax64(kmax[k], hi | (0xffffffffu - uint32_t(e)));
+            for (int u = 0; u < 4; ++u) c[u] = eb + stride * u < e1 ? int64_t(a.col[eb + stride * u]) : -1;
+            float t[4][RANGE_HB];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int k = 0; k < RANGE_HB; ++k)
+                    t[u][k] = (k < nh && c[u] >= 0) ? a.s1[(h0 + k) * a.s_hs + c[u] * a.s_rs] : 0.f;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (c[u] < 0) continue;
+                const uint32_t e = uint32_t(eb + stride * u);
+#pragma unroll
+                for (int k = 0; k < RANGE_HB; ++k) {
+                    if (k < nh) {
+                        const float z = r[k] + t[u][k];
+                        const unsigned long long hi = static_cast<unsigned long long>(order_bits(z)) << 32;
+                        kmin[k] = umin64(kmin[k], hi | e);
+                        kmax[k] = umax64(kmax[k], hi | (0xffffffffu - e));
+                    }
                 }
             }
+        }
+    };
+    if (int(blockIdx.x) < a.long_blocks) {
+        // the listed long rows: the whole workgroup walks each row (a hub row
+        // of over a thousand entries left to one 16-lane group kept it busy
+        // for dozens of dependent load rounds, longer than the rest together)
+        for (int64_t li = blockIdx.x; li < a.n_long; li += a.long_blocks) {
+            const int64_t i = a.long_rows[li];
+            visit(i, a.rowptr[i] + int32_t(threadIdx.x), a.rowptr[i + 1], 256);
+        }
+    } else {
+        // 16-lane groups walk the other rows (grid-stride), a lane taking
+        // every 16th edge of its group's row
+        const int64_t group = (int64_t(blockIdx.x - a.long_blocks) * blockDim.x + threadIdx.x) >> 4;
+        const int64_t groups = (int64_t(gridDim.x - a.long_blocks) * blockDim.x) >> 4;
+        for (int64_t i = group; i < a.n; i += groups) {
+            const int32_t e0 = a.rowptr[i], e1 = a.rowptr[i + 1];
+            if (e1 - e0 > a.long_min) continue;   // a long-row workgroup takes it
+            visit(i, e0 + int32_t(threadIdx.x & 15), e1, 16);
         }
     }
     __shared__ unsigned long long red[4][RANGE_HB][2];
     const int wave = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < RANGE_HB; ++k) {
+        if (k < nh) {   // wave-uniform: only the launch row's heads are reduced
 #pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) {
-            kmin[k] = umin64(kmin[k], shfl_xor64(kmin[k], o));
-            kmax[k] = umax64(kmax[k], shfl_xor64(kmax[k], o));
-        }
-        if ((threadIdx.x & 63) == 0) {
-            red[wave][k][0] = kmin[k];
-            red[wave][k][1] = kmax[k];
+            for (int o = 32; o >= 1; o >>= 1) {
+                kmin[k] = umin64(kmin[k], shfl_xor64(kmin[k], o));
+                kmax[k] = umax64(kmax[k], shfl_xor64(kmax[k], o));
+            }
+            if ((threadIdx.x & 63) == 0) {
+                red[wave][k][0] = kmin[k];
+                red[wave][k][1] = kmax[k];
+            }
         }
     }
     __syncthreads();
@@ -976,9 +1010,11 @@ extern "C" int64_t attgnn_edge_score_range_workspace_elems(int64_t heads)
 extern "C" int attgnn_edge_score_range(const int32_t *rowptr, const int32_t *col, int64_t n, int64_t nnz,
                                        int64_t heads, const float *s0, const float *s1, int64_t s_head_stride,
                                        int64_t s_row_stride, unsigned long long *workspace, int64_t workspace_elems,
-                                       float *range, int32_t *where, attgnn_stream_t stream)
+                                       float *range, int32_t *where, const int32_t *long_rows, int64_t n_long,
+                                       attgnn_stream_t stream)
 {
     if (n < 0 || nnz < 0 || nnz > INT32_MAX || heads <= 0 || heads > 256) return ATTGNN_E_ARG;
+    if (!long_args_ok(long_rows, n_long)) return ATTGNN_E_ARG;
     if (!range || !where) return ATTGNN_E_ARG;
     if (nnz > 0 && (n == 0 || !rowptr || !col || !s0 || !s1)) return ATTGNN_E_ARG;
     if (nnz > 0 && (!workspace || workspace_elems < attgnn_edge_score_range_workspace_elems(heads)))
@@ -986,14 +1022,116 @@ extern "C" int attgnn_edge_score_range(const int32_t *rowptr, const int32_t *col
     RangeArgs a{};
     a.rowptr = rowptr; a.col = col; a.s0 = s0; a.s1 = s1; a.n = n; a.s_hs = s_head_stride; a.s_rs = s_row_stride;
     a.part = workspace; a.range = range; a.where = where; a.heads = int(heads);
+    a.long_rows = long_rows; a.n_long = n_long; a.long_min = long_min_of(long_rows, n_long);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    // about 10 rows per 16-lane group on the attention graphs; every block
-    // writes its partial slot, so the finish reads only written slots
-    a.blocks = nnz > 0 ? int(std::min<int64_t>(std::max<int64_t>((n + 15) / 16, 1), ATTGNN_RANGE_BLOCKS)) : 0;
+    // up to 1,024 workgroups for the long rows, then a few rows per 16-lane
+    // group for the others (enough groups in flight for each row's dependent
+    // loads); every block writes its partial slot, so the finish reads only
+    // written slots
+    a.long_blocks = nnz > 0 ? int(std::min<int64_t>(n_long, 1024)) : 0;
+    a.blocks = nnz > 0 ? a.long_blocks + int(std::min<int64_t>(std::max<int64_t>((n + 63) / 64, 1),
+                                                                ATTGNN_RANGE_BLOCKS - 1024))
+                       : 0;
     if (a.blocks > 0)
         hipLaunchKernelGGL(range_kernel, dim3(unsigned(a.blocks), unsigned((heads + RANGE_HB - 1) / RANGE_HB)),
                            dim3(256), 0, s, a);
     hipLaunchKernelGGL(range_finish_kernel, dim3(unsigned(heads)), dim3(256), 0, s, a);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? ATTGNN_OK : int(e);
+}
+
+// ---------------------------------------------------------------------------
+// Gradient through the min-max logit map (see attgnn_minmax_score_grads in the
+// header): U_h partial sums per workgroup, a fixed-order finish per head, then
+// one elementwise pass that also applies the four point updates.
+namespace {
+
+constexpr int MG_BLOCKS = 256;
+
+struct MinmaxArgs {
+    const float *s0, *s1, *range, *ds0s, *ds1s;
+    const int32_t *where;
+    float *ds0, *ds1, *part;   // part: [heads][MG_BLOCKS] partial U, then [heads] k at the end
+    int64_t n, s_hs, s_rs;
+    int heads;
+};
+
+// block sum in a fixed tree (wave shuffles, then the 4 waves in order)
+__device__ __forceinline__ float block_sum_fixed(float v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    __shared__ float red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+__global__ __launch_bounds__(256) void minmax_u_kernel(MinmaxArgs a)
+{
+    const int h = blockIdx.y;
+    float u = 0.f;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < a.n; i += int64_t(gridDim.x) * 256) {
+        const int64_t si = h * a.s_hs + i * a.s_rs, di = int64_t(h) * a.n + i;
+        u += a.s0[si] * a.ds0s[di] + a.s1[si] * a.ds1s[di];
+    }
+    u = block_sum_fixed(u);
+    if (threadIdx.x == 0) a.part[int64_t(h) * MG_BLOCKS + blockIdx.x] = u;
+}
+
+__global__ __launch_bounds__(256) void minmax_k_kernel(MinmaxArgs a)
+{
+    const int h = blockIdx.x;
+    float u = a.part[int64_t(h) * MG_BLOCKS + threadIdx.x];
+    u = block_sum_fixed(u);
+    if (threadIdx.x == 0) {
+        const float c = a.range[3 * h + 2];
+        a.part[int64_t(a.heads) * MG_BLOCKS + h] = u * c * c;
+    }
+}
+
+__global__ __launch_bounds__(256) void minmax_apply_kernel(MinmaxArgs a)
+{
+    const int h = blockIdx.y;
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= a.n) return;
+    const float c = a.range[3 * h + 2], k = a.part[int64_t(a.heads) * MG_BLOCKS + h];
+    const int32_t *w = a.where + 4 * h;
+    const int64_t di = int64_t(h) * a.n + i;
+    float v0 = a.ds0s[di] * c, v1 = a.ds1s[di] * c;
+    if (i == w[2]) v0 -= k;   // row of the maximum's edge
+    if (i == w[3]) v1 -= k;   // its column
+    if (i == w[0]) v0 += k;   // row of the minimum's edge
+    if (i == w[1]) v1 += k;   // its column
+    a.ds0[di] = v0;
+    a.ds1[di] = v1;
+}
+
+}  // namespace
+
+extern "C" int64_t attgnn_minmax_grads_workspace_elems(int64_t heads)
+{
+    return heads > 0 ? heads * (MG_BLOCKS + 1) : 0;
+}
+
+extern "C" int attgnn_minmax_score_grads(const float *s0, const float *s1, int64_t n, int64_t heads,
+                                         int64_t s_head_stride, int64_t s_row_stride, const float *range,
+                                         const int32_t *where, const float *ds0s, const float *ds1s, float *ds0,
+                                         float *ds1, float *workspace, int64_t workspace_elems,
+                                         attgnn_stream_t stream)
+{
+    if (n < 0 || heads <= 0 || heads > 65535) return ATTGNN_E_ARG;
+    if (n == 0) return ATTGNN_OK;
+    if (!s0 || !s1 || !range || !where || !ds0s || !ds1s || !ds0 || !ds1 || !workspace) return ATTGNN_E_ARG;
+    if (workspace_elems < attgnn_minmax_grads_workspace_elems(heads)) return ATTGNN_E_ARG;
+    MinmaxArgs a{};
+    a.s0 = s0; a.s1 = s1; a.range = range; a.where = where; a.ds0s = ds0s; a.ds1s = ds1s;
+    a.ds0 = ds0; a.ds1 = ds1; a.part = workspace;
+    a.n = n; a.s_hs = s_head_stride; a.s_rs = s_row_stride; a.heads = int(heads);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(minmax_u_kernel, dim3(MG_BLOCKS, unsigned(heads)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(minmax_k_kernel, dim3(unsigned(heads)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(minmax_apply_kernel, dim3(unsigned((n + 255) / 256), unsigned(heads)), dim3(256), 0, s, a);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? ATTGNN_OK : int(e);
 }

@@ -444,9 +444,10 @@ class GATAttention(torch.autograd.Function):
             s1 = ops.node_scores(Xt, a1.to(Xt.dtype).contiguous(), "mhd", out_layout="nh")
         raw = None
         if minmax:
-            rng, where = ops.edge_score_range(g.rowptr, g.col, s0, s1, s_layout="nh")
-            scale = rng[:, 2].contiguous()
-            raw = (s0, s1, scale, where)
+            rng, where = ops.edge_score_range(g.rowptr, g.col, s0, s1, s_layout="nh",
+                                              long_rows=g.long_row_lists()[0])
+            raw = (s0, s1, rng, where)
+            scale = rng[:, 2]
             s0, s1 = s0 * scale, s1 * scale
         long_rows, _ = g.long_row_lists()
         rp, col, emap, win, at = g.fwd
@@ -493,7 +494,8 @@ class GATAttention(torch.autograd.Function):
                                              g.t_rowptr, g.t_perm, s_layout="nh", long_rows=long_rows,
                                              t_long_rows=t_long_rows)
         if ctx.minmax:
-            ds0, ds1 = _minmax_score_grads(*ctx.saved_tensors[8:], ds0, ds1)
+            s0r, s1r, rng, where = ctx.saved_tensors[8:]
+            ds0, ds1 = ops.minmax_score_grads(s0r, s1r, rng, where, ds0, ds1, s_layout="nh")
         # s0 = Xs.a0, s1 = Xt.a1: node-term gradients fused with the aggregation
         # gradient (one pass over X; GAT's shared X W gets the summed gradient)
         shared = ctx.shared and Xs.data_ptr() == Xt.data_ptr()
@@ -501,30 +503,6 @@ class GATAttention(torch.autograd.Function):
                                                     base=dXt, combine=shared)
         return (dXs, dXt, da0.to(a0.dtype).reshape(a0.shape), da1.to(a1.dtype).reshape(a1.shape),
                 None, None, None, None, None, None)
-
-
-def _minmax_score_grads(s0, s1, scale, where, ds0s, ds1s):
-    """Gradients of the raw node terms under the per-head temperature c_h =
-    1 / (max_h - min_h), from those of the scaled terms s0 c, s1 c (ds0s /
-    ds1s, fp32 [H, N]).  With U_h = sum_i s0[i,h] ds0s[h,i] + s1[i,h] ds1s[h,i]
-    = dL/dc_h and dc/dmax = -c^2, dc/dmin = c^2:  ds0 = c ds0s, ds1 = c ds1s,
-    then -U c^2 at the row and column of the maximum's edge and +U c^2 at
-    those of the minimum's (each update one element per head, applied in a
-    fixed order, so the result is deterministic).  c = 0 (all logits equal,
-    or no edges: then the edge indices are -1 and the updates add zero to the
-    last node) leaves zero gradients."""
-    c = scale.view(-1, 1)
-    U = (s0.t() * ds0s).sum(1) + (s1.t() * ds1s).sum(1)          # [H]
-    k = U * scale * scale
-    ds0 = ds0s * c
-    ds1 = ds1s * c
-    w = where.to(torch.int64)
-    hs = torch.arange(scale.numel(), device=scale.device)
-    ds0[hs, w[:, 2]] -= k
-    ds1[hs, w[:, 3]] -= k
-    ds0[hs, w[:, 0]] += k
-    ds1[hs, w[:, 1]] += k
-    return ds0, ds1
 
 
 class GATv2Attention(torch.autograd.Function):

@@ -822,13 +822,15 @@ def dtype_code(dtype):
 # fused edge softmax (CSR rows)
 # --------------------------------------------------------------------------
 
-def edge_score_range(rowptr, col, s0, s1, s_layout="nh"):
+def edge_score_range(rowptr, col, s0, s1, s_layout="nh", long_rows=None):
     """Per-head range of the logits s0[i] + s1[j] over the CSR's edges:
     (range fp32 [H, 3] = min, max, 1/(max - min) or 0; where int32 [H, 4] =
     row, column of the minimum's edge, then of the maximum's; -1 without
-    edges).  s0 / s1 fp32 node terms, [N, H] ("nh") or [H, N] ("hn")."""
-    dev = _need_gpu("edge_score_range", rowptr, col, s0, s1)
-    _need_i32("edge_score_range", rowptr, col)
+    edges).  s0 / s1 fp32 node terms, [N, H] ("nh") or [H, N] ("hn");
+    long_rows: the graph's long-row list (``long_rows(rowptr)``), whose rows
+    then get a workgroup each."""
+    dev = _need_gpu("edge_score_range", rowptr, col, s0, s1, long_rows)
+    _need_i32("edge_score_range", rowptr, col, *([] if long_rows is None else [long_rows]))
     if s0.shape != s1.shape or s0.dtype != torch.float32 or s1.dtype != torch.float32 or s0.dim() != 2:
         raise ValueError("s0 and s1 must be fp32 [N, H] / [H, N] tensors of one shape")
     heads = s0.shape[1] if s_layout == "nh" else s0.shape[0]
@@ -842,9 +844,36 @@ def edge_score_range(rowptr, col, s0, s1, s_layout="nh"):
     rng = torch.empty((heads, 3), dtype=torch.float32, device=dev)
     where = torch.empty((heads, 4), dtype=torch.int32, device=dev)
     check(lib().attgnn_edge_score_range(_ptr(rowptr), _ptr(col), n, col.numel(), heads, _ptr(s0), _ptr(s1), hs, rs,
-                                        _ptr(ws), ws_n, _ptr(rng), _ptr(where), _stream(dev)),
+                                        _ptr(ws), ws_n, _ptr(rng), _ptr(where), _ptr(long_rows),
+                                        0 if long_rows is None else long_rows.numel(), _stream(dev)),
           "attgnn_edge_score_range")
     return rng, where
+
+
+def minmax_score_grads(s0, s1, rng, where, ds0s, ds1s, s_layout="nh"):
+    """Gradients of the raw node terms s0 / s1 (fp32, [N, H] "nh" or [H, N]
+    "hn") under the min-max logit map, from ds0s / ds1s (fp32 [H, N]) of the
+    scaled terms; rng / where from edge_score_range.  Returns (ds0, ds1)
+    fp32 [H, N]."""
+    dev = _need_gpu("minmax_score_grads", s0, s1, rng, where, ds0s, ds1s)
+    heads, n = ds0s.shape
+    if ds1s.shape != ds0s.shape or ds0s.dtype != torch.float32 or ds1s.dtype != torch.float32:
+        raise ValueError("ds0s / ds1s must be fp32 [H, N] of one shape")
+    want = (n, heads) if s_layout == "nh" else (heads, n)
+    if tuple(s0.shape) != want or tuple(s1.shape) != want or s0.dtype != torch.float32 or s1.dtype != torch.float32:
+        raise ValueError(f"s0 / s1 must be fp32 {want}")
+    if tuple(rng.shape) != (heads, 3) or tuple(where.shape) != (heads, 4) or where.dtype != torch.int32:
+        raise ValueError("rng [H, 3] fp32 and where [H, 4] int32 expected")
+    s0, s1, rng, where = s0.contiguous(), s1.contiguous(), rng.contiguous(), where.contiguous()
+    ds0s, ds1s = ds0s.contiguous(), ds1s.contiguous()
+    hs, rs = (1, heads) if s_layout == "nh" else (n, 1)
+    ds0, ds1 = torch.empty_like(ds0s), torch.empty_like(ds1s)
+    ws_n = lib().attgnn_minmax_grads_workspace_elems(heads)
+    ws = torch.empty(ws_n, dtype=torch.float32, device=dev)
+    check(lib().attgnn_minmax_score_grads(_ptr(s0), _ptr(s1), n, heads, hs, rs, _ptr(rng), _ptr(where), _ptr(ds0s),
+                                          _ptr(ds1s), _ptr(ds0), _ptr(ds1), _ptr(ws), ws_n, _stream(dev)),
+          "attgnn_minmax_score_grads")
+    return ds0, ds1
 
 
 def long_rows(rowptr):

@@ -731,8 +731,10 @@ int attgnn_gather_rowsum(const int32_t *rowptr, const int64_t *perm, int64_t n, 
  * 4h+3] = (row, column) of the first edge in CSR order attaining the minimum,
  * then of the one attaining the maximum (-1 without edges).  workspace: a
  * device buffer of attgnn_edge_score_range_workspace_elems(heads) 64-bit words
- * (per-workgroup partial minima and maxima, overwritten).  Deterministic
- * (integer min / max reductions, no atomics).  heads <= 256, nnz < 2^31.
+ * (per-workgroup partial minima and maxima, overwritten).  long_rows /
+ * n_long: the list of attgnn_long_rows (NULL / 0 = none), whose rows then get
+ * a workgroup each, as in the softmax.  Deterministic (integer min / max
+ * reductions, no atomics).  heads <= 256, nnz < 2^31.
  *
  * Replaces the per-head torch.max / torch.min over the logits and the
  * division by their difference of the multi-head GATv2 layer
@@ -740,12 +742,33 @@ int attgnn_gather_rowsum(const int32_t *rowptr, const int64_t *perm, int64_t n, 
  *   s0 and s1 by range[3h+2] before attgnn_edge_softmax_fwd (the softmax is
  *   shift-invariant, so subtracting the minimum changes nothing).
  */
-#define ATTGNN_RANGE_BLOCKS 1024
+#define ATTGNN_RANGE_BLOCKS 8192
 int64_t attgnn_edge_score_range_workspace_elems(int64_t heads);
 int attgnn_edge_score_range(const int32_t *rowptr, const int32_t *col, int64_t n, int64_t nnz, int64_t heads,
                             const float *s0, const float *s1, int64_t s_head_stride, int64_t s_row_stride,
                             unsigned long long *workspace, int64_t workspace_elems, float *range, int32_t *where,
-                            attgnn_stream_t stream);
+                            const int32_t *long_rows, int64_t n_long, attgnn_stream_t stream);
+
+/*
+ * Gradients of the raw node terms of the min-max logit map from those of the
+ * scaled terms.  With c_h = range[3h+2] (attgnn_edge_score_range) the layer
+ * feeds s0 c and s1 c to the softmax; given ds0s / ds1s = dL/d(s0 c), dL/d(s1 c)
+ * (fp32, [heads][n] contiguous) this writes ds0 = c ds0s, ds1 = c ds1s, then,
+ * with U_h = sum_i s0[h,i] ds0s[h,i] + s1[h,i] ds1s[h,i] and k_h = U_h c_h^2,
+ * subtracts k_h at the row (ds0) and column (ds1) of the maximum's edge and
+ * adds k_h at those of the minimum's (where[4h..4h+3]; in that order, so an
+ * element hit twice gets (v - k) + k).  s0 / s1 strided like the scores of
+ * attgnn_edge_score_range.  workspace: attgnn_minmax_grads_workspace_elems(
+ * heads) floats.  Deterministic (fixed-order sums, no atomics).
+ *
+ * Replaces the autograd of (z - min) / (max - min) through torch.min / max of
+ * the multi-head GATv2 layer (end2end_head/gatv2_final/mgat_csr/mgat_conv.py:184-186).
+ */
+int64_t attgnn_minmax_grads_workspace_elems(int64_t heads);
+int attgnn_minmax_score_grads(const float *s0, const float *s1, int64_t n, int64_t heads, int64_t s_head_stride,
+                              int64_t s_row_stride, const float *range, const int32_t *where, const float *ds0s,
+                              const float *ds1s, float *ds0, float *ds1, float *workspace,
+                              int64_t workspace_elems, attgnn_stream_t stream);
 
 /* ====================================================================== */
 /* Block-diagonal packing for batched inference (device)                   */

@@ -256,6 +256,92 @@ def test_edge_score_range_vs_torch(cuda):
     assert torch.all(where.cpu() == -1) and torch.all(rng[:, 2].cpu() == 0)
 
 
+def test_edge_score_range_long_rows(cuda):
+    """With the long-row list (rows above 256 entries taken by whole
+    workgroups) the range is bitwise the one without it and torch's, on a
+    graph with hub rows of 300-5,000 entries."""
+    rng_np = np.random.default_rng(7)
+    n = 6000
+    deg = rng_np.integers(0, 12, n)
+    deg[[3, 100, 4096, 5999]] = [5000, 300, 1200, 257]
+    ip = np.concatenate([[0], np.cumsum(deg)])
+    cols = np.concatenate([np.sort(rng_np.choice(n, d, replace=False)) for d in deg]).astype(np.int32)
+    g = G.CSR(ip, cols, n)
+    rowptr = torch.from_numpy(g.rowptr32()).to(cuda)
+    col = torch.from_numpy(g.indices).to(cuda)
+    gen = torch.Generator().manual_seed(8)
+    heads = 5
+    s0 = torch.randn(n, heads, generator=gen).to(cuda)
+    s1 = torch.randn(n, heads, generator=gen).to(cuda)
+    s0[100, 2] = 50.0       # head 2's maximum lies in a long row
+    s0[3, 4] = -50.0        # head 4's minimum lies in the longest row
+    lr = ops.long_rows(rowptr)
+    assert sorted(lr.cpu().tolist()) == [3, 100, 4096, 5999]
+    a = ops.edge_score_range(rowptr, col, s0, s1)
+    b = ops.edge_score_range(rowptr, col, s0, s1, long_rows=lr)
+    assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
+    rows = torch.from_numpy(g.row_ids().astype(np.int64)).to(cuda)
+    z = s0[rows] + s1[col.long()]
+    assert torch.equal(b[0][:, 0], z.min(0).values) and torch.equal(b[0][:, 1], z.max(0).values)
+    assert b[1][2, 2].item() == 100 and b[1][4, 0].item() == 3
+
+
+def _minmax_score_grads_torch(s0, s1, scale, where, ds0s, ds1s):
+    """Torch statement of the gradient through the per-head temperature c_h =
+    1 / (max_h - min_h) (s0 / s1 fp32 [N, H], ds0s / ds1s of the scaled terms
+    fp32 [H, N]): with U_h = sum_i s0[i,h] ds0s[h,i] + s1[i,h] ds1s[h,i] =
+    dL/dc_h, dc/dmax = -c^2 and dc/dmin = c^2, ds0 = c ds0s, ds1 = c ds1s, then
+    -U c^2 at the row and column of the maximum's edge and +U c^2 at those of
+    the minimum's, in that order."""
+    c = scale.view(-1, 1)
+    U = (s0.t() * ds0s).sum(1) + (s1.t() * ds1s).sum(1)          # [H]
+    k = U * scale * scale
+    ds0 = ds0s * c
+    ds1 = ds1s * c
+    for h in range(scale.numel()):
+        r_min, c_min, r_max, c_max = (int(v) for v in where[h].tolist())
+        if r_max >= 0:
+            ds0[h, r_max] -= k[h]
+            ds1[h, c_max] -= k[h]
+            ds0[h, r_min] += k[h]
+            ds1[h, c_min] += k[h]
+    return ds0, ds1
+
+
+@pytest.mark.parametrize("layout", ["nh", "hn"])
+def test_minmax_score_grads_vs_torch(cuda, layout):
+    """attgnn_minmax_score_grads against the torch statement: equal to ~1e-6
+    (U is summed in another order) and bitwise away from the extreme edges'
+    rows and columns; a head whose minimum and maximum share a row and a
+    column, a head with zero temperature and no edges, and run-to-run
+    equality."""
+    n, heads = 5000, 3
+    gen = torch.Generator().manual_seed(11)
+    s0 = torch.randn(n, heads, generator=gen).to(cuda)
+    s1 = torch.randn(n, heads, generator=gen).to(cuda)
+    ds0s = torch.randn(heads, n, generator=gen).to(cuda)
+    ds1s = torch.randn(heads, n, generator=gen).to(cuda)
+    rng = torch.tensor([[-2.0, 3.0, 0.2], [-1.0, 1.5, 0.4], [0.0, 0.0, 0.0]], device=cuda)
+    where = torch.tensor([[17, 4001, 17, 4001], [3, 9, 2500, 77], [-1, -1, -1, -1]], dtype=torch.int32,
+                         device=cuda)
+    a0, a1 = (s0, s1) if layout == "nh" else (s0.t().contiguous(), s1.t().contiguous())
+    ds0, ds1 = ops.minmax_score_grads(a0, a1, rng, where, ds0s, ds1s, s_layout=layout)
+    again = ops.minmax_score_grads(a0, a1, rng, where, ds0s, ds1s, s_layout=layout)
+    assert torch.equal(ds0, again[0]) and torch.equal(ds1, again[1])
+    r0, r1 = _minmax_score_grads_torch(s0, s1, rng[:, 2], where, ds0s, ds1s)
+    assert torch.allclose(ds0, r0, rtol=1e-5, atol=1e-6) and torch.allclose(ds1, r1, rtol=1e-5, atol=1e-6)
+    keep0 = torch.ones(heads, n, dtype=torch.bool, device=cuda)
+    keep1 = keep0.clone()
+    for h, (rmin, cmin, rmax, cmax) in enumerate(where.tolist()):
+        if rmin >= 0:
+            keep0[h, [rmin, rmax]] = False
+            keep1[h, [cmin, cmax]] = False
+    assert torch.equal(ds0[keep0], r0[keep0]) and torch.equal(ds1[keep1], r1[keep1])
+    assert torch.all(ds0[2] == 0) and torch.all(ds1[2] == 0)
+    # head 0: min and max at one row / column -> (c ds - k) + k there
+    assert abs(float(ds0[0, 17]) - float(ds0s[0, 17] * 0.2)) < 1e-4
+
+
 def test_gcn_fp16_two_layers_vs_fp64(cuda):
     """The 2-layer GCN of the benchmark in its storage type -- FP16 tiles,
     features and weights, FP32 accumulation; 128 hidden features (compact
