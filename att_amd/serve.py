@@ -103,21 +103,29 @@ class PinnedRing:
     round-robin in 256-byte aligned pieces.  A piece is reused only after the
     uploads that read it have finished (an event recorded after them)."""
 
-    def __init__(self, nbytes=64 << 20):
-        self.buf = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
+    def __init__(self, nbytes=64 << 20, pin=True):
+        self.buf = torch.empty(nbytes, dtype=torch.uint8, pin_memory=pin)
         self.pos = 0
-        self.fences = collections.deque()      # (start, end, event) of pieces in flight
+        self.fences = []      # (start, end, event) of pieces whose uploads may be in flight
 
     def _take(self, n):
+        """Offset of a free piece of n bytes (rounded up to 256), or None if
+        the ring is smaller: waits for every earlier piece it overlaps (after
+        a wrap those can be of more than one earlier pass over the ring) and
+        drops the fences of finished uploads."""
         n = (n + 255) // 256 * 256
         if n > self.buf.numel():
             return None
         if self.pos + n > self.buf.numel():
             self.pos = 0
         start, end = self.pos, self.pos + n
-        # wait for the pieces this one overlaps
-        while self.fences and self.fences[0][0] < end and self.fences[0][1] > start:
-            self.fences.popleft()[2].synchronize()
+        keep = []
+        for f in self.fences:
+            if f[0] < end and f[1] > start:
+                f[2].synchronize()
+            elif not f[2].query():
+                keep.append(f)
+        self.fences = keep
         self.pos = end
         return start
 

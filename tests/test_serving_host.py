@@ -36,6 +36,49 @@ def test_cache_lru_budget_and_pins():
     assert 0 in c.entries and 1 not in c.entries
 
 
+class FakeEvent:
+    """Stands for a CUDA event: in flight until synchronised."""
+    def __init__(self, log, name):
+        self.log, self.name, self.done = log, name, False
+
+    def synchronize(self):
+        self.log.append(self.name)
+        self.done = True
+
+    def query(self):
+        return self.done
+
+
+def test_pinned_ring_waits_for_every_overlapping_piece():
+    """The staging ring hands out pieces round-robin; a new piece must wait
+    for every in-flight piece it overlaps, also when pieces of two earlier
+    passes over the ring are still in flight (a wrap that left an old piece
+    near the end while a newer one sits at the start)."""
+    ring = S.PinnedRing(nbytes=100 * 256, pin=False)
+    log = []
+
+    def take(name, units):
+        start = ring._take(units * 256)
+        ring.fences.append((start, start + units * 256, FakeEvent(log, name)))
+        return start // 256
+
+    # first pass: ten pieces of 10 units, 0..95 (the last of 5)
+    for k in range(9):
+        assert take(f"a{k}", 10) == 10 * k
+    assert take("z", 5) == 90
+    # wrap: a piece of 30 units at 0 waits for a0, a1, a2
+    assert take("c", 30) == 0
+    assert log == ["a0", "a1", "a2"]
+    # a piece of 80 units does not fit after 30: wrap again to 0; it overlaps
+    # c (second pass) and a3..a7 (first pass), not z at 90..95
+    log.clear()
+    assert take("d", 80) == 0
+    assert sorted(log) == sorted(["c", "a3", "a4", "a5", "a6", "a7"])
+    assert "z" not in log
+    # pieces larger than the ring are refused
+    assert ring._take(101 * 256) is None
+
+
 def test_requests_and_stats():
     r = S.make_requests(2000, 500.0, 64, seed=3)
     arr = np.array([x.arrival for x in r])
