@@ -70,8 +70,10 @@ def check_part_spans(rp, wide, name="structure"):
     read at the wrong slots (and, for a span that is not a multiple, past its
     end).  One device-to-host copy of ``rp`` per structure: the result is kept
     on the tensor with its version counter and reused until ``rp`` is written
-    in place.  Skipped while a stream is being captured into a graph (no host
-    copy is possible there; captures replay structures checked when eager)."""
+    in place by a torch op (a raw-pointer refill does not move the counter, so
+    such buffers are checked once).  Skipped while a stream is being captured
+    into a graph (no host copy is possible there; captures replay structures
+    checked when eager)."""
     if rp.numel() < 3:
         return
     try:
@@ -201,9 +203,9 @@ class DevicePlan:
 
     def fill(self, src, src_values, out_dtype, want_trans=False, compact=False, n_values=None):
         """One structure of this plan (keys as :func:`blocks_build`).
-        ``n_values``: the compact value count when the caller knows it (an
-        edge-id map over a CSR whose rows all lie in windows holds each edge
-        once: nnz), which saves its read-back."""
+        ``n_values``: an upper bound of the compact value count when the
+        caller knows one (an edge-id map holds each CSR entry at most once:
+        nnz), which saves its read-back; see :func:`compact_tiles_device`."""
         if src not in _SRCS:
             raise ValueError(f"unknown src {src!r}")
         if want_trans and src != "map":
@@ -269,7 +271,10 @@ def compact_tiles_device(val, n_col, window, workspace=None, positions=True, n_v
     """Compact form of n_col condensed columns of window x 32 tiles on the
     device (value tiles, or an int32 edge-id map): masks, block_base, values
     and (optionally) positions, as attgnn_blocks_compact_export lays them out.
-    Reads the value count back once, unless the caller gives it (``n_values``)."""
+    Reads the value count back once, unless the caller gives it: ``n_values``
+    must then be at least the count (the fill writes that many values; an
+    upper bound such as the CSR's nnz for an edge-id map only leaves unused
+    entries at the end)."""
     dev = _need_gpu("compact_tiles_device", val)
     L = lib()
     st = _stream(dev)
@@ -361,8 +366,11 @@ def canonical_tasks(rp, window_of_part, n_cols, wide, chunk_blocks=None):
     (attgnn_blocks_tasks_device), in the structure's part format with the slots
     past the total marked window -1.  An SpMM over these tasks groups every
     window's sum by the chunk size alone, so its result is bitwise the same
-    whichever partSize the structure was built with.  Kept on ``rp`` (with its
-    version counter) and reused until ``rp`` is written in place."""
+    whichever partSize the structure was built with.  Kept on ``rp`` (with the
+    version counters of ``rp`` and ``window_of_part``) and reused until either
+    is written in place by a torch op; a buffer refilled through a raw pointer
+    (another extension, a kernel of this library) does not move the counter
+    and must be passed as a new tensor or with ``rp._attgnn_tasks`` cleared."""
     dev = _need_gpu("canonical_tasks", rp, window_of_part)
     _need_i32("canonical_tasks", rp, window_of_part)
     parts = window_of_part.numel()
@@ -370,7 +378,7 @@ def canonical_tasks(rp, window_of_part, n_cols, wide, chunk_blocks=None):
         raise ValueError("rp and window_of_part sizes disagree")
     chunk = int(chunk_blocks or canonical_chunk(n_cols, wide))
     try:
-        version = rp._version
+        version = (rp._version, window_of_part._version)
     except RuntimeError:
         version = None
     tag = (version, rp.numel(), window_of_part.data_ptr(), int(n_cols), int(wide), chunk)
