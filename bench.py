@@ -326,6 +326,14 @@ def main():
                      symmetric=True, values=np.concatenate([vals, np.zeros(0, np.float32)]), build=args.build)
     log(f"window blocks ({args.build} build): U={graph.n_unique} parts={graph.fwd['parts']} "
         f"cols={graph.fwd['n_cols']} part_size={graph.part_size} ({time.perf_counter() - t:.1f}s)")
+    # the workload against the committed manifest (generator output, U16, condensed columns, CSR digest)
+    from att_amd.benchmarks import manifest as MF
+    workload_id = {"csr_sha256_16": MF.csr_digest(g), "U16": int(graph.n_unique),
+                   "condensed_cols_w32": int(graph.fwd["n_cols"])}
+    want = MF.load()["cfg2_gcn_114M"]
+    workload_id["matches_manifest"] = bool(args.scale == 1.0 and g.nnz == want["nnz"] and all(
+        workload_id[k] == want[k] for k in ("csr_sha256_16", "U16", "condensed_cols_w32")))
+    log(f"workload: {workload_id}")
 
     rng = torch.Generator().manual_seed(1)
     if args.mode == "kernels":
@@ -471,7 +479,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "2-layer GCN fwd+bwd (Adam), Chung-Lu gamma=2.1 seed 0",
                    "nodes": g.n, "nnz": g.nnz, "features": [F_IN, F_HID, F_OUT], "window": 16, "wide": 32,
-                   "tiles": "fp16", "accumulate": "fp32", "scale": args.scale},
+                   "tiles": "fp16", "accumulate": "fp32", "scale": args.scale, **workload_id},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "window-block SpMM, width 256, fp16, compact tiles (main + split-window combine)",

@@ -42,6 +42,16 @@ def gcn(cuda):
     return g, vals, graph
 
 
+def test_gcn_fullsize_matches_manifest(cuda, gcn):
+    """The graph is the committed manifest's configuration 2: CSR digest,
+    entries, and the device build's U16 and padded condensed columns."""
+    from att_amd.benchmarks import manifest as M
+    g, _, graph = gcn
+    want = M.load()["cfg2_gcn_114M"]
+    assert M.csr_digest(g) == want["csr_sha256_16"] and g.nnz == want["nnz"]
+    assert graph.n_unique == want["U16"] and graph.fwd["n_cols"] == want["condensed_cols_w32"]
+
+
 def test_gcn_fullsize_spmm(cuda, gcn):
     g, vals, graph = gcn
     assert abs(g.nnz - 114_600_000) < 0.005 * 114_600_000
