@@ -32,6 +32,7 @@ def main():
     ap.add_argument("--part-size", type=int, default=S.SERVE_PART_SIZE, help="blocks per part of cached graphs")
     ap.add_argument("--preprocess", choices=["host", "device"], default="device",
                     help="where cold-cache graphs are preprocessed")
+    ap.add_argument("--compute-streams", type=int, default=2, help="compute streams the batches alternate over")
     ap.add_argument("--saturated-only", action="store_true",
                     help="warm the cache, then one saturated batched run (for kernel traces)")
     args = ap.parse_args()
@@ -48,7 +49,8 @@ def main():
     # device (the server keeps the latter's cache)
     cold = {}
     for where in ((args.preprocess,) if args.saturated_only else ("host", "device")):
-        srv = S.Server(dev, models, pool, use_graph=not args.no_graph, part_size=args.part_size, preprocess=where)
+        srv = S.Server(dev, models, pool, use_graph=not args.no_graph, part_size=args.part_size, preprocess=where,
+                       compute_streams=args.compute_streams)
         warm = [S.Request(i, 0.0, "gcn" if i % 2 else "gat", i) for i in range(args.pool)]
         torch.cuda.synchronize()
         t = time.perf_counter()

@@ -552,14 +552,21 @@ class Request:
 class Server:
     def __init__(self, device, models, graph_source, hbm_budget=200 << 30,
                  capacities=(1 << 12, 1 << 14, 1 << 16, 1 << 18), use_graph=True, keep_results=False,
-                 max_in_flight=2, part_size=SERVE_PART_SIZE, preprocess="device"):
+                 max_in_flight=2, part_size=SERVE_PART_SIZE, preprocess="device", compute_streams=2):
         if preprocess not in ("host", "device"):
             raise ValueError(f"preprocess must be 'host' or 'device', got {preprocess!r}")
         self.device, self.models, self.source = device, models, graph_source
         self.part_size = part_size
         self.preprocess = preprocess
         self.cache = GraphCache(hbm_budget)
-        self.stream = torch.cuda.Stream(device)        # compute (default priority)
+        # compute streams (default priority): consecutive batches alternate
+        # between them, so a batch can start while the previous one's last
+        # kernels still occupy part of the GPU; a bucket's buffers are reused
+        # only after the batch that last used them has finished (Bucket.launch)
+        if compute_streams < 1:
+            raise ValueError("compute_streams must be at least 1")
+        self.streams = [torch.cuda.Stream(device) for _ in range(int(compute_streams))]
+        self.stream = self.streams[0]
         self.copy_stream = torch.cuda.Stream(device)   # uploads of new graphs
         self.ring = PinnedRing() if preprocess == "device" else None
         self.buckets = {k: [[Bucket(k, c, device, models, use_graph) for _ in range(2)] for c in capacities]
@@ -607,7 +614,8 @@ class Server:
             while pending and pending[0].arrival <= now:
                 r = pending.popleft()
                 queue[r.kind].append(r)
-            # retire finished batches (in order: one compute stream)
+            # retire finished batches in launch order (a later batch on the
+            # other stream may finish first; it waits for its turn here)
             while in_flight and in_flight[0][0].query():
                 ev, batch, graphs, out, offs = in_flight.popleft()
                 t = time.perf_counter() - t0
@@ -649,7 +657,8 @@ class Server:
                     t1 = time.perf_counter()
                     self.timing["form"] += t1 - (now + t0)
                     b = self._pick_bucket(kind, graphs)
-                    offs, ev = b.launch(graphs, self.stream, self.timing)
+                    stream = self.streams[self.batches % len(self.streams)]
+                    offs, ev = b.launch(graphs, stream, self.timing)
                     self.timing["launch"] += time.perf_counter() - t1
                     in_flight.append((ev, batch, graphs, b.out, offs))
                     self.batches += 1
