@@ -161,7 +161,12 @@ enum AMode { A_DENSE = 0, A_MAPPED = 1, A_COMPACT = 2, A_ATTN = 3, A_CMAP = 4, A
 // the block's 16 x 32 edge ids (-1: no edge), rows of 32 ints.
 template <int T> constexpr int dot_pitch() { return 32 * T + 16; }
 template <int T> constexpr int dot_rows_bytes() { return 32 * dot_pitch<T>() + 48; }
-template <int T> constexpr int dot_wave_bytes() { return dot_rows_bytes<T>() + 16 * 32 * 4; }
+// the id tile's rows are 36 words apart: row n's 16-byte writes by the lanes
+// of one 8-column group, and the column reads of one MFMA output row, then
+// fall in distinct banks (a 32-word pitch put every row's writes in the same
+// 4 banks, an 8-way conflict)
+constexpr int ID_PITCH = 36;
+template <int T> constexpr int dot_wave_bytes() { return dot_rows_bytes<T>() + 16 * ID_PITCH * 4; }
 
 __device__ __forceinline__ void wave_sync()
 {
@@ -185,7 +190,7 @@ struct KBlock {
     int64_t f0;
     bool f_ok, a_row_ok;
     float r1v, r2v;   // attention variant: the lane's window-row terms
-    int *dot_ids = nullptr;  // A_CMAP_DOT: the wave's 16 x 32 edge-id tile in LDS
+    int *dot_ids = nullptr;  // A_CMAP_DOT: the wave's 16 x 32 edge-id tile in LDS (row pitch ID_PITCH)
 
     __device__ __forceinline__ void cols(int kb, int (&cid)[8]) const
     {
@@ -269,7 +274,7 @@ struct KBlock {
                 }
             }
             if constexpr (AMODE == A_CMAP_DOT) {
-                u32x4 *d = reinterpret_cast<u32x4 *>(dot_ids + n * 32 + 8 * g);
+                u32x4 *d = reinterpret_cast<u32x4 *>(dot_ids + n * ID_PITCH + 8 * g);
                 d[0] = u32x4{uint32_t(id[0]), uint32_t(id[1]), uint32_t(id[2]), uint32_t(id[3])};
                 d[1] = u32x4{uint32_t(id[4]), uint32_t(id[5]), uint32_t(id[6]), uint32_t(id[7])};
             }
@@ -322,7 +327,7 @@ struct KBlock {
         const int prefix = (g > 0 ? t0 : 0) + (g > 1 ? t1 : 0) + (g > 2 ? t2 : 0);
         if (kg >= nreal) {
             if constexpr (AMODE == A_CMAP_DOT) {   // no entries: the id tile row stays empty
-                u32x4 *d = reinterpret_cast<u32x4 *>(dot_ids + n * 32 + 8 * g);
+                u32x4 *d = reinterpret_cast<u32x4 *>(dot_ids + n * ID_PITCH + 8 * g);
                 d[0] = u32x4{~0u, ~0u, ~0u, ~0u};
                 d[1] = u32x4{~0u, ~0u, ~0u, ~0u};
             }
@@ -362,7 +367,7 @@ struct KBlock {
         const int prefix = (g > 0 ? t0 : 0) + (g > 1 ? t1 : 0) + (g > 2 ? t2 : 0);
         if (kg >= nreal) {
             if constexpr (AMODE == A_CMAP_DOT) {   // no entries: the id tile row stays empty
-                u32x4 *d = reinterpret_cast<u32x4 *>(dot_ids + n * 32 + 8 * g);
+                u32x4 *d = reinterpret_cast<u32x4 *>(dot_ids + n * ID_PITCH + 8 * g);
                 d[0] = u32x4{~0u, ~0u, ~0u, ~0u};
                 d[1] = u32x4{~0u, ~0u, ~0u, ~0u};
             }
@@ -481,7 +486,7 @@ struct KBlock {
             for (int t = 0; t < 2; ++t)
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const int e = dot_ids[(4 * g + q) * 32 + 16 * t + n];
+                    const int e = dot_ids[(4 * g + q) * ID_PITCH + 16 * t + n];
                     if (e >= 0) out[e] = sacc[t][q];
                 }
             wave_sync();   // id tile and gathered rows read before the next block writes them
