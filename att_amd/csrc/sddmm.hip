@@ -211,10 +211,17 @@ __global__ __launch_bounds__(256) void sddmm_block_kernel(SddmmArgs a)
     }
 }
 
+// The staged 16 x 32 product tile has rows 36 floats apart: the MFMA output
+// writes of the four 16-lane groups (rows 4g+q, one column per lane) then fall
+// in distinct banks (a 32-float pitch put every group on the same 16 banks).
+// Tile position pos = row * 32 + column (the edge list's) sits at s_at(pos).
+constexpr int S_PITCH = 36;
+__device__ __forceinline__ int s_at(int pos) { return (pos >> 5) * S_PITCH + (pos & 31); }
+
 // SDDMM over compact edge lists: one wave per part, every head in the same
 // wave.  Per 32-column block the wave computes the 16x32 tile with MFMA (all
 // rows of the window against the block's 32 gathered columns), stages it in a
-// wave-private 2 KB LDS tile and writes only the block's real edges, listed as
+// wave-private 2.25 KB LDS tile and writes only the block's real edges, listed as
 // (packed position r*32+c, edge id) -- the dense edge-id map is never read.
 // Column ids and the edge list are loaded once per block for all heads.
 template <int KIND, bool VEC>
@@ -222,7 +229,7 @@ __global__ __launch_bounds__(256) void sddmm_edges_kernel(SddmmArgs a, const int
                                                           const uint16_t *epos)
 {
     constexpr int EB = KIND >= 2 ? 4 : 2;
-    __shared__ float tile[4][16 * 32];
+    __shared__ float tile[4][16 * S_PITCH];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform part
     const int64_t part = int64_t(blockIdx.x) * 4 + wave;
@@ -260,14 +267,14 @@ __global__ __launch_bounds__(256) void sddmm_edges_kernel(SddmmArgs a, const int
 #pragma unroll
             for (int t = 0; t < 2; ++t)
 #pragma unroll
-                for (int q = 0; q < 4; ++q) S[(4 * g + q) * 32 + 16 * t + n] = acc[t][q];
+                for (int q = 0; q < 4; ++q) S[(4 * g + q) * S_PITCH + 16 * t + n] = acc[t][q];
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            if (my_e >= 0) store_elt(a.out + (int64_t(h) * a.nnz + my_e) * ob, a.out_type, S[my_pos]);
+            if (my_e >= 0) store_elt(a.out + (int64_t(h) * a.nnz + my_e) * ob, a.out_type, S[s_at(my_pos)]);
             for (int j = 64 + lane; j < cnt; j += 64) {
                 const int e = eids[e0 + j];
-                store_elt(a.out + (int64_t(h) * a.nnz + e) * ob, a.out_type, S[epos[e0 + j]]);
+                store_elt(a.out + (int64_t(h) * a.nnz + e) * ob, a.out_type, S[s_at(epos[e0 + j])]);
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -288,7 +295,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void s
                                                                   const int32_t *eids, const uint16_t *epos)
 {
     constexpr int HPB = PAIRS / NDC;   // heads per batch
-    __shared__ float tile[4][16 * 32];
+    __shared__ float tile[4][16 * S_PITCH];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform part
     const int64_t part = int64_t(blockIdx.x) * 4 + wave;
@@ -377,14 +384,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void s
 #pragma unroll
                     for (int t = 0; t < 2; ++t)
 #pragma unroll
-                        for (int q = 0; q < 4; ++q) S[(4 * g + q) * 32 + 16 * t + n] = acc[hl][t][q];
+                        for (int q = 0; q < 4; ++q) S[(4 * g + q) * S_PITCH + 16 * t + n] = acc[hl][t][q];
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                    if (eid >= 0) store_elt(a.out + (h * a.nnz + eid) * ob, a.out_type, S[pos]);
+                    if (eid >= 0) store_elt(a.out + (h * a.nnz + eid) * ob, a.out_type, S[s_at(pos)]);
                     for (int j = 64 + lane; j < cnt; j += 64) {
                         const int e = eids[e0 + j];
-                        store_elt(a.out + (h * a.nnz + e) * ob, a.out_type, S[epos[e0 + j]]);
+                        store_elt(a.out + (h * a.nnz + e) * ob, a.out_type, S[s_at(epos[e0 + j])]);
                     }
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
