@@ -58,3 +58,34 @@ def test_fused_dot_rejects_bad_arguments_without_device():
                                                   0, dim, _lib.LAYOUT_HMD, 16, nul, dot_rows, nul, nul, 0, 0, nul)
     assert call(64, -1) == -1
     assert call(48, 0) == -4
+
+
+def test_range_and_minmax_reject_bad_arguments_without_device():
+    """attgnn_edge_score_range / attgnn_minmax_score_grads: sizes, missing
+    buffers, a workspace below the stated size and a long-row count without
+    its list are refused before any launch."""
+    L = _lib.lib()
+    nul = ctypes.c_void_p(0)
+    fake = ctypes.c_void_p(0x1000)    # never dereferenced: validation fails first
+    ws = L.attgnn_edge_score_range_workspace_elems(4)
+    assert ws == 4 * 2 * 8192 and L.attgnn_edge_score_range_workspace_elems(0) == 0
+    # heads out of range
+    assert L.attgnn_edge_score_range(fake, fake, 10, 20, 0, fake, fake, 1, 4, fake, ws, fake, fake, nul, 0, nul) == -1
+    assert L.attgnn_edge_score_range(fake, fake, 10, 20, 257, fake, fake, 1, 4, fake, ws, fake, fake, nul, 0, nul) == -1
+    # no range / where outputs
+    assert L.attgnn_edge_score_range(fake, fake, 10, 20, 4, fake, fake, 1, 4, fake, ws, nul, fake, nul, 0, nul) == -1
+    # workspace too small
+    assert L.attgnn_edge_score_range(fake, fake, 10, 20, 4, fake, fake, 1, 4, fake, ws - 1, fake, fake, nul, 0,
+                                     nul) == -1
+    # long-row count without a list
+    assert L.attgnn_edge_score_range(fake, fake, 10, 20, 4, fake, fake, 1, 4, fake, ws, fake, fake, nul, 3, nul) == -1
+    mg = L.attgnn_minmax_grads_workspace_elems(4)
+    assert mg == 4 * (256 + 1)
+    assert L.attgnn_minmax_score_grads(fake, fake, 10, 0, 1, 4, fake, fake, fake, fake, fake, fake, fake, mg,
+                                       nul) == -1
+    assert L.attgnn_minmax_score_grads(fake, fake, 10, 4, 1, 4, fake, fake, fake, fake, fake, fake, fake, mg - 1,
+                                       nul) == -1
+    assert L.attgnn_minmax_score_grads(fake, fake, 10, 4, 1, 4, fake, nul, fake, fake, fake, fake, fake, mg,
+                                       nul) == -1
+    # nothing to do for zero nodes: ok without touching the buffers
+    assert L.attgnn_minmax_score_grads(nul, nul, 0, 4, 1, 4, nul, nul, nul, nul, nul, nul, nul, 0, nul) == 0
