@@ -129,6 +129,29 @@ class PinnedRing:
         self.pos = end
         return start
 
+    def upload_many(self, tensors, device, stream):
+        """Copy several host tensors to ``device`` on ``stream`` with one copy
+        through the ring: they are staged at 256-byte aligned offsets of one
+        piece and come back as views of one device buffer (each its own
+        pinned copy if the piece exceeds the ring)."""
+        tensors = [t.contiguous() for t in tensors]
+        sizes = [t.numel() * t.element_size() for t in tensors]
+        offs, total = [], 0
+        for n in sizes:
+            offs.append(total)
+            total += (n + 255) // 256 * 256
+        start = self._take(total) if total else None
+        if start is None:
+            return [self.upload(t, device, stream) for t in tensors]
+        for t, o, n in zip(tensors, offs, sizes):
+            self.buf[start + o:start + o + n].copy_(t.view(-1).view(torch.uint8))
+        with torch.cuda.stream(stream):
+            dev_buf = self.buf[start:start + total].to(device, non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(stream)
+        self.fences.append((start, start + total, ev))
+        return [dev_buf[o:o + n].view(t.dtype).view(t.shape) for t, o, n in zip(tensors, offs, sizes)]
+
     def upload(self, t, device, stream):
         """Copy the host tensor ``t`` to ``device`` on ``stream`` through the ring
         (a pinned copy of its own if it exceeds the ring)."""
@@ -149,30 +172,28 @@ class PinnedRing:
 
 def prepare_device(csr, features, device, stream, part_size=SERVE_PART_SIZE, ring=None):
     """Device-side preprocessing of one graph for both models on `stream`: the
-    padded CSR and the normalising degrees are uploaded (through ``ring``, a
-    :class:`PinnedRing`, when given), the edge values d_i d_j are formed on the
-    device (the same float32 products as the host's), both window-block forms
-    are built from one device plan (one read-back of its sizes), and the
-    features are padded on the device.  Bit-identical to :func:`prepare_host`."""
+    padded CSR, the normalising degrees and the padded BF16 features go up in
+    one copy (through ``ring``, a :class:`PinnedRing`, when given), the tile
+    values d_i d_j are formed by the fill from the degrees (the same float32
+    products as the host's), and both window-block forms are built from one
+    device plan (one read-back of its sizes).  Bit-identical to
+    :func:`prepare_host`."""
     p = csr.padded(WINDOW)
     d = G.gcn_norm_degree(p)
     d[~np.isfinite(d)] = 0.0
-    feats = torch.from_numpy(np.asarray(features, np.float32)).to(torch.bfloat16)
-    up = (lambda t: ring.upload(t, device, stream)) if ring is not None else \
-        (lambda t: t.pin_memory().to(device, non_blocking=True))
+    Xh = torch.zeros((p.n, F_IN), dtype=torch.bfloat16)
+    Xh[: csr.n] = torch.from_numpy(np.asarray(features, np.float32))
+    host = [torch.from_numpy(p.rowptr32()), torch.from_numpy(p.indices),
+            torch.from_numpy(np.ascontiguousarray(d, np.float32)), Xh]
     with torch.cuda.stream(stream):
-        rowptr = up(torch.from_numpy(p.rowptr32()))
-        ccol = up(torch.from_numpy(p.indices))
-        dd = up(torch.from_numpy(np.ascontiguousarray(d, np.float32)))
-        rows = torch.repeat_interleave(torch.arange(p.n, device=device), rowptr[1:] - rowptr[:-1],
-                                       output_size=p.nnz)
-        vals = (dd[rows] * dd[ccol.long()]).contiguous()
+        if ring is not None:
+            rowptr, ccol, dd, Xd = ring.upload_many(host, device, stream)
+        else:
+            rowptr, ccol, dd, Xd = [t.pin_memory().to(device, non_blocking=True) for t in host]
         plan = ops.DevicePlan(rowptr, ccol, WINDOW, WIDE, part_size, emit_empty=True)
-        gb = plan.fill("edge_f32", vals, torch.bfloat16)
-        # every row lies in a window, so the edge-id map holds each edge once
+        gb = plan.fill("deg_f32", dd, torch.bfloat16)
+        # every row lies in a window, so the edge-id map holds each edge at most once
         mb = plan.fill("map", None, torch.int32, compact=True, n_values=p.nnz)
-        Xd = torch.zeros((p.n, F_IN), dtype=torch.bfloat16, device=device)
-        Xd[: csr.n].copy_(up(feats))
         gcn, gat = _dense_parts(gb), _compact_parts(mb, n_values=p.nnz)
     return {"n": csr.n, "n_pad": p.n, "nnz": csr.nnz, "gcn": gcn, "gat": {"rowptr": rowptr, "ccol": ccol, **gat},
             "X": Xd}
