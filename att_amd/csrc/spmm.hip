@@ -192,11 +192,18 @@ struct KBlock {
     float r1v, r2v;   // attention variant: the lane's window-row terms
     int *dot_ids = nullptr;  // A_CMAP_DOT: the wave's 16 x 32 edge-id tile in LDS (row pitch ID_PITCH)
 
+    // Column ids: read once per k-block, so nontemporal -- except for the
+    // compact 8-features-per-lane instance (width 256, two slices), where the
+    // cached load measured 1.3% faster (4.757 -> 4.695 ms on the 114.6M-entry
+    // graph) while at 4 features per lane it was 3.5% slower.
     __device__ __forceinline__ void cols(int kb, int (&cid)[8]) const
     {
         const int kg = kb + 8 * g;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) cid[i] = kg + i < nreal ? __builtin_nontemporal_load(a.col + c_begin + kg + i) : -1;
+        for (int i = 0; i < 8; ++i) {
+            if constexpr (AMODE == A_COMPACT && T == 8) cid[i] = kg + i < nreal ? a.col[c_begin + kg + i] : -1;
+            else cid[i] = kg + i < nreal ? __builtin_nontemporal_load(a.col + c_begin + kg + i) : -1;
+        }
     }
 
     __device__ __forceinline__ void tile(int kb, uint32_t (&av)[AW]) const
