@@ -195,13 +195,16 @@ struct KBlock {
     // Column ids: read once per k-block, so nontemporal -- except for the
     // compact 8-features-per-lane instance (width 256, two slices), where the
     // cached load measured 1.3% faster (4.757 -> 4.695 ms on the 114.6M-entry
-    // graph) while at 4 features per lane it was 3.5% slower.
+    // graph) while at 4 features per lane it was 3.5% slower, and for the
+    // compact edge lists, whose heads run in one workgroup and share the
+    // ids (and row masks) through the CU's L1 (fused attention backward
+    // 0.497 -> 0.476 ms at 4 heads x 64, 1.009 -> 0.947 at 8).
     __device__ __forceinline__ void cols(int kb, int (&cid)[8]) const
     {
         const int kg = kb + 8 * g;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            if constexpr (AMODE == A_COMPACT && T == 8) cid[i] = kg + i < nreal ? a.col[c_begin + kg + i] : -1;
+            if constexpr ((AMODE == A_COMPACT && T == 8) || AMODE == A_CMAP || AMODE == A_CMAP_DOT) cid[i] = kg + i < nreal ? a.col[c_begin + kg + i] : -1;
             else cid[i] = kg + i < nreal ? __builtin_nontemporal_load(a.col + c_begin + kg + i) : -1;
         }
     }
@@ -353,7 +356,10 @@ struct KBlock {
         mv = u32x4{0u, 0u, 0u, 0u};
         base = 0;
         if (kb < nreal) base = a.bbase[(int64_t(c_begin) + kb) >> 5];
-        if (kg < plen) mv = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(a.masks + int64_t(c_begin) + kg));
+        if (kg < plen) {
+            if constexpr (IDS) mv = *reinterpret_cast<const u32x4 *>(a.masks + int64_t(c_begin) + kg);
+            else mv = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(a.masks + int64_t(c_begin) + kg));
+        }
     }
 
     // tile_compact from prefetched masks (every lane of the wave must call it).
