@@ -51,6 +51,10 @@ _SIGNATURES = {
     "attgnn_spmm_block": (_int, [_vp, _vp, _vp, _int, _vp, _vp, _i64, _vp, _int, _i64, _vp, _int, _i64, _i64,
                                  _int, _int, _int, _vp, _vp, _vp, _i64, _int, _vp]),
     "attgnn_spmm_workspace_elems": (_i64, [_i64, _int, _i64]),
+    "attgnn_spmm_striped_workspace_elems": (_i64, [_i64, _int, _i64, _int]),
+    "attgnn_stripe_segments_device": (_int, [_vp, _vp, _vp, _i64, _int, _i64, _vp, _vp]),
+    "attgnn_spmm_compact_striped": (_int, [_vp, _vp, _vp, _vp, _vp, _int, _vp, _vp, _i64, _vp, _int, _vp, _int,
+                                           _i64, _vp, _int, _i64, _i64, _int, _int, _vp, _i64, _int, _vp]),
     "attgnn_blocks_tasks_max": (_i64, [_i64, _i64, _int, _int]),
     "attgnn_blocks_tasks_workspace_bytes": (_i64, [_i64]),
     "attgnn_blocks_tasks_device": (_int, [_vp, _vp, _i64, _i64, _int, _int, _vp, _vp, _vp, _i64, _vp, _i64, _vp,

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <mutex>
 
 #include "attgnn.h"
@@ -90,7 +91,23 @@ struct SpmmArgs {
     const uint8_t *dot_x;
     float *dot_out;
     int64_t dot_rows, dot_head_stride, dot_row_stride;
+    // striped node order (stripe_pos): with stripe_log2 > 0 window row r is
+    // stored at stripe_pos(r); with owners > 1 as well, the tasks are split by
+    // the owner of their columns (see the kernel) and oseg[part][o] holds
+    // the first column of owner o's segment of each part (o = 0..owners)
+    int stripe_log2, owners;
+    int64_t stripe_q, x_bytes;
+    const int32_t *oseg;
 };
+
+// Striped node order with 2^L owners of range q (a multiple of 64): node j
+// sits at stripe_pos(j) = ((j >> 6) mod 2^L) q + (j >> (6 + L)) 64 + (j mod 64),
+// i.e. 64-node stripes dealt round-robin to the owners, each owner's stripes
+// contiguous.  A bijection on [0, 2^L q); 16-row windows stay contiguous.
+__host__ __device__ __forceinline__ int64_t stripe_pos(int64_t j, int lg, int64_t q)
+{
+    return ((j >> 6) & ((1 << lg) - 1)) * q + ((j >> (6 + lg)) << 6) + (j & 63);
+}
 
 // Store n consecutive fp32 values as y_type (F32 / F16 / BF16).
 template <int N>
@@ -774,8 +791,250 @@ void spmm_block_kernel(SpmmArgs a)
             }
         } else {
             if (row >= a.m_ori) break;
-            store_row<T>(a.y, int64_t(head) * a.y_head_stride + row * a.y_row_stride + f0, a.y_type, v);
+            const int64_t orow = a.stripe_log2 ? stripe_pos(row, a.stripe_log2, a.stripe_q) : row;
+            store_row<T>(a.y, int64_t(head) * a.y_head_stride + orow * a.y_row_stride + f0, a.y_type, v);
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Owner-split SpMM over compact F16 / BF16 tiles in striped node order.  Every
+// part is taken by 8 waves, one per XCD: the wave on XCD x takes feature
+// slice x mod slices (T features per lane, slices * owners == 8) of the
+// columns owned by owner x / slices.  Striped column ids make each owner's
+// columns one id range, so a part's columns fall into consecutive owner
+// segments (oseg) and each XCD's L2 caches one owner's rows of one slice (a
+// quarter of X at 256 FP16 features, against half with slices alone).
+// Every task writes its fp32 tile; spmm_owner_combine_kernel sums them.
+//
+// The loads go through buffer resources, whose out-of-range reads return
+// zero: a column outside the owner's segment (or past the part) gets an
+// out-of-range offset instead of a branch, so the k-loop has no divergent
+// control flow and no zero-initialised load targets.  Requires X to span
+// fewer than 2^31 bytes (row offsets are 32-bit) and dim_n to be a multiple
+// of 16 T (every lane's features inside the row).
+
+constexpr int BUF_WORD3 = 0x00020000;     // raw buffer resource, untyped loads (gfx9 family)
+constexpr uint32_t BUF_OOB = 0x80000000u; // past every buffer the owner kernel reads: the load returns 0
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *p, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, int(bytes), BUF_WORD3);
+}
+
+// A k-block's structure, loaded one block ahead and spread over the wave
+// once (not once per lane): lane 16g + j (j < 8) holds the id of column
+// kb + 8g + j, lane 16g + j (j < 4) the row masks of columns kb + 8g + 2j and
+// kb + 8g + 2j + 1; base is the block's first packed value.
+struct OwnPre {
+    uint32_t id, mk;
+    int64_t base;
+};
+
+// The value of lane 16g + I for every lane of group g (ds_swizzle bit mode,
+// and-mask 0x10, or-mask I: within each 32-lane half the source keeps the
+// lane's group bit).
+template <int I>
+__device__ __forceinline__ uint32_t from_group_lane(uint32_t v)
+{
+    return uint32_t(__builtin_amdgcn_ds_swizzle(int(v), 0x10 | (I << 5)));
+}
+
+template <int KIND, int T>
+__global__ __launch_bounds__(256) void spmm_owner_kernel(SpmmArgs a)
+{
+    using KB = KBlock<KIND, T, A_COMPACT>;
+    constexpr int AW = KB::AW, XW = KB::XW;
+    static_assert((KIND == K_F16 || KIND == K_BF16) && (T == 4 || T == 8), "16-bit tiles, 4 or 8 features per lane");
+
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int xcd = int(blockIdx.x & 7);
+    const int slice = xcd % a.slices;
+    const int64_t part = int64_t(blockIdx.x >> 3) * 4 + wave;
+    if (part >= a.parts || a.win[part] < 0) return;
+    const int c_begin = a.rp[2 * part], plen = a.rp[2 * part + 2] - c_begin;
+    const int64_t so = part * (a.owners + 1) + xcd / a.slices;
+    const int lo = a.oseg[so] - c_begin, hi = a.oseg[so + 1] - c_begin;
+    const int g = lane >> 4, n = lane & 15;
+    const uint32_t rb = uint32_t(a.x_row_stride) * 2;                  // bytes per X row
+    const uint32_t fb = uint32_t(slice * 16 * T + n * T) * 2;          // the lane's features in a row
+    const __amdgpu_buffer_rsrc_t rx = buf_rsrc(a.x, uint32_t(a.x_bytes));
+    const __amdgpu_buffer_rsrc_t rc = buf_rsrc(a.col + c_begin, uint32_t(plen) * 4);
+    const __amdgpu_buffer_rsrc_t rm = buf_rsrc(a.masks + c_begin, uint32_t(plen) * 2);
+    const uint32_t id_off = n < 8 ? uint32_t(8 * g + n) * 4 : BUF_OOB;
+    const uint32_t mk_off = n < 4 ? uint32_t(8 * g + 2 * n) * 2 : BUF_OOB;
+
+    auto pre = [&](int kb, OwnPre &p) {
+        p.id = __builtin_amdgcn_raw_buffer_load_b32(rc, id_off + uint32_t(kb) * 4, 0, 0);
+        p.mk = __builtin_amdgcn_raw_buffer_load_b32(rm, mk_off + uint32_t(kb) * 2, 0, 0);
+        p.base = kb < plen ? a.bbase[(int64_t(c_begin) + kb) >> 5] : 0;
+    };
+
+    // A k-block is issued (its value and X-row loads started), then consumed
+    // (A operand formed, MFMAs) after the structure two blocks ahead is
+    // requested, so each block waits once, for its own loads.  Only columns in [lo, hi)
+    // contribute; FULL: the whole block lies in [lo, hi) (every block but a
+    // segment's first and last), so no column needs the range test.  The
+    // group's masks and ids come from lanes 16g + w / 16g + i
+    // (from_group_lane); the block's packed values are loaded 64 at a time,
+    // one per lane, and each lane takes its row's entries from the lanes that
+    // loaded them (ds_bpermute).
+    struct Blk {
+        uint32_t mine, rowbits, v;   // the lane's present columns, all its row's columns, its loaded value
+        int start, total;            // first packed entry of the lane's row; entries in the block
+        int64_t base;
+    };
+    const uint32_t low2 = ((1u << n) - 1u) * 0x10001u;   // rows below n, in both halves
+    auto issue = [&](auto full, int kb, const OwnPre &p, Blk &b, uint32_t (&xv)[8][XW]) {
+        constexpr bool FULL = decltype(full)::value;
+        const uint32_t mk[4] = {from_group_lane<0>(p.mk), from_group_lane<1>(p.mk), from_group_lane<2>(p.mk),
+                                from_group_lane<3>(p.mk)};
+        uint32_t tot = 0, below = 0, rowbits = 0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            tot += __builtin_popcount(mk[w]);
+            below += __builtin_popcount(mk[w] & low2);
+            rowbits |= ((mk[w] >> n) & 1u) << (2 * w);
+            rowbits |= ((mk[w] >> (16 + n)) & 1u) << (2 * w + 1);
+        }
+        // packed order of a block: 8-column groups, then rows, then columns
+        const int t0 = __shfl(int(tot), 0), t1 = __shfl(int(tot), 16), t2 = __shfl(int(tot), 32);
+        const int t3 = __shfl(int(tot), 48);
+        b.total = __builtin_amdgcn_readfirstlane(t0 + t1 + t2 + t3);
+        b.start = (g > 0 ? t0 : 0) + (g > 1 ? t1 : 0) + (g > 2 ? t2 : 0) + int(below);
+        uint32_t live = 0xffu;
+        if constexpr (!FULL) {
+            const int kg = kb + 8 * g;
+            const int l8 = min(max(lo - kg, 0), 8), h8 = min(max(hi - kg, 0), 8);
+            live = ((1u << h8) - 1u) & ~((1u << l8) - 1u);
+        }
+        b.rowbits = rowbits;
+        b.mine = rowbits & live;
+        b.base = p.base;
+        b.v = __builtin_amdgcn_raw_buffer_load_b16(buf_rsrc(a.val + p.base * 2, uint32_t(b.total) * 2),
+                                                   uint32_t(lane) * 2, 0, 0);
+        // ids below 2^24 (X spans < 2^31 bytes of >= 128-byte rows): 24-bit multiplies
+        const uint32_t ids[8] = {from_group_lane<0>(p.id), from_group_lane<1>(p.id), from_group_lane<2>(p.id),
+                                 from_group_lane<3>(p.id), from_group_lane<4>(p.id), from_group_lane<5>(p.id),
+                                 from_group_lane<6>(p.id), from_group_lane<7>(p.id)};
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const uint32_t off = FULL || ((live >> i) & 1u) ? __umul24(ids[i], rb) + fb : BUF_OOB;
+            if constexpr (T == 8) {
+                const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rx, off, 0, 0);
+#pragma unroll
+                for (int w = 0; w < 4; ++w) xv[i][w] = v[w];
+            } else {
+                const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(rx, off, 0, 0);
+                xv[i][0] = v[0];
+                xv[i][1] = v[1];
+            }
+        }
+    };
+    auto issue_any = [&](int kb, const OwnPre &p, Blk &b, uint32_t (&xv)[8][XW]) {
+        if (kb >= lo && kb + 32 <= hi) issue(std::true_type{}, kb, p, b, xv);
+        else issue(std::false_type{}, kb, p, b, xv);
+    };
+
+    f32x4 acc[T];
+#pragma unroll
+    for (int j = 0; j < T; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    auto consume = [&](const Blk &b, const uint32_t (&xv)[8][XW]) {
+        uint32_t e[8], src[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            src[i] = uint32_t(b.start) + __builtin_popcount(b.rowbits & ((1u << i) - 1u));
+            const uint32_t got = uint32_t(__builtin_amdgcn_ds_bpermute(int(src[i] * 4), int(b.v)));
+            e[i] = ((b.mine >> i) & 1u) && src[i] < 64u ? got : 0u;
+        }
+        for (int c = 64; c < b.total; c += 64) {   // uniform; blocks of more than 64 entries only
+            const uint32_t v = __builtin_amdgcn_raw_buffer_load_b16(
+                buf_rsrc(a.val + b.base * 2, uint32_t(b.total) * 2), uint32_t(c + lane) * 2, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const uint32_t got = uint32_t(__builtin_amdgcn_ds_bpermute(int((src[i] - c) * 4), int(v)));
+                if (((b.mine >> i) & 1u) && src[i] - uint32_t(c) < 64u) e[i] = got;
+            }
+        }
+        uint32_t av[AW];
+#pragma unroll
+        for (int w = 0; w < AW; ++w) av[w] = e[2 * w] | (e[2 * w + 1] << 16);
+        KB::mma(av, xv, acc);
+    };
+
+    // one block's loads in flight while the structure of the next two is
+    // prefetched (p0: block kb, p1: block kb + 32)
+    const int kb0 = lo & ~31;
+    OwnPre p0, p1;
+    pre(kb0, p0);
+    pre(kb0 + 32, p1);
+    for (int kb = kb0; kb < hi; kb += 32) {
+        Blk b;
+        uint32_t xv[8][XW];
+        issue_any(kb, p0, b, xv);
+        p0 = p1;
+        pre(kb + 64, p1);
+        consume(b, xv);
+    }
+
+    // the fp32 tile as 16-byte records [part][xcd][4 rows x T/4][lane]
+    f32x4 *dst = reinterpret_cast<f32x4 *>(a.partial) + (part * 8 + xcd) * T * 64 + lane;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int j = 0; j < T; j += 4)
+            __builtin_nontemporal_store(f32x4{acc[j][q], acc[j + 1][q], acc[j + 2][q], acc[j + 3][q]},
+                                        dst + (q * (T / 4) + j / 4) * 64);
+}
+
+// Sum of an owner-split SpMM's tiles: one thread per (part, slice, lane); the
+// thread of a window's first part adds the tiles of the window's parts in
+// part order, owners 0..owners-1 within each part, and stores the window's
+// rows 4g+q (g, n = lane / 16, lane % 16), features 16 T slice + T n.. at
+// their stripe positions.
+template <int T>
+__global__ __launch_bounds__(256) void spmm_owner_combine_kernel(SpmmArgs a)
+{
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int S = a.slices;
+    const int64_t p = t / (64 * S);
+    if (p >= a.parts) return;
+    const int rem = int(t - p * 64 * S);
+    const int s = rem >> 6, lane = rem & 63;
+    const int w = a.win[p];
+    if (w < 0 || (p > 0 && a.win[p - 1] == w)) return;
+    int64_t last = p;
+    while (last + 1 < a.parts && a.win[last + 1] == w) ++last;
+    float v[4][T];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int j = 0; j < T; ++j) v[q][j] = 0.f;
+    for (int64_t pp = p; pp <= last; ++pp)
+        for (int o = 0; o < a.owners; ++o) {
+            const f32x4 *src = reinterpret_cast<const f32x4 *>(a.partial) + (pp * 8 + o * S + s) * T * 64 + lane;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int j = 0; j < T; j += 4) {
+                    const f32x4 r = __builtin_nontemporal_load(src + (q * (T / 4) + j / 4) * 64);
+                    v[q][j] += r[0];
+                    v[q][j + 1] += r[1];
+                    v[q][j + 2] += r[2];
+                    v[q][j + 3] += r[3];
+                }
+        }
+    const int g = lane >> 4, n = lane & 15;
+    const int64_t f0 = int64_t(s) * 16 * T + int64_t(n) * T;
+    if (f0 >= a.dim_n) return;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int r = 4 * g + q;
+        if (r >= a.window) break;
+        const int64_t row = int64_t(w) * a.window + r;
+        if (row >= a.m_ori) break;
+        store_row<T>(a.y, stripe_pos(row, a.stripe_log2, a.stripe_q) * a.y_row_stride + f0, a.y_type, v[q]);
     }
 }
 
@@ -809,7 +1068,9 @@ __global__ __launch_bounds__(256) void spmm_combine_list_kernel(SpmmArgs a)
                 sv[0] += src[0];
             }
         }
-        store_row<V>(a.y, int64_t(head) * a.y_head_stride + (w * a.window + r) * a.y_row_stride + f, a.y_type, sv);
+        const int64_t row = w * a.window + r;
+        const int64_t orow = a.stripe_log2 ? stripe_pos(row, a.stripe_log2, a.stripe_q) : row;
+        store_row<V>(a.y, int64_t(head) * a.y_head_stride + orow * a.y_row_stride + f, a.y_type, sv);
     }
 }
 
@@ -929,10 +1190,9 @@ hipError_t launch_main(int t, const SpmmArgs &a, hipStream_t s)
 
 int y_bytes(int y_type) { return y_type == ATTGNN_F32 ? 4 : 2; }
 
-// Zero Y unless the caller guarantees every row is stored, run the main
-// kernel, then combine split windows.  Y is one contiguous buffer of
-// heads * m_ori * dim_n elements of y_type.
-hipError_t run_spmm(int kind, int amode, int t, const SpmmArgs &a, bool covered, hipStream_t s)
+// Zero Y (one contiguous buffer of heads * m_ori * dim_n elements of y_type)
+// unless the caller guarantees every row is stored.
+hipError_t zero_y(const SpmmArgs &a, bool covered, hipStream_t s)
 {
     const int64_t nbytes = int64_t(a.heads) * a.m_ori * a.dim_n * y_bytes(a.y_type);
     if (nbytes > 0 && !covered) {
@@ -942,9 +1202,16 @@ hipError_t run_spmm(int kind, int amode, int t, const SpmmArgs &a, bool covered,
         const int64_t blocks = std::min<int64_t>((std::max<int64_t>(n4, tail) + 255) / 256, 65536);
         hipLaunchKernelGGL(fill_zero_kernel, dim3(unsigned(blocks)), dim3(256), 0, s,
                            reinterpret_cast<u32x4 *>(yb), n4, yb + 16 * n4, tail);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
+        return hipGetLastError();
     }
+    return hipSuccess;
+}
+
+// Zero Y unless covered, run the main kernel, then combine split windows.
+hipError_t run_spmm(int kind, int amode, int t, const SpmmArgs &a, bool covered, hipStream_t s)
+{
+    hipError_t ez = zero_y(a, covered, s);
+    if (ez != hipSuccess) return ez;
     if (a.parts == 0) return hipSuccess;
     // the split-window list sits after the partial tiles in the workspace
     SpmmArgs b = a;
@@ -975,6 +1242,28 @@ hipError_t run_spmm(int kind, int amode, int t, const SpmmArgs &a, bool covered,
     else
         hipLaunchKernelGGL(spmm_combine_list_kernel<1>, dim3(unsigned(cblocks), unsigned(a.heads)), dim3(256), 0, s, b);
     return hipGetLastError();
+}
+
+template <int KIND, int T>
+hipError_t launch_owners(const SpmmArgs &a, hipStream_t s)
+{
+    const dim3 grid{static_cast<unsigned>((a.parts + 3) / 4 * 8)}, block{256};
+    hipLaunchKernelGGL((spmm_owner_kernel<KIND, T>), grid, block, 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int64_t threads = a.parts * a.slices * 64;
+    hipLaunchKernelGGL(spmm_owner_combine_kernel<T>, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// Owner-split SpMM (compact F16 / BF16 tiles, 4 or 8 features per lane): every
+// task writes its tile to the workspace, then the combine stores Y.
+hipError_t run_spmm_owners(int kind, int t, const SpmmArgs &a, bool covered, hipStream_t s)
+{
+    hipError_t e = zero_y(a, covered, s);
+    if (e != hipSuccess || a.parts == 0) return e;
+    if (kind == K_F16) return t == 8 ? launch_owners<K_F16, 8>(a, s) : launch_owners<K_F16, 4>(a, s);
+    return t == 8 ? launch_owners<K_BF16, 8>(a, s) : launch_owners<K_BF16, 4>(a, s);
 }
 
 }  // namespace
@@ -1326,5 +1615,123 @@ extern "C" int attgnn_spmm_compact(const int32_t *rp, const int32_t *col, const 
     a.pipe = pipe_of(flags, eb, false);
     a.xcd_slices = xcd_mapping(flags, a.slices);
     const hipError_t e = run_spmm(kind, A_COMPACT, t, a, (flags & ATTGNN_SPMM_ROWS_COVERED) != 0, s);
+    return e == hipSuccess ? ATTGNN_OK : int(e);
+}
+
+// ---------------------------------------------------------------------------
+// Striped node order (stripe_pos) and the owner-split SpMM.
+
+namespace {
+
+int log2_owners(int owners)
+{
+    switch (owners) { case 2: return 1; case 4: return 2; case 8: return 3; default: return -1; }
+}
+
+// First position of each owner's segment in every part's real columns
+// (ascending within a part): a binary search per (part, owner).
+__global__ __launch_bounds__(256) void stripe_segments_kernel(const int32_t *rp, const int32_t *win,
+                                                              const int32_t *col, int64_t parts, int owners,
+                                                              int64_t q, int32_t *seg)
+{
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t p = t / (owners + 1);
+    if (p >= parts) return;
+    const int o = int(t - p * (owners + 1));
+    const int c0 = rp[2 * p], c1 = rp[2 * p + 1];
+    if (win[p] < 0 || o == 0) { seg[t] = c0; return; }
+    if (o == owners) { seg[t] = c1; return; }
+    const int64_t target = int64_t(o) * q;
+    int lo = c0, hi = c1;
+    while (lo < hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if (col[mid] < target) lo = mid + 1;
+        else hi = mid;
+    }
+    seg[t] = lo;
+}
+
+constexpr int64_t OWNER_TILE_ELEMS = 8 * 64 * 4 * 8;   // per part: 8 tasks x 64 lanes x 4 rows x T <= 8
+
+}  // namespace
+
+extern "C" int attgnn_stripe_segments_device(const int32_t *rp, const int32_t *window_of_part, const int32_t *col,
+                                             int64_t parts, int owners, int64_t m, int32_t *owner_seg,
+                                             attgnn_stream_t stream)
+{
+    const int lg = log2_owners(owners);
+    if (lg < 0 || parts < 0 || m <= 0 || m % (int64_t(64) << lg) != 0) return ATTGNN_E_ARG;
+    if (parts == 0) return ATTGNN_OK;
+    if (!rp || !window_of_part || !col || !owner_seg) return ATTGNN_E_ARG;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int64_t threads = parts * (owners + 1);
+    hipLaunchKernelGGL(stripe_segments_kernel, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, s, rp,
+                       window_of_part, col, parts, owners, m >> lg, owner_seg);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? ATTGNN_OK : int(e);
+}
+
+extern "C" int64_t attgnn_spmm_striped_workspace_elems(int64_t parts, int window, int64_t dim_n, int owners)
+{
+    (void)owners;
+    return std::max<int64_t>(attgnn_spmm_workspace_elems(parts, window, dim_n), parts * OWNER_TILE_ELEMS);
+}
+
+extern "C" int attgnn_spmm_compact_striped(const int32_t *rp, const int32_t *col, const uint16_t *masks,
+                                           const int64_t *block_base, const void *values, int tile_type,
+                                           const int32_t *window_of_part, const int32_t *atomic, int64_t parts,
+                                           const int32_t *owner_seg, int owners, const void *X, int x_type,
+                                           int64_t n_x, void *Y, int y_type, int64_t m, int64_t dim_n, int window,
+                                           int mode, float *workspace, int64_t workspace_elems, int flags,
+                                           attgnn_stream_t stream)
+{
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int lg = log2_owners(owners);
+    if (lg < 0 || parts < 0 || m <= 0 || dim_n <= 0 || n_x < m) return ATTGNN_E_ARG;
+    if (m % (int64_t(64) << lg) != 0) return ATTGNN_E_ARG;
+    if (!(window == 8 || window == 16)) return ATTGNN_E_ARG;
+    const int kind = kind_of(mode, tile_type, x_type);
+    if (kind < 0 || kind == K_FP8) return ATTGNN_E_UNSUPPORTED;
+    if (!Y || !(y_type == ATTGNN_F32 || y_type == ATTGNN_F16 || y_type == ATTGNN_BF16)) return ATTGNN_E_ARG;
+    if (parts > 0 && (!rp || !col || !masks || !block_base || !values || !window_of_part || !atomic || !X ||
+                      !workspace))
+        return ATTGNN_E_ARG;
+    if (parts > 0 && workspace_elems < attgnn_spmm_striped_workspace_elems(parts, window, dim_n, owners))
+        return ATTGNN_E_ARG;
+    if (reinterpret_cast<uintptr_t>(masks) % 16 != 0) return ATTGNN_E_ARG;
+    const int eb = elt_bytes(kind);
+    int t = pick_t(dim_n, eb);
+    const int want = (flags >> 8) & 15;
+    if (want && want <= t && (want & (want - 1)) == 0 && dim_n % want == 0) t = want;
+    if (reinterpret_cast<uintptr_t>(X) % (t * eb) != 0) return ATTGNN_E_ARG;
+    if (reinterpret_cast<uintptr_t>(Y) % 16 != 0) return ATTGNN_E_ARG;
+
+    SpmmArgs a{};
+    a.rp = rp; a.col = col; a.win = window_of_part; a.atomic = atomic;
+    a.val = static_cast<const uint8_t *>(values); a.map = nullptr;
+    a.masks = masks; a.bbase = block_base;
+    a.x = static_cast<const uint8_t *>(X);
+    a.y = Y; a.partial = workspace;
+    a.parts = parts; a.m_ori = m; a.dim_n = dim_n;
+    a.x_head_stride = 0; a.x_row_stride = dim_n; a.y_head_stride = 0; a.y_row_stride = dim_n;
+    a.val_head_stride = 0;
+    a.slices = int((dim_n + 16 * t - 1) / (16 * t));
+    a.heads = 1; a.window = window; a.wide_log2 = 5; a.y_type = y_type;
+    a.pipe = pipe_of(flags, eb, false);
+    a.stripe_log2 = lg; a.stripe_q = m >> lg;
+    a.x_bytes = n_x * dim_n * eb;
+    const bool split = (kind == K_F16 || kind == K_BF16) && (t == 4 || t == 8) && a.slices * owners == 8 &&
+                       dim_n % (16 * t) == 0 && a.x_bytes < (int64_t(1) << 31) && dim_n * eb < (1 << 24) &&
+                       !(flags & ATTGNN_SPMM_NO_XCD_SLICES);
+    hipError_t e;
+    if (split) {
+        if (parts > 0 && !owner_seg) return ATTGNN_E_ARG;
+        a.owners = owners;
+        a.oseg = owner_seg;
+        e = run_spmm_owners(kind, t, a, (flags & ATTGNN_SPMM_ROWS_COVERED) != 0, s);
+    } else {
+        a.xcd_slices = xcd_mapping(flags, a.slices);
+        e = run_spmm(kind, A_COMPACT, t, a, (flags & ATTGNN_SPMM_ROWS_COVERED) != 0, s);
+    }
     return e == hipSuccess ? ATTGNN_OK : int(e);
 }

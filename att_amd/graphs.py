@@ -11,6 +11,11 @@ seed with numpy's PCG64:
                    i0 puts the largest expected degree at the structural cutoff
                    sqrt(N * mean_degree), where Chung-Lu probabilities stay <= 1.
 * ``band_random``  |i - j| <= half_width band plus uniform off-band entries.
+
+Striped node order (``stripe_pos``): 64-node stripes dealt round-robin to
+``owners`` (2, 4 or 8) contiguous ranges of ``q = m / owners`` positions, m a
+multiple of 64 * owners.  The SpMM's owner split relies on it: every XCD
+gathers only the feature rows of one owner range.
 """
 import numpy as np
 
@@ -166,6 +171,24 @@ def gcn_norm_values(csr, d=None):
     """A_hat values d_i * d_j (fp32) in CSR order."""
     d = gcn_norm_degree(csr) if d is None else d
     return (d[csr.row_ids()] * d[csr.indices]).astype(np.float32)
+
+
+def stripe_rows(n, owners):
+    """Rows of a graph of n nodes in striped order: n rounded up to a multiple
+    of 64 * owners (also a multiple of every window height)."""
+    if owners not in (2, 4, 8):
+        raise ValueError(f"owners must be 2, 4 or 8, got {owners}")
+    unit = 64 * owners
+    return max(-(-int(n) // unit) * unit, unit)
+
+
+def stripe_pos(j, owners, m):
+    """Striped position of node ids j (numpy integer array or torch tensor):
+    ((j >> 6) mod owners) * q + (j >> (6 + log2 owners)) * 64 + (j mod 64),
+    q = m / owners.  A bijection on [0, m)."""
+    lg = {2: 1, 4: 2, 8: 3}[owners]
+    q = m // owners
+    return ((j >> 6) & (owners - 1)) * q + ((j >> (6 + lg)) << 6) + (j & 63)
 
 
 def window_unique_columns(csr, window):

@@ -59,20 +59,31 @@ class GCNGraph:
     d = 1/sqrt(row count) (mdataset_fp16.py:54-56), rounded once to
     ``tile_dtype``.  Rows are padded to a multiple of ``window`` so every row
     lies in a window that has a part (no zero-fill before the SpMM).
+
+    ``stripe`` (2, 4 or 8 owners): striped node order (graphs.stripe_pos).
+    The windows keep the graph's own rows; the column ids become striped
+    positions and :meth:`spmm` takes and returns node rows in striped order
+    ([num_nodes, D], num_nodes = graphs.stripe_rows(n, stripe); convert with
+    :meth:`to_striped` / :meth:`from_striped`), so the SpMM can split every
+    part among the XCDs by owner (4 owners at 256 FP16 features: each XCD's
+    L2 caches a quarter of the gathered rows).  Compact tiles only.
     """
 
     def __init__(self, csr, device, window=16, wide=32, part_size=None, tile_dtype=torch.float16,
-                 symmetric=None, values=None, compact="auto", build="host", chunk_blocks=None):
+                 symmetric=None, values=None, compact="auto", build="host", chunk_blocks=None, stripe=None):
         if build not in ("host", "device"):
             raise ValueError(f"build must be 'host' or 'device', got {build!r}")
         self.build = build
         self.window, self.wide = window, wide
         self.chunk_blocks = chunk_blocks
+        self.stripe = stripe
+        if stripe and wide != 32:
+            raise ValueError("striped node order runs over compact tiles (wide 32)")
         # compact tiles win once a gathered row is >= 256 bytes; "auto" keeps
         # both forms and picks per call
-        self.compact = compact if wide == 32 else False
+        self.compact = True if stripe else (compact if wide == 32 else False)
         self.num_nodes_ori = csr.n
-        padded = csr.padded(window)
+        padded = csr.padded(64 * stripe if stripe else window)
         self.num_nodes = padded.n
         self.num_edges = csr.nnz
         if values is None:
@@ -105,20 +116,55 @@ class GCNGraph:
             else:
                 t, order = transpose_csr(padded)
                 self.bwd = self._build(t, np.asarray(values)[order], device)
+        if stripe:
+            self.stripe_index = G.stripe_pos(torch.arange(csr.n, dtype=torch.int64, device=device), stripe,
+                                             self.num_nodes)
+            for s in (self.fwd, self.bwd):
+                if "oseg" not in s:
+                    rp, win, _ = s["tasks"]
+                    s["oseg"] = ops.stripe_segments(rp, win, s["arrays"][1], stripe, self.num_nodes)
         # reference attribute names
         (self.row_pointers, self.column_index, self.degrees, self.t_window_rowTensor,
          self.t_atomicTensor) = self.fwd["arrays"]
 
     def _build_device(self, rp_d, col_d, vals_d, device):
+        if self.stripe:   # column ids -> striped positions, rows re-sorted
+            rows = torch.repeat_interleave(torch.arange(rp_d.numel() - 1, device=device),
+                                           (rp_d[1:] - rp_d[:-1]).long())
+            scol = G.stripe_pos(col_d.long(), self.stripe, rp_d.numel() - 1)
+            order = torch.sort(rows * (rp_d.numel() - 1) + scol).indices
+            col_d, vals_d = scol.index_select(0, order).int(), vals_d.index_select(0, order)
+            del rows, scol, order
         b = ops.blocks_build_device(rp_d, col_d, "edge_f32", vals_d, self.tile_dtype, self.window, self.wide,
                                     self.part_size, emit_empty=True, compact=bool(self.compact))
         return self._finish(b, device)
 
     def _build(self, csr, values, device):
-        b = ops.blocks_build(csr.rowptr32(), csr.indices, "edge_f32", np.asarray(values, np.float32),
+        values = np.asarray(values, np.float32)
+        if self.stripe:   # column ids -> striped positions, rows re-sorted
+            scol = G.stripe_pos(csr.indices.astype(np.int64), self.stripe, csr.n)
+            order = np.lexsort((scol, csr.row_ids()))
+            csr, values = G.CSR(csr.indptr, scol[order].astype(np.int32), csr.n), values[order]
+        b = ops.blocks_build(csr.rowptr32(), csr.indices, "edge_f32", values,
                              self.tile_dtype, self.window, self.wide, self.part_size, emit_empty=True,
                              compact=self.compact)
         return self._finish(b, device)
+
+    def to_striped(self, X):
+        """Node rows [n, ...] in the order :meth:`spmm` takes: [num_nodes, ...]
+        with node j at row stripe_pos(j) and zero padding rows (X itself
+        without striped order)."""
+        if not self.stripe:
+            return X
+        out = X.new_zeros((self.num_nodes,) + tuple(X.shape[1:]))
+        out.index_copy_(0, self.stripe_index, X)
+        return out
+
+    def from_striped(self, Y):
+        """The graph's node rows [n, ...] of rows in :meth:`spmm`'s order."""
+        if not self.stripe:
+            return Y[:self.num_nodes_ori]
+        return Y.index_select(0, self.stripe_index)
 
     def _finish(self, b, device):
         keys = ("rp", "col", "val", "window", "atomic")
@@ -138,6 +184,13 @@ class GCNGraph:
         s = self.bwd if transpose else self.fwd
         _, col, val, _, _ = s["arrays"]
         rp, win, at = s["tasks"]
+        if self.stripe:
+            if X.shape[0] != self.num_nodes:
+                raise ValueError(f"X must hold the {self.num_nodes} rows of the striped order (to_striped)")
+            masks, base, values = s["compact"]
+            return ops.spmm_compact_striped(rp, col, masks, base, values, win, at, s["oseg"], self.stripe,
+                                            X.contiguous(), self.window, out_dtype=out_dtype or X.dtype,
+                                            rows_covered=True)
         if X.shape[0] < self.num_nodes_ori:
             raise ValueError("X has fewer rows than the graph")
         use_compact = self.compact is True or (self.compact == "auto" and X.shape[1] * X.element_size() >= 256)

@@ -481,6 +481,64 @@ def spmm_compact(rp, col, masks, block_base, values, window_of_part, atomic, X, 
     return y
 
 
+def stripe_segments(rp, window_of_part, col, owners, m):
+    """attgnn_stripe_segments_device: int32 [parts, owners + 1], the first
+    position of each owner's segment in every part's real columns of a
+    structure whose column ids are striped positions (graphs.stripe_pos) of an
+    m-node graph.  Kept on ``rp`` (with the version counters of ``rp`` and
+    ``window_of_part``) like :func:`canonical_tasks`."""
+    dev = _need_gpu("stripe_segments", rp, window_of_part, col)
+    _need_i32("stripe_segments", rp, window_of_part, col)
+    parts = window_of_part.numel()
+    try:
+        version = (rp._version, window_of_part._version, col._version)
+    except RuntimeError:
+        version = None
+    tag = (version, rp.numel(), window_of_part.data_ptr(), col.data_ptr(), int(owners), int(m))
+    cached = getattr(rp, "_attgnn_oseg", None)
+    if version is not None and cached is not None and cached[0] == tag:
+        return cached[1]
+    seg = torch.empty((parts, owners + 1), dtype=torch.int32, device=dev)
+    check(lib().attgnn_stripe_segments_device(_ptr(rp), _ptr(window_of_part), _ptr(col), parts, int(owners), int(m),
+                                              _ptr(seg), _stream(dev)),
+          "attgnn_stripe_segments_device")
+    if version is not None:
+        rp._attgnn_oseg = (tag, seg)
+    return seg
+
+
+def spmm_compact_striped(rp, col, masks, block_base, values, window_of_part, atomic, owner_seg, owners, X, window,
+                         mode=None, out_dtype=torch.float32, rows_covered=False, workspace=None, pipe=None,
+                         split=True):
+    """attgnn_spmm_compact_striped: Y' = A X' in striped node order.
+
+    The structure's windows run over the graph's rows in its own order and its
+    column ids are striped positions (graphs.stripe_pos with ``owners``); X' is
+    [m, D] with node j's features at row stripe_pos(j), and Y' [m, D] receives
+    row j at stripe_pos(j).  Where the feature-slice count times ``owners`` is
+    8 (FP16 / BF16 at D = 256 with 4 owners) the parts are split by owner so
+    each XCD gathers one owner's rows of one slice; ``split=False`` keeps the
+    usual task mapping (results then differ in rounding only)."""
+    dev = _need_gpu("spmm_compact_striped", rp, col, masks, block_base, values, window_of_part, atomic, owner_seg, X)
+    _need_i32("spmm_compact_striped", rp, col, window_of_part, atomic, owner_seg)
+    parts = window_of_part.numel()
+    m, dim = X.shape
+    if owner_seg.shape != (parts, owners + 1):
+        raise ValueError(f"owner_seg must be [{parts}, {owners + 1}], got {tuple(owner_seg.shape)}")
+    y = torch.empty((m, dim), dtype=out_dtype, device=dev)
+    ws_n = lib().attgnn_spmm_striped_workspace_elems(parts, window, dim, owners) if parts else 0
+    ws = workspace if workspace is not None and workspace.numel() >= ws_n else \
+        torch.empty(max(ws_n, 1), dtype=torch.float32, device=dev)
+    flags = _spmm_flags(rows_covered, pipe, split)
+    rc = lib().attgnn_spmm_compact_striped(_ptr(rp), _ptr(col), _ptr(masks), _ptr(block_base), _ptr(values),
+                                           _TYPE_OF[values.dtype], _ptr(window_of_part), _ptr(atomic), parts,
+                                           _ptr(owner_seg), int(owners), _ptr(X), _TYPE_OF[X.dtype], m, _ptr(y),
+                                           _TYPE_OF[y.dtype], m, dim, window, _mode(mode, X.dtype), _ptr(ws), ws_n,
+                                           flags, _stream(dev))
+    check(rc, "attgnn_spmm_compact_striped")
+    return y
+
+
 def spmm_edge_map(rp, col, emap, edge_val, window_of_part, atomic, X, out_layout, window, wide,
                   m_ori=None, mode=None, x_layout="hmd", out_dtype=torch.float32, rows_covered=False, pipe=None,
                   workspace=None, xcd=True):

@@ -332,6 +332,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-precision", action="store_true", help="skip the precision and 16,384-node legs")
     ap.add_argument("--part-size", type=int, default=0)
+    ap.add_argument("--stripe", type=int, choices=[0, 2, 4, 8], default=4,
+                    help="striped node order with this many owners (0: the graph's own order)")
     ap.add_argument("--build", choices=["host", "device"], default="device",
                     help="where the window blocks are built (results are bit-identical)")
     ap.add_argument("--mode", choices=["train", "kernels"], default="train",
@@ -360,6 +362,7 @@ def main():
         log(f"cpu baseline: {cpu}")
     t = time.perf_counter()
     graph = GCNGraph(g, dev, window=16, wide=32, part_size=args.part_size or None, tile_dtype=torch.float16,
+                     stripe=args.stripe or None,
                      symmetric=True, values=np.concatenate([vals, np.zeros(0, np.float32)]), build=args.build)
     log(f"window blocks ({args.build} build): U={graph.n_unique} parts={graph.fwd['parts']} "
         f"cols={graph.fwd['n_cols']} part_size={graph.part_size} ({time.perf_counter() - t:.1f}s)")
@@ -376,7 +379,7 @@ def main():
     if args.mode == "kernels":
         # counter runs: 5 width-256 SpMMs and 5 copies of 1 GiB (the copy calibrates
         # FETCH_SIZE / WRITE_SIZE against a kernel of known traffic)
-        Xs = torch.randn(g.n, F_HID, generator=rng).half().to(dev)
+        Xs = graph.to_striped(torch.randn(g.n, F_HID, generator=rng).half().to(dev))
         src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
         for _ in range(5):
@@ -385,8 +388,11 @@ def main():
         torch.cuda.synchronize()
         log("kernels done: 5 x spmm(width 256) and 5 x copy(1 GiB)")
         return
-    X = torch.randn(g.n, F_IN, generator=rng).half().to(dev)
-    labels = torch.randint(0, N_CLASS, (g.n,), generator=torch.Generator().manual_seed(2)).to(dev)
+    # node rows in the order the graph's SpMMs take (striped: node j at row
+    # stripe_pos(j), zero padding rows); the loss averages the real nodes only
+    X = graph.to_striped(torch.randn(g.n, F_IN, generator=rng).half().to(dev))
+    labels = graph.to_striped(torch.randint(0, N_CLASS, (g.n,), generator=torch.Generator().manual_seed(2)).to(dev))
+    real = graph.stripe_index if graph.stripe else torch.arange(g.n, device=dev)
     torch.manual_seed(3)
 
     class GCN2(nn.Module):
@@ -414,7 +420,7 @@ def main():
         opt.zero_grad(set_to_none=True)
         # NLL loss (mean over nodes) as a gather: the same value as F.nll_loss,
         # whose 2-D reduction kernel takes ~4x longer here
-        loss = -model(X).gather(1, labels[:, None]).mean()
+        loss = -model(X).gather(1, labels[:, None]).squeeze(1).index_select(0, real).mean()
         loss.backward()
         opt.step()
         return loss
@@ -432,7 +438,7 @@ def main():
     log(f"step {ms:.3f} ms, loss {loss.item():.4f}")
 
     # ---- dominant kernel: width-256 fp16 SpMM, HIP events on its stream
-    Xs = torch.randn(g.n, F_HID, generator=rng).half().to(dev)
+    Xs = graph.to_striped(torch.randn(g.n, F_HID, generator=rng).half().to(dev))
     stream = torch.cuda.current_stream(dev)
     for _ in range(3):
         y = graph.spmm(Xs)
@@ -444,20 +450,24 @@ def main():
     ev1.record(stream)
     torch.cuda.synchronize()
     t_spmm = ev0.elapsed_time(ev1) / reps  # ms
-    M = g.n
+    M = graph.num_nodes if graph.stripe else g.n    # rows of Y written (striped: padding rows too)
     P, C = graph.fwd["parts"], graph.fwd["n_cols"]
     # compulsory bytes of one launch: X once, the compact structure as read
     # (rp, window, atomic; columns 4 B + row mask 2 B each; an 8-byte offset
-    # per 32-column block; the FP16 values), Y once
+    # per 32-column block; the FP16 values; striped: each part's owner
+    # segment bounds), Y once
     struct_bytes = 4 * (2 * P + 1) + 8 * P + C * (4 + 2) + (C // 32) * 8 + g.nnz * 2
+    if graph.stripe:
+        struct_bytes += 4 * (graph.stripe + 1) * graph.fwd["tasks"][1].numel()
     compulsory = g.n * F_HID * 2 + struct_bytes + M * F_HID * 2
-    gather_bytes = 4 * (M + 1) + g.nnz * (4 + 2) + graph.n_unique * F_HID * 2 + M * F_HID * 2
+    gather_bytes = 4 * (g.n + 1) + g.nnz * (4 + 2) + graph.n_unique * F_HID * 2 + g.n * F_HID * 2
     achieved = compulsory / (t_spmm * 1e-3) / 1e9
     # the compulsory bytes of the step's four SpMMs (two at width 256 over
     # the compact tiles, two at width 64 -- the 48 outputs padded -- over the
-    # dense FP16 tiles, C*16*2 bytes) per step time: a lower bound of the
-    # step's HBM traffic, below the roof
-    compulsory64 = g.n * 64 * 2 + 4 * (2 * P + 1) + 8 * P + C * (4 + 16 * 2) + M * 64 * 2
+    # compact tiles when striped, else the dense FP16 tiles, C*16*2 bytes)
+    # per step time: a lower bound of the step's HBM traffic, below the roof
+    struct64 = struct_bytes if graph.stripe else 4 * (2 * P + 1) + 8 * P + C * (4 + 16 * 2)
+    compulsory64 = g.n * 64 * 2 + struct64 + M * 64 * 2
     step_gbs = 2 * (compulsory + compulsory64) / (ms * 1e-3) / 1e9
     # accuracy of that SpMM vs FP64 on a row sample (same fp16-rounded A_hat and X)
     rows = np.arange(0, g.n, max(g.n // 2000, 1))
@@ -465,14 +475,14 @@ def main():
     sub_ix = np.concatenate([g.indices[g.indptr[r]:g.indptr[r + 1]] for r in rows])
     sub_v = np.concatenate([vals[g.indptr[r]:g.indptr[r + 1]] for r in rows]).astype(np.float16).astype(np.float64)
     import scipy.sparse as sp
-    ref = sp.csr_matrix((sub_v, sub_ix, sub_ip), shape=(rows.size, g.n)) @ Xs.double().cpu().numpy()
-    got = y[torch.from_numpy(rows).to(dev)].double().cpu().numpy()
+    ref = sp.csr_matrix((sub_v, sub_ix, sub_ip), shape=(rows.size, g.n)) @ graph.from_striped(Xs).double().cpu().numpy()
+    got = graph.from_striped(y)[torch.from_numpy(rows).to(dev)].double().cpu().numpy()
     rel_err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
 
     # plain FP32 GPU SpMM (torch.sparse on the same A_hat, int32 indices) for comparison
     A32 = torch.sparse_csr_tensor(torch.from_numpy(g.indptr.astype(np.int32)), torch.from_numpy(g.indices),
                                   torch.from_numpy(vals), (g.n, g.n)).to(dev)
-    X32 = Xs.float()
+    X32 = graph.from_striped(Xs).float()
     torch.sparse.mm(A32, X32)
     torch.cuda.synchronize()
     ev0.record(stream)
@@ -518,10 +528,14 @@ def main():
         "data": "synthetic",
         "config": {"workload": "2-layer GCN fwd+bwd (Adam), Chung-Lu gamma=2.1 seed 0",
                    "nodes": g.n, "nnz": g.nnz, "features": [F_IN, F_HID, F_OUT], "window": 16, "wide": 32,
-                   "tiles": "fp16", "accumulate": "fp32", "scale": args.scale, **workload_id},
+                   "tiles": "fp16", "accumulate": "fp32", "scale": args.scale,
+                   "node_order": f"striped, {graph.stripe} owners of 64-node stripes" if graph.stripe else "graph's own",
+                   **workload_id},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "window-block SpMM, width 256, fp16, compact tiles (main + split-window combine)",
+                     "kernel": ("window-block SpMM, width 256, fp16, compact tiles, striped node order with "
+                                f"{graph.stripe} owners (owner-split main kernel + tile combine)" if graph.stripe else
+                                "window-block SpMM, width 256, fp16, compact tiles (main + split-window combine)"),
                      "ms_per_launch": round(t_spmm, 4), "compulsory_bytes": int(compulsory),
                      "gather_bytes": int(gather_bytes),
                      "gather_gbs": round(gather_bytes / (t_spmm * 1e-3) / 1e9, 1),

@@ -347,6 +347,60 @@ int attgnn_spmm_compact(const int32_t *rp, const int32_t *col, const uint16_t *m
                         attgnn_stream_t stream);
 
 /*
+ * Striped node order and the owner-split SpMM.  No reference counterpart: the
+ * reference gathers X rows in the graph's own order
+ * (gcn/SpMM/src/spmmKernel.cu:158-265); these entry points compute the same
+ * product as attgnn_spmm_compact on a relabelled graph.
+ *
+ * With owners = 2^L (2, 4 or 8) and m nodes (m a multiple of 64 * owners,
+ * q = m / owners), node j sits at
+ *     stripe_pos(j) = ((j >> 6) mod owners) * q + (j >> (6 + L)) * 64 + (j mod 64),
+ * a bijection on [0, m): 64-node stripes dealt round-robin to the owners, each
+ * owner's stripes one contiguous range of q positions.  A striped structure
+ * is built (attgnn_blocks_build / _device, compact tiles) from the CSR whose
+ * rows stay in the graph's order and whose column ids are replaced by their
+ * stripe positions (re-sorted within each row); its parts' columns are then
+ * ascending, so each part's columns fall into consecutive owner segments.
+ *
+ * attgnn_stripe_segments_device: owner_seg[(owners + 1) p + o], o = 0..owners,
+ * the first position in part p's real columns [rp[2p], rp[2p+1]) whose id is
+ * >= o * q (owner_seg[.. + owners] = rp[2p+1]); parts with window < 0 get
+ * rp[2p] throughout.  Device int32, (owners + 1) * parts entries.
+ *
+ * attgnn_spmm_compact_striped: Y' = A X' over a striped structure.  X' [n_x >= m]
+ * [dim_n] holds node j's features at row stripe_pos(j); Y' [m][dim_n] (y_type)
+ * receives window row j at stripe_pos(j).  When the feature-slice count
+ * ceil(dim_n / (16 T)) (T = features per lane, 8 for 16-bit data at dim_n >=
+ * 128) times owners is 8 and the tiles are F16 or BF16, every part is taken
+ * by 8 tasks, one per XCD: feature slice x mod slices of owner x / slices's
+ * segment, so each XCD's L2 caches one owner's rows of one slice.  Every task
+ * stores its fp32 tile in the workspace and a second kernel sums each
+ * window's tiles in (part, owner) order: deterministic, and independent of
+ * the structure's partSize when the parts are a canonical task table
+ * (attgnn_blocks_tasks_device).  Otherwise (or with
+ * ATTGNN_SPMM_NO_XCD_SLICES) the tasks map as in attgnn_spmm_compact and
+ * only the row positions change.  The split also needs dim_n to be a multiple
+ * of 16 T, X to span fewer than 2^31 bytes and rows of fewer than 2^24 bytes
+ * (32-bit buffer offsets); otherwise the usual mapping runs.  mode NATIVE
+ * (F16 / BF16), BF16X3 or F32; no FP8.  workspace:
+ * attgnn_spmm_striped_workspace_elems floats.  flags as for
+ * attgnn_spmm_compact (the k-loop schedule bits apply to the usual mapping
+ * only; the split has one schedule).  ATTGNN_E_ARG for owners not 2 / 4 / 8, m not a
+ * multiple of 64 * owners, n_x < m, or a missing owner_seg where the split
+ * runs.
+ */
+int attgnn_stripe_segments_device(const int32_t *rp, const int32_t *window_of_part, const int32_t *col,
+                                  int64_t parts, int owners, int64_t m, int32_t *owner_seg,
+                                  attgnn_stream_t stream);
+int64_t attgnn_spmm_striped_workspace_elems(int64_t parts, int window, int64_t dim_n, int owners);
+int attgnn_spmm_compact_striped(const int32_t *rp, const int32_t *col, const uint16_t *masks,
+                                const int64_t *block_base, const void *values, int tile_type,
+                                const int32_t *window_of_part, const int32_t *atomic, int64_t parts,
+                                const int32_t *owner_seg, int owners, const void *X, int x_type, int64_t n_x,
+                                void *Y, int y_type, int64_t m, int64_t dim_n, int window, int mode,
+                                float *workspace, int64_t workspace_elems, int flags, attgnn_stream_t stream);
+
+/*
  * Edge-map SpMM: Y_h = A_h . X_h with A_h[i, j] = edge_val[h*nnz + map[slot(i,j)]],
  * tiles given by the edge-id map of attgnn_blocks_build(ATTGNN_SRC_MAP) (or its
  * transposed map, which aggregates with A^T on a symmetric pattern).
