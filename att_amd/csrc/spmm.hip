@@ -799,13 +799,14 @@ void spmm_block_kernel(SpmmArgs a)
 
 // ---------------------------------------------------------------------------
 // Owner-split SpMM over compact F16 / BF16 tiles in striped node order.  Every
-// part is taken by 8 waves, one per XCD: the wave on XCD x takes feature
-// slice x mod slices (T features per lane, slices * owners == 8) of the
-// columns owned by owner x / slices.  Striped column ids make each owner's
-// columns one id range, so a part's columns fall into consecutive owner
-// segments (oseg) and each XCD's L2 caches one owner's rows of one slice (a
-// quarter of X at 256 FP16 features, against half with slices alone).
-// Every task writes its fp32 tile; spmm_owner_combine_kernel sums them.
+// part is taken by K = owners * slices waves (K divides 8), one per XCD slot:
+// the wave on XCD x takes feature slice s of the columns owned by owner o,
+// (o, s) from x mod K (T features per lane).  Striped column ids make each
+// owner's columns one id range, so a part's columns fall into consecutive
+// owner segments (oseg) and each XCD's L2 caches one owner's rows of one
+// slice (a quarter of X at 256 FP16 features with 4 owners, against half
+// with slices alone).  Every task writes its fp32 tile;
+// spmm_owner_combine_kernel sums them.
 //
 // The loads go through buffer resources, whose out-of-range reads return
 // zero: a column outside the owner's segment (or past the part) gets an
@@ -849,12 +850,14 @@ __global__ __launch_bounds__(256) void spmm_owner_kernel(SpmmArgs a)
 
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int xcd = int(blockIdx.x & 7);
-    const int slice = xcd % a.slices;
-    const int64_t part = int64_t(blockIdx.x >> 3) * 4 + wave;
+    // K = owners * slices tasks per part on XCD slots x mod K; with K < 8 the
+    // XCDs x and x + K hold the same (owner, slice) and take alternate parts
+    const int xcd = int(blockIdx.x & 7), K = a.owners * a.slices;
+    const int slot = xcd % K, slice = slot % a.slices;
+    const int64_t part = ((int64_t(blockIdx.x >> 3) * (8 / K)) + xcd / K) * 4 + wave;
     if (part >= a.parts || a.win[part] < 0) return;
     const int c_begin = a.rp[2 * part], plen = a.rp[2 * part + 2] - c_begin;
-    const int64_t so = part * (a.owners + 1) + xcd / a.slices;
+    const int64_t so = part * (a.owners + 1) + slot / a.slices;
     const int lo = a.oseg[so] - c_begin, hi = a.oseg[so + 1] - c_begin;
     const int g = lane >> 4, n = lane & 15;
     const uint32_t rb = uint32_t(a.x_row_stride) * 2;                  // bytes per X row
@@ -978,8 +981,8 @@ __global__ __launch_bounds__(256) void spmm_owner_kernel(SpmmArgs a)
         consume(b, xv);
     }
 
-    // the fp32 tile as 16-byte records [part][xcd][4 rows x T/4][lane]
-    f32x4 *dst = reinterpret_cast<f32x4 *>(a.partial) + (part * 8 + xcd) * T * 64 + lane;
+    // the fp32 tile as 16-byte records [part][slot][4 rows x T/4][lane]
+    f32x4 *dst = reinterpret_cast<f32x4 *>(a.partial) + (part * K + slot) * T * 64 + lane;
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
@@ -1013,7 +1016,7 @@ __global__ __launch_bounds__(256) void spmm_owner_combine_kernel(SpmmArgs a)
         for (int j = 0; j < T; ++j) v[q][j] = 0.f;
     for (int64_t pp = p; pp <= last; ++pp)
         for (int o = 0; o < a.owners; ++o) {
-            const f32x4 *src = reinterpret_cast<const f32x4 *>(a.partial) + (pp * 8 + o * S + s) * T * 64 + lane;
+            const f32x4 *src = reinterpret_cast<const f32x4 *>(a.partial) + (pp * a.owners * S + o * S + s) * T * 64 + lane;
 #pragma unroll
             for (int q = 0; q < 4; ++q)
 #pragma unroll
@@ -1247,7 +1250,8 @@ hipError_t run_spmm(int kind, int amode, int t, const SpmmArgs &a, bool covered,
 template <int KIND, int T>
 hipError_t launch_owners(const SpmmArgs &a, hipStream_t s)
 {
-    const dim3 grid{static_cast<unsigned>((a.parts + 3) / 4 * 8)}, block{256};
+    const int64_t per_round = int64_t(8 / (a.owners * a.slices)) * 4;   // parts per 8 workgroups
+    const dim3 grid{static_cast<unsigned>((a.parts + per_round - 1) / per_round * 8)}, block{256};
     hipLaunchKernelGGL((spmm_owner_kernel<KIND, T>), grid, block, 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -1720,7 +1724,7 @@ extern "C" int attgnn_spmm_compact_striped(const int32_t *rp, const int32_t *col
     a.pipe = pipe_of(flags, eb, false);
     a.stripe_log2 = lg; a.stripe_q = m >> lg;
     a.x_bytes = n_x * dim_n * eb;
-    const bool split = (kind == K_F16 || kind == K_BF16) && (t == 4 || t == 8) && a.slices * owners == 8 &&
+    const bool split = (kind == K_F16 || kind == K_BF16) && (t == 4 || t == 8) && 8 % (a.slices * owners) == 0 &&
                        dim_n % (16 * t) == 0 && a.x_bytes < (int64_t(1) << 31) && dim_n * eb < (1 << 24) &&
                        !(flags & ATTGNN_SPMM_NO_XCD_SLICES);
     hipError_t e;
