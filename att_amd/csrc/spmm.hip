@@ -95,7 +95,7 @@ struct SpmmArgs {
     // stored at stripe_pos(r); with owners > 1 as well, the tasks are split by
     // the owner of their columns (see the kernel) and oseg[part][o] holds
     // the first column of owner o's segment of each part (o = 0..owners)
-    int stripe_log2, owners;
+    int stripe_log2, owners, groups;
     int64_t stripe_q, x_bytes;
     const int32_t *oseg;
 };
@@ -799,13 +799,14 @@ void spmm_block_kernel(SpmmArgs a)
 
 // ---------------------------------------------------------------------------
 // Owner-split SpMM over compact F16 / BF16 tiles in striped node order.  Every
-// part is taken by K = owners * slices waves (K divides 8), one per XCD slot:
-// the wave on XCD x takes feature slice s of the columns owned by owner o,
-// (o, s) from x mod K (T features per lane).  Striped column ids make each
-// owner's columns one id range, so a part's columns fall into consecutive
-// owner segments (oseg) and each XCD's L2 caches one owner's rows of one
-// slice (a quarter of X at 256 FP16 features with 4 owners, against half
-// with slices alone).  Every task writes its fp32 tile;
+// part is taken by K = groups * slices waves (K divides 8), one per XCD slot:
+// the wave on XCD x takes feature slice s (T features per lane) of the
+// columns owned by the owners of group gi -- gi, gi + groups, ... -- with
+// (gi, s) from x mod K, groups = min(owners, 8 / slices).  Striped column
+// ids make each owner's columns one id range, so a part's columns fall into
+// consecutive owner segments (oseg) and each XCD's L2 caches its owners' rows
+// of one slice (a quarter of X at 256 FP16 features, an eighth at 64, with 8
+// owners; half with slices alone).  Every task writes its fp32 tile;
 // spmm_owner_combine_kernel sums them.
 //
 // The loads go through buffer resources, whose out-of-range reads return
@@ -850,15 +851,15 @@ __global__ __launch_bounds__(256) void spmm_owner_kernel(SpmmArgs a)
 
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    // K = owners * slices tasks per part on XCD slots x mod K; with K < 8 the
-    // XCDs x and x + K hold the same (owner, slice) and take alternate parts
-    const int xcd = int(blockIdx.x & 7), K = a.owners * a.slices;
-    const int slot = xcd % K, slice = slot % a.slices;
+    // K = groups * slices tasks per part on XCD slots x mod K (with K < 8 the
+    // XCDs x and x + K hold the same task kind and take alternate parts);
+    // slot (group gi, slice s) takes slice s of owners gi, gi + groups, ...
+    const int xcd = int(blockIdx.x & 7), K = a.groups * a.slices;
+    const int slot = xcd % K, slice = slot % a.slices, group = slot / a.slices;
     const int64_t part = ((int64_t(blockIdx.x >> 3) * (8 / K)) + xcd / K) * 4 + wave;
     if (part >= a.parts || a.win[part] < 0) return;
     const int c_begin = a.rp[2 * part], plen = a.rp[2 * part + 2] - c_begin;
-    const int64_t so = part * (a.owners + 1) + slot / a.slices;
-    const int lo = a.oseg[so] - c_begin, hi = a.oseg[so + 1] - c_begin;
+    int lo = 0, hi = 0;   // the owner segment being walked, relative to c_begin
     const int g = lane >> 4, n = lane & 15;
     const uint32_t rb = uint32_t(a.x_row_stride) * 2;                  // bytes per X row
     const uint32_t fb = uint32_t(slice * 16 * T + n * T) * 2;          // the lane's features in a row
@@ -968,17 +969,22 @@ __global__ __launch_bounds__(256) void spmm_owner_kernel(SpmmArgs a)
 
     // one block's loads in flight while the structure of the next two is
     // prefetched (p0: block kb, p1: block kb + 32)
-    const int kb0 = lo & ~31;
-    OwnPre p0, p1;
-    pre(kb0, p0);
-    pre(kb0 + 32, p1);
-    for (int kb = kb0; kb < hi; kb += 32) {
-        Blk b;
-        uint32_t xv[8][XW];
-        issue_any(kb, p0, b, xv);
-        p0 = p1;
-        pre(kb + 64, p1);
-        consume(b, xv);
+    for (int o = group; o < a.owners; o += a.groups) {
+        const int64_t so = part * (a.owners + 1) + o;
+        lo = a.oseg[so] - c_begin;
+        hi = a.oseg[so + 1] - c_begin;
+        const int kb0 = lo & ~31;
+        OwnPre p0, p1;
+        pre(kb0, p0);
+        pre(kb0 + 32, p1);
+        for (int kb = kb0; kb < hi; kb += 32) {
+            Blk b;
+            uint32_t xv[8][XW];
+            issue_any(kb, p0, b, xv);
+            p0 = p1;
+            pre(kb + 64, p1);
+            consume(b, xv);
+        }
     }
 
     // the fp32 tile as 16-byte records [part][slot][4 rows x T/4][lane]
@@ -993,7 +999,7 @@ __global__ __launch_bounds__(256) void spmm_owner_kernel(SpmmArgs a)
 
 // Sum of an owner-split SpMM's tiles: one thread per (part, slice, lane); the
 // thread of a window's first part adds the tiles of the window's parts in
-// part order, owners 0..owners-1 within each part, and stores the window's
+// part order, owner groups 0..groups-1 within each part, and stores the window's
 // rows 4g+q (g, n = lane / 16, lane % 16), features 16 T slice + T n.. at
 // their stripe positions.
 template <int T>
@@ -1015,8 +1021,8 @@ __global__ __launch_bounds__(256) void spmm_owner_combine_kernel(SpmmArgs a)
 #pragma unroll
         for (int j = 0; j < T; ++j) v[q][j] = 0.f;
     for (int64_t pp = p; pp <= last; ++pp)
-        for (int o = 0; o < a.owners; ++o) {
-            const f32x4 *src = reinterpret_cast<const f32x4 *>(a.partial) + (pp * a.owners * S + o * S + s) * T * 64 + lane;
+        for (int gi = 0; gi < a.groups; ++gi) {
+            const f32x4 *src = reinterpret_cast<const f32x4 *>(a.partial) + (pp * a.groups * S + gi * S + s) * T * 64 + lane;
 #pragma unroll
             for (int q = 0; q < 4; ++q)
 #pragma unroll
@@ -1250,7 +1256,7 @@ hipError_t run_spmm(int kind, int amode, int t, const SpmmArgs &a, bool covered,
 template <int KIND, int T>
 hipError_t launch_owners(const SpmmArgs &a, hipStream_t s)
 {
-    const int64_t per_round = int64_t(8 / (a.owners * a.slices)) * 4;   // parts per 8 workgroups
+    const int64_t per_round = int64_t(8 / (a.groups * a.slices)) * 4;   // parts per 8 workgroups
     const dim3 grid{static_cast<unsigned>((a.parts + per_round - 1) / per_round * 8)}, block{256};
     hipLaunchKernelGGL((spmm_owner_kernel<KIND, T>), grid, block, 0, s, a);
     hipError_t e = hipGetLastError();
@@ -1724,7 +1730,9 @@ extern "C" int attgnn_spmm_compact_striped(const int32_t *rp, const int32_t *col
     a.pipe = pipe_of(flags, eb, false);
     a.stripe_log2 = lg; a.stripe_q = m >> lg;
     a.x_bytes = n_x * dim_n * eb;
-    const bool split = (kind == K_F16 || kind == K_BF16) && (t == 4 || t == 8) && 8 % (a.slices * owners) == 0 &&
+    // tasks per part: slices x owner groups, at most one per XCD
+    a.groups = std::max(1, std::min(owners, 8 / std::max(a.slices, 1)));
+    const bool split = (kind == K_F16 || kind == K_BF16) && (t == 4 || t == 8) && 8 % (a.slices * a.groups) == 0 &&
                        dim_n % (16 * t) == 0 && a.x_bytes < (int64_t(1) << 31) && dim_n * eb < (1 << 24) &&
                        !(flags & ATTGNN_SPMM_NO_XCD_SLICES);
     hipError_t e;

@@ -369,13 +369,14 @@ int attgnn_spmm_compact(const int32_t *rp, const int32_t *col, const uint16_t *m
  *
  * attgnn_spmm_compact_striped: Y' = A X' over a striped structure.  X' [n_x >= m]
  * [dim_n] holds node j's features at row stripe_pos(j); Y' [m][dim_n] (y_type)
- * receives window row j at stripe_pos(j).  When the feature-slice count
- * ceil(dim_n / (16 T)) (T = features per lane, 8 for 16-bit data at dim_n >=
- * 128) times owners is 8 and the tiles are F16 or BF16, every part is taken
- * by 8 tasks, one per XCD: feature slice x mod slices of owner x / slices's
- * segment, so each XCD's L2 caches one owner's rows of one slice.  Every task
- * stores its fp32 tile in the workspace and a second kernel sums each
- * window's tiles in (part, owner) order: deterministic, and independent of
+ * receives window row j at stripe_pos(j).  With F16 or BF16 tiles the parts
+ * are split by owner: with S = ceil(dim_n / (16 T)) feature slices (T =
+ * features per lane, 8 for 16-bit data at dim_n >= 128) and G = min(owners,
+ * 8 / S) owner groups, each part is taken by G * S tasks, at most one per
+ * XCD; the task on XCD slot (gi, s) takes slice s of the segments of owners
+ * gi, gi + G, ..., so each XCD's L2 caches its owners' rows of one slice.
+ * Every task stores its fp32 tile in the workspace and a second kernel sums
+ * each window's tiles in (part, group) order: deterministic, and independent of
  * the structure's partSize when the parts are a canonical task table
  * (attgnn_blocks_tasks_device).  Otherwise (or with
  * ATTGNN_SPMM_NO_XCD_SLICES) the tasks map as in attgnn_spmm_compact and

@@ -39,9 +39,12 @@ def _oracle(g, vals, X, dtype, transpose=False):
     return O.spmm(ip, ix, v, X.double().cpu().numpy())
 
 
+# owner groups x feature slices per part: 4x2, 4x2, 8x1, 2x4, 8x1, 4x1, 4x2 (two owners per task),
+# 2x4 (four owners per task); 48 features do not fill whole lanes: the usual mapping
 @pytest.mark.parametrize("owners,dim,dtype", [(4, 256, torch.float16), (4, 256, torch.bfloat16),
                                               (8, 128, torch.float16), (2, 512, torch.float16),
                                               (8, 64, torch.float16), (4, 64, torch.float16),
+                                              (8, 256, torch.float16), (8, 512, torch.bfloat16),
                                               (4, 48, torch.bfloat16)])
 def test_striped_spmm_vs_fp64(cuda, owners, dim, dtype):
     g = _graph(5000, 40, seed=owners + dim)
@@ -72,13 +75,12 @@ def test_striped_spmm_vs_fp64(cuda, owners, dim, dtype):
     assert O.rel_fro(graph.from_striped(Yu).double().cpu().numpy(), ref) < 2e-6
 
 
-@pytest.mark.parametrize("owners", [2, 4, 8])
-def test_striped_bitwise_across_part_sizes_and_builds(cuda, owners):
+@pytest.mark.parametrize("owners,dim", [(2, 512), (4, 256), (8, 128), (8, 256), (4, 64)])
+def test_striped_bitwise_across_part_sizes_and_builds(cuda, owners, dim):
     """The SpMM runs over the canonical task table: part sizes 1, 2, 32 and
     unbounded give the same bits; the host build equals the device build."""
-    g = _graph(3000, 60, seed=7 + owners)
+    g = _graph(3000, 60, seed=7 + owners + dim)
     vals = G.gcn_norm_values(g, G.gcn_norm_degree(g))
-    dim = {2: 512, 4: 256, 8: 128}[owners]   # fp16 feature slices x owners == 8: the owner split
     X = torch.randn(g.n, dim).half().to(cuda)
     outs = []
     for ps, build in ((1, "device"), (2, "host"), (32, "device"), (1 << 20, "host")):
