@@ -5,7 +5,8 @@ that do not need a full-size reference product:
   against FP64 on a strided row sample; A . 1 against the FP64 row sums of the
   FP16 tile values on every row; exact homogeneity A . (2X) = 2 (A . X) and
   run-to-run bitwise equality; one 2-layer training step with finite loss
-  and gradients.
+  and gradients; the same in striped node order (the benchmarked form)
+  against FP64, the graph's own order and its training step.
 * Attention graph (169,343 nodes, ~2.48M entries): the GAT layer at 1, 2, 4
   and 8 heads of 64 (BF16) against plain FP32 message passing with the same
   BF16-rounded inputs, forward and the weight gradient; the multi-head GATv2
@@ -87,6 +88,45 @@ def test_gcn_fullsize_training_step(cuda, gcn):
     assert torch.isfinite(loss)
     for c in (c1, c2):
         assert torch.isfinite(c.weights.grad).all() and c.weights.grad.abs().sum() > 0
+
+
+def test_gcn_fullsize_striped(cuda, gcn):
+    """The benchmarked form: the same graph in striped node order (4 owners,
+    owner-split SpMMs at widths 256 and 64) against FP64 on the row sample,
+    against the graph's own order, bitwise run to run; and one training step
+    whose loss and weight gradients match the graph's own order."""
+    g, vals, graph = gcn
+    st = GCNGraph(g, cuda, tile_dtype=torch.float16, symmetric=True, values=vals, build="device", stripe=4)
+    assert st.n_unique == graph.n_unique and st.fwd["n_cols"] == graph.fwd["n_cols"]
+    rows = np.arange(0, g.n, 100)
+    ip, ix, v = _row_sample(g.indptr, g.indices, vals.astype(np.float16).astype(np.float64), rows)
+    for dim in (256, 64):
+        X = torch.randn(g.n, dim, generator=torch.Generator().manual_seed(dim)).half().to(cuda)
+        Xs = st.to_striped(X)
+        ys = st.spmm(Xs, out_dtype=torch.float32)
+        y = st.from_striped(ys)
+        ref = sp.csr_matrix((v, ix, ip), shape=(rows.size, g.n)) @ X.double().cpu().numpy()
+        got = y[torch.from_numpy(rows).to(cuda)].double().cpu().numpy()
+        assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-5, dim
+        yo = graph.spmm(X, out_dtype=torch.float32)
+        assert ((y - yo).norm() / yo.norm()).item() < 1e-6, dim
+        assert torch.equal(st.spmm(Xs, out_dtype=torch.float32), ys)
+    res = []
+    for gr in (graph, st):
+        torch.manual_seed(3)
+        c1, c2 = GCNConv(602, 256).to(cuda), GCNConv(256, 48).to(cuda)
+        X = gr.to_striped(torch.randn(g.n, 602, generator=torch.Generator().manual_seed(1)).half().to(cuda))
+        out = gr.from_striped(c2(F.relu(c1(X, gr)), gr))
+        labels = torch.randint(0, 41, (g.n,), generator=torch.Generator().manual_seed(2)).to(cuda)
+        # summed, not averaged: the FP16 upstream gradients stay clear of the
+        # subnormal range, where the two orders' last-bit differences would
+        # decide which values flush
+        loss = F.nll_loss(F.log_softmax(out[:, :41].float(), 1), labels, reduction="sum")
+        loss.backward()
+        res.append((loss.item(), c1.weights.grad.float(), c2.weights.grad.float()))
+    (l0, a0, b0), (l1, a1, b1) = res
+    assert abs(l0 - l1) < 1e-4 * abs(l0)
+    assert ((a0 - a1).norm() / a0.norm()).item() < 1e-2 and ((b0 - b1).norm() / b0.norm()).item() < 1e-2
 
 
 @pytest.fixture(scope="module")
