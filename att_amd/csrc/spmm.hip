@@ -805,16 +805,17 @@ void spmm_block_kernel(SpmmArgs a)
 // (gi, s) from x mod K, groups = min(owners, 8 / slices).  Striped column
 // ids make each owner's columns one id range, so a part's columns fall into
 // consecutive owner segments (oseg) and each XCD's L2 caches its owners' rows
-// of one slice (a quarter of X at 256 FP16 features, an eighth at 64, with 8
-// owners; half with slices alone).  Every task writes its fp32 tile;
+// of one slice (with 4 owners a quarter of X, at 256 FP16 features and at 64;
+// half at 256 with slices alone).  Every task writes its fp32 tile;
 // spmm_owner_combine_kernel sums them.
 //
 // The loads go through buffer resources, whose out-of-range reads return
 // zero: a column outside the owner's segment (or past the part) gets an
 // out-of-range offset instead of a branch, so the k-loop has no divergent
 // control flow and no zero-initialised load targets.  Requires X to span
-// fewer than 2^31 bytes (row offsets are 32-bit) and dim_n to be a multiple
-// of 16 T (every lane's features inside the row).
+// fewer than 2^31 bytes and rows of fewer than 2^24 bytes (32-bit offsets,
+// 24-bit multiplies) and dim_n to be a multiple of 16 T (every lane's
+// features inside the row).
 
 constexpr int BUF_WORD3 = 0x00020000;     // raw buffer resource, untyped loads (gfx9 family)
 constexpr uint32_t BUF_OOB = 0x80000000u; // past every buffer the owner kernel reads: the load returns 0
@@ -877,9 +878,10 @@ __global__ __launch_bounds__(256) void spmm_owner_kernel(SpmmArgs a)
 
     // A k-block is issued (its value and X-row loads started), then consumed
     // (A operand formed, MFMAs) after the structure two blocks ahead is
-    // requested, so each block waits once, for its own loads.  Only columns in [lo, hi)
-    // contribute; FULL: the whole block lies in [lo, hi) (every block but a
-    // segment's first and last), so no column needs the range test.  The
+    // requested, so each block waits once, for its own loads.  Only columns
+    // in [lo, hi) contribute; FULL: the whole block lies in [lo, hi) (every
+    // block but a segment's first and last), so no column needs the range
+    // test.  The
     // group's masks and ids come from lanes 16g + w / 16g + i
     // (from_group_lane); the block's packed values are loaded 64 at a time,
     // one per lane, and each lane takes its row's entries from the lanes that
