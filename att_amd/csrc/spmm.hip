@@ -802,7 +802,8 @@ void spmm_block_kernel(SpmmArgs a)
 // part is taken by K = groups * slices waves (K divides 8), one per XCD slot:
 // the wave on XCD x takes feature slice s (T features per lane) of the
 // columns owned by the owners of group gi -- gi, gi + groups, ... -- with
-// (gi, s) from x mod K, groups = min(owners, 8 / slices).  Striped column
+// (gi, s) from x mod K (the host picks groups, see
+// attgnn_spmm_compact_striped).  Striped column
 // ids make each owner's columns one id range, so a part's columns fall into
 // consecutive owner segments (oseg) and each XCD's L2 caches its owners' rows
 // of one slice (with 4 owners a quarter of X, at 256 FP16 features and at 64;
@@ -1637,7 +1638,8 @@ namespace {
 
 int log2_owners(int owners)
 {
-    switch (owners) { case 2: return 1; case 4: return 2; case 8: return 3; default: return -1; }
+    switch (owners) { case 2: return 1; case 4: return 2; case 8: return 3; case 16: return 4; case 32: return 5;
+    default: return -1; }
 }
 
 // First position of each owner's segment in every part's real columns
@@ -1732,8 +1734,18 @@ extern "C" int attgnn_spmm_compact_striped(const int32_t *rp, const int32_t *col
     a.pipe = pipe_of(flags, eb, false);
     a.stripe_log2 = lg; a.stripe_q = m >> lg;
     a.x_bytes = n_x * dim_n * eb;
-    // tasks per part: slices x owner groups, at most one per XCD
-    a.groups = std::max(1, std::min(owners, 8 / std::max(a.slices, 1)));
+    // tasks per part: slices x owner groups, at most one per XCD.  Automatic
+    // group count: the smallest power of two that leaves each XCD about 16 MB
+    // of X (its owners' rows of its slice) -- on the 114.6M-entry graph 4
+    // groups at 256 FP16 features (15 MB; 3.70 ms against 4.38 with 2 and 5.23
+    // with 1) and 2 at 64 (15 MB; 1.18 ms against 1.26 with 4 and 1.40 with
+    // 8), with 8 owners.  A requested count (flags bits 12-14) may only be
+    // smaller than min(owners, 8 / slices).
+    const int max_groups = std::max(1, std::min(owners, 8 / std::max(a.slices, 1)));
+    a.groups = 1;
+    while (a.groups < max_groups && a.x_bytes > int64_t(a.groups) * a.slices * (int64_t(16) << 20)) a.groups *= 2;
+    const int want_groups = (flags >> 12) & 7;
+    if (want_groups) a.groups = std::min(max_groups, 1 << (want_groups - 1));
     const bool split = (kind == K_F16 || kind == K_BF16) && (t == 4 || t == 8) && 8 % (a.slices * a.groups) == 0 &&
                        dim_n % (16 * t) == 0 && a.x_bytes < (int64_t(1) << 31) && dim_n * eb < (1 << 24) &&
                        !(flags & ATTGNN_SPMM_NO_XCD_SLICES);

@@ -13,7 +13,7 @@ seed with numpy's PCG64:
 * ``band_random``  |i - j| <= half_width band plus uniform off-band entries.
 
 Striped node order (``stripe_pos``): 64-node stripes dealt round-robin to
-``owners`` (2, 4 or 8) contiguous ranges of ``q = m / owners`` positions, m a
+``owners`` (a power of two, 2 to 32) contiguous ranges of ``q = m / owners`` positions, m a
 multiple of 64 * owners.  The SpMM's owner split relies on it: every XCD
 gathers only the feature rows of one owner range.
 """
@@ -176,8 +176,8 @@ def gcn_norm_values(csr, d=None):
 def stripe_rows(n, owners):
     """Rows of a graph of n nodes in striped order: n rounded up to a multiple
     of 64 * owners (also a multiple of every window height)."""
-    if owners not in (2, 4, 8):
-        raise ValueError(f"owners must be 2, 4 or 8, got {owners}")
+    if owners not in (2, 4, 8, 16, 32):
+        raise ValueError(f"owners must be a power of two from 2 to 32, got {owners}")
     unit = 64 * owners
     return max(-(-int(n) // unit) * unit, unit)
 
@@ -186,7 +186,7 @@ def stripe_pos(j, owners, m):
     """Striped position of node ids j (numpy integer array or torch tensor):
     ((j >> 6) mod owners) * q + (j >> (6 + log2 owners)) * 64 + (j mod 64),
     q = m / owners.  A bijection on [0, m)."""
-    lg = {2: 1, 4: 2, 8: 3}[owners]
+    lg = {2: 1, 4: 2, 8: 3, 16: 4, 32: 5}[owners]
     q = m // owners
     return ((j >> 6) & (owners - 1)) * q + ((j >> (6 + lg)) << 6) + (j & 63)
 

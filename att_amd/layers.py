@@ -60,7 +60,7 @@ class GCNGraph:
     ``tile_dtype``.  Rows are padded to a multiple of ``window`` so every row
     lies in a window that has a part (no zero-fill before the SpMM).
 
-    ``stripe`` (2, 4 or 8 owners): striped node order (graphs.stripe_pos).
+    ``stripe`` (2 to 32 owners, a power of two): striped node order (graphs.stripe_pos).
     The windows keep the graph's own rows; the column ids become striped
     positions and :meth:`spmm` takes and returns node rows in striped order
     ([num_nodes, D], num_nodes = graphs.stripe_rows(n, stripe); convert with

@@ -509,7 +509,7 @@ def stripe_segments(rp, window_of_part, col, owners, m):
 
 def spmm_compact_striped(rp, col, masks, block_base, values, window_of_part, atomic, owner_seg, owners, X, window,
                          mode=None, out_dtype=torch.float32, rows_covered=False, workspace=None, pipe=None,
-                         split=True):
+                         split=True, groups=None):
     """attgnn_spmm_compact_striped: Y' = A X' in striped node order.
 
     The structure's windows run over the graph's rows in its own order and its
@@ -518,7 +518,10 @@ def spmm_compact_striped(rp, col, masks, block_base, values, window_of_part, ato
     row j at stripe_pos(j).  Where the feature-slice count times ``owners`` is
     8 (FP16 / BF16 at D = 256 with 4 owners) the parts are split by owner so
     each XCD gathers one owner's rows of one slice; ``split=False`` keeps the
-    usual task mapping (results then differ in rounding only)."""
+    usual task mapping (results then differ in rounding only).  ``groups``:
+    the owner groups per part (at most min(owners, 8 / slices); default: the
+    fewest that leave each XCD about 16 MB of X) -- each task walks the
+    segments of owners/groups owners in turn."""
     dev = _need_gpu("spmm_compact_striped", rp, col, masks, block_base, values, window_of_part, atomic, owner_seg, X)
     _need_i32("spmm_compact_striped", rp, col, window_of_part, atomic, owner_seg)
     parts = window_of_part.numel()
@@ -530,6 +533,10 @@ def spmm_compact_striped(rp, col, masks, block_base, values, window_of_part, ato
     ws = workspace if workspace is not None and workspace.numel() >= ws_n else \
         torch.empty(max(ws_n, 1), dtype=torch.float32, device=dev)
     flags = _spmm_flags(rows_covered, pipe, split)
+    if groups:
+        if groups & (groups - 1) or not 1 <= groups <= 8:
+            raise ValueError(f"groups must be 1, 2, 4 or 8, got {groups}")
+        flags |= (groups.bit_length()) << 12
     rc = lib().attgnn_spmm_compact_striped(_ptr(rp), _ptr(col), _ptr(masks), _ptr(block_base), _ptr(values),
                                            _TYPE_OF[values.dtype], _ptr(window_of_part), _ptr(atomic), parts,
                                            _ptr(owner_seg), int(owners), _ptr(X), _TYPE_OF[X.dtype], m, _ptr(y),

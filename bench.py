@@ -332,7 +332,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-precision", action="store_true", help="skip the precision and 16,384-node legs")
     ap.add_argument("--part-size", type=int, default=0)
-    ap.add_argument("--stripe", type=int, choices=[0, 2, 4, 8], default=4,
+    ap.add_argument("--stripe", type=int, choices=[0, 2, 4, 8, 16, 32], default=8,
                     help="striped node order with this many owners (0: the graph's own order)")
     ap.add_argument("--build", choices=["host", "device"], default="device",
                     help="where the window blocks are built (results are bit-identical)")

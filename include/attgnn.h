@@ -303,6 +303,9 @@ enum {
                                           gathered rows; this flag turns it off  */
     /* bits 8-11: features per lane (1, 2, 4 or 8; 0 = automatic), i.e. slices of
        16x that many features; only narrower than the automatic choice        */
+    /* bits 12-14 (attgnn_spmm_compact_striped): log2(owner groups) + 1, 0 =
+       automatic (the fewest leaving each XCD about 16 MB of X); at most
+       min(owners, 8 / slices)                                                  */
 };
 
 int64_t attgnn_spmm_workspace_elems(int64_t parts, int window, int64_t dim_n);
@@ -352,7 +355,7 @@ int attgnn_spmm_compact(const int32_t *rp, const int32_t *col, const uint16_t *m
  * (gcn/SpMM/src/spmmKernel.cu:158-265); these entry points compute the same
  * product as attgnn_spmm_compact on a relabelled graph.
  *
- * With owners = 2^L (2, 4 or 8) and m nodes (m a multiple of 64 * owners,
+ * With owners = 2^L (2 to 32) and m nodes (m a multiple of 64 * owners,
  * q = m / owners), node j sits at
  *     stripe_pos(j) = ((j >> 6) mod owners) * q + (j >> (6 + L)) * 64 + (j mod 64),
  * a bijection on [0, m): 64-node stripes dealt round-robin to the owners, each
@@ -371,8 +374,9 @@ int attgnn_spmm_compact(const int32_t *rp, const int32_t *col, const uint16_t *m
  * [dim_n] holds node j's features at row stripe_pos(j); Y' [m][dim_n] (y_type)
  * receives window row j at stripe_pos(j).  With F16 or BF16 tiles the parts
  * are split by owner: with S = ceil(dim_n / (16 T)) feature slices (T =
- * features per lane, 8 for 16-bit data at dim_n >= 128) and G = min(owners,
- * 8 / S) owner groups, each part is taken by G * S tasks, at most one per
+ * features per lane, 8 for 16-bit data at dim_n >= 128) and G owner groups
+ * (a power of two up to min(owners, 8 / S): by default the fewest that leave
+ * each XCD about 16 MB of X), each part is taken by G * S tasks, at most one per
  * XCD; the task on XCD slot (gi, s) takes slice s of the segments of owners
  * gi, gi + G, ..., so each XCD's L2 caches its owners' rows of one slice.
  * Every task stores its fp32 tile in the workspace and a second kernel sums
@@ -386,7 +390,8 @@ int attgnn_spmm_compact(const int32_t *rp, const int32_t *col, const uint16_t *m
  * (F16 / BF16), BF16X3 or F32; no FP8.  workspace:
  * attgnn_spmm_striped_workspace_elems floats.  flags as for
  * attgnn_spmm_compact (the k-loop schedule bits apply to the usual mapping
- * only; the split has one schedule).  ATTGNN_E_ARG for owners not 2 / 4 / 8, m not a
+ * only; the split has one schedule) plus the owner-group bits 12-14.
+ * ATTGNN_E_ARG for owners not a power of two in [2, 32], m not a
  * multiple of 64 * owners, n_x < m, or a missing owner_seg where the split
  * runs.
  */

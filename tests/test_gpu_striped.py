@@ -45,7 +45,7 @@ def _oracle(g, vals, X, dtype, transpose=False):
                                               (8, 128, torch.float16), (2, 512, torch.float16),
                                               (8, 64, torch.float16), (4, 64, torch.float16),
                                               (8, 256, torch.float16), (8, 512, torch.bfloat16),
-                                              (4, 48, torch.bfloat16)])
+                                              (32, 256, torch.float16), (4, 48, torch.bfloat16)])
 def test_striped_spmm_vs_fp64(cuda, owners, dim, dtype):
     g = _graph(5000, 40, seed=owners + dim)
     vals = G.gcn_norm_values(g, G.gcn_norm_degree(g))
@@ -66,16 +66,26 @@ def test_striped_spmm_vs_fp64(cuda, owners, dim, dtype):
     assert int(pad.sum()) == graph.num_nodes - g.n and not Yp[pad].any()
     # bitwise run to run
     assert torch.equal(Yp, graph.spmm(Xs, out_dtype=torch.float32))
-    # the usual task mapping gives the same sums up to rounding
+    # every owner-group count (each task walking owners/groups segments), and
+    # the usual task mapping, give the same sums up to rounding
     _, col, _, _, _ = graph.fwd["arrays"]
     rp, win, at = graph.fwd["tasks"]
     masks, base, values = graph.fwd["compact"]
+    slices = dim // 128 if dim >= 128 else 1   # fp16 / bf16: 8 features per lane from 128 up
+    groups = 1
+    while groups <= min(owners, 8 // slices):
+        Yg = ops.spmm_compact_striped(rp, col, masks, base, values, win, at, graph.fwd["oseg"], owners, Xs, 16,
+                                      out_dtype=torch.float32, groups=groups)
+        assert O.rel_fro(graph.from_striped(Yg).double().cpu().numpy(), ref) < 2e-6, groups
+        assert torch.equal(Yg, ops.spmm_compact_striped(rp, col, masks, base, values, win, at, graph.fwd["oseg"],
+                                                        owners, Xs, 16, out_dtype=torch.float32, groups=groups))
+        groups *= 2
     Yu = ops.spmm_compact_striped(rp, col, masks, base, values, win, at, graph.fwd["oseg"], owners, Xs, 16,
                                   out_dtype=torch.float32, split=False)
     assert O.rel_fro(graph.from_striped(Yu).double().cpu().numpy(), ref) < 2e-6
 
 
-@pytest.mark.parametrize("owners,dim", [(2, 512), (4, 256), (8, 128), (8, 256), (4, 64)])
+@pytest.mark.parametrize("owners,dim", [(2, 512), (4, 256), (8, 128), (8, 256), (4, 64), (16, 256)])
 def test_striped_bitwise_across_part_sizes_and_builds(cuda, owners, dim):
     """The SpMM runs over the canonical task table: part sizes 1, 2, 32 and
     unbounded give the same bits; the host build equals the device build."""
@@ -83,10 +93,17 @@ def test_striped_bitwise_across_part_sizes_and_builds(cuda, owners, dim):
     vals = G.gcn_norm_values(g, G.gcn_norm_degree(g))
     X = torch.randn(g.n, dim).half().to(cuda)
     outs = []
+    slices = dim // 128 if dim >= 128 else 1
     for ps, build in ((1, "device"), (2, "host"), (32, "device"), (1 << 20, "host")):
         graph = GCNGraph(g, cuda, tile_dtype=torch.float16, symmetric=True, values=vals, build=build,
                          part_size=ps, stripe=owners, chunk_blocks=3)
-        outs.append(graph.spmm(graph.to_striped(X)))
+        _, col, _, _, _ = graph.fwd["arrays"]
+        rp, win, at = graph.fwd["tasks"]
+        masks, base, values = graph.fwd["compact"]
+        Xs = graph.to_striped(X)
+        outs.append(torch.stack([ops.spmm_compact_striped(rp, col, masks, base, values, win, at, graph.fwd["oseg"],
+                                                          owners, Xs, 16, out_dtype=torch.float16, groups=gr)
+                                 for gr in (1, min(owners, 8 // slices))]))
     for y in outs[1:]:
         assert torch.equal(y, outs[0])
 

@@ -14,7 +14,7 @@ from att_amd import graphs as G
 from att_amd import ops
 
 
-@pytest.mark.parametrize("owners", [2, 4, 8])
+@pytest.mark.parametrize("owners", [2, 4, 8, 16, 32])
 @pytest.mark.parametrize("units", [1, 3, 17])
 def test_stripe_pos_bijection_and_ranges(owners, units):
     m = 64 * owners * units
@@ -80,7 +80,8 @@ def test_striped_entry_points_reject_bad_arguments_without_device():
         return L.attgnn_spmm_compact_striped(fake, fake, fake, fake, fake, _lib.F16, fake, fake, 0, fake, owners,
                                              fake, x, n_x, fake, _lib.F16, m, dim, window, _lib.MODE_NATIVE, nul, 0,
                                              0, nul)
-    assert spmm(3, 1024, 1024) == -1          # owners not 2 / 4 / 8
+    assert spmm(3, 1024, 1024) == -1          # owners not a power of two
+    assert spmm(64, 4096, 4096) == -1         # more than 32 owners
     assert spmm(4, 1000, 1000) == -1          # m not a multiple of 64 * owners
     assert spmm(4, 1024, 1000) == -1          # X shorter than m
     assert spmm(4, 1024, 1024, window=12) == -1
