@@ -65,8 +65,9 @@ class GCNGraph:
     positions and :meth:`spmm` takes and returns node rows in striped order
     ([num_nodes, D], num_nodes = graphs.stripe_rows(n, stripe); convert with
     :meth:`to_striped` / :meth:`from_striped`), so the SpMM can split every
-    part among the XCDs by owner (4 owners at 256 FP16 features: each XCD's
-    L2 caches a quarter of the gathered rows).  Compact tiles only.
+    part among the XCDs by owner (8 owners at 256 FP16 features, two per XCD
+    slot: each XCD's L2 caches a quarter of the gathered rows of its feature
+    slice).  Compact tiles only.
     """
 
     def __init__(self, csr, device, window=16, wide=32, part_size=None, tile_dtype=torch.float16,

@@ -516,7 +516,7 @@ def spmm_compact_striped(rp, col, masks, block_base, values, window_of_part, ato
     column ids are striped positions (graphs.stripe_pos with ``owners``); X' is
     [m, D] with node j's features at row stripe_pos(j), and Y' [m, D] receives
     row j at stripe_pos(j).  Where the feature-slice count times ``owners`` is
-    8 (FP16 / BF16 at D = 256 with 4 owners) the parts are split by owner so
+    8 (FP16 / BF16 at D = 256 with 4 owner groups) the parts are split by owner so
     each XCD gathers one owner's rows of one slice; ``split=False`` keeps the
     usual task mapping (results then differ in rounding only).  ``groups``:
     the owner groups per part (at most min(owners, 8 / slices); default: the
