@@ -35,7 +35,16 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None and t.numel() > 0 else ctypes.c_void_p(0)
 
 
+# torch's raw accessor of a device's current stream (what its own kernels
+# launch on); torch.cuda.current_stream builds a Stream object and resolves
+# the device on every call, which costs more than many of the launches here
+_RAW_STREAM = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
 def _stream(device):
+    """The current stream of ``device`` as a hipStream_t."""
+    if _RAW_STREAM is not None and device.index is not None:
+        return ctypes.c_void_p(_RAW_STREAM(device.index))
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
 
 
@@ -178,9 +187,14 @@ class DevicePlan:
     CSR: its workspace and the structure sizes read back once.  Every fill of
     the same (row, col, window, wide, part_size, emit_empty) reuses it -- the
     fill only reads the plan -- so a graph needing several tile types (values
-    and an edge-id map) pays the read-back once."""
+    and an edge-id map) pays the read-back once.
 
-    def __init__(self, row, col, window, wide, part_size, emit_empty):
+    ``sync=False``: the sizes are copied to pinned host memory behind an event
+    instead of read at once; :meth:`ready` polls it and :meth:`wait` (called
+    by :meth:`fill`) blocks until they arrive, so the host can do other work
+    while the plan runs."""
+
+    def __init__(self, row, col, window, wide, part_size, emit_empty, sync=True):
         dev = _need_gpu("blocks_plan_device", row, col)
         _need_i32("blocks_plan_device", row, col)
         if row.dim() != 1 or col.dim() != 1 or row.numel() < 1:
@@ -198,14 +212,39 @@ class DevicePlan:
         check(L.attgnn_blocks_plan_device(_ptr(row), rows1, _ptr(col), nnz, window, wide, part_size,
                                           int(self.emit_empty), _ptr(self.ws), ws_n, _ptr(totals), _stream(dev)),
               "attgnn_blocks_plan_device")
-        status, self.n_rp, self.n_col, self.n_val, self.n_parts, self.n_unique = [int(x) for x in totals[:6].cpu()]
+        self._pending = None
+        if sync:
+            self._sizes([int(x) for x in totals[:6].cpu()])
+        else:
+            host = torch.empty(8, dtype=torch.int64, pin_memory=True)
+            host.copy_(totals, non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(torch.cuda.current_stream(dev))
+            self._pending = (host, ev, totals)
+
+    def _sizes(self, t):
+        status, self.n_rp, self.n_col, self.n_val, self.n_parts, self.n_unique = t
         check(status, "attgnn_blocks_plan_device")
 
-    def fill(self, src, src_values, out_dtype, want_trans=False, compact=False, n_values=None):
+    def ready(self):
+        """True once the sizes are on the host (always for sync=True)."""
+        return self._pending is None or self._pending[1].query()
+
+    def wait(self):
+        """Block until the sizes are on the host and take them."""
+        if self._pending is not None:
+            host, ev, _ = self._pending
+            ev.synchronize()
+            self._pending = None
+            self._sizes([int(x) for x in host[:6].tolist()])
+
+    def fill(self, src, src_values, out_dtype, want_trans=False, compact=False, n_values=None, positions=True):
         """One structure of this plan (keys as :func:`blocks_build`).
         ``n_values``: an upper bound of the compact value count when the
         caller knows one (an edge-id map holds each CSR entry at most once:
-        nnz), which saves its read-back; see :func:`compact_tiles_device`."""
+        nnz), which saves its read-back; see :func:`compact_tiles_device`.
+        ``positions=False`` skips the compact form's per-entry positions."""
+        self.wait()
         if src not in _SRCS:
             raise ValueError(f"unknown src {src!r}")
         if want_trans and src != "map":
@@ -243,7 +282,8 @@ class DevicePlan:
         if compact:
             if self.wide != 32 or self.window > 16:
                 raise ValueError("compact tiles need wide = 32 and window <= 16")
-            out.update(compact_tiles_device(out["val"], self.n_col, self.window, n_values=n_values))
+            out.update(compact_tiles_device(out["val"], self.n_col, self.window, positions=positions,
+                                            n_values=n_values))
         del keep
         return out
 

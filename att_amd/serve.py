@@ -97,6 +97,26 @@ def prepare_host(csr, features, part_size=SERVE_PART_SIZE):
             "X": torch.from_numpy(X).to(torch.bfloat16)}
 
 
+class Converted:
+    """A host tensor to be staged as ``dtype`` with ``rows`` rows: its rows
+    converted (torch's round-to-nearest-even casts) and the rows past its own
+    zero."""
+
+    def __init__(self, src, dtype, rows):
+        self.src, self.dtype = src, dtype
+        self.shape = (int(rows),) + tuple(src.shape[1:])
+
+    def write_into(self, out):
+        n = self.src.shape[0]
+        out[:n].copy_(self.src)
+        out[n:].zero_()
+
+    def materialise(self):
+        out = torch.empty(self.shape, dtype=self.dtype)
+        self.write_into(out)
+        return out
+
+
 class PinnedRing:
     """Page-locked staging for host-to-device uploads: one pinned buffer
     allocated once (pinning per upload costs more than the copy), handed out
@@ -133,24 +153,31 @@ class PinnedRing:
         """Copy several host tensors to ``device`` on ``stream`` with one copy
         through the ring: they are staged at 256-byte aligned offsets of one
         piece and come back as views of one device buffer (each its own
-        pinned copy if the piece exceeds the ring)."""
-        tensors = [t.contiguous() for t in tensors]
-        sizes = [t.numel() * t.element_size() for t in tensors]
+        pinned copy if the piece exceeds the ring).  An entry may be a
+        :class:`Converted` instead of a tensor: its source is converted into
+        the staging piece in the same pass that stages it."""
+        items = [t if isinstance(t, Converted) else t.contiguous() for t in tensors]
+        metas = [(t.dtype, tuple(t.shape)) for t in items]
+        sizes = [int(np.prod(shape, dtype=np.int64)) * torch.empty(0, dtype=dt).element_size() for dt, shape in metas]
         offs, total = [], 0
         for n in sizes:
             offs.append(total)
             total += (n + 255) // 256 * 256
         start = self._take(total) if total else None
         if start is None:
-            return [self.upload(t, device, stream) for t in tensors]
-        for t, o, n in zip(tensors, offs, sizes):
-            self.buf[start + o:start + o + n].copy_(t.view(-1).view(torch.uint8))
+            return [self.upload(t.materialise() if isinstance(t, Converted) else t, device, stream) for t in items]
+        for t, o, n, (dt, shape) in zip(items, offs, sizes, metas):
+            piece = self.buf[start + o:start + o + n]
+            if isinstance(t, Converted):
+                t.write_into(piece.view(dt).view(shape))
+            else:
+                piece.copy_(t.view(-1).view(torch.uint8))
         with torch.cuda.stream(stream):
             dev_buf = self.buf[start:start + total].to(device, non_blocking=True)
             ev = torch.cuda.Event()
             ev.record(stream)
         self.fences.append((start, start + total, ev))
-        return [dev_buf[o:o + n].view(t.dtype).view(t.shape) for t, o, n in zip(tensors, offs, sizes)]
+        return [dev_buf[o:o + n].view(dt).view(shape) for o, n, (dt, shape) in zip(offs, sizes, metas)]
 
     def upload(self, t, device, stream):
         """Copy the host tensor ``t`` to ``device`` on ``stream`` through the ring
@@ -170,33 +197,54 @@ class PinnedRing:
         return out
 
 
+class DevicePrep:
+    """One graph's device preprocessing, started: the padded CSR, the
+    normalising degrees and the padded BF16 features are on their way up in
+    one copy (through ``ring``, a :class:`PinnedRing`, when given; the
+    features converted to BF16 while they are staged) and the window plan is
+    queued behind them on ``stream``, its sizes coming back behind an event.
+    :meth:`ready` polls the sizes; :meth:`finish` fills both window-block
+    forms from the plan."""
+
+    def __init__(self, csr, features, device, stream, part_size=SERVE_PART_SIZE, ring=None):
+        p = csr.padded(WINDOW)
+        d = G.gcn_norm_degree(p)
+        d[~np.isfinite(d)] = 0.0
+        X = Converted(torch.from_numpy(np.asarray(features, np.float32)), torch.bfloat16, p.n)
+        host = [torch.from_numpy(p.rowptr32()), torch.from_numpy(p.indices),
+                torch.from_numpy(np.ascontiguousarray(d, np.float32)), X]
+        self.csr, self.p, self.stream = csr, p, stream
+        with torch.cuda.stream(stream):
+            if ring is not None:
+                self.rowptr, self.ccol, self.dd, self.X = ring.upload_many(host, device, stream)
+            else:
+                self.rowptr, self.ccol, self.dd, self.X = [
+                    (t.materialise() if isinstance(t, Converted) else t).pin_memory().to(device, non_blocking=True)
+                    for t in host]
+            self.plan = ops.DevicePlan(self.rowptr, self.ccol, WINDOW, WIDE, part_size, emit_empty=True, sync=False)
+
+    def ready(self):
+        return self.plan.ready()
+
+    def finish(self):
+        """The built graph (the dict :class:`CachedGraph` takes)."""
+        p = self.p
+        with torch.cuda.stream(self.stream):
+            gb = self.plan.fill("deg_f32", self.dd, torch.bfloat16)
+            # every row lies in a window, so the edge-id map holds each edge at most once
+            mb = self.plan.fill("map", None, torch.int32, compact=True, n_values=p.nnz, positions=False)
+            gcn, gat = _dense_parts(gb), _compact_parts(mb, n_values=p.nnz)
+        return {"n": self.csr.n, "n_pad": p.n, "nnz": self.csr.nnz, "gcn": gcn,
+                "gat": {"rowptr": self.rowptr, "ccol": self.ccol, **gat}, "X": self.X}
+
+
 def prepare_device(csr, features, device, stream, part_size=SERVE_PART_SIZE, ring=None):
-    """Device-side preprocessing of one graph for both models on `stream`: the
-    padded CSR, the normalising degrees and the padded BF16 features go up in
-    one copy (through ``ring``, a :class:`PinnedRing`, when given), the tile
-    values d_i d_j are formed by the fill from the degrees (the same float32
-    products as the host's), and both window-block forms are built from one
-    device plan (one read-back of its sizes).  Bit-identical to
-    :func:`prepare_host`."""
-    p = csr.padded(WINDOW)
-    d = G.gcn_norm_degree(p)
-    d[~np.isfinite(d)] = 0.0
-    Xh = torch.zeros((p.n, F_IN), dtype=torch.bfloat16)
-    Xh[: csr.n] = torch.from_numpy(np.asarray(features, np.float32))
-    host = [torch.from_numpy(p.rowptr32()), torch.from_numpy(p.indices),
-            torch.from_numpy(np.ascontiguousarray(d, np.float32)), Xh]
-    with torch.cuda.stream(stream):
-        if ring is not None:
-            rowptr, ccol, dd, Xd = ring.upload_many(host, device, stream)
-        else:
-            rowptr, ccol, dd, Xd = [t.pin_memory().to(device, non_blocking=True) for t in host]
-        plan = ops.DevicePlan(rowptr, ccol, WINDOW, WIDE, part_size, emit_empty=True)
-        gb = plan.fill("deg_f32", dd, torch.bfloat16)
-        # every row lies in a window, so the edge-id map holds each edge at most once
-        mb = plan.fill("map", None, torch.int32, compact=True, n_values=p.nnz)
-        gcn, gat = _dense_parts(gb), _compact_parts(mb, n_values=p.nnz)
-    return {"n": csr.n, "n_pad": p.n, "nnz": csr.nnz, "gcn": gcn, "gat": {"rowptr": rowptr, "ccol": ccol, **gat},
-            "X": Xd}
+    """Device-side preprocessing of one graph for both models on `stream`
+    (:class:`DevicePrep` started and finished): the tile values d_i d_j are
+    formed by the fill from the degrees (the same float32 products as the
+    host's), and both window-block forms are built from one device plan (one
+    read-back of its sizes).  Bit-identical to :func:`prepare_host`."""
+    return DevicePrep(csr, features, device, stream, part_size, ring).finish()
 
 
 class CachedGraph:
@@ -270,14 +318,19 @@ class GraphCache:
         """Evict unpinned LRU entries until `need` more bytes fit; False if impossible."""
         if need + reserved > self.budget:
             return False
-        for key in list(self.entries):
-            if self.bytes + need + reserved <= self.budget:
-                break
-            cg = self.entries[key]
+        excess = self.bytes + need + reserved - self.budget
+        if excess <= 0:
+            return True
+        victims = []
+        for key, cg in self.entries.items():      # least recently used first
             if cg.pins == 0:
-                del self.entries[key]
-                self.bytes -= cg.nbytes
-                self.evictions += 1
+                victims.append(key)
+                excess -= cg.nbytes
+                if excess <= 0:
+                    break
+        for key in victims:
+            self.bytes -= self.entries.pop(key).nbytes
+            self.evictions += 1
         return self.bytes + need + reserved <= self.budget
 
     def put(self, cg):
@@ -573,7 +626,8 @@ class Request:
 class Server:
     def __init__(self, device, models, graph_source, hbm_budget=200 << 30,
                  capacities=(1 << 12, 1 << 14, 1 << 16, 1 << 18), use_graph=True, keep_results=False,
-                 max_in_flight=2, part_size=SERVE_PART_SIZE, preprocess="device", compute_streams=2):
+                 max_in_flight=2, part_size=SERVE_PART_SIZE, preprocess="device", compute_streams=2,
+                 prefetch=4):
         if preprocess not in ("host", "device"):
             raise ValueError(f"preprocess must be 'host' or 'device', got {preprocess!r}")
         self.device, self.models, self.source = device, models, graph_source
@@ -598,18 +652,71 @@ class Server:
         self.max_in_flight = max_in_flight
         self.batches = 0
         self.timing = collections.Counter()   # host seconds per phase (observability)
+        # device preparations of cache misses started ahead of their admission
+        # (key -> (DevicePrep, estimated bytes, reserved against the budget)):
+        # the queued requests nearest the head get their uploads and window
+        # plans queued while earlier work runs, so admitting them later does
+        # not wait for the plan's sizes
+        self.prefetch = int(prefetch) if preprocess == "device" else 0
+        self.preparing = collections.OrderedDict()
+
+    def _preparing_bytes(self):
+        return sum(est for _, est in self.preparing.values())
+
+    def _start(self, key):
+        """Start the device preparation of ``key``'s graph; False if its
+        bytes do not fit the budget now."""
+        csr, feats = self.source(key)
+        est = estimate_bytes(csr)
+        if not self.cache.make_room(est, self.reserved + self._preparing_bytes()):
+            return False
+        self.preparing[key] = (DevicePrep(csr, feats, self.device, self.copy_stream, self.part_size, self.ring), est)
+        return True
+
+    def _prefetch(self, queue):
+        """Start preparations for the first queued requests whose graphs are
+        neither cached nor being prepared, up to ``prefetch`` in progress
+        (looking at no more than 4x that many requests)."""
+        if len(self.preparing) >= self.prefetch:
+            return
+        for i, r in enumerate(queue):
+            if i >= 4 * self.prefetch or len(self.preparing) >= self.prefetch:
+                break
+            if r.key in self.preparing or r.key in self.cache.entries:
+                continue
+            if not self._start(r.key):
+                break
+
+    def _settle(self):
+        """Finish every preparation in progress into the cache."""
+        while self.preparing:
+            key, (prep, _) = self.preparing.popitem(last=False)
+            self.cache.put(CachedGraph(key, prep.finish(), self.device, self.copy_stream))
+
+    def reset_cache(self, budget_bytes=None):
+        """An empty cache (of ``budget_bytes``, default the current budget)
+        with no preparation in progress."""
+        self.preparing.clear()
+        self.cache = GraphCache(self.cache.budget if budget_bytes is None else budget_bytes)
 
     def _admit(self, req):
         """Graph of the request resident on the device, or None if it must wait."""
         cg = self.cache.get(req.key)
         if cg is not None:
             return cg
-        csr, feats = self.source(req.key)
-        if not self.cache.make_room(estimate_bytes(csr), self.reserved):
-            return None
         if self.preprocess == "device":
-            built = prepare_device(csr, feats, self.device, self.copy_stream, self.part_size, self.ring)
+            if req.key not in self.preparing and not self._start(req.key):
+                # graphs prepared ahead hold budget outside the cache: move
+                # them in (where they can be evicted) and try once more
+                self._settle()
+                if not self._start(req.key):
+                    return None
+            prep, _ = self.preparing.pop(req.key)
+            built = prep.finish()
         else:
+            csr, feats = self.source(req.key)
+            if not self.cache.make_room(estimate_bytes(csr), self.reserved):
+                return None
             built = prepare_host(csr, feats, self.part_size)
         cg = CachedGraph(req.key, built, self.device, self.copy_stream)
         self.cache.put(cg)
@@ -646,6 +753,9 @@ class Server:
                     if self.keep_results:
                         r.result = out[o:o + g.n].clone()
                 done += len(batch)
+            if self.prefetch:
+                for k in queue:
+                    self._prefetch(queue[k])
             if len(in_flight) < self.max_in_flight and (queue["gcn"] or queue["gat"]):
                 kind = min((k for k in queue if queue[k]), key=lambda k: queue[k][0].arrival)
                 batch, graphs = [], []

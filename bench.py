@@ -306,7 +306,7 @@ def serving_leg(dev, pool_size=1024, requests=4096, plain_requests=1024, cold_fr
     cold.prime()                               # bucket graphs captured; then the cache starts empty
     cold_res = {}
     for name, rate, seed in (("saturated", None, 21), ("3x_plain_capacity", 3 * cap_plain, 22)):
-        cold.cache = S.GraphCache(budget)
+        cold.reset_cache(budget)
         st = run(cold, requests, rate, seed)
         c = cold.cache
         cold_res[name] = {"req_per_s": round(st["req_per_s"], 1), "p50_ms": round(st["p50_ms"], 3),

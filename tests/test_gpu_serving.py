@@ -150,3 +150,35 @@ def test_prime_captures_every_bucket_and_keeps_results(cuda, setup):
     run_server(cuda, models, pool, eager, use_graph=False)
     for r, e in zip(reqs, eager):
         assert torch.equal(r.result, e.result), r.rid
+
+
+def test_prefetched_misses_serve_the_same_results(cuda, setup):
+    """Cache misses prepared ahead of their admission (uploads and window
+    plans queued early, sizes read back behind an event) serve the same
+    results as misses prepared at admission: bit for bit when every graph
+    fits (the same batches form), within BF16 tolerance under a budget of
+    about three graphs (where admission failures change the batches); the
+    budget holds and nothing is left in preparation."""
+    pool, models = setup
+    need = max(S.estimate_bytes(pool(k)[0]) for k in range(24))
+    for tight in (False, True):
+        outs = []
+        for prefetch in (0, 4):
+            srv = S.Server(cuda, models, pool, capacities=(1 << 14, 1 << 16), keep_results=True,
+                           hbm_budget=1 << 62, prefetch=prefetch)
+            if tight:
+                srv.cache.budget = srv.reserved + 3 * need
+            reqs = S.make_requests(40, None, 24, seed=14)
+            srv.run(reqs)
+            assert all(r.t_done is not None for r in reqs)
+            assert not srv.preparing
+            assert srv.cache.bytes + srv.reserved <= srv.cache.budget
+            if tight:
+                assert srv.cache.evictions > 0
+            outs.append(reqs)
+        for a, b in zip(*outs):
+            if tight:
+                err = ((a.result.float() - b.result.float()).norm() / b.result.float().norm()).item()
+                assert err < 1e-2, (a.rid, err)
+            else:
+                assert torch.equal(a.result, b.result), a.rid

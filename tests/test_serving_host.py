@@ -2,7 +2,9 @@
 request generation, latency statistics, graph pool, the admission estimate
 bounding the real footprint, and the pack tables: applied to CPU buffers they
 produce the window-block structure of the block-diagonal batch graph."""
+import collections
 import ctypes
+import types
 
 import numpy as np
 import pytest
@@ -197,3 +199,54 @@ def test_pack_tables_build_block_diagonal_structure(kind):
         assert np.array_equal(got_ids, e["values"].numpy()[: ebase[nb]])
         assert torch.equal(b.rowptr[: N + 1], torch.from_numpy(rowptr)) and bool((b.rowptr[N + 1:] == E).all())
         assert torch.equal(b.ccol[:E], torch.from_numpy(col))
+
+
+def test_converted_staging_rounds_like_torch_and_zero_pads():
+    """Features staged through Converted: converted to BF16 in the staging
+    pass with torch's round-to-nearest-even (the same bits as casting first),
+    rows past the source's own zero (also when the staging memory held
+    something before)."""
+    rng = np.random.default_rng(4)
+    f = rng.standard_normal((37, 128)).astype(np.float32) * 3.0
+    f[0, :4] = [1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8, -0.0, np.float32(3.0e38)]   # ties and edge values
+    src = torch.from_numpy(f)
+    c = S.Converted(src, torch.bfloat16, 48)
+    out = torch.full((48, 128), 7.0, dtype=torch.bfloat16)
+    c.write_into(out)
+    assert torch.equal(out[:37].view(torch.int16), src.to(torch.bfloat16).view(torch.int16))
+    assert not out[37:].float().any()
+    m = c.materialise()
+    assert m.shape == (48, 128) and m.dtype == torch.bfloat16 and torch.equal(m, out)
+
+
+def test_prefetch_selection():
+    """Server._prefetch starts preparations for the first queued requests
+    whose graphs are neither cached nor already in progress, keeps at most
+    ``prefetch`` in progress, looks at no more than 4x that many requests and
+    stops at the first graph that does not fit the budget."""
+    class Fake:
+        def __init__(self, prefetch, cached=(), room=99):
+            self.prefetch, self.preparing, self.started = prefetch, collections.OrderedDict(), []
+            self.cache = types.SimpleNamespace(entries=dict.fromkeys(cached))
+            self.room = room
+
+        def _start(self, key):
+            if self.room == 0:
+                return False
+            self.room -= 1
+            self.preparing[key] = (None, 1)
+            self.started.append(key)
+            return True
+
+    q = collections.deque(S.Request(i, 0.0, "gcn", k) for i, k in enumerate([5, 5, 1, 2, 3, 4, 6, 7]))
+    f = Fake(3, cached=[1])
+    S.Server._prefetch(f, q)
+    assert f.started == [5, 2, 3]          # 5 once, 1 cached
+    S.Server._prefetch(f, q)
+    assert f.started == [5, 2, 3]          # already 3 in progress
+    f = Fake(2, room=1)
+    S.Server._prefetch(f, q)
+    assert f.started == [5]                # the next one does not fit: stop
+    f = Fake(1, cached=[5, 1, 2, 3, 4])
+    S.Server._prefetch(f, q)
+    assert f.started == []                 # only the first 4 requests are looked at
