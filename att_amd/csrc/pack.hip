@@ -18,6 +18,8 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <cmath>
+#include <cstring>
 
 #include "attgnn.h"
 
@@ -98,3 +100,82 @@ extern "C" int attgnn_pack_segments(const void *segs, int64_t nsegs, const int32
 }
 
 extern "C" int64_t attgnn_pack_segment_bytes(void) { return int64_t(sizeof(PackSeg)); }
+
+// ---------------------------------------------------------------------------
+// Host staging of one serving graph: the CSR padded to whole windows, the
+// GCN normalising degrees and the features rounded to BF16, written into a
+// pinned buffer in the device layout and uploaded with one copy.
+
+namespace {
+
+int64_t stage_align(int64_t v) { return (v + 255) / 256 * 256; }
+
+// float bits -> bfloat16 bits, round to nearest even (as torch casts); NaN
+// -> the quiet NaN 0x7FC0.  A branch-free loop over distinct arrays, which the
+// host compiler vectorises.
+void to_bf16(const uint32_t *__restrict__ src, uint16_t *__restrict__ dst, int64_t n)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        const uint32_t u = src[i];
+        const uint32_t r = (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+        dst[i] = uint16_t((u & 0x7fffffffu) > 0x7f800000u ? 0x7fc0u : r);
+    }
+}
+
+}  // namespace
+
+extern "C" int64_t attgnn_serve_stage_layout(int64_t n, int64_t nnz, int64_t feat, int window, int64_t *offsets)
+{
+    if (n < 0 || nnz < 0 || nnz > INT32_MAX || feat <= 0 || window <= 0 || !offsets) return ATTGNN_E_ARG;
+    const int64_t n_pad = n ? (n + window - 1) / window * window : window;
+    offsets[0] = 0;                                       // row pointers, int32 [n_pad + 1]
+    offsets[1] = stage_align(4 * (n_pad + 1));            // column ids, int32 [nnz]
+    offsets[2] = offsets[1] + stage_align(4 * nnz);       // degrees d_i, float [n_pad]
+    offsets[3] = offsets[2] + stage_align(4 * n_pad);     // features, bf16 [n_pad][feat]
+    offsets[4] = n_pad;
+    return offsets[3] + stage_align(2 * n_pad * feat);
+}
+
+extern "C" int attgnn_serve_stage_host(const int64_t *indptr, const int32_t *indices, int64_t n, int64_t nnz,
+                                       const float *features, int64_t feat, int window, void *out,
+                                       int64_t out_bytes)
+{
+    int64_t off[5];
+    const int64_t total = attgnn_serve_stage_layout(n, nnz, feat, window, off);
+    if (total < 0) return int(total);
+    if (!indptr || !out || out_bytes < total || (nnz > 0 && !indices)) return ATTGNN_E_ARG;
+    if (indptr[0] != 0 || indptr[n] != nnz) return ATTGNN_E_ARG;
+    const int64_t n_pad = off[4];
+    uint8_t *p = static_cast<uint8_t *>(out);
+    int32_t *rp = reinterpret_cast<int32_t *>(p + off[0]);
+    float *d = reinterpret_cast<float *>(p + off[2]);
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t a = indptr[i], b = indptr[i + 1];
+        if (b < a) return ATTGNN_E_ARG;
+        rp[i] = int32_t(a);
+        // d_i = 1 / sqrt(row count) in float, 0 for an empty row
+        d[i] = b > a ? 1.0f / std::sqrt(float(b - a)) : 0.0f;
+    }
+    for (int64_t i = n; i <= n_pad; ++i) rp[i] = int32_t(nnz);
+    for (int64_t i = n; i < n_pad; ++i) d[i] = 0.0f;
+    if (nnz > 0) std::memcpy(p + off[1], indices, size_t(nnz) * 4);
+    uint16_t *x = reinterpret_cast<uint16_t *>(p + off[3]);
+    const int64_t nx = n * feat;
+    if (features) to_bf16(reinterpret_cast<const uint32_t *>(features), x, nx);
+    std::memset(x + nx, 0, size_t((n_pad - n) * feat) * 2);
+    return ATTGNN_OK;
+}
+
+extern "C" int attgnn_serve_stage(const int64_t *indptr, const int32_t *indices, int64_t n, int64_t nnz,
+                                  const float *features, int64_t feat, int window, void *pinned,
+                                  int64_t pinned_bytes, void *dev, attgnn_stream_t stream)
+{
+    if (!dev) return ATTGNN_E_ARG;
+    const int rc = attgnn_serve_stage_host(indptr, indices, n, nnz, features, feat, window, pinned, pinned_bytes);
+    if (rc != ATTGNN_OK) return rc;
+    int64_t off[5];
+    const int64_t total = attgnn_serve_stage_layout(n, nnz, feat, window, off);
+    const hipError_t e = hipMemcpyAsync(dev, pinned, size_t(total), hipMemcpyHostToDevice,
+                                        reinterpret_cast<hipStream_t>(stream));
+    return e == hipSuccess ? ATTGNN_OK : int(e);
+}

@@ -437,7 +437,7 @@ struct FillArgs {
 __global__ __launch_bounds__(THREADS) void window_fill_kernel(FillArgs a, Ws ws)
 {
     const int t = threadIdx.x;
-    if (blockIdx.x == 0 && t == 0) a.rp[0] = 0;
+    if (a.rp && blockIdx.x == 0 && t == 0) a.rp[0] = 0;
     const int W = a.window, wide = a.wide;
     const bool is_map = a.src == ATTGNN_SRC_MAP;
     for (int64_t i = blockIdx.x; i < a.nwin; i += gridDim.x) {
@@ -447,7 +447,8 @@ __global__ __launch_bounds__(THREADS) void window_fill_kernel(FillArgs a, Ws ws)
         const int64_t c0 = ws.ncol[i], pad = ws.ncol[i + 1] - c0;
         const int64_t v = ws.nuniq[i];
         const int64_t lo = a.row[i * W];
-        for (int64_t k = t; k < pad; k += THREADS) a.col_out[c0 + k] = k < v ? ws.uniq[lo + k] : -1;
+        if (a.col_out)
+            for (int64_t k = t; k < pad; k += THREADS) a.col_out[c0 + k] = k < v ? ws.uniq[lo + k] : -1;
         const int64_t s0 = c0 * W, ns = pad * W;
         if (is_map) {
             fill_range(a.val, s0, ns, 4, 0xFFFFFFFFu, t);
@@ -458,7 +459,7 @@ __global__ __launch_bounds__(THREADS) void window_fill_kernel(FillArgs a, Ws ws)
         // parts (example.cpp:112-152); every part but a window's last spans
         // part_size * wide columns
         const int64_t blocks = pad / wide;
-        for (int64_t k = t; k < np; k += THREADS) {
+        for (int64_t k = t; a.rp && k < np; k += THREADS) {
             int64_t real, padc;
             int at;
             if (blocks == 0) { real = 0; padc = 0; at = 0; }
@@ -480,8 +481,8 @@ __global__ __launch_bounds__(THREADS) void window_fill_kernel(FillArgs a, Ws ws)
             const int64_t pi = p0 + k;
             a.rp[2 * pi + 1] = int32_t(start + real);
             a.rp[2 * pi + 2] = int32_t(start + real + padc);
-            a.win[pi] = int32_t(i);
-            a.atomic[pi] = at;
+            if (a.win) a.win[pi] = int32_t(i);
+            if (a.atomic) a.atomic[pi] = at;
         }
         __syncthreads();   // tile initialisation before the scatter
         for (int p = 0; p < W; ++p) {
@@ -726,8 +727,9 @@ extern "C" int attgnn_blocks_fill_device(const int32_t *row, int64_t rows_plus1,
                                          int32_t *rp, int32_t *col_out, void *val, int32_t *map_t,
                                          int32_t *window_of_part, int32_t *atomic, attgnn_stream_t stream)
 {
-    if (!row || !workspace || !rp || rows_plus1 < 1 || window <= 0 || wide <= 0 || part_size <= 0 || nnz < 0)
+    if (!row || !workspace || rows_plus1 < 1 || window <= 0 || wide <= 0 || part_size <= 0 || nnz < 0)
         return ATTGNN_E_ARG;
+    if (!rp && (window_of_part || atomic)) return ATTGNN_E_ARG;   // the part table comes with rp
     if (src < ATTGNN_SRC_DEG_F16 || src > ATTGNN_SRC_MAP) return ATTGNN_E_ARG;
     const bool is_map = src == ATTGNN_SRC_MAP;
     if (is_map ? out_type != ATTGNN_I32

@@ -238,12 +238,16 @@ class DevicePlan:
             self._pending = None
             self._sizes([int(x) for x in host[:6].tolist()])
 
-    def fill(self, src, src_values, out_dtype, want_trans=False, compact=False, n_values=None, positions=True):
+    def fill(self, src, src_values, out_dtype, want_trans=False, compact=False, n_values=None, positions=True,
+             tiles_only=False):
         """One structure of this plan (keys as :func:`blocks_build`).
         ``n_values``: an upper bound of the compact value count when the
         caller knows one (an edge-id map holds each CSR entry at most once:
         nnz), which saves its read-back; see :func:`compact_tiles_device`.
-        ``positions=False`` skips the compact form's per-entry positions."""
+        ``positions=False`` skips the compact form's per-entry positions.
+        ``tiles_only``: no part table and no condensed columns (they equal
+        those of every other fill of this plan); the result lacks rp, col,
+        window and atomic."""
         self.wait()
         if src not in _SRCS:
             raise ValueError(f"unknown src {src!r}")
@@ -264,20 +268,19 @@ class DevicePlan:
                 raise TypeError(f"tile type {out_dtype} is not supported")
             keep = src_values
             sp = _ptr(keep)
-        out = {
-            "rp": torch.empty(self.n_rp, dtype=torch.int32, device=dev),
-            "col": torch.empty(self.n_col, dtype=torch.int32, device=dev),
-            "val": torch.empty(self.n_val, dtype=out_dtype, device=dev),
-            "window": torch.empty(self.n_parts, dtype=torch.int32, device=dev),
-            "atomic": torch.empty(self.n_parts, dtype=torch.int32, device=dev),
-            "map_t": torch.empty(self.n_val, dtype=torch.int32, device=dev) if want_trans else None,
-            "n_unique": self.n_unique,
-        }
+        out = {"val": torch.empty(self.n_val, dtype=out_dtype, device=dev),
+               "map_t": torch.empty(self.n_val, dtype=torch.int32, device=dev) if want_trans else None,
+               "n_unique": self.n_unique}
+        if not tiles_only:
+            out.update(rp=torch.empty(self.n_rp, dtype=torch.int32, device=dev),
+                       col=torch.empty(self.n_col, dtype=torch.int32, device=dev),
+                       window=torch.empty(self.n_parts, dtype=torch.int32, device=dev),
+                       atomic=torch.empty(self.n_parts, dtype=torch.int32, device=dev))
         check(lib().attgnn_blocks_fill_device(_ptr(self.row), self.row.numel(), _ptr(self.col), self.col.numel(),
                                               _SRCS[src], sp, _TYPE_OF[out_dtype], self.window, self.wide,
-                                              self.part_size, _ptr(self.ws), self.ws_n, _ptr(out["rp"]),
-                                              _ptr(out["col"]), _ptr(out["val"]), _ptr(out["map_t"]),
-                                              _ptr(out["window"]), _ptr(out["atomic"]), _stream(dev)),
+                                              self.part_size, _ptr(self.ws), self.ws_n, _ptr(out.get("rp")),
+                                              _ptr(out.get("col")), _ptr(out["val"]), _ptr(out["map_t"]),
+                                              _ptr(out.get("window")), _ptr(out.get("atomic")), _stream(dev)),
               "attgnn_blocks_fill_device")
         if compact:
             if self.wide != 32 or self.window > 16:

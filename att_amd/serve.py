@@ -97,31 +97,12 @@ def prepare_host(csr, features, part_size=SERVE_PART_SIZE):
             "X": torch.from_numpy(X).to(torch.bfloat16)}
 
 
-class Converted:
-    """A host tensor to be staged as ``dtype`` with ``rows`` rows: its rows
-    converted (torch's round-to-nearest-even casts) and the rows past its own
-    zero."""
-
-    def __init__(self, src, dtype, rows):
-        self.src, self.dtype = src, dtype
-        self.shape = (int(rows),) + tuple(src.shape[1:])
-
-    def write_into(self, out):
-        n = self.src.shape[0]
-        out[:n].copy_(self.src)
-        out[n:].zero_()
-
-    def materialise(self):
-        out = torch.empty(self.shape, dtype=self.dtype)
-        self.write_into(out)
-        return out
-
-
 class PinnedRing:
     """Page-locked staging for host-to-device uploads: one pinned buffer
     allocated once (pinning per upload costs more than the copy), handed out
     round-robin in 256-byte aligned pieces.  A piece is reused only after the
-    uploads that read it have finished (an event recorded after them)."""
+    uploads that read it have finished (:meth:`fence`: an event recorded
+    after them)."""
 
     def __init__(self, nbytes=64 << 20, pin=True):
         self.buf = torch.empty(nbytes, dtype=torch.uint8, pin_memory=pin)
@@ -149,78 +130,58 @@ class PinnedRing:
         self.pos = end
         return start
 
-    def upload_many(self, tensors, device, stream):
-        """Copy several host tensors to ``device`` on ``stream`` with one copy
-        through the ring: they are staged at 256-byte aligned offsets of one
-        piece and come back as views of one device buffer (each its own
-        pinned copy if the piece exceeds the ring).  An entry may be a
-        :class:`Converted` instead of a tensor: its source is converted into
-        the staging piece in the same pass that stages it."""
-        items = [t if isinstance(t, Converted) else t.contiguous() for t in tensors]
-        metas = [(t.dtype, tuple(t.shape)) for t in items]
-        sizes = [int(np.prod(shape, dtype=np.int64)) * torch.empty(0, dtype=dt).element_size() for dt, shape in metas]
-        offs, total = [], 0
-        for n in sizes:
-            offs.append(total)
-            total += (n + 255) // 256 * 256
-        start = self._take(total) if total else None
-        if start is None:
-            return [self.upload(t.materialise() if isinstance(t, Converted) else t, device, stream) for t in items]
-        for t, o, n, (dt, shape) in zip(items, offs, sizes, metas):
-            piece = self.buf[start + o:start + o + n]
-            if isinstance(t, Converted):
-                t.write_into(piece.view(dt).view(shape))
-            else:
-                piece.copy_(t.view(-1).view(torch.uint8))
-        with torch.cuda.stream(stream):
-            dev_buf = self.buf[start:start + total].to(device, non_blocking=True)
-            ev = torch.cuda.Event()
-            ev.record(stream)
-        self.fences.append((start, start + total, ev))
-        return [dev_buf[o:o + n].view(dt).view(shape) for o, n, (dt, shape) in zip(offs, sizes, metas)]
-
-    def upload(self, t, device, stream):
-        """Copy the host tensor ``t`` to ``device`` on ``stream`` through the ring
-        (a pinned copy of its own if it exceeds the ring)."""
-        t = t.contiguous()
-        n = t.numel() * t.element_size()
-        start = self._take(n)
-        if start is None:
-            return t.pin_memory().to(device, non_blocking=True)
-        view = self.buf[start:start + n].view(t.dtype).view(t.shape)
-        view.copy_(t)
-        with torch.cuda.stream(stream):
-            out = view.to(device, non_blocking=True)
-            ev = torch.cuda.Event()
-            ev.record(stream)
-        self.fences.append((start, start + (n + 255) // 256 * 256, ev))
-        return out
+    def fence(self, start, nbytes, stream):
+        """Mark the piece at ``start`` in use by work queued on ``stream`` so far."""
+        ev = torch.cuda.Event()
+        ev.record(stream)
+        self.fences.append((start, start + (nbytes + 255) // 256 * 256, ev))
 
 
 class DevicePrep:
-    """One graph's device preprocessing, started: the padded CSR, the
-    normalising degrees and the padded BF16 features are on their way up in
-    one copy (through ``ring``, a :class:`PinnedRing`, when given; the
-    features converted to BF16 while they are staged) and the window plan is
-    queued behind them on ``stream``, its sizes coming back behind an event.
-    :meth:`ready` polls the sizes; :meth:`finish` fills both window-block
-    forms from the plan."""
+    """One graph's device preprocessing, started: attgnn_serve_stage writes the
+    CSR padded to whole windows, the normalising degrees and the features
+    rounded to BF16 into pinned memory (a piece of ``ring``, a
+    :class:`PinnedRing`, when given) and uploads them in one copy; the window
+    plan is queued behind it on ``stream``, its sizes coming back behind an
+    event.  :meth:`ready` polls the sizes; :meth:`finish` fills both
+    window-block forms from the plan."""
 
     def __init__(self, csr, features, device, stream, part_size=SERVE_PART_SIZE, ring=None):
-        p = csr.padded(WINDOW)
-        d = G.gcn_norm_degree(p)
-        d[~np.isfinite(d)] = 0.0
-        X = Converted(torch.from_numpy(np.asarray(features, np.float32)), torch.bfloat16, p.n)
-        host = [torch.from_numpy(p.rowptr32()), torch.from_numpy(p.indices),
-                torch.from_numpy(np.ascontiguousarray(d, np.float32)), X]
-        self.csr, self.p, self.stream = csr, p, stream
+        L = _lib.lib()
+        indptr = np.ascontiguousarray(csr.indptr, np.int64)
+        indices = np.ascontiguousarray(csr.indices, np.int32)
+        feats = np.ascontiguousarray(features, np.float32)
+        if feats.shape != (csr.n, F_IN):
+            raise ValueError(f"features must be [{csr.n}, {F_IN}], got {feats.shape}")
+        off = (ctypes.c_int64 * 5)()
+        total = L.attgnn_serve_stage_layout(csr.n, csr.nnz, F_IN, WINDOW, off)
+        _lib.check(int(total) if total < 0 else 0, "attgnn_serve_stage_layout")
+        # the staged bytes: a piece of the ring, else a pinned buffer of their
+        # own kept until the plan's sizes arrive (the upload precedes the plan)
+        start = ring._take(total) if ring is not None else None
+        self._host = None
+        if start is None:
+            self._host = torch.empty(total, dtype=torch.uint8, pin_memory=True)
+            hptr = self._host.data_ptr()
+        else:
+            hptr = ring.buf.data_ptr() + start
+        dev_in = torch.empty(total, dtype=torch.uint8, device=device)
+        # the features: torch's (multi-threaded) cast straight into the staging
+        # bytes; attgnn_serve_stage writes the rest and uploads the whole piece
+        pinned = self._host if start is None else ring.buf[start:start + total]
+        pinned[off[3]:off[3] + 2 * csr.n * F_IN].view(torch.bfloat16).view(csr.n, F_IN).copy_(torch.from_numpy(feats))
+        _lib.check(L.attgnn_serve_stage(indptr.ctypes.data, indices.ctypes.data, csr.n, csr.nnz, None,
+                                        F_IN, WINDOW, ctypes.c_void_p(hptr), total, ctypes.c_void_p(dev_in.data_ptr()),
+                                        ctypes.c_void_p(stream.cuda_stream)), "attgnn_serve_stage")
+        if start is not None:
+            ring.fence(start, total, stream)
+        n_pad = int(off[4])
+        self.rowptr = dev_in[off[0]:off[0] + 4 * (n_pad + 1)].view(torch.int32)
+        self.ccol = dev_in[off[1]:off[1] + 4 * csr.nnz].view(torch.int32)
+        self.dd = dev_in[off[2]:off[2] + 4 * n_pad].view(torch.float32)
+        self.X = dev_in[off[3]:off[3] + 2 * n_pad * F_IN].view(torch.bfloat16).view(n_pad, F_IN)
+        self.csr, self.n_pad, self.stream = csr, n_pad, stream
         with torch.cuda.stream(stream):
-            if ring is not None:
-                self.rowptr, self.ccol, self.dd, self.X = ring.upload_many(host, device, stream)
-            else:
-                self.rowptr, self.ccol, self.dd, self.X = [
-                    (t.materialise() if isinstance(t, Converted) else t).pin_memory().to(device, non_blocking=True)
-                    for t in host]
             self.plan = ops.DevicePlan(self.rowptr, self.ccol, WINDOW, WIDE, part_size, emit_empty=True, sync=False)
 
     def ready(self):
@@ -228,13 +189,18 @@ class DevicePrep:
 
     def finish(self):
         """The built graph (the dict :class:`CachedGraph` takes)."""
-        p = self.p
+        nnz = self.csr.nnz
         with torch.cuda.stream(self.stream):
             gb = self.plan.fill("deg_f32", self.dd, torch.bfloat16)
-            # every row lies in a window, so the edge-id map holds each edge at most once
-            mb = self.plan.fill("map", None, torch.int32, compact=True, n_values=p.nnz, positions=False)
-            gcn, gat = _dense_parts(gb), _compact_parts(mb, n_values=p.nnz)
-        return {"n": self.csr.n, "n_pad": p.n, "nnz": self.csr.nnz, "gcn": gcn,
+            # the map's part table and condensed columns are the value
+            # structure's (one plan): the GAT form shares them.  Every row lies
+            # in a window, so the map holds each edge at most once
+            mb = self.plan.fill("map", None, torch.int32, compact=True, n_values=nnz, positions=False,
+                                tiles_only=True)
+            mb.update({k: gb[k] for k in ("rp", "col", "window", "atomic")})
+            gcn, gat = _dense_parts(gb), _compact_parts(mb, n_values=nnz)
+        self._host = None
+        return {"n": self.csr.n, "n_pad": self.n_pad, "nnz": nnz, "gcn": gcn,
                 "gat": {"rowptr": self.rowptr, "ccol": self.ccol, **gat}, "X": self.X}
 
 
@@ -270,8 +236,11 @@ class CachedGraph:
         self.need = {"gcn": (self.n_pad, self.gcn["window"].numel(), self.gcn["col"].numel(), 0),
                      "gat": (self.n_pad, self.gat["window"].numel(), self.gat["col"].numel(),
                              max(self.nnz, vals_gat))}
-        self.nbytes = sum(t.numel() * t.element_size() for t in
-                          list(self.gcn.values()) + list(self.gat.values()) + [self.X])
+        # device bytes held, each array once (the GCN and GAT forms share the
+        # part table and the condensed columns)
+        held = {t.data_ptr(): t.numel() * t.element_size() for t in
+                list(self.gcn.values()) + list(self.gat.values()) + [self.X]}
+        self.nbytes = sum(held.values())
         # (device pointer, positions) of every array the pack table copies, a
         # position being one part (rp: 2 ints), column, 32-column block, packed
         # value, row, or edge; rp and the CSR row pointers drop their final

@@ -149,7 +149,10 @@ int attgnn_blocks_compact_export(const attgnn_blocks *b, uint16_t *masks, int64_
  *    and calls
  *  fill: with the same row, col, window, wide and part_size and the same
  *    workspace (unchanged since the plan); map_t (MAP only, may be NULL)
- *    receives the transposed-edge map.
+ *    receives the transposed-edge map.  col_out and the part table (rp, with
+ *    window_of_part and atomic) may be NULL, and are then not written: a
+ *    second fill of the same plan in another tile type needs only the tiles
+ *    (window_of_part / atomic require rp).
  * workspace: device memory of attgnn_blocks_device_workspace_bytes() bytes.
  * row[rows_plus1], col[nnz], src_ptr: device pointers.
  */
@@ -854,6 +857,34 @@ int attgnn_minmax_score_grads(const float *s0, const float *s1, int64_t n, int64
 int attgnn_pack_segments(const void *segs, int64_t nsegs, const int32_t *chunk_seg, int64_t n_chunks,
                          int64_t chunk_bytes, attgnn_stream_t stream);
 int64_t attgnn_pack_segment_bytes(void);
+
+/*
+ * Host staging of one serving graph (no counterpart in the reference, which
+ * serves nothing).  attgnn_serve_stage_layout: the byte offsets, in one
+ * buffer, of the CSR row pointers padded to whole windows (int32
+ * [n_pad + 1], padding rows empty; offsets[0]), the column ids (int32 [nnz];
+ * offsets[1]), the degrees d_i = 1 / sqrt(row count) (float [n_pad], 0 for an
+ * empty row; offsets[2]) and the features (bf16 [n_pad][feat], padding rows
+ * zero; offsets[3]), each 256-byte aligned, with n_pad = n rounded up to a
+ * multiple of window (window for n = 0) in offsets[4]; returns the total
+ * bytes, or ATTGNN_E_ARG.
+ * attgnn_serve_stage_host: writes that buffer into the host memory `out` (at
+ * least the total bytes) from a CSR (indptr int64 [n + 1], indptr[0] = 0,
+ * indptr[n] = nnz; indices int32 [nnz]) and float features [n][feat]
+ * (rounded to nearest even as torch casts; NaN to the quiet NaN 0x7FC0);
+ * host work only.  features NULL: the feature rows [0, n) are left as the
+ * caller wrote them (a multi-threaded cast is faster than this single
+ * thread); the padding rows are zeroed either way.
+ * attgnn_serve_stage: the same into pinned host memory `pinned`, then one
+ * hipMemcpyAsync of the buffer to `dev` on `stream`.  The caller keeps
+ * `pinned` untouched until that copy has finished.
+ */
+int64_t attgnn_serve_stage_layout(int64_t n, int64_t nnz, int64_t feat, int window, int64_t *offsets);
+int attgnn_serve_stage_host(const int64_t *indptr, const int32_t *indices, int64_t n, int64_t nnz,
+                            const float *features, int64_t feat, int window, void *out, int64_t out_bytes);
+int attgnn_serve_stage(const int64_t *indptr, const int32_t *indices, int64_t n, int64_t nnz,
+                       const float *features, int64_t feat, int window, void *pinned, int64_t pinned_bytes,
+                       void *dev, attgnn_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 import torch
 
+from att_amd import _lib
 from att_amd import graphs as G
 from att_amd import ops
 from att_amd import serve as S
@@ -201,22 +202,64 @@ def test_pack_tables_build_block_diagonal_structure(kind):
         assert torch.equal(b.ccol[:E], torch.from_numpy(col))
 
 
-def test_converted_staging_rounds_like_torch_and_zero_pads():
-    """Features staged through Converted: converted to BF16 in the staging
-    pass with torch's round-to-nearest-even (the same bits as casting first),
-    rows past the source's own zero (also when the staging memory held
-    something before)."""
+def test_serve_stage_host_matches_numpy_and_torch():
+    """attgnn_serve_stage_host (host work only): the CSR padded to whole
+    windows (empty padding rows), d_i = 1/sqrt(row count) in float32 (0 for
+    empty rows, as graphs.gcn_norm_degree with the non-finite entries
+    zeroed), features rounded to BF16 exactly as torch casts (ties to even;
+    a NaN stays a NaN, whose bits torch's casts do not fix) with zero padding
+    rows, at the offsets
+    attgnn_serve_stage_layout reports; bad arguments are refused."""
+    E_ARG = -1   # ATTGNN_E_ARG
+    L = _lib.lib()
+    g0 = G.chung_lu(37, 6, 2.1, seed=5)
+    # row 3 emptied (its degree must come out 0, not inf)
+    keep = np.ones(g0.nnz, bool)
+    keep[g0.indptr[3]:g0.indptr[4]] = False
+    counts = np.diff(g0.indptr)
+    counts[3] = 0
+    g = G.CSR(np.concatenate([[0], np.cumsum(counts)]).astype(np.int64), g0.indices[keep].astype(np.int32), g0.n)
+    indptr = g.indptr
     rng = np.random.default_rng(4)
-    f = rng.standard_normal((37, 128)).astype(np.float32) * 3.0
-    f[0, :4] = [1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8, -0.0, np.float32(3.0e38)]   # ties and edge values
-    src = torch.from_numpy(f)
-    c = S.Converted(src, torch.bfloat16, 48)
-    out = torch.full((48, 128), 7.0, dtype=torch.bfloat16)
-    c.write_into(out)
-    assert torch.equal(out[:37].view(torch.int16), src.to(torch.bfloat16).view(torch.int16))
-    assert not out[37:].float().any()
-    m = c.materialise()
-    assert m.shape == (48, 128) and m.dtype == torch.bfloat16 and torch.equal(m, out)
+    f = rng.standard_normal((g.n, 128)).astype(np.float32) * 3.0
+    f[0, :6] = [1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8, -0.0, np.float32(3.3e38), np.nan, -np.inf]
+    for window in (16, 8):
+        off = (ctypes.c_int64 * 5)()
+        total = L.attgnn_serve_stage_layout(g.n, g.nnz, 128, window, off)
+        n_pad = off[4]
+        assert n_pad == -(-g.n // window) * window and all(o % 256 == 0 for o in off[:4]) and total >= off[3]
+        out = np.full(total, 0xAB, np.uint8)
+        rc = L.attgnn_serve_stage_host(indptr.ctypes.data, g.indices.ctypes.data, g.n, g.nnz, f.ctypes.data, 128,
+                                       window, out.ctypes.data, total)
+        assert rc == 0
+        p = g.padded(window)
+        assert np.array_equal(out[off[0]:off[0] + 4 * (n_pad + 1)].view(np.int32), p.rowptr32())
+        assert np.array_equal(out[off[1]:off[1] + 4 * g.nnz].view(np.int32), g.indices)
+        d = G.gcn_norm_degree(p)
+        d[~np.isfinite(d)] = 0.0
+        assert np.array_equal(out[off[2]:off[2] + 4 * n_pad].view(np.float32).view(np.uint32), d.view(np.uint32))
+        x = out[off[3]:off[3] + 2 * n_pad * 128].view(np.uint16).reshape(n_pad, 128)
+        want = torch.from_numpy(f).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+        nan = np.isnan(f)
+        assert np.array_equal(x[:g.n][~nan], want[~nan])
+        assert np.all((x[:g.n][nan] & 0x7F80) == 0x7F80) and np.all((x[:g.n][nan] & 0x7F) != 0)   # NaN stays NaN
+        assert x[0, 5] == 0xFF80 and x[0, 3] == 0x7F78 and x[0, 1] == 0x3F82 and x[0, 0] == 0x3F80
+        assert not x[g.n:].any()
+        assert out[off[2] + 4 * 3:off[2] + 4 * 4].view(np.float32)[0] == 0.0
+    # features NULL: the rows the caller wrote stay, the padding rows are zeroed
+    out = np.full(total, 0x5A, np.uint8)
+    assert L.attgnn_serve_stage_host(indptr.ctypes.data, g.indices.ctypes.data, g.n, g.nnz, None, 128, 8,
+                                     out.ctypes.data, total) == 0
+    x = out[off[3]:off[3] + 2 * off[4] * 128].view(np.uint16).reshape(off[4], 128)
+    assert np.all(x[:g.n] == 0x5A5A) and not x[g.n:].any()
+    # refused: buffer too small, indptr not ending at nnz, a negative count
+    small = np.zeros(16, np.uint8)
+    assert L.attgnn_serve_stage_host(indptr.ctypes.data, g.indices.ctypes.data, g.n, g.nnz, f.ctypes.data, 128, 16,
+                                     small.ctypes.data, 16) == E_ARG
+    out = np.zeros(total, np.uint8)
+    assert L.attgnn_serve_stage_host(indptr.ctypes.data, g.indices.ctypes.data, g.n, g.nnz - 1, f.ctypes.data, 128,
+                                     8, out.ctypes.data, total) == E_ARG
+    assert L.attgnn_serve_stage_layout(-1, 0, 128, 16, off) < 0
 
 
 def test_prefetch_selection():
