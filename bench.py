@@ -20,6 +20,8 @@ the HBM roof.  The per-window gather figure 4(M+1) + nnz*(4+2) + U*256*2 +
 M*256*2 (U = distinct columns summed over 16-row windows, the rows the kernel
 gathers) is reported beside it as ``gather_bytes``: most of those gathers are
 served by the L2s and the Infinity Cache, so it can exceed the HBM roof.
+``measured_copy_gbs`` is this GPU's device-to-device copy bandwidth (a 1 GiB
+copy, read plus write), reported beside the 8 TB/s datasheet peak.
 ``traffic`` is the measured L2-to-fabric bytes per launch from the committed
 rocprofv3 FETCH_SIZE / WRITE_SIZE passes (copy-calibrated), used only when
 they were taken on the same kernel sources (a digest of them is stored with
@@ -124,6 +126,25 @@ def measured_traffic():
         return None, "counters were taken on other kernel sources"
     b = int((t["fetch_kib"] * t["fetch_factor"] + t["write_kib"] * t["write_factor"]) * 1024)
     return b, f"rocprofv3 FETCH_SIZE x {t['fetch_factor']} + WRITE_SIZE x {t['write_factor']} ({t['source']})"
+
+
+def copy_bandwidth(dev, nbytes=1 << 30, reps=10):
+    """Device-to-device copy bandwidth on this GPU (GB/s): a 1 GiB tensor
+    copied ``reps`` times after a warm-up, timed with HIP events, counting
+    each copy's read and write (2 x nbytes)."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return 2 * nbytes / (ms * 1e-3) / 1e9
 
 
 def plain_gcn_step_ms(A32, n, dev, steps=3):
@@ -450,6 +471,7 @@ def main():
     ev1.record(stream)
     torch.cuda.synchronize()
     t_spmm = ev0.elapsed_time(ev1) / reps  # ms
+    copy_gbs = copy_bandwidth(dev)
     M = graph.num_nodes if graph.stripe else g.n    # rows of Y written (striped: padding rows too)
     P, C = graph.fwd["parts"], graph.fwd["n_cols"]
     # compulsory bytes of one launch: X once, the compact structure as read
@@ -533,6 +555,7 @@ def main():
                    **workload_id},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "measured_copy_gbs": round(copy_gbs, 1),
                      "kernel": ("window-block SpMM, width 256, fp16, compact tiles, striped node order with "
                                 f"{graph.stripe} owners (owner-split main kernel + tile combine)" if graph.stripe else
                                 "window-block SpMM, width 256, fp16, compact tiles (main + split-window combine)"),
