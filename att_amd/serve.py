@@ -165,7 +165,10 @@ class DevicePrep:
             hptr = self._host.data_ptr()
         else:
             hptr = ring.buf.data_ptr() + start
-        dev_in = torch.empty(total, dtype=torch.uint8, device=device)
+        # allocated on the stream that writes it (the raw upload below is
+        # invisible to torch's allocator, which orders reuse by that stream)
+        with torch.cuda.stream(stream):
+            dev_in = torch.empty(total, dtype=torch.uint8, device=device)
         # the features: torch's (multi-threaded) cast straight into the staging
         # bytes; attgnn_serve_stage writes the rest and uploads the whole piece
         pinned = self._host if start is None else ring.buf[start:start + total]
