@@ -102,7 +102,7 @@ def cpu_baseline(g, vals, dim=256, stride=1, warmups=3, runs=10):
 
 
 KERNEL_SOURCES = ("att_amd/csrc/spmm.hip", "att_amd/csrc/mfma.hpp", "att_amd/csrc/halfconv.hpp",
-                  "att_amd/csrc/attn.hpp", "include/attgnn.h")
+                  "att_amd/csrc/attn.hpp")
 
 
 def kernel_sources_digest():
