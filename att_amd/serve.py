@@ -828,24 +828,44 @@ class Server:
 class PlainServer:
     """One request at a time, FIFO, plain FP32 PyTorch: torch.sparse CSR SpMM
     for GCN, gather / scatter-softmax / index_add for GAT; structures and
-    features cached on the device as well."""
+    features cached on the device as well -- all of them, or with
+    ``budget_bytes`` an LRU of them under that many device bytes."""
 
-    def __init__(self, device, models, graph_source):
+    def __init__(self, device, models, graph_source, budget_bytes=None):
         self.device, self.models, self.source = device, models, graph_source
-        self.cache = {}
+        self.cache = collections.OrderedDict()
+        self.budget = budget_bytes
+        self.bytes = 0
+        self.misses = self.evictions = 0
+
+    @staticmethod
+    def entry_bytes(c):
+        return sum(t.numel() * t.element_size() for t in
+                   (c["A"].crow_indices(), c["A"].col_indices(), c["A"].values(), c["rows"], c["cols"], c["X"]))
 
     def _get(self, key):
-        if key not in self.cache:
-            from .benchmarks.plain import csr_tensor
-            csr, feats = self.source(key)
-            vals = G.gcn_norm_values(csr)
-            self.cache[key] = {
-                "A": csr_tensor(csr, vals, self.device),
-                "rows": torch.from_numpy(csr.row_ids().astype(np.int64)).to(self.device),
-                "cols": torch.from_numpy(csr.indices.astype(np.int64)).to(self.device),
-                "X": torch.from_numpy(np.asarray(feats, np.float32)).to(torch.bfloat16).float().to(self.device),
-                "n": csr.n}
-        return self.cache[key]
+        c = self.cache.get(key)
+        if c is not None:
+            self.cache.move_to_end(key)
+            return c
+        from .benchmarks.plain import csr_tensor
+        csr, feats = self.source(key)
+        vals = G.gcn_norm_values(csr)
+        c = {"A": csr_tensor(csr, vals, self.device),
+             "rows": torch.from_numpy(csr.row_ids().astype(np.int64)).to(self.device),
+             "cols": torch.from_numpy(csr.indices.astype(np.int64)).to(self.device),
+             "X": torch.from_numpy(np.asarray(feats, np.float32)).to(torch.bfloat16).float().to(self.device),
+             "n": csr.n}
+        c["nbytes"] = self.entry_bytes(c)
+        self.misses += 1
+        if self.budget is not None:
+            while self.cache and self.bytes + c["nbytes"] > self.budget:
+                _, old = self.cache.popitem(last=False)
+                self.bytes -= old["nbytes"]
+                self.evictions += 1
+        self.cache[key] = c
+        self.bytes += c["nbytes"]
+        return c
 
     def forward(self, kind, c):
         from .benchmarks.plain import gat_fp32

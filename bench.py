@@ -289,7 +289,9 @@ def serving_leg(dev, pool_size=1024, requests=4096, plain_requests=1024, cold_fr
     Cold: a server whose HBM budget holds only ``cold_fraction`` of the pool's
     structures starts with an empty cache, so admission, device preprocessing
     and LRU eviction all run inside the timed requests; saturated and at 3x
-    the plain capacity, with its cache misses and evictions."""
+    the plain capacity, with its cache misses and evictions; beside it the
+    plain server saturated under the same rule (an LRU holding
+    ``cold_fraction`` of its own pool's device bytes)."""
     import torch
     from att_amd import serve as S
 
@@ -333,6 +335,15 @@ def serving_leg(dev, pool_size=1024, requests=4096, plain_requests=1024, cold_fr
         cold_res[name] = {"req_per_s": round(st["req_per_s"], 1), "p50_ms": round(st["p50_ms"], 3),
                           "p99_ms": round(st["p99_ms"], 3), "within_p99_bound": st["p99_ms"] <= bound,
                           "cache_hits": c.hits, "cache_misses": c.misses, "evictions": c.evictions}
+    # the plain server under the same rule: an LRU whose budget holds a
+    # quarter of its own pool's device bytes, the saturated request sequence
+    plain_total = sum(c["nbytes"] for c in plain.cache.values())
+    plain_cold = S.PlainServer(dev, models, pool, budget_bytes=int(cold_fraction * plain_total))
+    st = run(plain_cold, requests, None, 21)
+    cold_res["plain_fp32_saturated"] = {"req_per_s": round(st["req_per_s"], 1), "p99_ms": round(st["p99_ms"], 3),
+                                        "cache_misses": plain_cold.misses, "evictions": plain_cold.evictions,
+                                        "budget_bytes": plain_cold.budget, "pool_bytes": int(plain_total)}
+    cold_res["saturated_ratio_vs_plain_cold"] = round(cold_res["saturated"]["req_per_s"] / st["req_per_s"], 2)
     return {"workload": f"{requests} requests, 50/50 GCN 2-layer / GAT 4-head, pool of {pool_size} Chung-Lu graphs "
                         "(N ~ lognormal(2000, 1.2) in [100, 200000]), 128 bf16 features",
             "plain_fp32_req_per_s": round(cap_plain, 1), "batched_req_per_s": round(sat["req_per_s"], 1),

@@ -182,3 +182,24 @@ def test_prefetched_misses_serve_the_same_results(cuda, setup):
                 assert err < 1e-2, (a.rid, err)
             else:
                 assert torch.equal(a.result, b.result), a.rid
+
+
+def test_plain_server_budget_lru(cuda, setup):
+    """The plain per-graph server under a byte budget (the cold baseline):
+    it evicts least recently used graphs, stays within the budget and serves
+    the same results as with every graph resident (to 1e-5: the plain GAT
+    sums with index_add atomics, whose order varies run to run)."""
+    pool, models = setup
+    warm = S.PlainServer(cuda, models, pool)
+    ref = S.make_requests(30, None, 24, seed=15)
+    warm.run(ref)
+    total = sum(c["nbytes"] for c in warm.cache.values())
+    cold = S.PlainServer(cuda, models, pool, budget_bytes=total // 4)
+    reqs = S.make_requests(30, None, 24, seed=15)
+    cold.run(reqs)
+    assert cold.evictions > 0 and cold.misses > len(warm.cache) // 4
+    assert cold.bytes <= cold.budget
+    assert sum(c["nbytes"] for c in cold.cache.values()) == cold.bytes
+    for r, e in zip(reqs, ref):
+        err = ((r.result - e.result).norm() / e.result.norm()).item()
+        assert err < 1e-5, (r.rid, err)
