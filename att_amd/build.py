@@ -16,7 +16,8 @@ OUT_DIR = os.path.join(PKG, "_build")
 LIB = os.path.join(OUT_DIR, "libattgnn.so")
 INCLUDE = os.path.join(ROOT, "include")
 
-HIP_SOURCES = ["spmm.hip", "sddmm.hip", "edge.hip", "pack.hip", "preprocess_gpu.hip", "gatv2.hip", "transpose.hip"]
+HIP_SOURCES = ["spmm.hip", "sddmm.hip", "edge.hip", "pack.hip", "preprocess_gpu.hip", "gatv2.hip", "transpose.hip",
+               "loss.hip"]
 HOST_SOURCES = ["preprocess.cpp"]
 ARCH = "gfx950"
 

@@ -1,0 +1,159 @@
+// Negative log-likelihood of a row-wise log-softmax -- the GCN training loss
+// -- and its gradient in one pass over the logits.
+//
+// The reference's GCN takes log_softmax of the last layer
+// (end2end_ori/gcn_final/mgat_csr/gat_mgnn.py:39) and nll_loss against the
+// labels (gat_mgnn.py:66); in PyTorch that is a chain of a dozen small
+// kernels forward and backward over [nodes, classes].  Here 16 lanes take
+// one selected row (4 rows per wave, each lane every 16th class, so a row is
+// read and written in contiguous pieces), reduce its maximum and its sum of
+// exponentials across the 16 lanes, and form the row's loss term
+// lse - z[label] and the gradient row (softmax - one-hot) / n, all in FP32,
+// storing the gradient in the logits' type; the loss terms are summed per
+// workgroup and the workgroup sums in a fixed order by one more workgroup,
+// so the loss is the same bits run to run.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "attgnn.h"
+#include "halfconv.hpp"
+
+namespace {
+
+constexpr int LOSS_THREADS = 256;
+
+__device__ __forceinline__ float load_logit(const void *p, int type, int64_t i)
+{
+    if (type == ATTGNN_F32) return static_cast<const float *>(p)[i];
+    const uint16_t b = static_cast<const uint16_t *>(p)[i];
+    return type == ATTGNN_F16 ? attgnn::f16_to_f32(b) : attgnn::bf16_to_f32(b);
+}
+
+__device__ __forceinline__ void store_grad(void *p, int type, int64_t i, float v)
+{
+    if (type == ATTGNN_F32) static_cast<float *>(p)[i] = v;
+    else static_cast<uint16_t *>(p)[i] = type == ATTGNN_F16 ? attgnn::f32_to_f16(v) : attgnn::f32_to_bf16(v);
+}
+
+// Sum over a workgroup of LOSS_THREADS threads, fixed order.
+__device__ float block_sum(float v, float *red)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    float s = 0.f;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < LOSS_THREADS / 64; ++w) s += red[w];
+    return s;   // valid in thread 0
+}
+
+struct LossArgs {
+    const void *logits;
+    void *grad;                 // NULL: no gradient
+    const float *grad_scale;    // NULL: 1
+    const int64_t *labels, *rows;
+    float *partial;
+    int64_t ld, n_sel, n_rows;
+    float scale;
+    int type, classes;
+};
+
+constexpr int ROW_LANES = 16;                           // lanes per row
+constexpr int ROWS_PER_BLOCK = LOSS_THREADS / ROW_LANES;
+
+// max / sum over the 16 lanes of a row (xor partners stay inside the group)
+__device__ __forceinline__ float row_max(float v)
+{
+    for (int o = ROW_LANES / 2; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ float row_sum(float v)
+{
+    for (int o = ROW_LANES / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__global__ __launch_bounds__(LOSS_THREADS) void nll_log_softmax_kernel(LossArgs a)
+{
+    __shared__ float red[LOSS_THREADS / 64];
+    const int k = threadIdx.x & (ROW_LANES - 1);
+    const int64_t i = int64_t(blockIdx.x) * ROWS_PER_BLOCK + (threadIdx.x / ROW_LANES);
+    float term = 0.f;
+    // the whole 16-lane group takes the same branches (i, r, y are per row)
+    if (i < a.n_sel) {
+        const int64_t r = a.rows ? a.rows[i] : i;
+        if (r < 0 || r >= a.n_rows) {   // a row index outside the logits: NaN loss, nothing stored
+            term = k == 0 ? NAN : 0.f;
+        } else {
+            const int64_t y = a.labels[r];
+            const int64_t base = r * a.ld;
+            float m = -INFINITY;
+            for (int c = k; c < a.classes; c += ROW_LANES) m = fmaxf(m, load_logit(a.logits, a.type, base + c));
+            m = row_max(m);
+            float s = 0.f;
+            for (int c = k; c < a.classes; c += ROW_LANES) s += expf(load_logit(a.logits, a.type, base + c) - m);
+            s = row_sum(s);
+            // log-sum-exp = m + log(s), kept apart: (z - m) - log(s) loses no
+            // bits to the size of m
+            const float ls = logf(s);
+            const bool ok = y >= 0 && y < a.classes;
+            if (k == 0) term = ok ? (m - load_logit(a.logits, a.type, base + y)) + ls : NAN;
+            if (a.grad) {
+                const float gs = a.scale * (a.grad_scale ? a.grad_scale[0] : 1.f);
+                for (int c = k; c < a.classes; c += ROW_LANES) {
+                    const float p = expf((load_logit(a.logits, a.type, base + c) - m) - ls);
+                    store_grad(a.grad, a.type, base + c, ok ? (p - (c == y ? 1.f : 0.f)) * gs : NAN);
+                }
+            }
+        }
+    }
+    if (a.partial) {
+        const float s = block_sum(term, red);
+        if (threadIdx.x == 0) a.partial[blockIdx.x] = s;
+    }
+}
+
+__global__ __launch_bounds__(LOSS_THREADS) void loss_total_kernel(const float *partial, int64_t n, float scale,
+                                                                   float *loss)
+{
+    __shared__ float red[LOSS_THREADS / 64];
+    float v = 0.f;
+    for (int64_t k = threadIdx.x; k < n; k += LOSS_THREADS) v += partial[k];
+    const float s = block_sum(v, red);
+    if (threadIdx.x == 0) loss[0] = s * scale;
+}
+
+}  // namespace
+
+extern "C" int64_t attgnn_nll_log_softmax_workspace_elems(int64_t n_sel)
+{
+    return n_sel < 0 ? -1 : std::max<int64_t>(1, (n_sel + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK);
+}
+
+extern "C" int attgnn_nll_log_softmax(const void *logits, int logits_type, int64_t n_rows, int64_t ld, int classes,
+                                      const int64_t *labels, const int64_t *rows, int64_t n_sel, float *loss,
+                                      void *grad, const float *grad_scale, float *workspace,
+                                      int64_t workspace_elems, attgnn_stream_t stream)
+{
+    if (!(logits_type == ATTGNN_F32 || logits_type == ATTGNN_F16 || logits_type == ATTGNN_BF16)) return ATTGNN_E_ARG;
+    if (n_rows < 0 || n_sel < 0 || classes <= 0 || ld < classes || (!loss && !grad)) return ATTGNN_E_ARG;
+    if (!rows && n_sel > n_rows) return ATTGNN_E_ARG;
+    if (n_sel == 0) return ATTGNN_E_ARG;   // the mean of no rows
+    if (!logits || !labels) return ATTGNN_E_ARG;
+    if (loss && (!workspace || workspace_elems < attgnn_nll_log_softmax_workspace_elems(n_sel))) return ATTGNN_E_ARG;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    LossArgs a;
+    a.logits = logits; a.grad = grad; a.grad_scale = grad_scale; a.labels = labels; a.rows = rows;
+    a.partial = loss ? workspace : nullptr;
+    a.ld = ld; a.n_sel = n_sel; a.n_rows = n_rows; a.scale = 1.0f / float(n_sel); a.type = logits_type; a.classes = classes;
+    const int64_t blocks = attgnn_nll_log_softmax_workspace_elems(n_sel);
+    hipLaunchKernelGGL(nll_log_softmax_kernel, dim3(unsigned(blocks)), dim3(LOSS_THREADS), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !loss) return e == hipSuccess ? ATTGNN_OK : int(e);
+    hipLaunchKernelGGL(loss_total_kernel, dim3(1), dim3(LOSS_THREADS), 0, s, workspace, blocks, a.scale, loss);
+    e = hipGetLastError();
+    return e == hipSuccess ? ATTGNN_OK : int(e);
+}

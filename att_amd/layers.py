@@ -203,6 +203,35 @@ class GCNGraph:
                               out_dtype=out_dtype or X.dtype, rows_covered=True)
 
 
+class NllLogSoftmax(torch.autograd.Function):
+    """Mean NLL of log_softmax over the selected rows (ops.nll_log_softmax):
+    the loss forward, the gradient scaled by the upstream one backward, one
+    kernel each."""
+
+    @staticmethod
+    def forward(ctx, logits, labels, classes, rows):
+        logits = logits.contiguous()
+        loss, _ = ops.nll_log_softmax(logits, labels, classes, rows, grad=False)
+        ctx.save_for_backward(logits, labels, rows)
+        ctx.classes = classes
+        return loss
+
+    @staticmethod
+    def backward(ctx, g):
+        logits, labels, rows = ctx.saved_tensors
+        _, grad = ops.nll_log_softmax(logits, labels, ctx.classes, rows, loss=False,
+                                      grad_scale=g.detach().float().reshape(1))
+        return grad, None, None, None
+
+
+def nll_log_softmax_loss(logits, labels, classes, rows=None):
+    """The reference GCN's loss, F.nll_loss(F.log_softmax(logits[:, :classes]),
+    labels) (end2end_ori/gcn_final/mgat_csr/gat_mgnn.py:39,66), averaged over
+    ``rows`` (int64 indices; None: all rows), as one fused kernel forward and
+    none but a scaling backward."""
+    return NllLogSoftmax.apply(logits, labels, int(classes), rows)
+
+
 def weight_grad(X, dY, min_rows_per_split=2048, max_splits=64):
     """X^T dY (fp32, or fp64 for fp64 inputs) for a long row dimension N: a plain GEMM gets a handful of
     output tiles that each walk all N rows, so the rows are split into S chunks

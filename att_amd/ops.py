@@ -550,6 +550,48 @@ def stripe_segments(rp, window_of_part, col, owners, m):
     return seg
 
 
+def nll_log_softmax(logits, labels, classes, rows=None, loss=True, grad=True, grad_scale=None):
+    """attgnn_nll_log_softmax: (loss, grad) for logits [n, ld] (fp32 / fp16 /
+    bf16, the first ``classes`` columns being the classes), int64 labels [n]
+    and the int64 row indices ``rows`` the mean runs over (None: every row).
+    loss is the FP32 0-d mean of -log_softmax(logits[r, :classes])[labels[r]];
+    grad its gradient w.r.t. logits times ``grad_scale`` (a 0-d fp32 device
+    tensor, None: 1), in their shape and type, zero outside the selected rows
+    and the class columns.  ``loss=False`` / ``grad=False`` skip one (None in
+    its place).  Bad labels or row indices give NaN."""
+    want_loss, want_grad = loss, grad
+    if not (want_loss or want_grad):
+        raise ValueError("nothing to compute")
+    dev = _need_gpu("nll_log_softmax", logits, labels, *([rows] if rows is not None else []))
+    if logits.dim() != 2 or not logits.is_contiguous():
+        raise ValueError("logits must be a contiguous 2-D tensor")
+    if logits.dtype not in (torch.float32, torch.float16, torch.bfloat16):
+        raise TypeError(f"logits must be fp32 / fp16 / bf16, got {logits.dtype}")
+    n, ld = logits.shape
+    if labels.dtype != torch.int64 or labels.shape != (n,) or not labels.is_contiguous():
+        raise ValueError(f"labels must be a contiguous int64 [{n}] tensor")
+    if rows is not None and (rows.dtype != torch.int64 or rows.dim() != 1 or not rows.is_contiguous()):
+        raise ValueError("rows must be a contiguous 1-D int64 tensor")
+    if not 0 < classes <= ld:
+        raise ValueError(f"classes must be in [1, {ld}], got {classes}")
+    n_sel = n if rows is None else rows.numel()
+    if n_sel == 0:
+        raise ValueError("the mean needs at least one row")
+    if grad_scale is not None and (grad_scale.dtype != torch.float32 or grad_scale.numel() != 1
+                                   or grad_scale.device != dev):
+        raise ValueError("grad_scale must be a one-element fp32 tensor on the logits' device")
+    grad = torch.zeros_like(logits) if want_grad else None
+    loss = torch.empty((), dtype=torch.float32, device=dev) if want_loss else None
+    ws_n = lib().attgnn_nll_log_softmax_workspace_elems(n_sel) if want_loss else 0
+    ws = torch.empty(ws_n, dtype=torch.float32, device=dev) if want_loss else None
+    gs = grad_scale.contiguous() if grad_scale is not None else None
+    check(lib().attgnn_nll_log_softmax(_ptr(logits), _TYPE_OF[logits.dtype], n, ld, int(classes), _ptr(labels),
+                                       _ptr(rows), n_sel, _ptr(loss), _ptr(grad), _ptr(gs), _ptr(ws), ws_n,
+                                       _stream(dev)),
+          "attgnn_nll_log_softmax")
+    return loss, grad
+
+
 def spmm_compact_striped(rp, col, masks, block_base, values, window_of_part, atomic, owner_seg, owners, X, window,
                          mode=None, out_dtype=torch.float32, rows_covered=False, workspace=None, pipe=None,
                          split=True, groups=None):

@@ -382,7 +382,7 @@ def main():
 
     from att_amd import graphs as G
     from att_amd import ops
-    from att_amd.layers import GCNConv, GCNGraph
+    from att_amd.layers import GCNConv, GCNGraph, nll_log_softmax_loss
 
     dev = torch.device("cuda", 0)
     g = build_graph(args.scale)
@@ -442,17 +442,16 @@ def main():
         def forward(self, x):
             h = F.relu(self.conv1(x, graph))
             h = F.dropout(h, 0.5, self.training)
-            h = self.conv2(h, graph)
-            return F.log_softmax(h[:, :N_CLASS].float(), dim=1)
+            return self.conv2(h, graph)          # logits: classes 0..40 of 48 columns
 
     model = GCN2().to(dev)
     opt = torch.optim.Adam(model.parameters(), lr=1e-2, fused=True)   # one kernel per step
 
     def step():
         opt.zero_grad(set_to_none=True)
-        # NLL loss (mean over nodes) as a gather: the same value as F.nll_loss,
-        # whose 2-D reduction kernel takes ~4x longer here
-        loss = -model(X).gather(1, labels[:, None]).squeeze(1).index_select(0, real).mean()
+        # F.nll_loss(F.log_softmax(logits[:, :41])) averaged over the real
+        # nodes, loss and gradient in one kernel
+        loss = nll_log_softmax_loss(model(X), labels, N_CLASS, real)
         loss.backward()
         opt.step()
         return loss

@@ -858,6 +858,34 @@ int attgnn_pack_segments(const void *segs, int64_t nsegs, const int32_t *chunk_s
                          int64_t chunk_bytes, attgnn_stream_t stream);
 int64_t attgnn_pack_segment_bytes(void);
 
+/* ====================================================================== */
+/* Training loss                                                           */
+/* ====================================================================== */
+
+/*
+ * Mean negative log-likelihood of a row-wise log-softmax and its gradient in
+ * one pass (replaces F.log_softmax + F.nll_loss of the reference's GCN,
+ * end2end_ori/gcn_final/mgat_csr/gat_mgnn.py:39,66).  logits: [n_rows][ld]
+ * of logits_type (F32 / F16 / BF16), the first `classes` columns of each row
+ * being the classes; labels: int64 [n_rows]; rows: the n_sel row indices the
+ * mean runs over (NULL: rows 0 .. n_sel-1).  loss (may be NULL): loss[0] =
+ * (1/n_sel) sum over the selected rows r of (logsumexp(logits[r, :classes])
+ * - logits[r, labels[r]]), summed in a fixed order (the same bits run to
+ * run), with workspace of attgnn_nll_log_softmax_workspace_elems(n_sel)
+ * floats.  grad (may be NULL, not both): for every selected row and class c,
+ * grad[r][c] = g (softmax(logits[r, :classes])[c] - [c == labels[r]]) / n_sel
+ * in logits_type, g = grad_scale[0] (a device float; NULL: 1) -- the
+ * backward of the mean scaled by the upstream gradient; the other elements
+ * of grad are not written.  FP32 arithmetic.  A label outside [0, classes)
+ * gives NaN for its row (loss and gradient), a row index outside [0, n_rows)
+ * NaN in the loss (nothing stored).
+ */
+int64_t attgnn_nll_log_softmax_workspace_elems(int64_t n_sel);
+int attgnn_nll_log_softmax(const void *logits, int logits_type, int64_t n_rows, int64_t ld, int classes,
+                           const int64_t *labels, const int64_t *rows, int64_t n_sel, float *loss, void *grad,
+                           const float *grad_scale, float *workspace, int64_t workspace_elems,
+                           attgnn_stream_t stream);
+
 /*
  * Host staging of one serving graph (no counterpart in the reference, which
  * serves nothing).  attgnn_serve_stage_layout: the byte offsets, in one

@@ -1,0 +1,83 @@
+"""Fused NLL of log_softmax (the GCN training loss) against the PyTorch chain
+of the reference's model: F.log_softmax over the class columns, the label's
+term gathered, the mean over the selected rows, autograd's gradient."""
+import pytest
+import torch
+import torch.nn.functional as F
+
+from att_amd import ops
+from att_amd.layers import nll_log_softmax_loss
+
+pytestmark = pytest.mark.gpu
+
+
+def _reference(logits, labels, classes, rows):
+    x = logits.detach().float().requires_grad_(True)
+    lp = F.log_softmax(x[:, :classes], dim=1)
+    term = -lp.gather(1, labels[:, None]).squeeze(1)
+    loss = (term if rows is None else term.index_select(0, rows)).mean()
+    loss.backward()
+    return loss.detach(), x.grad
+
+
+@pytest.mark.parametrize("dtype", [torch.float16, torch.bfloat16, torch.float32])
+@pytest.mark.parametrize("use_rows", [False, True])
+def test_loss_and_grad_match_torch(cuda, dtype, use_rows):
+    torch.manual_seed(3)
+    n, ld, classes = 5000, 48, 41
+    logits = (torch.randn(n, ld, device=cuda) * 3).to(dtype)
+    labels = torch.randint(0, classes, (n,), device=cuda)
+    rows = torch.randperm(n, device=cuda)[:3100].sort().values if use_rows else None
+    loss, grad = ops.nll_log_softmax(logits, labels, classes, rows)
+    ref_loss, ref_grad = _reference(logits, labels, classes, rows)
+    assert abs(loss.item() - ref_loss.item()) <= 2e-6 * abs(ref_loss.item())
+    # both gradients are (softmax - one-hot) / n formed in FP32 -- here as
+    # exp(z - lse), in autograd through log_softmax's backward -- so the
+    # softmax terms agree to a few 2^-24 (absolute, they are at most 1), then
+    # each is rounded once to the logits' type
+    n_sel = n if rows is None else rows.numel()
+    g, r = grad.float(), ref_grad.to(dtype).float()
+    ulp = {torch.float16: 2.0 ** -10, torch.bfloat16: 2.0 ** -7, torch.float32: 2.0 ** -23}[dtype]
+    tiny = {torch.float16: 2.0 ** -24, torch.bfloat16: 0.0, torch.float32: 0.0}[dtype]
+    assert torch.all((g - r).abs() <= ulp * r.abs() + 8 * 2.0 ** -24 / n_sel + tiny)
+    assert not grad[:, classes:].any()                       # padding classes untouched (zero)
+    if rows is not None:
+        other = torch.ones(n, dtype=torch.bool, device=cuda)
+        other[rows] = False
+        assert not grad[other].any()                         # rows outside the mean untouched
+    # bitwise run to run
+    loss2, grad2 = ops.nll_log_softmax(logits, labels, classes, rows)
+    assert torch.equal(loss, loss2) and torch.equal(grad, grad2)
+
+
+def test_autograd_function_and_bad_labels(cuda):
+    """The autograd function: its loss is the combined call's, its gradient
+    the reference's scaled by the upstream gradient inside the kernel (the
+    scaled FP32 value rounded once -- these gradients sit in FP16's
+    subnormal range, where scaling after the rounding would differ); the
+    loss-only and gradient-only calls equal the combined one."""
+    torch.manual_seed(4)
+    n, ld, classes = 700, 48, 41
+    logits = torch.randn(n, ld, device=cuda, dtype=torch.float16).requires_grad_(True)
+    labels = torch.randint(0, classes, (n,), device=cuda)
+    rows = torch.arange(0, n, 3, device=cuda)
+    loss_ref, grad = ops.nll_log_softmax(logits.detach(), labels, classes, rows)
+    lo, gn = ops.nll_log_softmax(logits.detach(), labels, classes, rows, grad=False)
+    ln, go = ops.nll_log_softmax(logits.detach(), labels, classes, rows, loss=False)
+    assert gn is None and ln is None and torch.equal(lo, loss_ref) and torch.equal(go, grad)
+    loss = nll_log_softmax_loss(logits, labels, classes, rows)
+    assert torch.equal(loss, loss_ref)
+    _, ref = _reference(logits, labels, classes, rows)
+    for scale in (1.0, 2.0, 0.3):
+        logits.grad = None
+        (scale * nll_log_softmax_loss(logits, labels, classes, rows)).backward()
+        g, r = logits.grad.float(), (scale * ref).half().float()
+        tol = 2.0 ** -10 * r.abs() + 8 * scale * 2.0 ** -24 / rows.numel() + 2.0 ** -24
+        assert torch.all((g - r).abs() <= tol), scale
+        if scale == 1.0:
+            assert torch.equal(logits.grad, grad)
+    bad = labels.clone()
+    bad[rows[5]] = classes                                   # a label outside the class columns
+    lb, gb = ops.nll_log_softmax(logits.detach(), bad, classes, rows)
+    assert torch.isnan(lb) and torch.isnan(gb[rows[5], :classes]).all()
+    assert not torch.isnan(gb[rows[6]]).any()

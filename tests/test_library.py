@@ -89,3 +89,27 @@ def test_range_and_minmax_reject_bad_arguments_without_device():
                                        nul) == -1
     # nothing to do for zero nodes: ok without touching the buffers
     assert L.attgnn_minmax_score_grads(nul, nul, 0, 4, 1, 4, nul, nul, nul, nul, nul, nul, nul, 0, nul) == 0
+
+
+def test_nll_log_softmax_rejects_bad_arguments_without_device():
+    """attgnn_nll_log_softmax: logit type, class count against the row
+    stride, an empty mean, missing buffers (neither loss nor gradient, no
+    logits, a loss without its workspace), a short workspace and more
+    selected rows than rows without a list are refused before any launch."""
+    L = _lib.lib()
+    nul = ctypes.c_void_p(0)
+    fake = ctypes.c_void_p(0x1000)    # never dereferenced: validation fails first
+    ws = L.attgnn_nll_log_softmax_workspace_elems(1000)
+    assert ws == 63 and L.attgnn_nll_log_softmax_workspace_elems(1) == 1   # 16 rows per workgroup
+    assert L.attgnn_nll_log_softmax_workspace_elems(-1) == -1
+
+    def call(t=1, n=1000, ld=48, c=41, rows=nul, sel=1000, loss=fake, grad=fake, wsp=fake, ws=ws, logits=fake):
+        return L.attgnn_nll_log_softmax(logits, t, n, ld, c, fake, rows, sel, loss, grad, nul, wsp, ws, nul)
+    assert call(t=3) == -1                 # an int32 type
+    assert call(c=49) == -1                # classes beyond the row stride
+    assert call(c=0) == -1
+    assert call(sel=0, ws=1) == -1         # the mean of no rows
+    assert call(sel=1001) == -1            # more rows than there are, without a list
+    assert call(loss=nul, grad=nul) == -1  # nothing to compute
+    assert call(logits=nul) == -1
+    assert call(wsp=nul) == -1 and call(ws=ws - 1) == -1   # a loss needs its workspace
