@@ -76,40 +76,71 @@ __device__ __forceinline__ float row_sum(float v)
     return v;
 }
 
+// One row on the 16 lanes of its group: lane k holds classes k, k + 16, ...
+// (NV of them in registers, so the row is read once; NV = 0: classes beyond
+// 64, re-read from memory for each pass).  Returns the row's loss term in
+// lane k == 0 of the group.
+template <int NV>
+__device__ __forceinline__ float loss_row(const LossArgs &a, int64_t i, int k)
+{
+    const int64_t r = a.rows ? a.rows[i] : i;
+    if (r < 0 || r >= a.n_rows) return k == 0 ? NAN : 0.f;   // outside the logits: NaN loss, nothing stored
+    const int64_t y = a.labels[r];
+    const int64_t base = r * a.ld;
+    float z[NV > 0 ? NV : 1];
+    float m = -INFINITY;
+    if constexpr (NV > 0) {
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+            const int c = k + j * ROW_LANES;
+            z[j] = c < a.classes ? load_logit(a.logits, a.type, base + c) : -INFINITY;
+            m = fmaxf(m, z[j]);
+        }
+    } else {
+        for (int c = k; c < a.classes; c += ROW_LANES) m = fmaxf(m, load_logit(a.logits, a.type, base + c));
+    }
+    m = row_max(m);
+    float s = 0.f;
+    if constexpr (NV > 0) {
+#pragma unroll
+        for (int j = 0; j < NV; ++j) s += expf(z[j] - m);   // exp(-inf) = 0 past the classes
+    } else {
+        for (int c = k; c < a.classes; c += ROW_LANES) s += expf(load_logit(a.logits, a.type, base + c) - m);
+    }
+    s = row_sum(s);
+    // log-sum-exp = m + log(s), kept apart: (z - m) - log(s) loses no bits
+    // to the size of m
+    const float ls = logf(s);
+    const bool ok = y >= 0 && y < a.classes;
+    const float term = k == 0 ? (ok ? (m - load_logit(a.logits, a.type, base + y)) + ls : NAN) : 0.f;
+    if (a.grad) {
+        const float gs = a.scale * (a.grad_scale ? a.grad_scale[0] : 1.f);
+        auto put = [&](int c, float zc) {
+            const float p = expf((zc - m) - ls);
+            store_grad(a.grad, a.type, base + c, ok ? (p - (c == y ? 1.f : 0.f)) * gs : NAN);
+        };
+        if constexpr (NV > 0) {
+#pragma unroll
+            for (int j = 0; j < NV; ++j)
+                if (k + j * ROW_LANES < a.classes) put(k + j * ROW_LANES, z[j]);
+        } else {
+            for (int c = k; c < a.classes; c += ROW_LANES) put(c, load_logit(a.logits, a.type, base + c));
+        }
+    }
+    return term;
+}
+
+// Row groups in a grid-stride loop (the 16 lanes of a group share every
+// branch, so the lane shuffles of a row stay within active lanes).
+template <int NV>
 __global__ __launch_bounds__(LOSS_THREADS) void nll_log_softmax_kernel(LossArgs a)
 {
     __shared__ float red[LOSS_THREADS / 64];
     const int k = threadIdx.x & (ROW_LANES - 1);
-    const int64_t i = int64_t(blockIdx.x) * ROWS_PER_BLOCK + (threadIdx.x / ROW_LANES);
     float term = 0.f;
-    // the whole 16-lane group takes the same branches (i, r, y are per row)
-    if (i < a.n_sel) {
-        const int64_t r = a.rows ? a.rows[i] : i;
-        if (r < 0 || r >= a.n_rows) {   // a row index outside the logits: NaN loss, nothing stored
-            term = k == 0 ? NAN : 0.f;
-        } else {
-            const int64_t y = a.labels[r];
-            const int64_t base = r * a.ld;
-            float m = -INFINITY;
-            for (int c = k; c < a.classes; c += ROW_LANES) m = fmaxf(m, load_logit(a.logits, a.type, base + c));
-            m = row_max(m);
-            float s = 0.f;
-            for (int c = k; c < a.classes; c += ROW_LANES) s += expf(load_logit(a.logits, a.type, base + c) - m);
-            s = row_sum(s);
-            // log-sum-exp = m + log(s), kept apart: (z - m) - log(s) loses no
-            // bits to the size of m
-            const float ls = logf(s);
-            const bool ok = y >= 0 && y < a.classes;
-            if (k == 0) term = ok ? (m - load_logit(a.logits, a.type, base + y)) + ls : NAN;
-            if (a.grad) {
-                const float gs = a.scale * (a.grad_scale ? a.grad_scale[0] : 1.f);
-                for (int c = k; c < a.classes; c += ROW_LANES) {
-                    const float p = expf((load_logit(a.logits, a.type, base + c) - m) - ls);
-                    store_grad(a.grad, a.type, base + c, ok ? (p - (c == y ? 1.f : 0.f)) * gs : NAN);
-                }
-            }
-        }
-    }
+    for (int64_t i = int64_t(blockIdx.x) * ROWS_PER_BLOCK + threadIdx.x / ROW_LANES; i < a.n_sel;
+         i += int64_t(gridDim.x) * ROWS_PER_BLOCK)
+        term += loss_row<NV>(a, i, k);
     if (a.partial) {
         const float s = block_sum(term, red);
         if (threadIdx.x == 0) a.partial[blockIdx.x] = s;
@@ -126,11 +157,13 @@ __global__ __launch_bounds__(LOSS_THREADS) void loss_total_kernel(const float *p
     if (threadIdx.x == 0) loss[0] = s * scale;
 }
 
+constexpr int64_t LOSS_MAX_BLOCKS = 2048;   // workgroups of the row kernel (its partial sums)
+
 }  // namespace
 
 extern "C" int64_t attgnn_nll_log_softmax_workspace_elems(int64_t n_sel)
 {
-    return n_sel < 0 ? -1 : std::max<int64_t>(1, (n_sel + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK);
+    return n_sel < 0 ? -1 : std::clamp<int64_t>((n_sel + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK, 1, LOSS_MAX_BLOCKS);
 }
 
 extern "C" int attgnn_nll_log_softmax(const void *logits, int logits_type, int64_t n_rows, int64_t ld, int classes,
@@ -150,7 +183,12 @@ extern "C" int attgnn_nll_log_softmax(const void *logits, int logits_type, int64
     a.partial = loss ? workspace : nullptr;
     a.ld = ld; a.n_sel = n_sel; a.n_rows = n_rows; a.scale = 1.0f / float(n_sel); a.type = logits_type; a.classes = classes;
     const int64_t blocks = attgnn_nll_log_softmax_workspace_elems(n_sel);
-    hipLaunchKernelGGL(nll_log_softmax_kernel, dim3(unsigned(blocks)), dim3(LOSS_THREADS), 0, s, a);
+    const dim3 grid{unsigned(blocks)}, block{unsigned(LOSS_THREADS)};
+    if (classes <= 16) hipLaunchKernelGGL(nll_log_softmax_kernel<1>, grid, block, 0, s, a);
+    else if (classes <= 32) hipLaunchKernelGGL(nll_log_softmax_kernel<2>, grid, block, 0, s, a);
+    else if (classes <= 48) hipLaunchKernelGGL(nll_log_softmax_kernel<3>, grid, block, 0, s, a);
+    else if (classes <= 64) hipLaunchKernelGGL(nll_log_softmax_kernel<4>, grid, block, 0, s, a);
+    else hipLaunchKernelGGL(nll_log_softmax_kernel<0>, grid, block, 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || !loss) return e == hipSuccess ? ATTGNN_OK : int(e);
     hipLaunchKernelGGL(loss_total_kernel, dim3(1), dim3(LOSS_THREADS), 0, s, workspace, blocks, a.scale, loss);

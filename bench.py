@@ -424,7 +424,8 @@ def main():
     # stripe_pos(j), zero padding rows); the loss averages the real nodes only
     X = graph.to_striped(torch.randn(g.n, F_IN, generator=rng).half().to(dev))
     labels = graph.to_striped(torch.randint(0, N_CLASS, (g.n,), generator=torch.Generator().manual_seed(2)).to(dev))
-    real = graph.stripe_index if graph.stripe else torch.arange(g.n, device=dev)
+    # the real nodes' rows, ascending (the loss reads consecutive rows together)
+    real = graph.stripe_index.sort().values if graph.stripe else torch.arange(g.n, device=dev)
     torch.manual_seed(3)
 
     class GCN2(nn.Module):

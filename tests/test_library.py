@@ -101,6 +101,7 @@ def test_nll_log_softmax_rejects_bad_arguments_without_device():
     fake = ctypes.c_void_p(0x1000)    # never dereferenced: validation fails first
     ws = L.attgnn_nll_log_softmax_workspace_elems(1000)
     assert ws == 63 and L.attgnn_nll_log_softmax_workspace_elems(1) == 1   # 16 rows per workgroup
+    assert L.attgnn_nll_log_softmax_workspace_elems(10 ** 6) == 2048      # at most 2048 workgroups
     assert L.attgnn_nll_log_softmax_workspace_elems(-1) == -1
 
     def call(t=1, n=1000, ld=48, c=41, rows=nul, sel=1000, loss=fake, grad=fake, wsp=fake, ws=ws, logits=fake):
