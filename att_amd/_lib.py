@@ -124,6 +124,8 @@ _SIGNATURES = {
     "attgnn_map_remap_device": (_int, [_vp, _i64, _vp, _i64, _vp, _vp]),
     "attgnn_pack_segments": (_int, [_vp, _i64, _vp, _i64, _i64, _vp]),
     "attgnn_pack_segment_bytes": (_i64, []),
+    "attgnn_relu_dropout_fwd": (_int, [_vp, _int, _i64, _c.c_float, _c.c_uint64, _vp, _vp, _vp]),
+    "attgnn_relu_dropout_bwd": (_int, [_vp, _int, _i64, _c.c_float, _vp, _vp, _vp]),
     "attgnn_nll_log_softmax_workspace_elems": (_i64, [_i64]),
     "attgnn_nll_log_softmax": (_int, [_vp, _int, _i64, _i64, _int, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _i64, _vp]),
     "attgnn_serve_stage_layout": (_i64, [_i64, _i64, _i64, _int, _vp]),

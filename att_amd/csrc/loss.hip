@@ -1,5 +1,6 @@
-// Negative log-likelihood of a row-wise log-softmax -- the GCN training loss
-// -- and its gradient in one pass over the logits.
+// Training-step kernels outside the sparse products: the GCN's loss (the
+// negative log-likelihood of a row-wise log-softmax and its gradient) and a
+// fused ReLU + dropout.
 //
 // The reference's GCN takes log_softmax of the last layer
 // (end2end_ori/gcn_final/mgat_csr/gat_mgnn.py:39) and nll_loss against the
@@ -159,7 +160,159 @@ __global__ __launch_bounds__(LOSS_THREADS) void loss_total_kernel(const float *p
 
 constexpr int64_t LOSS_MAX_BLOCKS = 2048;   // workgroups of the row kernel (its partial sums)
 
+// ---------------------------------------------------------------------------
+// ReLU + dropout, forward and backward, 8 elements per thread.  The keep
+// decision of element i depends only on (seed, i): keep when the top 24 bits
+// of lowbias32(lowbias32(seed_lo ^ seed_hi) + i) are at least p * 2^24, with
+// lowbias32 the 32-bit integer hash x ^= x >> 16; x *= 0x7feb352d;
+// x ^= x >> 15; x *= 0x846ca68b; x ^= x >> 16 (C. Wellons, "Hash function
+// prospector", 2018).  Bit b of mask byte j records whether element 8j + b
+// passed (positive and kept): the backward needs nothing else.
+
+__host__ __device__ inline uint32_t lowbias32(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+struct DropArgs {
+    const void *in;
+    void *out;
+    uint8_t *mask;          // forward: written
+    const uint8_t *mask_in; // backward: read
+    int64_t n8;             // groups of 8 elements
+    float scale;            // 1 / (1 - p), 0 when p >= 1
+    uint32_t thresh, key;   // keep when (hash >> 8) >= thresh; key = lowbias32(seed_lo ^ seed_hi)
+    int type;
+};
+
+template <int TYPE>
+__device__ __forceinline__ void load8(const void *p, int64_t g, float (&v)[8])
+{
+    if constexpr (TYPE == ATTGNN_F32) {
+        const float4 *q = static_cast<const float4 *>(p) + 2 * g;
+        const float4 a = q[0], b = q[1];
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else {
+        const uint4 u = static_cast<const uint4 *>(p)[g];
+        const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint16_t lo = uint16_t(w[j] & 0xffffu), hi = uint16_t(w[j] >> 16);
+            v[2 * j] = TYPE == ATTGNN_F16 ? attgnn::f16_to_f32(lo) : attgnn::bf16_to_f32(lo);
+            v[2 * j + 1] = TYPE == ATTGNN_F16 ? attgnn::f16_to_f32(hi) : attgnn::bf16_to_f32(hi);
+        }
+    }
+}
+
+template <int TYPE>
+__device__ __forceinline__ void store8(void *p, int64_t g, const float (&v)[8])
+{
+    if constexpr (TYPE == ATTGNN_F32) {
+        float4 *q = static_cast<float4 *>(p) + 2 * g;
+        q[0] = make_float4(v[0], v[1], v[2], v[3]);
+        q[1] = make_float4(v[4], v[5], v[6], v[7]);
+    } else {
+        uint32_t w[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint16_t lo = TYPE == ATTGNN_F16 ? attgnn::f32_to_f16(v[2 * j]) : attgnn::f32_to_bf16(v[2 * j]);
+            const uint16_t hi = TYPE == ATTGNN_F16 ? attgnn::f32_to_f16(v[2 * j + 1]) : attgnn::f32_to_bf16(v[2 * j + 1]);
+            w[j] = uint32_t(lo) | (uint32_t(hi) << 16);
+        }
+        static_cast<uint4 *>(p)[g] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+}
+
+template <int TYPE>
+__global__ __launch_bounds__(256) void relu_dropout_fwd_kernel(DropArgs a)
+{
+    for (int64_t g = int64_t(blockIdx.x) * 256 + threadIdx.x; g < a.n8; g += int64_t(gridDim.x) * 256) {
+        float v[8];
+        load8<TYPE>(a.in, g, v);
+        uint32_t bits = 0;
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const uint32_t h = lowbias32(a.key + uint32_t(g * 8 + b));
+            const bool pass = v[b] > 0.f && (h >> 8) >= a.thresh;
+            bits |= uint32_t(pass) << b;
+            v[b] = pass ? v[b] * a.scale : 0.f;
+        }
+        store8<TYPE>(a.out, g, v);
+        a.mask[g] = uint8_t(bits);
+    }
+}
+
+template <int TYPE>
+__global__ __launch_bounds__(256) void relu_dropout_bwd_kernel(DropArgs a)
+{
+    for (int64_t g = int64_t(blockIdx.x) * 256 + threadIdx.x; g < a.n8; g += int64_t(gridDim.x) * 256) {
+        float v[8];
+        load8<TYPE>(a.in, g, v);
+        const uint32_t bits = a.mask_in[g];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) v[b] = (bits >> b) & 1u ? v[b] * a.scale : 0.f;
+        store8<TYPE>(a.out, g, v);
+    }
+}
+
+int drop_setup(const void *in, int type, int64_t n, float p, const void *out, DropArgs &a)
+{
+    if (!(type == ATTGNN_F32 || type == ATTGNN_F16 || type == ATTGNN_BF16)) return ATTGNN_E_ARG;
+    if (n < 0 || n % 8 != 0 || !(p >= 0.f && p <= 1.f)) return ATTGNN_E_ARG;
+    if (n > 0 && (!in || !out)) return ATTGNN_E_ARG;
+    if (reinterpret_cast<uintptr_t>(in) % 16 || reinterpret_cast<uintptr_t>(out) % 16) return ATTGNN_E_ARG;
+    a.in = in; a.out = const_cast<void *>(out); a.n8 = n / 8; a.type = type;
+    a.scale = p >= 1.f ? 0.f : 1.f / (1.f - p);
+    // keep when the 24-bit draw is at least p * 2^24 (p = 1: never)
+    a.thresh = p >= 1.f ? (1u << 24) : uint32_t(p * 16777216.0f);
+    return ATTGNN_OK;
+}
+
+unsigned drop_grid(int64_t n8) { return unsigned(std::clamp<int64_t>((n8 + 255) / 256, 1, 8192)); }
+
 }  // namespace
+
+extern "C" int attgnn_relu_dropout_fwd(const void *x, int type, int64_t n, float p, uint64_t seed, void *y,
+                                       uint8_t *mask, attgnn_stream_t stream)
+{
+    DropArgs a{};
+    const int rc = drop_setup(x, type, n, p, y, a);
+    if (rc != ATTGNN_OK) return rc;
+    if (n > 0 && !mask) return ATTGNN_E_ARG;
+    if (n == 0) return ATTGNN_OK;
+    a.mask = mask;
+    a.key = lowbias32(uint32_t(seed) ^ uint32_t(seed >> 32));
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const dim3 grid{drop_grid(a.n8)}, block{256u};
+    if (type == ATTGNN_F16) hipLaunchKernelGGL(relu_dropout_fwd_kernel<ATTGNN_F16>, grid, block, 0, s, a);
+    else if (type == ATTGNN_BF16) hipLaunchKernelGGL(relu_dropout_fwd_kernel<ATTGNN_BF16>, grid, block, 0, s, a);
+    else hipLaunchKernelGGL(relu_dropout_fwd_kernel<ATTGNN_F32>, grid, block, 0, s, a);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? ATTGNN_OK : int(e);
+}
+
+extern "C" int attgnn_relu_dropout_bwd(const void *dy, int type, int64_t n, float p, const uint8_t *mask, void *dx,
+                                       attgnn_stream_t stream)
+{
+    DropArgs a{};
+    const int rc = drop_setup(dy, type, n, p, dx, a);
+    if (rc != ATTGNN_OK) return rc;
+    if (n > 0 && !mask) return ATTGNN_E_ARG;
+    if (n == 0) return ATTGNN_OK;
+    a.mask_in = mask;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const dim3 grid{drop_grid(a.n8)}, block{256u};
+    if (type == ATTGNN_F16) hipLaunchKernelGGL(relu_dropout_bwd_kernel<ATTGNN_F16>, grid, block, 0, s, a);
+    else if (type == ATTGNN_BF16) hipLaunchKernelGGL(relu_dropout_bwd_kernel<ATTGNN_BF16>, grid, block, 0, s, a);
+    else hipLaunchKernelGGL(relu_dropout_bwd_kernel<ATTGNN_F32>, grid, block, 0, s, a);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? ATTGNN_OK : int(e);
+}
 
 extern "C" int64_t attgnn_nll_log_softmax_workspace_elems(int64_t n_sel)
 {

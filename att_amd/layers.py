@@ -203,6 +203,35 @@ class GCNGraph:
                               out_dtype=out_dtype or X.dtype, rows_covered=True)
 
 
+class ReluDropout(torch.autograd.Function):
+    """F.dropout(F.relu(x), p) as one kernel each way (ops.relu_dropout): the
+    forward keeps a one-bit mask of the elements that passed."""
+
+    @staticmethod
+    def forward(ctx, x, p, seed):
+        y, mask = ops.relu_dropout(x.contiguous(), p, seed)
+        ctx.save_for_backward(mask)
+        ctx.p = p
+        return y
+
+    @staticmethod
+    def backward(ctx, dy):
+        (mask,) = ctx.saved_tensors
+        return ops.relu_dropout_bwd(dy.contiguous(), ctx.p, mask), None, None
+
+
+def relu_dropout(x, p=0.5, training=True, generator=None):
+    """F.dropout(F.relu(x), p, training) (the reference GCN between its
+    layers, end2end_ori/gcn_final/mgat_csr/gat_mgnn.py:17-40) fused.  Each
+    call draws a 62-bit seed from ``generator`` (a CPU torch.Generator; None:
+    the default one), so the masks follow torch.manual_seed; out of training,
+    or with p = 0, it is F.relu."""
+    if not training or p == 0.0:
+        return F.relu(x)
+    seed = int(torch.randint(0, 1 << 62, (1,), generator=generator))
+    return ReluDropout.apply(x, float(p), seed)
+
+
 class NllLogSoftmax(torch.autograd.Function):
     """Mean NLL of log_softmax over the selected rows (ops.nll_log_softmax):
     the loss forward, the gradient scaled by the upstream one backward, one

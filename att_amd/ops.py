@@ -550,6 +550,41 @@ def stripe_segments(rp, window_of_part, col, owners, m):
     return seg
 
 
+def _drop_check(name, t):
+    if t.dtype not in (torch.float32, torch.float16, torch.bfloat16):
+        raise TypeError(f"{name}: fp32 / fp16 / bf16 tensors only, got {t.dtype}")
+    if not t.is_contiguous() or t.numel() % 8 or t.data_ptr() % 16:
+        raise ValueError(f"{name}: needs a contiguous tensor of a multiple of 8 elements, 16-byte aligned")
+
+
+def relu_dropout(x, p, seed):
+    """attgnn_relu_dropout_fwd: (y, mask) with y = x / (1 - p) where x > 0
+    and the element is kept, else 0; the keep decision of element i is a hash
+    of (seed, i); mask holds one bit per element (uint8 [numel / 8])."""
+    dev = _need_gpu("relu_dropout", x)
+    _drop_check("relu_dropout", x)
+    if not 0.0 <= p <= 1.0:
+        raise ValueError(f"p must be in [0, 1], got {p}")
+    y = torch.empty_like(x)
+    mask = torch.empty(x.numel() // 8, dtype=torch.uint8, device=dev)
+    check(lib().attgnn_relu_dropout_fwd(_ptr(x), _TYPE_OF[x.dtype], x.numel(), float(p),
+                                        int(seed) & 0xFFFFFFFFFFFFFFFF, _ptr(y), _ptr(mask), _stream(dev)),
+          "attgnn_relu_dropout_fwd")
+    return y, mask
+
+
+def relu_dropout_bwd(dy, p, mask):
+    """attgnn_relu_dropout_bwd: dx = dy / (1 - p) where the mask bit is set, else 0."""
+    dev = _need_gpu("relu_dropout_bwd", dy, mask)
+    _drop_check("relu_dropout_bwd", dy)
+    if mask.dtype != torch.uint8 or mask.numel() != dy.numel() // 8:
+        raise ValueError("mask must be uint8 [numel / 8]")
+    dx = torch.empty_like(dy)
+    check(lib().attgnn_relu_dropout_bwd(_ptr(dy), _TYPE_OF[dy.dtype], dy.numel(), float(p), _ptr(mask), _ptr(dx),
+                                        _stream(dev)), "attgnn_relu_dropout_bwd")
+    return dx
+
+
 def nll_log_softmax(logits, labels, classes, rows=None, loss=True, grad=True, grad_scale=None):
     """attgnn_nll_log_softmax: (loss, grad) for logits [n, ld] (fp32 / fp16 /
     bf16, the first ``classes`` columns being the classes), int64 labels [n]

@@ -382,7 +382,7 @@ def main():
 
     from att_amd import graphs as G
     from att_amd import ops
-    from att_amd.layers import GCNConv, GCNGraph, nll_log_softmax_loss
+    from att_amd.layers import GCNConv, GCNGraph, nll_log_softmax_loss, relu_dropout
 
     dev = torch.device("cuda", 0)
     g = build_graph(args.scale)
@@ -441,8 +441,7 @@ def main():
                 self.conv2.weights[:, :N_CLASS] = w2
 
         def forward(self, x):
-            h = F.relu(self.conv1(x, graph))
-            h = F.dropout(h, 0.5, self.training)
+            h = relu_dropout(self.conv1(x, graph), 0.5, self.training)   # F.dropout(F.relu(.)) fused
             return self.conv2(h, graph)          # logits: classes 0..40 of 48 columns
 
     model = GCN2().to(dev)

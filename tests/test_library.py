@@ -114,3 +114,22 @@ def test_nll_log_softmax_rejects_bad_arguments_without_device():
     assert call(loss=nul, grad=nul) == -1  # nothing to compute
     assert call(logits=nul) == -1
     assert call(wsp=nul) == -1 and call(ws=ws - 1) == -1   # a loss needs its workspace
+
+
+def test_relu_dropout_rejects_bad_arguments_without_device():
+    """attgnn_relu_dropout_fwd / _bwd: element counts not a multiple of 8,
+    p outside [0, 1], unaligned or missing buffers and a missing mask are
+    refused before any launch."""
+    L = _lib.lib()
+    nul = ctypes.c_void_p(0)
+    al = ctypes.c_void_p(0x1000)
+    mis = ctypes.c_void_p(0x1008)
+    assert L.attgnn_relu_dropout_fwd(al, _lib.F16, 12, 0.5, 7, al, al, nul) == -1      # 12 % 8
+    assert L.attgnn_relu_dropout_fwd(al, _lib.F16, 16, 1.5, 7, al, al, nul) == -1      # p > 1
+    assert L.attgnn_relu_dropout_fwd(al, _lib.F16, 16, -0.1, 7, al, al, nul) == -1
+    assert L.attgnn_relu_dropout_fwd(mis, _lib.F16, 16, 0.5, 7, al, al, nul) == -1     # unaligned input
+    assert L.attgnn_relu_dropout_fwd(al, _lib.F16, 16, 0.5, 7, al, nul, nul) == -1     # no mask
+    assert L.attgnn_relu_dropout_fwd(al, _lib.FP8E4M3, 16, 0.5, 7, al, al, nul) == -1  # not fp32/16/bf16
+    assert L.attgnn_relu_dropout_fwd(nul, _lib.F16, 0, 0.5, 7, nul, nul, nul) == 0     # nothing to do
+    assert L.attgnn_relu_dropout_bwd(al, _lib.BF16, 16, 0.5, nul, al, nul) == -1       # no mask
+    assert L.attgnn_relu_dropout_bwd(al, _lib.BF16, 16, 0.5, al, mis, nul) == -1       # unaligned output
