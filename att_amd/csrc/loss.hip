@@ -238,9 +238,12 @@ __global__ __launch_bounds__(256) void relu_dropout_fwd_kernel(DropArgs a)
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
             const uint32_t h = lowbias32(a.key + uint32_t(g * 8 + b));
-            const bool pass = v[b] > 0.f && (h >> 8) >= a.thresh;
+            const bool keep = (h >> 8) >= a.thresh;
+            const bool pass = keep && v[b] > 0.f;
             bits |= uint32_t(pass) << b;
-            v[b] = pass ? v[b] * a.scale : 0.f;
+            // a kept NaN stays NaN (as relu and dropout leave it); its
+            // gradient is 0 (relu's backward passes only positive inputs)
+            v[b] = pass || (keep && v[b] != v[b]) ? v[b] * a.scale : 0.f;
         }
         store8<TYPE>(a.out, g, v);
         a.mask[g] = uint8_t(bits);

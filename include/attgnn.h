@@ -889,12 +889,13 @@ int attgnn_nll_log_softmax(const void *logits, int logits_type, int64_t n_rows, 
 /*
  * ReLU + dropout in one pass (replaces F.relu and F.dropout between the
  * reference GCN's layers, end2end_ori/gcn_final/mgat_csr/gat_mgnn.py:17-40).
- * Forward: y[i] = x[i] / (1 - p) if x[i] > 0 and element i is kept, else 0;
+ * Forward: y[i] = x[i] / (1 - p) if x[i] > 0 (or NaN) and element i is kept,
+ * else 0;
  * element i is kept when the top 24 bits of lowbias32(lowbias32(seed_lo ^
  * seed_hi) + i) are at least p * 2^24 (lowbias32: x ^= x >> 16;
  * x *= 0x7feb352d; x ^= x >> 15; x *= 0x846ca68b; x ^= x >> 16; i taken
  * mod 2^32), so the mask depends on (seed, i) only; bit b of mask[j] records
- * whether element 8j + b passed.  Backward: dx[i] = dy[i] / (1 - p) where the
+ * whether element 8j + b passed (kept and positive).  Backward: dx[i] = dy[i] / (1 - p) where the
  * bit is set, else 0.  n (elements) a multiple of 8, x / y / dy / dx
  * 16-byte aligned, type F32 / F16 / BF16, p in [0, 1]; mask holds n / 8
  * bytes.  No counterpart kernel in the reference (torch ops there).

@@ -35,24 +35,28 @@ def test_forward_matches_keep_rule(cuda, dtype, p):
     torch.manual_seed(0)
     n = 4096 * 24
     x = torch.randn(n, device=cuda).to(dtype)
+    x[5::1009] = float("nan")                     # NaN stays NaN where kept, as relu and dropout leave it
     seed = 0x123456789ABCDEF
     y, mask = ops.relu_dropout(x, p, seed)
     keep = torch.from_numpy(keep_oracle(seed, n, p)).to(cuda)
     passed = keep & (x > 0)
     # the kernel's scale: 1 / (1 - p) formed in float32
     scale = np.float32(0.0) if p >= 1.0 else np.float32(1.0) / (np.float32(1.0) - np.float32(p))
-    want = torch.where(passed, (x.float() * torch.tensor(scale, device=cuda)).to(dtype), torch.zeros_like(x))
-    assert torch.equal(y, want)
+    kept_nan = keep & torch.isnan(x)
+    want = torch.where(passed | kept_nan, (x.float() * torch.tensor(scale, device=cuda)).to(dtype),
+                       torch.zeros_like(x))
+    assert torch.equal(torch.isnan(y), kept_nan)
+    assert torch.equal(torch.nan_to_num(y), torch.nan_to_num(want))
     bits = ((mask[:, None].int() >> torch.arange(8, device=cuda)) & 1).bool().reshape(-1)
     assert torch.equal(bits, passed)
     if 0.0 < p < 1.0:
         rate = keep.float().mean().item()
         assert abs(rate - (1.0 - p)) < 5 * np.sqrt(p * (1 - p) / n)
-    y2, mask2 = ops.relu_dropout(x, p, seed)
-    assert torch.equal(y, y2) and torch.equal(mask, mask2)
+    y2, mask2 = ops.relu_dropout(x, p, seed)     # the same bits (NaN included) run to run
+    assert torch.equal(y.view(torch.uint8), y2.view(torch.uint8)) and torch.equal(mask, mask2)
     if 0.0 < p < 1.0:
-        y3, _ = ops.relu_dropout(x, p, seed + 1)
-        assert not torch.equal(y, y3)
+        _, mask3 = ops.relu_dropout(x, p, seed + 1)
+        assert not torch.equal(mask, mask3)
 
 
 @pytest.mark.parametrize("dtype", [torch.float16, torch.bfloat16])
