@@ -372,8 +372,8 @@ def main():
                     help="kernels: only the width-256 SpMM and a 1 GiB copy (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: the row sharding across GPUs (whole windows per GPU, "
+              "an all-gather of X W per layer) is a design, not built; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch
