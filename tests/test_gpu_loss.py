@@ -84,3 +84,62 @@ def test_autograd_function_and_bad_labels(cuda):
     lb, gb = ops.nll_log_softmax(logits.detach(), bad, classes, rows)
     assert torch.isnan(lb) and torch.isnan(gb[rows[5], :classes]).all()
     assert not torch.isnan(gb[rows[6]]).any()
+
+
+@pytest.mark.parametrize("stripe", [None, 8])
+def test_gcn_step_with_fused_ops_equals_torch_ops(cuda, stripe):
+    """The benchmark's 2-layer GCN step (GCNConv, fused ReLU + dropout, fused
+    loss over the real rows) against the same step with F.relu, the same
+    dropout mask applied by torch, F.log_softmax and the gathered NLL: the
+    loss and both weight gradients agree to FP16 / FP32 rounding."""
+    import torch.nn.functional as F
+    from att_amd import graphs as G
+    from att_amd.layers import GCNConv, GCNGraph
+    g = G.chung_lu(3000, 20, 2.1, seed=11)
+    vals = G.gcn_norm_values(g, G.gcn_norm_degree(g))
+    graph = GCNGraph(g, cuda, tile_dtype=torch.float16, symmetric=True, values=vals, build="device", stripe=stripe)
+    gen = torch.Generator().manual_seed(2)
+    X = graph.to_striped(torch.randn(g.n, 96, generator=gen).half().to(cuda))
+    labels = graph.to_striped(torch.randint(0, 41, (g.n,), generator=gen).to(cuda))
+    real = graph.stripe_index.sort().values if stripe else torch.arange(g.n, device=cuda)
+    torch.manual_seed(7)
+    c1, c2 = GCNConv(96, 64).to(cuda), GCNConv(64, 48).to(cuda)
+    seed = 0x5EED
+
+    # fused: the ops the benchmark step uses
+    y1 = c1(X, graph)
+    h = ReluDropoutSeeded.apply(y1, seed)
+    loss_f = nll_log_softmax_loss(c2(h, graph), labels, 41, real)
+    loss_f.backward()
+    gf = [c1.weights.grad.clone(), c2.weights.grad.clone()]
+    for c in (c1, c2):
+        c.weights.grad = None
+
+    # torch ops with the fused op's mask
+    y1 = c1(X, graph)
+    _, mask = ops.relu_dropout(y1.detach(), 0.5, seed)
+    keep = ((mask[:, None].int() >> torch.arange(8, device=cuda)) & 1).bool().reshape(y1.shape)
+    keep = keep | (y1.detach() <= 0)     # the mask records kept positive entries; relu zeroes the rest anyway
+    h = F.relu(y1) * keep * 2.0
+    lp = F.log_softmax(c2(h, graph)[:, :41].float(), dim=1)
+    loss_t = -lp.gather(1, labels[:, None]).squeeze(1).index_select(0, real).mean()
+    loss_t.backward()
+    gt = [c1.weights.grad, c2.weights.grad]
+    assert abs(loss_f.item() - loss_t.item()) <= 1e-5 * abs(loss_t.item())
+    for a, b in zip(gf, gt):
+        assert ((a - b).norm() / b.norm()).item() < 2e-3
+
+
+class ReluDropoutSeeded(torch.autograd.Function):
+    """layers.ReluDropout with a fixed seed (the step above fixes its mask)."""
+
+    @staticmethod
+    def forward(ctx, x, seed):
+        y, mask = ops.relu_dropout(x.contiguous(), 0.5, seed)
+        ctx.save_for_backward(mask)
+        return y
+
+    @staticmethod
+    def backward(ctx, dy):
+        (mask,) = ctx.saved_tensors
+        return ops.relu_dropout_bwd(dy.contiguous(), 0.5, mask), None
